@@ -1,0 +1,311 @@
+"""MI355X-native GraphVine: dynamic graph store + analytics on one GPU.
+
+Python mirror of the reference's host interface over the C ABI in
+include/gv.h (libgv.so, hand-written HIP kernels for gfx950).  Names follow
+the reference:
+
+    GraphVine(...)            include/dynamicStructure/graphvine.cuh:29-105
+      .bulkBuild / .batchInsert / .batchDelete / .compaction / .getVertexSize
+    static_pagerank(...)      include/algorithm/pagerank/pagerank.cuh:11
+    static_tc(...)            include/algorithm/tc/tc.cuh:10
+    bc(...)                   include/algorithm/bc/bc.cuh:8
+
+Differences from the reference that are part of the contract: vertex IDs are
+0-based; update batches are COO arrays (host numpy or device tensors) rather
+than a CSR object; errors raise GraphVineError instead of exiting.  There is
+no CPU fallback: if libgv.so is missing or the GPU is absent, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgv.so")
+
+GV_OK = 0
+GV_E_INVALID = -1
+GV_E_RANGE = -2
+GV_E_NOMEM = -3
+GV_E_POOL = -4
+GV_E_HIP = -5
+GV_E_UNSUPPORTED = -6
+GV_MEM_HOST = 0
+GV_MEM_DEVICE = 1
+
+EXPORTED = [
+    "gv_config_default", "gv_create", "gv_destroy", "gv_bulk_build", "gv_batch_insert",
+    "gv_batch_delete", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
+    "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_triangle_count",
+    "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_sync", "gv_get_stream",
+    "gv_last_error",
+]
+
+
+class GvConfig(ctypes.Structure):
+    _fields_ = [
+        ("block_slots", ctypes.c_uint32),
+        ("pool_blocks", ctypes.c_uint64),
+        ("pool_fraction", ctypes.c_double),
+        ("compact_threshold", ctypes.c_float),
+        ("pr_thread_max", ctypes.c_uint32),
+        ("pr_group_max", ctypes.c_uint32),
+        ("pr_wave_max", ctypes.c_uint32),
+        ("maintain_in_edges", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class GvBatchStats(ctypes.Structure):
+    _fields_ = [
+        ("requested", ctypes.c_uint64),
+        ("directed_entries", ctypes.c_uint64),
+        ("applied", ctypes.c_uint64),
+        ("touched", ctypes.c_uint64),
+        ("blocks_allocated", ctypes.c_uint64),
+        ("live_edges", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+class GraphVineError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"gv status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Load libgv.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m gv_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u32, u64, i32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "gv_config_default": [ctypes.POINTER(GvConfig)],
+        "gv_create": [ctypes.POINTER(GvConfig), u32, i32, ctypes.POINTER(P)],
+        "gv_destroy": [P],
+        "gv_bulk_build": [P, P, P, u64, i32, ctypes.POINTER(GvBatchStats)],
+        "gv_batch_insert": [P, P, P, u64, i32, ctypes.POINTER(GvBatchStats)],
+        "gv_batch_delete": [P, P, P, u64, i32, ctypes.POINTER(GvBatchStats)],
+        "gv_compact": [P, ctypes.c_float, ctypes.POINTER(u64)],
+        "gv_num_vertices": [P, ctypes.POINTER(u32)],
+        "gv_num_edges": [P, ctypes.POINTER(u64)],
+        "gv_degrees": [P, P, i32],
+        "gv_pool_stats": [P, ctypes.POINTER(u64), ctypes.POINTER(u64)],
+        "gv_export_csr": [P, P, P, i32],
+        "gv_neighbors": [P, P, u32, P, P, i32],
+        "gv_pagerank": [P, ctypes.c_float, ctypes.c_double, i32, P, i32, ctypes.POINTER(i32),
+                        ctypes.POINTER(ctypes.c_double)],
+        "gv_triangle_count": [P, P, i32, ctypes.POINTER(u64)],
+        "gv_bc": [P, P, u32, P, i32],
+        "gv_gen_rmat": [P, u64, i32, u64, u32, u32, u32, P, P],
+        "gv_gen_uniform": [P, u64, u32, u64, P, P],
+        "gv_gen_sample": [P, u64, u32, P, P, u64, P, P],
+        "gv_sync": [P],
+        "gv_get_stream": [P, ctypes.POINTER(P)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.gv_last_error.argtypes = []
+    L.gv_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != GV_OK:
+        raise GraphVineError(rc, lib().gv_last_error().decode(errors="replace"))
+
+
+def default_config(**overrides):
+    cfg = GvConfig()
+    _check(lib().gv_config_default(ctypes.byref(cfg)))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _is_device(x):
+    return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
+
+
+def _ptr_u32(x):
+    """(pointer, keepalive, mem) for a uint32 host array or device tensor."""
+    if _is_device(x):
+        import torch
+        if x.dtype not in (torch.int32, torch.uint32):
+            raise TypeError("device arrays must be int32/uint32")
+        if not x.is_contiguous():
+            raise ValueError("device arrays must be contiguous")
+        return ctypes.c_void_p(x.data_ptr()), x, GV_MEM_DEVICE
+    a = np.ascontiguousarray(x, dtype=np.uint32)
+    return a.ctypes.data_as(ctypes.c_void_p), a, GV_MEM_HOST
+
+
+class GraphVine:
+    """Device-resident dynamic graph (reference: class GraphVine)."""
+
+    def __init__(self, num_vertices, directed=False, config=None, **overrides):
+        cfg = config if config is not None else default_config(**overrides)
+        h = ctypes.c_void_p()
+        _check(lib().gv_create(ctypes.byref(cfg), int(num_vertices), 1 if directed else 0,
+                               ctypes.byref(h)))
+        self._h = h
+        self.num_vertices = int(num_vertices)
+        self.directed = bool(directed)
+        self.config = cfg
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise GraphVineError(GV_E_INVALID, "graph is closed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().gv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _update(self, fn, src, dst, with_stats):
+        ps, ks, ms = _ptr_u32(src)
+        pd, kd, md = _ptr_u32(dst)
+        if ms != md:
+            raise ValueError("src and dst must both be host or both be device arrays")
+        n = int(len(src))
+        if int(len(dst)) != n:
+            raise ValueError("src and dst lengths differ")
+        st = GvBatchStats()
+        _check(fn(self.handle, ps, pd, n, ms, ctypes.byref(st) if with_stats else None))
+        return st.as_dict() if with_stats else None
+
+    # reference: GraphVine::bulkBuild (graphvine.cu:181-220)
+    def bulkBuild(self, src, dst, stats=True):
+        return self._update(lib().gv_bulk_build, src, dst, stats)
+
+    # reference: GraphVine::batchInsert (graphvine.cu:222-371)
+    def batchInsert(self, src, dst, stats=True):
+        return self._update(lib().gv_batch_insert, src, dst, stats)
+
+    # reference: GraphVine::batchDelete (graphvine.cu:373-451)
+    def batchDelete(self, src, dst, stats=True):
+        return self._update(lib().gv_batch_delete, src, dst, stats)
+
+    # reference: GraphVine::compaction (graphvine.cu:453-464)
+    def compaction(self, tombstone_fraction=0.0):
+        n = ctypes.c_uint64()
+        _check(lib().gv_compact(self.handle, float(tombstone_fraction), ctypes.byref(n)))
+        return int(n.value)
+
+    # reference: GraphVine::getVertexSize (graphvine.cuh:56)
+    def getVertexSize(self):
+        return self.num_vertices
+
+    def sync(self):
+        _check(lib().gv_sync(self.handle))
+
+    def stream(self):
+        p = ctypes.c_void_p()
+        _check(lib().gv_get_stream(self.handle, ctypes.byref(p)))
+        return p.value
+
+    def num_edges(self):
+        n = ctypes.c_uint64()
+        _check(lib().gv_num_edges(self.handle, ctypes.byref(n)))
+        return int(n.value)
+
+    def degrees(self):
+        out = np.empty(self.num_vertices, dtype=np.uint32)
+        _check(lib().gv_degrees(self.handle, out.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST))
+        return out
+
+    def pool_stats(self):
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().gv_pool_stats(self.handle, ctypes.byref(f), ctypes.byref(t)))
+        return int(f.value), int(t.value)
+
+    def export_csr(self):
+        """Sorted CSR of the live graph: (offsets[V+1] uint64, cols uint32)."""
+        off = np.empty(self.num_vertices + 1, dtype=np.uint64)
+        _check(lib().gv_export_csr(self.handle, off.ctypes.data_as(ctypes.c_void_p), None,
+                                   GV_MEM_HOST))
+        cols = np.empty(int(off[-1]), dtype=np.uint32)
+        _check(lib().gv_export_csr(self.handle, off.ctypes.data_as(ctypes.c_void_p),
+                                   cols.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST))
+        return off, cols
+
+    def export_csr_device(self, offsets, cols):
+        """Sorted CSR into device tensors (uint64/int64 offsets[V+1], int32 cols)."""
+        _check(lib().gv_export_csr(self.handle, ctypes.c_void_p(offsets.data_ptr()),
+                                   ctypes.c_void_p(cols.data_ptr()) if cols is not None else None,
+                                   GV_MEM_DEVICE))
+
+    def neighbors(self, vs):
+        """1-hop lookups: (offs[n+1], sorted neighbours concatenated)."""
+        v = np.ascontiguousarray(vs, dtype=np.uint32)
+        n = len(v)
+        offs = np.empty(n + 1, dtype=np.uint64)
+        _check(lib().gv_neighbors(self.handle, v.ctypes.data_as(ctypes.c_void_p), n,
+                                  offs.ctypes.data_as(ctypes.c_void_p), None, GV_MEM_HOST))
+        out = np.empty(int(offs[-1]), dtype=np.uint32)
+        _check(lib().gv_neighbors(self.handle, v.ctypes.data_as(ctypes.c_void_p), n,
+                                  offs.ctypes.data_as(ctypes.c_void_p),
+                                  out.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST))
+        return offs, out
+
+
+# reference: static_pagerank (pagerank.cu:3-68)
+def static_pagerank(graph, alpha=0.85, epsilon=1e-6, max_iter=100):
+    """Returns (x float32[V], iterations, final L1 residual)."""
+    x = np.empty(graph.num_vertices, dtype=np.float32)
+    it = ctypes.c_int()
+    res = ctypes.c_double()
+    _check(lib().gv_pagerank(graph.handle, float(alpha), float(epsilon), int(max_iter),
+                             x.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST, ctypes.byref(it),
+                             ctypes.byref(res)))
+    return x, int(it.value), float(res.value)
+
+
+# reference: static_tc (tc.cu:3-27)
+def static_tc(graph, per_vertex=True):
+    """Returns (per-vertex uint64 counts or None, total)."""
+    pv = np.empty(graph.num_vertices, dtype=np.uint64) if per_vertex else None
+    tot = ctypes.c_uint64()
+    _check(lib().gv_triangle_count(graph.handle,
+                                   pv.ctypes.data_as(ctypes.c_void_p) if per_vertex else None,
+                                   GV_MEM_HOST, ctypes.byref(tot)))
+    return pv, int(tot.value)
+
+
+# reference: bc (bc.cu:3-88); sources are explicit (the reference ignores its
+# argument and runs every vertex, bc.cu:42)
+def bc(graph, sources):
+    s = np.ascontiguousarray(sources, dtype=np.uint32)
+    out = np.empty(graph.num_vertices, dtype=np.float32)
+    _check(lib().gv_bc(graph.handle, s.ctypes.data_as(ctypes.c_void_p), len(s),
+                       out.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST))
+    return out

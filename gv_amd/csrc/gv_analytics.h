@@ -1,0 +1,56 @@
+// gv_analytics.h -- host-side entry points of PageRank, triangle counting and
+// betweenness centrality over the device-resident store (gv_analytics.hip).
+#pragma once
+#include "gv_internal.h"
+
+namespace gv {
+
+// Device state of one PageRank run.
+struct PrCtl {
+  double residual;
+  unsigned int iters;
+  unsigned int done;
+  unsigned int blocks_done;
+  unsigned int n_bin[4];
+  unsigned int cursor[4];
+  unsigned int n_chunks;      // chunks of the chunked bin (counted)
+  unsigned int chunk_cursor;  // chunk allocation cursor
+  unsigned int pad;
+};
+
+struct PrWork {
+  uint32_t V = 0;
+  float *x[2] = {nullptr, nullptr};
+  float *contrib[2] = {nullptr, nullptr};
+  uint32_t *list = nullptr;      // vertices grouped by degree bin
+  uint32_t *hub_base = nullptr;  // per chunked vertex: first chunk partial
+  uint32_t *hub_cnt = nullptr;   // per chunked vertex: chunks finished this iteration
+  uint32_t *chunk_hub = nullptr; // per chunk: index into the chunked-vertex list
+  float *chunk_part = nullptr;   // per chunk: (sum, compensation)
+  double *partials = nullptr;    // per block residual partials, then per hub
+  uint64_t chunk_cap = 0, part_cap = 0;
+  PrCtl *ctl = nullptr;
+  PrCtl *h_ctl = nullptr;        // pinned host mirror
+};
+
+struct PrBins {
+  uint32_t thread_max, group_max, wave_max;
+};
+
+gv_status pr_reserve(PrWork &w, uint32_t V);
+void pr_release(PrWork &w);
+// Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
+// graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
+gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins, float alpha,
+                       double eps, int max_iter, float *out_dev, int *iters, double *residual,
+                       PrWork &w, hipStream_t s);
+
+// Triangle count on an undirected store.  per_vertex_dev may be null.
+gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,
+                             hipStream_t s);
+
+// Brandes BC from the given device source list; out_dev[V] receives BC.
+gv_status bc_run(const Store &st, const uint32_t *sources_host, uint32_t n_sources,
+                 float *out_dev, hipStream_t s);
+
+}  // namespace gv

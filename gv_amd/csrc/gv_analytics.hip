@@ -1,0 +1,736 @@
+// gv_analytics.hip -- PageRank, triangle counting and betweenness centrality
+// over the device-resident block store.
+//
+// PageRank (static_pagerank, pagerank.cu:3-68): pull form over the
+// in-adjacency so every vertex's sum is computed by one fixed lane schedule
+// (deterministic, no FP32 atomics).  Vertices are binned by slot count:
+// one lane, 8 lanes, one wavefront, or PR chunks of wave_max slots on many
+// wavefronts whose partials are combined in chunk order.  Per-lane Kahan sums
+// are merged with TwoSum.  The L1 residual is reduced in FP64 in a fixed
+// order by the last block of each iteration, which also sets the done flag,
+// so iterations need no host round trip.
+//
+// Triangle counting (static_tc, tc.cu:3-27, tc_kernel.cu:3-74): degree-ordered
+// orientation (rank = (degree, id)), sorted oriented lists, one wavefront per
+// root vertex intersecting N+(v) for all v in N+(u) against N+(u) staged in
+// LDS.  Each triangle is found once; per-vertex counts attribute it to its
+// minimum vertex ID.
+//
+// BC (bc, bc.cu:3-88): level-synchronous Brandes.  Forward: frontier expansion
+// with CAS on dist and FP64 atomic sigma (integer-valued, so order-free and
+// exact to 2^53).  Backward: per level, a pull over out-neighbours one level
+// deeper with FP32 Kahan sums.
+#include "gv_analytics.h"
+#include "gv_store.h"
+
+#include <vector>
+
+namespace gv {
+
+// ---------------------------------------------------------------- helpers --
+__device__ __forceinline__ void two_sum_merge(float &s, float &c, float s2, float c2) {
+  float t = s + s2;
+  float bp = t - s;
+  float e = (s - (t - bp)) + (s2 - bp);
+  s = t;
+  c = c + c2 - e;
+}
+
+__device__ __forceinline__ unsigned int wave_atomic_add32(unsigned int *ctr, uint32_t inc) {
+  uint32_t incl = wave_incl_scan<uint32_t>(inc);
+  uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+  uint32_t base = 0;
+  if (lane_id() == WAVE - 1 && total) base = atomicAdd(ctr, total);
+  base = __shfl(base, WAVE - 1, WAVE);
+  return base + incl - inc;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum_f64(double v) {
+  __shared__ double ws[NT / WAVE];
+  v = wave_sum(v);
+  const int w = threadIdx.x / WAVE;
+  if (lane_id() == 0) ws[w] = v;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) t += ws[i];
+  __syncthreads();
+  return t;
+}
+
+// =============================================================== PageRank ==
+constexpr int PR_THREADS = 256;
+constexpr int PR_GROUP = 8;
+
+gv_status pr_reserve(PrWork &w, uint32_t V) {
+  if (w.V >= V && w.ctl) return GV_OK;
+  pr_release(w);
+  w.V = V;
+  size_t n = (size_t)(V ? V : 1);
+  for (int i = 0; i < 2; ++i) {
+    GV_HIP(hipMalloc(&w.x[i], n * sizeof(float)));
+    GV_HIP(hipMalloc(&w.contrib[i], n * sizeof(float)));
+  }
+  GV_HIP(hipMalloc(&w.list, n * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&w.hub_base, n * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&w.hub_cnt, n * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&w.ctl, sizeof(PrCtl)));
+  GV_HIP(hipHostMalloc(&w.h_ctl, sizeof(PrCtl), hipHostMallocDefault));
+  return GV_OK;
+}
+
+void pr_release(PrWork &w) {
+  for (int i = 0; i < 2; ++i) {
+    hipFree(w.x[i]);
+    hipFree(w.contrib[i]);
+  }
+  hipFree(w.list);
+  hipFree(w.hub_base);
+  hipFree(w.hub_cnt);
+  hipFree(w.chunk_hub);
+  hipFree(w.chunk_part);
+  hipFree(w.partials);
+  hipFree(w.ctl);
+  if (w.h_ctl) hipHostFree(w.h_ctl);
+  w = PrWork();
+}
+
+__device__ __forceinline__ int pr_bin(uint32_t used, PrBins b) {
+  return used <= b.thread_max ? 0 : used <= b.group_max ? 1 : used <= b.wave_max ? 2 : 3;
+}
+
+__global__ void k_pr_reset(PrCtl *c) {
+  PrCtl z = {};
+  *c = z;
+}
+
+__global__ void k_pr_init(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t d = deg_out[v];
+    x[v] = x0;
+    cb[v] = d ? x0 / (float)d : 0.f;
+  }
+}
+
+__global__ void k_pr_count(const uint32_t *used, uint32_t V, PrBins b, PrCtl *ctl) {
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t v = base + threadIdx.x;
+    int bin = -1;
+    uint32_t nch = 0;
+    if (v < V) {
+      uint32_t u = used[v];
+      bin = pr_bin(u, b);
+      if (bin == 3) nch = (u + b.wave_max - 1) / b.wave_max;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t c = wave_sum<uint32_t>(bin == k ? 1u : 0u);
+      if (lane_id() == 0 && c) atomicAdd(&ctl->n_bin[k], c);
+    }
+    uint32_t cc = wave_sum<uint32_t>(nch);
+    if (lane_id() == 0 && cc) atomicAdd(&ctl->n_chunks, cc);
+  }
+}
+
+__global__ void k_pr_place(const uint32_t *used, uint32_t V, PrBins b, uint32_t *list,
+                           PrCtl *ctl) {
+  const uint32_t o1 = ctl->n_bin[0], o2 = o1 + ctl->n_bin[1], o3 = o2 + ctl->n_bin[2];
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t v = base + threadIdx.x;
+    int bin = -1;
+    if (v < V) bin = pr_bin(used[v], b);
+    uint32_t p0 = wave_atomic_add32(&ctl->cursor[0], bin == 0);
+    uint32_t p1 = wave_atomic_add32(&ctl->cursor[1], bin == 1);
+    uint32_t p2 = wave_atomic_add32(&ctl->cursor[2], bin == 2);
+    uint32_t p3 = wave_atomic_add32(&ctl->cursor[3], bin == 3);
+    if (bin == 0) list[p0] = (uint32_t)v;
+    if (bin == 1) list[o1 + p1] = (uint32_t)v;
+    if (bin == 2) list[o2 + p2] = (uint32_t)v;
+    if (bin == 3) list[o3 + p3] = (uint32_t)v;
+  }
+}
+
+// Chunk lists of the vertices in the chunked bin.  Chunk c of hub h lives at
+// hub_base[h] + c, so partials combine in chunk order whatever the schedule.
+__global__ void k_pr_hubs(const uint32_t *used, const uint32_t *list, uint32_t o3, uint32_t n3,
+                          uint32_t wave_max, uint32_t *hub_base, uint32_t *hub_cnt,
+                          uint32_t *chunk_hub, unsigned int *cursor) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n3;
+       h += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t v = list[o3 + h];
+    uint32_t nch = (used[v] + wave_max - 1) / wave_max;
+    uint32_t b = atomicAdd(cursor, nch);
+    hub_base[h] = b;
+    hub_cnt[h] = 0;
+    for (uint32_t c = 0; c < nch; ++c) chunk_hub[b + c] = (uint32_t)h;
+  }
+}
+
+struct PrArgs {
+  Store in;
+  const uint32_t *deg_out;
+  const uint32_t *list;
+  const uint32_t *hub_base;
+  uint32_t *hub_cnt;
+  const uint32_t *chunk_hub;
+  float2 *chunk_part;
+  double *partials;
+  PrCtl *ctl;
+  float alpha, base;
+  double eps;
+  uint32_t n0, n1, n2, n3, o1, o2, o3;
+  uint32_t nb0, nb1, nb2, nb3, nblocks;
+  uint32_t n_chunks, wave_max;
+  uint32_t max_iter;
+};
+
+__device__ __forceinline__ void pr_acc(const PrArgs &a, const float *cb, uint32_t tbl, uint32_t pos,
+                                       float &s, float &c) {
+  uint32_t w = a.in.slots[slot_index(a.in.arena, tbl, pos, a.in.log2B, a.in.B)];
+  if (w != TOMB) kahan_add(s, c, cb[w]);
+}
+
+__device__ __forceinline__ double pr_finish(const PrArgs &a, uint32_t v, float s, float c,
+                                            const float *x, float *xn, float *cn) {
+  float xv = a.base + a.alpha * (s - c);
+  xn[v] = xv;
+  uint32_t d = a.deg_out[v];
+  cn[v] = d ? xv / (float)d : 0.f;
+  return fabs((double)xv - (double)x[v]);
+}
+
+__global__ __launch_bounds__(PR_THREADS) void k_pr_iter(PrArgs a, const float *x, const float *cb,
+                                                        float *xn, float *cn) {
+  if (a.ctl->done) return;
+  const uint32_t b = blockIdx.x;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = tid / WAVE;
+  double diff = 0.0;
+  if (b < a.nb0) {
+    uint32_t i = b * PR_THREADS + tid;
+    if (i < a.n0) {
+      uint32_t v = a.list[i];
+      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      float s = 0.f, c = 0.f;
+      for (uint32_t j = 0; j < us; ++j) pr_acc(a, cb, tbl, j, s, c);
+      diff = pr_finish(a, v, s, c, x, xn, cn);
+    }
+  } else if (b < a.nb0 + a.nb1) {
+    uint32_t g = ((b - a.nb0) * PR_THREADS + tid) / PR_GROUP;
+    uint32_t gl = tid & (PR_GROUP - 1);
+    uint32_t v = 0, us = 0, tbl = 0;
+    if (g < a.n1) {
+      v = a.list[a.o1 + g];
+      us = a.in.used[v];
+      tbl = a.in.tbl_off[v];
+    }
+    float s = 0.f, c = 0.f;
+    for (uint32_t j = gl; j < us; j += PR_GROUP) pr_acc(a, cb, tbl, j, s, c);
+#pragma unroll
+    for (int d = PR_GROUP / 2; d >= 1; d >>= 1) {
+      float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
+      two_sum_merge(s, c, s2, c2);
+    }
+    if (g < a.n1 && gl == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
+  } else if (b < a.nb0 + a.nb1 + a.nb2) {
+    uint32_t g = (b - a.nb0 - a.nb1) * (PR_THREADS / WAVE) + wid;
+    if (g < a.n2) {
+      uint32_t v = a.list[a.o2 + g];
+      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      float s = 0.f, c = 0.f;
+      for (uint32_t j = lane; j < us; j += WAVE) pr_acc(a, cb, tbl, j, s, c);
+#pragma unroll
+      for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
+        two_sum_merge(s, c, s2, c2);
+      }
+      if (lane == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
+    }
+  } else {
+    uint32_t ci = (b - a.nb0 - a.nb1 - a.nb2) * (PR_THREADS / WAVE) + wid;
+    if (ci < a.n_chunks) {
+      uint32_t h = a.chunk_hub[ci];
+      uint32_t v = a.list[a.o3 + h];
+      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      uint32_t hb = a.hub_base[h];
+      uint32_t c0 = (ci - hb) * a.wave_max;
+      uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
+      float s = 0.f, c = 0.f;
+      for (uint32_t j = c0 + lane; j < c1; j += WAVE) pr_acc(a, cb, tbl, j, s, c);
+#pragma unroll
+      for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
+        two_sum_merge(s, c, s2, c2);
+      }
+      if (lane == 0) {
+        a.chunk_part[ci] = make_float2(s, c);
+        __threadfence();
+        uint32_t nch = (us + a.wave_max - 1) / a.wave_max;
+        uint32_t old = atomicAdd(&a.hub_cnt[h], 1u);
+        if (old == nch - 1) {
+          __threadfence();
+          float ts = 0.f, tc = 0.f;
+          for (uint32_t k = 0; k < nch; ++k) {
+            float2 p = a.chunk_part[hb + k];
+            two_sum_merge(ts, tc, p.x, p.y);
+          }
+          a.partials[a.nblocks + h] = pr_finish(a, v, ts, tc, x, xn, cn);
+          a.hub_cnt[h] = 0;
+        }
+      }
+    }
+  }
+  double bs = block_sum_f64<PR_THREADS>(diff);
+  __shared__ bool last;
+  if (tid == 0) {
+    a.partials[b] = bs;
+    __threadfence();
+    unsigned t = atomicAdd(&a.ctl->blocks_done, 1u);
+    last = (t == a.nblocks - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // Fixed-order reduction of all block and hub partials.
+  const uint32_t np = a.nblocks + a.n3;
+  double r = 0.0;
+  for (uint32_t i = tid; i < np; i += PR_THREADS) r += a.partials[i];
+  r = block_sum_f64<PR_THREADS>(r);
+  if (tid == 0) {
+    a.ctl->residual = r;
+    unsigned it = a.ctl->iters + 1;
+    a.ctl->iters = it;
+    if (r < a.eps || it >= a.max_iter) a.ctl->done = 1;
+    a.ctl->blocks_done = 0;
+  }
+}
+
+gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins, float alpha,
+                       double eps, int max_iter, float *out_dev, int *iters, double *residual,
+                       PrWork &w, hipStream_t s) {
+  const uint32_t V = in.V;
+  if (iters) *iters = 0;
+  if (residual) *residual = 0.0;
+  if (!V) return GV_OK;
+  gv_status st = pr_reserve(w, V);
+  if (st) return st;
+  const float x0 = 1.0f / (float)V;
+  k_pr_reset<<<1, 1, 0, s>>>(w.ctl);
+  k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0);
+  if (max_iter <= 0) {
+    GV_HIP(hipMemcpyAsync(out_dev, w.x[0], (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return GV_OK;
+  }
+  k_pr_count<<<grid_for(V), 256, 0, s>>>(in.used, V, bins, w.ctl);
+  GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  const PrCtl hc = *w.h_ctl;
+  const uint32_t n0 = hc.n_bin[0], n1 = hc.n_bin[1], n2 = hc.n_bin[2], n3 = hc.n_bin[3];
+  const uint32_t nchunks = hc.n_chunks;
+  if (nchunks > w.chunk_cap) {
+    hipFree(w.chunk_hub);
+    hipFree(w.chunk_part);
+    w.chunk_cap = nchunks + 1024;
+    GV_HIP(hipMalloc(&w.chunk_hub, w.chunk_cap * sizeof(uint32_t)));
+    GV_HIP(hipMalloc(&w.chunk_part, w.chunk_cap * sizeof(float2)));
+  }
+  k_pr_place<<<grid_for(V), 256, 0, s>>>(in.used, V, bins, w.list, w.ctl);
+  PrArgs a;
+  a.in = in;
+  a.deg_out = deg_out;
+  a.list = w.list;
+  a.hub_base = w.hub_base;
+  a.hub_cnt = w.hub_cnt;
+  a.chunk_hub = w.chunk_hub;
+  a.chunk_part = (float2 *)w.chunk_part;
+  a.ctl = w.ctl;
+  a.alpha = alpha;
+  a.base = (1.0f - alpha) / (float)V;
+  a.eps = eps;
+  a.n0 = n0;
+  a.n1 = n1;
+  a.n2 = n2;
+  a.n3 = n3;
+  a.o1 = n0;
+  a.o2 = n0 + n1;
+  a.o3 = n0 + n1 + n2;
+  a.nb0 = (uint32_t)cdiv(n0, PR_THREADS);
+  a.nb1 = (uint32_t)cdiv((uint64_t)n1 * PR_GROUP, PR_THREADS);
+  a.nb2 = (uint32_t)cdiv(n2, PR_THREADS / WAVE);
+  a.nb3 = (uint32_t)cdiv(nchunks, PR_THREADS / WAVE);
+  a.nblocks = a.nb0 + a.nb1 + a.nb2 + a.nb3;
+  a.n_chunks = nchunks;
+  a.wave_max = bins.wave_max;
+  a.max_iter = (uint32_t)max_iter;
+  if (n3)
+    k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(in.used, w.list, a.o3, n3, bins.wave_max, w.hub_base,
+                                           w.hub_cnt, w.chunk_hub, &w.ctl->chunk_cursor);
+  if (a.nblocks + n3 > w.part_cap) {
+    hipFree(w.partials);
+    w.part_cap = a.nblocks + n3 + 1024;
+    GV_HIP(hipMalloc(&w.partials, w.part_cap * sizeof(double)));
+  }
+  a.partials = w.partials;
+  int launched = 0;
+  const int check_every = 8;
+  while (launched < max_iter) {
+    int t = launched;
+    k_pr_iter<<<a.nblocks, PR_THREADS, 0, s>>>(a, w.x[t & 1], w.contrib[t & 1], w.x[(t + 1) & 1],
+                                               w.contrib[(t + 1) & 1]);
+    ++launched;
+    if (launched % check_every == 0 && launched < max_iter) {
+      GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+      GV_HIP(hipStreamSynchronize(s));
+      if (w.h_ctl->done) break;
+    }
+  }
+  GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  const uint32_t it = w.h_ctl->iters;
+  GV_HIP(hipMemcpyAsync(out_dev, w.x[it & 1], (size_t)V * sizeof(float), hipMemcpyDeviceToDevice,
+                        s));
+  if (iters) *iters = (int)it;
+  if (residual) *residual = w.h_ctl->residual;
+  return GV_OK;
+}
+
+// ===================================================== triangle counting ==
+constexpr uint32_t TC_LDS = 2048;  // oriented-list entries staged per wavefront
+
+__device__ __forceinline__ bool tc_higher(uint32_t dw, uint32_t w, uint32_t du, uint32_t u) {
+  return dw > du || (dw == du && w > u);
+}
+
+__global__ __launch_bounds__(256) void k_tc_deg(Store st, uint64_t *dplus) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < st.V;
+       u += nwaves) {
+    const uint32_t us = st.used[u], du = st.deg[u], tbl = st.tbl_off[u];
+    uint32_t cnt = 0;
+    for (uint32_t j = lane; j < us; j += WAVE) {
+      uint32_t w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
+      if (w != TOMB && tc_higher(st.deg[w], w, du, (uint32_t)u)) ++cnt;
+    }
+    cnt = wave_sum(cnt);
+    if (lane == 0) dplus[u] = cnt;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tc_fill(Store st, const uint64_t *off, uint32_t bV,
+                                                 uint64_t *keys) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < st.V;
+       u += nwaves) {
+    const uint32_t us = st.used[u], du = st.deg[u], tbl = st.tbl_off[u];
+    const uint64_t o = off[u], tag = u << bV;
+    uint32_t wpos = 0;
+    for (uint32_t base = 0; base < us; base += WAVE) {
+      uint32_t j = base + lane;
+      bool keep = false;
+      uint32_t w = TOMB;
+      if (j < us) {
+        w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
+        keep = w != TOMB && tc_higher(st.deg[w], w, du, (uint32_t)u);
+      }
+      unsigned long long m = __ballot(keep);
+      if (keep) keys[o + wpos + mask_lt_count(m)] = tag | w;
+      wpos += (uint32_t)__popcll(m);
+    }
+  }
+}
+
+__device__ __forceinline__ bool tc_find(const uint32_t *a, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && a[lo] == x;
+}
+
+template <bool PV>
+__global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uint32_t *cols,
+                                                  uint32_t V, unsigned long long *pv,
+                                                  unsigned long long *total) {
+  __shared__ uint32_t s_nbr[4][TC_LDS];
+  __shared__ uint32_t s_pre[4][WAVE];
+  __shared__ uint32_t s_v[4][WAVE];
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / WAVE;
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  unsigned long long cnt = 0;
+  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < V; u += nwaves) {
+    const uint64_t ou = off[u];
+    const uint32_t du = (uint32_t)(off[u + 1] - ou);
+    if (du < 2) continue;
+    const uint32_t *nu = cols + ou;
+    const bool lds = du <= TC_LDS;
+    if (lds)
+      for (uint32_t j = lane; j < du; j += WAVE) s_nbr[wid][j] = nu[j];
+    const uint32_t *su = lds ? s_nbr[wid] : nu;
+    unsigned long long own = 0;
+    for (uint32_t c0 = 0; c0 < du; c0 += WAVE) {
+      uint32_t j = c0 + lane;
+      uint32_t v = 0, len = 0;
+      uint64_t ov = 0;
+      if (j < du) {
+        v = su[j];
+        ov = off[v];
+        len = (uint32_t)(off[v + 1] - ov);
+      }
+      uint32_t pre = wave_incl_scan<uint32_t>(len);
+      uint32_t W = __shfl(pre, WAVE - 1, WAVE);
+      s_pre[wid][lane] = pre;
+      s_v[wid][lane] = v;
+      for (uint32_t fb = 0; fb < W; fb += WAVE) {
+        uint32_t f = fb + lane;
+        if (f < W) {
+          uint32_t lo = 0, hi = WAVE - 1;  // first k with pre[k] > f
+          while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s_pre[wid][mid] > f) hi = mid; else lo = mid + 1;
+          }
+          uint32_t vk = s_v[wid][lo];
+          uint32_t start = lo ? s_pre[wid][lo - 1] : 0u;
+          uint32_t wv = cols[off[vk] + (f - start)];
+          if (tc_find(su, du, wv)) {
+            ++cnt;
+            if (PV) {
+              uint32_t mn = (uint32_t)u < vk ? (uint32_t)u : vk;
+              mn = mn < wv ? mn : wv;
+              if (mn == (uint32_t)u) ++own;
+              else atomicAdd(&pv[mn], 1ull);
+            }
+          }
+        }
+      }
+    }
+    if (PV) {
+      own = wave_sum(own);
+      if (lane == 0 && own) atomicAdd(&pv[u], own);
+    }
+  }
+  cnt = wave_sum(cnt);
+  if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
+gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,
+                             hipStream_t s) {
+  const uint32_t V = st.V;
+  *total = 0;
+  if (per_vertex_dev) fill_u64(per_vertex_dev, 0, V, s);
+  if (!V) return GV_OK;
+  const uint32_t bV = id_bits(V);
+  uint64_t *off = nullptr, *tot_dev = nullptr;
+  void *tmp = nullptr;
+  GV_HIP(hipMalloc(&off, ((size_t)V + 1) * sizeof(uint64_t)));
+  GV_HIP(hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1)));
+  GV_HIP(hipMalloc(&tot_dev, sizeof(uint64_t)));
+  k_tc_deg<<<grid_for((uint64_t)V * WAVE, 256, 8192), 256, 0, s>>>(st, off);
+  scan_u64(off, off, V, nullptr, tmp, s);
+  uint64_t E = 0;
+  GV_HIP(hipMemcpyAsync(&E, off + V, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  uint64_t *k0 = nullptr, *k1 = nullptr;
+  uint32_t *cols = nullptr;
+  void *stmp = nullptr;
+  gv_status rc = GV_OK;
+  do {
+    if (hipMalloc(&k0, (E ? E : 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&k1, (E ? E : 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&cols, (E ? E : 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&stmp, sort_tmp_bytes(E)) != hipSuccess) {
+      set_error("triangle count: device allocation failed");
+      rc = GV_E_NOMEM;
+      break;
+    }
+    k_tc_fill<<<grid_for((uint64_t)V * WAVE, 256, 8192), 256, 0, s>>>(st, off, bV, k0);
+    uint64_t *sorted = sort_keys(k0, k1, E, 2 * bV, stmp, s);
+    keys_to_cols(sorted, E, bV, cols, s);
+    fill_u64(tot_dev, 0, 1, s);
+    unsigned grid = grid_for((uint64_t)V * WAVE, 256, 8192);
+    if (per_vertex_dev)
+      k_tc_count<true><<<grid, 256, 0, s>>>(off, cols, V, (unsigned long long *)per_vertex_dev,
+                                            (unsigned long long *)tot_dev);
+    else
+      k_tc_count<false><<<grid, 256, 0, s>>>(off, cols, V, nullptr,
+                                             (unsigned long long *)tot_dev);
+    if (hipMemcpyAsync(total, tot_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("triangle count: HIP error");
+      rc = GV_E_HIP;
+    }
+  } while (0);
+  hipFree(k0);
+  hipFree(k1);
+  hipFree(cols);
+  hipFree(stmp);
+  hipFree(off);
+  hipFree(tmp);
+  hipFree(tot_dev);
+  return rc;
+}
+
+// ========================================================= betweenness ==
+__global__ void k_bc_init(int *dist, double *sigma, float *delta, uint32_t V) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    dist[v] = -1;
+    sigma[v] = 0.0;
+    delta[v] = 0.f;
+  }
+}
+
+__global__ void k_bc_src(int *dist, double *sigma, uint32_t *queue, uint32_t src,
+                         unsigned int *cursor) {
+  dist[src] = 0;
+  sigma[src] = 1.0;
+  queue[0] = src;
+  *cursor = 0;
+}
+
+__global__ __launch_bounds__(256) void k_bc_fwd(Store st, int *dist, double *sigma,
+                                                uint32_t *queue, uint32_t qs, uint32_t qe, int d,
+                                                unsigned int *cursor) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint32_t nf = qe - qs;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < nf; i += nwaves) {
+    const uint32_t u = queue[qs + i];
+    const double su = sigma[u];
+    const uint32_t us = st.used[u], tbl = st.tbl_off[u];
+    for (uint32_t base = 0; base < us; base += WAVE) {
+      uint32_t j = base + lane;
+      bool fresh = false;
+      uint32_t w = TOMB;
+      if (j < us) {
+        w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
+        if (w != TOMB) {
+          int old = atomicCAS(&dist[w], -1, d + 1);
+          fresh = old == -1;
+          if (fresh || old == d + 1) atomicAdd(&sigma[w], su);
+        }
+      }
+      unsigned long long m = __ballot(fresh);
+      uint32_t pos = 0;
+      if (lane == 0 && m) pos = atomicAdd(cursor, (unsigned)__popcll(m));
+      pos = __shfl(pos, 0, WAVE);
+      if (fresh) queue[qe + pos + mask_lt_count(m)] = w;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bc_bwd(Store st, const int *dist, const double *sigma,
+                                                float *delta, const uint32_t *queue, uint32_t qs,
+                                                uint32_t qe, int d, uint32_t src, float *bc,
+                                                float *bcc) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint32_t nf = qe - qs;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < nf; i += nwaves) {
+    const uint32_t v = queue[qs + i];
+    const double sv = sigma[v];
+    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
+    float s = 0.f, c = 0.f;
+    for (uint32_t j = lane; j < us; j += WAVE) {
+      uint32_t w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
+      if (w != TOMB && dist[w] == d + 1) {
+        float coef = (float)(sv / sigma[w]);
+        kahan_add(s, c, coef * (1.0f + delta[w]));
+      }
+    }
+#pragma unroll
+    for (int k = WAVE / 2; k >= 1; k >>= 1) {
+      float s2 = __shfl_xor(s, k, WAVE), c2 = __shfl_xor(c, k, WAVE);
+      two_sum_merge(s, c, s2, c2);
+    }
+    if (lane == 0) {
+      float dv = s - c;
+      delta[v] = dv;
+      if (v != src) {
+        float b = bc[v], bcomp = bcc[v];
+        kahan_add(b, bcomp, dv);
+        bc[v] = b;
+        bcc[v] = bcomp;
+      }
+    }
+  }
+}
+
+gv_status bc_run(const Store &st, const uint32_t *sources, uint32_t n_sources, float *out_dev,
+                 hipStream_t s) {
+  const uint32_t V = st.V;
+  if (!V) return GV_OK;
+  int *dist = nullptr;
+  double *sigma = nullptr;
+  float *delta = nullptr, *bcc = nullptr;
+  uint32_t *queue = nullptr;
+  unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  gv_status rc = GV_OK;
+  do {
+    if (hipMalloc(&dist, (size_t)V * sizeof(int)) != hipSuccess ||
+        hipMalloc(&sigma, (size_t)V * sizeof(double)) != hipSuccess ||
+        hipMalloc(&delta, (size_t)V * sizeof(float)) != hipSuccess ||
+        hipMalloc(&bcc, (size_t)V * sizeof(float)) != hipSuccess ||
+        hipMalloc(&queue, (size_t)V * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&cursor, sizeof(unsigned int)) != hipSuccess ||
+        hipHostMalloc(&h_cursor, sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
+      set_error("bc: device allocation failed");
+      rc = GV_E_NOMEM;
+      break;
+    }
+    fill_f32(out_dev, 0.f, V, s);
+    fill_f32(bcc, 0.f, V, s);
+    for (uint32_t si = 0; si < n_sources && rc == GV_OK; ++si) {
+      const uint32_t src = sources[si];
+      k_bc_init<<<grid_for(V), 256, 0, s>>>(dist, sigma, delta, V);
+      k_bc_src<<<1, 1, 0, s>>>(dist, sigma, queue, src, cursor);
+      std::vector<uint32_t> lvl{0u, 1u};
+      int d = 0;
+      while (true) {
+        uint32_t qs = lvl[d], qe = lvl[d + 1];
+        if (qe == qs) break;
+        fill_u32(cursor, 0, 1, s);
+        k_bc_fwd<<<grid_for((uint64_t)(qe - qs) * WAVE, 256, 8192), 256, 0, s>>>(
+            st, dist, sigma, queue, qs, qe, d, cursor);
+        if (hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+          set_error("bc: HIP error");
+          rc = GV_E_HIP;
+          break;
+        }
+        lvl.push_back(qe + *h_cursor);
+        ++d;
+      }
+      // levels 0..D-1 are non-empty; level D-1 has no successors
+      const int D = (int)lvl.size() - 2;
+      for (int lev = D - 2; lev >= 0 && rc == GV_OK; --lev) {
+        uint32_t qs = lvl[lev], qe = lvl[lev + 1];
+        k_bc_bwd<<<grid_for((uint64_t)(qe - qs) * WAVE, 256, 8192), 256, 0, s>>>(
+            st, dist, sigma, delta, queue, qs, qe, lev, src, out_dev, bcc);
+      }
+    }
+    if (rc == GV_OK && hipStreamSynchronize(s) != hipSuccess) {
+      set_error("bc: HIP error");
+      rc = GV_E_HIP;
+    }
+  } while (0);
+  hipFree(dist);
+  hipFree(sigma);
+  hipFree(delta);
+  hipFree(bcc);
+  hipFree(queue);
+  hipFree(cursor);
+  if (h_cursor) hipHostFree(h_cursor);
+  return rc;
+}
+
+}  // namespace gv

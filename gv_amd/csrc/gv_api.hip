@@ -1,0 +1,614 @@
+// gv_api.hip -- the C ABI (include/gv.h) over the store and analytics
+// kernels.  Host orchestration only: every device step is a kernel launch on
+// the handle's own stream; the host waits only where a call must return a
+// value (stats, sizes, analytic results).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gv_analytics.h"
+#include "gv_store.h"
+
+namespace gv {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+gv_status hip_fail(hipError_t e, const char *what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? GV_E_NOMEM : GV_E_HIP;
+}
+}  // namespace gv
+
+using namespace gv;
+
+struct gv_graph {
+  gv_config cfg;
+  uint32_t V = 0;
+  int directed = 0;
+  int has_in = 0;
+  hipStream_t stream = nullptr;
+  Store st[2];
+  Scratch sc[2];
+  uint32_t *vlist = nullptr;
+  PrWork pr;
+  Ctl *h_ctl = nullptr;  // pinned, 2 entries
+  uint32_t *in_src = nullptr, *in_dst = nullptr;
+  uint64_t in_cap = 0;
+};
+
+extern "C" {
+
+const char *gv_last_error(void) { return g_err.c_str(); }
+
+gv_status gv_config_default(gv_config *cfg) {
+  if (!cfg) return GV_E_INVALID;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->block_slots = 64;
+  cfg->pool_blocks = 0;
+  cfg->pool_fraction = 0.25;
+  cfg->compact_threshold = 0.25f;
+  cfg->pr_thread_max = 8;
+  cfg->pr_group_max = 96;
+  cfg->pr_wave_max = 4096;
+  cfg->maintain_in_edges = 1;
+  cfg->device = 0;
+  return GV_OK;
+}
+
+}  // extern "C"
+
+static gv_status check_graph(gv_graph *g) {
+  if (!g) {
+    set_error("null graph handle");
+    return GV_E_INVALID;
+  }
+  hipError_t e = hipSetDevice(g->cfg.device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return GV_OK;
+}
+
+static gv_status read_ctl(gv_graph *g) {
+  int n = g->has_in ? 2 : 1;
+  for (int k = 0; k < n; ++k)
+    GV_HIP(hipMemcpyAsync(&g->h_ctl[k], g->st[k].ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                          g->stream));
+  GV_HIP(hipStreamSynchronize(g->stream));
+  return GV_OK;
+}
+
+__global__ void k_clear_abort(Ctl *c) { c->abort = 0; }
+
+// Report and clear a sticky abort code left by the device.
+static gv_status take_abort(gv_graph *g) {
+  unsigned ab = g->h_ctl[0].abort;
+  if (g->has_in && !ab) ab = g->h_ctl[1].abort;
+  if (!ab) return GV_OK;
+  for (int k = 0; k < (g->has_in ? 2 : 1); ++k)
+    k_clear_abort<<<1, 1, 0, g->stream>>>(g->st[k].ctl);
+  GV_HIP(hipStreamSynchronize(g->stream));
+  if (ab == ABORT_RANGE) {
+    set_error("vertex ID out of range; batch not applied");
+    return GV_E_RANGE;
+  }
+  if (ab == ABORT_POOL) {
+    set_error("edge-block pool exhausted; batch not applied");
+    return GV_E_POOL;
+  }
+  set_error("device abort");
+  return GV_E_HIP;
+}
+
+// Bring a host or device (src,dst) batch to device pointers.
+static gv_status device_inputs(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                               int mem, const uint32_t **ds, const uint32_t **dd) {
+  if (mem == GV_MEM_DEVICE) {
+    *ds = src;
+    *dd = dst;
+    return GV_OK;
+  }
+  if (mem != GV_MEM_HOST) {
+    set_error("mem must be GV_MEM_HOST or GV_MEM_DEVICE");
+    return GV_E_INVALID;
+  }
+  if (n > g->in_cap) {
+    hipFree(g->in_src);
+    hipFree(g->in_dst);
+    g->in_src = g->in_dst = nullptr;
+    uint64_t cap = std::max<uint64_t>(n, 1024);
+    GV_HIP(hipMalloc(&g->in_src, cap * sizeof(uint32_t)));
+    GV_HIP(hipMalloc(&g->in_dst, cap * sizeof(uint32_t)));
+    g->in_cap = cap;
+  }
+  GV_HIP(hipMemcpyAsync(g->in_src, src, n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+  GV_HIP(hipMemcpyAsync(g->in_dst, dst, n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+  *ds = g->in_src;
+  *dd = g->in_dst;
+  return GV_OK;
+}
+
+static void fill_stats(gv_graph *g, uint64_t n, gv_batch_stats *stats) {
+  const Ctl &c = g->h_ctl[0];
+  stats->requested = n;
+  stats->directed_entries = c.n_valid;
+  stats->applied = c.applied;
+  stats->touched = c.n_segs;
+  stats->blocks_allocated = c.need_blocks + (g->has_in ? g->h_ctl[1].need_blocks : 0);
+  stats->live_edges = c.live_edges;
+}
+
+enum class Op { Insert, Delete };
+
+static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                             int mem, gv_batch_stats *stats, Op op) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (n && (!src || !dst)) {
+    set_error("null batch pointer");
+    return GV_E_INVALID;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return GV_OK;
+  if (n > 0x7FFFFFFFull) {
+    set_error("batch larger than 2^31 pairs; split it");
+    return GV_E_INVALID;
+  }
+  const uint32_t *ds = nullptr, *dd = nullptr;
+  rc = device_inputs(g, src, dst, n, mem, &ds, &dd);
+  if (rc) return rc;
+  hipStream_t s = g->stream;
+  const int nst = g->has_in ? 2 : 1;
+  uint64_t m[2] = {0, 0};
+  uint64_t *keys[2] = {nullptr, nullptr};
+  for (int k = 0; k < nst; ++k) {
+    const int mirror = g->directed ? 0 : 1;
+    rc = scratch_reserve(g->sc[k], g->st[k], mirror ? 2 * n : n);
+    if (rc) return rc;
+    op_begin(g->st[k], s);
+    keys[k] = stage(g->st[k], g->sc[k], ds, dd, n, mirror, k == 1 ? 1 : 0, m[k], s);
+    segment(g->st[k], g->sc[k], keys[k], m[k], s);
+    if (op == Op::Insert) insert_plan(g->st[k], g->sc[k], keys[k], m[k], s);
+    else delete_apply(g->st[k], g->sc[k], keys[k], m[k], s);
+  }
+  if (op == Op::Insert) {
+    commit_pair(g->st[0], g->sc[0], nst > 1 ? &g->st[1] : nullptr, nst > 1 ? &g->sc[1] : nullptr, s);
+    for (int k = 0; k < nst; ++k) insert_write(g->st[k], g->sc[k], keys[k], m[k], s);
+  }
+  GV_HIP(hipGetLastError());
+  if (!stats) return GV_OK;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  fill_stats(g, n, stats);
+  return take_abort(g);
+}
+
+extern "C" {
+
+gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed,
+                    gv_graph **out) {
+  if (!out) {
+    set_error("null output handle");
+    return GV_E_INVALID;
+  }
+  *out = nullptr;
+  gv_config cfg;
+  if (cfg_in) cfg = *cfg_in; else gv_config_default(&cfg);
+  if (cfg.block_slots != 32 && cfg.block_slots != 64 && cfg.block_slots != 128) {
+    set_error("block_slots must be 32, 64 or 128");
+    return GV_E_INVALID;
+  }
+  if (num_vertices >= 0xFFFFFFFFu) {
+    set_error("num_vertices must be below 2^32-1 (0xFFFFFFFF marks a tombstone)");
+    return GV_E_INVALID;
+  }
+  if (cfg.pr_thread_max > cfg.pr_group_max || cfg.pr_group_max > cfg.pr_wave_max ||
+      cfg.pr_wave_max == 0) {
+    set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
+    return GV_E_INVALID;
+  }
+  hipError_t e = hipSetDevice(cfg.device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  gv_graph *g = new gv_graph();
+  g->cfg = cfg;
+  g->V = num_vertices;
+  g->directed = directed ? 1 : 0;
+  g->has_in = (directed && cfg.maintain_in_edges) ? 1 : 0;
+  const int nst = g->has_in ? 2 : 1;
+  uint64_t P = cfg.pool_blocks;
+  const uint64_t bytes_per_block = (uint64_t)cfg.block_slots * 4 + 4 + 16;
+  if (P == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)1 << 32;
+    double frac = cfg.pool_fraction > 0 && cfg.pool_fraction < 0.9 ? cfg.pool_fraction : 0.25;
+    P = (uint64_t)((double)fr * frac / (double)bytes_per_block) / nst;
+  }
+  P = std::max<uint64_t>(P, 64);
+  P = std::min<uint64_t>(P, 0x3FFFFFFFull);
+  uint64_t A = std::min<uint64_t>(4 * P + (uint64_t)num_vertices + 1024, 0xFFFFFFF0ull);
+  gv_status rc = GV_OK;
+  do {
+    if ((e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking)) != hipSuccess) {
+      rc = hip_fail(e, "hipStreamCreate");
+      break;
+    }
+    for (int k = 0; k < nst && !rc; ++k) rc = store_alloc(g->st[k], num_vertices, cfg.block_slots, P, A);
+    if (rc) break;
+    if ((e = hipMalloc(&g->vlist, (size_t)(num_vertices ? num_vertices : 1) * 4)) != hipSuccess) {
+      rc = hip_fail(e, "hipMalloc");
+      break;
+    }
+    if ((e = hipHostMalloc(&g->h_ctl, 2 * sizeof(Ctl), hipHostMallocDefault)) != hipSuccess) {
+      rc = hip_fail(e, "hipHostMalloc");
+      break;
+    }
+    std::memset(g->h_ctl, 0, 2 * sizeof(Ctl));
+    for (int k = 0; k < nst; ++k) store_reset(g->st[k], g->stream);
+    if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) rc = hip_fail(e, "store init");
+  } while (0);
+  if (rc) {
+    gv_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return GV_OK;
+}
+
+gv_status gv_destroy(gv_graph *g) {
+  if (!g) return GV_OK;
+  hipSetDevice(g->cfg.device);
+  if (g->stream) hipStreamSynchronize(g->stream);
+  for (int k = 0; k < 2; ++k) {
+    store_release(g->st[k]);
+    scratch_release(g->sc[k]);
+  }
+  pr_release(g->pr);
+  hipFree(g->vlist);
+  hipFree(g->in_src);
+  hipFree(g->in_dst);
+  if (g->h_ctl) hipHostFree(g->h_ctl);
+  if (g->stream) hipStreamDestroy(g->stream);
+  delete g;
+  return GV_OK;
+}
+
+gv_status gv_bulk_build(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n, int mem,
+                        gv_batch_stats *stats) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  for (int k = 0; k < (g->has_in ? 2 : 1); ++k) store_reset(g->st[k], g->stream);
+  return apply_batch(g, src, dst, n, mem, stats, Op::Insert);
+}
+
+gv_status gv_batch_insert(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                          int mem, gv_batch_stats *stats) {
+  return apply_batch(g, src, dst, n, mem, stats, Op::Insert);
+}
+
+gv_status gv_batch_delete(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                          int mem, gv_batch_stats *stats) {
+  return apply_batch(g, src, dst, n, mem, stats, Op::Delete);
+}
+
+gv_status gv_compact(gv_graph *g, float frac, uint64_t *vertices_compacted) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (frac < 0.f) frac = g->cfg.compact_threshold;
+  for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
+    op_begin(g->st[k], g->stream);
+    compact(g->st[k], frac, g->vlist, g->stream);
+  }
+  GV_HIP(hipGetLastError());
+  if (!vertices_compacted) return GV_OK;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  *vertices_compacted = g->h_ctl[0].compacted;
+  return GV_OK;
+}
+
+gv_status gv_sync(gv_graph *g) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  return take_abort(g);
+}
+
+gv_status gv_get_stream(gv_graph *g, void **hip_stream) {
+  if (!g || !hip_stream) return GV_E_INVALID;
+  *hip_stream = (void *)g->stream;
+  return GV_OK;
+}
+
+gv_status gv_num_vertices(gv_graph *g, uint32_t *v) {
+  if (!g || !v) return GV_E_INVALID;
+  *v = g->V;
+  return GV_OK;
+}
+
+gv_status gv_num_edges(gv_graph *g, uint64_t *live) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!live) return GV_E_INVALID;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  *live = g->h_ctl[0].live_edges;
+  return GV_OK;
+}
+
+gv_status gv_pool_stats(gv_graph *g, uint64_t *free_blocks, uint64_t *total_blocks) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  uint64_t f = 0, t = 0;
+  for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
+    f += g->h_ctl[k].free_top;
+    t += g->st[k].P;
+  }
+  if (free_blocks) *free_blocks = f;
+  if (total_blocks) *total_blocks = t;
+  return GV_OK;
+}
+
+gv_status gv_degrees(gv_graph *g, uint32_t *deg, int mem) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!deg) return GV_E_INVALID;
+  GV_HIP(hipMemcpyAsync(deg, g->st[0].deg, (size_t)g->V * 4,
+                        mem == GV_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        g->stream));
+  GV_HIP(hipStreamSynchronize(g->stream));
+  return GV_OK;
+}
+
+gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!offsets) return GV_E_INVALID;
+  hipStream_t s = g->stream;
+  const uint32_t V = g->V, bV = id_bits(V);
+  uint64_t *off = nullptr, *k0 = nullptr, *k1 = nullptr;
+  uint32_t *dcols = nullptr;
+  void *tmp = nullptr;
+  uint64_t E = 0;
+  do {
+    hipError_t e;
+    if ((e = hipMalloc(&off, ((size_t)V + 1) * 8)) != hipSuccess ||
+        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess) {
+      rc = hip_fail(e, "export alloc");
+      break;
+    }
+    degrees_u64(g->st[0], off, s);
+    scan_u64(off, off, V, nullptr, tmp, s);
+    if ((e = hipMemcpyAsync(&E, off + V, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      rc = hip_fail(e, "export");
+      break;
+    }
+    if ((e = hipMemcpyAsync(offsets, off, ((size_t)V + 1) * 8,
+                            mem == GV_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                            s)) != hipSuccess) {
+      rc = hip_fail(e, "export");
+      break;
+    }
+    if (cols && E) {
+      hipFree(tmp);
+      tmp = nullptr;
+      if ((e = hipMalloc(&k0, E * 8)) != hipSuccess || (e = hipMalloc(&k1, E * 8)) != hipSuccess ||
+          (e = hipMalloc(&tmp, sort_tmp_bytes(E))) != hipSuccess) {
+        rc = hip_fail(e, "export alloc");
+        break;
+      }
+      if (mem != GV_MEM_DEVICE && (e = hipMalloc(&dcols, E * 4)) != hipSuccess) {
+        rc = hip_fail(e, "export alloc");
+        break;
+      }
+      export_keys(g->st[0], off, k0, s);
+      uint64_t *sorted = sort_keys(k0, k1, E, 2 * bV, tmp, s);
+      uint32_t *cdst = mem == GV_MEM_DEVICE ? cols : dcols;
+      keys_to_cols(sorted, E, bV, cdst, s);
+      if (mem != GV_MEM_DEVICE &&
+          (e = hipMemcpyAsync(cols, dcols, E * 4, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+        rc = hip_fail(e, "export");
+        break;
+      }
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) rc = hip_fail(e, "export");
+  } while (0);
+  hipFree(off);
+  hipFree(k0);
+  hipFree(k1);
+  hipFree(dcols);
+  hipFree(tmp);
+  return rc;
+}
+
+gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs, uint32_t *out,
+                       int mem) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!offs || (n && !vs)) return GV_E_INVALID;
+  hipStream_t s = g->stream;
+  const uint32_t V = g->V, bV = id_bits(V), qb = id_bits(n ? n : 1);
+  if (qb + bV > 64) {
+    set_error("too many queries for one call");
+    return GV_E_INVALID;
+  }
+  if (mem == GV_MEM_HOST)
+    for (uint32_t i = 0; i < n; ++i)
+      if (vs[i] >= V) {
+        set_error("query vertex out of range");
+        return GV_E_RANGE;
+      }
+  const hipMemcpyKind out_kind = mem == GV_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  uint32_t *dvs = nullptr, *dout = nullptr;
+  uint64_t *cnt = nullptr, *k0 = nullptr, *k1 = nullptr;
+  void *tmp = nullptr;
+  uint64_t T = 0;
+  do {
+    hipError_t e;
+    if ((e = hipMalloc(&cnt, ((size_t)n + 1) * 8)) != hipSuccess ||
+        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)n + 1))) != hipSuccess) {
+      rc = hip_fail(e, "neighbors alloc");
+      break;
+    }
+    const uint32_t *qv = vs;
+    if (mem != GV_MEM_DEVICE && n) {
+      if ((e = hipMalloc(&dvs, (size_t)n * 4)) != hipSuccess ||
+          (e = hipMemcpyAsync(dvs, vs, (size_t)n * 4, hipMemcpyHostToDevice, s)) != hipSuccess) {
+        rc = hip_fail(e, "neighbors");
+        break;
+      }
+      qv = dvs;
+    }
+    neighbor_counts(g->st[0], qv, n, cnt, s);
+    scan_u64(cnt, cnt, n, nullptr, tmp, s);
+    if ((e = hipMemcpyAsync(&T, cnt + n, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(offs, cnt, ((size_t)n + 1) * 8, out_kind, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      rc = hip_fail(e, "neighbors");
+      break;
+    }
+    if (out && T) {
+      hipFree(tmp);
+      tmp = nullptr;
+      if ((e = hipMalloc(&k0, T * 8)) != hipSuccess || (e = hipMalloc(&k1, T * 8)) != hipSuccess ||
+          (e = hipMalloc(&tmp, sort_tmp_bytes(T))) != hipSuccess ||
+          (mem != GV_MEM_DEVICE && (e = hipMalloc(&dout, T * 4)) != hipSuccess)) {
+        rc = hip_fail(e, "neighbors alloc");
+        break;
+      }
+      neighbor_keys(g->st[0], qv, n, cnt, qb, k0, s);
+      uint64_t *sorted = sort_keys(k0, k1, T, qb + bV, tmp, s);
+      uint32_t *cdst = mem == GV_MEM_DEVICE ? out : dout;
+      keys_to_cols(sorted, T, bV, cdst, s);
+      if (mem != GV_MEM_DEVICE &&
+          (e = hipMemcpyAsync(out, dout, T * 4, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+        rc = hip_fail(e, "neighbors");
+        break;
+      }
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) rc = hip_fail(e, "neighbors");
+  } while (0);
+  hipFree(dvs);
+  hipFree(dout);
+  hipFree(cnt);
+  hipFree(k0);
+  hipFree(k1);
+  hipFree(tmp);
+  return rc;
+}
+
+gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
+                      int *iters, double *residual) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!out) return GV_E_INVALID;
+  if (g->directed && !g->has_in) {
+    set_error("PageRank on a directed graph needs maintain_in_edges");
+    return GV_E_UNSUPPORTED;
+  }
+  const Store &in = g->has_in ? g->st[1] : g->st[0];
+  PrBins bins{g->cfg.pr_thread_max, g->cfg.pr_group_max, g->cfg.pr_wave_max};
+  float *dout = out;
+  if (mem != GV_MEM_DEVICE) {
+    hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
+    if (e != hipSuccess) return hip_fail(e, "pagerank alloc");
+  }
+  rc = pagerank_run(in, g->st[0].deg, bins, alpha, eps, max_iter, dout, iters, residual, g->pr,
+                    g->stream);
+  if (!rc && mem != GV_MEM_DEVICE) {
+    hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "pagerank copy");
+  }
+  if (mem != GV_MEM_DEVICE) hipFree(dout);
+  return rc;
+}
+
+gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!total) return GV_E_INVALID;
+  if (g->directed) {
+    set_error("triangle counting is defined on undirected graphs");
+    return GV_E_UNSUPPORTED;
+  }
+  uint64_t *dpv = per_vertex;
+  if (per_vertex && mem != GV_MEM_DEVICE) {
+    hipError_t e = hipMalloc(&dpv, (size_t)(g->V ? g->V : 1) * 8);
+    if (e != hipSuccess) return hip_fail(e, "tc alloc");
+  }
+  rc = triangle_count_run(g->st[0], dpv, total, g->stream);
+  if (!rc && per_vertex && mem != GV_MEM_DEVICE) {
+    hipError_t e = hipMemcpyAsync(per_vertex, dpv, (size_t)g->V * 8, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "tc copy");
+  }
+  if (per_vertex && mem != GV_MEM_DEVICE) hipFree(dpv);
+  return rc;
+}
+
+gv_status gv_bc(gv_graph *g, const uint32_t *sources, uint32_t n_sources, float *out, int mem) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!out || (n_sources && !sources)) return GV_E_INVALID;
+  std::vector<uint32_t> hs(n_sources);
+  if (n_sources) {
+    if (mem == GV_MEM_DEVICE) {
+      GV_HIP(hipMemcpy(hs.data(), sources, (size_t)n_sources * 4, hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(hs.data(), sources, (size_t)n_sources * 4);
+    }
+  }
+  for (uint32_t x : hs)
+    if (x >= g->V) {
+      set_error("BC source out of range");
+      return GV_E_RANGE;
+    }
+  float *dout = out;
+  if (mem != GV_MEM_DEVICE) {
+    hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
+    if (e != hipSuccess) return hip_fail(e, "bc alloc");
+  }
+  rc = bc_run(g->st[0], hs.data(), n_sources, dout, g->stream);
+  if (!rc && mem != GV_MEM_DEVICE) {
+    hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "bc copy");
+  }
+  if (mem != GV_MEM_DEVICE) hipFree(dout);
+  return rc;
+}
+
+gv_status gv_gen_rmat(gv_graph *g, uint64_t key, int scale, uint64_t n, uint32_t ta, uint32_t tab,
+                      uint32_t tabc, uint32_t *src_dev, uint32_t *dst_dev) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (scale < 1 || scale > 31 || (n && (!src_dev || !dst_dev))) return GV_E_INVALID;
+  gen_rmat(key, scale, n, ta, tab, tabc, src_dev, dst_dev, g->stream);
+  GV_HIP(hipGetLastError());
+  return GV_OK;
+}
+
+gv_status gv_gen_uniform(gv_graph *g, uint64_t key, uint32_t V, uint64_t n, uint32_t *src_dev,
+                         uint32_t *dst_dev) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (n && (!src_dev || !dst_dev)) return GV_E_INVALID;
+  gen_uniform(key, V, n, src_dev, dst_dev, g->stream);
+  GV_HIP(hipGetLastError());
+  return GV_OK;
+}
+
+gv_status gv_gen_sample(gv_graph *g, uint64_t key, uint32_t V, const uint64_t *offsets_dev,
+                        const uint32_t *cols_dev, uint64_t n, uint32_t *src_dev,
+                        uint32_t *dst_dev) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (n && (!src_dev || !dst_dev || !offsets_dev)) return GV_E_INVALID;
+  gen_sample(key, V, offsets_dev, cols_dev, n, src_dev, dst_dev, g->stream);
+  GV_HIP(hipGetLastError());
+  return GV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// gv_internal.h -- device data model and shared primitives of the MI355X
+// GraphVine store.  Not part of the public ABI (include/gv.h is).
+//
+// Layout in HBM (one Store per adjacency direction):
+//   slots[P*B]      edge-block pool; block b owns slots [b*B, (b+1)*B), B in
+//                   {32,64,128} uint32 destination IDs (128/256/512 B, i.e.
+//                   whole 128 B lines; B=64 is one dword per lane of a wave).
+//   free_ids[P]     free-block stack; entries [0, free_top) are free block IDs.
+//   deg[V]          live out-degree |N(v)|.
+//   used[V]         slots in use (live + tombstones); slots [0, used) valid.
+//   tbl_off[V]      offset of v's block table in arena[].
+//   tbl_cap[V]      capacity of that table (power of two, 0 = none).
+//   arena[A]        block tables: arena[tbl_off[v] + i] = i-th block of v in
+//                   level order.  The reference's complete binary tree of
+//                   blocks (children 2i+1, 2i+2; device_functions.cu:42-100)
+//                   is kept as this level order, but block i is addressed in
+//                   O(1) by index instead of a root-to-node pointer chase.
+// Slot j of v lives at slots[arena[tbl_off[v] + (j >> log2B)] * B + (j & (B-1))].
+// A tombstone is TOMB (0xFFFFFFFF); IDs must be < 0xFFFFFFFF.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/gv.h"
+
+namespace gv {
+
+constexpr uint32_t TOMB = 0xFFFFFFFFu;
+constexpr uint64_t KEY_SENT = ~0ull;
+constexpr int WAVE = 64;
+
+// Device-resident control block of one Store.  Counters are 64-bit so the
+// pool and arena can exceed 2^32 slots.
+struct Ctl {
+  unsigned long long free_top;    // number of free block IDs
+  unsigned long long arena_top;   // bump pointer into arena
+  unsigned long long live_edges;  // directed live edges
+  // per-operation scratch counters (reset by k_op_begin)
+  unsigned long long need_blocks;
+  unsigned long long need_arena;
+  unsigned long long blk_base;    // absolute index into free_ids of this op's blocks
+  unsigned long long arena_base;  // absolute arena offset of this op's tables
+  unsigned long long applied;
+  unsigned long long blocks_freed;
+  unsigned long long compacted;
+  unsigned int n_valid;           // valid (non-sentinel) keys of the staged batch
+  unsigned int n_segs;            // source segments of the staged batch
+  unsigned int n_items;           // work items of the membership/tombstone pass
+  unsigned int abort;             // sticky error code (positive GV error magnitude)
+  unsigned int pad[4];
+};
+
+struct Store {
+  uint32_t V = 0, B = 64, log2B = 6;
+  uint64_t P = 0;
+  uint64_t A = 0;  // arena entries
+  uint32_t *slots = nullptr;
+  uint32_t *free_ids = nullptr;
+  uint32_t *deg = nullptr, *used = nullptr, *tbl_off = nullptr, *tbl_cap = nullptr;
+  uint32_t *arena = nullptr;
+  Ctl *ctl = nullptr;
+};
+
+// Batch staging scratch, sized for `cap` directed entries.
+struct Scratch {
+  uint64_t cap = 0;
+  uint64_t item_cap = 0;
+  uint64_t *k0 = nullptr, *k1 = nullptr;  // radix sort ping-pong
+  uint32_t *flag = nullptr;               // per key: 1 = survives
+  uint32_t *segof = nullptr;              // per key: inclusive head count (= seg + 1)
+  uint32_t *rank = nullptr;               // per key: exclusive survivor prefix (cap+1)
+  uint32_t *seg_start = nullptr;          // per segment: first key (cap+1)
+  uint32_t *seg_src = nullptr;            // per segment: source vertex
+  uint32_t *seg_used = nullptr;           // per segment: used slots before the op
+  uint32_t *seg_cnt = nullptr;            // per segment: items, then blocks/deletes
+  uint32_t *item_start = nullptr;         // per segment: first item (cap+1)
+  uint32_t *item_seg = nullptr;           // per item: segment
+  uint32_t *seg_blk = nullptr;            // per segment: relative block base
+  uint32_t *seg_tbl = nullptr;            // per segment: relative arena base or ~0
+  void *tmp = nullptr;                    // scan/sort partials
+  size_t tmp_bytes = 0;
+};
+
+// ------------------------------------------------------------ host errors --
+void set_error(const std::string &msg);
+gv_status hip_fail(hipError_t e, const char *what);
+
+#define GV_HIP(call)                                              \
+  do {                                                            \
+    hipError_t _e = (call);                                       \
+    if (_e != hipSuccess) return ::gv::hip_fail(_e, #call);       \
+  } while (0)
+
+inline uint32_t ceil_log2(uint64_t x) {
+  uint32_t r = 0;
+  while ((1ull << r) < x) ++r;
+  return r;
+}
+inline uint32_t id_bits(uint32_t V) {
+  uint32_t b = ceil_log2(V);
+  return b ? b : 1;
+}
+inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// ------------------------------------------------------- device helpers --
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// popcount of mask bits below this lane
+__device__ __forceinline__ uint32_t mask_lt_count(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Inclusive prefix sum over all 64 lanes (every lane must be active).
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    T o = __shfl_up(v, d, WAVE);
+    if (l >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
+  return v;
+}
+
+// Wave-aggregated atomic add: every lane of the wave calls it (inc may be 0);
+// one atomic per wave; returns this lane's exclusive base.
+__device__ __forceinline__ unsigned long long wave_atomic_add(unsigned long long *ctr,
+                                                              uint32_t inc) {
+  unsigned long long incl = wave_incl_scan<unsigned long long>(inc);
+  unsigned long long total = __shfl(incl, WAVE - 1, WAVE);
+  unsigned long long base = 0;
+  if (lane_id() == WAVE - 1 && total) base = atomicAdd(ctr, total);
+  base = __shfl(base, WAVE - 1, WAVE);
+  return base + incl - inc;
+}
+
+__device__ __forceinline__ uint32_t slot_index(const uint32_t *arena, uint32_t tbl, uint32_t pos,
+                                               uint32_t log2B, uint32_t B) {
+  uint32_t blk = arena[tbl + (pos >> log2B)];
+  return blk * B + (pos & (B - 1));
+}
+
+// TwoSum-compensated pair (value, compensation) add.
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+  float y = x - c;
+  float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// --------------------------------------------------------- primitives --
+// (gv_prims.hip)
+size_t scan_tmp_bytes(uint64_t n);
+// Exclusive scan of in[0..n_eff) into out[0..n_eff], out[n_eff] = total, with
+// n_eff = n_dev ? min(n, *n_dev) : n.  in may alias out.
+void scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *n_dev, void *tmp,
+              hipStream_t s);
+void scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, const uint32_t *n_dev, void *tmp,
+              hipStream_t s);
+size_t sort_tmp_bytes(uint64_t n);
+// LSD radix sort of n keys on their low `bits` bits; returns the buffer that
+// holds the result (a or b).
+uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp, hipStream_t s);
+
+}  // namespace gv

@@ -1,0 +1,64 @@
+// gv_store.h -- host-side entry points of the store kernels (gv_store.hip).
+#pragma once
+#include "gv_internal.h"
+
+namespace gv {
+
+constexpr uint32_t PROBE_CHUNK = 512;  // slots per membership/tombstone work item
+
+// Abort codes kept in Ctl::abort (magnitudes of the GV_E_* values).
+constexpr uint32_t ABORT_RANGE = 2;
+constexpr uint32_t ABORT_POOL = 4;
+
+gv_status store_alloc(Store &st, uint32_t V, uint32_t B, uint64_t P, uint64_t A);
+void store_release(Store &st);
+void store_reset(Store &st, hipStream_t s);
+gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m);
+void scratch_release(Scratch &sc);
+
+// Per-op counter reset (keeps the sticky abort code).
+void op_begin(Store &st, hipStream_t s);
+
+// Stage n (src,dst) pairs as sorted keys in sc; mirror adds (dst,src) too,
+// swap stores (dst,src) instead.  Returns the sorted key buffer; m = keys.
+uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                int mirror, int swap, uint64_t &m, hipStream_t s);
+// Segment the staged keys and build the probe work list.
+void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// Insert: probe existing slots, plan block/table allocation.
+void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// Joint all-or-nothing pool check over one or two stores (b may be null).
+void commit_pair(Store &a, Scratch &sa, Store *b, Scratch *sb, hipStream_t s);
+void insert_write(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// Delete: tombstone matches and fix degrees.
+void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// Compaction of vertices whose tombstones exceed frac of their slots.
+void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s);
+
+// Read-back helpers.
+void degrees_u64(const Store &st, uint64_t *out, hipStream_t s);
+void export_keys(const Store &st, const uint64_t *offsets, uint64_t *keys, hipStream_t s);
+void keys_to_cols(const uint64_t *keys, uint64_t n, uint32_t bV, uint32_t *cols, hipStream_t s);
+void neighbor_counts(const Store &st, const uint32_t *vs, uint32_t n, uint64_t *cnt,
+                     hipStream_t s);
+void neighbor_keys(const Store &st, const uint32_t *vs, uint32_t n, const uint64_t *offs,
+                   uint32_t qbits, uint64_t *keys, hipStream_t s);
+
+// Generators.
+void gen_rmat(uint64_t key, int scale, uint64_t n, uint32_t ta, uint32_t tab, uint32_t tabc,
+              uint32_t *src, uint32_t *dst, hipStream_t s);
+void gen_uniform(uint64_t key, uint32_t V, uint64_t n, uint32_t *src, uint32_t *dst,
+                 hipStream_t s);
+void gen_sample(uint64_t key, uint32_t V, const uint64_t *offsets, const uint32_t *cols,
+                uint64_t n, uint32_t *src, uint32_t *dst, hipStream_t s);
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
+void fill_f32(float *p, float v, uint64_t n, hipStream_t s);
+
+inline unsigned grid_for(uint64_t n, unsigned threads = 256, unsigned cap = 4096) {
+  uint64_t g = cdiv(n ? n : 1, threads);
+  return (unsigned)(g < cap ? g : cap);
+}
+
+}  // namespace gv

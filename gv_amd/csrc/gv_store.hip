@@ -1,0 +1,704 @@
+// gv_store.hip -- batch staging, insert, delete, compaction and read-back
+// kernels of the MI355X GraphVine store.
+//
+// Update pipeline for one batch (all stream-ordered, no host round trip):
+//   k_stage        (src,dst) -> packed keys (u<<bV | v); self-loops and bad IDs
+//                  become the sentinel (sorts last); bad IDs set abort=RANGE.
+//   sort_keys      LSD radix sort on 2*bV bits.
+//   k_seg_mark     head-of-source-segment and intra-batch duplicate flags.
+//   scan           head flags -> segment index per key.
+//   k_seg_build    per segment: first key, source, slots in use, work items.
+//   scan + expand  work items of PROBE_CHUNK slots (hub adjacencies are split
+//                  across many wavefronts).
+//   k_probe        one wavefront per work item reads a chunk of the source's
+//                  slots (coalesced, one block per 64 lanes) and binary-searches
+//                  each live slot in the source's sorted batch segment:
+//                  insert -> mark the batch key as already present;
+//                  delete -> tombstone the slot.
+//   insert only:   scan of survivors -> k_ins_plan (wave-aggregated atomic
+//                  reservation of pool blocks and table space) -> k_commit
+//                  (all-or-nothing pool check) -> k_ins_tables -> k_ins_write.
+//   delete only:   k_del_finish (degrees, counters).
+// The reference does the same work with O(V) host and device passes per batch
+// (graphvine.cu:222-451); here every pass is O(batch + touched adjacency).
+#include "gv_store.h"
+
+namespace gv {
+
+// ------------------------------------------------------------------ fills --
+template <typename T>
+__global__ void k_fill(T *p, T v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (n) k_fill<uint32_t><<<grid_for(n), 256, 0, s>>>(p, v, n);
+}
+void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (n) k_fill<uint64_t><<<grid_for(n), 256, 0, s>>>(p, v, n);
+}
+void fill_f32(float *p, float v, uint64_t n, hipStream_t s) {
+  if (n) k_fill<float><<<grid_for(n), 256, 0, s>>>(p, v, n);
+}
+
+// ------------------------------------------------------------ store init --
+gv_status store_alloc(Store &st, uint32_t V, uint32_t B, uint64_t P, uint64_t A) {
+  st.V = V;
+  st.B = B;
+  st.log2B = ceil_log2(B);
+  st.P = P;
+  st.A = A;
+  if (A >= 0xFFFFFFFFull || P >= 0xFFFFFFFFull) {
+    set_error("pool or table arena exceeds 2^32 entries");
+    return GV_E_INVALID;
+  }
+  GV_HIP(hipMalloc(&st.slots, P * B * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.free_ids, P * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.deg, (size_t)(V ? V : 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.used, (size_t)(V ? V : 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.tbl_off, (size_t)(V ? V : 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.tbl_cap, (size_t)(V ? V : 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.arena, A * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.ctl, sizeof(Ctl)));
+  return GV_OK;
+}
+
+void store_release(Store &st) {
+  hipFree(st.slots);
+  hipFree(st.free_ids);
+  hipFree(st.deg);
+  hipFree(st.used);
+  hipFree(st.tbl_off);
+  hipFree(st.tbl_cap);
+  hipFree(st.arena);
+  hipFree(st.ctl);
+  st = Store();
+}
+
+__global__ void k_store_reset(uint32_t *free_ids, uint64_t P, uint32_t *deg, uint32_t *used,
+                              uint32_t *tbl_off, uint32_t *tbl_cap, uint32_t V, Ctl *ctl) {
+  uint64_t n = P > V ? P : V;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < P) free_ids[i] = (uint32_t)i;
+    if (i < V) {
+      deg[i] = 0;
+      used[i] = 0;
+      tbl_off[i] = 0;
+      tbl_cap[i] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Ctl c = {};
+    c.free_top = P;
+    *ctl = c;
+  }
+}
+
+void store_reset(Store &st, hipStream_t s) {
+  uint64_t n = st.P > st.V ? st.P : st.V;
+  k_store_reset<<<grid_for(n), 256, 0, s>>>(st.free_ids, st.P, st.deg, st.used, st.tbl_off,
+                                            st.tbl_cap, st.V, st.ctl);
+}
+
+gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m) {
+  uint64_t item_need = m + cdiv(st.P * (uint64_t)st.B, PROBE_CHUNK) + 1;
+  if (m <= sc.cap && item_need <= sc.item_cap) return GV_OK;
+  scratch_release(sc);
+  uint64_t cap = 1024;
+  while (cap < m) cap <<= 1;
+  sc.cap = cap;
+  sc.item_cap = cap + cdiv(st.P * (uint64_t)st.B, PROBE_CHUNK) + 1;
+  GV_HIP(hipMalloc(&sc.k0, cap * sizeof(uint64_t)));
+  GV_HIP(hipMalloc(&sc.k1, cap * sizeof(uint64_t)));
+  uint32_t **arrs[] = {&sc.flag, &sc.segof, &sc.rank, &sc.seg_start, &sc.seg_src, &sc.seg_used,
+                       &sc.seg_cnt, &sc.item_start, &sc.seg_blk, &sc.seg_tbl};
+  for (auto a : arrs) GV_HIP(hipMalloc(a, (cap + 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&sc.item_seg, sc.item_cap * sizeof(uint32_t)));
+  size_t t = sort_tmp_bytes(cap);
+  size_t t2 = scan_tmp_bytes(cap + 1);
+  sc.tmp_bytes = t > t2 ? t : t2;
+  GV_HIP(hipMalloc(&sc.tmp, sc.tmp_bytes));
+  return GV_OK;
+}
+
+void scratch_release(Scratch &sc) {
+  hipFree(sc.k0);
+  hipFree(sc.k1);
+  uint32_t *arrs[] = {sc.flag,     sc.segof,      sc.rank,    sc.seg_start, sc.seg_src,
+                      sc.seg_used, sc.seg_cnt,    sc.item_start, sc.seg_blk, sc.seg_tbl,
+                      sc.item_seg};
+  for (auto a : arrs) hipFree(a);
+  hipFree(sc.tmp);
+  sc = Scratch();
+}
+
+__global__ void k_op_begin(Ctl *ctl) {
+  ctl->need_blocks = 0;
+  ctl->need_arena = 0;
+  ctl->blk_base = 0;
+  ctl->arena_base = 0;
+  ctl->applied = 0;
+  ctl->blocks_freed = 0;
+  ctl->compacted = 0;
+  ctl->n_valid = 0;
+  ctl->n_segs = 0;
+  ctl->n_items = 0;
+}
+void op_begin(Store &st, hipStream_t s) { k_op_begin<<<1, 1, 0, s>>>(st.ctl); }
+
+// --------------------------------------------------------------- staging --
+__global__ void k_stage(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t V,
+                        uint32_t bV, int mirror, int swap, uint64_t *keys, Ctl *ctl) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = src[i], v = dst[i];
+    if (swap) {
+      uint32_t t = u;
+      u = v;
+      v = t;
+    }
+    bool bad = u >= V || v >= V;
+    if (bad) atomicCAS(&ctl->abort, 0u, ABORT_RANGE);
+    bool drop = bad || u == v;
+    uint64_t k = drop ? KEY_SENT : (((uint64_t)u << bV) | v);
+    if (mirror) {
+      keys[2 * i] = k;
+      keys[2 * i + 1] = drop ? KEY_SENT : (((uint64_t)v << bV) | u);
+    } else {
+      keys[i] = k;
+    }
+  }
+}
+
+uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                int mirror, int swap, uint64_t &m, hipStream_t s) {
+  m = mirror ? 2 * n : n;
+  if (!n) return sc.k0;
+  const uint32_t bV = id_bits(st.V);
+  k_stage<<<grid_for(n), 256, 0, s>>>(src, dst, n, st.V, bV, mirror, swap, sc.k0, st.ctl);
+  return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s);
+}
+
+// ---------------------------------------------------------- segmentation --
+__global__ void k_seg_mark(const uint64_t *keys, uint64_t m, uint32_t bV, uint32_t *head,
+                           uint32_t *flag, Ctl *ctl) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    bool valid = k != KEY_SENT;
+    uint64_t prev = i ? keys[i - 1] : KEY_SENT;
+    bool h = valid && (i == 0 || (prev >> bV) != (k >> bV));
+    bool dup = valid && i > 0 && prev == k;
+    head[i] = h ? 1u : 0u;
+    flag[i] = (valid && !dup) ? 1u : 0u;
+    if (valid) {
+      uint64_t next = (i + 1 < m) ? keys[i + 1] : KEY_SENT;
+      if (next == KEY_SENT) ctl->n_valid = (unsigned)(i + 1);
+    }
+  }
+}
+
+struct SegArgs {
+  uint32_t *segof, *seg_start, *seg_src, *seg_used, *seg_cnt, *seg_blk;
+};
+
+__global__ void k_seg_build(const uint64_t *keys, uint64_t m, uint32_t bV, SegArgs a,
+                            const uint32_t *used, Ctl *ctl) {
+  const uint32_t nv = ctl->n_valid;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    bool h = i == 0 || (keys[i - 1] >> bV) != (k >> bV);
+    uint32_t sidx = a.segof[i] + (h ? 1u : 0u) - 1u;
+    a.segof[i] = sidx;
+    if (h) {
+      uint32_t u = (uint32_t)(k >> bV);
+      uint32_t uo = used[u];
+      a.seg_start[sidx] = (uint32_t)i;
+      a.seg_src[sidx] = u;
+      a.seg_used[sidx] = uo;
+      a.seg_cnt[sidx] = (uo + PROBE_CHUNK - 1) / PROBE_CHUNK;
+      a.seg_blk[sidx] = 0;
+    }
+    if (i == 0) {
+      uint32_t T = a.segof[m];
+      a.seg_start[T] = nv;
+      ctl->n_segs = T;
+    }
+  }
+}
+
+__global__ void k_item_expand(const uint32_t *seg_cnt, const uint32_t *item_start,
+                              uint32_t *item_seg, Ctl *ctl) {
+  const uint32_t T = ctl->n_segs;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < T;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t b = item_start[s], c = seg_cnt[s];
+    for (uint32_t j = 0; j < c; ++j) item_seg[b + j] = (uint32_t)s;
+    if (s == 0) ctl->n_items = item_start[T];
+  }
+}
+
+void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  const uint32_t bV = id_bits(st.V);
+  k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, st.ctl);
+  scan_u32(sc.segof, sc.segof, m, nullptr, sc.tmp, s);
+  SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
+  k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.ctl);
+  scan_u32(sc.seg_cnt, sc.item_start, m, &st.ctl->n_segs, sc.tmp, s);
+  k_item_expand<<<grid_for(m), 256, 0, s>>>(sc.seg_cnt, sc.item_start, sc.item_seg, st.ctl);
+}
+
+// ----------------------------------------------------------------- probe --
+struct ProbeArgs {
+  const uint64_t *keys;
+  const uint32_t *item_seg, *item_start, *seg_src, *seg_used, *seg_start;
+  uint32_t *flag;     // insert: survivor flags
+  uint32_t *seg_del;  // delete: per-segment tombstone counts
+};
+
+template <bool DEL>
+__global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t bV, Ctl *ctl) {
+  if (DEL && ctl->abort) return;
+  const uint32_t nitems = ctl->n_items;
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < nitems;
+       it += nwaves) {
+    const uint32_t sidx = a.item_seg[it];
+    const uint32_t c = (uint32_t)it - a.item_start[sidx];
+    const uint32_t u = a.seg_src[sidx];
+    const uint32_t uo = a.seg_used[sidx];
+    const uint32_t tbl = st.tbl_off[u];
+    const uint32_t ks = a.seg_start[sidx], ke = a.seg_start[sidx + 1];
+    const uint64_t ukey = (uint64_t)u << bV;
+    uint32_t cnt = 0;
+#pragma unroll 2
+    for (uint32_t r = 0; r < PROBE_CHUNK / WAVE; ++r) {
+      uint32_t pos = c * PROBE_CHUNK + r * WAVE + lane;
+      if (pos < uo) {
+        uint32_t idx = slot_index(st.arena, tbl, pos, st.log2B, st.B);
+        uint32_t w = st.slots[idx];
+        if (w != TOMB) {
+          uint64_t key = ukey | w;
+          uint32_t lo = ks, hi = ke;
+          while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (a.keys[mid] < key) lo = mid + 1; else hi = mid;
+          }
+          if (lo < ke && a.keys[lo] == key) {
+            if (DEL) {
+              st.slots[idx] = TOMB;
+              ++cnt;
+            } else {
+              a.flag[lo] = 0;
+            }
+          }
+        }
+      }
+    }
+    if (DEL) {
+      cnt = wave_sum(cnt);
+      if (lane == 0 && cnt) atomicAdd(&a.seg_del[sidx], cnt);
+    }
+  }
+}
+
+static unsigned probe_grid(uint64_t items_bound) {
+  uint64_t g = cdiv(items_bound ? items_bound : 1, 4);  // 4 waves per block
+  return (unsigned)(g < 8192 ? g : 8192);
+}
+
+// ---------------------------------------------------------------- insert --
+struct PlanArgs {
+  const uint32_t *seg_src, *seg_used, *seg_start, *rank;
+  uint32_t *seg_blk, *seg_tbl;
+};
+
+__device__ __forceinline__ uint32_t pow2ceil(uint32_t x) {
+  return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
+}
+
+__global__ void k_ins_plan(Store st, PlanArgs a, Ctl *ctl) {
+  const uint32_t T = ctl->n_segs;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t sidx = base + threadIdx.x;
+    bool act = sidx < T;
+    uint32_t nblk = 0, ncap = 0;
+    if (act) {
+      uint32_t u = a.seg_src[sidx];
+      uint32_t uo = a.seg_used[sidx];
+      uint32_t surv = a.rank[a.seg_start[sidx + 1]] - a.rank[a.seg_start[sidx]];
+      uint32_t nbo = (uo + st.B - 1) >> st.log2B;
+      uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
+      nblk = nbn - nbo;
+      if (nbn > st.tbl_cap[u]) ncap = pow2ceil(nbn);
+    }
+    unsigned long long rb = wave_atomic_add(&ctl->need_blocks, nblk);
+    unsigned long long ra = wave_atomic_add(&ctl->need_arena, ncap);
+    if (act) {
+      a.seg_blk[sidx] = (uint32_t)rb;
+      a.seg_tbl[sidx] = ncap ? (uint32_t)ra : 0xFFFFFFFFu;
+    }
+  }
+}
+
+__device__ bool commit_one(Ctl *c, uint64_t A) {
+  return c->need_blocks <= c->free_top && c->arena_top + c->need_arena <= A;
+}
+__device__ void commit_take(Ctl *c) {
+  c->blk_base = c->free_top - c->need_blocks;
+  c->free_top = c->blk_base;
+  c->arena_base = c->arena_top;
+  c->arena_top += c->need_arena;
+}
+
+// All-or-nothing: either both stores get their blocks or neither does.
+// rank_a / rank_b hold the survivor prefix of each store's staged batch.
+__global__ void k_commit(Ctl *a, uint64_t Aa, Ctl *b, uint64_t Ab, const uint32_t *rank_a,
+                         const uint32_t *rank_b) {
+  unsigned ab = a->abort | (b ? b->abort : 0u);
+  if (ab) {
+    a->abort = a->abort ? a->abort : ab;
+    if (b) b->abort = b->abort ? b->abort : ab;
+    return;
+  }
+  bool ok = commit_one(a, Aa) && (!b || commit_one(b, Ab));
+  if (!ok) {
+    a->abort = ABORT_POOL;
+    if (b) b->abort = ABORT_POOL;
+    return;
+  }
+  commit_take(a);
+  unsigned long long ap = rank_a[a->n_valid];
+  a->applied = ap;
+  a->live_edges += ap;
+  if (b) {
+    commit_take(b);
+    unsigned long long bp = rank_b[b->n_valid];
+    b->applied = bp;
+    b->live_edges += bp;
+  }
+}
+
+struct TblArgs {
+  const uint32_t *seg_src, *seg_used, *seg_start, *rank, *seg_blk, *seg_tbl;
+};
+
+__global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
+  if (ctl->abort) return;
+  const uint32_t T = ctl->n_segs;
+  const uint64_t bb0 = ctl->blk_base, ab0 = ctl->arena_base;
+  for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx < T;
+       sidx += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = a.seg_src[sidx];
+    uint32_t uo = a.seg_used[sidx];
+    uint32_t surv = a.rank[a.seg_start[sidx + 1]] - a.rank[a.seg_start[sidx]];
+    if (!surv) continue;
+    uint32_t nbo = (uo + st.B - 1) >> st.log2B;
+    uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
+    uint32_t tbl = st.tbl_off[u];
+    uint32_t rt = a.seg_tbl[sidx];
+    if (rt != 0xFFFFFFFFu) {
+      uint32_t noff = (uint32_t)(ab0 + rt);
+      for (uint32_t j = 0; j < nbo; ++j) st.arena[noff + j] = st.arena[tbl + j];
+      tbl = noff;
+      st.tbl_off[u] = noff;
+      st.tbl_cap[u] = pow2ceil(nbn);
+    }
+    uint64_t bb = bb0 + a.seg_blk[sidx];
+    for (uint32_t j = nbo; j < nbn; ++j) st.arena[tbl + j] = st.free_ids[bb + (j - nbo)];
+    st.used[u] = uo + surv;
+    st.deg[u] += surv;
+  }
+}
+
+struct WriteArgs {
+  const uint64_t *keys;
+  const uint32_t *flag, *segof, *rank, *seg_start, *seg_src, *seg_used;
+};
+
+__global__ void k_ins_write(Store st, WriteArgs a, uint32_t bV, Ctl *ctl) {
+  if (ctl->abort) return;
+  const uint32_t nv = ctl->n_valid;
+  const uint64_t mask = (1ull << bV) - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (!a.flag[i]) continue;
+    uint32_t sidx = a.segof[i];
+    uint32_t pos = a.seg_used[sidx] + (a.rank[i] - a.rank[a.seg_start[sidx]]);
+    uint32_t u = a.seg_src[sidx];
+    uint32_t idx = slot_index(st.arena, st.tbl_off[u], pos, st.log2B, st.B);
+    st.slots[idx] = (uint32_t)(a.keys[i] & mask);
+  }
+}
+
+void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  const uint32_t bV = id_bits(st.V);
+  ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, sc.flag,
+               nullptr};
+  k_probe<false><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+  scan_u32(sc.flag, sc.rank, m, &st.ctl->n_valid, sc.tmp, s);
+  PlanArgs pl{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.seg_blk, sc.seg_tbl};
+  k_ins_plan<<<grid_for(m), 256, 0, s>>>(st, pl, st.ctl);
+}
+
+void commit_pair(Store &a, Scratch &sa, Store *b, Scratch *sb, hipStream_t s) {
+  k_commit<<<1, 1, 0, s>>>(a.ctl, a.A, b ? b->ctl : nullptr, b ? b->A : 0, sa.rank,
+                           sb ? sb->rank : nullptr);
+}
+
+void insert_write(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  const uint32_t bV = id_bits(st.V);
+  TblArgs ta{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.seg_blk, sc.seg_tbl};
+  k_ins_tables<<<grid_for(m), 256, 0, s>>>(st, ta, st.ctl);
+  WriteArgs wa{keys, sc.flag, sc.segof, sc.rank, sc.seg_start, sc.seg_src, sc.seg_used};
+  k_ins_write<<<grid_for(m), 256, 0, s>>>(st, wa, bV, st.ctl);
+}
+
+// ---------------------------------------------------------------- delete --
+__global__ void k_del_finish(Store st, const uint32_t *seg_src, const uint32_t *seg_del,
+                             Ctl *ctl) {
+  if (ctl->abort) return;
+  const uint32_t T = ctl->n_segs;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t sidx = base + threadIdx.x;
+    uint32_t d = 0;
+    if (sidx < T) {
+      d = seg_del[sidx];
+      if (d) st.deg[seg_src[sidx]] -= d;
+    }
+    uint32_t tot = wave_sum(d);
+    if (lane_id() == 0 && tot) {
+      atomicAdd(&ctl->applied, (unsigned long long)tot);
+      atomicAdd(&ctl->live_edges, (unsigned long long)(0ull - (unsigned long long)tot));
+    }
+  }
+}
+
+void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  const uint32_t bV = id_bits(st.V);
+  ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, nullptr,
+               sc.seg_blk};
+  k_probe<true><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+  k_del_finish<<<grid_for(m), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+}
+
+// ------------------------------------------------------------ compaction --
+__global__ void k_cmp_select(Store st, float frac, uint32_t *vlist, Ctl *ctl) {
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < st.V;
+       base += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t v = base + threadIdx.x;
+    bool sel = false;
+    if (v < st.V) {
+      uint32_t us = st.used[v], d = st.deg[v];
+      uint32_t tomb = us - d;
+      sel = tomb > 0 && (frac <= 0.f || (float)tomb > frac * (float)us);
+    }
+    unsigned long long pos = wave_atomic_add(&ctl->compacted, sel ? 1u : 0u);
+    if (sel) vlist[pos] = (uint32_t)v;
+  }
+}
+
+// One wavefront packs one vertex in place: round r reads 64 slots, keeps the
+// live ones at the running write position (never ahead of the read
+// position), then the emptied tail blocks go back on the free stack.
+// Intent of compactionVertexCentric (delete_kernels.cu:50-261) without its
+// defects D5/D6.
+__global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist, Ctl *ctl) {
+  const uint64_t n = ctl->compacted;
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < n; i += nwaves) {
+    const uint32_t u = vlist[i];
+    const uint32_t uo = st.used[u];
+    const uint32_t tbl = st.tbl_off[u];
+    uint32_t wpos = 0;
+    for (uint32_t base = 0; base < uo; base += WAVE) {
+      uint32_t pos = base + lane;
+      uint32_t w = TOMB;
+      if (pos < uo) w = st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)];
+      bool keep = w != TOMB;
+      unsigned long long m = __ballot(keep);
+      if (keep) {
+        uint32_t dp = wpos + mask_lt_count(m);
+        st.slots[slot_index(st.arena, tbl, dp, st.log2B, st.B)] = w;
+      }
+      wpos += (uint32_t)__popcll(m);
+    }
+    const uint32_t nbo = (uo + st.B - 1) >> st.log2B;
+    const uint32_t nbn = (wpos + st.B - 1) >> st.log2B;
+    const uint32_t nf = nbo - nbn;
+    unsigned long long fb = 0;
+    if (lane == 0 && nf) fb = atomicAdd(&ctl->free_top, (unsigned long long)nf);
+    fb = __shfl(fb, 0, WAVE);
+    for (uint32_t j = lane; j < nf; j += WAVE) st.free_ids[fb + j] = st.arena[tbl + nbn + j];
+    if (lane == 0) {
+      st.used[u] = wpos;
+      if (nf) atomicAdd(&ctl->blocks_freed, (unsigned long long)nf);
+    }
+  }
+}
+
+void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s) {
+  k_cmp_select<<<grid_for(st.V), 256, 0, s>>>(st, frac, vlist, st.ctl);
+  k_cmp_run<<<probe_grid(st.V), 256, 0, s>>>(st, vlist, st.ctl);
+}
+
+// ------------------------------------------------------------- read-back --
+__global__ void k_deg64(const uint32_t *deg, uint32_t V, uint64_t *out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = deg[i];
+}
+void degrees_u64(const Store &st, uint64_t *out, hipStream_t s) {
+  k_deg64<<<grid_for(st.V), 256, 0, s>>>(st.deg, st.V, out);
+}
+
+// Live slots of each vertex, as keys (tag << bV | dst), packed at offs[tag].
+__global__ __launch_bounds__(256) void k_emit_keys(Store st, const uint32_t *vs, uint32_t n,
+                                                   const uint64_t *offs, uint32_t bV,
+                                                   uint64_t *keys) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < n; i += nwaves) {
+    const uint32_t u = vs ? vs[i] : (uint32_t)i;
+    if (u >= st.V) continue;
+    const uint32_t uo = st.used[u];
+    const uint32_t tbl = st.tbl_off[u];
+    const uint64_t o = offs[i];
+    const uint64_t tag = (uint64_t)i << bV;
+    uint32_t wpos = 0;
+    for (uint32_t base = 0; base < uo; base += WAVE) {
+      uint32_t pos = base + lane;
+      uint32_t w = TOMB;
+      if (pos < uo) w = st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)];
+      bool keep = w != TOMB;
+      unsigned long long m = __ballot(keep);
+      if (keep) keys[o + wpos + mask_lt_count(m)] = tag | w;
+      wpos += (uint32_t)__popcll(m);
+    }
+  }
+}
+
+void export_keys(const Store &st, const uint64_t *offsets, uint64_t *keys, hipStream_t s) {
+  k_emit_keys<<<probe_grid(st.V), 256, 0, s>>>(st, nullptr, st.V, offsets, id_bits(st.V), keys);
+}
+
+__global__ void k_keys_to_cols(const uint64_t *keys, uint64_t n, uint64_t mask, uint32_t *cols) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    cols[i] = (uint32_t)(keys[i] & mask);
+}
+void keys_to_cols(const uint64_t *keys, uint64_t n, uint32_t bV, uint32_t *cols, hipStream_t s) {
+  if (n) k_keys_to_cols<<<grid_for(n), 256, 0, s>>>(keys, n, (1ull << bV) - 1, cols);
+}
+
+__global__ void k_nbr_counts(const uint32_t *deg, uint32_t V, const uint32_t *vs, uint32_t n,
+                             uint64_t *cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = vs[i];
+    cnt[i] = u < V ? deg[u] : 0;
+  }
+}
+void neighbor_counts(const Store &st, const uint32_t *vs, uint32_t n, uint64_t *cnt,
+                     hipStream_t s) {
+  if (n) k_nbr_counts<<<grid_for(n), 256, 0, s>>>(st.deg, st.V, vs, n, cnt);
+}
+void neighbor_keys(const Store &st, const uint32_t *vs, uint32_t n, const uint64_t *offs,
+                   uint32_t qbits, uint64_t *keys, hipStream_t s) {
+  (void)qbits;
+  if (n) k_emit_keys<<<probe_grid(n), 256, 0, s>>>(st, vs, n, offs, id_bits(st.V), keys);
+}
+
+// ------------------------------------------------------------ generators --
+__device__ __forceinline__ uint64_t fmix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t rand64(uint64_t key, uint64_t ctr) {
+  return fmix64(key + (ctr + 1) * 0x9E3779B97F4A7C15ull);
+}
+__device__ __forceinline__ uint32_t mulhi32(uint32_t r, uint32_t n) {
+  return (uint32_t)(((uint64_t)r * n) >> 32);
+}
+
+__global__ void k_gen_rmat(uint64_t key, int scale, uint64_t n, uint32_t ta, uint32_t tab,
+                           uint32_t tabc, uint32_t *src, uint32_t *dst) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = 0, v = 0;
+    for (int l = 0; l < scale; ++l) {
+      uint32_t r = (uint32_t)(rand64(key, e * (uint64_t)scale + (uint64_t)l) >> 32);
+      uint32_t bit = 1u << (scale - 1 - l);
+      if (r < ta) {
+      } else if (r < tab) {
+        v |= bit;
+      } else if (r < tabc) {
+        u |= bit;
+      } else {
+        u |= bit;
+        v |= bit;
+      }
+    }
+    src[e] = u;
+    dst[e] = v;
+  }
+}
+
+__global__ void k_gen_uniform(uint64_t key, uint32_t V, uint64_t n, uint32_t *src, uint32_t *dst) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r = rand64(key, i);
+    uint32_t u = mulhi32((uint32_t)(r >> 32), V);
+    uint32_t v = mulhi32((uint32_t)r, V);
+    if (u == v && V > 1) v = (v + 1 == V) ? 0 : v + 1;
+    src[i] = u;
+    dst[i] = v;
+  }
+}
+
+__global__ void k_gen_sample(uint64_t key, uint32_t V, const uint64_t *offsets,
+                             const uint32_t *cols, uint64_t n, uint32_t *src, uint32_t *dst) {
+  const uint64_t E = offsets[V];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (E == 0) {
+      src[i] = dst[i] = 0;
+      continue;
+    }
+    uint64_t idx = rand64(key, i) % E;
+    uint32_t lo = 0, hi = V;
+    while (hi - lo > 1) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (offsets[mid] <= idx) lo = mid; else hi = mid;
+    }
+    src[i] = lo;
+    dst[i] = cols[idx];
+  }
+}
+
+void gen_rmat(uint64_t key, int scale, uint64_t n, uint32_t ta, uint32_t tab, uint32_t tabc,
+              uint32_t *src, uint32_t *dst, hipStream_t s) {
+  if (n) k_gen_rmat<<<grid_for(n, 256, 16384), 256, 0, s>>>(key, scale, n, ta, tab, tabc, src, dst);
+}
+void gen_uniform(uint64_t key, uint32_t V, uint64_t n, uint32_t *src, uint32_t *dst,
+                 hipStream_t s) {
+  if (n) k_gen_uniform<<<grid_for(n), 256, 0, s>>>(key, V, n, src, dst);
+}
+void gen_sample(uint64_t key, uint32_t V, const uint64_t *offsets, const uint32_t *cols,
+                uint64_t n, uint32_t *src, uint32_t *dst, hipStream_t s) {
+  if (n) k_gen_sample<<<grid_for(n), 256, 0, s>>>(key, V, offsets, cols, n, src, dst);
+}
+
+}  // namespace gv

@@ -1,0 +1,144 @@
+/*
+ * gv.h -- C ABI of the MI355X-native GraphVine dynamic graph store.
+ *
+ * Every entry point returns a gv_status (0 = GV_OK, negative = GV_E_*) and
+ * never exits or throws; gv_last_error() returns thread-local text for the
+ * last failure.  All device work of a handle is ordered on that handle's own
+ * HIP stream.  A handle is not thread-safe: use one handle per host thread, or
+ * serialise through one queue.
+ *
+ * Pointers are plain host or device addresses; a `mem` argument
+ * (GV_MEM_HOST / GV_MEM_DEVICE) says which.  Vertex IDs are 0-based uint32.
+ *
+ * Reference interfaces replaced (paths relative to the GraphVine sources):
+ *   GraphVine::GraphVine(CSR*)        include/dynamicStructure/graphvine.cuh:31
+ *   GraphVine::bulkBuild()            include/dynamicStructure/graphvine.cuh:37
+ *   GraphVine::batchInsert(CSR*,kk)   include/dynamicStructure/graphvine.cuh:39
+ *   GraphVine::batchDelete(CSR*,kk)   include/dynamicStructure/graphvine.cuh:41
+ *   GraphVine::compaction()           include/dynamicStructure/graphvine.cuh:43
+ *   GraphVine::getVertexDictionary()  include/dynamicStructure/graphvine.cuh:47
+ *   GraphVine::getVertexSize()        include/dynamicStructure/graphvine.cuh:56
+ *   static_pagerank(...)              include/algorithm/pagerank/pagerank.cuh:11
+ *   static_tc(...)                    include/algorithm/tc/tc.cuh:10
+ *   bc(GraphVine*, int)               include/algorithm/bc/bc.cuh:8
+ *   EdgePreallocatedQueue (pool)      include/dynamicStructure/graphStructures.cuh:103-108
+ */
+#ifndef GV_H
+#define GV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int gv_status;
+
+#define GV_OK 0
+#define GV_E_INVALID -1     /* bad argument or handle */
+#define GV_E_RANGE -2       /* a vertex ID >= num_vertices; the batch was not applied */
+#define GV_E_NOMEM -3       /* device allocation failed */
+#define GV_E_POOL -4        /* edge-block pool exhausted; the batch was not applied */
+#define GV_E_HIP -5         /* HIP runtime error */
+#define GV_E_UNSUPPORTED -6 /* operation not defined for this graph kind */
+
+#define GV_MEM_HOST 0
+#define GV_MEM_DEVICE 1
+
+typedef struct gv_graph gv_graph;
+
+/* Store configuration.  gv_config_default() fills every field. */
+typedef struct gv_config {
+  uint32_t block_slots;       /* slots per edge block: 32, 64 or 128 (4 B each) */
+  uint64_t pool_blocks;       /* edge blocks in the pool; 0 = sized from free HBM */
+  double pool_fraction;       /* share of free HBM used when pool_blocks == 0 */
+  float compact_threshold;    /* default tombstone fraction for gv_compact(g, -1) */
+  uint32_t pr_thread_max;     /* PageRank degree bins: <= this -> 1 lane per vertex */
+  uint32_t pr_group_max;      /*   <= this -> 8 lanes per vertex */
+  uint32_t pr_wave_max;       /*   <= this -> 1 wavefront; larger -> chunked */
+  int maintain_in_edges;      /* directed graphs: keep the in-adjacency mirror */
+  int device;                 /* HIP device ordinal */
+} gv_config;
+
+/* Result of one update call. */
+typedef struct gv_batch_stats {
+  uint64_t requested;        /* (src,dst) pairs passed in */
+  uint64_t directed_entries; /* directed entries after self-loop removal */
+  uint64_t applied;          /* slots written (insert) or tombstoned (delete) */
+  uint64_t touched;          /* distinct source vertices in the batch */
+  uint64_t blocks_allocated; /* edge blocks taken from the pool */
+  uint64_t live_edges;       /* directed live edges after the call */
+} gv_batch_stats;
+
+gv_status gv_config_default(gv_config *cfg);
+
+/* GraphVine(CSR*) without the bulk build: an empty graph on num_vertices
+ * vertices.  directed = 0 stores every edge in both directions. */
+gv_status gv_create(const gv_config *cfg, uint32_t num_vertices, int directed, gv_graph **out);
+gv_status gv_destroy(gv_graph *g);
+
+/* GraphVine::bulkBuild: clear the graph, then insert the COO edge list. */
+gv_status gv_bulk_build(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                        int mem, gv_batch_stats *stats);
+/* GraphVine::batchInsert: N(u) |= {v}; duplicates and self-loops dropped. */
+gv_status gv_batch_insert(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                          int mem, gv_batch_stats *stats);
+/* GraphVine::batchDelete: N(u) \= {v} (tombstones); missing edges are no-ops. */
+gv_status gv_batch_delete(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                          int mem, gv_batch_stats *stats);
+/* GraphVine::compaction: pack every vertex whose tombstones exceed
+ * tombstone_fraction of its slots (0 = any tombstone, < 0 = the configured
+ * default) and return emptied blocks to the pool. */
+gv_status gv_compact(gv_graph *g, float tombstone_fraction, uint64_t *vertices_compacted);
+
+/* Stats reads (these synchronise the handle's stream). */
+gv_status gv_num_vertices(gv_graph *g, uint32_t *v);
+gv_status gv_num_edges(gv_graph *g, uint64_t *live_edges);
+gv_status gv_degrees(gv_graph *g, uint32_t *deg, int mem);
+gv_status gv_pool_stats(gv_graph *g, uint64_t *free_blocks, uint64_t *total_blocks);
+
+/* Sorted CSR export: offsets[V+1], cols[offsets[V]] (ascending per vertex). */
+gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem);
+/* 1-hop lookups: offs[n+1]; out[offs[n]] sorted per query (out may be NULL
+ * to size it first). */
+gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs, uint32_t *out,
+                       int mem);
+
+/* static_pagerank: FP32 values, per-vertex compensated pull sums, FP64 L1
+ * residual; stops after the first update with residual < eps or after
+ * max_iter updates.  out[V] receives x. */
+gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
+                      int *iters, double *residual);
+/* static_tc on an undirected graph: exact uint64 counts; per_vertex (may be
+ * NULL) gets t[u] = triangles whose minimum vertex ID is u. */
+gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total);
+/* bc: Brandes betweenness from the given sources (unnormalised, not halved);
+ * sigma in FP64, dependencies in FP32 with compensated sums. */
+gv_status gv_bc(gv_graph *g, const uint32_t *sources, uint32_t n_sources, float *out, int mem);
+
+/* Device generators for update batches and synthetic graphs.  Each draws
+ * from a counter-based SplitMix64 stream (output i = mix(key + (i+1)*golden))
+ * so any range of a stream can be produced independently.
+ *   rmat:    one draw per level, high 32 bits vs cumulative quadrant
+ *            thresholds ta=a, tab=a+b, tabc=a+b+c scaled by 2^32;
+ *   uniform: u = hi32*V>>32, v = lo32*V>>32, a self-loop moves v to v+1 mod V;
+ *   sample:  entry (draw % offsets[V]) of a sorted CSR (with replacement).
+ * Outputs are device arrays of n entries. */
+gv_status gv_gen_rmat(gv_graph *g, uint64_t key, int scale, uint64_t n, uint32_t ta, uint32_t tab,
+                      uint32_t tabc, uint32_t *src_dev, uint32_t *dst_dev);
+gv_status gv_gen_uniform(gv_graph *g, uint64_t key, uint32_t V, uint64_t n, uint32_t *src_dev,
+                         uint32_t *dst_dev);
+gv_status gv_gen_sample(gv_graph *g, uint64_t key, uint32_t V, const uint64_t *offsets_dev,
+                        const uint32_t *cols_dev, uint64_t n, uint32_t *src_dev,
+                        uint32_t *dst_dev);
+
+/* Stream plumbing. */
+gv_status gv_sync(gv_graph *g);
+gv_status gv_get_stream(gv_graph *g, void **hip_stream);
+const char *gv_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+/*
+ * gv_omp.c -- OpenMP CSR baseline (CPU timing reference only).
+ *
+ * TEST/BENCH INFRASTRUCTURE ONLY: used by bench.py's cpu_baseline leg and by
+ * tests that check it against the sequential oracle.  It computes the same
+ * set semantics as gv_oracle.c on an immutable sorted CSR that is rebuilt per
+ * update batch by a parallel sort-merge (the usual static-graph answer to a
+ * dynamic workload), plus pull PageRank.
+ */
+#include <omp.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+static uint32_t id_bits(uint32_t V) {
+  uint32_t b = 0;
+  while ((1ull << b) < V) ++b;
+  return b ? b : 1;
+}
+
+/* Parallel LSD radix sort (8-bit digits) of n keys on their low `bits` bits;
+ * the result ends in a. */
+static void psort(uint64_t *a, uint64_t *t, uint64_t n, uint32_t bits) {
+  int nt = omp_get_max_threads();
+  uint64_t *hist = (uint64_t *)malloc(sizeof(uint64_t) * 256 * (size_t)nt);
+  uint64_t *src = a, *dst = t;
+  for (uint32_t sh = 0; sh < bits; sh += 8) {
+#pragma omp parallel num_threads(nt)
+    {
+      int tid = omp_get_thread_num();
+      uint64_t lo = n * (uint64_t)tid / nt, hi = n * (uint64_t)(tid + 1) / nt;
+      uint64_t *h = hist + 256 * (size_t)tid;
+      memset(h, 0, 256 * sizeof(uint64_t));
+      for (uint64_t i = lo; i < hi; ++i) h[(src[i] >> sh) & 255]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        uint64_t s = 0;
+        for (int d = 0; d < 256; ++d)
+          for (int k = 0; k < nt; ++k) {
+            uint64_t c = hist[256 * (size_t)k + d];
+            hist[256 * (size_t)k + d] = s;
+            s += c;
+          }
+      }
+      for (uint64_t i = lo; i < hi; ++i) dst[h[(src[i] >> sh) & 255]++] = src[i];
+    }
+    uint64_t *x = src;
+    src = dst;
+    dst = x;
+  }
+  if (src != a) memcpy(a, src, n * sizeof(uint64_t));
+  free(hist);
+}
+
+/* Stage (src,dst) pairs into sorted unique keys; mirror adds (dst,src). */
+static uint64_t stage(uint32_t V, int directed, const uint32_t *src, const uint32_t *dst, uint64_t n,
+                      uint64_t **out) {
+  uint32_t bV = id_bits(V);
+  uint64_t m = directed ? n : 2 * n;
+  uint64_t *k = (uint64_t *)malloc(sizeof(uint64_t) * (m ? m : 1));
+  uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * (m ? m : 1));
+  const uint64_t SENT = ~0ull;
+#pragma omp parallel for schedule(static)
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t u = src[i], v = dst[i];
+    int drop = u == v || u >= V || v >= V;
+    if (directed) k[i] = drop ? SENT : (((uint64_t)u << bV) | v);
+    else {
+      k[2 * i] = drop ? SENT : (((uint64_t)u << bV) | v);
+      k[2 * i + 1] = drop ? SENT : (((uint64_t)v << bV) | u);
+    }
+  }
+  psort(k, t, m, 2 * bV);
+  free(t);
+  /* unique + drop sentinels (sequential; batches are small next to E) */
+  uint64_t w = 0;
+  for (uint64_t i = 0; i < m; ++i)
+    if (k[i] != SENT && (w == 0 || k[w - 1] != k[i])) k[w++] = k[i];
+  *out = k;
+  return w;
+}
+
+static uint64_t lower_src(const uint64_t *k, uint64_t m, uint64_t key) {
+  uint64_t lo = 0, hi = m;
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    if (k[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/* New CSR = (old U ins) \ del per vertex, deletes applied after inserts
+ * (callers pass one kind per call to keep batch order).  off_out must hold
+ * V+1 entries, cols_out enough for the result (old E + inserts).
+ * Returns the new edge count. */
+uint64_t omp_apply_batch(uint32_t V, int directed, const uint64_t *off, const uint32_t *cols,
+                         const uint32_t *src, const uint32_t *dst, uint64_t n, int del,
+                         uint64_t *off_out, uint32_t *cols_out) {
+  uint32_t bV = id_bits(V);
+  uint64_t *k = NULL;
+  uint64_t m = stage(V, directed, src, dst, n, &k);
+  uint64_t *cnt = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)V + 1));
+  uint64_t mask = (1ull << bV) - 1;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (uint32_t u = 0; u < V; ++u) {
+    uint64_t b0 = lower_src(k, m, (uint64_t)u << bV), b1 = lower_src(k, m, (uint64_t)(u + 1) << bV);
+    uint64_t i = off[u], ie = off[u + 1], j = b0, c = 0;
+    while (i < ie || j < b1) {
+      uint32_t bj = j < b1 ? (uint32_t)(k[j] & mask) : 0;
+      if (j >= b1 || (i < ie && cols[i] < bj)) { ++c; ++i; }
+      else if (i >= ie || bj < cols[i]) { if (!del) ++c; ++j; }
+      else { if (!del) ++c; ++i; ++j; }
+    }
+    cnt[u] = c;
+  }
+  uint64_t s = 0;
+  for (uint32_t u = 0; u < V; ++u) { off_out[u] = s; s += cnt[u]; }
+  off_out[V] = s;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (uint32_t u = 0; u < V; ++u) {
+    uint64_t b0 = lower_src(k, m, (uint64_t)u << bV), b1 = lower_src(k, m, (uint64_t)(u + 1) << bV);
+    uint64_t i = off[u], ie = off[u + 1], j = b0, o = off_out[u];
+    while (i < ie || j < b1) {
+      uint32_t bj = j < b1 ? (uint32_t)(k[j] & mask) : 0;
+      if (j >= b1 || (i < ie && cols[i] < bj)) cols_out[o++] = cols[i++];
+      else if (i >= ie || bj < cols[i]) { if (!del) cols_out[o++] = bj; ++j; }
+      else { if (!del) cols_out[o++] = cols[i]; ++i; ++j; }
+    }
+  }
+  free(cnt);
+  free(k);
+  return s;
+}
+
+/* Pull PageRank over a symmetric CSR (in = out), same rule as the oracle;
+ * FP64.  Returns the number of updates. */
+int omp_pagerank_sym(uint32_t V, const uint64_t *off, const uint32_t *cols, double alpha, double eps,
+                     int max_iter, double *x) {
+  double *c = (double *)malloc(sizeof(double) * (V ? V : 1));
+  double *xn = (double *)malloc(sizeof(double) * (V ? V : 1));
+  for (uint32_t v = 0; v < V; ++v) x[v] = 1.0 / V;
+  double base = (1.0 - alpha) / V;
+  int it = 0;
+  while (it < max_iter) {
+#pragma omp parallel for schedule(static)
+    for (uint32_t v = 0; v < V; ++v) {
+      uint64_t d = off[v + 1] - off[v];
+      c[v] = d ? x[v] / (double)d : 0.0;
+    }
+    double r = 0.0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : r)
+    for (uint32_t v = 0; v < V; ++v) {
+      double s = 0.0;
+      for (uint64_t j = off[v]; j < off[v + 1]; ++j) s += c[cols[j]];
+      xn[v] = base + alpha * s;
+      r += fabs(xn[v] - x[v]);
+    }
+    memcpy(x, xn, sizeof(double) * V);
+    ++it;
+    if (r < eps) break;
+  }
+  free(c);
+  free(xn);
+  return it;
+}
+
+int omp_threads(void) { return omp_get_max_threads(); }
