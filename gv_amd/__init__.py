@@ -143,7 +143,12 @@ def _is_device(x):
 
 
 def _ptr_u32(x):
-    """(pointer, keepalive, mem) for a uint32 host array or device tensor."""
+    """(pointer, keepalive, mem) for a uint32 host array, a gv_amd.hip
+    DeviceArray or a device tensor."""
+    if hasattr(x, "ptr") and hasattr(x, "nbytes") and not isinstance(x, np.ndarray):
+        if np.dtype(x.dtype) != np.uint32:
+            raise TypeError("device arrays must be uint32")
+        return x.ptr, x, GV_MEM_DEVICE
     if _is_device(x):
         import torch
         if x.dtype not in (torch.int32, torch.uint32):
@@ -309,3 +314,41 @@ def bc(graph, sources):
     _check(lib().gv_bc(graph.handle, s.ctypes.data_as(ctypes.c_void_p), len(s),
                        out.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST))
     return out
+
+
+# Device generators (counter-based SplitMix64 streams; identical streams to
+# the host restatement, so batches can be produced on the GPU at scale).
+def gen_rmat(graph, key, scale, n, a=0.57, b=0.19, c=0.19):
+    from .hip import DeviceArray
+    s4 = 4294967296.0
+    src, dst = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
+    _check(lib().gv_gen_rmat(graph.handle, key, scale, n, int(a * s4), int((a + b) * s4),
+                             int((a + b + c) * s4), src.ptr, dst.ptr))
+    return src, dst
+
+
+def gen_uniform(graph, key, V, n):
+    from .hip import DeviceArray
+    src, dst = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
+    _check(lib().gv_gen_uniform(graph.handle, key, V, n, src.ptr, dst.ptr))
+    return src, dst
+
+
+def gen_sample(graph, key, V, offsets_dev, cols_dev, n):
+    """Sample n edges of a sorted device CSR (offsets uint64[V+1], cols uint32)."""
+    from .hip import DeviceArray
+    src, dst = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
+    _check(lib().gv_gen_sample(graph.handle, key, V, offsets_dev.ptr, cols_dev.ptr, n, src.ptr,
+                               dst.ptr))
+    return src, dst
+
+
+def export_csr_device(graph):
+    """Sorted CSR as device arrays (offsets uint64[V+1], cols uint32)."""
+    from .hip import DeviceArray
+    off = DeviceArray(graph.num_vertices + 1, np.uint64)
+    _check(lib().gv_export_csr(graph.handle, off.ptr, None, GV_MEM_DEVICE))
+    E = int(off.numpy()[-1])
+    cols = DeviceArray(max(E, 1), np.uint32)
+    _check(lib().gv_export_csr(graph.handle, off.ptr, cols.ptr, GV_MEM_DEVICE))
+    return off, cols, E

@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_tc_fill(Store st, const uint64_t *off, 
   for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < st.V;
        u += nwaves) {
     const uint32_t us = st.used[u], du = st.deg[u], tbl = st.tbl_off[u];
-    const uint64_t o = off[u], tag = u << bV;
+    const uint64_t o = off[u], cap = off[u + 1] - off[u], tag = u << bV;
     uint32_t wpos = 0;
     for (uint32_t base = 0; base < us; base += WAVE) {
       uint32_t j = base + lane;
@@ -440,7 +440,8 @@ __global__ __launch_bounds__(256) void k_tc_fill(Store st, const uint64_t *off, 
         keep = w != TOMB && tc_higher(st.deg[w], w, du, (uint32_t)u);
       }
       unsigned long long m = __ballot(keep);
-      if (keep) keys[o + wpos + mask_lt_count(m)] = tag | w;
+      uint32_t r = wpos + mask_lt_count(m);
+      if (keep && r < cap) keys[o + r] = tag | w;
       wpos += (uint32_t)__popcll(m);
     }
   }

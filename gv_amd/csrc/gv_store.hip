@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_emit_keys(Store st, const uint32_t *vs,
     if (u >= st.V) continue;
     const uint32_t uo = st.used[u];
     const uint32_t tbl = st.tbl_off[u];
-    const uint64_t o = offs[i];
+    const uint64_t o = offs[i], cap = offs[i + 1] - offs[i];
     const uint64_t tag = (uint64_t)i << bV;
     uint32_t wpos = 0;
     for (uint32_t base = 0; base < uo; base += WAVE) {
@@ -583,7 +583,8 @@ __global__ __launch_bounds__(256) void k_emit_keys(Store st, const uint32_t *vs,
       if (pos < uo) w = st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)];
       bool keep = w != TOMB;
       unsigned long long m = __ballot(keep);
-      if (keep) keys[o + wpos + mask_lt_count(m)] = tag | w;
+      uint32_t r = wpos + mask_lt_count(m);
+      if (keep && r < cap) keys[o + r] = tag | w;
       wpos += (uint32_t)__popcll(m);
     }
   }

@@ -1,0 +1,44 @@
+"""CPU checks of the C-ABI library: it loads and exports every entry point
+declared in include/gv.h.  No compute calls (there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import gv_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(gv_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(gv_amd.LIB_PATH)
+    decl = declared_symbols()
+    assert len(decl) >= 20
+    missing = [n for n in sorted(decl) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(gv_amd.EXPORTED) <= decl
+
+
+def test_python_binding_loads_and_config_defaults():
+    cfg = gv_amd.default_config()
+    assert cfg.block_slots == 64
+    assert cfg.pr_thread_max <= cfg.pr_group_max <= cfg.pr_wave_max
+    assert 0 < cfg.compact_threshold < 1
+    cfg2 = gv_amd.default_config(block_slots=128)
+    assert cfg2.block_slots == 128
+
+
+def test_struct_layout_matches_header():
+    # gv_config: u32, (pad), u64, f64, f32, 3 x u32, 2 x int -> 48 bytes
+    assert ctypes.sizeof(gv_amd.GvConfig) == 48
+    assert ctypes.sizeof(gv_amd.GvBatchStats) == 48

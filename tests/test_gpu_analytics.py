@@ -1,0 +1,175 @@
+"""GPU parity of PageRank, triangle counting and betweenness centrality.
+
+Tolerances (FP32 device arithmetic vs the FP64 oracle):
+  PageRank  L1 <= 1e-6 at the same iteration count (and the device's own
+            stopping iteration within 1 of the oracle's);
+  TC        exact (uint64 per vertex and total);
+  BC        relative 1e-5 per vertex (|g - o| <= 1e-5 * max(|o|, 1)).
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import gv_amd as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PR_L1 = 1e-6
+BC_REL = 1e-5
+
+
+def make(V, directed, pool=None, **kw):
+    cfg = G.default_config(**kw)
+    cfg.pool_blocks = pool if pool is not None else 4 * V + 4096
+    return G.GraphVine(V, directed=directed, config=cfg)
+
+
+def build_pair(V, directed, s, d, **kw):
+    g = make(V, directed, **kw)
+    o = O.OracleGraph(V, directed)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    return g, o
+
+
+def check_pagerank(g, o, eps=1e-6, max_iter=100, same_stop=True):
+    x, it, r = G.static_pagerank(g, 0.85, eps, max_iter)
+    xo, ito, _ = o.pagerank(0.85, eps, max_iter)
+    if same_stop:
+        assert abs(it - ito) <= 1
+    else:
+        # eps below the FP32 iterate's rounding floor: the device runs to
+        # max_iter or stops on an exact FP32 fixed point
+        assert it >= ito - 1
+    xr, _, _ = o.pagerank(0.85, eps, max_iter, fixed_iters=it)
+    l1 = float(np.abs(x.astype(np.float64) - xr).sum())
+    assert l1 <= PR_L1, l1
+    return l1, it
+
+
+def check_bc(g, o, sources):
+    b = G.bc(g, sources).astype(np.float64)
+    bo = o.bc(sources)
+    err = np.abs(b - bo) / np.maximum(np.abs(bo), 1.0)
+    assert err.max() <= BC_REL, err.max()
+
+
+def test_pagerank_closed_forms():
+    V = 9
+    g, o = build_pair(V, True, np.arange(V), (np.arange(V) + 1) % V)
+    x, it, r = G.static_pagerank(g, 0.85, 1e-6, 50)
+    assert it == 1 and np.allclose(x, 1.0 / V, rtol=1e-6)
+    g2, _ = build_pair(2, True, [0], [1])
+    x, it, _ = G.static_pagerank(g2, 0.85, 1e-30, 3)
+    a = 0.85
+    xa, xb = 0.5, 0.5
+    for _ in range(3):
+        xa, xb = (1 - a) / 2, (1 - a) / 2 + a * xa
+    assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_pagerank_config1_after_updates(directed):
+    V = 1 << 14
+    s, d = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 14, 16 * V)
+    g, o = build_pair(V, directed, s, d, pool=6 * V)
+    check_pagerank(g, o)
+    for b in range(3):
+        si, di = O.gen_uniform(O.stream_key(1, 1000 + b), V, 4096)
+        g.batchInsert(si, di)
+        o.insert(si, di)
+        off, cols = o.export_csr()
+        sd, dd = O.gen_sample(O.stream_key(1, 2000 + b), V, off, cols, 4096)
+        g.batchDelete(sd, dd)  # tombstones left in place on purpose
+        o.delete(sd, dd)
+        check_pagerank(g, o)
+    g.compaction(0.0)
+    check_pagerank(g, o)
+
+
+def test_pagerank_all_degree_bins_and_chunked_hubs():
+    # small thresholds push vertices through every bin incl. chunked hubs
+    V = 3000
+    s, d = O.gen_rmat(O.stream_key(3, O.STREAM_RMAT), 12, 60000)
+    s, d = s % V, d % V
+    star = np.zeros(V - 1, np.uint32), np.arange(1, V, dtype=np.uint32)
+    s = np.concatenate([s, star[0]])
+    d = np.concatenate([d, star[1]])
+    g, o = build_pair(V, False, s, d, pool=8 * V, pr_thread_max=2, pr_group_max=16,
+                      pr_wave_max=128)
+    check_pagerank(g, o)
+    check_pagerank(g, o, eps=1e-9, max_iter=200, same_stop=False)
+
+
+@pytest.mark.parametrize("n", [3, 7, 40])
+def test_tc_complete_graph(n):
+    e = list(itertools.combinations(range(n), 2))
+    g, o = build_pair(n, False, [a for a, _ in e], [b for _, b in e])
+    pv, tot = G.static_tc(g)
+    assert tot == math.comb(n, 3)
+    assert pv.tolist() == [math.comb(n - 1 - i, 2) for i in range(n)]
+
+
+def test_tc_config1_symmetrized_exact():
+    V = 1 << 14
+    s, d = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 14, 16 * V)
+    g, o = build_pair(V, False, s, d, pool=6 * V)
+    pv, tot = G.static_tc(g)
+    pvo, toto = o.triangles()
+    assert tot == toto and np.array_equal(pv, pvo)
+    _, tot2 = G.static_tc(g, per_vertex=False)
+    assert tot2 == toto
+    # after updates with tombstones
+    si, di = O.gen_uniform(O.stream_key(1, 1000), V, 4096)
+    g.batchInsert(si, di)
+    o.insert(si, di)
+    off, cols = o.export_csr()
+    sd, dd = O.gen_sample(O.stream_key(1, 2000), V, off, cols, 4096)
+    g.batchDelete(sd, dd)
+    o.delete(sd, dd)
+    pv, tot = G.static_tc(g)
+    pvo, toto = o.triangles()
+    assert tot == toto and np.array_equal(pv, pvo)
+
+
+def test_tc_rejects_directed():
+    g, _ = build_pair(10, True, [0, 1], [1, 2])
+    with pytest.raises(G.GraphVineError) as e:
+        G.static_tc(g)
+    assert e.value.status == G.GV_E_UNSUPPORTED
+
+
+def test_bc_closed_forms():
+    n = 12
+    g, _ = build_pair(n, False, np.arange(n - 1), np.arange(1, n))
+    b = G.bc(g, np.arange(n))
+    assert np.allclose(b, [2 * i * (n - 1 - i) for i in range(n)], rtol=1e-6)
+    g2, _ = build_pair(n, False, np.zeros(n - 1), np.arange(1, n))
+    b2 = G.bc(g2, np.arange(n))
+    assert abs(b2[0] - (n - 1) * (n - 2)) < 1e-3 and np.allclose(b2[1:], 0)
+
+
+def test_bc_config1_16_sources():
+    V = 1 << 14
+    s, d = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 14, 16 * V)
+    g, o = build_pair(V, True, s, d, pool=6 * V)
+    rng = np.random.default_rng(1)
+    src = rng.choice(V, 16, replace=False).astype(np.uint32)
+    check_bc(g, o, src)
+    # undirected view, with tombstones present
+    gu, ou = build_pair(V, False, s, d, pool=6 * V)
+    off, cols = ou.export_csr()
+    sd, dd = O.gen_sample(O.stream_key(1, 2000), V, off, cols, 4096)
+    gu.batchDelete(sd, dd)
+    ou.delete(sd, dd)
+    check_bc(gu, ou, src)
+
+
+def test_bc_rejects_bad_source():
+    g, _ = build_pair(10, False, [0, 1], [1, 2])
+    with pytest.raises(G.GraphVineError) as e:
+        G.bc(g, [3, 10])
+    assert e.value.status == G.GV_E_RANGE

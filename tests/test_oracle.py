@@ -1,0 +1,415 @@
+"""Pins the CPU oracle (oracle/gv_oracle.c) without the reference binary.
+
+The reference ships no tests, fixtures or golden vectors, and it is CUDA-only
+(not buildable here), so every oracle function is pinned by a published
+known-answer value, a closed form, brute force on small inputs, or a library
+routine (networkx), chosen so a plausible slip (dropped term, wrong index,
+off-by-one level, swapped operands) fails at least one check.
+"""
+import itertools
+import json
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+MASK = (1 << 64) - 1
+
+
+def py_fmix(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+def py_rand(key, i):
+    return py_fmix((key + (i + 1) * 0x9E3779B97F4A7C15) & MASK)
+
+
+# ------------------------------------------------------------- generators --
+
+def test_splitmix64_known_answers():
+    # Published SplitMix64 outputs for state 0 (Steele, Lea & Flood 2014;
+    # the reference sequence used to seed xoshiro generators).
+    assert O.lib().orc_rand64(0, 0) == 0xE220A8397B1DCDAF
+    assert O.lib().orc_rand64(0, 1) == 0x6E789E6AA1B965F4
+    assert O.lib().orc_rand64(0, 2) == 0x06C45D188009454F
+
+
+def test_rmat_matches_independent_python():
+    key = O.stream_key(7, O.STREAM_RMAT)
+    src, dst = O.gen_rmat(key, 10, 500)
+    ta, tab, tabc = O.thresholds()
+    for e in range(500):
+        u = v = 0
+        for lvl in range(10):
+            r = py_rand(key, e * 10 + lvl) >> 32
+            bit = 1 << (9 - lvl)
+            if r < ta:
+                pass
+            elif r < tab:
+                v |= bit
+            elif r < tabc:
+                u |= bit
+            else:
+                u |= bit
+                v |= bit
+        assert (src[e], dst[e]) == (u, v)
+
+
+def test_rmat_quadrant_frequencies():
+    # a/b/c/d = 0.57/0.19/0.19/0.05 -> top-bit quadrant frequencies
+    src, dst = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 12, 200000)
+    top_u = src >> 11
+    top_v = dst >> 11
+    f = [np.mean((top_u == a) & (top_v == b)) for a, b in [(0, 0), (0, 1), (1, 0), (1, 1)]]
+    assert abs(f[0] - 0.57) < 0.005 and abs(f[1] - 0.19) < 0.005
+    assert abs(f[2] - 0.19) < 0.005 and abs(f[3] - 0.05) < 0.005
+
+
+def test_uniform_and_sample_match_python():
+    key = O.stream_key(3, O.STREAM_INSERT)
+    V = 1000
+    src, dst = O.gen_uniform(key, V, 300)
+    for i in range(300):
+        r = py_rand(key, i)
+        u = ((r >> 32) * V) >> 32
+        v = ((r & 0xFFFFFFFF) * V) >> 32
+        if u == v:
+            v = 0 if v + 1 == V else v + 1
+        assert (src[i], dst[i]) == (u, v)
+        assert src[i] != dst[i]
+    off = np.array([0, 2, 2, 5], np.uint64)
+    cols = np.array([1, 2, 0, 1, 2], np.uint32)
+    s, d = O.gen_sample(key, 3, off, cols, 50)
+    for i in range(50):
+        idx = py_rand(key, i) % 5
+        u = int(np.searchsorted(off, idx, side="right") - 1)
+        assert (s[i], d[i]) == (u, cols[idx])
+
+
+def test_generator_golden_digest():
+    with open(os.path.join(GOLDEN, "generators.json")) as f:
+        gold = json.load(f)
+    for case in gold["rmat"]:
+        s, d = O.gen_rmat(O.stream_key(case["seed"], O.STREAM_RMAT), case["scale"], case["n"])
+        assert s[:8].tolist() == case["src_head"] and d[:8].tolist() == case["dst_head"]
+        k = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+        assert int(np.bitwise_xor.reduce(k * np.uint64(0x9E3779B97F4A7C15))) == case["xor_digest"]
+
+
+# ------------------------------------------------------------------ store --
+
+def brute_apply(adj, src, dst, directed, delete):
+    changed = 0
+    for u, v in zip(src.tolist(), dst.tolist()):
+        if u == v:
+            continue
+        pairs = [(u, v)] if directed else [(u, v), (v, u)]
+        for a, b in pairs:
+            if delete and b in adj[a]:
+                adj[a].discard(b)
+                changed += 1
+            elif not delete and b not in adj[a]:
+                adj[a].add(b)
+                changed += 1
+    return changed
+
+
+def csr_of(adj, V):
+    off = [0]
+    cols = []
+    for u in range(V):
+        cols += sorted(adj[u])
+        off.append(len(cols))
+    return np.array(off, np.uint64), np.array(cols, np.uint32)
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_store_matches_brute_force_sets(directed):
+    rng = np.random.default_rng(5)
+    V = 60
+    g = O.OracleGraph(V, directed)
+    adj = [set() for _ in range(V)]
+    s = rng.integers(0, V, 400).astype(np.uint32)
+    d = rng.integers(0, V, 400).astype(np.uint32)
+    assert g.bulk_build(s, d) == brute_apply(adj, s, d, directed, False)
+    for step in range(30):
+        s = rng.integers(0, V, 50).astype(np.uint32)
+        d = rng.integers(0, V, 50).astype(np.uint32)
+        if step % 3 == 2:  # deletes of existing edges plus misses
+            off, cols = csr_of(adj, V)
+            if len(cols):
+                idx = rng.integers(0, len(cols), 40)
+                s[:40] = np.searchsorted(off, idx, side="right") - 1
+                d[:40] = cols[idx]
+            assert g.delete(s, d) == brute_apply(adj, s, d, directed, True)
+        else:
+            assert g.insert(s, d) == brute_apply(adj, s, d, directed, False)
+        off, cols = g.export_csr()
+        eo, ec = csr_of(adj, V)
+        assert np.array_equal(off, eo) and np.array_equal(cols, ec)
+        assert np.array_equal(g.degrees(), np.diff(eo).astype(np.uint32))
+
+
+def test_store_special_cases():
+    g = O.OracleGraph(5, True)
+    assert g.bulk_build([0, 0, 0, 1], [1, 1, 0, 2]) == 2  # dup + self-loop dropped
+    assert g.insert([0], [1]) == 0  # existing edge: no-op
+    assert g.delete([3], [4]) == 0  # missing edge: no-op
+    assert g.delete([0, 1], [1, 2]) == 2
+    assert g.num_edges() == 0
+    with pytest.raises(ValueError):
+        g.insert([0], [5])
+    assert g.num_edges() == 0
+    u = O.OracleGraph(4, False)
+    assert u.bulk_build([0, 1], [1, 0]) == 2  # both directions, one undirected edge
+    assert u.degrees().tolist() == [1, 1, 0, 0]
+
+
+def test_store_hub_over_reference_limit():
+    # The reference caps a vertex at 4096 blocks x 128 = 524,288 edges
+    # (constants.cuh:10); the contract has no cap.
+    V = 600_001
+    g = O.OracleGraph(V, True)
+    d = np.arange(1, V, dtype=np.uint32)
+    assert g.bulk_build(np.zeros(V - 1, np.uint32), d) == V - 1
+    assert g.degrees()[0] == V - 1
+
+
+# -------------------------------------------------------------- PageRank --
+
+def test_pagerank_cycle_is_uniform_every_iteration():
+    V = 7
+    g = O.OracleGraph(V, True)
+    g.bulk_build(np.arange(V), (np.arange(V) + 1) % V)
+    for k in range(1, 5):
+        x, it, r = g.pagerank(fixed_iters=k)
+        assert it == k and np.allclose(x, 1.0 / V, rtol=0, atol=1e-15) and r < 1e-15
+
+
+def test_pagerank_two_vertex_dangling_dropped():
+    a = 0.85
+    g = O.OracleGraph(2, True)
+    g.bulk_build([0], [1])
+    xa, xb = 0.5, 0.5
+    for k in range(1, 6):
+        xa, xb = (1 - a) / 2, (1 - a) / 2 + a * xa
+        x, it, _ = g.pagerank(alpha=a, fixed_iters=k)
+        assert abs(x[0] - xa) < 1e-15 and abs(x[1] - xb) < 1e-15
+    # dangling mass dropped: sum < 1
+    assert x.sum() < 1.0
+
+
+def test_pagerank_star_fixed_point():
+    # undirected star, centre 0 with m leaves: x_c = (1-a)/V + a*m*x_l,
+    # x_l = (1-a)/V + a*x_c/m
+    a, m = 0.85, 6
+    V = m + 1
+    g = O.OracleGraph(V, False)
+    g.bulk_build(np.zeros(m), np.arange(1, V))
+    x, it, r = g.pagerank(alpha=a, eps=1e-14, max_iter=10000)
+    b = (1 - a) / V
+    xl = (b + a * b / m) / (1 - a * a)
+    xc = b + a * m * xl
+    assert abs(x[0] - xc) < 1e-12 and np.allclose(x[1:], xl, atol=1e-12)
+
+
+def test_pagerank_stopping_rule_and_max_iter():
+    g = O.OracleGraph(50, True)
+    s, d = O.gen_rmat(O.stream_key(2, O.STREAM_RMAT), 6, 400)
+    g.bulk_build(s % 50, d % 50)
+    x, it, r = g.pagerank(eps=1e-6, max_iter=1000)
+    assert r < 1e-6
+    _, _, r_prev = g.pagerank(fixed_iters=it - 1)
+    assert r_prev >= 1e-6  # first update below eps stops the loop
+    _, it3, _ = g.pagerank(eps=1e-30, max_iter=3)
+    assert it3 == 3
+
+
+def test_pagerank_matches_networkx_on_dangling_free_graph():
+    nx = pytest.importorskip("networkx")
+    V = 200
+    g = O.OracleGraph(V, False)
+    s, d = O.gen_rmat(O.stream_key(4, O.STREAM_RMAT), 8, 3000)
+    s, d = s % V, d % V
+    # ring keeps every vertex non-dangling
+    s = np.concatenate([s, np.arange(V, dtype=np.uint32)])
+    d = np.concatenate([d, ((np.arange(V) + 1) % V).astype(np.uint32)])
+    g.bulk_build(s, d)
+    x, _, _ = g.pagerank(eps=1e-13, max_iter=1000)
+    G = nx.Graph()
+    G.add_nodes_from(range(V))
+    G.add_edges_from((int(a), int(b)) for a, b in zip(s, d) if a != b)
+    ref = nx.pagerank(G, alpha=0.85, tol=1e-14, max_iter=1000)
+    assert max(abs(x[v] - ref[v]) for v in range(V)) < 1e-10
+
+
+# ------------------------------------------------------- triangle counting --
+
+def brute_tc(adj, V):
+    t = [0] * V
+    for u in range(V):
+        for v in range(u + 1, V):
+            for w in range(v + 1, V):
+                if v in adj[u] and w in adj[v] and u in adj[w]:
+                    t[u] += 1
+    return t
+
+
+@pytest.mark.parametrize("n", [3, 5, 9])
+def test_tc_complete_graph_closed_form(n):
+    g = O.OracleGraph(n, False)
+    e = list(itertools.combinations(range(n), 2))
+    g.bulk_build([a for a, _ in e], [b for _, b in e])
+    pv, tot = g.triangles()
+    assert tot == math.comb(n, 3)
+    assert pv.tolist() == [math.comb(n - 1 - i, 2) for i in range(n)]
+
+
+def test_tc_wheel_and_bipartite():
+    m = 8
+    g = O.OracleGraph(m + 1, False)
+    s = [0] * m + list(range(1, m + 1))
+    d = list(range(1, m + 1)) + [1 + (i % m) for i in range(1, m + 1)]
+    g.bulk_build(s, d)
+    pv, tot = g.triangles()
+    assert tot == m and pv[0] == m
+    b = O.OracleGraph(10, False)
+    b.bulk_build([i for i in range(5) for _ in range(5)], [j for _ in range(5) for j in range(5, 10)])
+    assert b.triangles()[1] == 0
+
+
+def test_tc_brute_force_directed_and_undirected():
+    rng = np.random.default_rng(9)
+    for directed in (True, False):
+        V = 25
+        g = O.OracleGraph(V, directed)
+        s = rng.integers(0, V, 150).astype(np.uint32)
+        d = rng.integers(0, V, 150).astype(np.uint32)
+        g.bulk_build(s, d)
+        adj = [set() for _ in range(V)]
+        brute_apply(adj, s, d, directed, False)
+        pv, tot = g.triangles()
+        assert pv.tolist() == brute_tc(adj, V) and tot == sum(brute_tc(adj, V))
+
+
+def test_tc_total_matches_networkx():
+    nx = pytest.importorskip("networkx")
+    V = 300
+    s, d = O.gen_rmat(O.stream_key(6, O.STREAM_RMAT), 9, 4000)
+    s, d = s % V, d % V
+    g = O.OracleGraph(V, False)
+    g.bulk_build(s, d)
+    G = nx.Graph()
+    G.add_edges_from((int(a), int(b)) for a, b in zip(s, d) if a != b)
+    assert g.triangles()[1] == sum(nx.triangles(G).values()) // 3
+
+
+# -------------------------------------------------------------------- BC --
+
+def test_bc_path_star_complete_closed_forms():
+    n = 9
+    p = O.OracleGraph(n, False)
+    p.bulk_build(np.arange(n - 1), np.arange(1, n))
+    bcv = p.bc(np.arange(n))
+    assert np.allclose(bcv, [2 * i * (n - 1 - i) for i in range(n)])
+    st = O.OracleGraph(n, False)
+    st.bulk_build(np.zeros(n - 1), np.arange(1, n))
+    bcs = st.bc(np.arange(n))
+    assert abs(bcs[0] - (n - 1) * (n - 2)) < 1e-9 and np.allclose(bcs[1:], 0)
+    k = O.OracleGraph(6, False)
+    e = list(itertools.combinations(range(6), 2))
+    k.bulk_build([a for a, _ in e], [b for _, b in e])
+    assert np.allclose(k.bc(np.arange(6)), 0)
+
+
+def test_bc_matches_networkx_directed_all_sources():
+    nx = pytest.importorskip("networkx")
+    V = 120
+    s, d = O.gen_rmat(O.stream_key(8, O.STREAM_RMAT), 7, 900)
+    s, d = s % V, d % V
+    g = O.OracleGraph(V, True)
+    g.bulk_build(s, d)
+    G = nx.DiGraph()
+    G.add_nodes_from(range(V))
+    G.add_edges_from((int(a), int(b)) for a, b in zip(s, d) if a != b)
+    ref = nx.betweenness_centrality(G, normalized=False)
+    bcv = g.bc(np.arange(V))
+    assert max(abs(bcv[v] - ref[v]) for v in range(V)) < 1e-9
+
+
+def brute_bc(adj, V, sources):
+    """Enumerate all shortest paths (BFS layering + path counting by DFS)."""
+    out = [0.0] * V
+    for s in sources:
+        dist = {s: 0}
+        q = [s]
+        for u in q:
+            for w in adj[u]:
+                if w not in dist:
+                    dist[w] = dist[u] + 1
+                    q.append(w)
+        for t in dist:
+            if t == s:
+                continue
+            paths = []
+
+            def walk(u, path):
+                if u == t:
+                    paths.append(list(path))
+                    return
+                for w in adj[u]:
+                    if dist.get(w) == dist[u] + 1 and dist[w] <= dist[t]:
+                        path.append(w)
+                        walk(w, path)
+                        path.pop()
+
+            walk(s, [s])
+            for pth in paths:
+                for v in pth[1:-1]:
+                    out[v] += 1.0 / len(paths)
+    return out
+
+
+def test_bc_subset_brute_force():
+    rng = np.random.default_rng(2)
+    V = 18
+    s = rng.integers(0, V, 45).astype(np.uint32)
+    d = rng.integers(0, V, 45).astype(np.uint32)
+    g = O.OracleGraph(V, True)
+    g.bulk_build(s, d)
+    adj = [set() for _ in range(V)]
+    brute_apply(adj, s, d, True, False)
+    src = [0, 3, 7, 11]
+    assert np.allclose(g.bc(src), brute_bc(adj, V, src), atol=1e-12)
+
+
+# ------------------------------------------------------ OpenMP baseline --
+
+def test_omp_baseline_matches_oracle():
+    V = 2000
+    s, d = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 11, 20000)
+    g = O.OracleGraph(V, False)
+    g.bulk_build(s % V, d % V)
+    off, cols = g.export_csr()
+    for b in range(3):
+        si, di = O.gen_uniform(O.stream_key(1, 100 + b), V, 1000)
+        g.insert(si, di)
+        off, cols = O.omp_apply_batch(V, False, off, cols, si, di, delete=False)
+        eo, ec = g.export_csr()
+        assert np.array_equal(off, eo) and np.array_equal(cols, ec)
+        sd, dd = O.gen_sample(O.stream_key(1, 200 + b), V, off, cols, 1000)
+        g.delete(sd, dd)
+        off, cols = O.omp_apply_batch(V, False, off, cols, sd, dd, delete=True)
+        eo, ec = g.export_csr()
+        assert np.array_equal(off, eo) and np.array_equal(cols, ec)
+    x, it = O.omp_pagerank_sym(V, off, cols)
+    xo, ito, _ = g.pagerank()
+    assert it == ito and np.abs(x - xo).sum() < 1e-12
