@@ -1,0 +1,234 @@
+"""Benchmark: plain-form dynamic graph updates on undirected R-MAT scale 22.
+
+Workload (one GPU): undirected R-MAT scale 22 (4,194,304 vertices), edge
+factor 16, a/b/c/d = 0.57/0.19/0.19/0.05, seed 2, symmetrised (~134M directed
+slots).  One step = one update batch of 1,000,000 undirected edges: 500,000
+deletes sampled from existing edges, then 500,000 uniform-random inserts,
+then compaction of vertices whose tombstones exceed 25% of their slots, each
+issued as its own launch sequence (the plain form).  After every timed batch
+PageRank (FP32, alpha 0.85, L1 residual 1e-6) and an exact triangle count run
+on the updated graph and are timed separately.
+
+All inputs are generated on the device before the timed region (synthetic,
+counter-based SplitMix64 streams).  value = edge updates/s over the K timed
+batches (updates = undirected edge updates requested).
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--scale", type=int, default=22)
+    p.add_argument("--edge-factor", type=int, default=16)
+    p.add_argument("--batch", type=int, default=1_000_000)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--analytics-every", type=int, default=1,
+                   help="run PageRank + TC after every k-th timed batch (0 = never)")
+    p.add_argument("--no-tc", action="store_true")
+    p.add_argument("--cpu-baseline", type=int, default=1, help="1 = time the CPU baseline")
+    p.add_argument("--cpu-batches", type=int, default=2)
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import gv_amd as G
+    from gv_amd.hip import DeviceArray, Event, synchronize
+
+    STREAM_RMAT, STREAM_INS, STREAM_DEL = 1, 2, 3
+
+    def skey(seed, stream):
+        # same derivation as the host restatement: mix(mix(seed) ^ stream*C)
+        M = (1 << 64) - 1
+
+        def fmix(z):
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+            return z ^ (z >> 31)
+        return fmix(fmix(seed) ^ ((stream * 0xD6E8FEB86659FD93) & M))
+
+    V = 1 << args.scale
+    n_raw = args.edge_factor * V
+    half = args.batch // 2
+    t_setup = time.time()
+    g = G.GraphVine(V, directed=False)
+    stream = g.stream()
+    src, dst = G.gen_rmat(g, skey(args.seed, STREAM_RMAT), args.scale, n_raw)
+    st = g.bulkBuild(src, dst)
+    src.free()
+    dst.free()
+    E0 = st["live_edges"]
+    off, cols, E = G.export_csr_device(g)
+    nb = args.warmup + args.steps
+    batches = []
+    for b in range(nb):
+        ds, dd = G.gen_sample(g, skey(args.seed, STREAM_DEL * 1000 + b), V, off, cols, half)
+        is_, id_ = G.gen_uniform(g, skey(args.seed, STREAM_INS * 1000 + b), V, half)
+        batches.append((ds, dd, is_, id_))
+    off.free()
+    cols.free()
+    synchronize()
+    setup_s = time.time() - t_setup
+
+    pr_out = DeviceArray(V, np.float32)
+
+    def run_batch(b):
+        ds, dd, is_, id_ = batches[b]
+        g.batchDelete(ds, dd, stats=False)
+        g.batchInsert(is_, id_, stats=False)
+        G.lib().gv_compact(g.handle, 0.25, None)
+
+    for b in range(args.warmup):
+        run_batch(b)
+    g.sync()
+    synchronize()
+
+    upd_ms = []
+    pr_ms, pr_kernel_ms, pr_iters, tc_ms, tc_total = [], [], [], [], []
+    ev0, ev1 = Event(), Event()
+    for k in range(args.steps):
+        b = args.warmup + k
+        ev0.record(stream)
+        run_batch(b)
+        ev1.record(stream)
+        ev1.synchronize()
+        g.sync()  # raises if the device rejected the batch
+        upd_ms.append(ev0.elapsed_ms(ev1))
+        if args.analytics_every and (k + 1) % args.analytics_every == 0:
+            ev0.record(stream)
+            prs = G.pagerank_device(g, pr_out, 0.85, 1e-6, 100)
+            ev1.record(stream)
+            ev1.synchronize()
+            pr_ms.append(ev0.elapsed_ms(ev1))
+            pr_kernel_ms.append(prs["kernel_ms"])
+            pr_iters.append(prs["iters"])
+            pr_edges = prs["edges"]
+            if not args.no_tc:
+                ev0.record(stream)
+                _, tot = G.static_tc(g, per_vertex=False)
+                ev1.record(stream)
+                ev1.synchronize()
+                tc_ms.append(ev0.elapsed_ms(ev1))
+                tc_total.append(tot)
+    E_end = g.num_edges()
+    tot_upd_ms = float(sum(upd_ms))
+    value = args.batch * args.steps / (tot_upd_ms / 1e3)
+
+    out = {
+        "metric": "edge updates/s (plain form, 1M-edge batches) + PageRank GTEPS",
+        "value": value,
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tot_upd_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (updates), f32+f64 residual (PageRank), u64 (TC)",
+        "data": "synthetic (device-generated R-MAT and update batches)",
+        "config": {
+            "workload": f"plain update stream on undirected R-MAT scale {args.scale}, "
+                        f"edge factor {args.edge_factor}, seed {args.seed}; "
+                        f"{args.batch} edges/batch (50% delete sampled, 50% uniform insert), "
+                        "compaction at 25% tombstones",
+            "vertices": V,
+            "directed_edges_initial": int(E0),
+            "directed_edges_final": int(E_end),
+            "batch_edges": args.batch,
+        },
+        "update_ms_per_batch": [round(x, 4) for x in upd_ms],
+        "setup_s": round(setup_s, 2),
+    }
+    if pr_ms:
+        it = float(np.mean(pr_iters))
+        k_ms = float(np.mean(pr_kernel_ms))
+        per_iter_ms = k_ms / it
+        bytes_per_iter = 8.0 * pr_edges + 16.0 * V
+        achieved = bytes_per_iter / (per_iter_ms / 1e3) / 1e9
+        out["pagerank"] = {
+            "iters_mean": it,
+            "call_ms_mean": float(np.mean(pr_ms)),
+            "kernel_ms_per_iter": per_iter_ms,
+            "gteps": pr_edges / (per_iter_ms / 1e3) / 1e9,
+            "gteps_whole_call": pr_edges * it / (float(np.mean(pr_ms)) / 1e3) / 1e9,
+        }
+        out["roofline"] = {
+            "kernel": "k_pr_iter (PageRank pull iteration)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_iter,
+        }
+    if tc_ms:
+        out["triangle_count"] = {"ms_mean": float(np.mean(tc_ms)), "total_last": int(tc_total[-1])}
+
+    if args.cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(g, V, args, batches):
+    """OpenMP CSR baseline (sorted CSR rebuilt per batch by parallel
+    sort-merge) on the same graph and the first batches, timed on this host."""
+    from oracle import oracle as O
+    off, cols = g.export_csr()
+    nbat = min(args.cpu_batches, len(batches))
+    t_total = 0.0
+    done = 0
+    for b in range(nbat):
+        ds, dd, is_, id_ = batches[b]
+        sd, dd_h, si, di = ds.numpy(), dd.numpy(), is_.numpy(), id_.numpy()
+        t0 = time.perf_counter()
+        off, cols = O.omp_apply_batch(V, False, off, cols, sd, dd_h, delete=True)
+        off, cols = O.omp_apply_batch(V, False, off, cols, si, di, delete=False)
+        t_total += time.perf_counter() - t0
+        done += args.batch
+    return {
+        "value": done / t_total,
+        "unit": "edge updates/s",
+        "cores": O.omp_threads(),
+        "kind": "port",
+        "sample": f"{nbat} of the same 1M-edge batches applied to the same scale-{args.scale} "
+                  "graph by OpenMP sorted-CSR rebuild (parallel radix sort + merge)",
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

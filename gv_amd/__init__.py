@@ -36,7 +36,8 @@ GV_MEM_DEVICE = 1
 EXPORTED = [
     "gv_config_default", "gv_create", "gv_destroy", "gv_bulk_build", "gv_batch_insert",
     "gv_batch_delete", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
-    "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_triangle_count",
+    "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_pagerank_ex",
+    "gv_triangle_count",
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_sync", "gv_get_stream",
     "gv_last_error",
 ]
@@ -68,6 +69,15 @@ class GvBatchStats(ctypes.Structure):
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+class GvPagerankStats(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int),
+        ("residual", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("edges", ctypes.c_uint64),
+    ]
 
 
 class GraphVineError(RuntimeError):
@@ -105,6 +115,8 @@ def lib():
         "gv_neighbors": [P, P, u32, P, P, i32],
         "gv_pagerank": [P, ctypes.c_float, ctypes.c_double, i32, P, i32, ctypes.POINTER(i32),
                         ctypes.POINTER(ctypes.c_double)],
+        "gv_pagerank_ex": [P, ctypes.c_float, ctypes.c_double, i32, P, i32,
+                           ctypes.POINTER(GvPagerankStats)],
         "gv_triangle_count": [P, P, i32, ctypes.POINTER(u64)],
         "gv_bc": [P, P, u32, P, i32],
         "gv_gen_rmat": [P, u64, i32, u64, u32, u32, u32, P, P],
@@ -293,6 +305,15 @@ def static_pagerank(graph, alpha=0.85, epsilon=1e-6, max_iter=100):
                              x.ctypes.data_as(ctypes.c_void_p), GV_MEM_HOST, ctypes.byref(it),
                              ctypes.byref(res)))
     return x, int(it.value), float(res.value)
+
+
+def pagerank_device(graph, out_dev, alpha=0.85, epsilon=1e-6, max_iter=100):
+    """PageRank into a device float32[V] buffer; returns the stats dict
+    (iters, residual, kernel_ms = summed iteration-kernel device time, edges)."""
+    st = GvPagerankStats()
+    _check(lib().gv_pagerank_ex(graph.handle, float(alpha), float(epsilon), int(max_iter),
+                                out_dev.ptr, GV_MEM_DEVICE, ctypes.byref(st)))
+    return {f: getattr(st, f) for f, _ in st._fields_}
 
 
 # reference: static_tc (tc.cu:3-27)

@@ -3,6 +3,8 @@
 #pragma once
 #include "gv_internal.h"
 
+#include <vector>
+
 namespace gv {
 
 // Device state of one PageRank run.
@@ -31,6 +33,7 @@ struct PrWork {
   uint64_t chunk_cap = 0, part_cap = 0;
   PrCtl *ctl = nullptr;
   PrCtl *h_ctl = nullptr;        // pinned host mirror
+  std::vector<hipEvent_t> ev;    // per-iteration start/stop events (timed runs)
 };
 
 struct PrBins {
@@ -43,7 +46,7 @@ void pr_release(PrWork &w);
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins, float alpha,
                        double eps, int max_iter, float *out_dev, int *iters, double *residual,
-                       PrWork &w, hipStream_t s);
+                       PrWork &w, hipStream_t s, double *iter_ms = nullptr);
 
 // Triangle count on an undirected store.  per_vertex_dev may be null.
 gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,

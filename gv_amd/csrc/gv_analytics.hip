@@ -93,6 +93,7 @@ void pr_release(PrWork &w) {
   hipFree(w.partials);
   hipFree(w.ctl);
   if (w.h_ctl) hipHostFree(w.h_ctl);
+  for (hipEvent_t e : w.ev) hipEventDestroy(e);
   w = PrWork();
 }
 
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(PR_THREADS) void k_pr_iter(PrArgs a, const float *x
 
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins, float alpha,
                        double eps, int max_iter, float *out_dev, int *iters, double *residual,
-                       PrWork &w, hipStream_t s) {
+                       PrWork &w, hipStream_t s, double *iter_ms) {
   const uint32_t V = in.V;
   if (iters) *iters = 0;
   if (residual) *residual = 0.0;
@@ -376,12 +377,21 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     GV_HIP(hipMalloc(&w.partials, w.part_cap * sizeof(double)));
   }
   a.partials = w.partials;
+  if (iter_ms) {
+    while (w.ev.size() < 2 * (size_t)max_iter) {
+      hipEvent_t e;
+      GV_HIP(hipEventCreate(&e));
+      w.ev.push_back(e);
+    }
+  }
   int launched = 0;
   const int check_every = 8;
   while (launched < max_iter) {
     int t = launched;
+    if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
     k_pr_iter<<<a.nblocks, PR_THREADS, 0, s>>>(a, w.x[t & 1], w.contrib[t & 1], w.x[(t + 1) & 1],
                                                w.contrib[(t + 1) & 1]);
+    if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
     ++launched;
     if (launched % check_every == 0 && launched < max_iter) {
       GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
@@ -396,6 +406,15 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
                         s));
   if (iters) *iters = (int)it;
   if (residual) *residual = w.h_ctl->residual;
+  if (iter_ms) {
+    double tot = 0.0;
+    for (uint32_t t = 0; t < it && (int)t < launched; ++t) {
+      float ms = 0.f;
+      GV_HIP(hipEventElapsedTime(&ms, w.ev[2 * t], w.ev[2 * t + 1]));
+      tot += ms;
+    }
+    *iter_ms = tot;
+  }
   return GV_OK;
 }
 

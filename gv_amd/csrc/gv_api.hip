@@ -498,8 +498,8 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
   return rc;
 }
 
-gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
-                      int *iters, double *residual) {
+static gv_status pagerank_common(gv_graph *g, float alpha, double eps, int max_iter, float *out,
+                                 int mem, int *iters, double *residual, double *iter_ms) {
   gv_status rc = check_graph(g);
   if (rc) return rc;
   if (!out) return GV_E_INVALID;
@@ -515,7 +515,7 @@ gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float 
     if (e != hipSuccess) return hip_fail(e, "pagerank alloc");
   }
   rc = pagerank_run(in, g->st[0].deg, bins, alpha, eps, max_iter, dout, iters, residual, g->pr,
-                    g->stream);
+                    g->stream, iter_ms);
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
@@ -523,6 +523,24 @@ gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float 
   }
   if (mem != GV_MEM_DEVICE) hipFree(dout);
   return rc;
+}
+
+gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
+                      int *iters, double *residual) {
+  return pagerank_common(g, alpha, eps, max_iter, out, mem, iters, residual, nullptr);
+}
+
+gv_status gv_pagerank_ex(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
+                         gv_pagerank_stats *stats) {
+  if (!stats) return GV_E_INVALID;
+  std::memset(stats, 0, sizeof(*stats));
+  gv_status rc = pagerank_common(g, alpha, eps, max_iter, out, mem, &stats->iters,
+                                 &stats->residual, &stats->kernel_ms);
+  if (rc) return rc;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  stats->edges = g->h_ctl[g->has_in ? 1 : 0].live_edges;
+  return GV_OK;
 }
 
 gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total) {

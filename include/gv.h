@@ -110,6 +110,17 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
  * max_iter updates.  out[V] receives x. */
 gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
                       int *iters, double *residual);
+/* gv_pagerank plus measurement: kernel_ms is the summed device time of the
+ * iteration kernels (HIP events on the handle's stream); edges is the live
+ * in-adjacency size each iteration reads. */
+typedef struct gv_pagerank_stats {
+  int iters;
+  double residual;
+  double kernel_ms;
+  uint64_t edges;
+} gv_pagerank_stats;
+gv_status gv_pagerank_ex(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
+                         gv_pagerank_stats *stats);
 /* static_tc on an undirected graph: exact uint64 counts; per_vertex (may be
  * NULL) gets t[u] = triangles whose minimum vertex ID is u. */
 gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total);
