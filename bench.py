@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_CALIB_BYTES = 256 << 20
 
 
 def parse():
@@ -103,6 +104,16 @@ def main():
         run_batch(b)
     g.sync()
     synchronize()
+
+    # Plain device-to-device copy of a known size (the runtime's copy kernel):
+    # the calibration point for HBM byte counters collected with --pmc.
+    calib = DeviceArray(COPY_CALIB_BYTES // 4, np.uint32)
+    calib2 = DeviceArray(COPY_CALIB_BYTES // 4, np.uint32)
+    from gv_amd.hip import D2D, hip
+    hip().hipMemcpy(calib2.ptr, calib.ptr, COPY_CALIB_BYTES, D2D)
+    synchronize()
+    calib.free()
+    calib2.free()
 
     upd_ms = []
     pr_ms, pr_kernel_ms, pr_iters, tc_ms, tc_total = [], [], [], [], []

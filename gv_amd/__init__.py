@@ -32,10 +32,12 @@ GV_E_HIP = -5
 GV_E_UNSUPPORTED = -6
 GV_MEM_HOST = 0
 GV_MEM_DEVICE = 1
+GV_OP_INSERT = 0
+GV_OP_DELETE = 1
 
 EXPORTED = [
     "gv_config_default", "gv_create", "gv_destroy", "gv_bulk_build", "gv_batch_insert",
-    "gv_batch_delete", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
+    "gv_batch_delete", "gv_batch_apply", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
     "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_pagerank_ex",
     "gv_triangle_count",
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_sync", "gv_get_stream",
@@ -107,6 +109,8 @@ def lib():
         "gv_batch_insert": [P, P, P, u64, i32, ctypes.POINTER(GvBatchStats)],
         "gv_batch_delete": [P, P, P, u64, i32, ctypes.POINTER(GvBatchStats)],
         "gv_compact": [P, ctypes.c_float, ctypes.POINTER(u64)],
+        "gv_batch_apply": [P, P, P, P, P, u32, i32, ctypes.c_float,
+                           ctypes.POINTER(GvBatchStats)],
         "gv_num_vertices": [P, ctypes.POINTER(u32)],
         "gv_num_edges": [P, ctypes.POINTER(u64)],
         "gv_degrees": [P, P, i32],
@@ -231,6 +235,25 @@ class GraphVine:
     # reference: GraphVine::batchDelete (graphvine.cu:373-451)
     def batchDelete(self, src, dst, stats=True):
         return self._update(lib().gv_batch_delete, src, dst, stats)
+
+    def applyBatches(self, src, dst, offsets, kinds, compact_fraction=-1.0, stats=True):
+        """Batched form: batches [offsets[b], offsets[b+1]) of src/dst with
+        kinds[b] in {GV_OP_INSERT, GV_OP_DELETE}, applied as one pipeline with
+        the same final graph as applying them in order."""
+        ps, ks, ms = _ptr_u32(src)
+        pd, kd, md = _ptr_u32(dst)
+        if ms != md:
+            raise ValueError("src and dst must both be host or both be device arrays")
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        kin = np.ascontiguousarray(kinds, dtype=np.int32)
+        nb = len(kin)
+        if len(off) != nb + 1 or int(off[-1]) > min(len(src), len(dst)):
+            raise ValueError("offsets must have n_batches+1 entries within the arrays")
+        st = GvBatchStats()
+        _check(lib().gv_batch_apply(self.handle, ps, pd, off.ctypes.data_as(ctypes.c_void_p),
+                                    kin.ctypes.data_as(ctypes.c_void_p), nb, ms,
+                                    float(compact_fraction), ctypes.byref(st) if stats else None))
+        return st.as_dict() if stats else None
 
     # reference: GraphVine::compaction (graphvine.cu:453-464)
     def compaction(self, tombstone_fraction=0.0):

@@ -36,32 +36,10 @@ __device__ __forceinline__ void two_sum_merge(float &s, float &c, float s2, floa
   c = c + c2 - e;
 }
 
-__device__ __forceinline__ unsigned int wave_atomic_add32(unsigned int *ctr, uint32_t inc) {
-  uint32_t incl = wave_incl_scan<uint32_t>(inc);
-  uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-  uint32_t base = 0;
-  if (lane_id() == WAVE - 1 && total) base = atomicAdd(ctr, total);
-  base = __shfl(base, WAVE - 1, WAVE);
-  return base + incl - inc;
-}
-
-template <int NT>
-__device__ __forceinline__ double block_sum_f64(double v) {
-  __shared__ double ws[NT / WAVE];
-  v = wave_sum(v);
-  const int w = threadIdx.x / WAVE;
-  if (lane_id() == 0) ws[w] = v;
-  __syncthreads();
-  double t = 0;
-#pragma unroll
-  for (int i = 0; i < NT / WAVE; ++i) t += ws[i];
-  __syncthreads();
-  return t;
-}
-
 // =============================================================== PageRank ==
 constexpr int PR_THREADS = 256;
 constexpr int PR_GROUP = 8;
+constexpr uint32_t PR_GRID = 2048;  // persistent grid: 256 CUs x 8 blocks
 
 gv_status pr_reserve(PrWork &w, uint32_t V) {
   if (w.V >= V && w.ctl) return GV_OK;
@@ -115,43 +93,73 @@ __global__ void k_pr_init(float *x, float *cb, const uint32_t *deg_out, uint32_t
   }
 }
 
-__global__ void k_pr_count(const uint32_t *used, uint32_t V, PrBins b, PrCtl *ctl) {
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t v = base + threadIdx.x;
-    int bin = -1;
-    uint32_t nch = 0;
-    if (v < V) {
-      uint32_t u = used[v];
-      bin = pr_bin(u, b);
-      if (bin == 3) nch = (u + b.wave_max - 1) / b.wave_max;
+// Bin counts: per-block totals, five atomics per block.
+__global__ __launch_bounds__(256) void k_pr_count(const uint32_t *used, uint32_t V, PrBins b,
+                                                  PrCtl *ctl) {
+  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, nch = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = used[v];
+    int bin = pr_bin(u, b);
+    c0 += bin == 0;
+    c1 += bin == 1;
+    c2 += bin == 2;
+    if (bin == 3) {
+      ++c3;
+      nch += (u + b.wave_max - 1) / b.wave_max;
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      uint32_t c = wave_sum<uint32_t>(bin == k ? 1u : 0u);
-      if (lane_id() == 0 && c) atomicAdd(&ctl->n_bin[k], c);
-    }
-    uint32_t cc = wave_sum<uint32_t>(nch);
-    if (lane_id() == 0 && cc) atomicAdd(&ctl->n_chunks, cc);
+  }
+  uint32_t t0 = block_sum<uint32_t, 256>(c0), t1 = block_sum<uint32_t, 256>(c1);
+  uint32_t t2 = block_sum<uint32_t, 256>(c2), t3 = block_sum<uint32_t, 256>(c3);
+  uint32_t tn = block_sum<uint32_t, 256>(nch);
+  if (threadIdx.x == 0) {
+    if (t0) atomicAdd(&ctl->n_bin[0], t0);
+    if (t1) atomicAdd(&ctl->n_bin[1], t1);
+    if (t2) atomicAdd(&ctl->n_bin[2], t2);
+    if (t3) atomicAdd(&ctl->n_bin[3], t3);
+    if (tn) atomicAdd(&ctl->n_chunks, tn);
   }
 }
 
-__global__ void k_pr_place(const uint32_t *used, uint32_t V, PrBins b, uint32_t *list,
-                           PrCtl *ctl) {
-  const uint32_t o1 = ctl->n_bin[0], o2 = o1 + ctl->n_bin[1], o3 = o2 + ctl->n_bin[2];
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < V;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t v = base + threadIdx.x;
-    int bin = -1;
-    if (v < V) bin = pr_bin(used[v], b);
-    uint32_t p0 = wave_atomic_add32(&ctl->cursor[0], bin == 0);
-    uint32_t p1 = wave_atomic_add32(&ctl->cursor[1], bin == 1);
-    uint32_t p2 = wave_atomic_add32(&ctl->cursor[2], bin == 2);
-    uint32_t p3 = wave_atomic_add32(&ctl->cursor[3], bin == 3);
-    if (bin == 0) list[p0] = (uint32_t)v;
-    if (bin == 1) list[o1 + p1] = (uint32_t)v;
-    if (bin == 2) list[o2 + p2] = (uint32_t)v;
-    if (bin == 3) list[o3 + p3] = (uint32_t)v;
+// Placement: each block owns a contiguous vertex range, reserves its share of
+// every bin with one atomic per bin, then writes its vertices in order.
+__global__ __launch_bounds__(256) void k_pr_place(const uint32_t *used, uint32_t V, PrBins b,
+                                                  uint32_t *list, PrCtl *ctl) {
+  const uint64_t chunk = cdiv_dev(V, gridDim.x);
+  const uint64_t lo = blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < V ? lo + chunk : V;
+  uint32_t c[4] = {0, 0, 0, 0};
+  for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
+    int bin = pr_bin(used[v], b);
+    c[0] += bin == 0;
+    c[1] += bin == 1;
+    c[2] += bin == 2;
+    c[3] += bin == 3;
+  }
+  __shared__ uint32_t base[4];
+  uint32_t tot[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tot[k] = block_sum<uint32_t, 256>(c[k]);
+  if (threadIdx.x == 0) {
+    uint32_t off = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      base[k] = off + (tot[k] ? atomicAdd(&ctl->cursor[k], tot[k]) : 0u);
+      off += ctl->n_bin[k];
+    }
+  }
+  __syncthreads();
+  uint32_t bb[4] = {base[0], base[1], base[2], base[3]};
+  for (uint64_t t0 = lo; t0 < hi; t0 += blockDim.x) {
+    uint64_t v = t0 + threadIdx.x;
+    int bin = v < hi ? pr_bin(used[v], b) : -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t tt;
+      uint32_t ex = block_excl_scan<uint32_t, 256>(bin == k ? 1u : 0u, tt);
+      if (bin == k) list[bb[k] + ex] = (uint32_t)v;
+      bb[k] += tt;
+    }
   }
 }
 
@@ -171,6 +179,8 @@ __global__ void k_pr_hubs(const uint32_t *used, const uint32_t *list, uint32_t o
   }
 }
 
+// Work units of one iteration: 256 single-lane vertices, 32 eight-lane
+// vertices, 4 one-wavefront vertices, or 4 hub chunks.
 struct PrArgs {
   Store in;
   const uint32_t *deg_out;
@@ -184,15 +194,40 @@ struct PrArgs {
   float alpha, base;
   double eps;
   uint32_t n0, n1, n2, n3, o1, o2, o3;
-  uint32_t nb0, nb1, nb2, nb3, nblocks;
+  uint32_t U0, U1, U2, U3, U;
   uint32_t n_chunks, wave_max;
   uint32_t max_iter;
 };
 
-__device__ __forceinline__ void pr_acc(const PrArgs &a, const float *cb, uint32_t tbl, uint32_t pos,
-                                       float &s, float &c) {
-  uint32_t w = a.in.slots[slot_index(a.in.arena, tbl, pos, a.in.log2B, a.in.B)];
-  if (w != TOMB) kahan_add(s, c, cb[w]);
+// Kahan-accumulate contributions of slots start, start+STRIDE, ... < end.
+// Four slot reads are issued before their four gathers, so each lane keeps
+// several independent loads in flight.
+template <int STRIDE>
+__device__ __forceinline__ void pr_gather(const PrArgs &a, const float *cb, uint32_t tbl,
+                                          uint32_t start, uint32_t end, float &s, float &c) {
+  for (uint32_t j = start; j < end; j += 4 * STRIDE) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t pos = j + k * STRIDE;
+      w[k] = pos < end ? a.in.slots[slot_index(a.in.arena, tbl, pos, a.in.log2B, a.in.B)] : TOMB;
+    }
+    float g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = w[k] != TOMB ? cb[w[k]] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (w[k] != TOMB) kahan_add(s, c, g[k]);
+  }
+}
+
+template <int WIDTH>
+__device__ __forceinline__ void group_merge(float &s, float &c) {
+#pragma unroll
+  for (int d = WIDTH / 2; d >= 1; d >>= 1) {
+    float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
+    two_sum_merge(s, c, s2, c2);
+  }
 }
 
 __device__ __forceinline__ double pr_finish(const PrArgs &a, uint32_t v, float s, float c,
@@ -207,101 +242,90 @@ __device__ __forceinline__ double pr_finish(const PrArgs &a, uint32_t v, float s
 __global__ __launch_bounds__(PR_THREADS) void k_pr_iter(PrArgs a, const float *x, const float *cb,
                                                         float *xn, float *cn) {
   if (a.ctl->done) return;
-  const uint32_t b = blockIdx.x;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wid = tid / WAVE;
   double diff = 0.0;
-  if (b < a.nb0) {
-    uint32_t i = b * PR_THREADS + tid;
-    if (i < a.n0) {
-      uint32_t v = a.list[i];
-      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
-      float s = 0.f, c = 0.f;
-      for (uint32_t j = 0; j < us; ++j) pr_acc(a, cb, tbl, j, s, c);
-      diff = pr_finish(a, v, s, c, x, xn, cn);
-    }
-  } else if (b < a.nb0 + a.nb1) {
-    uint32_t g = ((b - a.nb0) * PR_THREADS + tid) / PR_GROUP;
-    uint32_t gl = tid & (PR_GROUP - 1);
-    uint32_t v = 0, us = 0, tbl = 0;
-    if (g < a.n1) {
-      v = a.list[a.o1 + g];
-      us = a.in.used[v];
-      tbl = a.in.tbl_off[v];
-    }
-    float s = 0.f, c = 0.f;
-    for (uint32_t j = gl; j < us; j += PR_GROUP) pr_acc(a, cb, tbl, j, s, c);
-#pragma unroll
-    for (int d = PR_GROUP / 2; d >= 1; d >>= 1) {
-      float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
-      two_sum_merge(s, c, s2, c2);
-    }
-    if (g < a.n1 && gl == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
-  } else if (b < a.nb0 + a.nb1 + a.nb2) {
-    uint32_t g = (b - a.nb0 - a.nb1) * (PR_THREADS / WAVE) + wid;
-    if (g < a.n2) {
-      uint32_t v = a.list[a.o2 + g];
-      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
-      float s = 0.f, c = 0.f;
-      for (uint32_t j = lane; j < us; j += WAVE) pr_acc(a, cb, tbl, j, s, c);
-#pragma unroll
-      for (int d = WAVE / 2; d >= 1; d >>= 1) {
-        float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
-        two_sum_merge(s, c, s2, c2);
+  for (uint32_t u = blockIdx.x; u < a.U; u += gridDim.x) {
+    if (u < a.U0) {
+      uint32_t i = u * PR_THREADS + tid;
+      if (i < a.n0) {
+        uint32_t v = a.list[i];
+        uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+        float s = 0.f, c = 0.f;
+        pr_gather<1>(a, cb, tbl, 0, us, s, c);
+        diff += pr_finish(a, v, s, c, x, xn, cn);
       }
-      if (lane == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
-    }
-  } else {
-    uint32_t ci = (b - a.nb0 - a.nb1 - a.nb2) * (PR_THREADS / WAVE) + wid;
-    if (ci < a.n_chunks) {
-      uint32_t h = a.chunk_hub[ci];
-      uint32_t v = a.list[a.o3 + h];
-      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
-      uint32_t hb = a.hub_base[h];
-      uint32_t c0 = (ci - hb) * a.wave_max;
-      uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
-      float s = 0.f, c = 0.f;
-      for (uint32_t j = c0 + lane; j < c1; j += WAVE) pr_acc(a, cb, tbl, j, s, c);
-#pragma unroll
-      for (int d = WAVE / 2; d >= 1; d >>= 1) {
-        float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
-        two_sum_merge(s, c, s2, c2);
+    } else if (u < a.U0 + a.U1) {
+      uint32_t g = (u - a.U0) * (PR_THREADS / PR_GROUP) + tid / PR_GROUP;
+      uint32_t gl = tid & (PR_GROUP - 1);
+      uint32_t v = 0, us = 0, tbl = 0;
+      if (g < a.n1) {
+        v = a.list[a.o1 + g];
+        us = a.in.used[v];
+        tbl = a.in.tbl_off[v];
       }
-      if (lane == 0) {
-        a.chunk_part[ci] = make_float2(s, c);
-        __threadfence();
-        uint32_t nch = (us + a.wave_max - 1) / a.wave_max;
-        uint32_t old = atomicAdd(&a.hub_cnt[h], 1u);
-        if (old == nch - 1) {
+      float s = 0.f, c = 0.f;
+      pr_gather<PR_GROUP>(a, cb, tbl, gl, us, s, c);
+      group_merge<PR_GROUP>(s, c);
+      if (g < a.n1 && gl == 0) diff += pr_finish(a, v, s, c, x, xn, cn);
+    } else if (u < a.U0 + a.U1 + a.U2) {
+      uint32_t g = (u - a.U0 - a.U1) * (PR_THREADS / WAVE) + wid;
+      if (g < a.n2) {
+        uint32_t v = a.list[a.o2 + g];
+        uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+        float s = 0.f, c = 0.f;
+        pr_gather<WAVE>(a, cb, tbl, lane, us, s, c);
+        group_merge<WAVE>(s, c);
+        if (lane == 0) diff += pr_finish(a, v, s, c, x, xn, cn);
+      }
+    } else {
+      uint32_t ci = (u - a.U0 - a.U1 - a.U2) * (PR_THREADS / WAVE) + wid;
+      if (ci < a.n_chunks) {
+        uint32_t h = a.chunk_hub[ci];
+        uint32_t v = a.list[a.o3 + h];
+        uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+        uint32_t hb = a.hub_base[h];
+        uint32_t c0 = (ci - hb) * a.wave_max;
+        uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
+        float s = 0.f, c = 0.f;
+        pr_gather<WAVE>(a, cb, tbl, c0 + lane, c1, s, c);
+        group_merge<WAVE>(s, c);
+        if (lane == 0) {
+          a.chunk_part[ci] = make_float2(s, c);
           __threadfence();
-          float ts = 0.f, tc = 0.f;
-          for (uint32_t k = 0; k < nch; ++k) {
-            float2 p = a.chunk_part[hb + k];
-            two_sum_merge(ts, tc, p.x, p.y);
+          uint32_t nch = (us + a.wave_max - 1) / a.wave_max;
+          uint32_t old = atomicAdd(&a.hub_cnt[h], 1u);
+          if (old == nch - 1) {
+            __threadfence();
+            float ts = 0.f, tc = 0.f;
+            for (uint32_t k = 0; k < nch; ++k) {
+              float2 p = a.chunk_part[hb + k];
+              two_sum_merge(ts, tc, p.x, p.y);
+            }
+            a.partials[gridDim.x + h] = pr_finish(a, v, ts, tc, x, xn, cn);
+            a.hub_cnt[h] = 0;
           }
-          a.partials[a.nblocks + h] = pr_finish(a, v, ts, tc, x, xn, cn);
-          a.hub_cnt[h] = 0;
         }
       }
     }
   }
-  double bs = block_sum_f64<PR_THREADS>(diff);
+  double bs = block_sum<double, PR_THREADS>(diff);
   __shared__ bool last;
   if (tid == 0) {
-    a.partials[b] = bs;
+    a.partials[blockIdx.x] = bs;
     __threadfence();
     unsigned t = atomicAdd(&a.ctl->blocks_done, 1u);
-    last = (t == a.nblocks - 1);
+    last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
   __threadfence();
   // Fixed-order reduction of all block and hub partials.
-  const uint32_t np = a.nblocks + a.n3;
+  const uint32_t np = gridDim.x + a.n3;
   double r = 0.0;
   for (uint32_t i = tid; i < np; i += PR_THREADS) r += a.partials[i];
-  r = block_sum_f64<PR_THREADS>(r);
+  r = block_sum<double, PR_THREADS>(r);
   if (tid == 0) {
     a.ctl->residual = r;
     unsigned it = a.ctl->iters + 1;
@@ -325,9 +349,10 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0);
   if (max_iter <= 0) {
     GV_HIP(hipMemcpyAsync(out_dev, w.x[0], (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    GV_HIP(hipStreamSynchronize(s));
     return GV_OK;
   }
-  k_pr_count<<<grid_for(V), 256, 0, s>>>(in.used, V, bins, w.ctl);
+  k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(in.used, V, bins, w.ctl);
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
   const PrCtl hc = *w.h_ctl;
@@ -340,7 +365,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     GV_HIP(hipMalloc(&w.chunk_hub, w.chunk_cap * sizeof(uint32_t)));
     GV_HIP(hipMalloc(&w.chunk_part, w.chunk_cap * sizeof(float2)));
   }
-  k_pr_place<<<grid_for(V), 256, 0, s>>>(in.used, V, bins, w.list, w.ctl);
+  k_pr_place<<<grid_for(V, 256, 512), 256, 0, s>>>(in.used, V, bins, w.list, w.ctl);
   PrArgs a;
   a.in = in;
   a.deg_out = deg_out;
@@ -360,20 +385,21 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   a.o1 = n0;
   a.o2 = n0 + n1;
   a.o3 = n0 + n1 + n2;
-  a.nb0 = (uint32_t)cdiv(n0, PR_THREADS);
-  a.nb1 = (uint32_t)cdiv((uint64_t)n1 * PR_GROUP, PR_THREADS);
-  a.nb2 = (uint32_t)cdiv(n2, PR_THREADS / WAVE);
-  a.nb3 = (uint32_t)cdiv(nchunks, PR_THREADS / WAVE);
-  a.nblocks = a.nb0 + a.nb1 + a.nb2 + a.nb3;
+  a.U0 = (uint32_t)cdiv(n0, PR_THREADS);
+  a.U1 = (uint32_t)cdiv(n1, PR_THREADS / PR_GROUP);
+  a.U2 = (uint32_t)cdiv(n2, PR_THREADS / WAVE);
+  a.U3 = (uint32_t)cdiv(nchunks, PR_THREADS / WAVE);
+  a.U = a.U0 + a.U1 + a.U2 + a.U3;
   a.n_chunks = nchunks;
   a.wave_max = bins.wave_max;
   a.max_iter = (uint32_t)max_iter;
+  const uint32_t grid = a.U < PR_GRID ? (a.U ? a.U : 1) : PR_GRID;
   if (n3)
     k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(in.used, w.list, a.o3, n3, bins.wave_max, w.hub_base,
                                            w.hub_cnt, w.chunk_hub, &w.ctl->chunk_cursor);
-  if (a.nblocks + n3 > w.part_cap) {
+  if ((uint64_t)grid + n3 > w.part_cap) {
     hipFree(w.partials);
-    w.part_cap = a.nblocks + n3 + 1024;
+    w.part_cap = (uint64_t)grid + n3 + 1024;
     GV_HIP(hipMalloc(&w.partials, w.part_cap * sizeof(double)));
   }
   a.partials = w.partials;
@@ -389,7 +415,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   while (launched < max_iter) {
     int t = launched;
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
-    k_pr_iter<<<a.nblocks, PR_THREADS, 0, s>>>(a, w.x[t & 1], w.contrib[t & 1], w.x[(t + 1) & 1],
+    k_pr_iter<<<grid, PR_THREADS, 0, s>>>(a, w.x[t & 1], w.contrib[t & 1], w.x[(t + 1) & 1],
                                                w.contrib[(t + 1) & 1]);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
     ++launched;
@@ -419,7 +445,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
 }
 
 // ===================================================== triangle counting ==
-constexpr uint32_t TC_LDS = 2048;  // oriented-list entries staged per wavefront
+constexpr uint32_t TC_HASH = 2048;  // LDS hash slots per wavefront (8 KiB)
 
 __device__ __forceinline__ bool tc_higher(uint32_t dw, uint32_t w, uint32_t du, uint32_t u) {
   return dw > du || (dw == du && w > u);
@@ -476,60 +502,109 @@ __device__ __forceinline__ bool tc_find(const uint32_t *a, uint32_t n, uint32_t 
 }
 
 template <bool PV>
+__device__ __forceinline__ void tc_hit(uint32_t u, uint32_t v, uint32_t w, unsigned long long &cnt,
+                                       unsigned long long &own, unsigned long long *pv) {
+  ++cnt;
+  if (PV) {
+    uint32_t mn = u < v ? u : v;
+    mn = mn < w ? mn : w;
+    if (mn == u) ++own;
+    else atomicAdd(&pv[mn], 1ull);
+  }
+}
+
+__device__ __forceinline__ uint32_t tc_hash(uint32_t w, uint32_t mask) {
+  return (w * 0x9E3779B1u) >> 7 & mask;
+}
+
+// One wavefront per root u with |N+(u)| >= 2.  N+(u) is held as the
+// intersection set: sorted in registers (<= 64 entries, six-step ds_bpermute
+// search), in an LDS open-addressing hash (<= TC_HASH/2 entries), or searched
+// in the sorted global list.  The wedges (v in N+(u), w in N+(v)) are
+// flattened 64 at a time; each lane walks its own cursor through the 64-entry
+// prefix of |N+(v)|, so the mapping needs no search.
+template <bool PV>
 __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uint32_t *cols,
                                                   uint32_t V, unsigned long long *pv,
                                                   unsigned long long *total) {
-  __shared__ uint32_t s_nbr[4][TC_LDS];
+  __shared__ uint32_t s_tab[4][TC_HASH];
   __shared__ uint32_t s_pre[4][WAVE];
   __shared__ uint32_t s_v[4][WAVE];
+  __shared__ uint64_t s_off[4][WAVE];
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x / WAVE;
+  uint32_t *tab = s_tab[wid];
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   unsigned long long cnt = 0;
-  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < V; u += nwaves) {
+  for (uint64_t uu = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; uu < V;
+       uu += nwaves) {
+    const uint32_t u = (uint32_t)uu;
     const uint64_t ou = off[u];
     const uint32_t du = (uint32_t)(off[u + 1] - ou);
     if (du < 2) continue;
     const uint32_t *nu = cols + ou;
-    const bool lds = du <= TC_LDS;
-    if (lds)
-      for (uint32_t j = lane; j < du; j += WAVE) s_nbr[wid][j] = nu[j];
-    const uint32_t *su = lds ? s_nbr[wid] : nu;
+    // mode 0: registers, 1: LDS hash, 2: global binary search
+    const int mode = du <= WAVE ? 0 : (du <= TC_HASH / 2 ? 1 : 2);
+    uint32_t sv = 0xFFFFFFFFu, hmask = 0;
+    if (mode == 0) {
+      sv = lane < du ? nu[lane] : 0xFFFFFFFFu;
+    } else if (mode == 1) {
+      uint32_t hsz = 1u << (32 - __clz(2 * du - 1));
+      hmask = hsz - 1;
+      for (uint32_t j = lane; j < hsz; j += WAVE) tab[j] = 0xFFFFFFFFu;
+      for (uint32_t j = lane; j < du; j += WAVE) {
+        uint32_t x = nu[j];
+        uint32_t h = tc_hash(x, hmask);
+        while (atomicCAS(&tab[h], 0xFFFFFFFFu, x) != 0xFFFFFFFFu) h = (h + 1) & hmask;
+      }
+    }
     unsigned long long own = 0;
     for (uint32_t c0 = 0; c0 < du; c0 += WAVE) {
       uint32_t j = c0 + lane;
       uint32_t v = 0, len = 0;
       uint64_t ov = 0;
       if (j < du) {
-        v = su[j];
+        v = nu[j];
         ov = off[v];
         len = (uint32_t)(off[v + 1] - ov);
       }
       uint32_t pre = wave_incl_scan<uint32_t>(len);
-      uint32_t W = __shfl(pre, WAVE - 1, WAVE);
+      const uint32_t W = __shfl(pre, WAVE - 1, WAVE);
       s_pre[wid][lane] = pre;
       s_v[wid][lane] = v;
+      s_off[wid][lane] = ov;
+      uint32_t k = 0;  // this lane's current v index (advances monotonically)
       for (uint32_t fb = 0; fb < W; fb += WAVE) {
-        uint32_t f = fb + lane;
-        if (f < W) {
-          uint32_t lo = 0, hi = WAVE - 1;  // first k with pre[k] > f
-          while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (s_pre[wid][mid] > f) hi = mid; else lo = mid + 1;
-          }
-          uint32_t vk = s_v[wid][lo];
-          uint32_t start = lo ? s_pre[wid][lo - 1] : 0u;
-          uint32_t wv = cols[off[vk] + (f - start)];
-          if (tc_find(su, du, wv)) {
-            ++cnt;
-            if (PV) {
-              uint32_t mn = (uint32_t)u < vk ? (uint32_t)u : vk;
-              mn = mn < wv ? mn : wv;
-              if (mn == (uint32_t)u) ++own;
-              else atomicAdd(&pv[mn], 1ull);
+        const uint32_t f = fb + lane;
+        const bool act = f < W;
+        uint32_t wv = 0xFFFFFFFFu, vk = 0;
+        if (act) {
+          while (s_pre[wid][k] <= f) ++k;
+          const uint32_t start = k ? s_pre[wid][k - 1] : 0u;
+          vk = s_v[wid][k];
+          wv = cols[s_off[wid][k] + (f - start)];
+        }
+        bool found = false;
+        if (mode == 0) {
+          // both shuffles run on every lane: a ds_bpermute issued under a
+          // partial exec mask reads zero from the disabled source lanes
+          uint32_t p = wave_lower_bound(sv, wv);
+          uint32_t hit = __shfl(sv, (int)p, WAVE);
+          found = act && hit == wv;
+        } else if (act) {
+          if (mode == 1) {
+            uint32_t h = tc_hash(wv, hmask);
+            while (true) {
+              uint32_t e = tab[h];
+              if (e == wv) { found = true; break; }
+              if (e == 0xFFFFFFFFu) break;
+              h = (h + 1) & hmask;
             }
+          } else {
+            found = tc_find(nu, du, wv);
           }
         }
+        if (found) tc_hit<PV>(u, vk, wv, cnt, own, pv);
       }
     }
     if (PV) {

@@ -34,6 +34,9 @@ struct gv_graph {
   Ctl *h_ctl = nullptr;  // pinned, 2 entries
   uint32_t *in_src = nullptr, *in_dst = nullptr;
   uint64_t in_cap = 0;
+  uint64_t *grp_off = nullptr;   // device copy of a group's batch offsets
+  uint32_t *grp_kind = nullptr;  // device copy of a group's batch kinds
+  uint32_t grp_cap = 0;
 };
 
 extern "C" {
@@ -181,7 +184,104 @@ static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *d
   return take_abort(g);
 }
 
+// A group of batches in one pipeline (see gv_batch_apply in gv.h).
+static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *dst,
+                             const uint64_t *offsets, const int *kinds, uint32_t nb, int mem,
+                             float compact_fraction, gv_batch_stats *stats) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (nb == 0) return GV_OK;
+  if (!offsets || !kinds) {
+    set_error("null offsets or kinds");
+    return GV_E_INVALID;
+  }
+  const uint64_t n = offsets[nb];
+  if (offsets[0] != 0) {
+    set_error("offsets[0] must be 0");
+    return GV_E_INVALID;
+  }
+  for (uint32_t b = 0; b < nb; ++b) {
+    if (offsets[b + 1] < offsets[b]) {
+      set_error("batch offsets must be non-decreasing");
+      return GV_E_INVALID;
+    }
+    if (kinds[b] != GV_OP_INSERT && kinds[b] != GV_OP_DELETE) {
+      set_error("batch kind must be GV_OP_INSERT or GV_OP_DELETE");
+      return GV_E_INVALID;
+    }
+  }
+  if (2 * id_bits(g->V) + id_bits(nb) > 64) {
+    set_error("too many batches in one group for this vertex count");
+    return GV_E_INVALID;
+  }
+  if (n > 0x7FFFFFFFull) {
+    set_error("group larger than 2^31 pairs; split it");
+    return GV_E_INVALID;
+  }
+  if (n && (!src || !dst)) {
+    set_error("null batch pointer");
+    return GV_E_INVALID;
+  }
+  hipStream_t s = g->stream;
+  if (n) {
+    const uint32_t *ds = nullptr, *dd = nullptr;
+    rc = device_inputs(g, src, dst, n, mem, &ds, &dd);
+    if (rc) return rc;
+    if (nb > g->grp_cap) {
+      hipFree(g->grp_off);
+      hipFree(g->grp_kind);
+      g->grp_off = nullptr;
+      g->grp_kind = nullptr;
+      uint32_t cap = std::max<uint32_t>(nb, 64);
+      GV_HIP(hipMalloc(&g->grp_off, ((size_t)cap + 1) * sizeof(uint64_t)));
+      GV_HIP(hipMalloc(&g->grp_kind, (size_t)cap * sizeof(uint32_t)));
+      g->grp_cap = cap;
+    }
+    std::vector<uint32_t> hk(nb);
+    for (uint32_t b = 0; b < nb; ++b) hk[b] = kinds[b] == GV_OP_DELETE ? OP_DELETE : OP_INSERT;
+    GV_HIP(hipMemcpyAsync(g->grp_off, offsets, ((size_t)nb + 1) * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, s));
+    GV_HIP(hipMemcpyAsync(g->grp_kind, hk.data(), (size_t)nb * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    const int nst = g->has_in ? 2 : 1;
+    uint64_t m[2] = {0, 0};
+    uint64_t *keys[2] = {nullptr, nullptr};
+    for (int k = 0; k < nst; ++k) {
+      const int mirror = g->directed ? 0 : 1;
+      rc = scratch_reserve(g->sc[k], g->st[k], mirror ? 2 * n : n);
+      if (rc) return rc;
+      op_begin(g->st[k], s);
+      keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, g->grp_off, g->grp_kind, nb, mirror,
+                            k == 1 ? 1 : 0, m[k], s);
+      segment(g->st[k], g->sc[k], keys[k], m[k], s, g->sc[k].op);
+      mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
+    }
+    commit_pair(g->st[0], g->sc[0], nst > 1 ? &g->st[1] : nullptr, nst > 1 ? &g->sc[1] : nullptr,
+                s);
+    for (int k = 0; k < nst; ++k) insert_write(g->st[k], g->sc[k], keys[k], m[k], s);
+  }
+  if (compact_fraction >= 0.f) {
+    for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
+      if (!n) op_begin(g->st[k], s);
+      compact(g->st[k], compact_fraction, g->vlist, s);
+    }
+  }
+  GV_HIP(hipGetLastError());
+  if (!stats) return GV_OK;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  fill_stats(g, n, stats);
+  return take_abort(g);
+}
+
 extern "C" {
+
+gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
+                         const uint64_t *offsets, const int *kinds, uint32_t n_batches, int mem,
+                         float compact_fraction, gv_batch_stats *stats) {
+  return apply_group(g, src, dst, offsets, kinds, n_batches, mem, compact_fraction, stats);
+}
 
 gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed,
                     gv_graph **out) {
@@ -264,6 +364,8 @@ gv_status gv_destroy(gv_graph *g) {
   hipFree(g->vlist);
   hipFree(g->in_src);
   hipFree(g->in_dst);
+  hipFree(g->grp_off);
+  hipFree(g->grp_kind);
   if (g->h_ctl) hipHostFree(g->h_ctl);
   if (g->stream) hipStreamDestroy(g->stream);
   delete g;

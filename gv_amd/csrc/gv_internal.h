@@ -79,8 +79,9 @@ struct Scratch {
   uint32_t *seg_cnt = nullptr;            // per segment: items, then blocks/deletes
   uint32_t *item_start = nullptr;         // per segment: first item (cap+1)
   uint32_t *item_seg = nullptr;           // per item: segment
-  uint32_t *seg_blk = nullptr;            // per segment: relative block base
-  uint32_t *seg_tbl = nullptr;            // per segment: relative arena base or ~0
+  uint32_t *seg_blk = nullptr;            // per segment: tombstones written (delete)
+  uint64_t *plan = nullptr;               // per segment: (new blocks << 32 | table cap), scanned
+  uint32_t *op = nullptr;                 // per key: final op of a merged group
   void *tmp = nullptr;                    // scan/sort partials
   size_t tmp_bytes = 0;
 };
@@ -105,6 +106,7 @@ inline uint32_t id_bits(uint32_t V) {
   return b ? b : 1;
 }
 inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__device__ __forceinline__ uint64_t cdiv_dev(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // ------------------------------------------------------- device helpers --
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -131,6 +133,53 @@ __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int d = WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
   return v;
+}
+
+// Sum over the block (all NT threads must call it).
+template <typename T, int NT>
+__device__ __forceinline__ T block_sum(T v) {
+  __shared__ T ws[NT / WAVE];
+  v = wave_sum(v);
+  if (lane_id() == 0) ws[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  T t = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) t += ws[i];
+  __syncthreads();
+  return t;
+}
+
+// Exclusive scan over the block; returns this thread's exclusive prefix and
+// the block total (all NT threads must call it).
+template <typename T, int NT>
+__device__ __forceinline__ T block_excl_scan(T v, T &total) {
+  __shared__ T wsum[NT / WAVE];
+  T incl = wave_incl_scan<T>(v);
+  const int w = threadIdx.x / WAVE;
+  if (lane_id() == WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  T before = 0, all = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) {
+    T x = wsum[i];
+    if (i < w) before += x;
+    all += x;
+  }
+  __syncthreads();
+  total = all;
+  return before + incl - v;
+}
+
+// Lower bound of x in the ascending values held one per lane (sv, padded with
+// 0xFFFFFFFF past the segment): six uniform ds_bpermute steps, no divergence.
+__device__ __forceinline__ uint32_t wave_lower_bound(uint32_t sv, uint32_t x) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t step = WAVE / 2; step >= 1; step >>= 1) {
+    uint32_t v = __shfl(sv, (int)(pos + step - 1), WAVE);
+    if (v < x) pos += step;
+  }
+  return pos;
 }
 
 // Wave-aggregated atomic add: every lane of the wave calls it (inc may be 0);

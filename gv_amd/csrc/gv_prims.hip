@@ -25,27 +25,6 @@ __device__ __forceinline__ uint64_t eff_n(uint64_t n, const uint32_t *n_dev) {
   return d < n ? d : n;
 }
 
-// Exclusive scan over the block; returns this thread's exclusive prefix and
-// the block total.
-template <typename T, int NT>
-__device__ __forceinline__ T block_excl_scan(T v, T &total) {
-  __shared__ T wsum[NT / WAVE];
-  T incl = wave_incl_scan<T>(v);
-  const int w = threadIdx.x / WAVE;
-  if (lane_id() == WAVE - 1) wsum[w] = incl;
-  __syncthreads();
-  T before = 0, all = 0;
-#pragma unroll
-  for (int i = 0; i < NT / WAVE; ++i) {
-    T x = wsum[i];
-    if (i < w) before += x;
-    all += x;
-  }
-  __syncthreads();
-  total = all;
-  return before + incl - v;
-}
-
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T *in, uint64_t n,
                                                               const uint32_t *n_dev, T *tile_sums) {

@@ -6,6 +6,10 @@ namespace gv {
 
 constexpr uint32_t PROBE_CHUNK = 512;  // slots per membership/tombstone work item
 
+// Operation of a key in a merged group of batches.
+constexpr uint32_t OP_INSERT = 0;
+constexpr uint32_t OP_DELETE = 1;
+
 // Abort codes kept in Ctl::abort (magnitudes of the GV_E_* values).
 constexpr uint32_t ABORT_RANGE = 2;
 constexpr uint32_t ABORT_POOL = 4;
@@ -23,8 +27,20 @@ void op_begin(Store &st, hipStream_t s);
 // swap stores (dst,src) instead.  Returns the sorted key buffer; m = keys.
 uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst, uint64_t n,
                 int mirror, int swap, uint64_t &m, hipStream_t s);
-// Segment the staged keys and build the probe work list.
-void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// Stage a group of nb batches (entries [boff[b], boff[b+1]) of batch b, op
+// kinds[b]; both device arrays) as unique sorted keys with their final op in
+// sc.op.  Returns the key buffer (m keys, sentinel-padded).
+uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst,
+                      uint64_t n, const uint64_t *boff, const uint32_t *kinds, uint32_t nb,
+                      int mirror, int swap, uint64_t &m, hipStream_t s);
+// Segment the staged keys and build the probe work list (op: per-key final
+// operation of a merged group, or null).
+void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s,
+             const uint32_t *op = nullptr);
+// Merged group: probe (tombstone / mark present), fix degrees, plan inserts.
+void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// Survivor scan + per-segment block/table plan (after an insert probe).
+void insert_alloc(Store &st, Scratch &sc, uint64_t m, hipStream_t s);
 // Insert: probe existing slots, plan block/table allocation.
 void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Joint all-or-nothing pool check over one or two stores (b may be null).

@@ -15,9 +15,10 @@
 //                  each live slot in the source's sorted batch segment:
 //                  insert -> mark the batch key as already present;
 //                  delete -> tombstone the slot.
-//   insert only:   scan of survivors -> k_ins_plan (wave-aggregated atomic
-//                  reservation of pool blocks and table space) -> k_commit
-//                  (all-or-nothing pool check) -> k_ins_tables -> k_ins_write.
+//   insert only:   scan of survivors -> k_ins_plan (new blocks and table
+//                  capacity per segment, packed and scanned together) ->
+//                  k_commit (all-or-nothing pool check, bulk pop of the free
+//                  stack) -> k_ins_tables -> k_ins_write.
 //   delete only:   k_del_finish (degrees, counters).
 // The reference does the same work with O(V) host and device passes per batch
 // (graphvine.cu:222-451); here every pass is O(batch + touched adjacency).
@@ -113,8 +114,9 @@ gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m) {
   GV_HIP(hipMalloc(&sc.k0, cap * sizeof(uint64_t)));
   GV_HIP(hipMalloc(&sc.k1, cap * sizeof(uint64_t)));
   uint32_t **arrs[] = {&sc.flag, &sc.segof, &sc.rank, &sc.seg_start, &sc.seg_src, &sc.seg_used,
-                       &sc.seg_cnt, &sc.item_start, &sc.seg_blk, &sc.seg_tbl};
+                       &sc.seg_cnt, &sc.item_start, &sc.seg_blk, &sc.op};
   for (auto a : arrs) GV_HIP(hipMalloc(a, (cap + 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&sc.plan, (cap + 1) * sizeof(uint64_t)));
   GV_HIP(hipMalloc(&sc.item_seg, sc.item_cap * sizeof(uint32_t)));
   size_t t = sort_tmp_bytes(cap);
   size_t t2 = scan_tmp_bytes(cap + 1);
@@ -126,10 +128,11 @@ gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m) {
 void scratch_release(Scratch &sc) {
   hipFree(sc.k0);
   hipFree(sc.k1);
-  uint32_t *arrs[] = {sc.flag,     sc.segof,      sc.rank,    sc.seg_start, sc.seg_src,
-                      sc.seg_used, sc.seg_cnt,    sc.item_start, sc.seg_blk, sc.seg_tbl,
-                      sc.item_seg};
+  uint32_t *arrs[] = {sc.flag,    sc.segof,   sc.rank,       sc.seg_start, sc.seg_src,
+                      sc.seg_used, sc.seg_cnt, sc.item_start, sc.seg_blk,   sc.item_seg,
+                      sc.op};
   for (auto a : arrs) hipFree(a);
+  hipFree(sc.plan);
   hipFree(sc.tmp);
   sc = Scratch();
 }
@@ -181,9 +184,90 @@ uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst
   return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s);
 }
 
+// Group staging: key = ((u << bV | v) << sb) | batch, so after the sort the
+// entries of one edge sit together in batch order and the last one holds the
+// edge's final operation.
+__global__ void k_stage_seq(const uint32_t *src, const uint32_t *dst, uint64_t n,
+                            const uint64_t *boff, uint32_t nb, uint32_t V, uint32_t bV,
+                            uint32_t sb, int mirror, int swap, uint64_t *keys, Ctl *ctl) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nb;  // largest b with boff[b] <= i
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (boff[mid] <= i) lo = mid; else hi = mid;
+    }
+    uint32_t u = src[i], v = dst[i];
+    if (swap) {
+      uint32_t t = u;
+      u = v;
+      v = t;
+    }
+    bool bad = u >= V || v >= V;
+    if (bad) atomicCAS(&ctl->abort, 0u, ABORT_RANGE);
+    bool drop = bad || u == v;
+    uint64_t k = drop ? KEY_SENT : (((((uint64_t)u << bV) | v) << sb) | lo);
+    if (mirror) {
+      keys[2 * i] = k;
+      keys[2 * i + 1] = drop ? KEY_SENT : (((((uint64_t)v << bV) | u) << sb) | lo);
+    } else {
+      keys[i] = k;
+    }
+  }
+}
+
+__device__ __forceinline__ bool last_of_edge(const uint64_t *keys, uint64_t m, uint64_t i,
+                                             uint32_t sb) {
+  uint64_t k = keys[i];
+  if (k == KEY_SENT) return false;
+  return i + 1 == m || (keys[i + 1] >> sb) != (k >> sb);
+}
+
+__global__ void k_last_mark(const uint64_t *keys, uint64_t m, uint32_t sb, uint32_t *flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    flag[i] = last_of_edge(keys, m, i, sb) ? 1u : 0u;
+}
+
+// Keep the last entry of every edge (without its batch tag) and its op; pad
+// the tail with the sentinel so the array stays m long.
+__global__ void k_last_compact(const uint64_t *keys, uint64_t m, uint32_t sb,
+                               const uint32_t *pos, const uint32_t *kinds, uint64_t *out,
+                               uint32_t *op) {
+  const uint32_t total = pos[m];
+  const uint64_t smask = (1ull << sb) - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (last_of_edge(keys, m, i, sb)) {
+      uint64_t k = keys[i];
+      out[pos[i]] = k >> sb;
+      op[pos[i]] = kinds[k & smask];
+    }
+    if (i >= total) out[i] = KEY_SENT;
+  }
+}
+
+uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst,
+                      uint64_t n, const uint64_t *boff, const uint32_t *kinds, uint32_t nb,
+                      int mirror, int swap, uint64_t &m, hipStream_t s) {
+  m = mirror ? 2 * n : n;
+  if (!n) return sc.k0;
+  const uint32_t bV = id_bits(st.V), sb = id_bits(nb);
+  k_stage_seq<<<grid_for(n), 256, 0, s>>>(src, dst, n, boff, nb, st.V, bV, sb, mirror, swap,
+                                          sc.k0, st.ctl);
+  uint64_t *sorted = sort_keys(sc.k0, sc.k1, m, 2 * bV + sb, sc.tmp, s);
+  uint64_t *other = sorted == sc.k0 ? sc.k1 : sc.k0;
+  k_last_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.segof);
+  scan_u32(sc.segof, sc.rank, m, nullptr, sc.tmp, s);
+  k_last_compact<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.rank, kinds, other, sc.op);
+  return other;
+}
+
 // ---------------------------------------------------------- segmentation --
+// op (may be null): per-key operation of a merged group, OP_DELETE keys are
+// never insert candidates.
 __global__ void k_seg_mark(const uint64_t *keys, uint64_t m, uint32_t bV, uint32_t *head,
-                           uint32_t *flag, Ctl *ctl) {
+                           uint32_t *flag, const uint32_t *op, Ctl *ctl) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t k = keys[i];
@@ -192,7 +276,7 @@ __global__ void k_seg_mark(const uint64_t *keys, uint64_t m, uint32_t bV, uint32
     bool h = valid && (i == 0 || (prev >> bV) != (k >> bV));
     bool dup = valid && i > 0 && prev == k;
     head[i] = h ? 1u : 0u;
-    flag[i] = (valid && !dup) ? 1u : 0u;
+    flag[i] = (valid && !dup && (!op || op[i] == OP_INSERT)) ? 1u : 0u;
     if (valid) {
       uint64_t next = (i + 1 < m) ? keys[i + 1] : KEY_SENT;
       if (next == KEY_SENT) ctl->n_valid = (unsigned)(i + 1);
@@ -241,10 +325,11 @@ __global__ void k_item_expand(const uint32_t *seg_cnt, const uint32_t *item_star
   }
 }
 
-void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s,
+             const uint32_t *op) {
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
-  k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, st.ctl);
+  k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, op, st.ctl);
   scan_u32(sc.segof, sc.segof, m, nullptr, sc.tmp, s);
   SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
   k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.ctl);
@@ -256,15 +341,40 @@ void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream
 struct ProbeArgs {
   const uint64_t *keys;
   const uint32_t *item_seg, *item_start, *seg_src, *seg_used, *seg_start;
-  uint32_t *flag;     // insert: survivor flags
-  uint32_t *seg_del;  // delete: per-segment tombstone counts
+  uint32_t *flag;       // insert: survivor flags
+  uint32_t *seg_del;    // delete: per-segment tombstone counts
+  const uint32_t *op;   // mixed: per-key operation
 };
 
-template <bool DEL>
+// Probe modes: every matching slot is an existing edge; what happens to it
+// depends on the batch kind (or, in a merged group, on the key's last op).
+enum : int { PROBE_INSERT = 0, PROBE_DELETE = 1, PROBE_MIXED = 2 };
+
+// Act on a live slot that matches batch key p.
+template <int MODE>
+__device__ __forceinline__ void probe_hit(const Store &st, const ProbeArgs &a, uint32_t idx,
+                                          uint32_t p, uint32_t &cnt) {
+  bool del = MODE == PROBE_DELETE || (MODE == PROBE_MIXED && a.op[p] == OP_DELETE);
+  if (del) {
+    st.slots[idx] = TOMB;
+    ++cnt;
+  } else {
+    a.flag[p] = 0;
+  }
+}
+
+constexpr int PROBE_ROUNDS = PROBE_CHUNK / WAVE;
+
+// One wavefront per work item (PROBE_CHUNK slots of one source).  All eight
+// 64-slot rounds are loaded before any test so the slot reads overlap; the
+// source's sorted batch segment is held in registers (one key per lane) when
+// it has at most 64 keys, otherwise searched in memory.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t bV, Ctl *ctl) {
-  if (DEL && ctl->abort) return;
+  if (MODE != PROBE_INSERT && ctl->abort) return;
   const uint32_t nitems = ctl->n_items;
   const uint32_t lane = lane_id();
+  const uint64_t mask = (1ull << bV) - 1;
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < nitems;
        it += nwaves) {
@@ -274,33 +384,39 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
     const uint32_t uo = a.seg_used[sidx];
     const uint32_t tbl = st.tbl_off[u];
     const uint32_t ks = a.seg_start[sidx], ke = a.seg_start[sidx + 1];
-    const uint64_t ukey = (uint64_t)u << bV;
-    uint32_t cnt = 0;
-#pragma unroll 2
-    for (uint32_t r = 0; r < PROBE_CHUNK / WAVE; ++r) {
+    const uint32_t k = ke - ks;
+    uint32_t idx[PROBE_ROUNDS], w[PROBE_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < PROBE_ROUNDS; ++r) {
       uint32_t pos = c * PROBE_CHUNK + r * WAVE + lane;
-      if (pos < uo) {
-        uint32_t idx = slot_index(st.arena, tbl, pos, st.log2B, st.B);
-        uint32_t w = st.slots[idx];
-        if (w != TOMB) {
-          uint64_t key = ukey | w;
-          uint32_t lo = ks, hi = ke;
-          while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (a.keys[mid] < key) lo = mid + 1; else hi = mid;
-          }
-          if (lo < ke && a.keys[lo] == key) {
-            if (DEL) {
-              st.slots[idx] = TOMB;
-              ++cnt;
-            } else {
-              a.flag[lo] = 0;
-            }
-          }
+      idx[r] = pos < uo ? slot_index(st.arena, tbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int r = 0; r < PROBE_ROUNDS; ++r) w[r] = idx[r] != 0xFFFFFFFFu ? st.slots[idx[r]] : TOMB;
+    uint32_t cnt = 0;
+    if (k <= (uint32_t)WAVE) {
+      const uint32_t sv = lane < k ? (uint32_t)(a.keys[ks + lane] & mask) : 0xFFFFFFFFu;
+#pragma unroll
+      for (int r = 0; r < PROBE_ROUNDS; ++r) {
+        uint32_t p = wave_lower_bound(sv, w[r]);
+        uint32_t hit = __shfl(sv, (int)p, WAVE);
+        if (w[r] != TOMB && hit == w[r]) probe_hit<MODE>(st, a, idx[r], ks + p, cnt);
+      }
+    } else {
+      const uint64_t ukey = (uint64_t)u << bV;
+#pragma unroll
+      for (int r = 0; r < PROBE_ROUNDS; ++r) {
+        if (w[r] == TOMB) continue;
+        uint64_t key = ukey | w[r];
+        uint32_t lo = ks, hi = ke;
+        while (lo < hi) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (a.keys[mid] < key) lo = mid + 1; else hi = mid;
         }
+        if (lo < ke && a.keys[lo] == key) probe_hit<MODE>(st, a, idx[r], lo, cnt);
       }
     }
-    if (DEL) {
+    if (MODE != PROBE_INSERT) {
       cnt = wave_sum(cnt);
       if (lane == 0 && cnt) atomicAdd(&a.seg_del[sidx], cnt);
     }
@@ -315,35 +431,27 @@ static unsigned probe_grid(uint64_t items_bound) {
 // ---------------------------------------------------------------- insert --
 struct PlanArgs {
   const uint32_t *seg_src, *seg_used, *seg_start, *rank;
-  uint32_t *seg_blk, *seg_tbl;
+  uint64_t *plan;
 };
 
 __device__ __forceinline__ uint32_t pow2ceil(uint32_t x) {
   return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
 }
 
+// Per segment: new blocks (high 32 bits) and new table capacity (low 32 bits)
+// packed in one word, so a single exclusive scan gives every segment its
+// block base and table base with no atomics.
 __global__ void k_ins_plan(Store st, PlanArgs a, Ctl *ctl) {
   const uint32_t T = ctl->n_segs;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t sidx = base + threadIdx.x;
-    bool act = sidx < T;
-    uint32_t nblk = 0, ncap = 0;
-    if (act) {
-      uint32_t u = a.seg_src[sidx];
-      uint32_t uo = a.seg_used[sidx];
-      uint32_t surv = a.rank[a.seg_start[sidx + 1]] - a.rank[a.seg_start[sidx]];
-      uint32_t nbo = (uo + st.B - 1) >> st.log2B;
-      uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
-      nblk = nbn - nbo;
-      if (nbn > st.tbl_cap[u]) ncap = pow2ceil(nbn);
-    }
-    unsigned long long rb = wave_atomic_add(&ctl->need_blocks, nblk);
-    unsigned long long ra = wave_atomic_add(&ctl->need_arena, ncap);
-    if (act) {
-      a.seg_blk[sidx] = (uint32_t)rb;
-      a.seg_tbl[sidx] = ncap ? (uint32_t)ra : 0xFFFFFFFFu;
-    }
+  for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx < T;
+       sidx += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = a.seg_src[sidx];
+    uint32_t uo = a.seg_used[sidx];
+    uint32_t surv = a.rank[a.seg_start[sidx + 1]] - a.rank[a.seg_start[sidx]];
+    uint32_t nbo = (uo + st.B - 1) >> st.log2B;
+    uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
+    uint32_t ncap = nbn > st.tbl_cap[u] ? pow2ceil(nbn) : 0u;
+    a.plan[sidx] = ((uint64_t)(nbn - nbo) << 32) | ncap;
   }
 }
 
@@ -358,9 +466,19 @@ __device__ void commit_take(Ctl *c) {
 }
 
 // All-or-nothing: either both stores get their blocks or neither does.
-// rank_a / rank_b hold the survivor prefix of each store's staged batch.
+// rank_* hold each store's survivor prefix, plan_* the scanned plan words.
 __global__ void k_commit(Ctl *a, uint64_t Aa, Ctl *b, uint64_t Ab, const uint32_t *rank_a,
-                         const uint32_t *rank_b) {
+                         const uint32_t *rank_b, const uint64_t *plan_a, const uint64_t *plan_b) {
+  {
+    uint64_t pa = plan_a[a->n_segs];
+    a->need_blocks = pa >> 32;
+    a->need_arena = pa & 0xFFFFFFFFull;
+    if (b) {
+      uint64_t pb = plan_b[b->n_segs];
+      b->need_blocks = pb >> 32;
+      b->need_arena = pb & 0xFFFFFFFFull;
+    }
+  }
   unsigned ab = a->abort | (b ? b->abort : 0u);
   if (ab) {
     a->abort = a->abort ? a->abort : ab;
@@ -375,18 +493,19 @@ __global__ void k_commit(Ctl *a, uint64_t Aa, Ctl *b, uint64_t Ab, const uint32_
   }
   commit_take(a);
   unsigned long long ap = rank_a[a->n_valid];
-  a->applied = ap;
+  a->applied += ap;
   a->live_edges += ap;
   if (b) {
     commit_take(b);
     unsigned long long bp = rank_b[b->n_valid];
-    b->applied = bp;
+    b->applied += bp;
     b->live_edges += bp;
   }
 }
 
 struct TblArgs {
-  const uint32_t *seg_src, *seg_used, *seg_start, *rank, *seg_blk, *seg_tbl;
+  const uint32_t *seg_src, *seg_used, *seg_start, *rank;
+  const uint64_t *plan;
 };
 
 __global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
@@ -402,15 +521,15 @@ __global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
     uint32_t nbo = (uo + st.B - 1) >> st.log2B;
     uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
     uint32_t tbl = st.tbl_off[u];
-    uint32_t rt = a.seg_tbl[sidx];
-    if (rt != 0xFFFFFFFFu) {
-      uint32_t noff = (uint32_t)(ab0 + rt);
+    const uint64_t pl = a.plan[sidx];
+    if (nbn > st.tbl_cap[u]) {
+      uint32_t noff = (uint32_t)(ab0 + (pl & 0xFFFFFFFFull));
       for (uint32_t j = 0; j < nbo; ++j) st.arena[noff + j] = st.arena[tbl + j];
       tbl = noff;
       st.tbl_off[u] = noff;
       st.tbl_cap[u] = pow2ceil(nbn);
     }
-    uint64_t bb = bb0 + a.seg_blk[sidx];
+    uint64_t bb = bb0 + (pl >> 32);
     for (uint32_t j = nbo; j < nbn; ++j) st.arena[tbl + j] = st.free_ids[bb + (j - nbo)];
     st.used[u] = uo + surv;
     st.deg[u] += surv;
@@ -441,22 +560,28 @@ void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipSt
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, sc.flag,
-               nullptr};
-  k_probe<false><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+               nullptr, nullptr};
+  k_probe<PROBE_INSERT><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+  insert_alloc(st, sc, m, s);
+}
+
+void insert_alloc(Store &st, Scratch &sc, uint64_t m, hipStream_t s) {
+  if (!m) return;
   scan_u32(sc.flag, sc.rank, m, &st.ctl->n_valid, sc.tmp, s);
-  PlanArgs pl{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.seg_blk, sc.seg_tbl};
+  PlanArgs pl{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.plan};
   k_ins_plan<<<grid_for(m), 256, 0, s>>>(st, pl, st.ctl);
+  scan_u64(sc.plan, sc.plan, m, &st.ctl->n_segs, sc.tmp, s);
 }
 
 void commit_pair(Store &a, Scratch &sa, Store *b, Scratch *sb, hipStream_t s) {
   k_commit<<<1, 1, 0, s>>>(a.ctl, a.A, b ? b->ctl : nullptr, b ? b->A : 0, sa.rank,
-                           sb ? sb->rank : nullptr);
+                           sb ? sb->rank : nullptr, sa.plan, sb ? sb->plan : nullptr);
 }
 
 void insert_write(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
-  TblArgs ta{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.seg_blk, sc.seg_tbl};
+  TblArgs ta{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.plan};
   k_ins_tables<<<grid_for(m), 256, 0, s>>>(st, ta, st.ctl);
   WriteArgs wa{keys, sc.flag, sc.segof, sc.rank, sc.seg_start, sc.seg_src, sc.seg_used};
   k_ins_write<<<grid_for(m), 256, 0, s>>>(st, wa, bV, st.ctl);
@@ -467,19 +592,18 @@ __global__ void k_del_finish(Store st, const uint32_t *seg_src, const uint32_t *
                              Ctl *ctl) {
   if (ctl->abort) return;
   const uint32_t T = ctl->n_segs;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t sidx = base + threadIdx.x;
-    uint32_t d = 0;
-    if (sidx < T) {
-      d = seg_del[sidx];
-      if (d) st.deg[seg_src[sidx]] -= d;
-    }
-    uint32_t tot = wave_sum(d);
-    if (lane_id() == 0 && tot) {
-      atomicAdd(&ctl->applied, (unsigned long long)tot);
-      atomicAdd(&ctl->live_edges, (unsigned long long)(0ull - (unsigned long long)tot));
-    }
+  unsigned long long mine = 0;
+  for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx < T;
+       sidx += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t d = seg_del[sidx];
+    if (d) st.deg[seg_src[sidx]] -= d;
+    mine += d;
+  }
+  // one atomic pair per block (the grid is capped at a few hundred blocks)
+  unsigned long long tot = block_sum<unsigned long long, 256>(mine);
+  if (threadIdx.x == 0 && tot) {
+    atomicAdd(&ctl->applied, tot);
+    atomicAdd(&ctl->live_edges, 0ull - tot);
   }
 }
 
@@ -487,24 +611,53 @@ void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipS
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, nullptr,
-               sc.seg_blk};
-  k_probe<true><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
-  k_del_finish<<<grid_for(m), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+               sc.seg_blk, nullptr};
+  k_probe<PROBE_DELETE><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+  k_del_finish<<<grid_for(m, 256, 256), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+}
+
+// Merged group: one probe pass tombstones keys whose last op is a delete and
+// marks present keys whose last op is an insert; then the insert machinery
+// places the remaining inserts.
+void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  const uint32_t bV = id_bits(st.V);
+  ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, sc.flag,
+               sc.seg_blk, sc.op};
+  k_probe<PROBE_MIXED><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+  k_del_finish<<<grid_for(m, 256, 256), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+  insert_alloc(st, sc, m, s);
 }
 
 // ------------------------------------------------------------ compaction --
-__global__ void k_cmp_select(Store st, float frac, uint32_t *vlist, Ctl *ctl) {
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < st.V;
-       base += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t v = base + threadIdx.x;
-    bool sel = false;
-    if (v < st.V) {
-      uint32_t us = st.used[v], d = st.deg[v];
-      uint32_t tomb = us - d;
-      sel = tomb > 0 && (frac <= 0.f || (float)tomb > frac * (float)us);
-    }
-    unsigned long long pos = wave_atomic_add(&ctl->compacted, sel ? 1u : 0u);
-    if (sel) vlist[pos] = (uint32_t)v;
+__device__ __forceinline__ bool cmp_wanted(const Store &st, uint32_t v, float frac) {
+  uint32_t us = st.used[v], d = st.deg[v];
+  uint32_t tomb = us - d;
+  return tomb > 0 && (frac <= 0.f || (float)tomb > frac * (float)us);
+}
+
+// Each block owns a contiguous vertex range: it counts its selections, takes
+// one base with a single atomic, then writes them in vertex order.
+__global__ __launch_bounds__(256) void k_cmp_select(Store st, float frac, uint32_t *vlist,
+                                                    Ctl *ctl) {
+  const uint64_t chunk = cdiv_dev(st.V, gridDim.x);
+  const uint64_t lo = blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < st.V ? lo + chunk : st.V;
+  uint32_t mine = 0;
+  for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) mine += cmp_wanted(st, (uint32_t)v, frac);
+  uint32_t tot = block_sum<uint32_t, 256>(mine);
+  __shared__ unsigned long long base;
+  if (threadIdx.x == 0) base = tot ? atomicAdd(&ctl->compacted, (unsigned long long)tot) : 0ull;
+  __syncthreads();
+  if (!tot) return;
+  unsigned long long b = base;
+  for (uint64_t t0 = lo; t0 < hi; t0 += blockDim.x) {
+    uint64_t v = t0 + threadIdx.x;
+    uint32_t sel = v < hi ? cmp_wanted(st, (uint32_t)v, frac) : 0u;
+    uint32_t ttot;
+    uint32_t ex = block_excl_scan<uint32_t, 256>(sel, ttot);
+    if (sel) vlist[b + ex] = (uint32_t)v;
+    b += ttot;
   }
 }
 
@@ -549,7 +702,7 @@ __global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist
 }
 
 void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s) {
-  k_cmp_select<<<grid_for(st.V), 256, 0, s>>>(st, frac, vlist, st.ctl);
+  k_cmp_select<<<grid_for(st.V, 256, 512), 256, 0, s>>>(st, frac, vlist, st.ctl);
   k_cmp_run<<<probe_grid(st.V), 256, 0, s>>>(st, vlist, st.ctl);
 }
 

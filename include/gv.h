@@ -87,6 +87,20 @@ gv_status gv_batch_insert(gv_graph *g, const uint32_t *src, const uint32_t *dst,
 /* GraphVine::batchDelete: N(u) \= {v} (tombstones); missing edges are no-ops. */
 gv_status gv_batch_delete(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
                           int mem, gv_batch_stats *stats);
+/* Batched form: a group of update batches in one pipeline.  Batch b is
+ * entries [offsets[b], offsets[b+1]) of src/dst (offsets: host array of
+ * n_batches+1, offsets[0] = 0) with kind kinds[b] (GV_OP_INSERT or
+ * GV_OP_DELETE, host array).  The resulting graph equals applying the batches
+ * one after another (an edge's final membership is decided by its last
+ * operation in the group); stats cover the whole group (applied = slots
+ * written + slots tombstoned).  compact_fraction >= 0 runs gv_compact with that
+ * fraction at the end.  No counterpart in the reference, which applies one
+ * CSR batch per call (graphvine.cu:222-451). */
+#define GV_OP_INSERT 0
+#define GV_OP_DELETE 1
+gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
+                         const uint64_t *offsets, const int *kinds, uint32_t n_batches, int mem,
+                         float compact_fraction, gv_batch_stats *stats);
 /* GraphVine::compaction: pack every vertex whose tombstones exceed
  * tombstone_fraction of its slots (0 = any tombstone, < 0 = the configured
  * default) and return emptied blocks to the pool. */
