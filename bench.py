@@ -16,6 +16,7 @@ batches (updates = undirected edge updates requested).
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -44,6 +45,7 @@ def parse():
     p.add_argument("--no-tc", action="store_true")
     p.add_argument("--cpu-baseline", type=int, default=1, help="1 = time the CPU baseline")
     p.add_argument("--cpu-batches", type=int, default=2)
+    p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
     return p.parse_args()
 
 
@@ -201,7 +203,64 @@ def main():
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
     g.close()
+    if args.batched:
+        out["batched"] = run_batched(args, G, skey, V, n_raw, batches, E_end, value)
     print(json.dumps(out), flush=True)
+
+
+def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):
+    """Batched form: the same K timed batches (each a delete half then an
+    insert half) applied as one group by gv_batch_apply, followed by one
+    compaction at 25% tombstones, on a fresh graph brought to the same state
+    as the plain run's starting point."""
+    from gv_amd.hip import D2D, DeviceArray, Event, hip, synchronize
+    g2 = G.GraphVine(V, directed=False)
+    src, dst = G.gen_rmat(g2, skey(args.seed, 1), args.scale, n_raw)
+    g2.bulkBuild(src, dst)
+    src.free()
+    dst.free()
+    for b in range(args.warmup):
+        ds, dd, is_, id_ = batches[b]
+        g2.batchDelete(ds, dd, stats=False)
+        g2.batchInsert(is_, id_, stats=False)
+        G.lib().gv_compact(g2.handle, 0.25, None)
+    half = args.batch // 2
+    parts = []
+    for k in range(args.steps):
+        ds, dd, is_, id_ = batches[args.warmup + k]
+        parts += [(ds, dd, G.GV_OP_DELETE), (is_, id_, G.GV_OP_INSERT)]
+    n = half * len(parts)
+    cs, cd = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
+    offsets = [0]
+    for i, (s, d, _) in enumerate(parts):
+        o = i * half * 4
+        hip().hipMemcpy(ctypes.c_void_p(cs.ptr.value + o), s.ptr, half * 4, D2D)
+        hip().hipMemcpy(ctypes.c_void_p(cd.ptr.value + o), d.ptr, half * 4, D2D)
+        offsets.append(offsets[-1] + half)
+    kinds = [p[2] for p in parts]
+    g2.sync()
+    synchronize()
+    stream = g2.stream()
+    ev0, ev1 = Event(), Event()
+    ev0.record(stream)
+    g2.applyBatches(cs, cd, offsets, kinds, compact_fraction=0.25, stats=False)
+    ev1.record(stream)
+    ev1.synchronize()
+    g2.sync()
+    ms = ev0.elapsed_ms(ev1)
+    E2 = g2.num_edges()
+    g2.close()
+    val = args.batch * args.steps / (ms / 1e3)
+    return {
+        "value": val,
+        "unit": "edge updates/s",
+        "ms_total": ms,
+        "batches_in_group": len(parts),
+        "speedup_vs_plain": val / plain_value,
+        "final_edges_equal_plain": bool(E2 == E_plain_end),
+        "form": "one gv_batch_apply over the same timed batches (delete half, insert half "
+                "each) + one compaction at 25% tombstones",
+    }
 
 
 def cpu_baseline(g, V, args, batches):

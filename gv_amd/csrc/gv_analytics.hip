@@ -39,7 +39,6 @@ __device__ __forceinline__ void two_sum_merge(float &s, float &c, float s2, floa
 // =============================================================== PageRank ==
 constexpr int PR_THREADS = 256;
 constexpr int PR_GROUP = 8;
-constexpr uint32_t PR_GRID = 2048;  // persistent grid: 256 CUs x 8 blocks
 
 gv_status pr_reserve(PrWork &w, uint32_t V) {
   if (w.V >= V && w.ctl) return GV_OK;
@@ -239,99 +238,98 @@ __device__ __forceinline__ double pr_finish(const PrArgs &a, uint32_t v, float s
   return fabs((double)xv - (double)x[v]);
 }
 
+// One work unit per workgroup, heaviest first: hub chunks, one-wavefront
+// vertices, eight-lane vertices, single-lane vertices.  The hardware dispatcher
+// balances the units over the CUs; each workgroup writes its residual partial
+// to its own slot, so no counter is shared between workgroups.
 __global__ __launch_bounds__(PR_THREADS) void k_pr_iter(PrArgs a, const float *x, const float *cb,
                                                         float *xn, float *cn) {
   if (a.ctl->done) return;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wid = tid / WAVE;
+  const uint32_t u = blockIdx.x;
   double diff = 0.0;
-  for (uint32_t u = blockIdx.x; u < a.U; u += gridDim.x) {
-    if (u < a.U0) {
-      uint32_t i = u * PR_THREADS + tid;
-      if (i < a.n0) {
-        uint32_t v = a.list[i];
-        uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
-        float s = 0.f, c = 0.f;
-        pr_gather<1>(a, cb, tbl, 0, us, s, c);
-        diff += pr_finish(a, v, s, c, x, xn, cn);
-      }
-    } else if (u < a.U0 + a.U1) {
-      uint32_t g = (u - a.U0) * (PR_THREADS / PR_GROUP) + tid / PR_GROUP;
-      uint32_t gl = tid & (PR_GROUP - 1);
-      uint32_t v = 0, us = 0, tbl = 0;
-      if (g < a.n1) {
-        v = a.list[a.o1 + g];
-        us = a.in.used[v];
-        tbl = a.in.tbl_off[v];
-      }
+  if (u < a.U3) {
+    uint32_t ci = u * (PR_THREADS / WAVE) + wid;
+    if (ci < a.n_chunks) {
+      uint32_t h = a.chunk_hub[ci];
+      uint32_t v = a.list[a.o3 + h];
+      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      uint32_t hb = a.hub_base[h];
+      uint32_t c0 = (ci - hb) * a.wave_max;
+      uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
       float s = 0.f, c = 0.f;
-      pr_gather<PR_GROUP>(a, cb, tbl, gl, us, s, c);
-      group_merge<PR_GROUP>(s, c);
-      if (g < a.n1 && gl == 0) diff += pr_finish(a, v, s, c, x, xn, cn);
-    } else if (u < a.U0 + a.U1 + a.U2) {
-      uint32_t g = (u - a.U0 - a.U1) * (PR_THREADS / WAVE) + wid;
-      if (g < a.n2) {
-        uint32_t v = a.list[a.o2 + g];
-        uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
-        float s = 0.f, c = 0.f;
-        pr_gather<WAVE>(a, cb, tbl, lane, us, s, c);
-        group_merge<WAVE>(s, c);
-        if (lane == 0) diff += pr_finish(a, v, s, c, x, xn, cn);
-      }
-    } else {
-      uint32_t ci = (u - a.U0 - a.U1 - a.U2) * (PR_THREADS / WAVE) + wid;
-      if (ci < a.n_chunks) {
-        uint32_t h = a.chunk_hub[ci];
-        uint32_t v = a.list[a.o3 + h];
-        uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
-        uint32_t hb = a.hub_base[h];
-        uint32_t c0 = (ci - hb) * a.wave_max;
-        uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
-        float s = 0.f, c = 0.f;
-        pr_gather<WAVE>(a, cb, tbl, c0 + lane, c1, s, c);
-        group_merge<WAVE>(s, c);
-        if (lane == 0) {
-          a.chunk_part[ci] = make_float2(s, c);
+      pr_gather<WAVE>(a, cb, tbl, c0 + lane, c1, s, c);
+      group_merge<WAVE>(s, c);
+      if (lane == 0) {
+        a.chunk_part[ci] = make_float2(s, c);
+        __threadfence();
+        uint32_t nch = (us + a.wave_max - 1) / a.wave_max;
+        uint32_t old = atomicAdd(&a.hub_cnt[h], 1u);
+        if (old == nch - 1) {
           __threadfence();
-          uint32_t nch = (us + a.wave_max - 1) / a.wave_max;
-          uint32_t old = atomicAdd(&a.hub_cnt[h], 1u);
-          if (old == nch - 1) {
-            __threadfence();
-            float ts = 0.f, tc = 0.f;
-            for (uint32_t k = 0; k < nch; ++k) {
-              float2 p = a.chunk_part[hb + k];
-              two_sum_merge(ts, tc, p.x, p.y);
-            }
-            a.partials[gridDim.x + h] = pr_finish(a, v, ts, tc, x, xn, cn);
-            a.hub_cnt[h] = 0;
+          float ts = 0.f, tc = 0.f;
+          for (uint32_t k = 0; k < nch; ++k) {
+            float2 p = a.chunk_part[hb + k];
+            two_sum_merge(ts, tc, p.x, p.y);
           }
+          a.partials[a.U + h] = pr_finish(a, v, ts, tc, x, xn, cn);
+          a.hub_cnt[h] = 0;
         }
       }
     }
+  } else if (u < a.U3 + a.U2) {
+    uint32_t g = (u - a.U3) * (PR_THREADS / WAVE) + wid;
+    if (g < a.n2) {
+      uint32_t v = a.list[a.o2 + g];
+      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      float s = 0.f, c = 0.f;
+      pr_gather<WAVE>(a, cb, tbl, lane, us, s, c);
+      group_merge<WAVE>(s, c);
+      if (lane == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
+    }
+  } else if (u < a.U3 + a.U2 + a.U1) {
+    uint32_t g = (u - a.U3 - a.U2) * (PR_THREADS / PR_GROUP) + tid / PR_GROUP;
+    uint32_t gl = tid & (PR_GROUP - 1);
+    uint32_t v = 0, us = 0, tbl = 0;
+    if (g < a.n1) {
+      v = a.list[a.o1 + g];
+      us = a.in.used[v];
+      tbl = a.in.tbl_off[v];
+    }
+    float s = 0.f, c = 0.f;
+    pr_gather<PR_GROUP>(a, cb, tbl, gl, us, s, c);
+    group_merge<PR_GROUP>(s, c);
+    if (g < a.n1 && gl == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
+  } else {
+    uint32_t i = (u - a.U3 - a.U2 - a.U1) * PR_THREADS + tid;
+    if (i < a.n0) {
+      uint32_t v = a.list[i];
+      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      float s = 0.f, c = 0.f;
+      pr_gather<1>(a, cb, tbl, 0, us, s, c);
+      diff = pr_finish(a, v, s, c, x, xn, cn);
+    }
   }
   double bs = block_sum<double, PR_THREADS>(diff);
-  __shared__ bool last;
-  if (tid == 0) {
-    a.partials[blockIdx.x] = bs;
-    __threadfence();
-    unsigned t = atomicAdd(&a.ctl->blocks_done, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // Fixed-order reduction of all block and hub partials.
-  const uint32_t np = gridDim.x + a.n3;
+  if (tid == 0) a.partials[u] = bs;
+}
+
+// Fixed-order FP64 reduction of the workgroup and hub partials, then the
+// stopping rule of static_pagerank (pagerank.cu:56): done after the first
+// update whose residual is below eps, or after max_iter updates.
+__global__ __launch_bounds__(1024) void k_pr_resid(PrArgs a) {
+  if (a.ctl->done) return;
+  const uint32_t np = a.U + a.n3;
   double r = 0.0;
-  for (uint32_t i = tid; i < np; i += PR_THREADS) r += a.partials[i];
-  r = block_sum<double, PR_THREADS>(r);
-  if (tid == 0) {
+  for (uint32_t i = threadIdx.x; i < np; i += 1024) r += a.partials[i];
+  r = block_sum<double, 1024>(r);
+  if (threadIdx.x == 0) {
     a.ctl->residual = r;
     unsigned it = a.ctl->iters + 1;
     a.ctl->iters = it;
     if (r < a.eps || it >= a.max_iter) a.ctl->done = 1;
-    a.ctl->blocks_done = 0;
   }
 }
 
@@ -393,7 +391,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   a.n_chunks = nchunks;
   a.wave_max = bins.wave_max;
   a.max_iter = (uint32_t)max_iter;
-  const uint32_t grid = a.U < PR_GRID ? (a.U ? a.U : 1) : PR_GRID;
+  const uint32_t grid = a.U ? a.U : 1;
   if (n3)
     k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(in.used, w.list, a.o3, n3, bins.wave_max, w.hub_base,
                                            w.hub_cnt, w.chunk_hub, &w.ctl->chunk_cursor);
@@ -416,7 +414,8 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     int t = launched;
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
     k_pr_iter<<<grid, PR_THREADS, 0, s>>>(a, w.x[t & 1], w.contrib[t & 1], w.x[(t + 1) & 1],
-                                               w.contrib[(t + 1) & 1]);
+                                          w.contrib[(t + 1) & 1]);
+    k_pr_resid<<<1, 1024, 0, s>>>(a);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
     ++launched;
     if (launched % check_every == 0 && launched < max_iter) {
