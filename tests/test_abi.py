@@ -42,3 +42,28 @@ def test_struct_layout_matches_header():
     # gv_config: u32, (pad), u64, f64, f32, 3 x u32, 2 x int -> 48 bytes
     assert ctypes.sizeof(gv_amd.GvConfig) == 48
     assert ctypes.sizeof(gv_amd.GvBatchStats) == 48
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        import pytest
+        pytest.skip("g++ not available")
+    src = tmp_path / "use_gv.cpp"
+    src.write_text(
+        '#include "gv.hpp"\n'
+        "int main() {\n"
+        "  std::vector<uint32_t> s{0, 1}, d{1, 2};\n"
+        "  gv::GraphVine g(3, s, d);\n"
+        "  std::vector<float> x;\n"
+        "  gv::static_pagerank(&g, 0.85f, 1e-6f, 10, x);\n"
+        "  std::vector<uint64_t> t;\n"
+        "  gv::static_tc(&g, t);\n"
+        "  return (int)gv::bc(&g, {0}).size() == 3 ? 0 : 1;\n"
+        "}\n")
+    lib_dir = os.path.dirname(gv_amd.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", lib_dir, "-lgv", "-o", str(tmp_path / "use_gv")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
