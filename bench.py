@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--cpu-baseline", type=int, default=1, help="1 = time the CPU baseline")
     p.add_argument("--cpu-batches", type=int, default=2)
     p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
+    p.add_argument("--pr-layout", type=int, default=1,
+                   help="PageRank layout: 1 = pull over the store, 0 = XCD-segmented snapshot")
     return p.parse_args()
 
 
@@ -75,7 +77,7 @@ def main():
     n_raw = args.edge_factor * V
     half = args.batch // 2
     t_setup = time.time()
-    g = G.GraphVine(V, directed=False)
+    g = G.GraphVine(V, directed=False, pr_layout=args.pr_layout)
     stream = g.stream()
     src, dst = G.gen_rmat(g, skey(args.seed, STREAM_RMAT), args.scale, n_raw)
     st = g.bulkBuild(src, dst)

@@ -56,6 +56,7 @@ class GvConfig(ctypes.Structure):
         ("pr_wave_max", ctypes.c_uint32),
         ("maintain_in_edges", ctypes.c_int),
         ("device", ctypes.c_int),
+        ("pr_layout", ctypes.c_int),
     ]
 
 

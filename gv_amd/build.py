@@ -18,7 +18,8 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgv.so")
 
-SOURCES = ["gv_prims.hip", "gv_store.hip", "gv_analytics.hip", "gv_api.hip"]
+SOURCES = ["gv_prims.hip", "gv_store.hip", "gv_analytics.hip", "gv_pagerank_seg.hip",
+           "gv_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GV_OFFLOAD_ARCH", "gfx950")

@@ -39,6 +39,7 @@ __device__ __forceinline__ void two_sum_merge(float &s, float &c, float s2, floa
 // =============================================================== PageRank ==
 constexpr int PR_THREADS = 256;
 constexpr int PR_GROUP = 8;
+constexpr int PR_ILP = 8;  // independent slot reads / gathers in flight per lane
 
 gv_status pr_reserve(PrWork &w, uint32_t V) {
   if (w.V >= V && w.ctl) return GV_OK;
@@ -162,6 +163,15 @@ __global__ __launch_bounds__(256) void k_pr_place(const uint32_t *used, uint32_t
   }
 }
 
+void pr_ctl_reset(PrCtl *ctl, hipStream_t s) { k_pr_reset<<<1, 1, 0, s>>>(ctl); }
+void pr_bins_count(const uint32_t *deg, uint32_t V, const PrBins &b, PrCtl *ctl, hipStream_t s) {
+  k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(deg, V, b, ctl);
+}
+void pr_bins_place(const uint32_t *deg, uint32_t V, const PrBins &b, uint32_t *list, PrCtl *ctl,
+                   hipStream_t s) {
+  k_pr_place<<<grid_for(V, 256, 512), 256, 0, s>>>(deg, V, b, list, ctl);
+}
+
 // Chunk lists of the vertices in the chunked bin.  Chunk c of hub h lives at
 // hub_base[h] + c, so partials combine in chunk order whatever the schedule.
 __global__ void k_pr_hubs(const uint32_t *used, const uint32_t *list, uint32_t o3, uint32_t n3,
@@ -199,23 +209,23 @@ struct PrArgs {
 };
 
 // Kahan-accumulate contributions of slots start, start+STRIDE, ... < end.
-// Four slot reads are issued before their four gathers, so each lane keeps
-// several independent loads in flight.
+// PR_ILP slot reads are issued before their PR_ILP gathers, so each lane
+// keeps several independent loads in flight.
 template <int STRIDE>
 __device__ __forceinline__ void pr_gather(const PrArgs &a, const float *cb, uint32_t tbl,
                                           uint32_t start, uint32_t end, float &s, float &c) {
-  for (uint32_t j = start; j < end; j += 4 * STRIDE) {
-    uint32_t w[4];
+  for (uint32_t j = start; j < end; j += PR_ILP * STRIDE) {
+    uint32_t w[PR_ILP];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < PR_ILP; ++k) {
       uint32_t pos = j + k * STRIDE;
       w[k] = pos < end ? a.in.slots[slot_index(a.in.arena, tbl, pos, a.in.log2B, a.in.B)] : TOMB;
     }
-    float g[4];
+    float g[PR_ILP];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) g[k] = w[k] != TOMB ? cb[w[k]] : 0.f;
+    for (int k = 0; k < PR_ILP; ++k) g[k] = w[k] != TOMB ? cb[w[k]] : 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < PR_ILP; ++k)
       if (w[k] != TOMB) kahan_add(s, c, g[k]);
   }
 }

@@ -31,6 +31,7 @@ struct gv_graph {
   Scratch sc[2];
   uint32_t *vlist = nullptr;
   PrWork pr;
+  PsWork ps;
   Ctl *h_ctl = nullptr;  // pinned, 2 entries
   uint32_t *in_src = nullptr, *in_dst = nullptr;
   uint64_t in_cap = 0;
@@ -55,6 +56,7 @@ gv_status gv_config_default(gv_config *cfg) {
   cfg->pr_wave_max = 4096;
   cfg->maintain_in_edges = 1;
   cfg->device = 0;
+  cfg->pr_layout = 1;
   return GV_OK;
 }
 
@@ -361,6 +363,7 @@ gv_status gv_destroy(gv_graph *g) {
     scratch_release(g->sc[k]);
   }
   pr_release(g->pr);
+  ps_release(g->ps);
   hipFree(g->vlist);
   hipFree(g->in_src);
   hipFree(g->in_dst);
@@ -616,8 +619,14 @@ static gv_status pagerank_common(gv_graph *g, float alpha, double eps, int max_i
     hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
     if (e != hipSuccess) return hip_fail(e, "pagerank alloc");
   }
-  rc = pagerank_run(in, g->st[0].deg, bins, alpha, eps, max_iter, dout, iters, residual, g->pr,
-                    g->stream, iter_ms);
+  if (g->cfg.pr_layout == 0) {
+    PrBins sbins{256, 256, 16384};
+    rc = pagerank_seg_run(in, g->st[0].deg, sbins, alpha, eps, max_iter, dout, iters, residual,
+                          g->pr, g->ps, g->stream, iter_ms);
+  } else {
+    rc = pagerank_run(in, g->st[0].deg, bins, alpha, eps, max_iter, dout, iters, residual, g->pr,
+                      g->stream, iter_ms);
+  }
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);

@@ -221,5 +221,8 @@ size_t sort_tmp_bytes(uint64_t n);
 // LSD radix sort of n keys on their low `bits` bits; returns the buffer that
 // holds the result (a or b).
 uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp, hipStream_t s);
+// Same, ordering on bits [lo, hi) only (stable for keys equal there).
+uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
+                          void *tmp, hipStream_t s);
 
 }  // namespace gv

@@ -200,13 +200,18 @@ size_t sort_tmp_bytes(uint64_t n) {
 
 uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp,
                     hipStream_t s) {
+  return sort_keys_range(a, b, n, 0, bits, tmp, s);
+}
+
+uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
+                          void *tmp, hipStream_t s) {
   if (n <= 1) return a;
   uint64_t nt = cdiv(n, SORT_TILE);
   uint64_t h = 256 * nt;
   uint32_t *hist = (uint32_t *)tmp;
   void *stmp = (void *)(((uintptr_t)(hist + h + 1) + 255) & ~(uintptr_t)255);
   uint64_t *src = a, *dst = b;
-  for (uint32_t shift = 0; shift < bits; shift += 8) {
+  for (uint32_t shift = lo; shift < hi; shift += 8) {
     k_radix_hist<<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(src, n, shift, hist, nt);
     scan_u32(hist, hist, h, nullptr, stmp, s);
     k_radix_scatter<<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(src, dst, n, shift, hist, nt);

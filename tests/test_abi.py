@@ -39,8 +39,8 @@ def test_python_binding_loads_and_config_defaults():
 
 
 def test_struct_layout_matches_header():
-    # gv_config: u32, (pad), u64, f64, f32, 3 x u32, 2 x int -> 48 bytes
-    assert ctypes.sizeof(gv_amd.GvConfig) == 48
+    # gv_config: u32, (pad), u64, f64, f32, 3 x u32, 3 x int, (pad) -> 56 bytes
+    assert ctypes.sizeof(gv_amd.GvConfig) == 56
     assert ctypes.sizeof(gv_amd.GvBatchStats) == 48
 
 

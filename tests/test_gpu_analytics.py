@@ -71,11 +71,12 @@ def test_pagerank_closed_forms():
     assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("directed", [True, False])
-def test_pagerank_config1_after_updates(directed):
+def test_pagerank_config1_after_updates(directed, layout):
     V = 1 << 14
     s, d = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 14, 16 * V)
-    g, o = build_pair(V, directed, s, d, pool=6 * V)
+    g, o = build_pair(V, directed, s, d, pool=6 * V, pr_layout=layout)
     check_pagerank(g, o)
     for b in range(3):
         si, di = O.gen_uniform(O.stream_key(1, 1000 + b), V, 4096)
@@ -99,9 +100,27 @@ def test_pagerank_all_degree_bins_and_chunked_hubs():
     s = np.concatenate([s, star[0]])
     d = np.concatenate([d, star[1]])
     g, o = build_pair(V, False, s, d, pool=8 * V, pr_thread_max=2, pr_group_max=16,
-                      pr_wave_max=128)
+                      pr_wave_max=128, pr_layout=1)
     check_pagerank(g, o)
     check_pagerank(g, o, eps=1e-9, max_iter=200, same_stop=False)
+
+
+@pytest.mark.parametrize("V", [5, 9, 20001, 70000])
+def test_pagerank_segmented_snapshot_bins_and_odd_sizes(V):
+    # hub with > 16384 in-edges (one workgroup per segment), vertex counts
+    # that are not powers of two (cycle-walked relabelling), tiny graphs
+    rng = np.random.default_rng(V)
+    s = rng.integers(0, V, 6 * V).astype(np.uint32)
+    d = rng.integers(0, V, 6 * V).astype(np.uint32)
+    s = np.concatenate([s, np.zeros(V - 1, np.uint32)])
+    d = np.concatenate([d, np.arange(1, V, dtype=np.uint32)])
+    for directed in (False, True):
+        g, o = build_pair(V, directed, s, d, pool=16 * V + 4096, pr_layout=0)
+        check_pagerank(g, o)
+        x1, it1, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+        x2, it2, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+        assert it1 == it2 and np.array_equal(x1, x2)  # deterministic
+        g.close()
 
 
 @pytest.mark.parametrize("n", [3, 7, 40])
