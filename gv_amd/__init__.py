@@ -39,6 +39,7 @@ EXPORTED = [
     "gv_config_default", "gv_create", "gv_destroy", "gv_bulk_build", "gv_batch_insert",
     "gv_batch_delete", "gv_batch_apply", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
     "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_pagerank_ex",
+    "gv_pagerank_opts_default", "gv_pagerank_run", "gv_pagerank_personalized",
     "gv_triangle_count",
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_sync", "gv_get_stream",
     "gv_last_error",
@@ -83,6 +84,17 @@ class GvPagerankStats(ctypes.Structure):
     ]
 
 
+class GvPagerankOpts(ctypes.Structure):
+    _fields_ = [
+        ("alpha", ctypes.c_float),
+        ("eps", ctypes.c_double),
+        ("max_iter", ctypes.c_int),
+        ("init", ctypes.c_void_p),
+        ("init_mem", ctypes.c_int),
+        ("compensated", ctypes.c_int),
+    ]
+
+
 class GraphVineError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"gv status {status}: {msg}")
@@ -122,6 +134,11 @@ def lib():
                         ctypes.POINTER(ctypes.c_double)],
         "gv_pagerank_ex": [P, ctypes.c_float, ctypes.c_double, i32, P, i32,
                            ctypes.POINTER(GvPagerankStats)],
+        "gv_pagerank_opts_default": [ctypes.POINTER(GvPagerankOpts)],
+        "gv_pagerank_run": [P, ctypes.POINTER(GvPagerankOpts), P, i32,
+                            ctypes.POINTER(GvPagerankStats)],
+        "gv_pagerank_personalized": [P, ctypes.c_float, ctypes.c_double, i32, P, u32, P, i32,
+                                     ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)],
         "gv_triangle_count": [P, P, i32, ctypes.POINTER(u64)],
         "gv_bc": [P, P, u32, P, i32],
         "gv_gen_rmat": [P, u64, i32, u64, u32, u32, u32, P, P],
@@ -338,6 +355,43 @@ def pagerank_device(graph, out_dev, alpha=0.85, epsilon=1e-6, max_iter=100):
     _check(lib().gv_pagerank_ex(graph.handle, float(alpha), float(epsilon), int(max_iter),
                                 out_dev.ptr, GV_MEM_DEVICE, ctypes.byref(st)))
     return {f: getattr(st, f) for f, _ in st._fields_}
+
+
+def pagerank(graph, alpha=0.85, epsilon=1e-6, max_iter=100, init=None, compensated=True,
+             stats=False):
+    """PageRank with a warm start (init: float32[V] host array) and the choice
+    of compensated or plain FP32 sums.  Returns (x, iters, residual[, stats])."""
+    o = GvPagerankOpts()
+    _check(lib().gv_pagerank_opts_default(ctypes.byref(o)))
+    o.alpha, o.eps, o.max_iter = float(alpha), float(epsilon), int(max_iter)
+    o.compensated = 1 if compensated else 0
+    keep = None
+    if init is not None:
+        keep = np.ascontiguousarray(init, dtype=np.float32)
+        if len(keep) != graph.num_vertices:
+            raise ValueError("init must have num_vertices entries")
+        o.init = keep.ctypes.data_as(ctypes.c_void_p)
+        o.init_mem = GV_MEM_HOST
+    x = np.empty(graph.num_vertices, dtype=np.float32)
+    st = GvPagerankStats()
+    _check(lib().gv_pagerank_run(graph.handle, ctypes.byref(o), x.ctypes.data_as(ctypes.c_void_p),
+                                 GV_MEM_HOST, ctypes.byref(st)))
+    d = {f: getattr(st, f) for f, _ in st._fields_}
+    return (x, d["iters"], d["residual"], d) if stats else (x, d["iters"], d["residual"])
+
+
+def pagerank_personalized(graph, seeds, alpha=0.85, epsilon=1e-6, max_iter=100):
+    """Up to 64 personalized PageRank vectors at once: returns (X[V, n], iters,
+    largest per-column residual)."""
+    sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+    X = np.empty((graph.num_vertices, len(sd)), dtype=np.float32)
+    it = ctypes.c_int()
+    r = ctypes.c_double()
+    _check(lib().gv_pagerank_personalized(graph.handle, float(alpha), float(epsilon),
+                                          int(max_iter), sd.ctypes.data_as(ctypes.c_void_p),
+                                          len(sd), X.ctypes.data_as(ctypes.c_void_p),
+                                          GV_MEM_HOST, ctypes.byref(it), ctypes.byref(r)))
+    return X, int(it.value), float(r.value)
 
 
 # reference: static_tc (tc.cu:3-27)

@@ -36,6 +36,15 @@ struct PrWork {
   std::vector<hipEvent_t> ev;    // per-iteration start/stop events (timed runs)
 };
 
+// Run parameters of one PageRank call.
+struct PrOpts {
+  float alpha = 0.85f;
+  double eps = 1e-6;
+  int max_iter = 100;
+  const float *init = nullptr;  // device vector to start from (warm start), or null = 1/V
+  int compensated = 1;          // per-vertex Kahan sums
+};
+
 struct PrBins {
   uint32_t thread_max, group_max, wave_max;
 };
@@ -64,17 +73,16 @@ void pr_bins_place(const uint32_t *deg, uint32_t V, const PrBins &b, uint32_t *l
 void ps_release(PsWork &w);
 // PageRank on the XCD-segmented snapshot; same contract as pagerank_run.
 gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
-                           float alpha, double eps, int max_iter, float *out_dev, int *iters,
-                           double *residual, PrWork &pw, PsWork &w, hipStream_t s,
-                           double *iter_ms = nullptr);
+                           const PrOpts &o, float *out_dev, int *iters, double *residual,
+                           PrWork &pw, PsWork &w, hipStream_t s, double *iter_ms = nullptr);
 
 gv_status pr_reserve(PrWork &w, uint32_t V);
 void pr_release(PrWork &w);
 // Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
-gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins, float alpha,
-                       double eps, int max_iter, float *out_dev, int *iters, double *residual,
-                       PrWork &w, hipStream_t s, double *iter_ms = nullptr);
+gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
+                       const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
+                       hipStream_t s, double *iter_ms = nullptr);
 
 // Triangle count on an undirected store.  per_vertex_dev may be null.
 gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,
@@ -83,5 +91,15 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
 // Brandes BC from the given device source list; out_dev[V] receives BC.
 gv_status bc_run(const Store &st, const uint32_t *sources_host, uint32_t n_sources,
                  float *out_dev, hipStream_t s);
+
+// Personalized PageRank for ncols <= 64 seeds (device array); out_dev is
+// row-major V x ncols.  max_residual is the largest per-column L1 residual of
+// the last update.
+gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double eps, int max_iter,
+                  const uint32_t *seeds_dev, uint32_t ncols, float *out_dev, int *iters,
+                  double *max_residual, double *iter_ms, hipStream_t s);
+// Brandes BC from host source list, 32 sources per batch.
+gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
+                       float *out_dev, hipStream_t s);
 
 }  // namespace gv

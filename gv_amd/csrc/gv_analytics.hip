@@ -84,12 +84,15 @@ __global__ void k_pr_reset(PrCtl *c) {
   *c = z;
 }
 
-__global__ void k_pr_init(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0) {
+// x0 = 1/V, or a caller's vector (warm start of an incremental refresh).
+__global__ void k_pr_init(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0,
+                          const float *init) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t d = deg_out[v];
-    x[v] = x0;
-    cb[v] = d ? x0 / (float)d : 0.f;
+    float xv = init ? init[v] : x0;
+    x[v] = xv;
+    cb[v] = d ? xv / (float)d : 0.f;
   }
 }
 
@@ -206,6 +209,7 @@ struct PrArgs {
   uint32_t U0, U1, U2, U3, U;
   uint32_t n_chunks, wave_max;
   uint32_t max_iter;
+  uint32_t compensated;  // 1: Kahan sums per lane; 0: plain FP32 sums
 };
 
 // Kahan-accumulate contributions of slots start, start+STRIDE, ... < end.
@@ -224,9 +228,14 @@ __device__ __forceinline__ void pr_gather(const PrArgs &a, const float *cb, uint
     float g[PR_ILP];
 #pragma unroll
     for (int k = 0; k < PR_ILP; ++k) g[k] = w[k] != TOMB ? cb[w[k]] : 0.f;
+    if (a.compensated) {
 #pragma unroll
-    for (int k = 0; k < PR_ILP; ++k)
-      if (w[k] != TOMB) kahan_add(s, c, g[k]);
+      for (int k = 0; k < PR_ILP; ++k)
+        if (w[k] != TOMB) kahan_add(s, c, g[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < PR_ILP; ++k) s += g[k];
+    }
   }
 }
 
@@ -343,9 +352,12 @@ __global__ __launch_bounds__(1024) void k_pr_resid(PrArgs a) {
   }
 }
 
-gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins, float alpha,
-                       double eps, int max_iter, float *out_dev, int *iters, double *residual,
-                       PrWork &w, hipStream_t s, double *iter_ms) {
+gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
+                       const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
+                       hipStream_t s, double *iter_ms) {
+  const float alpha = o.alpha;
+  const double eps = o.eps;
+  const int max_iter = o.max_iter;
   const uint32_t V = in.V;
   if (iters) *iters = 0;
   if (residual) *residual = 0.0;
@@ -354,7 +366,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   if (st) return st;
   const float x0 = 1.0f / (float)V;
   k_pr_reset<<<1, 1, 0, s>>>(w.ctl);
-  k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0);
+  k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0, o.init);
   if (max_iter <= 0) {
     GV_HIP(hipMemcpyAsync(out_dev, w.x[0], (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
     GV_HIP(hipStreamSynchronize(s));
@@ -401,6 +413,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   a.n_chunks = nchunks;
   a.wave_max = bins.wave_max;
   a.max_iter = (uint32_t)max_iter;
+  a.compensated = o.compensated ? 1u : 0u;
   const uint32_t grid = a.U ? a.U : 1;
   if (n3)
     k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(in.used, w.list, a.o3, n3, bins.wave_max, w.hub_base,

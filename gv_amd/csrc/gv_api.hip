@@ -603,55 +603,111 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
   return rc;
 }
 
-static gv_status pagerank_common(gv_graph *g, float alpha, double eps, int max_iter, float *out,
-                                 int mem, int *iters, double *residual, double *iter_ms) {
+static gv_status pagerank_common(gv_graph *g, const gv_pagerank_opts *po, float *out, int mem,
+                                 int *iters, double *residual, double *iter_ms) {
   gv_status rc = check_graph(g);
   if (rc) return rc;
-  if (!out) return GV_E_INVALID;
+  if (!out || !po) return GV_E_INVALID;
   if (g->directed && !g->has_in) {
     set_error("PageRank on a directed graph needs maintain_in_edges");
     return GV_E_UNSUPPORTED;
   }
   const Store &in = g->has_in ? g->st[1] : g->st[0];
   PrBins bins{g->cfg.pr_thread_max, g->cfg.pr_group_max, g->cfg.pr_wave_max};
-  float *dout = out;
+  const size_t vb = (size_t)(g->V ? g->V : 1) * 4;
+  float *dout = out, *dinit = nullptr;
   if (mem != GV_MEM_DEVICE) {
-    hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
+    hipError_t e = hipMalloc(&dout, vb);
     if (e != hipSuccess) return hip_fail(e, "pagerank alloc");
+  }
+  PrOpts o;
+  o.alpha = po->alpha;
+  o.eps = po->eps;
+  o.max_iter = po->max_iter;
+  o.compensated = po->compensated;
+  if (po->init) {
+    if (po->init_mem == GV_MEM_DEVICE) {
+      o.init = po->init;
+    } else {
+      hipError_t e = hipMalloc(&dinit, vb);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(dinit, po->init, (size_t)g->V * 4, hipMemcpyHostToDevice, g->stream);
+      if (e != hipSuccess) {
+        if (mem != GV_MEM_DEVICE) hipFree(dout);
+        hipFree(dinit);
+        return hip_fail(e, "pagerank init copy");
+      }
+      o.init = dinit;
+    }
   }
   if (g->cfg.pr_layout == 0) {
     PrBins sbins{256, 256, 16384};
-    rc = pagerank_seg_run(in, g->st[0].deg, sbins, alpha, eps, max_iter, dout, iters, residual,
-                          g->pr, g->ps, g->stream, iter_ms);
+    rc = pagerank_seg_run(in, g->st[0].deg, sbins, o, dout, iters, residual, g->pr, g->ps,
+                          g->stream, iter_ms);
   } else {
-    rc = pagerank_run(in, g->st[0].deg, bins, alpha, eps, max_iter, dout, iters, residual, g->pr,
-                      g->stream, iter_ms);
+    rc = pagerank_run(in, g->st[0].deg, bins, o, dout, iters, residual, g->pr, g->stream,
+                      iter_ms);
   }
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
     if (e != hipSuccess) rc = hip_fail(e, "pagerank copy");
   }
+  if (dinit) {
+    hipStreamSynchronize(g->stream);
+    hipFree(dinit);
+  }
   if (mem != GV_MEM_DEVICE) hipFree(dout);
   return rc;
 }
 
+gv_status gv_pagerank_opts_default(gv_pagerank_opts *o) {
+  if (!o) return GV_E_INVALID;
+  std::memset(o, 0, sizeof(*o));
+  o->alpha = 0.85f;
+  o->eps = 1e-6;
+  o->max_iter = 100;
+  o->init = nullptr;
+  o->init_mem = GV_MEM_HOST;
+  o->compensated = 1;
+  return GV_OK;
+}
+
+gv_status gv_pagerank_run(gv_graph *g, const gv_pagerank_opts *opts, float *out, int mem,
+                          gv_pagerank_stats *stats) {
+  gv_pagerank_stats st;
+  std::memset(&st, 0, sizeof(st));
+  gv_status rc = pagerank_common(g, opts, out, mem, &st.iters, &st.residual,
+                                 stats ? &st.kernel_ms : nullptr);
+  if (rc) return rc;
+  if (stats) {
+    rc = read_ctl(g);
+    if (rc) return rc;
+    st.edges = g->h_ctl[g->has_in ? 1 : 0].live_edges;
+    *stats = st;
+  }
+  return GV_OK;
+}
+
 gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
                       int *iters, double *residual) {
-  return pagerank_common(g, alpha, eps, max_iter, out, mem, iters, residual, nullptr);
+  gv_pagerank_opts o;
+  gv_pagerank_opts_default(&o);
+  o.alpha = alpha;
+  o.eps = eps;
+  o.max_iter = max_iter;
+  return pagerank_common(g, &o, out, mem, iters, residual, nullptr);
 }
 
 gv_status gv_pagerank_ex(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
                          gv_pagerank_stats *stats) {
   if (!stats) return GV_E_INVALID;
-  std::memset(stats, 0, sizeof(*stats));
-  gv_status rc = pagerank_common(g, alpha, eps, max_iter, out, mem, &stats->iters,
-                                 &stats->residual, &stats->kernel_ms);
-  if (rc) return rc;
-  rc = read_ctl(g);
-  if (rc) return rc;
-  stats->edges = g->h_ctl[g->has_in ? 1 : 0].live_edges;
-  return GV_OK;
+  gv_pagerank_opts o;
+  gv_pagerank_opts_default(&o);
+  o.alpha = alpha;
+  o.eps = eps;
+  o.max_iter = max_iter;
+  return gv_pagerank_run(g, &o, out, mem, stats);
 }
 
 gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total) {
@@ -699,12 +755,70 @@ gv_status gv_bc(gv_graph *g, const uint32_t *sources, uint32_t n_sources, float 
     hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
     if (e != hipSuccess) return hip_fail(e, "bc alloc");
   }
-  rc = bc_run(g->st[0], hs.data(), n_sources, dout, g->stream);
+  rc = n_sources > 1 ? bc_multi_run(g->st[0], hs.data(), n_sources, dout, g->stream)
+                     : bc_run(g->st[0], hs.data(), n_sources, dout, g->stream);
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
     if (e != hipSuccess) rc = hip_fail(e, "bc copy");
   }
+  if (mem != GV_MEM_DEVICE) hipFree(dout);
+  return rc;
+}
+
+gv_status gv_pagerank_personalized(gv_graph *g, float alpha, double eps, int max_iter,
+                                   const uint32_t *seeds, uint32_t n_cols, float *out, int mem,
+                                   int *iters, double *max_residual) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!out || (n_cols && !seeds)) return GV_E_INVALID;
+  if (n_cols > 64) {
+    set_error("at most 64 personalized columns per call");
+    return GV_E_INVALID;
+  }
+  if (g->directed && !g->has_in) {
+    set_error("PageRank on a directed graph needs maintain_in_edges");
+    return GV_E_UNSUPPORTED;
+  }
+  std::vector<uint32_t> hs(n_cols);
+  if (n_cols) {
+    if (mem == GV_MEM_DEVICE) {
+      GV_HIP(hipMemcpy(hs.data(), seeds, (size_t)n_cols * 4, hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(hs.data(), seeds, (size_t)n_cols * 4);
+    }
+  }
+  for (uint32_t x : hs)
+    if (x >= g->V) {
+      set_error("personalized PageRank seed out of range");
+      return GV_E_RANGE;
+    }
+  const Store &in = g->has_in ? g->st[1] : g->st[0];
+  uint32_t *dseeds = nullptr;
+  float *dout = out;
+  const size_t ob = (size_t)(g->V ? g->V : 1) * (n_cols ? n_cols : 1) * 4;
+  do {
+    hipError_t e;
+    if ((e = hipMalloc(&dseeds, 64 * sizeof(uint32_t))) != hipSuccess ||
+        (n_cols && (e = hipMemcpyAsync(dseeds, hs.data(), (size_t)n_cols * 4,
+                                       hipMemcpyHostToDevice, g->stream)) != hipSuccess)) {
+      rc = hip_fail(e, "personalized PageRank");
+      break;
+    }
+    if (mem != GV_MEM_DEVICE && (e = hipMalloc(&dout, ob)) != hipSuccess) {
+      dout = nullptr;
+      rc = hip_fail(e, "personalized PageRank alloc");
+      break;
+    }
+    rc = ppr_run(in, g->st[0].deg, alpha, eps, max_iter, dseeds, n_cols, dout, iters,
+                 max_residual, nullptr, g->stream);
+    if (!rc && mem != GV_MEM_DEVICE) {
+      e = hipMemcpyAsync(out, dout, (size_t)g->V * n_cols * 4, hipMemcpyDeviceToHost, g->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "personalized PageRank copy");
+    }
+  } while (0);
+  hipFree(dseeds);
   if (mem != GV_MEM_DEVICE) hipFree(dout);
   return rc;
 }

@@ -1,0 +1,458 @@
+// gv_multi.hip -- many-vector analytics: personalized PageRank as an FP32
+// column block, and Brandes BC for a batch of sources at once.
+//
+// Personalized PageRank (the reference's static_pagerank iteration,
+// pagerank.cu:17-59, with the teleport term concentrated on one seed per
+// column): x_c' = (1-a) e_{s_c} + a * sum_{u in N_in(v)} x_c[u]/deg(u),
+// dangling mass dropped, x_c starts at e_{s_c}; stop after the first update
+// whose largest per-column L1 residual is below eps, or after max_iter.
+// Layout: X[v*64 + c] (row of 64 FP32 columns = 256 B).  One wavefront per
+// vertex, lane = column: every in-edge u reads the contiguous row
+// CB[u*64 .. u*64+63], so the gathers are whole, coalesced 256-byte rows and
+// the block is HBM-bound instead of gather-bound.  Kahan per lane.
+//
+// Multi-source BC (bc, bc.cu:3-88; Brandes as in the single-source path):
+// sources are processed S = 32 at a time.  dist/sigma/delta are [V][S]
+// arrays; a group of S lanes handles one frontier vertex, lane = source, and
+// the frontier of a level is the union over the batch's sources (a vertex is
+// queued once for every distinct depth at which some source reaches it).  Forward:
+// CAS on dist[w][s], FP64 atomic sigma; backward: per-lane Kahan sums over
+// successors, then the S dependencies are added in a fixed lane order to
+// BC[v] (FP32 Kahan).
+#include <vector>
+
+#include "gv_analytics.h"
+#include "gv_store.h"
+
+namespace gv {
+
+constexpr int PPR_COLS = 64;
+constexpr int PPR_THREADS = 256;
+constexpr uint32_t PPR_GRID = 2048;
+
+struct PprArgs {
+  Store in;
+  const uint32_t *deg_out;
+  const uint32_t *seeds;
+  uint32_t ncols;
+  float alpha;
+  double eps;
+  uint32_t max_iter;
+  double *partials;  // [gridDim][64]
+  PrCtl *ctl;
+  double *col_res;   // [64] residual of the last update per column
+};
+
+__global__ void k_ppr_init(float *X, float *CB, const uint32_t *deg_out, const uint32_t *seeds,
+                           uint32_t ncols, uint32_t V) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)V * PPR_COLS;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t v = (uint32_t)(i / PPR_COLS), c = (uint32_t)(i % PPR_COLS);
+    float x = (c < ncols && seeds[c] == v) ? 1.0f : 0.0f;
+    uint32_t d = deg_out[v];
+    X[i] = x;
+    CB[i] = d ? x / (float)d : 0.f;
+  }
+}
+
+__global__ __launch_bounds__(PPR_THREADS) void k_ppr_iter(PprArgs a, const float *X, const float *CB,
+                                                          float *Xn, float *CBn) {
+  if (a.ctl->done) return;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const uint32_t V = a.in.V;
+  const uint32_t seed = lane < a.ncols ? a.seeds[lane] : 0xFFFFFFFFu;
+  double diff = 0.0;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * (PPR_THREADS / WAVE); v0 < V;
+       v0 += (uint64_t)gridDim.x * (PPR_THREADS / WAVE)) {
+    const uint64_t v = v0 + wid;
+    if (v >= V) continue;  // uniform per wavefront
+    const uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+    float s = 0.f, c = 0.f;
+    for (uint32_t base = 0; base < us; base += WAVE) {
+      const uint32_t n = us - base < (uint32_t)WAVE ? us - base : (uint32_t)WAVE;
+      uint32_t slot = TOMB;
+      if (lane < n) slot = a.in.slots[slot_index(a.in.arena, tbl, base + lane, a.in.log2B, a.in.B)];
+      for (uint32_t t = 0; t < n; t += 8) {
+        uint32_t u[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) u[k] = __shfl(slot, (int)((t + k) & (WAVE - 1)), WAVE);
+        float g[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          g[k] = (t + k < n && u[k] != TOMB) ? CB[(uint64_t)u[k] * PPR_COLS + lane] : 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (t + k < n && u[k] != TOMB) kahan_add(s, c, g[k]);
+      }
+    }
+    float xv = a.alpha * (s - c) + ((uint32_t)v == seed ? 1.0f - a.alpha : 0.0f);
+    if (lane >= a.ncols) xv = 0.f;
+    const uint64_t idx = v * PPR_COLS + lane;
+    const uint32_t d = a.deg_out[v];
+    diff += fabs((double)xv - (double)X[idx]);
+    Xn[idx] = xv;
+    CBn[idx] = d ? xv / (float)d : 0.f;
+  }
+  // per-column partial of this workgroup, waves added in a fixed order
+  __shared__ double red[PPR_THREADS / WAVE][WAVE];
+  red[wid][lane] = diff;
+  __syncthreads();
+  if (wid == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < PPR_THREADS / WAVE; ++w) t += red[w][lane];
+    a.partials[(uint64_t)blockIdx.x * PPR_COLS + lane] = t;
+  }
+}
+
+__global__ __launch_bounds__(PPR_COLS) void k_ppr_resid(PprArgs a, uint32_t nblocks) {
+  if (a.ctl->done) return;
+  const uint32_t c = threadIdx.x;
+  double r = 0.0;
+  for (uint32_t b = 0; b < nblocks; ++b) r += a.partials[(uint64_t)b * PPR_COLS + c];
+  a.col_res[c] = r;
+  double m = c < a.ncols ? r : 0.0;
+#pragma unroll
+  for (int d = WAVE / 2; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d, WAVE));
+  if (c == 0) {
+    a.ctl->residual = m;
+    unsigned it = a.ctl->iters + 1;
+    a.ctl->iters = it;
+    if (m < a.eps || it >= a.max_iter) a.ctl->done = 1;
+  }
+}
+
+// Copy the first ncols columns of the 64-wide block out (row-major V x ncols).
+__global__ void k_ppr_out(const float *X, uint32_t V, uint32_t ncols, float *out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)V * ncols;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = X[(i / ncols) * PPR_COLS + i % ncols];
+}
+
+gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double eps, int max_iter,
+                  const uint32_t *seeds_dev, uint32_t ncols, float *out_dev, int *iters,
+                  double *max_residual, double *iter_ms, hipStream_t s) {
+  const uint32_t V = in.V;
+  if (iters) *iters = 0;
+  if (max_residual) *max_residual = 0.0;
+  if (!V || !ncols) return GV_OK;
+  const size_t blk = (size_t)V * PPR_COLS * sizeof(float);
+  float *X[2] = {nullptr, nullptr}, *CB[2] = {nullptr, nullptr};
+  double *partials = nullptr, *col_res = nullptr;
+  PrCtl *ctl = nullptr, *h_ctl = nullptr;
+  std::vector<hipEvent_t> ev;
+  gv_status rc = GV_OK;
+  do {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = hipMalloc(&X[i], blk);
+      if (e == hipSuccess) e = hipMalloc(&CB[i], blk);
+    }
+    if (e == hipSuccess) e = hipMalloc(&partials, (size_t)PPR_GRID * PPR_COLS * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&col_res, PPR_COLS * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ctl, sizeof(PrCtl));
+    if (e == hipSuccess) e = hipHostMalloc(&h_ctl, sizeof(PrCtl), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "personalized PageRank alloc");
+      break;
+    }
+    pr_ctl_reset(ctl, s);
+    k_ppr_init<<<grid_for((uint64_t)V * PPR_COLS, 256, 16384), 256, 0, s>>>(X[0], CB[0], deg_out,
+                                                                           seeds_dev, ncols, V);
+    PprArgs a;
+    a.in = in;
+    a.deg_out = deg_out;
+    a.seeds = seeds_dev;
+    a.ncols = ncols;
+    a.alpha = alpha;
+    a.eps = eps;
+    a.max_iter = (uint32_t)(max_iter > 0 ? max_iter : 0);
+    a.partials = partials;
+    a.ctl = ctl;
+    a.col_res = col_res;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(cdiv(V, PPR_THREADS / WAVE), PPR_GRID);
+    int launched = 0;
+    for (; launched < max_iter && rc == GV_OK; ++launched) {
+      const int t = launched;
+      if (iter_ms) {
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        ev.push_back(e0);
+        ev.push_back(e1);
+        hipEventRecord(e0, s);
+      }
+      k_ppr_iter<<<grid, PPR_THREADS, 0, s>>>(a, X[t & 1], CB[t & 1], X[(t + 1) & 1],
+                                              CB[(t + 1) & 1]);
+      k_ppr_resid<<<1, PPR_COLS, 0, s>>>(a, grid);
+      if (iter_ms) hipEventRecord(ev.back(), s);
+      if ((launched + 1) % 8 == 0) {
+        if (hipMemcpyAsync(h_ctl, ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+          rc = hip_fail(hipGetLastError(), "personalized PageRank");
+          break;
+        }
+        if (h_ctl->done) {
+          ++launched;
+          break;
+        }
+      }
+    }
+    if (rc) break;
+    if ((e = hipMemcpyAsync(h_ctl, ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      rc = hip_fail(e, "personalized PageRank");
+      break;
+    }
+    const uint32_t it = h_ctl->iters;
+    k_ppr_out<<<grid_for((uint64_t)V * ncols, 256, 16384), 256, 0, s>>>(X[it & 1], V, ncols,
+                                                                       out_dev);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+      rc = hip_fail(e, "personalized PageRank");
+      break;
+    }
+    if (iters) *iters = (int)it;
+    if (max_residual) *max_residual = h_ctl->residual;
+    if (iter_ms) {
+      double tot = 0.0;
+      for (uint32_t t = 0; t < it && 2 * t + 1 < ev.size(); ++t) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev[2 * t], ev[2 * t + 1]);
+        tot += ms;
+      }
+      *iter_ms = tot;
+    }
+  } while (0);
+  for (hipEvent_t e : ev) hipEventDestroy(e);
+  for (int i = 0; i < 2; ++i) {
+    hipFree(X[i]);
+    hipFree(CB[i]);
+  }
+  hipFree(partials);
+  hipFree(col_res);
+  hipFree(ctl);
+  if (h_ctl) hipHostFree(h_ctl);
+  return rc;
+}
+
+// ================================================================ BC ==
+constexpr int BCM_S = 32;  // sources per batch = lanes per frontier vertex
+
+__global__ void k_bcm_init(int *dist, double *sigma, float *delta, uint32_t *stamp, uint32_t V) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)V * BCM_S;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    dist[i] = -1;
+    sigma[i] = 0.0;
+    delta[i] = 0.f;
+    if (i < V) stamp[i] = 0;
+  }
+}
+
+// Sources of the batch enter level 0 (stamp = level + 1 marks queue entry).
+__global__ void k_bcm_src(const uint32_t *src, uint32_t ns, int *dist, double *sigma,
+                          uint32_t *stamp, uint32_t *queue, unsigned int *cursor) {
+  const uint32_t s = threadIdx.x;
+  if (s >= ns) return;
+  const uint32_t v = src[s];
+  dist[(uint64_t)v * BCM_S + s] = 0;
+  sigma[(uint64_t)v * BCM_S + s] = 1.0;
+  if (atomicMax(&stamp[v], 1u) < 1u) queue[atomicAdd(cursor, 1u)] = v;
+}
+
+__global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *sigma,
+                                                 uint32_t *stamp, uint32_t *queue, uint32_t qs,
+                                                 uint32_t qe, int d, unsigned int *cursor) {
+  const uint32_t lane = lane_id();
+  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;  // two vertices per wavefront
+  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
+  const uint32_t nf = qe - qs;
+  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
+       g0 < nf; g0 += ngroups) {
+    const uint64_t gi = g0 + sub;
+    const bool have = gi < nf;
+    uint32_t v = 0, us = 0, tbl = 0;
+    int dv = -2;
+    double sv = 0.0;
+    if (have) {
+      v = queue[qs + gi];
+      us = st.used[v];
+      tbl = st.tbl_off[v];
+      dv = dist[(uint64_t)v * BCM_S + s];
+      sv = sigma[(uint64_t)v * BCM_S + s];
+    }
+    const bool act = dv == d;
+    // both half-waves loop to the larger slot count; shuffles stay uniform
+    const uint32_t us_max = max(us, __shfl_xor(us, BCM_S, WAVE));
+    for (uint32_t base = 0; base < us_max; base += BCM_S) {
+      uint32_t slot = TOMB;
+      if (base + s < us) slot = st.slots[slot_index(st.arena, tbl, base + s, st.log2B, st.B)];
+      for (uint32_t t = 0; t < (uint32_t)BCM_S; ++t) {
+        const uint32_t w = __shfl(slot, (int)t, BCM_S);
+        bool fresh = false;
+        if (act && w != TOMB) {
+          int old = atomicCAS(&dist[(uint64_t)w * BCM_S + s], -1, d + 1);
+          fresh = old == -1;
+          if (fresh || old == d + 1) atomicAdd(&sigma[(uint64_t)w * BCM_S + s], sv);
+        }
+        const unsigned long long m = __ballot(fresh);
+        const unsigned long long mine = (m >> (sub * BCM_S)) & 0xFFFFFFFFull;
+        if (s == 0 && mine) {
+          if (atomicMax(&stamp[w], (uint32_t)(d + 2)) < (uint32_t)(d + 2))
+            queue[qe + atomicAdd(cursor, 1u)] = w;
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void bcm_merge(float &s, float &c, float s2, float c2) {
+  float t = s + s2;
+  float bp = t - s;
+  float e = (s - (t - bp)) + (s2 - bp);
+  s = t;
+  c = c + c2 - e;
+}
+
+__global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, const double *sigma,
+                                                 float *delta, const uint32_t *queue, uint32_t qs,
+                                                 uint32_t qe, int d, const uint32_t *src,
+                                                 uint32_t ns, float *bc, float *bcc) {
+  const uint32_t lane = lane_id();
+  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
+  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
+  const uint32_t nf = qe - qs;
+  const uint32_t my_src = s < ns ? src[s] : 0xFFFFFFFFu;
+  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
+       g0 < nf; g0 += ngroups) {
+    const uint64_t gi = g0 + sub;
+    const bool have = gi < nf;
+    uint32_t v = 0, us = 0, tbl = 0;
+    int dv = -2;
+    double sv = 0.0;
+    if (have) {
+      v = queue[qs + gi];
+      us = st.used[v];
+      tbl = st.tbl_off[v];
+      dv = dist[(uint64_t)v * BCM_S + s];
+      sv = sigma[(uint64_t)v * BCM_S + s];
+    }
+    const bool act = dv == d;
+    float ds = 0.f, dc = 0.f;
+    const uint32_t us_max = max(us, __shfl_xor(us, BCM_S, WAVE));
+    for (uint32_t base = 0; base < us_max; base += BCM_S) {
+      uint32_t slot = TOMB;
+      if (base + s < us) slot = st.slots[slot_index(st.arena, tbl, base + s, st.log2B, st.B)];
+      for (uint32_t t = 0; t < (uint32_t)BCM_S; ++t) {
+        const uint32_t w = __shfl(slot, (int)t, BCM_S);
+        if (act && w != TOMB && dist[(uint64_t)w * BCM_S + s] == d + 1) {
+          const uint64_t wi = (uint64_t)w * BCM_S + s;
+          float coef = (float)(sv / sigma[wi]);
+          kahan_add(ds, dc, coef * (1.0f + delta[wi]));
+        }
+      }
+    }
+    float dvl = act ? ds - dc : 0.f;
+    if (act) delta[(uint64_t)v * BCM_S + s] = dvl;
+    // add this vertex's dependencies over the batch's sources in lane order
+    float bs = (act && v != my_src) ? dvl : 0.f, bcomp = 0.f;
+#pragma unroll
+    for (int k = BCM_S / 2; k >= 1; k >>= 1) {
+      float s2 = __shfl_xor(bs, k, BCM_S), c2 = __shfl_xor(bcomp, k, BCM_S);
+      bcm_merge(bs, bcomp, s2, c2);
+    }
+    if (have && s == 0) {
+      float b = bc[v], bcv = bcc[v];
+      kahan_add(b, bcv, bs - bcomp);
+      bc[v] = b;
+      bcc[v] = bcv;
+    }
+  }
+}
+
+gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
+                       float *out_dev, hipStream_t s) {
+  const uint32_t V = st.V;
+  if (!V) return GV_OK;
+  int *dist = nullptr;
+  double *sigma = nullptr;
+  float *delta = nullptr, *bcc = nullptr;
+  uint32_t *stamp = nullptr, *queue = nullptr, *dsrc = nullptr;
+  unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  gv_status rc = GV_OK;
+  const uint64_t n2 = (uint64_t)V * BCM_S;
+  do {
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&dist, n2 * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&sigma, n2 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&delta, n2 * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&bcc, (size_t)V * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&stamp, (size_t)V * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&queue, n2 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&dsrc, BCM_S * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&cursor, sizeof(unsigned int))) != hipSuccess ||
+        (e = hipHostMalloc(&h_cursor, sizeof(unsigned int), hipHostMallocDefault)) != hipSuccess) {
+      rc = hip_fail(e, "bc alloc");
+      break;
+    }
+    fill_f32(out_dev, 0.f, V, s);
+    fill_f32(bcc, 0.f, V, s);
+    for (uint32_t b0 = 0; b0 < n_sources && rc == GV_OK; b0 += BCM_S) {
+      const uint32_t ns = std::min<uint32_t>(BCM_S, n_sources - b0);
+      if ((e = hipMemcpyAsync(dsrc, sources + b0, ns * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              s)) != hipSuccess) {
+        rc = hip_fail(e, "bc");
+        break;
+      }
+      k_bcm_init<<<grid_for(n2, 256, 16384), 256, 0, s>>>(dist, sigma, delta, stamp, V);
+      fill_u32(cursor, 0, 1, s);
+      k_bcm_src<<<1, BCM_S, 0, s>>>(dsrc, ns, dist, sigma, stamp, queue, cursor);
+      if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
+              hipSuccess ||
+          (e = hipStreamSynchronize(s)) != hipSuccess) {
+        rc = hip_fail(e, "bc");
+        break;
+      }
+      std::vector<uint32_t> lvl{0u, *h_cursor};
+      int d = 0;
+      while (true) {
+        const uint32_t qs = lvl[d], qe = lvl[d + 1];
+        if (qe == qs) break;
+        fill_u32(cursor, 0, 1, s);
+        k_bcm_fwd<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
+            st, dist, sigma, stamp, queue, qs, qe, d, cursor);
+        if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
+                hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess) {
+          rc = hip_fail(e, "bc");
+          break;
+        }
+        lvl.push_back(qe + *h_cursor);
+        ++d;
+      }
+      if (rc) break;
+      // queue[lvl[k], lvl[k+1]) = union frontier of level k: every vertex
+      // that some source of the batch reaches at depth k (a vertex appears
+      // once per distinct depth, stamp-deduplicated).  The deepest level has
+      // no successors, so the backward sweep starts one level above it.
+      const int D = (int)lvl.size() - 2;
+      for (int lev = D - 2; lev >= 0; --lev) {
+        const uint32_t qs = lvl[lev], qe = lvl[lev + 1];
+        k_bcm_bwd<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
+            st, dist, sigma, delta, queue, qs, qe, lev, dsrc, ns, out_dev, bcc);
+      }
+    }
+    if (rc == GV_OK && (hipStreamSynchronize(s) != hipSuccess)) rc = hip_fail(hipGetLastError(), "bc");
+  } while (0);
+  hipFree(dist);
+  hipFree(sigma);
+  hipFree(delta);
+  hipFree(bcc);
+  hipFree(stamp);
+  hipFree(queue);
+  hipFree(dsrc);
+  hipFree(cursor);
+  if (h_cursor) hipHostFree(h_cursor);
+  return rc;
+}
+
+}  // namespace gv

@@ -120,17 +120,19 @@ __global__ void k_ps_split(const uint64_t *keys, uint64_t E, PsGeom g, uint32_t 
 
 __global__ void k_ps_vertex_init(const uint32_t *deg_in, const uint32_t *deg_out, PsGeom g,
                                  uint32_t light_max, uint32_t *din_p, uint32_t *dout_p,
-                                 uint32_t *heavy, float *xp, float *cp, float x0) {
+                                 uint32_t *heavy, float *xp, float *cp, float x0,
+                                 const float *init) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t j = ps_perm((uint32_t)v, g.mask, g.V);
     uint32_t d = deg_out[v];
     uint32_t di = deg_in[v];
     if (di > light_max) atomicOr(&heavy[j >> 5], 1u << (j & 31));
+    float xv = init ? init[v] : x0;
     din_p[j] = di;
     dout_p[j] = d;
-    xp[j] = x0;
-    cp[j] = d ? x0 / (float)d : 0.f;
+    xp[j] = xv;
+    cp[j] = d ? xv / (float)d : 0.f;
   }
 }
 
@@ -356,9 +358,11 @@ static gv_status ps_reserve(PsWork &w, uint32_t V, uint64_t E) {
 }
 
 gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
-                           float alpha, double eps, int max_iter, float *out_dev, int *iters,
-                           double *residual, PrWork &pw, PsWork &w, hipStream_t s,
-                           double *iter_ms) {
+                           const PrOpts &o, float *out_dev, int *iters, double *residual,
+                           PrWork &pw, PsWork &w, hipStream_t s, double *iter_ms) {
+  const float alpha = o.alpha;
+  const double eps = o.eps;
+  const int max_iter = o.max_iter;
   const uint32_t V = in.V;
   if (iters) *iters = 0;
   if (residual) *residual = 0.0;
@@ -425,7 +429,7 @@ gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBin
   pr_ctl_reset(pw.ctl, s);
   fill_u32(w.heavy, 0, cdiv(V, 32) + 1, s);
   k_ps_vertex_init<<<grid_for(V), 256, 0, s>>>(in.deg, deg_out, g, bins.thread_max, w.din,
-                                               w.dout, w.heavy, w.xp[0], w.cp[0], x0);
+                                               w.dout, w.heavy, w.xp[0], w.cp[0], x0, o.init);
   if (max_iter <= 0) {
     k_ps_unpermute<<<grid_for(V), 256, 0, s>>>(w.xp[0], g, out_dev);
     GV_HIP(hipStreamSynchronize(s));

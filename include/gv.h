@@ -136,11 +136,37 @@ typedef struct gv_pagerank_stats {
 } gv_pagerank_stats;
 gv_status gv_pagerank_ex(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
                          gv_pagerank_stats *stats);
+/* Full-control PageRank: warm start from a previous vector (an incremental
+ * refresh after updates converges to the same fixed point in fewer
+ * iterations; the reference's dynamic_pagerank heuristic, pagerank.cu:70-132,
+ * has no exact semantics) and plain vs compensated FP32 sums.  stats may be
+ * NULL. */
+typedef struct gv_pagerank_opts {
+  float alpha;       /* damping, 0.85 */
+  double eps;        /* L1 residual threshold */
+  int max_iter;      /* update limit */
+  const float *init; /* NULL: x0 = 1/V; else V start values */
+  int init_mem;      /* GV_MEM_HOST or GV_MEM_DEVICE for init */
+  int compensated;   /* 1: per-vertex Kahan sums (default); 0: plain FP32 sums */
+} gv_pagerank_opts;
+gv_status gv_pagerank_opts_default(gv_pagerank_opts *opts);
+gv_status gv_pagerank_run(gv_graph *g, const gv_pagerank_opts *opts, float *out, int mem,
+                          gv_pagerank_stats *stats);
+/* Personalized PageRank for up to 64 seeds in one FP32 column block: column
+ * c starts at e_{seeds[c]} and teleports there, x_c' = (1-alpha) e_{s_c} +
+ * alpha * sum_{u in N_in(v)} x_c[u]/|N(u)| (dangling mass dropped, as in
+ * static_pagerank); stops after the first update whose largest per-column L1
+ * residual is below eps, or after max_iter.  seeds: host or device (mem);
+ * out: row-major V x n_cols (mem).  No counterpart in the reference. */
+gv_status gv_pagerank_personalized(gv_graph *g, float alpha, double eps, int max_iter,
+                                   const uint32_t *seeds, uint32_t n_cols, float *out, int mem,
+                                   int *iters, double *max_residual);
 /* static_tc on an undirected graph: exact uint64 counts; per_vertex (may be
  * NULL) gets t[u] = triangles whose minimum vertex ID is u. */
 gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total);
 /* bc: Brandes betweenness from the given sources (unnormalised, not halved);
- * sigma in FP64, dependencies in FP32 with compensated sums. */
+ * sigma in FP64, dependencies in FP32 with compensated sums.  Several sources
+ * run 32 at a time in one level-synchronous sweep. */
 gv_status gv_bc(gv_graph *g, const uint32_t *sources, uint32_t n_sources, float *out, int mem);
 
 /* Device generators for update batches and synthetic graphs.  Each draws

@@ -349,6 +349,53 @@ int orc_pagerank(const orc_graph *g, double alpha, double eps, int max_iter, int
   return it;
 }
 
+/* Personalized PageRank (static_pagerank's iteration with the teleport term
+ * on one seed per column): x_c0 = e_{s_c};
+ *   x_c'[v] = (1-a)[v == s_c] + a * sum_{u: v in N(u)} x_c[u]/|N(u)|,
+ * dangling mass dropped; stop after the first update whose largest
+ * per-column L1 residual is below eps (or max_iter; fixed_iters > 0 runs
+ * exactly that many).  x is row-major V x ncols, FP64.  Returns updates. */
+int orc_pagerank_personalized(const orc_graph *g, double alpha, double eps, int max_iter,
+                              int fixed_iters, const uint32_t *seeds, uint32_t ncols, double *x,
+                              double *max_residual) {
+  uint32_t V = g->V;
+  size_t n = (size_t)V * ncols;
+  double *acc = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double *r = (double *)malloc(sizeof(double) * (ncols ? ncols : 1));
+  for (size_t i = 0; i < n; ++i) x[i] = 0.0;
+  for (uint32_t c = 0; c < ncols; ++c) x[(size_t)seeds[c] * ncols + c] = 1.0;
+  int limit = fixed_iters > 0 ? fixed_iters : max_iter, it = 0;
+  double mr = 0.0;
+  while (it < limit) {
+    memset(acc, 0, sizeof(double) * n);
+    for (uint32_t u = 0; u < V; ++u) {
+      uint32_t d = g->out[u].n;
+      if (!d) continue;
+      for (uint32_t j = 0; j < d; ++j) {
+        uint32_t v = g->out[u].a[j];
+        for (uint32_t c = 0; c < ncols; ++c)
+          acc[(size_t)v * ncols + c] += x[(size_t)u * ncols + c] / (double)d;
+      }
+    }
+    for (uint32_t c = 0; c < ncols; ++c) r[c] = 0.0;
+    for (uint32_t v = 0; v < V; ++v)
+      for (uint32_t c = 0; c < ncols; ++c) {
+        size_t i = (size_t)v * ncols + c;
+        double xn = alpha * acc[i] + (seeds[c] == v ? 1.0 - alpha : 0.0);
+        r[c] += fabs(xn - x[i]);
+        x[i] = xn;
+      }
+    mr = 0.0;
+    for (uint32_t c = 0; c < ncols; ++c) mr = r[c] > mr ? r[c] : mr;
+    ++it;
+    if (fixed_iters <= 0 && mr < eps) break;
+  }
+  if (max_residual) *max_residual = mr;
+  free(acc);
+  free(r);
+  return it;
+}
+
 static int has_edge(const adjv *a, uint32_t x) {
   uint32_t lo = 0, hi = a->n;
   while (lo < hi) {

@@ -60,6 +60,9 @@ void orc_export_csr(const orc_graph *g, uint64_t *offsets, uint32_t *cols);
 /* analytics */
 int orc_pagerank(const orc_graph *g, double alpha, double eps, int max_iter, int fixed_iters,
                  double *x, double *residual);
+int orc_pagerank_personalized(const orc_graph *g, double alpha, double eps, int max_iter,
+                              int fixed_iters, const uint32_t *seeds, uint32_t ncols, double *x,
+                              double *max_residual);
 uint64_t orc_triangle_count(const orc_graph *g, uint64_t *per_vertex);
 void orc_bc(const orc_graph *g, const uint32_t *sources, uint32_t n_sources, double *bc);
 

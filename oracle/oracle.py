@@ -57,6 +57,7 @@ def lib():
         "orc_degrees": ([P, P], None),
         "orc_export_csr": ([P, P, P], None),
         "orc_pagerank": ([P, f64, f64, i32, i32, P, ctypes.POINTER(f64)], i32),
+        "orc_pagerank_personalized": ([P, f64, f64, i32, i32, P, u32, P, ctypes.POINTER(f64)], i32),
         "orc_triangle_count": ([P, P], u64),
         "orc_bc": ([P, P, u32, P], None),
         "omp_apply_batch": ([u32, i32, P, P, P, P, u64, i32, P, P], u64),
@@ -175,6 +176,14 @@ class OracleGraph:
         x = np.empty(self.V, np.float64)
         r = ctypes.c_double()
         it = lib().orc_pagerank(self._g, alpha, eps, max_iter, fixed_iters, _p(x), ctypes.byref(r))
+        return x, int(it), float(r.value)
+
+    def pagerank_personalized(self, seeds, alpha=0.85, eps=1e-6, max_iter=100, fixed_iters=0):
+        sd = u32(seeds)
+        x = np.empty((self.V, len(sd)), np.float64)
+        r = ctypes.c_double()
+        it = lib().orc_pagerank_personalized(self._g, alpha, eps, max_iter, fixed_iters, _p(sd),
+                                             len(sd), _p(x), ctypes.byref(r))
         return x, int(it), float(r.value)
 
     def triangles(self):

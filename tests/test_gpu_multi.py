@@ -1,0 +1,98 @@
+"""GPU parity of the many-vector analytics and PageRank options: the
+64-column personalized PageRank block, multi-source BC (32 sources per
+sweep), warm-started and plain-sum PageRank."""
+import numpy as np
+import pytest
+
+import gv_amd as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def build_pair(V, directed, s, d, **kw):
+    cfg = G.default_config(**kw)
+    cfg.pool_blocks = 8 * V + 4096
+    g = G.GraphVine(V, directed=directed, config=cfg)
+    o = O.OracleGraph(V, directed)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    return g, o
+
+
+@pytest.mark.parametrize("directed", [False, True])
+@pytest.mark.parametrize("ncols", [1, 7, 64])
+def test_personalized_pagerank_block(directed, ncols):
+    V = 3000
+    s, d = O.gen_rmat(O.stream_key(21, O.STREAM_RMAT), 12, 40000)
+    g, o = build_pair(V, directed, s % V, d % V)
+    seeds = np.random.default_rng(ncols).choice(V, ncols, replace=False).astype(np.uint32)
+    X, it, r = G.pagerank_personalized(g, seeds, 0.85, 1e-6, 100)
+    Xo, ito, _ = o.pagerank_personalized(seeds, 0.85, 1e-6, 100)
+    assert abs(it - ito) <= 1
+    Xr, _, _ = o.pagerank_personalized(seeds, 0.85, 1e-6, 100, fixed_iters=it)
+    l1 = np.abs(X.astype(np.float64) - Xr).sum(axis=0)
+    assert l1.max() <= 1e-6, l1.max()
+    g.close()
+
+
+def test_personalized_matches_global_when_averaged():
+    V = 64
+    s, d = O.gen_rmat(O.stream_key(22, O.STREAM_RMAT), 6, 500)
+    g, o = build_pair(V, False, s % V, d % V)
+    X, it, _ = G.pagerank_personalized(g, np.arange(V), 0.85, 1e-30, 12)
+    x, itg, _ = G.static_pagerank(g, 0.85, 1e-30, 12)
+    assert it == itg == 12
+    assert np.abs(X.mean(axis=1) - x).max() < 1e-6
+    g.close()
+
+
+@pytest.mark.parametrize("nsrc", [2, 33, 70])
+def test_multi_source_bc(nsrc):
+    V = 1 << 12
+    s, d = O.gen_rmat(O.stream_key(23, O.STREAM_RMAT), 12, 16 * V)
+    for directed in (True, False):
+        g, o = build_pair(V, directed, s, d)
+        src = np.random.default_rng(nsrc).choice(V, nsrc, replace=False).astype(np.uint32)
+        b = G.bc(g, src).astype(np.float64)
+        bo = o.bc(src)
+        err = np.abs(b - bo) / np.maximum(np.abs(bo), 1.0)
+        assert err.max() <= 1e-5, err.max()
+        g.close()
+
+
+def test_multi_source_bc_repeated_source_and_path():
+    n = 40
+    g, o = build_pair(n, False, np.arange(n - 1), np.arange(1, n))
+    src = np.array([0, 5, 5, 39, 20], np.uint32)
+    b = G.bc(g, src).astype(np.float64)
+    assert np.allclose(b, o.bc(src), rtol=1e-6, atol=1e-4)
+    g.close()
+
+
+def test_pagerank_warm_start_and_plain_sums():
+    V = 1 << 13
+    s, d = O.gen_rmat(O.stream_key(24, O.STREAM_RMAT), 13, 16 * V)
+    g, o = build_pair(V, False, s, d)
+    x0, it0, _ = G.pagerank(g, epsilon=1e-7)
+    # restarting from the converged vector stops after the first update, whose
+    # residual is already below eps
+    again, ita, ra = G.pagerank(g, epsilon=1e-7, init=x0)
+    assert ita <= 2 and ra < 1e-7
+    assert np.abs(again - x0).sum() < 1e-6
+    # after a perturbation a warm start need not take fewer iterations (the
+    # uniform edges join isolated vertices into small bipartite pieces whose
+    # error decays at rate alpha from any start), but it reaches the same point
+    si, di = O.gen_uniform(O.stream_key(24, 1000), V, 2000)
+    g.batchInsert(si, di)
+    o.insert(si, di)
+    cold, itc, _ = G.pagerank(g)
+    warm, itw, _, st = G.pagerank(g, init=x0, stats=True)
+    assert st["iters"] == itw
+    xo, _, _ = o.pagerank(eps=1e-12, max_iter=1000)
+    # both reach the oracle's fixed point within the stopping tolerance
+    assert np.abs(warm - xo).sum() < 2e-5 and np.abs(cold - xo).sum() < 2e-5
+    plain, itp, _ = G.pagerank(g, compensated=False)
+    xr, _, _ = o.pagerank(fixed_iters=itp)
+    assert np.abs(plain - xr).sum() <= 1e-5
+    g.close()
