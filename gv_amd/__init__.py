@@ -41,7 +41,9 @@ EXPORTED = [
     "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_pagerank_ex",
     "gv_pagerank_opts_default", "gv_pagerank_run", "gv_pagerank_personalized",
     "gv_triangle_count",
-    "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_sync", "gv_get_stream",
+    "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_gen_chung_lu",
+    "gv_get_config", "gv_set_tuning", "gv_reserve", "gv_capture_begin", "gv_capture_end",
+    "gv_capture_launch", "gv_capture_destroy", "gv_sync", "gv_get_stream",
     "gv_last_error",
 ]
 
@@ -144,6 +146,15 @@ def lib():
         "gv_gen_rmat": [P, u64, i32, u64, u32, u32, u32, P, P],
         "gv_gen_uniform": [P, u64, u32, u64, P, P],
         "gv_gen_sample": [P, u64, u32, P, P, u64, P, P],
+        "gv_gen_chung_lu": [P, u64, u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, u64,
+                            P, P],
+        "gv_get_config": [P, ctypes.POINTER(GvConfig)],
+        "gv_set_tuning": [P, ctypes.POINTER(GvConfig)],
+        "gv_reserve": [P, u64, u32],
+        "gv_capture_begin": [P],
+        "gv_capture_end": [P, ctypes.POINTER(P)],
+        "gv_capture_launch": [P, ctypes.POINTER(GvBatchStats)],
+        "gv_capture_destroy": [P],
         "gv_sync": [P],
         "gv_get_stream": [P, ctypes.POINTER(P)],
     }
@@ -285,6 +296,32 @@ class GraphVine:
 
     def sync(self):
         _check(lib().gv_sync(self.handle))
+
+    def get_config(self):
+        cfg = GvConfig()
+        _check(lib().gv_get_config(self.handle, ctypes.byref(cfg)))
+        return cfg
+
+    def set_tuning(self, **fields):
+        """Change compact_threshold / pr_thread_max / pr_group_max /
+        pr_wave_max / pr_layout of the live handle."""
+        cfg = self.get_config()
+        for k, v in fields.items():
+            if k not in ("compact_threshold", "pr_thread_max", "pr_group_max", "pr_wave_max",
+                         "pr_layout"):
+                raise ValueError(f"{k} is not a live tuning field")
+            setattr(cfg, k, v)
+        _check(lib().gv_set_tuning(self.handle, ctypes.byref(cfg)))
+        self.config = self.get_config()
+
+    def reserve(self, max_pairs, max_batches=1):
+        """Size staging buffers so captured updates never allocate."""
+        _check(lib().gv_reserve(self.handle, int(max_pairs), int(max_batches)))
+
+    def capture(self):
+        """Context manager recording the update/compaction calls made inside
+        it; on exit .result holds a Capture that replays them."""
+        return _CaptureScope(self)
 
     def stream(self):
         p = ctypes.c_void_p()
@@ -440,6 +477,59 @@ def gen_sample(graph, key, V, offsets_dev, cols_dev, n):
     _check(lib().gv_gen_sample(graph.handle, key, V, offsets_dev.ptr, cols_dev.ptr, n, src.ptr,
                                dst.ptr))
     return src, dst
+
+
+def gen_chung_lu(graph, key, V, n, gamma=2.1, avg_deg=24.0, max_deg=200000.0):
+    """n Chung-Lu edges (endpoints drawn in proportion to power-law weights)."""
+    from .hip import DeviceArray
+    src, dst = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
+    _check(lib().gv_gen_chung_lu(graph.handle, key, V, float(gamma), float(avg_deg),
+                                 float(max_deg), n, src.ptr, dst.ptr))
+    return src, dst
+
+
+class Capture:
+    """A recorded update/compaction pipeline (gv_capture)."""
+
+    def __init__(self, graph, handle):
+        self.graph = graph
+        self._c = handle
+
+    def launch(self, stats=False):
+        st = GvBatchStats()
+        _check(lib().gv_capture_launch(self._c, ctypes.byref(st) if stats else None))
+        return st.as_dict() if stats else None
+
+    def close(self):
+        if getattr(self, "_c", None) is not None:
+            lib().gv_capture_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _CaptureScope:
+    def __init__(self, graph):
+        self.graph = graph
+        self.result = None
+
+    def __enter__(self):
+        _check(lib().gv_capture_begin(self.graph.handle))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        c = ctypes.c_void_p()
+        rc = lib().gv_capture_end(self.graph.handle, ctypes.byref(c))
+        if et is None:
+            _check(rc)
+            self.result = Capture(self.graph, c)
+        elif rc == 0:
+            lib().gv_capture_destroy(c)
+        return False
 
 
 def export_csr_device(graph):

@@ -3,6 +3,7 @@
 // the handle's own stream; the host waits only where a call must return a
 // value (stats, sizes, analytic results).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,6 +22,8 @@ gv_status hip_fail(hipError_t e, const char *what) {
 
 using namespace gv;
 
+struct gv_capture;
+
 struct gv_graph {
   gv_config cfg;
   uint32_t V = 0;
@@ -38,6 +41,26 @@ struct gv_graph {
   uint64_t *grp_off = nullptr;   // device copy of a group's batch offsets
   uint32_t *grp_kind = nullptr;  // device copy of a group's batch kinds
   uint32_t grp_cap = 0;
+  // Stream capture (gv_capture_begin .. gv_capture_end).
+  int capturing = 0;
+  gv_status cap_err = GV_OK;        // first failure of a call made while capturing
+  uint64_t cap_pairs = 0;           // update pairs recorded in the open capture
+  std::vector<void *> cap_pinned;   // pinned host copies read by recorded copies
+  uint64_t buf_gen = 0;             // bumped whenever a buffer a capture may use moves
+  std::vector<gv_capture *> captures;  // live captures (detached by gv_destroy)
+  // Chung-Lu cumulative table on the device, cached by its parameters.
+  uint64_t *cl_cum = nullptr;
+  uint32_t cl_V = 0;
+  double cl_gamma = 0, cl_avg = 0, cl_max = 0;
+};
+
+struct gv_capture {
+  gv_graph *g = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t gen = 0;
+  uint64_t pairs = 0;
+  std::vector<void *> pinned;
 };
 
 extern "C" {
@@ -62,9 +85,15 @@ gv_status gv_config_default(gv_config *cfg) {
 
 }  // extern "C"
 
-static gv_status check_graph(gv_graph *g) {
+// capturable: the call may be recorded into an open stream capture.
+static gv_status check_graph(gv_graph *g, bool capturable = false) {
   if (!g) {
     set_error("null graph handle");
+    return GV_E_INVALID;
+  }
+  if (g->capturing && !capturable) {
+    set_error("call not allowed while the handle's stream is being captured");
+    if (!g->cap_err) g->cap_err = GV_E_INVALID;
     return GV_E_INVALID;
   }
   hipError_t e = hipSetDevice(g->cfg.device);
@@ -115,6 +144,10 @@ static gv_status device_inputs(gv_graph *g, const uint32_t *src, const uint32_t 
     set_error("mem must be GV_MEM_HOST or GV_MEM_DEVICE");
     return GV_E_INVALID;
   }
+  if (g->capturing) {
+    set_error("a captured update needs device inputs (GV_MEM_DEVICE)");
+    return GV_E_INVALID;
+  }
   if (n > g->in_cap) {
     hipFree(g->in_src);
     hipFree(g->in_dst);
@@ -141,14 +174,51 @@ static void fill_stats(gv_graph *g, uint64_t n, gv_batch_stats *stats) {
   stats->live_edges = c.live_edges;
 }
 
+// Scratch for m staged keys in store k.  Inside a capture the buffers must
+// already be large enough (gv_reserve), since recorded launches keep their
+// addresses; outside, a reallocation bumps buf_gen so that older captures
+// refuse to launch.
+static gv_status reserve_keys(gv_graph *g, int k, uint64_t m) {
+  if (scratch_fits(g->sc[k], g->st[k], m)) return GV_OK;
+  if (g->capturing) {
+    set_error("capture: staging buffers too small; call gv_reserve before gv_capture_begin");
+    return GV_E_INVALID;
+  }
+  ++g->buf_gen;
+  return scratch_reserve(g->sc[k], g->st[k], m);
+}
+
+static gv_status reserve_groups(gv_graph *g, uint32_t nb) {
+  if (nb <= g->grp_cap) return GV_OK;
+  if (g->capturing) {
+    set_error("capture: group buffers too small; call gv_reserve before gv_capture_begin");
+    return GV_E_INVALID;
+  }
+  ++g->buf_gen;
+  hipFree(g->grp_off);
+  hipFree(g->grp_kind);
+  g->grp_off = nullptr;
+  g->grp_kind = nullptr;
+  g->grp_cap = 0;
+  uint32_t cap = std::max<uint32_t>(nb, 64);
+  GV_HIP(hipMalloc(&g->grp_off, ((size_t)cap + 1) * sizeof(uint64_t)));
+  GV_HIP(hipMalloc(&g->grp_kind, (size_t)cap * sizeof(uint32_t)));
+  g->grp_cap = cap;
+  return GV_OK;
+}
+
 enum class Op { Insert, Delete };
 
 static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
                              int mem, gv_batch_stats *stats, Op op) {
-  gv_status rc = check_graph(g);
+  gv_status rc = check_graph(g, true);
   if (rc) return rc;
   if (n && (!src || !dst)) {
     set_error("null batch pointer");
+    return GV_E_INVALID;
+  }
+  if (g->capturing && stats) {
+    set_error("a captured update cannot return stats; pass NULL");
     return GV_E_INVALID;
   }
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -166,8 +236,12 @@ static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *d
   uint64_t *keys[2] = {nullptr, nullptr};
   for (int k = 0; k < nst; ++k) {
     const int mirror = g->directed ? 0 : 1;
-    rc = scratch_reserve(g->sc[k], g->st[k], mirror ? 2 * n : n);
+    rc = reserve_keys(g, k, mirror ? 2 * n : n);
     if (rc) return rc;
+  }
+  if (g->capturing) g->cap_pairs += n;
+  for (int k = 0; k < nst; ++k) {
+    const int mirror = g->directed ? 0 : 1;
     op_begin(g->st[k], s);
     keys[k] = stage(g->st[k], g->sc[k], ds, dd, n, mirror, k == 1 ? 1 : 0, m[k], s);
     segment(g->st[k], g->sc[k], keys[k], m[k], s);
@@ -190,8 +264,12 @@ static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *d
 static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *dst,
                              const uint64_t *offsets, const int *kinds, uint32_t nb, int mem,
                              float compact_fraction, gv_batch_stats *stats) {
-  gv_status rc = check_graph(g);
+  gv_status rc = check_graph(g, true);
   if (rc) return rc;
+  if (g->capturing && stats) {
+    set_error("a captured update cannot return stats; pass NULL");
+    return GV_E_INVALID;
+  }
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (nb == 0) return GV_OK;
   if (!offsets || !kinds) {
@@ -230,29 +308,39 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
     const uint32_t *ds = nullptr, *dd = nullptr;
     rc = device_inputs(g, src, dst, n, mem, &ds, &dd);
     if (rc) return rc;
-    if (nb > g->grp_cap) {
-      hipFree(g->grp_off);
-      hipFree(g->grp_kind);
-      g->grp_off = nullptr;
-      g->grp_kind = nullptr;
-      uint32_t cap = std::max<uint32_t>(nb, 64);
-      GV_HIP(hipMalloc(&g->grp_off, ((size_t)cap + 1) * sizeof(uint64_t)));
-      GV_HIP(hipMalloc(&g->grp_kind, (size_t)cap * sizeof(uint32_t)));
-      g->grp_cap = cap;
-    }
-    std::vector<uint32_t> hk(nb);
-    for (uint32_t b = 0; b < nb; ++b) hk[b] = kinds[b] == GV_OP_DELETE ? OP_DELETE : OP_INSERT;
-    GV_HIP(hipMemcpyAsync(g->grp_off, offsets, ((size_t)nb + 1) * sizeof(uint64_t),
-                          hipMemcpyHostToDevice, s));
-    GV_HIP(hipMemcpyAsync(g->grp_kind, hk.data(), (size_t)nb * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, s));
+    rc = reserve_groups(g, nb);
+    if (rc) return rc;
     const int nst = g->has_in ? 2 : 1;
+    for (int k = 0; k < nst; ++k) {
+      rc = reserve_keys(g, k, g->directed ? n : 2 * n);
+      if (rc) return rc;
+    }
+    // Offsets and kinds go through one host block: pageable for an eager call
+    // (a pageable host-to-device copy has consumed its source when
+    // hipMemcpyAsync returns), pinned and kept alive by the capture when the
+    // copy is recorded, since every replay reads it again.
+    const size_t ob = ((size_t)nb + 1) * sizeof(uint64_t), kb = (size_t)nb * sizeof(uint32_t);
+    std::vector<uint8_t> eager_buf;
+    uint8_t *hb = nullptr;
+    if (g->capturing) {
+      void *p = nullptr;
+      GV_HIP(hipHostMalloc(&p, ob + kb, hipHostMallocDefault));
+      g->cap_pinned.push_back(p);
+      hb = (uint8_t *)p;
+      g->cap_pairs += n;
+    } else {
+      eager_buf.resize(ob + kb);
+      hb = eager_buf.data();
+    }
+    std::memcpy(hb, offsets, ob);
+    uint32_t *hk = (uint32_t *)(hb + ob);
+    for (uint32_t b = 0; b < nb; ++b) hk[b] = kinds[b] == GV_OP_DELETE ? OP_DELETE : OP_INSERT;
+    GV_HIP(hipMemcpyAsync(g->grp_off, hb, ob, hipMemcpyHostToDevice, s));
+    GV_HIP(hipMemcpyAsync(g->grp_kind, hk, kb, hipMemcpyHostToDevice, s));
     uint64_t m[2] = {0, 0};
     uint64_t *keys[2] = {nullptr, nullptr};
     for (int k = 0; k < nst; ++k) {
       const int mirror = g->directed ? 0 : 1;
-      rc = scratch_reserve(g->sc[k], g->st[k], mirror ? 2 * n : n);
-      if (rc) return rc;
       op_begin(g->st[k], s);
       keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, g->grp_off, g->grp_kind, nb, mirror,
                             k == 1 ? 1 : 0, m[k], s);
@@ -279,10 +367,17 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
 
 extern "C" {
 
+// A failure while the stream is being captured poisons the capture.
+static gv_status note_capture(gv_graph *g, gv_status rc) {
+  if (rc && g && g->capturing && !g->cap_err) g->cap_err = rc;
+  return rc;
+}
+
 gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
                          const uint64_t *offsets, const int *kinds, uint32_t n_batches, int mem,
                          float compact_fraction, gv_batch_stats *stats) {
-  return apply_group(g, src, dst, offsets, kinds, n_batches, mem, compact_fraction, stats);
+  return note_capture(
+      g, apply_group(g, src, dst, offsets, kinds, n_batches, mem, compact_fraction, stats));
 }
 
 gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed,
@@ -357,6 +452,15 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
 gv_status gv_destroy(gv_graph *g) {
   if (!g) return GV_OK;
   hipSetDevice(g->cfg.device);
+  if (g->capturing) {
+    hipGraph_t open = nullptr;
+    hipStreamEndCapture(g->stream, &open);
+    if (open) hipGraphDestroy(open);
+    g->capturing = 0;
+  }
+  for (void *p : g->cap_pinned) hipHostFree(p);
+  for (gv_capture *c : g->captures) c->g = nullptr;
+  hipFree(g->cl_cum);
   if (g->stream) hipStreamSynchronize(g->stream);
   for (int k = 0; k < 2; ++k) {
     store_release(g->st[k]);
@@ -385,17 +489,21 @@ gv_status gv_bulk_build(gv_graph *g, const uint32_t *src, const uint32_t *dst, u
 
 gv_status gv_batch_insert(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
                           int mem, gv_batch_stats *stats) {
-  return apply_batch(g, src, dst, n, mem, stats, Op::Insert);
+  return note_capture(g, apply_batch(g, src, dst, n, mem, stats, Op::Insert));
 }
 
 gv_status gv_batch_delete(gv_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t n,
                           int mem, gv_batch_stats *stats) {
-  return apply_batch(g, src, dst, n, mem, stats, Op::Delete);
+  return note_capture(g, apply_batch(g, src, dst, n, mem, stats, Op::Delete));
 }
 
 gv_status gv_compact(gv_graph *g, float frac, uint64_t *vertices_compacted) {
-  gv_status rc = check_graph(g);
+  gv_status rc = check_graph(g, true);
   if (rc) return rc;
+  if (g->capturing && vertices_compacted) {
+    set_error("a captured compaction cannot return a count; pass NULL");
+    return g->cap_err = GV_E_INVALID;
+  }
   if (frac < 0.f) frac = g->cfg.compact_threshold;
   for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
     op_begin(g->st[k], g->stream);
@@ -850,6 +958,203 @@ gv_status gv_gen_sample(gv_graph *g, uint64_t key, uint32_t V, const uint64_t *o
   if (rc) return rc;
   if (n && (!src_dev || !dst_dev || !offsets_dev)) return GV_E_INVALID;
   gen_sample(key, V, offsets_dev, cols_dev, n, src_dev, dst_dev, g->stream);
+  GV_HIP(hipGetLastError());
+  return GV_OK;
+}
+
+// ------------------------------------------------------------ capture --
+gv_status gv_reserve(gv_graph *g, uint64_t max_pairs, uint32_t max_batches) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (max_pairs > 0x7FFFFFFFull) {
+    set_error("max_pairs above 2^31");
+    return GV_E_INVALID;
+  }
+  for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
+    rc = reserve_keys(g, k, g->directed ? max_pairs : 2 * max_pairs);
+    if (rc) return rc;
+  }
+  return reserve_groups(g, max_batches);
+}
+
+gv_status gv_capture_begin(gv_graph *g) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  GV_HIP(hipStreamSynchronize(g->stream));
+  g->cap_err = GV_OK;
+  g->cap_pairs = 0;
+  GV_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
+  g->capturing = 1;
+  return GV_OK;
+}
+
+gv_status gv_capture_end(gv_graph *g, gv_capture **out) {
+  if (!g || !out) {
+    set_error("null handle");
+    return GV_E_INVALID;
+  }
+  *out = nullptr;
+  if (!g->capturing) {
+    set_error("no capture is open on this handle");
+    return GV_E_INVALID;
+  }
+  hipSetDevice(g->cfg.device);
+  g->capturing = 0;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(g->stream, &graph);
+  std::vector<void *> pinned;
+  pinned.swap(g->cap_pinned);
+  gv_status rc = g->cap_err;
+  if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  if (!rc && (e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+    rc = hip_fail(e, "hipGraphInstantiate");
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    for (void *p : pinned) hipHostFree(p);
+    if (g->cap_err) set_error("a call inside the capture failed; nothing was captured");
+    g->cap_err = GV_OK;
+    return rc;
+  }
+  gv_capture *c = new gv_capture();
+  c->g = g;
+  c->graph = graph;
+  c->exec = exec;
+  c->gen = g->buf_gen;
+  c->pairs = g->cap_pairs;
+  c->pinned.swap(pinned);
+  g->captures.push_back(c);
+  *out = c;
+  return GV_OK;
+}
+
+gv_status gv_capture_launch(gv_capture *c, gv_batch_stats *stats) {
+  if (!c) {
+    set_error("null capture");
+    return GV_E_INVALID;
+  }
+  gv_graph *g = c->g;
+  if (!g) {
+    set_error("the graph of this capture was destroyed");
+    return GV_E_INVALID;
+  }
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (c->gen != g->buf_gen) {
+    set_error("buffers used by this capture were reallocated after it was made; capture again");
+    return GV_E_INVALID;
+  }
+  GV_HIP(hipGraphLaunch(c->exec, g->stream));
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (!stats) return GV_OK;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  fill_stats(g, c->pairs, stats);
+  return take_abort(g);
+}
+
+gv_status gv_capture_destroy(gv_capture *c) {
+  if (!c) return GV_OK;
+  if (gv_graph *g = c->g) {
+    hipSetDevice(g->cfg.device);
+    hipStreamSynchronize(g->stream);
+    g->captures.erase(std::remove(g->captures.begin(), g->captures.end(), c), g->captures.end());
+  }
+  if (c->exec) hipGraphExecDestroy(c->exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (void *p : c->pinned) hipHostFree(p);
+  delete c;
+  return GV_OK;
+}
+
+// ------------------------------------------------------------- tuning --
+gv_status gv_get_config(gv_graph *g, gv_config *cfg) {
+  if (!g || !cfg) return GV_E_INVALID;
+  *cfg = g->cfg;
+  return GV_OK;
+}
+
+gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!cfg) return GV_E_INVALID;
+  if (cfg->pr_thread_max > cfg->pr_group_max || cfg->pr_group_max > cfg->pr_wave_max ||
+      cfg->pr_wave_max == 0) {
+    set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
+    return GV_E_INVALID;
+  }
+  if (cfg->pr_layout != 0 && cfg->pr_layout != 1) {
+    set_error("pr_layout must be 0 or 1");
+    return GV_E_INVALID;
+  }
+  g->cfg.compact_threshold = cfg->compact_threshold;
+  g->cfg.pr_thread_max = cfg->pr_thread_max;
+  g->cfg.pr_group_max = cfg->pr_group_max;
+  g->cfg.pr_wave_max = cfg->pr_wave_max;
+  g->cfg.pr_layout = cfg->pr_layout;
+  return GV_OK;
+}
+
+}  // extern "C"
+
+// Chung-Lu cumulative weight table: w_i = (i+1)^(-1/(gamma-1)) scaled to
+// average degree avg_deg, capped at max_deg, accumulated in vertex order in
+// FP64 and scaled to 2^62.  The operations are kept in this exact order
+// (no contraction) so every build of the table gives the same integers.
+static void chung_lu_table(uint32_t V, double gamma, double avg_deg, double max_deg,
+                           std::vector<uint64_t> &cum) {
+#pragma clang fp contract(off)
+  std::vector<double> w(V);
+  const double ex = -1.0 / (gamma - 1.0);
+  double s = 0.0;
+  for (uint32_t i = 0; i < V; ++i) {
+    w[i] = std::pow((double)i + 1.0, ex);
+    s += w[i];
+  }
+  const double scale = avg_deg * (double)V / s;
+  double tot = 0.0;
+  for (uint32_t i = 0; i < V; ++i) {
+    w[i] *= scale;
+    if (w[i] > max_deg) w[i] = max_deg;
+    tot += w[i];
+  }
+  cum.assign((size_t)V + 1, 0);
+  double acc = 0.0;
+  for (uint32_t i = 0; i < V; ++i) {
+    acc += w[i];
+    cum[(size_t)i + 1] = (uint64_t)(acc / tot * 4611686018427387904.0);
+  }
+  cum[V] = 4611686018427387904ull;
+}
+
+extern "C" {
+
+gv_status gv_gen_chung_lu(gv_graph *g, uint64_t key, uint32_t V, double gamma, double avg_deg,
+                          double max_deg, uint64_t n, uint32_t *src_dev, uint32_t *dst_dev) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (V == 0 || !(gamma > 1.0) || !(avg_deg > 0.0) || !(max_deg > 0.0) ||
+      (n && (!src_dev || !dst_dev))) {
+    set_error("Chung-Lu needs V > 0, gamma > 1, avg_deg > 0, max_deg > 0");
+    return GV_E_INVALID;
+  }
+  if (!g->cl_cum || g->cl_V != V || g->cl_gamma != gamma || g->cl_avg != avg_deg ||
+      g->cl_max != max_deg) {
+    std::vector<uint64_t> cum;
+    chung_lu_table(V, gamma, avg_deg, max_deg, cum);
+    GV_HIP(hipStreamSynchronize(g->stream));
+    hipFree(g->cl_cum);
+    g->cl_cum = nullptr;
+    g->cl_V = 0;
+    GV_HIP(hipMalloc(&g->cl_cum, cum.size() * sizeof(uint64_t)));
+    GV_HIP(hipMemcpy(g->cl_cum, cum.data(), cum.size() * sizeof(uint64_t),
+                     hipMemcpyHostToDevice));
+    g->cl_V = V;
+    g->cl_gamma = gamma;
+    g->cl_avg = avg_deg;
+    g->cl_max = max_deg;
+  }
+  gen_table(key, g->cl_cum, V, n, src_dev, dst_dev, g->stream);
   GV_HIP(hipGetLastError());
   return GV_OK;
 }

@@ -18,6 +18,8 @@ gv_status store_alloc(Store &st, uint32_t V, uint32_t B, uint64_t P, uint64_t A)
 void store_release(Store &st);
 void store_reset(Store &st, hipStream_t s);
 gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m);
+// True when scratch_reserve(sc, st, m) would not reallocate.
+bool scratch_fits(const Scratch &sc, const Store &st, uint64_t m);
 void scratch_release(Scratch &sc);
 
 // Per-op counter reset (keeps the sticky abort code).
@@ -67,6 +69,9 @@ void gen_uniform(uint64_t key, uint32_t V, uint64_t n, uint32_t *src, uint32_t *
                  hipStream_t s);
 void gen_sample(uint64_t key, uint32_t V, const uint64_t *offsets, const uint32_t *cols,
                 uint64_t n, uint32_t *src, uint32_t *dst, hipStream_t s);
+// Endpoints drawn against a device cumulative table cum[0..V] scaled to 2^62.
+void gen_table(uint64_t key, const uint64_t *cum, uint32_t V, uint64_t n, uint32_t *src,
+               uint32_t *dst, hipStream_t s);
 
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);

@@ -103,9 +103,13 @@ void store_reset(Store &st, hipStream_t s) {
                                             st.tbl_cap, st.V, st.ctl);
 }
 
-gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m) {
+bool scratch_fits(const Scratch &sc, const Store &st, uint64_t m) {
   uint64_t item_need = m + cdiv(st.P * (uint64_t)st.B, PROBE_CHUNK) + 1;
-  if (m <= sc.cap && item_need <= sc.item_cap) return GV_OK;
+  return m <= sc.cap && item_need <= sc.item_cap;
+}
+
+gv_status scratch_reserve(Scratch &sc, const Store &st, uint64_t m) {
+  if (scratch_fits(sc, st, m)) return GV_OK;
   scratch_release(sc);
   uint64_t cap = 1024;
   while (cap < m) cap <<= 1;
@@ -840,6 +844,34 @@ __global__ void k_gen_sample(uint64_t key, uint32_t V, const uint64_t *offsets,
     src[i] = lo;
     dst[i] = cols[idx];
   }
+}
+
+// Endpoint draws against a cumulative weight table cum[0..V] (cum[0] = 0,
+// cum[V] = 2^62): draw i picks the largest vertex lo with cum[lo] <= r_i >> 2.
+// Edge e takes draw 2e as its source and draw 2e+1 as its destination.
+__global__ void k_gen_table(uint64_t key, const uint64_t *cum, uint32_t V, uint64_t n,
+                            uint32_t *src, uint32_t *dst) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t out[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint64_t r = rand64(key, 2 * e + k) >> 2;
+      uint32_t lo = 0, hi = V;
+      while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (cum[mid] <= r) lo = mid; else hi = mid;
+      }
+      out[k] = lo;
+    }
+    src[e] = out[0];
+    dst[e] = out[1];
+  }
+}
+
+void gen_table(uint64_t key, const uint64_t *cum, uint32_t V, uint64_t n, uint32_t *src,
+               uint32_t *dst, hipStream_t s) {
+  if (n) k_gen_table<<<grid_for(n), 256, 0, s>>>(key, cum, V, n, src, dst);
 }
 
 void gen_rmat(uint64_t key, int scale, uint64_t n, uint32_t ta, uint32_t tab, uint32_t tabc,

@@ -61,6 +61,16 @@ class DeviceArray:
             _ok(hip().hipMemcpy(d.ptr, a.ctypes.data_as(ctypes.c_void_p), d.nbytes, H2D), "H2D")
         return d
 
+    def upload(self, a):
+        """Overwrite the first len(a) entries from a host array (synchronous;
+        the caller orders it after any stream work that reads the buffer)."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if len(a) > self.n:
+            raise ValueError("host array larger than the device buffer")
+        if len(a):
+            _ok(hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                len(a) * self.dtype.itemsize, H2D), "H2D")
+
     def numpy(self, n=None):
         n = self.n if n is None else int(n)
         out = np.empty(n, self.dtype)

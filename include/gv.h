@@ -185,6 +185,53 @@ gv_status gv_gen_sample(gv_graph *g, uint64_t key, uint32_t V, const uint64_t *o
                         const uint32_t *cols_dev, uint64_t n, uint32_t *src_dev,
                         uint32_t *dst_dev);
 
+/* Chung-Lu endpoints: draw i (draw 2e is the source of edge e, draw 2e+1 its
+ * destination) is the vertex lo with cum[lo] <= (r_i >> 2) < cum[lo+1],
+ * r_i the SplitMix64 stream value, where cum is the FP64 running sum in
+ * vertex order of w_i = (i+1)^(-1/(gamma-1)) scaled to average degree
+ * avg_deg and capped at max_deg, normalised to 2^62.  The table is built on
+ * the host once per parameter set and kept with the handle; the draws run on
+ * the device.  Synthetic input of the power-law workloads; the reference has
+ * no generator of this kind (its only generator is csr.cu:80-125). */
+gv_status gv_gen_chung_lu(gv_graph *g, uint64_t key, uint32_t V, double gamma, double avg_deg,
+                          double max_deg, uint64_t n, uint32_t *src_dev, uint32_t *dst_dev);
+
+/* Tuning.  gv_set_tuning changes, on a live handle, the fields that do not
+ * move memory: compact_threshold, pr_thread_max, pr_group_max, pr_wave_max
+ * and pr_layout (the other fields of cfg are ignored).  Used by the
+ * autotuner; no reference counterpart (the reference's block size and
+ * launch shapes are compile-time constants, constants.cuh:4-17). */
+gv_status gv_get_config(gv_graph *g, gv_config *cfg);
+gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg);
+
+/* Captured pipelines.  Records update and compaction calls issued on the
+ * handle's stream into one HIP graph that gv_capture_launch replays with a
+ * single launch.  No reference counterpart (the reference runs each stage
+ * synchronously, graphvine.cu:222-464); for callers that replay a fixed
+ * update -> compact sequence over device buffers they refill between
+ * launches.
+ *   - gv_reserve sizes the staging buffers for groups of up to max_pairs
+ *     (src,dst) pairs and max_batches batches: a call inside a capture never
+ *     allocates device memory and fails if the reservation is too small;
+ *   - between gv_capture_begin and gv_capture_end only gv_batch_insert,
+ *     gv_batch_delete and gv_batch_apply with GV_MEM_DEVICE inputs and NULL
+ *     stats, and gv_compact with a NULL count, are allowed; every other call
+ *     on the handle fails and a failed call makes gv_capture_end fail;
+ *   - a replay reads the device inputs at the addresses given at capture time
+ *     (their contents may change between launches) and the batch offsets and
+ *     kinds given at capture time;
+ *   - a later reallocation of a buffer the capture uses (a group larger than
+ *     reserved) invalidates it: gv_capture_launch then returns GV_E_INVALID;
+ *   - gv_capture_launch with stats waits for the replay; the counters in
+ *     stats describe its last update call, live_edges the whole graph.
+ * A capture outliving its graph can only be destroyed. */
+typedef struct gv_capture gv_capture;
+gv_status gv_reserve(gv_graph *g, uint64_t max_pairs, uint32_t max_batches);
+gv_status gv_capture_begin(gv_graph *g);
+gv_status gv_capture_end(gv_graph *g, gv_capture **out);
+gv_status gv_capture_launch(gv_capture *c, gv_batch_stats *stats);
+gv_status gv_capture_destroy(gv_capture *c);
+
 /* Stream plumbing. */
 gv_status gv_sync(gv_graph *g);
 gv_status gv_get_stream(gv_graph *g, void **hip_stream);

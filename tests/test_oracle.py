@@ -102,6 +102,24 @@ def test_generator_golden_digest():
         assert s[:8].tolist() == case["src_head"] and d[:8].tolist() == case["dst_head"]
         k = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
         assert int(np.bitwise_xor.reduce(k * np.uint64(0x9E3779B97F4A7C15))) == case["xor_digest"]
+    for case in gold["chung_lu"]:
+        s, d = O.gen_chung_lu(O.stream_key(case["seed"], O.STREAM_CHUNG_LU), case["V"],
+                              case["gamma"], case["avg_deg"], case["max_deg"], case["n"])
+        assert s[:8].tolist() == case["src_head"] and d[:8].tolist() == case["dst_head"]
+        k = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+        assert int(np.bitwise_xor.reduce(k * np.uint64(0x9E3779B97F4A7C15))) == case["xor_digest"]
+
+
+def test_chung_lu_degree_law():
+    """Endpoint frequencies follow the capped power-law weights: the heaviest
+    vertex's share of 2n draws matches w_0 / sum(w) within sampling error."""
+    V, n, gamma, avg, mx = 20000, 200000, 2.1, 24.0, 1e9
+    s, d = O.gen_chung_lu(O.stream_key(7, O.STREAM_CHUNG_LU), V, gamma, avg, mx, n)
+    w = (np.arange(V, dtype=np.float64) + 1.0) ** (-1.0 / (gamma - 1.0))
+    p0 = w[0] / w.sum()
+    c0 = int((s == 0).sum() + (d == 0).sum())
+    assert abs(c0 - 2 * n * p0) < 5 * np.sqrt(2 * n * p0)
+    assert s.max() < V and d.max() < V
 
 
 # ------------------------------------------------------------------ store --
