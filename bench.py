@@ -48,7 +48,29 @@ def parse():
     p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
     p.add_argument("--pr-layout", type=int, default=1,
                    help="PageRank layout: 1 = pull over the store, 0 = XCD-segmented snapshot")
+    p.add_argument("--workload", default="rmat22",
+                   choices=["rmat22", "chunglu", "precision", "serve"],
+                   help="rmat22: plain vs batched update stream on undirected R-MAT (default); "
+                        "chunglu: 256 uneven batches on a Chung-Lu graph (plain, batched, "
+                        "captured), 64-column personalized PageRank, BC from 256 sources; "
+                        "precision: compensated vs plain PageRank and BC throughput on directed "
+                        "R-MAT scale 24; serve: request loop with Poisson arrivals")
+    p.add_argument("--autotune", type=int, default=1,
+                   help="chunglu: 1 = sweep block size and PageRank bins first")
+    p.add_argument("--cl-vertices", type=int, default=8_000_000)
+    p.add_argument("--cl-batches", type=int, default=256)
     return p.parse_args()
+
+
+def skey(seed, stream):
+    """Stream key of the counter-based generators: mix(mix(seed) ^ stream*C)."""
+    M = (1 << 64) - 1
+
+    def fmix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    return fmix(fmix(seed) ^ ((stream * 0xD6E8FEB86659FD93) & M))
 
 
 def main():
@@ -57,21 +79,25 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "chunglu":
+        from gv_amd.workloads import run_chunglu
+        out = run_chunglu(args)
+    elif args.workload == "precision":
+        from gv_amd.workloads import run_precision
+        out = run_precision(args)
+    elif args.workload == "serve":
+        from gv_amd.workloads import run_serve
+        out = run_serve(args)
+    else:
+        out = run_rmat(args)
+    print(json.dumps(out), flush=True)
 
+
+def run_rmat(args):
     import gv_amd as G
     from gv_amd.hip import DeviceArray, Event, synchronize
 
     STREAM_RMAT, STREAM_INS, STREAM_DEL = 1, 2, 3
-
-    def skey(seed, stream):
-        # same derivation as the host restatement: mix(mix(seed) ^ stream*C)
-        M = (1 << 64) - 1
-
-        def fmix(z):
-            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
-            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
-            return z ^ (z >> 31)
-        return fmix(fmix(seed) ^ ((stream * 0xD6E8FEB86659FD93) & M))
 
     V = 1 << args.scale
     n_raw = args.edge_factor * V
@@ -207,7 +233,7 @@ def main():
     g.close()
     if args.batched:
         out["batched"] = run_batched(args, G, skey, V, n_raw, batches, E_end, value)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):

@@ -452,6 +452,19 @@ def bc(graph, sources):
     return out
 
 
+def stream_key(seed, stream):
+    """Key of generator stream `stream` under `seed`: mix(mix(seed) ^ stream*C)
+    with the SplitMix64 finaliser, so every (seed, stream) pair names an
+    independent counter-based stream."""
+    M = (1 << 64) - 1
+
+    def fmix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    return fmix(fmix(int(seed)) ^ ((int(stream) * 0xD6E8FEB86659FD93) & M))
+
+
 # Device generators (counter-based SplitMix64 streams; identical streams to
 # the host restatement, so batches can be produced on the GPU at scale).
 def gen_rmat(graph, key, scale, n, a=0.57, b=0.19, c=0.19):

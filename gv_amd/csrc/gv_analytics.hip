@@ -730,9 +730,15 @@ __global__ __launch_bounds__(256) void k_bc_fwd(Store st, int *dist, double *sig
       if (j < us) {
         w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
         if (w != TOMB) {
-          int old = atomicCAS(&dist[w], -1, d + 1);
-          fresh = old == -1;
-          if (fresh || old == d + 1) atomicAdd(&sigma[w], su);
+          // dist moves from -1 to a level once, so a plain read of any other
+          // value is final; only a -1 needs the CAS to decide.
+          int cur = dist[w];
+          if (cur == -1) {
+            const int old = atomicCAS(&dist[w], -1, d + 1);
+            fresh = old == -1;
+            cur = fresh ? d + 1 : old;
+          }
+          if (cur == d + 1) atomicAdd(&sigma[w], su);
         }
       }
       unsigned long long m = __ballot(fresh);

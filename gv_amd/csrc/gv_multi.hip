@@ -236,7 +236,8 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
 }
 
 // ================================================================ BC ==
-constexpr int BCM_S = 32;  // sources per batch = lanes per frontier vertex
+constexpr int BCM_S = 32;   // sources per batch = lanes per frontier vertex
+constexpr int BCM_ILP = 8;  // neighbour rows requested before the first is used
 
 __global__ void k_bcm_init(int *dist, double *sigma, float *delta, uint32_t *stamp, uint32_t V) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)V * BCM_S;
@@ -259,46 +260,104 @@ __global__ void k_bcm_src(const uint32_t *src, uint32_t ns, int *dist, double *s
   if (atomicMax(&stamp[v], 1u) < 1u) queue[atomicAdd(cursor, 1u)] = v;
 }
 
+// Work items: a frontier vertex with more than BCM_CH slots is split into
+// chunks of BCM_CH slots, each its own item, so a hub's adjacency is walked
+// by many lane groups at once.  cnt/coff hold the per-vertex item counts and
+// their exclusive scan (coff[nf] = items of the level).
+constexpr uint32_t BCM_CH = 256;
+
+__global__ void k_bcm_count(Store st, const uint32_t *q, uint32_t nf, uint32_t *cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = st.used[q[i]];
+    cnt[i] = u > BCM_CH ? (u + BCM_CH - 1) / BCM_CH : 1u;
+  }
+}
+
+// Frontier index j of item i: the largest j < nf with coff[j] <= i.
+__device__ __forceinline__ uint32_t bcm_owner(const uint32_t *coff, uint32_t nf, uint32_t i) {
+  uint32_t lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (coff[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One item per lane group: the group's vertex, its slot range [c0, c1), and
+// the lane's (dist, sigma) for source s.
+struct BcmItem {
+  uint32_t v, tbl, c0, c1;
+  int dv;
+  double sv;
+};
+
+__device__ __forceinline__ BcmItem bcm_item(const Store &st, const int *dist, const double *sigma,
+                                            const uint32_t *queue, uint32_t qs,
+                                            const uint32_t *coff, uint32_t nf, uint64_t gi,
+                                            bool have, uint32_t s) {
+  BcmItem it{0, 0, 0, 0, -2, 0.0};
+  if (have) {
+    const uint32_t j = bcm_owner(coff, nf, (uint32_t)gi);
+    it.v = queue[qs + j];
+    const uint32_t us = st.used[it.v];
+    it.tbl = st.tbl_off[it.v];
+    it.c0 = ((uint32_t)gi - coff[j]) * BCM_CH;
+    it.c1 = it.c0 + BCM_CH < us ? it.c0 + BCM_CH : us;
+    it.dv = dist[(uint64_t)it.v * BCM_S + s];
+    it.sv = sigma[(uint64_t)it.v * BCM_S + s];
+  }
+  return it;
+}
+
 __global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *sigma,
                                                  uint32_t *stamp, uint32_t *queue, uint32_t qs,
-                                                 uint32_t qe, int d, unsigned int *cursor) {
+                                                 uint32_t qe, int d, unsigned int *cursor,
+                                                 const uint32_t *coff) {
   const uint32_t lane = lane_id();
-  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;  // two vertices per wavefront
+  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;  // two items per wavefront
   const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
   const uint32_t nf = qe - qs;
+  const uint32_t nitems = coff[nf];
   for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
-       g0 < nf; g0 += ngroups) {
+       g0 < nitems; g0 += ngroups) {
     const uint64_t gi = g0 + sub;
-    const bool have = gi < nf;
-    uint32_t v = 0, us = 0, tbl = 0;
-    int dv = -2;
-    double sv = 0.0;
-    if (have) {
-      v = queue[qs + gi];
-      us = st.used[v];
-      tbl = st.tbl_off[v];
-      dv = dist[(uint64_t)v * BCM_S + s];
-      sv = sigma[(uint64_t)v * BCM_S + s];
-    }
-    const bool act = dv == d;
-    // both half-waves loop to the larger slot count; shuffles stay uniform
-    const uint32_t us_max = max(us, __shfl_xor(us, BCM_S, WAVE));
-    for (uint32_t base = 0; base < us_max; base += BCM_S) {
+    const BcmItem it = bcm_item(st, dist, sigma, queue, qs, coff, nf, gi, gi < nitems, s);
+    const bool act = it.dv == d;
+    const uint32_t span = it.c1 - it.c0;
+    // both half-waves loop to the larger span; shuffles stay uniform
+    const uint32_t span_max = max(span, __shfl_xor(span, BCM_S, WAVE));
+    for (uint32_t base = 0; base < span_max; base += BCM_S) {
       uint32_t slot = TOMB;
-      if (base + s < us) slot = st.slots[slot_index(st.arena, tbl, base + s, st.log2B, st.B)];
-      for (uint32_t t = 0; t < (uint32_t)BCM_S; ++t) {
-        const uint32_t w = __shfl(slot, (int)t, BCM_S);
-        bool fresh = false;
-        if (act && w != TOMB) {
-          int old = atomicCAS(&dist[(uint64_t)w * BCM_S + s], -1, d + 1);
-          fresh = old == -1;
-          if (fresh || old == d + 1) atomicAdd(&sigma[(uint64_t)w * BCM_S + s], sv);
-        }
-        const unsigned long long m = __ballot(fresh);
-        const unsigned long long mine = (m >> (sub * BCM_S)) & 0xFFFFFFFFull;
-        if (s == 0 && mine) {
-          if (atomicMax(&stamp[w], (uint32_t)(d + 2)) < (uint32_t)(d + 2))
-            queue[qe + atomicAdd(cursor, 1u)] = w;
+      if (base + s < span)
+        slot = st.slots[slot_index(st.arena, it.tbl, it.c0 + base + s, st.log2B, st.B)];
+      for (uint32_t t0 = 0; t0 < (uint32_t)BCM_S; t0 += BCM_ILP) {
+        // BCM_ILP neighbours' distance rows are requested before any is used
+        uint32_t w[BCM_ILP];
+        int cur[BCM_ILP];
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) w[k] = __shfl(slot, (int)(t0 + k), BCM_S);
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k)
+          cur[k] = (act && w[k] != TOMB) ? dist[(uint64_t)w[k] * BCM_S + s] : -2;
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) {
+          // dist moves from -1 to a level once per batch, so a plain read of
+          // any other value is final; only a -1 needs the CAS to decide.
+          bool fresh = false;
+          const uint64_t wi = (uint64_t)w[k] * BCM_S + s;
+          if (cur[k] == -1) {
+            const int old = atomicCAS(&dist[wi], -1, d + 1);
+            fresh = old == -1;
+            cur[k] = fresh ? d + 1 : old;
+          }
+          if (cur[k] == d + 1) atomicAdd(&sigma[wi], it.sv);
+          const unsigned long long m = __ballot(fresh);
+          const unsigned long long mine = (m >> (sub * BCM_S)) & 0xFFFFFFFFull;
+          if (s == 0 && mine) {
+            if (atomicMax(&stamp[w[k]], (uint32_t)(d + 2)) < (uint32_t)(d + 2))
+              queue[qe + atomicAdd(cursor, 1u)] = w[k];
+          }
         }
       }
     }
@@ -313,10 +372,63 @@ __device__ __forceinline__ void bcm_merge(float &s, float &c, float s2, float c2
   c = c + c2 - e;
 }
 
+// Backward, per item: Kahan sum over the item's successors of
+// sigma_v/sigma_w (1 + delta_w), written as a (sum, compensation) partial.
 __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, const double *sigma,
-                                                 float *delta, const uint32_t *queue, uint32_t qs,
-                                                 uint32_t qe, int d, const uint32_t *src,
-                                                 uint32_t ns, float *bc, float *bcc) {
+                                                 const float *delta, const uint32_t *queue,
+                                                 uint32_t qs, uint32_t qe, int d,
+                                                 const uint32_t *coff, float2 *part) {
+  const uint32_t lane = lane_id();
+  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
+  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
+  const uint32_t nf = qe - qs;
+  const uint32_t nitems = coff[nf];
+  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
+       g0 < nitems; g0 += ngroups) {
+    const uint64_t gi = g0 + sub;
+    const bool have = gi < nitems;
+    const BcmItem it = bcm_item(st, dist, sigma, queue, qs, coff, nf, gi, have, s);
+    const bool act = it.dv == d;
+    float ds = 0.f, dc = 0.f;
+    const uint32_t span = it.c1 - it.c0;
+    const uint32_t span_max = max(span, __shfl_xor(span, BCM_S, WAVE));
+    for (uint32_t base = 0; base < span_max; base += BCM_S) {
+      uint32_t slot = TOMB;
+      if (base + s < span)
+        slot = st.slots[slot_index(st.arena, it.tbl, it.c0 + base + s, st.log2B, st.B)];
+      for (uint32_t t0 = 0; t0 < (uint32_t)BCM_S; t0 += BCM_ILP) {
+        // rows of BCM_ILP neighbours in flight; sums stay in neighbour order
+        uint32_t w[BCM_ILP];
+        bool hit[BCM_ILP];
+        double sw[BCM_ILP];
+        float dw[BCM_ILP];
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) w[k] = __shfl(slot, (int)(t0 + k), BCM_S);
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k)
+          hit[k] = act && w[k] != TOMB && dist[(uint64_t)w[k] * BCM_S + s] == d + 1;
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) {
+          const uint64_t wi = (uint64_t)w[k] * BCM_S + s;
+          sw[k] = hit[k] ? sigma[wi] : 1.0;
+          dw[k] = hit[k] ? delta[wi] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k)
+          if (hit[k]) kahan_add(ds, dc, (float)(it.sv / sw[k]) * (1.0f + dw[k]));
+      }
+    }
+    if (have) part[gi * BCM_S + s] = make_float2(ds, dc);
+  }
+}
+
+// Per frontier vertex: merge its items' partials in chunk order, store
+// delta, then add the S dependencies to BC[v] in a fixed lane order.
+__global__ __launch_bounds__(256) void k_bcm_fin(const uint32_t *queue, uint32_t qs, uint32_t qe,
+                                                 int d, const int *dist, float *delta,
+                                                 const float2 *part, const uint32_t *coff,
+                                                 const uint32_t *src, uint32_t ns, float *bc,
+                                                 float *bcc) {
   const uint32_t lane = lane_id();
   const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
   const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
@@ -324,36 +436,21 @@ __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, cons
   const uint32_t my_src = s < ns ? src[s] : 0xFFFFFFFFu;
   for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
        g0 < nf; g0 += ngroups) {
-    const uint64_t gi = g0 + sub;
-    const bool have = gi < nf;
-    uint32_t v = 0, us = 0, tbl = 0;
-    int dv = -2;
-    double sv = 0.0;
-    if (have) {
-      v = queue[qs + gi];
-      us = st.used[v];
-      tbl = st.tbl_off[v];
-      dv = dist[(uint64_t)v * BCM_S + s];
-      sv = sigma[(uint64_t)v * BCM_S + s];
-    }
-    const bool act = dv == d;
+    const uint64_t j = g0 + sub;
+    const bool have = j < nf;
+    uint32_t v = 0;
+    bool act = false;
     float ds = 0.f, dc = 0.f;
-    const uint32_t us_max = max(us, __shfl_xor(us, BCM_S, WAVE));
-    for (uint32_t base = 0; base < us_max; base += BCM_S) {
-      uint32_t slot = TOMB;
-      if (base + s < us) slot = st.slots[slot_index(st.arena, tbl, base + s, st.log2B, st.B)];
-      for (uint32_t t = 0; t < (uint32_t)BCM_S; ++t) {
-        const uint32_t w = __shfl(slot, (int)t, BCM_S);
-        if (act && w != TOMB && dist[(uint64_t)w * BCM_S + s] == d + 1) {
-          const uint64_t wi = (uint64_t)w * BCM_S + s;
-          float coef = (float)(sv / sigma[wi]);
-          kahan_add(ds, dc, coef * (1.0f + delta[wi]));
-        }
+    if (have) {
+      v = queue[qs + j];
+      act = dist[(uint64_t)v * BCM_S + s] == d;
+      for (uint32_t i = coff[j]; i < coff[j + 1]; ++i) {
+        const float2 p = part[(uint64_t)i * BCM_S + s];
+        bcm_merge(ds, dc, p.x, p.y);
       }
     }
-    float dvl = act ? ds - dc : 0.f;
+    const float dvl = act ? ds - dc : 0.f;
     if (act) delta[(uint64_t)v * BCM_S + s] = dvl;
-    // add this vertex's dependencies over the batch's sources in lane order
     float bs = (act && v != my_src) ? dvl : 0.f, bcomp = 0.f;
 #pragma unroll
     for (int k = BCM_S / 2; k >= 1; k >>= 1) {
@@ -369,6 +466,15 @@ __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, cons
   }
 }
 
+__global__ void k_used_sum(Store st, unsigned long long *tot) {
+  unsigned long long acc = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < st.V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    acc += st.used[v];
+  acc = block_sum<unsigned long long, 256>(acc);
+  if (threadIdx.x == 0 && acc) atomicAdd(tot, acc);
+}
+
 gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
                        float *out_dev, hipStream_t s) {
   const uint32_t V = st.V;
@@ -376,24 +482,61 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
   int *dist = nullptr;
   double *sigma = nullptr;
   float *delta = nullptr, *bcc = nullptr;
-  uint32_t *stamp = nullptr, *queue = nullptr, *dsrc = nullptr;
+  float2 *part = nullptr;
+  uint32_t *stamp = nullptr, *queue = nullptr, *dsrc = nullptr, *cnt = nullptr, *coff = nullptr;
   unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  unsigned long long *used_tot = nullptr, h_used = 0;
+  void *tmp = nullptr;
   gv_status rc = GV_OK;
   const uint64_t n2 = (uint64_t)V * BCM_S;
   do {
     hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&used_tot, sizeof(unsigned long long))) != hipSuccess) {
+      rc = hip_fail(e, "bc alloc");
+      break;
+    }
+    fill_u64((uint64_t *)used_tot, 0, 1, s);
+    k_used_sum<<<grid_for(V, 256, 1024), 256, 0, s>>>(st, used_tot);
+    if ((e = hipMemcpyAsync(&h_used, used_tot, sizeof(h_used), hipMemcpyDeviceToHost, s)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      rc = hip_fail(e, "bc");
+      break;
+    }
+    // items of one level: at most one per frontier vertex plus one per BCM_CH slots
+    const uint64_t max_items = (uint64_t)V + h_used / BCM_CH + 1;
     if ((e = hipMalloc(&dist, n2 * sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&sigma, n2 * sizeof(double))) != hipSuccess ||
         (e = hipMalloc(&delta, n2 * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&bcc, (size_t)V * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&stamp, (size_t)V * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&queue, n2 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&cnt, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&coff, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&part, max_items * BCM_S * sizeof(float2))) != hipSuccess ||
+        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess ||
         (e = hipMalloc(&dsrc, BCM_S * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&cursor, sizeof(unsigned int))) != hipSuccess ||
         (e = hipHostMalloc(&h_cursor, sizeof(unsigned int), hipHostMallocDefault)) != hipSuccess) {
       rc = hip_fail(e, "bc alloc");
       break;
     }
+    if (max_items > 0xFFFFFFFFull) {
+      set_error("bc: too many work items for 32-bit offsets");
+      rc = GV_E_INVALID;
+      break;
+    }
+    // item offsets of the frontier segment [qs, qe); a level's frontier holds
+    // each vertex at most once, so nf <= V
+    auto level_items = [&](uint32_t qs, uint32_t qe) {
+      const uint32_t nf = qe - qs;
+      k_bcm_count<<<grid_for(nf), 256, 0, s>>>(st, queue + qs, nf, cnt);
+      scan_u32(cnt, coff, nf, nullptr, tmp, s);
+    };
+    auto item_grid = [&](uint32_t nf) {
+      const uint64_t bound = (uint64_t)nf + h_used / BCM_CH + 1;
+      return grid_for(bound * BCM_S, 256, 8192);
+    };
     fill_f32(out_dev, 0.f, V, s);
     fill_f32(bcc, 0.f, V, s);
     for (uint32_t b0 = 0; b0 < n_sources && rc == GV_OK; b0 += BCM_S) {
@@ -418,8 +561,9 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
         const uint32_t qs = lvl[d], qe = lvl[d + 1];
         if (qe == qs) break;
         fill_u32(cursor, 0, 1, s);
-        k_bcm_fwd<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
-            st, dist, sigma, stamp, queue, qs, qe, d, cursor);
+        level_items(qs, qe);
+        k_bcm_fwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, stamp, queue, qs, qe, d,
+                                                     cursor, coff);
         if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
                 hipSuccess ||
             (e = hipStreamSynchronize(s)) != hipSuccess) {
@@ -437,8 +581,11 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
       const int D = (int)lvl.size() - 2;
       for (int lev = D - 2; lev >= 0; --lev) {
         const uint32_t qs = lvl[lev], qe = lvl[lev + 1];
-        k_bcm_bwd<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
-            st, dist, sigma, delta, queue, qs, qe, lev, dsrc, ns, out_dev, bcc);
+        level_items(qs, qe);
+        k_bcm_bwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, delta, queue, qs, qe, lev,
+                                                     coff, part);
+        k_bcm_fin<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
+            queue, qs, qe, lev, dist, delta, part, coff, dsrc, ns, out_dev, bcc);
       }
     }
     if (rc == GV_OK && (hipStreamSynchronize(s) != hipSuccess)) rc = hip_fail(hipGetLastError(), "bc");
@@ -447,10 +594,15 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
   hipFree(sigma);
   hipFree(delta);
   hipFree(bcc);
+  hipFree(part);
   hipFree(stamp);
   hipFree(queue);
+  hipFree(cnt);
+  hipFree(coff);
+  hipFree(tmp);
   hipFree(dsrc);
   hipFree(cursor);
+  hipFree(used_tot);
   if (h_cursor) hipHostFree(h_cursor);
   return rc;
 }

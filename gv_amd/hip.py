@@ -61,6 +61,18 @@ class DeviceArray:
             _ok(hip().hipMemcpy(d.ptr, a.ctypes.data_as(ctypes.c_void_p), d.nbytes, H2D), "H2D")
         return d
 
+    def view(self, start, n):
+        """Non-owning view of entries [start, start+n) (valid while self lives)."""
+        start, n = int(start), int(n)
+        if start < 0 or n < 0 or start + n > self.n:
+            raise ValueError("view out of range")
+        v = DeviceArray.__new__(DeviceArray)
+        v.dtype = self.dtype
+        v.n = n
+        v.ptr = ctypes.c_void_p((self.ptr.value or 0) + start * self.dtype.itemsize)
+        v._base = self
+        return v
+
     def upload(self, a):
         """Overwrite the first len(a) entries from a host array (synchronous;
         the caller orders it after any stream work that reads the buffer)."""
@@ -80,6 +92,10 @@ class DeviceArray:
         return out
 
     def free(self):
+        if getattr(self, "_base", None) is not None:
+            self._base = None  # a view owns nothing
+            self.ptr = ctypes.c_void_p()
+            return
         if self.ptr:
             hip().hipFree(self.ptr)
             self.ptr = ctypes.c_void_p()

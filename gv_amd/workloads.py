@@ -1,0 +1,208 @@
+"""Benchmark drivers for the store's larger workloads (the reference's driver
+is main.cu:20-134, whose benchmark switch :76-128 is dead code).
+
+Each run_* function builds its synthetic input on the device, times the
+workload with HIP events on the graph's stream and returns one JSON-ready
+dict.  Nothing here checks results against a host restatement; the parity
+tests do that at sizes a CPU finishes in seconds.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+STREAM_RMAT = 1
+STREAM_CHUNG_LU = 4
+
+
+def _events():
+    from .hip import Event
+    return Event(), Event()
+
+
+def _timed(g, fn):
+    """Run fn() (which enqueues work on g's stream) between two events and
+    return (result, elapsed ms)."""
+    e0, e1 = _events()
+    s = g.stream()
+    e0.record(s)
+    r = fn()
+    e1.record(s)
+    e1.synchronize()
+    return r, e0.elapsed_ms(e1)
+
+
+# ------------------------------------------------------------- Chung-Lu --
+def chung_lu_stream(V, n_batches, seed=4):
+    """Sizes (log-uniform in [1e3, 1e6]) and kinds (insert with probability
+    0.7) of the Chung-Lu update stream, from a seeded host generator."""
+    from . import GV_OP_DELETE, GV_OP_INSERT
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n_batches)).astype(np.int64)
+    kinds = [GV_OP_INSERT if r < 0.7 else GV_OP_DELETE for r in rng.random(n_batches)]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    return sizes, kinds, offs
+
+
+def run_chunglu(args):
+    """Chung-Lu power-law graph (gamma 2.1, average degree 24, degree cap
+    200k, undirected, seed 4) and a stream of uneven batches (log-uniform
+    sizes in [1k, 1M], 70% inserts drawn from the same weights, 30% deletes
+    sampled from the built graph).  The stream runs three ways on fresh
+    copies of the same graph:
+      plain:    one gv_batch_insert / gv_batch_delete per batch, compaction
+                (25% tombstones) after every delete batch;
+      batched:  one gv_batch_apply over all batches + one compaction;
+      captured: the batched pipeline recorded once (gv_capture_*) and
+                replayed by one graph launch.
+    The three final graphs must be identical.  Then 64 personalized PageRank
+    columns and BC from 256 sources (32 per pass) run on the result."""
+    from . import (GV_MEM_DEVICE, GV_OP_INSERT, GraphVine, _check, default_config,
+                   export_csr_device, gen_chung_lu, lib, stream_key)
+    from .autotune import tune
+    from .hip import DeviceArray
+
+    V = int(args.cl_vertices)
+    seed = 4
+    gamma, avg, mx = 2.1, 24.0, 200_000.0
+    n_graph = int(V * avg / 2)
+    nb = int(args.cl_batches)
+    sizes, kinds, offs = chung_lu_stream(V, nb, seed)
+    total = int(offs[-1])
+    t_setup = time.time()
+
+    g = GraphVine(V, directed=False)
+    gsrc, gdst = gen_chung_lu(g, stream_key(seed, STREAM_CHUNG_LU), V, n_graph, gamma, avg, mx)
+    g.bulkBuild(gsrc, gdst, stats=False)
+    off, cols, E0 = export_csr_device(g)
+    cs, cd = DeviceArray(total, np.uint32), DeviceArray(total, np.uint32)
+    for b in range(nb):
+        o, n = int(offs[b]), int(sizes[b])
+        vs, vd = cs.view(o, n), cd.view(o, n)
+        if kinds[b] == GV_OP_INSERT:
+            _check(lib().gv_gen_chung_lu(g.handle, stream_key(seed, 6000 + b), V, gamma, avg, mx,
+                                         n, vs.ptr, vd.ptr))
+        else:
+            _check(lib().gv_gen_sample(g.handle, stream_key(seed, 7000 + b), V, off.ptr,
+                                       cols.ptr, n, vs.ptr, vd.ptr))
+    g.sync()
+    off.free()
+    cols.free()
+    g.close()
+    setup_s = time.time() - t_setup
+
+    def build(h):
+        h.bulkBuild(gsrc, gdst, stats=False)
+
+    def batched(h):
+        h.applyBatches(cs, cd, offs, kinds, compact_fraction=0.25, stats=False)
+
+    tuned = None
+    cfg_kw = {}
+    if getattr(args, "autotune", 1):
+        tuned = tune(V, False, build, batched)
+        b = tuned["best"]
+        cfg_kw = {"block_slots": b["block_slots"], "pr_thread_max": b["pr_thread_max"],
+                  "pr_group_max": b["pr_group_max"], "pr_wave_max": b["pr_wave_max"]}
+
+    def fresh():
+        h = GraphVine(V, directed=False, config=default_config(**cfg_kw))
+        build(h)
+        h.sync()
+        return h
+
+    forms = {}
+    # plain
+    h = fresh()
+
+    def plain():
+        for b in range(nb):
+            o, n = int(offs[b]), int(sizes[b])
+            vs, vd = cs.view(o, n), cd.view(o, n)
+            if kinds[b] == GV_OP_INSERT:
+                _check(lib().gv_batch_insert(h.handle, vs.ptr, vd.ptr, n, GV_MEM_DEVICE, None))
+            else:
+                _check(lib().gv_batch_delete(h.handle, vs.ptr, vd.ptr, n, GV_MEM_DEVICE, None))
+                _check(lib().gv_compact(h.handle, 0.25, None))
+    _, ms = _timed(h, plain)
+    h.sync()
+    forms["plain"] = {"ms": ms, "edge_updates_per_s": total / (ms / 1e3)}
+    ref_off, ref_cols = h.export_csr()
+    E_end = h.num_edges()
+    h.close()
+    # batched
+    h = fresh()
+    _, ms = _timed(h, lambda: batched(h))
+    h.sync()
+    forms["batched"] = {"ms": ms, "edge_updates_per_s": total / (ms / 1e3)}
+    o2, c2 = h.export_csr()
+    eq_batched = bool(np.array_equal(o2, ref_off) and np.array_equal(c2, ref_cols))
+    del o2, c2
+    h.close()
+    # captured
+    h = fresh()
+    h.reserve(total, nb)
+    with h.capture() as cap:
+        batched(h)
+    c = cap.result
+    _, ms = _timed(h, c.launch)
+    h.sync()
+    forms["captured"] = {"ms": ms, "edge_updates_per_s": total / (ms / 1e3)}
+    o3, c3 = h.export_csr()
+    eq_captured = bool(np.array_equal(o3, ref_off) and np.array_equal(c3, ref_cols))
+    del o3, c3, ref_off, ref_cols
+    c.close()
+
+    # analytics on the updated graph
+    rng = np.random.default_rng(seed + 100)
+    E = h.num_edges()
+    seeds = DeviceArray.from_numpy(rng.choice(V, 64, replace=False).astype(np.uint32))
+    X = DeviceArray(V * 64, np.float32)
+    it, res = ctypes.c_int(), ctypes.c_double()
+    _, ppr_ms = _timed(h, lambda: _check(lib().gv_pagerank_personalized(
+        h.handle, 0.85, 1e-6, 100, seeds.ptr, 64, X.ptr, GV_MEM_DEVICE, ctypes.byref(it),
+        ctypes.byref(res))))
+    X.free()
+    srcs = DeviceArray.from_numpy(rng.choice(V, 256, replace=False).astype(np.uint32))
+    bco = DeviceArray(V, np.float32)
+    _, bc_ms = _timed(h, lambda: _check(lib().gv_bc(h.handle, srcs.ptr, 256, bco.ptr,
+                                                      GV_MEM_DEVICE)))
+    bco.free()
+    h.close()
+
+    cap_rate = forms["captured"]["edge_updates_per_s"]
+    return {
+        "metric": "edge updates/s (Chung-Lu stream of uneven batches, captured batched form)",
+        "value": cap_rate,
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": forms["captured"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (updates), f32 (personalized PageRank), f64 sigma + f32 delta (BC)",
+        "data": "synthetic (device-generated Chung-Lu graph and update batches)",
+        "config": {
+            "workload": f"Chung-Lu V={V}, gamma {gamma}, avg degree {avg}, cap {mx:.0f}, "
+                        f"undirected, seed {seed}; {nb} batches log-uniform in [1k, 1M], "
+                        "70% insert / 30% delete; compaction at 25% tombstones",
+            "vertices": V,
+            "directed_edges_initial": int(E0),
+            "directed_edges_final": int(E_end),
+            "updates_total": total,
+            "block_slots": cfg_kw.get("block_slots", 64),
+        },
+        "forms": forms,
+        "speedup_batched_vs_plain": forms["batched"]["edge_updates_per_s"]
+        / forms["plain"]["edge_updates_per_s"],
+        "speedup_captured_vs_plain": cap_rate / forms["plain"]["edge_updates_per_s"],
+        "final_graphs_equal": {"batched": eq_batched, "captured": eq_captured},
+        "personalized_pagerank_64": {"ms": ppr_ms, "iters": int(it.value),
+                                     "max_residual": float(res.value),
+                                     "gteps_columns": 64.0 * E * it.value / (ppr_ms / 1e3) / 1e9},
+        "bc_256_sources": {"ms": bc_ms, "batches_of": 32},
+        "autotune": tuned,
+        "setup_s": round(setup_s, 2),
+    }
