@@ -67,3 +67,37 @@ def test_cpp_facade_compiles_and_links(tmp_path):
                         "-L", lib_dir, "-lgv", "-o", str(tmp_path / "use_gv")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_c(tmp_path):
+    """sizeof and every field offset of the public structs, compiled by the
+    host C compiler from the headers, equal the ctypes mirrors."""
+    import shutil
+    import subprocess
+    import pytest
+    from gv_amd import serve
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    pairs = [("gv_config", gv_amd.GvConfig), ("gv_batch_stats", gv_amd.GvBatchStats),
+             ("gv_pagerank_stats", gv_amd.GvPagerankStats),
+             ("gv_pagerank_opts", gv_amd.GvPagerankOpts), ("gv_request", serve.GvRequest),
+             ("gv_response", serve.GvResponse), ("gv_server_opts", serve.GvServerOpts),
+             ("gv_server_stats", serve.GvServerStats)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gv_serve.h"',
+             'int main(void) {']
+    expect = []
+    for cname, ct in pairs:
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        expect.append(ctypes.sizeof(ct))
+        for f, _ in ct._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {f}));')
+            expect.append(getattr(ct, f).offset)
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == expect
