@@ -352,9 +352,16 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
     for (int k = 0; k < nst; ++k) insert_write(g->st[k], g->sc[k], keys[k], m[k], s);
   }
   if (compact_fraction >= 0.f) {
+    // with updates: only the vertices this group touched can have gained
+    // tombstones, so only they are candidates; an empty group compacts all
     for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
-      if (!n) op_begin(g->st[k], s);
-      compact(g->st[k], compact_fraction, g->vlist, s);
+      if (n) {
+        compact_touched(g->st[k], g->sc[k], g->directed ? n : 2 * n, compact_fraction, g->vlist,
+                        s);
+      } else {
+        op_begin(g->st[k], s);
+        compact(g->st[k], compact_fraction, g->vlist, s);
+      }
     }
   }
   GV_HIP(hipGetLastError());

@@ -97,6 +97,50 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T *in, T *out
   }
 }
 
+// Up to SCAN_FUSED_TILES tiles: the apply kernel adds up the tile sums
+// before its own tile (and all of them for the total) itself, so the spine
+// launch goes away.  Tile sums are added in tile order by one fixed
+// reduction tree, so the result does not depend on scheduling.
+constexpr uint64_t SCAN_FUSED_TILES = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply_fused(const T *in, T *out, uint64_t n,
+                                                                   const uint32_t *n_dev,
+                                                                   const T *tile_sums,
+                                                                   uint64_t nt) {
+  const uint64_t ne = eff_n(n, n_dev);
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  T before = 0, all = 0;
+#pragma unroll
+  for (int j = 0; j < (int)(SCAN_FUSED_TILES / SCAN_THREADS); ++j) {
+    const uint64_t t = (uint64_t)j * SCAN_THREADS + threadIdx.x;
+    const T v = t < nt ? tile_sums[t] : 0;
+    all += v;
+    if (t < blockIdx.x) before += v;
+  }
+  before = block_sum<T, SCAN_THREADS>(before);
+  all = block_sum<T, SCAN_THREADS>(all);
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[ne] = all;
+  if (base >= ne) return;
+  T v[SCAN_ITEMS];
+  T s = 0;
+  const uint64_t mine = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    uint64_t i = mine + j;
+    v[j] = i < ne ? in[i] : 0;
+    s += v[j];
+  }
+  T tot;
+  T ex = block_excl_scan<T, SCAN_THREADS>(s, tot) + before;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    uint64_t i = mine + j;
+    if (i < ne) out[i] = ex;
+    ex += v[j];
+  }
+}
+
 size_t scan_tmp_bytes(uint64_t n) { return (cdiv(n ? n : 1, SCAN_TILE) + 1) * sizeof(uint64_t); }
 
 template <typename T>
@@ -105,6 +149,11 @@ static void scan_impl(const T *in, T *out, uint64_t n, const uint32_t *n_dev, vo
   uint64_t nt = cdiv(n ? n : 1, SCAN_TILE);
   T *ts = (T *)tmp;
   k_scan_reduce<T><<<dim3((unsigned)nt), dim3(SCAN_THREADS), 0, s>>>(in, n, n_dev, ts);
+  if (nt <= SCAN_FUSED_TILES) {
+    k_scan_apply_fused<T><<<dim3((unsigned)nt), dim3(SCAN_THREADS), 0, s>>>(in, out, n, n_dev, ts,
+                                                                          nt);
+    return;
+  }
   k_scan_spine<T><<<dim3(1), dim3(SPINE_THREADS), 0, s>>>(ts, nt);
   k_scan_apply<T><<<dim3((unsigned)nt), dim3(SCAN_THREADS), 0, s>>>(in, out, n, n_dev, ts, nt);
 }
@@ -139,6 +188,13 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint64_t *key
 // Stable scatter.  Wave w of the tile owns keys [w*1024, (w+1)*1024) of the
 // tile; round j covers 64 consecutive keys, one per lane, so (w, j, lane) is
 // index order and ranks assigned round by round are stable.
+// RAW_HIST: offs is the raw digit-major histogram (at most SORT_SMALL_TILES
+// tiles) and each block derives its digit offsets from it, so the pass needs
+// no separate scan: offset(d, t) = sum of all counts of digits < d + counts
+// of digit d in tiles < t.
+constexpr uint64_t SORT_SMALL_TILES = 128;
+
+template <bool RAW_HIST>
 __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *in, uint64_t *out,
                                                                 uint64_t n, uint32_t shift,
                                                                 const uint32_t *offs, uint64_t nt) {
@@ -173,7 +229,19 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
   __syncthreads();
   {
     const uint32_t d = threadIdx.x;  // SORT_THREADS == 256 digits
-    uint32_t acc = offs[(uint64_t)d * nt + blockIdx.x];
+    uint32_t acc;
+    if (RAW_HIST) {
+      uint32_t tot = 0, below = 0;
+      for (uint64_t t = 0; t < nt; ++t) {
+        const uint32_t c = offs[(uint64_t)d * nt + t];
+        tot += c;
+        if (t < blockIdx.x) below += c;
+      }
+      uint32_t all;
+      acc = block_excl_scan<uint32_t, SORT_THREADS>(tot, all) + below;
+    } else {
+      acc = offs[(uint64_t)d * nt + blockIdx.x];
+    }
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
       uint32_t c = wcnt[i][d];
@@ -213,8 +281,14 @@ uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uin
   uint64_t *src = a, *dst = b;
   for (uint32_t shift = lo; shift < hi; shift += 8) {
     k_radix_hist<<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(src, n, shift, hist, nt);
-    scan_u32(hist, hist, h, nullptr, stmp, s);
-    k_radix_scatter<<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(src, dst, n, shift, hist, nt);
+    if (nt <= SORT_SMALL_TILES) {
+      k_radix_scatter<true><<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(src, dst, n, shift,
+                                                                              hist, nt);
+    } else {
+      scan_u32(hist, hist, h, nullptr, stmp, s);
+      k_radix_scatter<false><<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(src, dst, n, shift,
+                                                                               hist, nt);
+    }
     uint64_t *t = src;
     src = dst;
     dst = t;

@@ -52,6 +52,10 @@ void insert_write(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipS
 void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Compaction of vertices whose tombstones exceed frac of their slots.
 void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s);
+// Compaction restricted to the sources of the last staged batch of sc (m =
+// staged key bound), for pipelines that compact what they just updated.
+void compact_touched(Store &st, const Scratch &sc, uint64_t m, float frac, uint32_t *vlist,
+                     hipStream_t s);
 
 // Read-back helpers.
 void degrees_u64(const Store &st, uint64_t *out, hipStream_t s);

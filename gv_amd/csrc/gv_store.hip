@@ -427,9 +427,13 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
   }
 }
 
+// Grid of a grid-stride wavefront loop over at most items_bound items: one
+// block (4 waves) per 4 items, capped at 2048 blocks -- 8 resident blocks on
+// each of the 256 CUs -- so an over-estimated bound does not dispatch
+// thousands of blocks that exit at once.
 static unsigned probe_grid(uint64_t items_bound) {
   uint64_t g = cdiv(items_bound ? items_bound : 1, 4);  // 4 waves per block
-  return (unsigned)(g < 8192 ? g : 8192);
+  return (unsigned)(g < 2048 ? g : 2048);
 }
 
 // ---------------------------------------------------------------- insert --
@@ -705,9 +709,46 @@ __global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist
   }
 }
 
+// Same selection over the sources of the last staged batch only (segments
+// [0, ctl->n_segs) of sc): a vertex the batch did not touch gained no
+// tombstones from it.
+__global__ __launch_bounds__(256) void k_cmp_select_touched(Store st, float frac,
+                                                            const uint32_t *seg_src,
+                                                            uint32_t *vlist, Ctl *ctl) {
+  const uint64_t ns = ctl->n_segs;
+  const uint64_t chunk = cdiv_dev(ns, gridDim.x);
+  const uint64_t lo = blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < ns ? lo + chunk : ns;
+  uint32_t mine = 0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    mine += cmp_wanted(st, seg_src[i], frac);
+  uint32_t tot = block_sum<uint32_t, 256>(mine);
+  __shared__ unsigned long long base;
+  if (threadIdx.x == 0) base = tot ? atomicAdd(&ctl->compacted, (unsigned long long)tot) : 0ull;
+  __syncthreads();
+  if (!tot) return;
+  unsigned long long b = base;
+  for (uint64_t t0 = lo; t0 < hi; t0 += blockDim.x) {
+    uint64_t i = t0 + threadIdx.x;
+    uint32_t v = i < hi ? seg_src[i] : 0u;
+    uint32_t sel = i < hi ? cmp_wanted(st, v, frac) : 0u;
+    uint32_t ttot;
+    uint32_t ex = block_excl_scan<uint32_t, 256>(sel, ttot);
+    if (sel) vlist[b + ex] = v;
+    b += ttot;
+  }
+}
+
 void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s) {
   k_cmp_select<<<grid_for(st.V, 256, 512), 256, 0, s>>>(st, frac, vlist, st.ctl);
   k_cmp_run<<<probe_grid(st.V), 256, 0, s>>>(st, vlist, st.ctl);
+}
+
+void compact_touched(Store &st, const Scratch &sc, uint64_t m, float frac, uint32_t *vlist,
+                     hipStream_t s) {
+  k_cmp_select_touched<<<grid_for(m, 256, 512), 256, 0, s>>>(st, frac, sc.seg_src, vlist,
+                                                              st.ctl);
+  k_cmp_run<<<probe_grid(m), 256, 0, s>>>(st, vlist, st.ctl);
 }
 
 // ------------------------------------------------------------- read-back --

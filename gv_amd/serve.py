@@ -142,16 +142,19 @@ def delete(src, dst):
     return _update(GV_REQ_DELETE, src, dst)
 
 
-def neighbors(vs, out_cap):
+def neighbors(vs, out_cap=None):
+    """1-hop lookup of vs; out_cap=None keeps only the offsets (the lists
+    are produced but not copied out)."""
     r = Request(GV_REQ_NEIGHBORS)
     v = np.ascontiguousarray(vs, dtype=np.uint32)
     r._offs = np.zeros(len(v) + 1, np.uint64)
-    r._out = np.empty(max(int(out_cap), 1), np.uint32)
+    r._out = np.empty(max(int(out_cap or 0), 1), np.uint32)
     r.c.vs = v.ctypes.data_as(P)
     r.c.n = len(v)
     r.c.offs = r._offs.ctypes.data_as(P)
-    r.c.out = r._out.ctypes.data_as(P)
-    r.c.out_cap = int(out_cap)
+    if out_cap is not None:
+        r.c.out = r._out.ctypes.data_as(P)
+        r.c.out_cap = int(out_cap)
     r.keep += [v]
     return r
 

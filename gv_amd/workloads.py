@@ -206,3 +206,175 @@ def run_chunglu(args):
         "autotune": tuned,
         "setup_s": round(setup_s, 2),
     }
+
+
+# --------------------------------------------------------------- serving --
+def serve_trace(V, seconds, seed=5, update_rate=2000.0, query_rate=100.0,
+                query_mix=(0.99, 0.005, 0.005)):
+    """Arrival times and request shapes of the serving trace: updates arrive
+    as a Poisson stream (exponential gaps, mean 1/update_rate) with sizes
+    log-uniform in [100, 100k] (70% inserts); queries arrive as a second
+    Poisson stream split into neighbour lookups (1k vertices), PageRank
+    refreshes and single-source BC by query_mix.  Returns a list of
+    (arrival_us, kind, size_or_source) sorted by arrival."""
+    rng = np.random.default_rng(seed)
+    ev = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1e6 / update_rate)
+        if t >= seconds * 1e6:
+            break
+        n = int(np.exp(rng.uniform(np.log(100), np.log(100_000))))
+        ev.append((t, "insert" if rng.random() < 0.7 else "delete", n))
+    t = 0.0
+    while True:
+        t += rng.exponential(1e6 / query_rate)
+        if t >= seconds * 1e6:
+            break
+        u = rng.random()
+        if u < query_mix[0]:
+            ev.append((t, "neighbors", 1000))
+        elif u < query_mix[0] + query_mix[1]:
+            ev.append((t, "pagerank", 0))
+        else:
+            ev.append((t, "bc", int(rng.integers(0, V))))
+    ev.sort(key=lambda e: e[0])
+    return ev
+
+
+def _pct(x, q):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
+
+
+def run_serve(args):
+    """Serving form: directed R-MAT (default scale 26, edge factor 16,
+    seed 5), device-resident with its in-edge mirror; the trace of
+    serve_trace() replayed open-loop through the request loop.  Reports
+    p50/p99 latency (arrival -> device work done) overall and per kind, and
+    the sustained rate of applied edge updates."""
+    from . import GraphVine, export_csr_device, gen_rmat, stream_key
+    from . import serve as S
+    from .hip import DeviceArray
+
+    scale = int(getattr(args, "serve_scale", 26))
+    seconds = float(getattr(args, "serve_seconds", 10.0))
+    seed = 5
+    V = 1 << scale
+    n_raw = 16 * V
+    t_setup = time.time()
+    g = GraphVine(V, directed=True)
+    # bulk build in chunks of at most 2^27 pairs so staging stays small
+    chunk = 1 << 27
+    first = True
+    for c0 in range(0, n_raw, chunk):
+        n = min(chunk, n_raw - c0)
+        # edges c0 .. c0+n-1 of the R-MAT stream: output i of a stream keyed
+        # k is mix(k + (i+1)*GOLD), so shifting the key by c0*scale*GOLD
+        # continues the stream at edge c0
+        key = (stream_key(seed, STREAM_RMAT) + c0 * scale * 0x9E3779B97F4A7C15) % (1 << 64)
+        src, dst = gen_rmat(g, key, scale, n)
+        if first:
+            g.bulkBuild(src, dst, stats=False)
+            first = False
+        else:
+            g.batchInsert(src, dst, stats=False)
+        g.sync()
+        src.free()
+        dst.free()
+    E0 = g.num_edges()
+    ev = serve_trace(V, seconds, seed)
+    upd = [(i, e) for i, e in enumerate(ev) if e[1] in ("insert", "delete")]
+    total = sum(e[2] for _, e in upd)
+    us, ud = DeviceArray(max(total, 1), np.uint32), DeviceArray(max(total, 1), np.uint32)
+    off, cols, _ = export_csr_device(g)
+    from . import _check, lib
+    at = 0
+    where = {}
+    for j, (i, e) in enumerate(upd):
+        n = e[2]
+        vs, vd = us.view(at, n), ud.view(at, n)
+        if e[1] == "insert":
+            _check(lib().gv_gen_uniform(g.handle, stream_key(seed, 10_000_000 + j), V, n,
+                                        vs.ptr, vd.ptr))
+        else:
+            _check(lib().gv_gen_sample(g.handle, stream_key(seed, 20_000_000 + j), V, off.ptr,
+                                       cols.ptr, n, vs.ptr, vd.ptr))
+        where[i] = (at, n)
+        at += n
+    g.sync()
+    off.free()
+    cols.free()
+    rng = np.random.default_rng(seed + 1)
+    reqs, arr = [], []
+    for i, (t, kind, x) in enumerate(ev):
+        if kind in ("insert", "delete"):
+            a, n = where[i]
+            f = S.insert if kind == "insert" else S.delete
+            reqs.append(f(us.view(a, n), ud.view(a, n)))
+        elif kind == "neighbors":
+            reqs.append(S.neighbors(rng.integers(0, V, x).astype(np.uint32)))
+        elif kind == "pagerank":
+            reqs.append(S.pagerank())
+        else:
+            reqs.append(S.bc(x))
+        arr.append(t)
+    setup_s = time.time() - t_setup
+    with S.Server(g) as srv:
+        # one warm-up refresh so the first PageRank of the trace is incremental
+        w = S.pagerank()
+        srv.wait(srv.submit(w))
+        t0 = time.time()
+        resp = srv.replay(reqs, arr)
+        wall = time.time() - t0
+        st = srv.stats()
+    E_end = g.num_edges()
+    g.close()
+    lat, svc = {}, {}
+    step_ms = {}
+    for (t, kind, _), r in zip(ev, resp):
+        lat.setdefault(kind, []).append((r["done_us"] - r["arrival_us"]) / 1e3)
+        svc.setdefault(kind, []).append((r["done_us"] - r["start_us"]) / 1e3)
+        step_ms[r["step"]] = (r["done_us"] - r["start_us"]) / 1e3
+    all_ms = [x for v in lat.values() for x in v]
+    first_arrival = min(r["arrival_us"] for r in resp)
+    last_done = max(r["done_us"] for r in resp)
+    span_s = (last_done - first_arrival) / 1e6
+    pr_it = [r["iters"] for (t, k, _), r in zip(ev, resp) if k == "pagerank"]
+    return {
+        "metric": "request latency p99 (serving: Poisson update arrivals + interleaved queries)",
+        "value": _pct(all_ms, 99),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (updates, lookups), f32 + f64 residual (PageRank), f64 sigma (BC)",
+        "data": "synthetic (device-generated R-MAT, update batches; host-generated arrivals)",
+        "config": {
+            "workload": f"directed R-MAT scale {scale}, edge factor 16, seed {seed}, in-edge "
+                        f"mirror; {seconds:.0f} s trace: updates Poisson 2000/s, sizes "
+                        "log-uniform [100, 100k], 70% insert / 30% delete; queries Poisson "
+                        "100/s: 99% 1-hop lookups of 1000 vertices, 0.5% PageRank refresh "
+                        "(warm start, L1 1e-6), 0.5% single-source BC",
+            "vertices": V,
+            "directed_edges_initial": int(E0),
+            "directed_edges_final": int(E_end),
+        },
+        "latency_ms": {k: {"n": len(v), "p50": _pct(v, 50), "p99": _pct(v, 99),
+                           "max": max(v)} for k, v in lat.items()},
+        "service_ms": {k: {"p50": _pct(v, 50), "p99": _pct(v, 99)} for k, v in svc.items()},
+        "device_busy_fraction": sum(step_ms.values()) / 1e3 / span_s,
+        "p50_ms": _pct(all_ms, 50),
+        "p99_ms": _pct(all_ms, 99),
+        "sustained_edge_updates_per_s": st["update_pairs_applied"] / span_s,
+        "offered_edge_updates": int(total),
+        "trace_span_s": span_s,
+        "replay_wall_s": wall,
+        "steps_served": int(st["steps"]),
+        "refused": int(st["refused"]),
+        "pagerank_refresh_iters_mean": float(np.mean(pr_it)) if pr_it else None,
+        "setup_s": round(setup_s, 2),
+    }

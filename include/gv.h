@@ -94,9 +94,11 @@ gv_status gv_batch_delete(gv_graph *g, const uint32_t *src, const uint32_t *dst,
  * GV_OP_DELETE, host array).  The resulting graph equals applying the batches
  * one after another (an edge's final membership is decided by its last
  * operation in the group); stats cover the whole group (applied = slots
- * written + slots tombstoned).  compact_fraction >= 0 runs gv_compact with that
- * fraction at the end.  No counterpart in the reference, which applies one
- * CSR batch per call (graphvine.cu:222-451). */
+ * written + slots tombstoned).  compact_fraction >= 0 then compacts, at that
+ * tombstone fraction, the vertices the group touched (only they can have
+ * gained tombstones from it; an empty group compacts every vertex, like
+ * gv_compact).  No counterpart in the reference, which applies one CSR batch
+ * per call (graphvine.cu:222-451). */
 #define GV_OP_INSERT 0
 #define GV_OP_DELETE 1
 gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
