@@ -47,7 +47,8 @@ def parse():
     p.add_argument("--cpu-batches", type=int, default=2)
     p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
     p.add_argument("--pr-layout", type=int, default=1,
-                   help="PageRank layout: 1 = pull over the store, 0 = XCD-segmented snapshot")
+                   help="PageRank layout: 1 = CSR snapshot (default), 2 = pull over the "
+                        "store, 0 = XCD-segmented snapshot")
     p.add_argument("--workload", default="rmat22",
                    choices=["rmat22", "chunglu", "precision", "serve"],
                    help="rmat22: plain vs batched update stream on undirected R-MAT (default); "
@@ -60,7 +61,7 @@ def parse():
     p.add_argument("--cl-vertices", type=int, default=8_000_000)
     p.add_argument("--cl-batches", type=int, default=256)
     p.add_argument("--serve-scale", type=int, default=26)
-    p.add_argument("--serve-seconds", type=float, default=10.0)
+    p.add_argument("--serve-seconds", type=float, default=20.0)
     return p.parse_args()
 
 

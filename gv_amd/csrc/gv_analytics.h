@@ -26,7 +26,6 @@ struct PrWork {
   float *contrib[2] = {nullptr, nullptr};
   uint32_t *list = nullptr;      // vertices grouped by degree bin
   uint32_t *hub_base = nullptr;  // per chunked vertex: first chunk partial
-  uint32_t *hub_cnt = nullptr;   // per chunked vertex: chunks finished this iteration
   uint32_t *chunk_hub = nullptr; // per chunk: index into the chunked-vertex list
   float *chunk_part = nullptr;   // per chunk: (sum, compensation)
   double *partials = nullptr;    // per block residual partials, then per hub
@@ -34,6 +33,16 @@ struct PrWork {
   PrCtl *ctl = nullptr;
   PrCtl *h_ctl = nullptr;        // pinned host mirror
   std::vector<hipEvent_t> ev;    // per-iteration start/stop events (timed runs)
+  // CSR snapshot of the in-adjacency (live slots only), rebuilt per call.
+  uint64_t *snap_off = nullptr;   // [V+1]
+  uint32_t *snap_cols = nullptr;  // [E]
+  uint64_t snap_cap = 0;
+  uint32_t *snap_istart = nullptr;  // [V+1] first chunk item of each vertex
+  uint32_t *snap_ivert = nullptr;   // per item: its vertex
+  uint64_t *snap_ipos = nullptr;    // per item: live count, then output position
+  uint64_t snap_icap = 0;
+  void *snap_tmp = nullptr;
+  size_t snap_tmp_bytes = 0;
 };
 
 // Run parameters of one PageRank call.
@@ -80,9 +89,11 @@ gv_status pr_reserve(PrWork &w, uint32_t V);
 void pr_release(PrWork &w);
 // Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
+// snapshot = 1: iterate over a CSR copy of the live in-adjacency built at the
+// start of the call; 0: iterate over the block store itself.
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
                        const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
-                       hipStream_t s, double *iter_ms = nullptr);
+                       hipStream_t s, double *iter_ms = nullptr, int snapshot = 1);
 
 // Triangle count on an undirected store.  per_vertex_dev may be null.
 gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,

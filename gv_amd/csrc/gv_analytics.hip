@@ -3,12 +3,17 @@
 //
 // PageRank (static_pagerank, pagerank.cu:3-68): pull form over the
 // in-adjacency so every vertex's sum is computed by one fixed lane schedule
-// (deterministic, no FP32 atomics).  Vertices are binned by slot count:
-// one lane, 8 lanes, one wavefront, or PR chunks of wave_max slots on many
-// wavefronts whose partials are combined in chunk order.  Per-lane Kahan sums
-// are merged with TwoSum.  The L1 residual is reduced in FP64 in a fixed
-// order by the last block of each iteration, which also sets the done flag,
-// so iterations need no host round trip.
+// (deterministic, no FP32 atomics).  By default each call first copies the
+// live in-adjacency into a CSR snapshot (one pass over the slots), so the
+// iterations stream plain rows instead of walking block tables and skipping
+// tombstones.  Vertices are binned by row length: one lane, 8 lanes, one
+// wavefront, or chunks of wave_max entries on many wavefronts whose partials
+// a follow-up kernel combines in chunk order (a kernel boundary instead of a
+// device-scope fence, which on a multi-XCD part would write back L2 per
+// chunk).  Each bin is its own launch with its own load depth.  Per-lane
+// Kahan sums are merged with TwoSum.  The L1 residual is reduced in FP64 in
+// a fixed order by a one-block kernel, which also sets the done flag, so
+// iterations need no host round trip.
 //
 // Triangle counting (static_tc, tc.cu:3-27, tc_kernel.cu:3-74): degree-ordered
 // orientation (rank = (degree, id)), sorted oriented lists, one wavefront per
@@ -39,7 +44,6 @@ __device__ __forceinline__ void two_sum_merge(float &s, float &c, float s2, floa
 // =============================================================== PageRank ==
 constexpr int PR_THREADS = 256;
 constexpr int PR_GROUP = 8;
-constexpr int PR_ILP = 8;  // independent slot reads / gathers in flight per lane
 
 gv_status pr_reserve(PrWork &w, uint32_t V) {
   if (w.V >= V && w.ctl) return GV_OK;
@@ -52,7 +56,6 @@ gv_status pr_reserve(PrWork &w, uint32_t V) {
   }
   GV_HIP(hipMalloc(&w.list, n * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&w.hub_base, n * sizeof(uint32_t)));
-  GV_HIP(hipMalloc(&w.hub_cnt, n * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&w.ctl, sizeof(PrCtl)));
   GV_HIP(hipHostMalloc(&w.h_ctl, sizeof(PrCtl), hipHostMallocDefault));
   return GV_OK;
@@ -65,13 +68,18 @@ void pr_release(PrWork &w) {
   }
   hipFree(w.list);
   hipFree(w.hub_base);
-  hipFree(w.hub_cnt);
   hipFree(w.chunk_hub);
   hipFree(w.chunk_part);
   hipFree(w.partials);
   hipFree(w.ctl);
   if (w.h_ctl) hipHostFree(w.h_ctl);
   for (hipEvent_t e : w.ev) hipEventDestroy(e);
+  hipFree(w.snap_off);
+  hipFree(w.snap_cols);
+  hipFree(w.snap_istart);
+  hipFree(w.snap_ivert);
+  hipFree(w.snap_ipos);
+  hipFree(w.snap_tmp);
   w = PrWork();
 }
 
@@ -178,27 +186,175 @@ void pr_bins_place(const uint32_t *deg, uint32_t V, const PrBins &b, uint32_t *l
 // Chunk lists of the vertices in the chunked bin.  Chunk c of hub h lives at
 // hub_base[h] + c, so partials combine in chunk order whatever the schedule.
 __global__ void k_pr_hubs(const uint32_t *used, const uint32_t *list, uint32_t o3, uint32_t n3,
-                          uint32_t wave_max, uint32_t *hub_base, uint32_t *hub_cnt,
-                          uint32_t *chunk_hub, unsigned int *cursor) {
+                          uint32_t wave_max, uint32_t *hub_base, uint32_t *chunk_hub,
+                          unsigned int *cursor) {
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n3;
        h += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t v = list[o3 + h];
     uint32_t nch = (used[v] + wave_max - 1) / wave_max;
     uint32_t b = atomicAdd(cursor, nch);
     hub_base[h] = b;
-    hub_cnt[h] = 0;
     for (uint32_t c = 0; c < nch; ++c) chunk_hub[b + c] = (uint32_t)h;
   }
 }
 
+// ------------------------------------------------------------ snapshot --
+// CSR copy of the live in-adjacency.  Each vertex's slots are cut into
+// chunk items of SNAP_CH slots; an item's live count is its length when the
+// vertex has no tombstones, else counted; one scan of the counts in
+// (vertex, chunk) order gives every item's output position, and the offset
+// of vertex v is the position of its first item.  Entries keep slot order.
+constexpr uint32_t SNAP_CH = 1024;
+
+__global__ void k_snap_items(const uint32_t *used, uint32_t V, uint32_t *ncks) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    ncks[v] = (used[v] + SNAP_CH - 1) / SNAP_CH;
+}
+
+__global__ void k_snap_expand(const uint32_t *istart, uint32_t V, uint32_t *ivert) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = istart[v]; i < istart[v + 1]; ++i) ivert[i] = (uint32_t)v;
+}
+
+// One wavefront per item: live slots of the item's chunk.
+__global__ __launch_bounds__(256) void k_snap_count(Store st, const uint32_t *istart,
+                                                    const uint32_t *ivert, uint64_t *ipos) {
+  const uint32_t ni = istart[st.V];
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
+       i += nwaves) {
+    const uint32_t v = ivert[i];
+    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
+    const uint32_t lo = ((uint32_t)i - istart[v]) * SNAP_CH;
+    const uint32_t hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
+    uint32_t cnt = hi - lo;
+    if (us != st.deg[v]) {
+      cnt = 0;
+      for (uint32_t p = lo; p < hi; p += WAVE) {
+        const bool live = p + lane < hi &&
+                          st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)] != TOMB;
+        cnt += (uint32_t)__popcll(__ballot(live));
+      }
+    }
+    if (lane == 0) ipos[i] = cnt;
+  }
+}
+
+__global__ void k_snap_off(const uint32_t *istart, uint32_t V, const uint64_t *ipos,
+                           uint64_t *off) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    off[v] = ipos[istart[v]];
+}
+
+// One wavefront per item: live slots packed, in slot order, at the item's
+// output position.
+__global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *istart,
+                                                    const uint32_t *ivert, const uint64_t *ipos,
+                                                    uint32_t *cols) {
+  const uint32_t ni = istart[st.V];
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
+       i += nwaves) {
+    const uint32_t v = ivert[i];
+    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
+    const uint32_t lo = ((uint32_t)i - istart[v]) * SNAP_CH;
+    const uint32_t hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
+    uint64_t dst = ipos[i];
+    for (uint32_t p = lo; p < hi; p += WAVE) {
+      uint32_t w = TOMB;
+      if (p + lane < hi) w = st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)];
+      const bool live = w != TOMB;
+      const unsigned long long m = __ballot(live);
+      if (live) cols[dst + mask_lt_count(m)] = w;
+      dst += (uint64_t)__popcll(m);
+    }
+  }
+}
+
+static gv_status pr_snapshot(const Store &in, PrWork &w, hipStream_t s) {
+  const uint32_t V = in.V;
+  // items: at most one per vertex plus one per SNAP_CH slots of the pool
+  const uint64_t icap = (uint64_t)V + in.P * (uint64_t)in.B / SNAP_CH + 1;
+  if (icap > 0xFFFFFFFFull) {
+    set_error("pagerank snapshot: too many chunk items");
+    return GV_E_INVALID;
+  }
+  if (!w.snap_istart || icap > w.snap_icap) {
+    hipFree(w.snap_istart);
+    hipFree(w.snap_ivert);
+    hipFree(w.snap_ipos);
+    hipFree(w.snap_off);
+    hipFree(w.snap_tmp);
+    w.snap_istart = w.snap_ivert = nullptr;
+    w.snap_ipos = w.snap_off = nullptr;
+    w.snap_tmp = nullptr;
+    w.snap_icap = 0;
+    GV_HIP(hipMalloc(&w.snap_istart, ((size_t)V + 1) * sizeof(uint32_t)));
+    GV_HIP(hipMalloc(&w.snap_ivert, icap * sizeof(uint32_t)));
+    GV_HIP(hipMalloc(&w.snap_ipos, (icap + 1) * sizeof(uint64_t)));
+    GV_HIP(hipMalloc(&w.snap_off, ((size_t)V + 1) * sizeof(uint64_t)));
+    w.snap_tmp_bytes = std::max(scan_tmp_bytes((uint64_t)V + 1), scan_tmp_bytes(icap + 1));
+    GV_HIP(hipMalloc(&w.snap_tmp, w.snap_tmp_bytes));
+    w.snap_icap = icap;
+  }
+  k_snap_items<<<grid_for(V), 256, 0, s>>>(in.used, V, w.snap_istart);
+  scan_u32(w.snap_istart, w.snap_istart, V, nullptr, w.snap_tmp, s);
+  k_snap_expand<<<grid_for(V), 256, 0, s>>>(w.snap_istart, V, w.snap_ivert);
+  k_snap_count<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(in, w.snap_istart, w.snap_ivert,
+                                                                 w.snap_ipos);
+  scan_u64(w.snap_ipos, w.snap_ipos, icap, w.snap_istart + V, w.snap_tmp, s);
+  k_snap_off<<<grid_for((uint64_t)V + 1), 256, 0, s>>>(w.snap_istart, V, w.snap_ipos, w.snap_off);
+  uint64_t E = 0;
+  GV_HIP(hipMemcpyAsync(&E, w.snap_off + V, sizeof(E), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  if (!w.snap_cols || E > w.snap_cap) {
+    hipFree(w.snap_cols);
+    w.snap_cols = nullptr;
+    w.snap_cap = 0;
+    const uint64_t cap = E + E / 8 + 1024;
+    GV_HIP(hipMalloc(&w.snap_cols, cap * sizeof(uint32_t)));
+    w.snap_cap = cap;
+  }
+  k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(in, w.snap_istart, w.snap_ivert,
+                                                                 w.snap_ipos, w.snap_cols);
+  return GV_OK;
+}
+
+// ------------------------------------------------------ iteration kernel --
+// Adjacency sources of the pull: the block store itself (slot counts include
+// tombstones, skipped), or the CSR snapshot (live entries only, streamed
+// with non-temporal loads so they do not displace contributions in L2).
+struct StoreAdj {
+  Store st;
+  __device__ __forceinline__ uint32_t len(uint32_t v) const { return st.used[v]; }
+  __device__ __forceinline__ uint64_t base(uint32_t v) const { return st.tbl_off[v]; }
+  __device__ __forceinline__ uint32_t at(uint64_t b, uint32_t pos) const {
+    return st.slots[slot_index(st.arena, (uint32_t)b, pos, st.log2B, st.B)];
+  }
+};
+struct CsrAdj {
+  const uint64_t *off;
+  const uint32_t *cols;
+  __device__ __forceinline__ uint32_t len(uint32_t v) const {
+    return (uint32_t)(off[v + 1] - off[v]);
+  }
+  __device__ __forceinline__ uint64_t base(uint32_t v) const { return off[v]; }
+  __device__ __forceinline__ uint32_t at(uint64_t b, uint32_t pos) const {
+    return __builtin_nontemporal_load(cols + b + pos);
+  }
+};
+
 // Work units of one iteration: 256 single-lane vertices, 32 eight-lane
 // vertices, 4 one-wavefront vertices, or 4 hub chunks.
 struct PrArgs {
-  Store in;
   const uint32_t *deg_out;
   const uint32_t *list;
   const uint32_t *hub_base;
-  uint32_t *hub_cnt;
   const uint32_t *chunk_hub;
   float2 *chunk_part;
   double *partials;
@@ -212,29 +368,30 @@ struct PrArgs {
   uint32_t compensated;  // 1: Kahan sums per lane; 0: plain FP32 sums
 };
 
-// Kahan-accumulate contributions of slots start, start+STRIDE, ... < end.
-// PR_ILP slot reads are issued before their PR_ILP gathers, so each lane
-// keeps several independent loads in flight.
-template <int STRIDE>
-__device__ __forceinline__ void pr_gather(const PrArgs &a, const float *cb, uint32_t tbl,
-                                          uint32_t start, uint32_t end, float &s, float &c) {
-  for (uint32_t j = start; j < end; j += PR_ILP * STRIDE) {
-    uint32_t w[PR_ILP];
+// Kahan-accumulate contributions of positions start, start+STRIDE, ... < end
+// of the adjacency row at b.  ILP entry reads are issued before their ILP
+// gathers, so each lane keeps many independent loads in flight.
+template <int STRIDE, int ILP, class Adj>
+__device__ __forceinline__ void pr_gather(const PrArgs &a, const Adj &adj, const float *cb,
+                                          uint64_t b, uint32_t start, uint32_t end, float &s,
+                                          float &c) {
+  for (uint32_t j = start; j < end; j += ILP * STRIDE) {
+    uint32_t w[ILP];
 #pragma unroll
-    for (int k = 0; k < PR_ILP; ++k) {
-      uint32_t pos = j + k * STRIDE;
-      w[k] = pos < end ? a.in.slots[slot_index(a.in.arena, tbl, pos, a.in.log2B, a.in.B)] : TOMB;
+    for (int k = 0; k < ILP; ++k) {
+      const uint32_t pos = j + k * STRIDE;
+      w[k] = pos < end ? adj.at(b, pos) : TOMB;
     }
-    float g[PR_ILP];
+    float g[ILP];
 #pragma unroll
-    for (int k = 0; k < PR_ILP; ++k) g[k] = w[k] != TOMB ? cb[w[k]] : 0.f;
+    for (int k = 0; k < ILP; ++k) g[k] = w[k] != TOMB ? cb[w[k]] : 0.f;
     if (a.compensated) {
 #pragma unroll
-      for (int k = 0; k < PR_ILP; ++k)
+      for (int k = 0; k < ILP; ++k)
         if (w[k] != TOMB) kahan_add(s, c, g[k]);
     } else {
 #pragma unroll
-      for (int k = 0; k < PR_ILP; ++k) s += g[k];
+      for (int k = 0; k < ILP; ++k) s += g[k];
     }
   }
 }
@@ -257,82 +414,99 @@ __device__ __forceinline__ double pr_finish(const PrArgs &a, uint32_t v, float s
   return fabs((double)xv - (double)x[v]);
 }
 
-// One work unit per workgroup, heaviest first: hub chunks, one-wavefront
-// vertices, eight-lane vertices, single-lane vertices.  The hardware dispatcher
-// balances the units over the CUs; each workgroup writes its residual partial
-// to its own slot, so no counter is shared between workgroups.
-__global__ __launch_bounds__(PR_THREADS) void k_pr_iter(PrArgs a, const float *x, const float *cb,
-                                                        float *xn, float *cn) {
+// One work unit per workgroup: hub chunks (BIN 3), one-wavefront vertices
+// (2), eight-lane vertices (1), single-lane vertices (0).  Each bin is its own
+// launch, heaviest first, so each gets the register budget -- and so the
+// occupancy -- of its own load depth (WILP for the wavefront-wide rows).
+// Unit u of the iteration is block u - (units of the bins launched before);
+// each workgroup writes its residual partial to slot u, so no counter is
+// shared between workgroups.
+template <class Adj, int WILP, int BIN>
+__global__ __launch_bounds__(PR_THREADS) void k_pr_iter(PrArgs a, Adj adj, const float *x,
+                                                        const float *cb, float *xn, float *cn) {
   if (a.ctl->done) return;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t wid = tid / WAVE;
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = blockIdx.x + (BIN == 3   ? 0u
+                                   : BIN == 2 ? a.U3
+                                   : BIN == 1 ? a.U3 + a.U2
+                                              : a.U3 + a.U2 + a.U1);
   double diff = 0.0;
-  if (u < a.U3) {
+  if (BIN == 3) {
     uint32_t ci = u * (PR_THREADS / WAVE) + wid;
     if (ci < a.n_chunks) {
       uint32_t h = a.chunk_hub[ci];
       uint32_t v = a.list[a.o3 + h];
-      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      uint32_t us = adj.len(v);
+      uint64_t b = adj.base(v);
       uint32_t hb = a.hub_base[h];
       uint32_t c0 = (ci - hb) * a.wave_max;
       uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
       float s = 0.f, c = 0.f;
-      pr_gather<WAVE>(a, cb, tbl, c0 + lane, c1, s, c);
+      pr_gather<WAVE, WILP>(a, adj, cb, b, c0 + lane, c1, s, c);
       group_merge<WAVE>(s, c);
-      if (lane == 0) {
-        a.chunk_part[ci] = make_float2(s, c);
-        __threadfence();
-        uint32_t nch = (us + a.wave_max - 1) / a.wave_max;
-        uint32_t old = atomicAdd(&a.hub_cnt[h], 1u);
-        if (old == nch - 1) {
-          __threadfence();
-          float ts = 0.f, tc = 0.f;
-          for (uint32_t k = 0; k < nch; ++k) {
-            float2 p = a.chunk_part[hb + k];
-            two_sum_merge(ts, tc, p.x, p.y);
-          }
-          a.partials[a.U + h] = pr_finish(a, v, ts, tc, x, xn, cn);
-          a.hub_cnt[h] = 0;
-        }
-      }
+      // chunks of a hub are merged by k_pr_hubfin after this launch: the
+      // kernel boundary makes the partials visible without a device fence
+      if (lane == 0) a.chunk_part[ci] = make_float2(s, c);
     }
-  } else if (u < a.U3 + a.U2) {
+  } else if (BIN == 2) {
     uint32_t g = (u - a.U3) * (PR_THREADS / WAVE) + wid;
     if (g < a.n2) {
       uint32_t v = a.list[a.o2 + g];
-      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      uint32_t us = adj.len(v);
+      uint64_t b = adj.base(v);
       float s = 0.f, c = 0.f;
-      pr_gather<WAVE>(a, cb, tbl, lane, us, s, c);
+      pr_gather<WAVE, WILP>(a, adj, cb, b, lane, us, s, c);
       group_merge<WAVE>(s, c);
       if (lane == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
     }
-  } else if (u < a.U3 + a.U2 + a.U1) {
+  } else if (BIN == 1) {
     uint32_t g = (u - a.U3 - a.U2) * (PR_THREADS / PR_GROUP) + tid / PR_GROUP;
     uint32_t gl = tid & (PR_GROUP - 1);
-    uint32_t v = 0, us = 0, tbl = 0;
+    uint32_t v = 0, us = 0;
+    uint64_t b = 0;
     if (g < a.n1) {
       v = a.list[a.o1 + g];
-      us = a.in.used[v];
-      tbl = a.in.tbl_off[v];
+      us = adj.len(v);
+      b = adj.base(v);
     }
     float s = 0.f, c = 0.f;
-    pr_gather<PR_GROUP>(a, cb, tbl, gl, us, s, c);
+    pr_gather<PR_GROUP, WILP>(a, adj, cb, b, gl, us, s, c);
     group_merge<PR_GROUP>(s, c);
     if (g < a.n1 && gl == 0) diff = pr_finish(a, v, s, c, x, xn, cn);
   } else {
     uint32_t i = (u - a.U3 - a.U2 - a.U1) * PR_THREADS + tid;
     if (i < a.n0) {
       uint32_t v = a.list[i];
-      uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+      uint32_t us = adj.len(v);
+      uint64_t b = adj.base(v);
       float s = 0.f, c = 0.f;
-      pr_gather<1>(a, cb, tbl, 0, us, s, c);
+      pr_gather<1, WILP>(a, adj, cb, b, 0, us, s, c);
       diff = pr_finish(a, v, s, c, x, xn, cn);
     }
   }
   double bs = block_sum<double, PR_THREADS>(diff);
   if (tid == 0) a.partials[u] = bs;
+}
+
+// Hub finish: the chunk partials of each chunked vertex merged in chunk
+// order, then the vertex update and its residual partial.
+template <class Adj>
+__global__ void k_pr_hubfin(PrArgs a, Adj adj, const float *x, float *xn, float *cn) {
+  if (a.ctl->done) return;
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < a.n3;
+       h += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = a.list[a.o3 + h];
+    const uint32_t nch = (adj.len(v) + a.wave_max - 1) / a.wave_max;
+    const uint32_t hb = a.hub_base[h];
+    float ts = 0.f, tc = 0.f;
+    for (uint32_t k = 0; k < nch; ++k) {
+      const float2 p = a.chunk_part[hb + k];
+      two_sum_merge(ts, tc, p.x, p.y);
+    }
+    a.partials[a.U + h] = pr_finish(a, (uint32_t)v, ts, tc, x, xn, cn);
+  }
 }
 
 // Fixed-order FP64 reduction of the workgroup and hub partials, then the
@@ -354,7 +528,7 @@ __global__ __launch_bounds__(1024) void k_pr_resid(PrArgs a) {
 
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
                        const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
-                       hipStream_t s, double *iter_ms) {
+                       hipStream_t s, double *iter_ms, int snapshot) {
   const float alpha = o.alpha;
   const double eps = o.eps;
   const int max_iter = o.max_iter;
@@ -372,7 +546,14 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     GV_HIP(hipStreamSynchronize(s));
     return GV_OK;
   }
-  k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(in.used, V, bins, w.ctl);
+  if (snapshot) {
+    st = pr_snapshot(in, w, s);
+    if (st) return st;
+  }
+  // row lengths the bins are cut by: live entries for the snapshot, slots
+  // (tombstones included) for the store
+  const uint32_t *rowlen = snapshot ? in.deg : in.used;
+  k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(rowlen, V, bins, w.ctl);
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
   const PrCtl hc = *w.h_ctl;
@@ -385,13 +566,11 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     GV_HIP(hipMalloc(&w.chunk_hub, w.chunk_cap * sizeof(uint32_t)));
     GV_HIP(hipMalloc(&w.chunk_part, w.chunk_cap * sizeof(float2)));
   }
-  k_pr_place<<<grid_for(V, 256, 512), 256, 0, s>>>(in.used, V, bins, w.list, w.ctl);
+  k_pr_place<<<grid_for(V, 256, 512), 256, 0, s>>>(rowlen, V, bins, w.list, w.ctl);
   PrArgs a;
-  a.in = in;
   a.deg_out = deg_out;
   a.list = w.list;
   a.hub_base = w.hub_base;
-  a.hub_cnt = w.hub_cnt;
   a.chunk_hub = w.chunk_hub;
   a.chunk_part = (float2 *)w.chunk_part;
   a.ctl = w.ctl;
@@ -416,11 +595,12 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   a.compensated = o.compensated ? 1u : 0u;
   const uint32_t grid = a.U ? a.U : 1;
   if (n3)
-    k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(in.used, w.list, a.o3, n3, bins.wave_max, w.hub_base,
-                                           w.hub_cnt, w.chunk_hub, &w.ctl->chunk_cursor);
-  if ((uint64_t)grid + n3 > w.part_cap) {
+    k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(rowlen, w.list, a.o3, n3, bins.wave_max, w.hub_base,
+                                           w.chunk_hub, &w.ctl->chunk_cursor);
+  const uint64_t nparts = (uint64_t)grid + n3;
+  if (nparts > w.part_cap) {
     hipFree(w.partials);
-    w.part_cap = (uint64_t)grid + n3 + 1024;
+    w.part_cap = nparts + 1024;
     GV_HIP(hipMalloc(&w.partials, w.part_cap * sizeof(double)));
   }
   a.partials = w.partials;
@@ -431,13 +611,32 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
       w.ev.push_back(e);
     }
   }
+  const StoreAdj sadj{in};
+  const CsrAdj cadj{w.snap_off, w.snap_cols};
   int launched = 0;
   const int check_every = 8;
   while (launched < max_iter) {
     int t = launched;
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
-    k_pr_iter<<<grid, PR_THREADS, 0, s>>>(a, w.x[t & 1], w.contrib[t & 1], w.x[(t + 1) & 1],
-                                          w.contrib[(t + 1) & 1]);
+    const float *x = w.x[t & 1], *cb = w.contrib[t & 1];
+    float *xn = w.x[(t + 1) & 1], *cn = w.contrib[(t + 1) & 1];
+    if (snapshot) {
+      if (a.U3) {
+        k_pr_iter<CsrAdj, 32, 3><<<a.U3, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
+        k_pr_hubfin<CsrAdj><<<grid_for(a.n3), 256, 0, s>>>(a, cadj, x, xn, cn);
+      }
+      if (a.U2) k_pr_iter<CsrAdj, 32, 2><<<a.U2, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
+      if (a.U1) k_pr_iter<CsrAdj, 16, 1><<<a.U1, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
+      if (a.U0) k_pr_iter<CsrAdj, 8, 0><<<a.U0, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
+    } else {
+      if (a.U3) {
+        k_pr_iter<StoreAdj, 8, 3><<<a.U3, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
+        k_pr_hubfin<StoreAdj><<<grid_for(a.n3), 256, 0, s>>>(a, sadj, x, xn, cn);
+      }
+      if (a.U2) k_pr_iter<StoreAdj, 8, 2><<<a.U2, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
+      if (a.U1) k_pr_iter<StoreAdj, 8, 1><<<a.U1, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
+      if (a.U0) k_pr_iter<StoreAdj, 8, 0><<<a.U0, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
+    }
     k_pr_resid<<<1, 1024, 0, s>>>(a);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
     ++launched;

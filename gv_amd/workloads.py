@@ -210,7 +210,7 @@ def run_chunglu(args):
 
 # --------------------------------------------------------------- serving --
 def serve_trace(V, seconds, seed=5, update_rate=2000.0, query_rate=100.0,
-                query_mix=(0.99, 0.005, 0.005)):
+                query_mix=(0.995, 0.002, 0.003)):
     """Arrival times and request shapes of the serving trace: updates arrive
     as a Poisson stream (exponential gaps, mean 1/update_rate) with sizes
     log-uniform in [100, 100k] (70% inserts); queries arrive as a second
@@ -357,8 +357,8 @@ def run_serve(args):
             "workload": f"directed R-MAT scale {scale}, edge factor 16, seed {seed}, in-edge "
                         f"mirror; {seconds:.0f} s trace: updates Poisson 2000/s, sizes "
                         "log-uniform [100, 100k], 70% insert / 30% delete; queries Poisson "
-                        "100/s: 99% 1-hop lookups of 1000 vertices, 0.5% PageRank refresh "
-                        "(warm start, L1 1e-6), 0.5% single-source BC",
+                        "100/s: 99.5% 1-hop lookups of 1000 vertices, 0.2% PageRank refresh "
+                        "(warm start, L1 1e-6), 0.3% single-source BC",
             "vertices": V,
             "directed_edges_initial": int(E0),
             "directed_edges_final": int(E_end),

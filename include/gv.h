@@ -59,7 +59,9 @@ typedef struct gv_config {
   uint32_t pr_wave_max;       /*   <= this -> 1 wavefront; larger -> chunked */
   int maintain_in_edges;      /* directed graphs: keep the in-adjacency mirror */
   int device;                 /* HIP device ordinal */
-  int pr_layout;              /* PageRank: 1 = pull over the store (default), 0 = XCD-segmented snapshot */
+  int pr_layout;              /* PageRank: 1 = CSR snapshot of the live in-adjacency, rebuilt per
+                                 call (default); 2 = pull over the block store; 0 = XCD-segmented
+                                 relabelled snapshot */
 } gv_config;
 
 /* Result of one update call. */

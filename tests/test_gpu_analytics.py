@@ -71,7 +71,7 @@ def test_pagerank_closed_forms():
     assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("directed", [True, False])
 def test_pagerank_config1_after_updates(directed, layout):
     V = 1 << 14
@@ -91,7 +91,8 @@ def test_pagerank_config1_after_updates(directed, layout):
     check_pagerank(g, o)
 
 
-def test_pagerank_all_degree_bins_and_chunked_hubs():
+@pytest.mark.parametrize("layout", [1, 2])
+def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     # small thresholds push vertices through every bin incl. chunked hubs
     V = 3000
     s, d = O.gen_rmat(O.stream_key(3, O.STREAM_RMAT), 12, 60000)
@@ -100,9 +101,17 @@ def test_pagerank_all_degree_bins_and_chunked_hubs():
     s = np.concatenate([s, star[0]])
     d = np.concatenate([d, star[1]])
     g, o = build_pair(V, False, s, d, pool=8 * V, pr_thread_max=2, pr_group_max=16,
-                      pr_wave_max=128, pr_layout=1)
+                      pr_wave_max=128, pr_layout=layout)
     check_pagerank(g, o)
     check_pagerank(g, o, eps=1e-9, max_iter=200, same_stop=False)
+    # tombstones on the hub (degree > one snapshot chunk) and elsewhere
+    off, cols = o.export_csr()
+    sd, dd = O.gen_sample(O.stream_key(3, 2000), V, off, cols, 2500)
+    sd = np.concatenate([sd, np.zeros(700, np.uint32)])
+    dd = np.concatenate([dd, np.arange(1, 1400, 2, dtype=np.uint32)])
+    g.batchDelete(sd, dd)
+    o.delete(sd, dd)
+    check_pagerank(g, o)
 
 
 @pytest.mark.parametrize("V", [5, 9, 20001, 70000])

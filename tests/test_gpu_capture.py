@@ -198,7 +198,7 @@ def test_set_tuning_keeps_pagerank():
     g.bulkBuild(s, d)
     x0, _, _ = G.pagerank(g, epsilon=1e-7)
     for tm, gm, wm, layout in [(1, 1, 64, 1), (32, 512, 100_000, 1), (8, 96, 4096, 0),
-                               (8, 96, 4096, 1)]:
+                               (8, 96, 4096, 2), (1, 1, 64, 2), (8, 96, 4096, 1)]:
         g.set_tuning(pr_thread_max=tm, pr_group_max=gm, pr_wave_max=wm, pr_layout=layout)
         cfg = g.get_config()
         assert (cfg.pr_thread_max, cfg.pr_group_max, cfg.pr_wave_max, cfg.pr_layout) == \
