@@ -47,8 +47,9 @@ def parse():
     p.add_argument("--cpu-batches", type=int, default=2)
     p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
     p.add_argument("--pr-layout", type=int, default=1,
-                   help="PageRank layout: 1 = CSR snapshot (default), 2 = pull over the "
-                        "store, 0 = XCD-segmented snapshot")
+                   help="PageRank layout: 1 = degree-relabelled CSR snapshot (default), "
+                        "3 = snapshot in vertex order, 2 = pull over the store, "
+                        "0 = XCD-segmented snapshot")
     p.add_argument("--workload", default="rmat22",
                    choices=["rmat22", "chunglu", "precision", "serve"],
                    help="rmat22: plain vs batched update stream on undirected R-MAT (default); "

@@ -43,6 +43,13 @@ struct PrWork {
   uint64_t snap_icap = 0;
   void *snap_tmp = nullptr;
   size_t snap_tmp_bytes = 0;
+  // Degree-ordered relabelling of the snapshot: new vertex r is old vertex
+  // rl_order[r]; rl_newid is the inverse; rl_len / rl_dout are in-row length
+  // and out-degree in the new order.
+  uint32_t *rl_order = nullptr, *rl_newid = nullptr, *rl_len = nullptr, *rl_dout = nullptr;
+  uint64_t *rl_keys[2] = {nullptr, nullptr};
+  void *rl_tmp = nullptr;
+  uint32_t rl_V = 0;
 };
 
 // Run parameters of one PageRank call.
@@ -90,7 +97,8 @@ void pr_release(PrWork &w);
 // Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
 // snapshot = 1: iterate over a CSR copy of the live in-adjacency built at the
-// start of the call; 0: iterate over the block store itself.
+// start of the call, vertices relabelled by descending out-degree; 2: the
+// same copy in the original vertex order; 0: iterate over the block store.
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
                        const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
                        hipStream_t s, double *iter_ms = nullptr, int snapshot = 1);

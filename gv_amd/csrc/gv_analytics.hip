@@ -80,6 +80,13 @@ void pr_release(PrWork &w) {
   hipFree(w.snap_ivert);
   hipFree(w.snap_ipos);
   hipFree(w.snap_tmp);
+  hipFree(w.rl_order);
+  hipFree(w.rl_newid);
+  hipFree(w.rl_len);
+  hipFree(w.rl_dout);
+  hipFree(w.rl_keys[0]);
+  hipFree(w.rl_keys[1]);
+  hipFree(w.rl_tmp);
   w = PrWork();
 }
 
@@ -92,16 +99,23 @@ __global__ void k_pr_reset(PrCtl *c) {
   *c = z;
 }
 
-// x0 = 1/V, or a caller's vector (warm start of an incremental refresh).
+// x0 = 1/V, or a caller's vector (warm start of an incremental refresh);
+// order (optional) maps the working vertex order to the caller's.
 __global__ void k_pr_init(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0,
-                          const float *init) {
+                          const float *init, const uint32_t *order) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t d = deg_out[v];
-    float xv = init ? init[v] : x0;
+    float xv = init ? init[order ? order[v] : v] : x0;
     x[v] = xv;
     cb[v] = d ? xv / (float)d : 0.f;
   }
+}
+
+__global__ void k_pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = x[newid[v]];
 }
 
 // Bin counts: per-block totals, five atomics per block.
@@ -251,9 +265,12 @@ __global__ void k_snap_off(const uint32_t *istart, uint32_t V, const uint64_t *i
 }
 
 // One wavefront per item: live slots packed, in slot order, at the item's
-// output position.
+// output position.  With a relabelling (newid non-null) the row of old vertex
+// v is row newid[v] of the snapshot (offsets off) and every entry w is
+// written as newid[w]; the entry order within the row is kept.
 __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *istart,
                                                     const uint32_t *ivert, const uint64_t *ipos,
+                                                    const uint32_t *newid, const uint64_t *off,
                                                     uint32_t *cols) {
   const uint32_t ni = istart[st.V];
   const uint32_t lane = lane_id();
@@ -265,18 +282,45 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
     const uint32_t lo = ((uint32_t)i - istart[v]) * SNAP_CH;
     const uint32_t hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
     uint64_t dst = ipos[i];
+    if (newid) dst = off[newid[v]] + (dst - ipos[istart[v]]);
     for (uint32_t p = lo; p < hi; p += WAVE) {
       uint32_t w = TOMB;
       if (p + lane < hi) w = st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)];
       const bool live = w != TOMB;
       const unsigned long long m = __ballot(live);
-      if (live) cols[dst + mask_lt_count(m)] = w;
+      if (live) cols[dst + mask_lt_count(m)] = newid ? newid[w] : w;
       dst += (uint64_t)__popcll(m);
     }
   }
 }
 
-static gv_status pr_snapshot(const Store &in, PrWork &w, hipStream_t s) {
+// Relabelling keys: (V - out-degree) above the vertex ID, so an ascending sort
+// lists vertices by descending out-degree, ties by ID.
+__global__ void k_rl_keys(const uint32_t *deg_out, uint32_t V, uint32_t bV, uint64_t *keys) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = deg_out[v] < V ? deg_out[v] : V;
+    keys[v] = ((uint64_t)(V - d) << bV) | v;
+  }
+}
+
+__global__ void k_rl_perm(const uint64_t *sorted, uint32_t V, uint32_t bV, const uint32_t *rowlen,
+                          const uint32_t *deg_out, uint32_t *order, uint32_t *newid,
+                          uint32_t *len_new, uint32_t *dout_new, uint64_t *off_new) {
+  const uint64_t mask = (1ull << bV) - 1;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(sorted[r] & mask);
+    order[r] = v;
+    newid[v] = (uint32_t)r;
+    len_new[r] = rowlen[v];
+    dout_new[r] = deg_out[v];
+    off_new[r] = rowlen[v];
+  }
+}
+
+static gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
+                             hipStream_t s) {
   const uint32_t V = in.V;
   // items: at most one per vertex plus one per SNAP_CH slots of the pool
   const uint64_t icap = (uint64_t)V + in.P * (uint64_t)in.B / SNAP_CH + 1;
@@ -308,7 +352,40 @@ static gv_status pr_snapshot(const Store &in, PrWork &w, hipStream_t s) {
   k_snap_count<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(in, w.snap_istart, w.snap_ivert,
                                                                  w.snap_ipos);
   scan_u64(w.snap_ipos, w.snap_ipos, icap, w.snap_istart + V, w.snap_tmp, s);
-  k_snap_off<<<grid_for((uint64_t)V + 1), 256, 0, s>>>(w.snap_istart, V, w.snap_ipos, w.snap_off);
+  if (relabel) {
+    if (w.rl_V < V || !w.rl_order) {
+      hipFree(w.rl_order);
+      hipFree(w.rl_newid);
+      hipFree(w.rl_len);
+      hipFree(w.rl_dout);
+      hipFree(w.rl_keys[0]);
+      hipFree(w.rl_keys[1]);
+      hipFree(w.rl_tmp);
+      w.rl_order = w.rl_newid = w.rl_len = w.rl_dout = nullptr;
+      w.rl_keys[0] = w.rl_keys[1] = nullptr;
+      w.rl_tmp = nullptr;
+      w.rl_V = 0;
+      const size_t n = (size_t)V + 1;
+      GV_HIP(hipMalloc(&w.rl_order, n * sizeof(uint32_t)));
+      GV_HIP(hipMalloc(&w.rl_newid, n * sizeof(uint32_t)));
+      GV_HIP(hipMalloc(&w.rl_len, n * sizeof(uint32_t)));
+      GV_HIP(hipMalloc(&w.rl_dout, n * sizeof(uint32_t)));
+      GV_HIP(hipMalloc(&w.rl_keys[0], n * sizeof(uint64_t)));
+      GV_HIP(hipMalloc(&w.rl_keys[1], n * sizeof(uint64_t)));
+      GV_HIP(hipMalloc(&w.rl_tmp, sort_tmp_bytes(n)));
+      w.rl_V = V;
+    }
+    const uint32_t bV = id_bits(V);
+    k_rl_keys<<<grid_for(V), 256, 0, s>>>(deg_out, V, bV, w.rl_keys[0]);
+    const uint64_t *sorted = sort_keys_range(w.rl_keys[0], w.rl_keys[1], V, 0,
+                                             bV + id_bits(V + 1), w.rl_tmp, s);
+    k_rl_perm<<<grid_for(V), 256, 0, s>>>(sorted, V, bV, in.deg, deg_out, w.rl_order,
+                                          w.rl_newid, w.rl_len, w.rl_dout, w.snap_off);
+    scan_u64(w.snap_off, w.snap_off, V, nullptr, w.snap_tmp, s);
+  } else {
+    k_snap_off<<<grid_for((uint64_t)V + 1), 256, 0, s>>>(w.snap_istart, V, w.snap_ipos,
+                                                         w.snap_off);
+  }
   uint64_t E = 0;
   GV_HIP(hipMemcpyAsync(&E, w.snap_off + V, sizeof(E), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
@@ -320,8 +397,9 @@ static gv_status pr_snapshot(const Store &in, PrWork &w, hipStream_t s) {
     GV_HIP(hipMalloc(&w.snap_cols, cap * sizeof(uint32_t)));
     w.snap_cap = cap;
   }
-  k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(in, w.snap_istart, w.snap_ivert,
-                                                                 w.snap_ipos, w.snap_cols);
+  k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(
+      in, w.snap_istart, w.snap_ivert, w.snap_ipos, relabel ? w.rl_newid : nullptr, w.snap_off,
+      w.snap_cols);
   return GV_OK;
 }
 
@@ -540,19 +618,25 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   if (st) return st;
   const float x0 = 1.0f / (float)V;
   k_pr_reset<<<1, 1, 0, s>>>(w.ctl);
-  k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0, o.init);
   if (max_iter <= 0) {
+    k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0, o.init, nullptr);
     GV_HIP(hipMemcpyAsync(out_dev, w.x[0], (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
     GV_HIP(hipStreamSynchronize(s));
     return GV_OK;
   }
+  const bool relabel = snapshot == 1;
   if (snapshot) {
-    st = pr_snapshot(in, w, s);
+    st = pr_snapshot(in, deg_out, relabel, w, s);
     if (st) return st;
   }
+  // with a relabelled snapshot every per-vertex array below is in the new
+  // vertex order; the result is permuted back at the end
+  if (relabel) deg_out = w.rl_dout;
+  k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0, o.init,
+                                        relabel ? w.rl_order : nullptr);
   // row lengths the bins are cut by: live entries for the snapshot, slots
   // (tombstones included) for the store
-  const uint32_t *rowlen = snapshot ? in.deg : in.used;
+  const uint32_t *rowlen = relabel ? w.rl_len : snapshot ? in.deg : in.used;
   k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(rowlen, V, bins, w.ctl);
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
@@ -649,8 +733,11 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
   const uint32_t it = w.h_ctl->iters;
-  GV_HIP(hipMemcpyAsync(out_dev, w.x[it & 1], (size_t)V * sizeof(float), hipMemcpyDeviceToDevice,
-                        s));
+  if (relabel)
+    k_pr_unpermute<<<grid_for(V), 256, 0, s>>>(w.x[it & 1], w.rl_newid, V, out_dev);
+  else
+    GV_HIP(hipMemcpyAsync(out_dev, w.x[it & 1], (size_t)V * sizeof(float),
+                          hipMemcpyDeviceToDevice, s));
   if (iters) *iters = (int)it;
   if (residual) *residual = w.h_ctl->residual;
   if (iter_ms) {

@@ -60,8 +60,9 @@ typedef struct gv_config {
   int maintain_in_edges;      /* directed graphs: keep the in-adjacency mirror */
   int device;                 /* HIP device ordinal */
   int pr_layout;              /* PageRank: 1 = CSR snapshot of the live in-adjacency, rebuilt per
-                                 call (default); 2 = pull over the block store; 0 = XCD-segmented
-                                 relabelled snapshot */
+                                 call, vertices relabelled by descending out-degree (default);
+                                 3 = the same snapshot in the original vertex order; 2 = pull
+                                 over the block store; 0 = XCD-segmented relabelled snapshot */
 } gv_config;
 
 /* Result of one update call. */

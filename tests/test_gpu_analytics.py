@@ -71,7 +71,7 @@ def test_pagerank_closed_forms():
     assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
 @pytest.mark.parametrize("directed", [True, False])
 def test_pagerank_config1_after_updates(directed, layout):
     V = 1 << 14
@@ -91,7 +91,7 @@ def test_pagerank_config1_after_updates(directed, layout):
     check_pagerank(g, o)
 
 
-@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("layout", [1, 2, 3])
 def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     # small thresholds push vertices through every bin incl. chunked hubs
     V = 3000
