@@ -61,6 +61,7 @@ def parse():
                    help="chunglu: 1 = sweep block size and PageRank bins first")
     p.add_argument("--cl-vertices", type=int, default=8_000_000)
     p.add_argument("--cl-batches", type=int, default=256)
+    p.add_argument("--prec-scale", type=int, default=24)
     p.add_argument("--serve-scale", type=int, default=26)
     p.add_argument("--serve-seconds", type=float, default=20.0)
     return p.parse_args()

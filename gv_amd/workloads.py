@@ -252,7 +252,7 @@ def run_serve(args):
     serve_trace() replayed open-loop through the request loop.  Reports
     p50/p99 latency (arrival -> device work done) overall and per kind, and
     the sustained rate of applied edge updates."""
-    from . import GraphVine, export_csr_device, gen_rmat, stream_key
+    from . import GraphVine, export_csr_device, stream_key
     from . import serve as S
     from .hip import DeviceArray
 
@@ -260,27 +260,9 @@ def run_serve(args):
     seconds = float(getattr(args, "serve_seconds", 10.0))
     seed = 5
     V = 1 << scale
-    n_raw = 16 * V
     t_setup = time.time()
     g = GraphVine(V, directed=True)
-    # bulk build in chunks of at most 2^27 pairs so staging stays small
-    chunk = 1 << 27
-    first = True
-    for c0 in range(0, n_raw, chunk):
-        n = min(chunk, n_raw - c0)
-        # edges c0 .. c0+n-1 of the R-MAT stream: output i of a stream keyed
-        # k is mix(k + (i+1)*GOLD), so shifting the key by c0*scale*GOLD
-        # continues the stream at edge c0
-        key = (stream_key(seed, STREAM_RMAT) + c0 * scale * 0x9E3779B97F4A7C15) % (1 << 64)
-        src, dst = gen_rmat(g, key, scale, n)
-        if first:
-            g.bulkBuild(src, dst, stats=False)
-            first = False
-        else:
-            g.batchInsert(src, dst, stats=False)
-        g.sync()
-        src.free()
-        dst.free()
+    build_rmat_chunked(g, scale, seed)
     E0 = g.num_edges()
     ev = serve_trace(V, seconds, seed)
     upd = [(i, e) for i, e in enumerate(ev) if e[1] in ("insert", "delete")]
@@ -376,5 +358,78 @@ def run_serve(args):
         "steps_served": int(st["steps"]),
         "refused": int(st["refused"]),
         "pagerank_refresh_iters_mean": float(np.mean(pr_it)) if pr_it else None,
+        "setup_s": round(setup_s, 2),
+    }
+
+
+# ------------------------------------------------------------- precision --
+def build_rmat_chunked(g, scale, seed, edge_factor=16, chunk=1 << 27):
+    """Bulk-build edges 0 .. edge_factor*2^scale-1 of the R-MAT stream of
+    `seed` into g in chunks (the staging buffers stay chunk-sized).  Output i
+    of a stream keyed k is mix(k + (i+1)*GOLD), so shifting the key by
+    c0*scale*GOLD continues the stream at edge c0."""
+    from . import gen_rmat, stream_key
+    n_raw = edge_factor << scale
+    for c0 in range(0, n_raw, chunk):
+        n = min(chunk, n_raw - c0)
+        key = (stream_key(seed, STREAM_RMAT) + c0 * scale * 0x9E3779B97F4A7C15) % (1 << 64)
+        src, dst = gen_rmat(g, key, scale, n)
+        (g.bulkBuild if c0 == 0 else g.batchInsert)(src, dst, stats=False)
+        g.sync()
+        src.free()
+        dst.free()
+
+
+def run_precision(args):
+    """Precision form, throughput side: directed R-MAT (default scale 24,
+    seed 3).  PageRank with per-vertex Kahan sums and an FP64 residual to L1
+    1e-8 (at most 300 updates), then plain FP32 sums for the same number of
+    updates; BC from 64 sources (FP64 sigma, FP32 compensated
+    dependencies).  The errors against FP64 references are measured by
+    tests/precision_report.py."""
+    from . import GV_MEM_DEVICE, GraphVine, _check, lib, pagerank
+    from .hip import DeviceArray
+
+    scale = int(getattr(args, "prec_scale", 24))
+    V = 1 << scale
+    seed = 3
+    t0 = time.time()
+    g = GraphVine(V, directed=True)
+    build_rmat_chunked(g, scale, seed)
+    E = g.num_edges()
+    setup_s = time.time() - t0
+    for comp in (True, False):
+        pagerank(g, epsilon=1e-8, max_iter=2, compensated=comp)  # warm-up
+    xc, itc, rc, stc = pagerank(g, epsilon=1e-8, max_iter=300, compensated=True, stats=True)
+    xp, itp, rp, stp = pagerank(g, epsilon=0.0, max_iter=itc, compensated=False, stats=True)
+    comp_ms, plain_ms = stc["kernel_ms"] / itc, stp["kernel_ms"] / itp
+    rng = np.random.default_rng(seed)
+    srcs = DeviceArray.from_numpy(rng.choice(V, 64, replace=False).astype(np.uint32))
+    out = DeviceArray(V, np.float32)
+    _, bc_ms = _timed(g, lambda: _check(lib().gv_bc(g.handle, srcs.ptr, 64, out.ptr,
+                                                     GV_MEM_DEVICE)))
+    g.close()
+    gteps = E / (comp_ms / 1e3) / 1e9
+    return {
+        "metric": "PageRank GTEPS (compensated FP32 sums, FP64 residual)",
+        "value": gteps,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": itc,
+        "warmup": 1,
+        "ms_per_step": comp_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 + Kahan (PageRank), f64 residual; f64 sigma + f32 Kahan delta (BC)",
+        "data": "synthetic (device-generated R-MAT)",
+        "config": {"workload": f"directed R-MAT scale {scale}, edge factor 16, seed {seed}; "
+                               "PageRank to L1 1e-8 (max 300 updates), BC from 64 sources",
+                   "vertices": V, "edges": int(E)},
+        "pagerank": {"updates": itc, "final_residual": rc,
+                     "compensated_ms_per_update": comp_ms, "plain_ms_per_update": plain_ms,
+                     "plain_gteps": E / (plain_ms / 1e3) / 1e9,
+                     "compensated_cost_vs_plain": comp_ms / plain_ms},
+        "bc_64_sources": {"ms": bc_ms, "per_source_ms": bc_ms / 64},
         "setup_s": round(setup_s, 2),
     }

@@ -5,7 +5,9 @@
  * tests that check it against the sequential oracle.  It computes the same
  * set semantics as gv_oracle.c on an immutable sorted CSR that is rebuilt per
  * update batch by a parallel sort-merge (the usual static-graph answer to a
- * dynamic workload), plus pull PageRank.
+ * dynamic workload), plus pull PageRank (symmetric and directed) and
+ * Brandes BC parallel over sources, all in FP64: they are the FP64 references
+ * of the precision checks and the CPU timings of the analytics.
  */
 #include <omp.h>
 #include <math.h>
@@ -164,6 +166,118 @@ int omp_pagerank_sym(uint32_t V, const uint64_t *off, const uint32_t *cols, doub
   free(c);
   free(xn);
   return it;
+}
+
+/* In-lists of a CSR of out-lists: in_off[V+1], in_cols[E], each in-list in
+ * ascending source order (counting sort by destination). */
+void omp_transpose(uint32_t V, const uint64_t *off, const uint32_t *cols, uint64_t *in_off,
+                   uint32_t *in_cols) {
+  const uint64_t E = off[V];
+  memset(in_off, 0, sizeof(uint64_t) * ((size_t)V + 1));
+  for (uint64_t j = 0; j < E; ++j) in_off[cols[j] + 1]++;
+  for (uint32_t v = 0; v < V; ++v) in_off[v + 1] += in_off[v];
+  uint64_t *pos = (uint64_t *)malloc(sizeof(uint64_t) * (V ? V : 1));
+  memcpy(pos, in_off, sizeof(uint64_t) * V);
+  for (uint32_t u = 0; u < V; ++u)
+    for (uint64_t j = off[u]; j < off[u + 1]; ++j) in_cols[pos[cols[j]]++] = u;
+  free(pos);
+}
+
+/* Pull PageRank on a directed graph (in-lists + out-degrees), FP64, the
+ * oracle's rule: x0 = 1/V, dangling mass dropped, stop after the first update
+ * whose L1 residual is below eps or after max_iter; fixed_iters > 0 runs
+ * exactly that many updates.  Returns the number of updates. */
+int omp_pagerank_dir(uint32_t V, const uint64_t *in_off, const uint32_t *in_cols,
+                     const uint32_t *out_deg, double alpha, double eps, int max_iter,
+                     int fixed_iters, double *x, double *residual) {
+  double *c = (double *)malloc(sizeof(double) * (V ? V : 1));
+  double *xn = (double *)malloc(sizeof(double) * (V ? V : 1));
+  for (uint32_t v = 0; v < V; ++v) x[v] = 1.0 / V;
+  const double base = (1.0 - alpha) / V;
+  const int limit = fixed_iters > 0 ? fixed_iters : max_iter;
+  int it = 0;
+  double r = 0.0;
+  while (it < limit) {
+#pragma omp parallel for schedule(static)
+    for (uint32_t v = 0; v < V; ++v) c[v] = out_deg[v] ? x[v] / (double)out_deg[v] : 0.0;
+    r = 0.0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : r)
+    for (uint32_t v = 0; v < V; ++v) {
+      double s = 0.0;
+      for (uint64_t j = in_off[v]; j < in_off[v + 1]; ++j) s += c[in_cols[j]];
+      xn[v] = base + alpha * s;
+      r += fabs(xn[v] - x[v]);
+    }
+    memcpy(x, xn, sizeof(double) * V);
+    ++it;
+    if (fixed_iters <= 0 && r < eps) break;
+  }
+  if (residual) *residual = r;
+  free(c);
+  free(xn);
+  return it;
+}
+
+/* Brandes BC over a CSR of out-lists from the given sources (unnormalised,
+ * not halved, BC[v] = sum over sources s != v of delta_s(v)), FP64 sigma and
+ * delta.  Sources are split statically over the threads; each thread keeps
+ * its own BC vector and the vectors are added in thread order. */
+void omp_bc_dir(uint32_t V, const uint64_t *off, const uint32_t *cols, const uint32_t *src,
+                uint32_t ns, double *bc) {
+  const int nt = omp_get_max_threads();
+  double *part = (double *)calloc((size_t)nt * (V ? V : 1), sizeof(double));
+#pragma omp parallel num_threads(nt)
+  {
+    const int tid = omp_get_thread_num();
+    double *b = part + (size_t)tid * (V ? V : 1);
+    int *dist = (int *)malloc(sizeof(int) * (V ? V : 1));
+    double *sigma = (double *)malloc(sizeof(double) * (V ? V : 1));
+    double *delta = (double *)malloc(sizeof(double) * (V ? V : 1));
+    uint32_t *q = (uint32_t *)malloc(sizeof(uint32_t) * (V ? V : 1));
+    for (uint32_t k = (uint32_t)tid; k < ns; k += (uint32_t)nt) {
+      const uint32_t s = src[k];
+      for (uint32_t v = 0; v < V; ++v) {
+        dist[v] = -1;
+        sigma[v] = 0.0;
+        delta[v] = 0.0;
+      }
+      uint64_t qh = 0, qt = 0;
+      dist[s] = 0;
+      sigma[s] = 1.0;
+      q[qt++] = s;
+      while (qh < qt) {
+        const uint32_t u = q[qh++];
+        for (uint64_t j = off[u]; j < off[u + 1]; ++j) {
+          const uint32_t w = cols[j];
+          if (dist[w] < 0) {
+            dist[w] = dist[u] + 1;
+            q[qt++] = w;
+          }
+          if (dist[w] == dist[u] + 1) sigma[w] += sigma[u];
+        }
+      }
+      for (uint64_t i = qt; i-- > 0;) {
+        const uint32_t v = q[i];
+        double d = 0.0;
+        for (uint64_t j = off[v]; j < off[v + 1]; ++j) {
+          const uint32_t w = cols[j];
+          if (dist[w] == dist[v] + 1) d += sigma[v] / sigma[w] * (1.0 + delta[w]);
+        }
+        delta[v] = d;
+        if (v != s) b[v] += d;
+      }
+    }
+    free(dist);
+    free(sigma);
+    free(delta);
+    free(q);
+  }
+  for (uint32_t v = 0; v < V; ++v) {
+    double t = 0.0;
+    for (int k = 0; k < nt; ++k) t += part[(size_t)k * V + v];
+    bc[v] = t;
+  }
+  free(part);
 }
 
 int omp_threads(void) { return omp_get_max_threads(); }

@@ -63,6 +63,9 @@ def lib():
         "omp_apply_batch": ([u32, i32, P, P, P, P, u64, i32, P, P], u64),
         "omp_pagerank_sym": ([u32, P, P, f64, f64, i32, P], i32),
         "omp_threads": ([], i32),
+        "omp_transpose": ([u32, P, P, P, P], None),
+        "omp_pagerank_dir": ([u32, P, P, P, f64, f64, i32, i32, P, ctypes.POINTER(f64)], i32),
+        "omp_bc_dir": ([u32, P, P, P, u32, P], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -217,6 +220,39 @@ def omp_pagerank_sym(V, off, cols, alpha=0.85, eps=1e-6, max_iter=100):
     cols = u32(cols) if len(cols) else np.zeros(1, np.uint32)
     it = lib().omp_pagerank_sym(V, _p(off), _p(cols), alpha, eps, max_iter, _p(x))
     return x, int(it)
+
+
+def omp_transpose(V, off, cols):
+    """In-lists (offsets, cols) of a CSR of out-lists."""
+    off = np.ascontiguousarray(off, np.uint64)
+    cols = u32(cols) if len(cols) else np.zeros(1, np.uint32)
+    ioff = np.empty(V + 1, np.uint64)
+    icols = np.empty(max(int(off[-1]), 1), np.uint32)
+    lib().omp_transpose(V, _p(off), _p(cols), _p(ioff), _p(icols))
+    return ioff, icols[: int(off[-1])]
+
+
+def omp_pagerank_dir(V, off, cols, alpha=0.85, eps=1e-6, max_iter=100, fixed_iters=0):
+    """FP64 pull PageRank of the directed graph given by out-list CSR
+    (off, cols); returns (x, iters, residual)."""
+    off = np.ascontiguousarray(off, np.uint64)
+    ioff, icols = omp_transpose(V, off, cols)
+    icols = icols if len(icols) else np.zeros(1, np.uint32)
+    od = np.diff(off).astype(np.uint32)
+    x = np.empty(V, np.float64)
+    r = ctypes.c_double()
+    it = lib().omp_pagerank_dir(V, _p(ioff), _p(icols), _p(od), alpha, eps, max_iter, fixed_iters,
+                                _p(x), ctypes.byref(r))
+    return x, int(it), float(r.value)
+
+
+def omp_bc_dir(V, off, cols, sources):
+    off = np.ascontiguousarray(off, np.uint64)
+    cols = u32(cols) if len(cols) else np.zeros(1, np.uint32)
+    s = u32(sources)
+    out = np.empty(V, np.float64)
+    lib().omp_bc_dir(V, _p(off), _p(cols), _p(s) if len(s) else None, len(s), _p(out))
+    return out
 
 
 def omp_threads():

@@ -431,3 +431,27 @@ def test_omp_baseline_matches_oracle():
     x, it = O.omp_pagerank_sym(V, off, cols)
     xo, ito, _ = g.pagerank()
     assert it == ito and np.abs(x - xo).sum() < 1e-12
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_omp_fp64_references_match_the_oracle(directed):
+    """The OpenMP FP64 PageRank and BC (the references of the precision
+    checks) give the sequential oracle's values on the same graph."""
+    V = 1 << 11
+    s, d = O.gen_rmat(O.stream_key(61, O.STREAM_RMAT), 11, 8 * V)
+    o = O.OracleGraph(V, directed)
+    o.bulk_build(s, d)
+    off, cols = o.export_csr()
+    x, it, r = O.omp_pagerank_dir(V, off, cols, eps=1e-10, max_iter=500)
+    xo, ito, ro = o.pagerank(eps=1e-10, max_iter=500)
+    assert it == ito and abs(r - ro) <= 1e-12 * max(r, 1e-30) + 1e-18
+    assert np.abs(x - xo).max() <= 1e-15
+    x5, _, _ = O.omp_pagerank_dir(V, off, cols, fixed_iters=5)
+    xo5, _, _ = o.pagerank(fixed_iters=5)
+    assert np.abs(x5 - xo5).max() <= 1e-15
+    src = np.random.default_rng(61).choice(V, 9, replace=False).astype(np.uint32)
+    b, bo = O.omp_bc_dir(V, off, cols, src), o.bc(src)
+    assert np.allclose(b, bo, rtol=1e-12, atol=1e-9)
+    ioff, icols = O.omp_transpose(V, off, cols)
+    assert ioff[-1] == off[-1]
+    assert np.array_equal(np.diff(ioff), np.bincount(cols, minlength=V))
