@@ -43,7 +43,8 @@ EXPORTED = [
     "gv_triangle_count",
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_gen_chung_lu",
     "gv_get_config", "gv_set_tuning", "gv_reserve", "gv_capture_begin", "gv_capture_end",
-    "gv_capture_launch", "gv_capture_destroy", "gv_sync", "gv_get_stream",
+    "gv_capture_launch", "gv_capture_destroy", "gv_read_matrix_market", "gv_sync",
+    "gv_get_stream",
     "gv_last_error",
 ]
 
@@ -155,6 +156,8 @@ def lib():
         "gv_capture_end": [P, ctypes.POINTER(P)],
         "gv_capture_launch": [P, ctypes.POINTER(GvBatchStats)],
         "gv_capture_destroy": [P],
+        "gv_read_matrix_market": [ctypes.c_char_p, ctypes.POINTER(u32), ctypes.POINTER(u64), P,
+                                  P, ctypes.POINTER(i32)],
         "gv_sync": [P],
         "gv_get_stream": [P, ctypes.POINTER(P)],
     }
@@ -490,6 +493,21 @@ def gen_sample(graph, key, V, offsets_dev, cols_dev, n):
     _check(lib().gv_gen_sample(graph.handle, key, V, offsets_dev.ptr, cols_dev.ptr, n, src.ptr,
                                dst.ptr))
     return src, dst
+
+
+def read_matrix_market(path):
+    """(num_vertices, src, dst, symmetric) of a MatrixMarket coordinate file,
+    0-based uint32 host arrays (reference: readFile, marketReader.cu:3-121)."""
+    V, n, sym = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_int()
+    p = os.fsencode(path)
+    _check(lib().gv_read_matrix_market(p, ctypes.byref(V), ctypes.byref(n), None, None,
+                                       ctypes.byref(sym)))
+    src = np.empty(max(n.value, 1), np.uint32)
+    dst = np.empty(max(n.value, 1), np.uint32)
+    _check(lib().gv_read_matrix_market(p, ctypes.byref(V), ctypes.byref(n),
+                                       src.ctypes.data_as(ctypes.c_void_p),
+                                       dst.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sym)))
+    return int(V.value), src[: n.value], dst[: n.value], bool(sym.value)
 
 
 def gen_chung_lu(graph, key, V, n, gamma=2.1, avg_deg=24.0, max_deg=200000.0):

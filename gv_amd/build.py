@@ -19,7 +19,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgv.so")
 
 SOURCES = ["gv_prims.hip", "gv_store.hip", "gv_analytics.hip", "gv_pagerank_seg.hip",
-           "gv_multi.hip", "gv_api.hip", "gv_serve.hip"]
+           "gv_multi.hip", "gv_api.hip", "gv_serve.hip", "gv_io.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GV_OFFLOAD_ARCH", "gfx950")

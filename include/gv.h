@@ -201,6 +201,19 @@ gv_status gv_gen_sample(gv_graph *g, uint64_t key, uint32_t V, const uint64_t *o
 gv_status gv_gen_chung_lu(gv_graph *g, uint64_t key, uint32_t V, double gamma, double avg_deg,
                           double max_deg, uint64_t n, uint32_t *src_dev, uint32_t *dst_dev);
 
+/* MatrixMarket coordinate file -> 0-based (src,dst) pairs (readFile,
+ * src/reader/marketReader.cu:3-121).  Lines starting with '%' are skipped;
+ * the first other line holds "rows cols nnz"; each entry line holds a 1-based
+ * "row col [value ...]" (values are ignored).  num_vertices = max(rows, cols).
+ * Two calls: with src = dst = NULL it returns num_vertices and *n = nnz; then
+ * with buffers of *n entries it fills them.  *symmetric (may be NULL) is 1
+ * when the banner declares a symmetric or Hermitian matrix, whose file holds
+ * one triangle: load it into an undirected graph (or insert both ways).
+ * Fails with GV_E_INVALID on an unreadable file, a malformed line, an entry
+ * outside 1..rows x 1..cols or an entry count other than nnz. */
+gv_status gv_read_matrix_market(const char *path, uint32_t *num_vertices, uint64_t *n,
+                                uint32_t *src, uint32_t *dst, int *symmetric);
+
 /* Tuning.  gv_set_tuning changes, on a live handle, the fields that do not
  * move memory: compact_threshold, pr_thread_max, pr_group_max, pr_wave_max
  * and pr_layout (the other fields of cfg are ignored).  Used by the

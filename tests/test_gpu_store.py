@@ -232,3 +232,25 @@ def test_device_resident_batches_and_device_export():
     eo, ec = o.export_csr()
     assert np.array_equal(off.numpy(), eo) and np.array_equal(cols.numpy(E), ec)
     g.close()
+
+
+def test_graph_from_matrix_market_file(tmp_path):
+    """A symmetric MatrixMarket file (one triangle stored) loaded into an
+    undirected graph gives the oracle's symmetric graph of the same pairs."""
+    V = 2000
+    s, d = O.gen_rmat(O.stream_key(81, O.STREAM_RMAT), 11, 20000)
+    s, d = s % V, d % V
+    lo, hi = np.minimum(s, d), np.maximum(s, d)
+    keep = lo != hi
+    lines = ["%%MatrixMarket matrix coordinate pattern symmetric", f"{V} {V} {int(keep.sum())}"]
+    lines += [f"{h + 1} {l + 1}" for l, h in zip(lo[keep], hi[keep])]
+    p = tmp_path / "g.mtx"
+    p.write_text("\n".join(lines) + "\n")
+    Vr, ms, md, sym = G.read_matrix_market(str(p))
+    assert Vr == V and sym
+    g = make(V, False)
+    g.bulkBuild(ms, md)
+    o = O.OracleGraph(V, False)
+    o.bulk_build(s, d)
+    assert_same(g, o)
+    g.close()
