@@ -444,6 +444,11 @@ struct PrArgs {
   uint32_t n_chunks, wave_max;
   uint32_t max_iter;
   uint32_t compensated;  // 1: Kahan sums per lane; 0: plain FP32 sums
+  uint32_t V;
+  // staged form (k_pr_hot): contributions of vertices [0, nh) come from LDS;
+  // work items are the hub chunks, then bin-2 rows from ib2, bin-1 groups of
+  // 8 rows from ib1, bin-0 groups of 64 rows from ib0, n_items in all
+  uint32_t nh, ib2, ib1, ib0, n_items;
 };
 
 // Kahan-accumulate contributions of positions start, start+STRIDE, ... < end
@@ -604,6 +609,181 @@ __global__ __launch_bounds__(1024) void k_pr_resid(PrArgs a) {
   }
 }
 
+// ------------------------------------------- staged iteration (LDS hot set) --
+// With the snapshot relabelled by descending out-degree, the most gathered
+// contributions are those of the lowest labels: at R-MAT scale 22 the first
+// 32768 vertices are the sources of about half of all edges.  The gather rate
+// of random 4-byte loads, not HBM bandwidth, bounds the pull, so every
+// workgroup copies cb[0, nh) into LDS once per iteration and takes those
+// gathers from LDS; only the rest go through L1/L2.  One workgroup per CU
+// (128 KiB of LDS each), persistent: wave w of W takes work items w, 2W-1-w,
+// 2W+w, ... (a snake over items ordered heaviest first, so every wave gets a
+// near-equal share of edges with no shared counter).
+constexpr int PRH_THREADS = 1024;
+constexpr uint32_t PRH_HOT = 32768;
+
+// Offset past the contribution buffer's range: a raw buffer load there
+// returns 0 without a memory access.
+constexpr uint32_t PRH_OOB = 0x80000000u;
+
+// Both loads of every entry are issued unconditionally (no exec-mask
+// branches, so all ILP loads are in flight together): the LDS read at
+// w mod PRH_HOT, and a buffer load of cb[w] whose offset is pushed out of
+// range for hot and empty entries, so only cold entries reach L1/L2.
+template <int STRIDE, int ILP>
+__device__ __forceinline__ void prh_gather(const PrArgs &a, const CsrAdj &adj,
+                                           __amdgpu_buffer_rsrc_t rcb, const float *s_hot,
+                                           uint64_t b, uint32_t start, uint32_t end, float &s,
+                                           float &c) {
+  for (uint32_t j = start; j < end; j += ILP * STRIDE) {
+    uint32_t w[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint32_t pos = j + k * STRIDE;
+      w[k] = pos < end ? adj.at(b, pos) : TOMB;
+    }
+    float hv[ILP], gv[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint32_t off = w[k] < a.nh || w[k] == TOMB ? PRH_OOB : w[k] * 4u;
+      gv[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rcb, (int)off, 0, 0));
+      hv[k] = s_hot[w[k] & (PRH_HOT - 1)];
+    }
+    // keep every load of the batch ahead of the first use
+    __builtin_amdgcn_sched_barrier(0);
+    float g[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) g[k] = w[k] < a.nh ? hv[k] : gv[k];
+    if (a.compensated) {
+#pragma unroll
+      for (int k = 0; k < ILP; ++k)
+        if (w[k] != TOMB) kahan_add(s, c, g[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < ILP; ++k) s += g[k];
+    }
+  }
+}
+
+template <int ILP_W, int ILP_G, int ILP_L>
+__global__ __launch_bounds__(PRH_THREADS) void k_pr_hot(PrArgs a, CsrAdj adj, const float *x,
+                                                        const float *cb, float *xn, float *cn) {
+  __shared__ float s_hot[PRH_HOT];
+  if (a.ctl->done) return;
+  {
+    // all of a thread's 16-byte loads in flight at once, then the LDS stores
+    constexpr int PER = PRH_HOT / 4 / PRH_THREADS;
+    const uint32_t n4 = a.nh / 4;
+    const float4 *cb4 = reinterpret_cast<const float4 *>(cb);
+    float4 *s4 = reinterpret_cast<float4 *>(s_hot);
+    float4 t[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * PRH_THREADS;
+      t[k] = i < n4 ? cb4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * PRH_THREADS;
+      if (i < n4) s4[i] = t[k];
+    }
+    for (uint32_t i = n4 * 4 + threadIdx.x; i < a.nh; i += PRH_THREADS) s_hot[i] = cb[i];
+  }
+  __syncthreads();
+  // gfx9 buffer descriptor: raw (stride 0), range V*4 bytes
+  const __amdgpu_buffer_rsrc_t rcb =
+      __builtin_amdgcn_make_buffer_rsrc((void *)cb, (short)0, (int)(a.V * 4u), 0x00020000);
+  const uint32_t lane = lane_id();
+  const uint32_t W = gridDim.x * (PRH_THREADS / WAVE);
+  const uint32_t gw = blockIdx.x * (PRH_THREADS / WAVE) + threadIdx.x / WAVE;
+  double diff = 0.0;
+  for (uint32_t r = 0;; ++r) {
+    const uint32_t it = r * W + ((r & 1) ? W - 1 - gw : gw);
+    if (it >= a.n_items) break;  // only the last round is partial
+    float s = 0.f, c = 0.f;
+    if (it < a.ib2) {
+      // hub chunk: its partial is merged by k_pr_hotfin in chunk order
+      const uint32_t h = a.chunk_hub[it];
+      const uint32_t v = a.list[a.o3 + h];
+      const uint32_t us = adj.len(v);
+      const uint32_t c0 = (it - a.hub_base[h]) * a.wave_max;
+      const uint32_t c1 = c0 + a.wave_max < us ? c0 + a.wave_max : us;
+      prh_gather<WAVE, ILP_W>(a, adj, rcb, s_hot, adj.base(v), c0 + lane, c1, s, c);
+      group_merge<WAVE>(s, c);
+      if (lane == 0) a.chunk_part[it] = make_float2(s, c);
+    } else if (it < a.ib1) {
+      const uint32_t v = a.list[a.o2 + (it - a.ib2)];
+      prh_gather<WAVE, ILP_W>(a, adj, rcb, s_hot, adj.base(v), lane, adj.len(v), s, c);
+      group_merge<WAVE>(s, c);
+      if (lane == 0) diff += pr_finish(a, v, s, c, x, xn, cn);
+    } else if (it < a.ib0) {
+      const uint32_t g = (it - a.ib1) * (WAVE / PR_GROUP) + lane / PR_GROUP;
+      const uint32_t gl = lane & (PR_GROUP - 1);
+      uint32_t v = 0, us = 0;
+      uint64_t b = 0;
+      if (g < a.n1) {
+        v = a.list[a.o1 + g];
+        us = adj.len(v);
+        b = adj.base(v);
+      }
+      prh_gather<PR_GROUP, ILP_G>(a, adj, rcb, s_hot, b, gl, us, s, c);
+      group_merge<PR_GROUP>(s, c);
+      if (g < a.n1 && gl == 0) diff += pr_finish(a, v, s, c, x, xn, cn);
+    } else {
+      const uint32_t i = (it - a.ib0) * WAVE + lane;
+      if (i < a.n0) {
+        const uint32_t v = a.list[i];
+        prh_gather<1, ILP_L>(a, adj, rcb, s_hot, adj.base(v), 0, adj.len(v), s, c);
+        diff += pr_finish(a, v, s, c, x, xn, cn);
+      }
+    }
+  }
+  const double bs = block_sum<double, PRH_THREADS>(diff);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+}
+
+// Hub chunks merged in chunk order, the hub updates, then the fixed-order
+// FP64 residual over the workgroup and hub partials and the stopping rule of
+// static_pagerank (pagerank.cu:56).
+__global__ __launch_bounds__(1024) void k_pr_hotfin(PrArgs a, CsrAdj adj, const float *x,
+                                                    float *xn, float *cn, uint32_t nparts) {
+  if (a.ctl->done) return;
+  double r = 0.0;
+  for (uint32_t h = threadIdx.x; h < a.n3; h += 1024) {
+    const uint32_t v = a.list[a.o3 + h];
+    const uint32_t nch = (adj.len(v) + a.wave_max - 1) / a.wave_max;
+    const uint32_t hb = a.hub_base[h];
+    float ts = 0.f, tc = 0.f;
+    for (uint32_t k = 0; k < nch; ++k) {
+      const float2 p = a.chunk_part[hb + k];
+      two_sum_merge(ts, tc, p.x, p.y);
+    }
+    r += pr_finish(a, v, ts, tc, x, xn, cn);
+  }
+  for (uint32_t i = threadIdx.x; i < nparts; i += 1024) r += a.partials[i];
+  r = block_sum<double, 1024>(r);
+  if (threadIdx.x == 0) {
+    a.ctl->residual = r;
+    const unsigned it = a.ctl->iters + 1;
+    a.ctl->iters = it;
+    if (r < a.eps || it >= a.max_iter) a.ctl->done = 1;
+  }
+}
+
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
                        const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
                        hipStream_t s, double *iter_ms, int snapshot) {
@@ -677,7 +857,16 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   a.wave_max = bins.wave_max;
   a.max_iter = (uint32_t)max_iter;
   a.compensated = o.compensated ? 1u : 0u;
-  const uint32_t grid = a.U ? a.U : 1;
+  a.V = V;
+  a.nh = V < PRH_HOT ? V : PRH_HOT;
+  a.ib2 = nchunks;
+  a.ib1 = a.ib2 + n2;
+  a.ib0 = a.ib1 + (uint32_t)cdiv(n1, WAVE / PR_GROUP);
+  a.n_items = a.ib0 + (uint32_t)cdiv(n0, WAVE);
+  // staged form: one persistent workgroup per CU, relabelled snapshot only
+  const bool staged = relabel && V <= PRH_OOB / 4;
+  const uint32_t hot_grid = (uint32_t)device_cus();
+  const uint32_t grid = staged ? hot_grid : a.U ? a.U : 1;
   if (n3)
     k_pr_hubs<<<grid_for(n3), 256, 0, s>>>(rowlen, w.list, a.o3, n3, bins.wave_max, w.hub_base,
                                            w.chunk_hub, &w.ctl->chunk_cursor);
@@ -704,7 +893,10 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
     const float *x = w.x[t & 1], *cb = w.contrib[t & 1];
     float *xn = w.x[(t + 1) & 1], *cn = w.contrib[(t + 1) & 1];
-    if (snapshot) {
+    if (staged) {
+      k_pr_hot<16, 8, 8><<<hot_grid, PRH_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
+      k_pr_hotfin<<<1, 1024, 0, s>>>(a, cadj, x, xn, cn, hot_grid);
+    } else if (snapshot) {
       if (a.U3) {
         k_pr_iter<CsrAdj, 32, 3><<<a.U3, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
         k_pr_hubfin<CsrAdj><<<grid_for(a.n3), 256, 0, s>>>(a, cadj, x, xn, cn);
@@ -721,7 +913,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
       if (a.U1) k_pr_iter<StoreAdj, 8, 1><<<a.U1, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
       if (a.U0) k_pr_iter<StoreAdj, 8, 0><<<a.U0, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
     }
-    k_pr_resid<<<1, 1024, 0, s>>>(a);
+    if (!staged) k_pr_resid<<<1, 1024, 0, s>>>(a);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
     ++launched;
     if (launched % check_every == 0 && launched < max_iter) {

@@ -114,6 +114,30 @@ def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     check_pagerank(g, o)
 
 
+@pytest.mark.parametrize("V", [40000, 1 << 17])
+@pytest.mark.parametrize("directed", [False, True])
+def test_pagerank_staged_hot_and_cold_sources(V, directed):
+    # more vertices than the 32768 contributions staged in LDS, so gathers
+    # take both the LDS and the global path; small bin limits put vertices in
+    # every bin, and a star hub is split into many chunks
+    s, d = O.gen_rmat(O.stream_key(5, O.STREAM_RMAT), 17, 12 * V)
+    s, d = s % V, d % V
+    s = np.concatenate([s, np.full(V - 1, 7, np.uint32)])
+    d = np.concatenate([d, np.arange(V, dtype=np.uint32)[np.arange(V) != 7]])
+    g, o = build_pair(V, directed, s, d, pool=24 * V, pr_thread_max=4, pr_group_max=48,
+                      pr_wave_max=1024, pr_layout=1)
+    check_pagerank(g, o)
+    x1, it1, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+    x2, it2, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+    assert it1 == it2 and np.array_equal(x1, x2)  # deterministic
+    off, cols = o.export_csr()
+    sd, dd = O.gen_sample(O.stream_key(5, 2000), V, off, cols, V // 4)
+    g.batchDelete(sd, dd)  # tombstones: the snapshot keeps live entries only
+    o.delete(sd, dd)
+    check_pagerank(g, o)
+    g.close()
+
+
 @pytest.mark.parametrize("V", [5, 9, 20001, 70000])
 def test_pagerank_segmented_snapshot_bins_and_odd_sizes(V):
     # hub with > 16384 in-edges (one workgroup per segment), vertex counts
