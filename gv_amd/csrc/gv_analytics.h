@@ -7,6 +7,21 @@
 
 namespace gv {
 
+// Flat stream layout of the PageRank snapshot (gv_pagerank_flat.hip): the
+// row-major in-adjacency is cut into tiles of PF_T entries; lane l of a
+// wavefront owns entries [l*PF_K, (l+1)*PF_K) of a tile, and entry k of
+// every lane is stored contiguously (PF_T/PF_K lanes apart in the stream
+// order), so each of a lane's PF_K loads is one contiguous 256-byte read for
+// the wavefront.  The first entry of every row carries PF_FLAG.
+constexpr uint32_t PF_K = 16;
+constexpr uint32_t PF_T = 64 * PF_K;
+constexpr uint32_t PF_FLAG = 0x80000000u;
+constexpr uint32_t PF_PAD = 0x7FFFFFFFu;  // padding entry of the last tile
+__host__ __device__ inline uint64_t pf_phys(uint64_t p) {
+  const uint64_t t = p / PF_T, r = p % PF_T;
+  return t * PF_T + (r % PF_K) * 64 + r / PF_K;
+}
+
 // Device state of one PageRank run.
 struct PrCtl {
   double residual;
@@ -50,6 +65,17 @@ struct PrWork {
   uint64_t *rl_keys[2] = {nullptr, nullptr};
   void *rl_tmp = nullptr;
   uint32_t rl_V = 0;
+  uint64_t snap_E = 0;  // live entries of the last snapshot
+  // flat iteration (per snapshot): per tile, the nonempty row in progress
+  // before the tile and the partial sums of its first and last rows; per
+  // nonempty row, its label, stream offset and in-tile sum; empty rows' labels
+  uint32_t pf_V = 0;
+  uint64_t pf_tiles_cap = 0;
+  int32_t *pf_tbase = nullptr;
+  float2 *pf_P = nullptr, *pf_Q = nullptr, *pf_S = nullptr;
+  uint32_t *pf_nzlabel = nullptr, *pf_nzoff = nullptr, *pf_empty = nullptr, *pf_nzidx = nullptr;
+  uint32_t *pf_lidx = nullptr, *pf_long = nullptr;  // rows spanning many tiles
+  double *pf_partials = nullptr;
 };
 
 // Run parameters of one PageRank call.
@@ -93,12 +119,28 @@ gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBin
                            PrWork &pw, PsWork &w, hipStream_t s, double *iter_ms = nullptr);
 
 gv_status pr_reserve(PrWork &w, uint32_t V);
+// CSR snapshot of the live in-adjacency (relabelled by descending out-degree
+// when relabel); flat != 0 writes the tiled stream layout above.
+gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
+                      hipStream_t s, int flat);
+void pr_init_vectors(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0,
+                     const float *init, const uint32_t *order, hipStream_t s);
+void pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out, hipStream_t s);
+// Flat-stream PageRank over the relabelled snapshot; same contract as
+// pagerank_run.  Returns GV_E_UNSUPPORTED when the graph exceeds the
+// layout's 32-bit stream positions (the caller then uses the row form).
+gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOpts &o,
+                            float *out_dev, int *iters, double *residual, PrWork &w,
+                            hipStream_t s, double *iter_ms);
 void pr_release(PrWork &w);
 // Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
-// snapshot = 1: iterate over a CSR copy of the live in-adjacency built at the
-// start of the call, vertices relabelled by descending out-degree; 2: the
-// same copy in the original vertex order; 0: iterate over the block store.
+// snapshot = 1: iterate over a copy of the live in-adjacency built at the
+// start of the call, vertices relabelled by descending out-degree, as the
+// flat tiled stream (falling back to the per-row form of 3 when the graph
+// exceeds its 32-bit positions); 3: the same relabelled copy as CSR rows,
+// one row per work item; 2: CSR rows in the original vertex order; 0:
+// iterate over the block store.
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
                        const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
                        hipStream_t s, double *iter_ms = nullptr, int snapshot = 1);

@@ -87,6 +87,17 @@ void pr_release(PrWork &w) {
   hipFree(w.rl_keys[0]);
   hipFree(w.rl_keys[1]);
   hipFree(w.rl_tmp);
+  hipFree(w.pf_tbase);
+  hipFree(w.pf_P);
+  hipFree(w.pf_Q);
+  hipFree(w.pf_S);
+  hipFree(w.pf_nzlabel);
+  hipFree(w.pf_nzoff);
+  hipFree(w.pf_empty);
+  hipFree(w.pf_nzidx);
+  hipFree(w.pf_lidx);
+  hipFree(w.pf_long);
+  hipFree(w.pf_partials);
   w = PrWork();
 }
 
@@ -189,6 +200,13 @@ __global__ __launch_bounds__(256) void k_pr_place(const uint32_t *used, uint32_t
 }
 
 void pr_ctl_reset(PrCtl *ctl, hipStream_t s) { k_pr_reset<<<1, 1, 0, s>>>(ctl); }
+void pr_init_vectors(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0,
+                     const float *init, const uint32_t *order, hipStream_t s) {
+  k_pr_init<<<grid_for(V), 256, 0, s>>>(x, cb, deg_out, V, x0, init, order);
+}
+void pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out, hipStream_t s) {
+  k_pr_unpermute<<<grid_for(V), 256, 0, s>>>(x, newid, V, out);
+}
 void pr_bins_count(const uint32_t *deg, uint32_t V, const PrBins &b, PrCtl *ctl, hipStream_t s) {
   k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(deg, V, b, ctl);
 }
@@ -267,11 +285,14 @@ __global__ void k_snap_off(const uint32_t *istart, uint32_t V, const uint64_t *i
 // One wavefront per item: live slots packed, in slot order, at the item's
 // output position.  With a relabelling (newid non-null) the row of old vertex
 // v is row newid[v] of the snapshot (offsets off) and every entry w is
-// written as newid[w]; the entry order within the row is kept.
+// written as newid[w]; the entry order within the row is kept.  flat != 0
+// writes the tiled stream layout of the flat PageRank iteration instead:
+// entry p of the row-major stream is stored at pf_phys(p) and the first
+// entry of every row carries PF_FLAG.
 __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *istart,
                                                     const uint32_t *ivert, const uint64_t *ipos,
                                                     const uint32_t *newid, const uint64_t *off,
-                                                    uint32_t *cols) {
+                                                    uint32_t *cols, uint32_t flat) {
   const uint32_t ni = istart[st.V];
   const uint32_t lane = lane_id();
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
@@ -282,13 +303,19 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
     const uint32_t lo = ((uint32_t)i - istart[v]) * SNAP_CH;
     const uint32_t hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
     uint64_t dst = ipos[i];
-    if (newid) dst = off[newid[v]] + (dst - ipos[istart[v]]);
+    const uint64_t row0 = newid ? off[newid[v]] : ipos[istart[v]];
+    if (newid) dst = row0 + (dst - ipos[istart[v]]);
     for (uint32_t p = lo; p < hi; p += WAVE) {
       uint32_t w = TOMB;
       if (p + lane < hi) w = st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)];
       const bool live = w != TOMB;
       const unsigned long long m = __ballot(live);
-      if (live) cols[dst + mask_lt_count(m)] = newid ? newid[w] : w;
+      if (live) {
+        const uint64_t q = dst + mask_lt_count(m);
+        const uint32_t val = newid ? newid[w] : w;
+        if (flat) cols[pf_phys(q)] = val | (q == row0 ? PF_FLAG : 0u);
+        else cols[q] = val;
+      }
       dst += (uint64_t)__popcll(m);
     }
   }
@@ -319,8 +346,8 @@ __global__ void k_rl_perm(const uint64_t *sorted, uint32_t V, uint32_t bV, const
   }
 }
 
-static gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
-                             hipStream_t s) {
+gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
+                      hipStream_t s, int flat) {
   const uint32_t V = in.V;
   // items: at most one per vertex plus one per SNAP_CH slots of the pool
   const uint64_t icap = (uint64_t)V + in.P * (uint64_t)in.B / SNAP_CH + 1;
@@ -389,17 +416,20 @@ static gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool rela
   uint64_t E = 0;
   GV_HIP(hipMemcpyAsync(&E, w.snap_off + V, sizeof(E), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
-  if (!w.snap_cols || E > w.snap_cap) {
+  w.snap_E = E;
+  // the flat layout pads the stream to whole tiles
+  const uint64_t need = flat ? cdiv(E, PF_T) * PF_T : E;
+  if (!w.snap_cols || need > w.snap_cap) {
     hipFree(w.snap_cols);
     w.snap_cols = nullptr;
     w.snap_cap = 0;
-    const uint64_t cap = E + E / 8 + 1024;
+    const uint64_t cap = need + need / 8 + PF_T;
     GV_HIP(hipMalloc(&w.snap_cols, cap * sizeof(uint32_t)));
     w.snap_cap = cap;
   }
   k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(
       in, w.snap_istart, w.snap_ivert, w.snap_ipos, relabel ? w.rl_newid : nullptr, w.snap_off,
-      w.snap_cols);
+      w.snap_cols, flat ? 1u : 0u);
   return GV_OK;
 }
 
@@ -796,6 +826,11 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   if (!V) return GV_OK;
   gv_status st = pr_reserve(w, V);
   if (st) return st;
+  if (snapshot == 1) {
+    st = pagerank_flat_run(in, deg_out, o, out_dev, iters, residual, w, s, iter_ms);
+    if (st != GV_E_UNSUPPORTED) return st;
+    st = GV_OK;
+  }
   const float x0 = 1.0f / (float)V;
   k_pr_reset<<<1, 1, 0, s>>>(w.ctl);
   if (max_iter <= 0) {
@@ -804,9 +839,9 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     GV_HIP(hipStreamSynchronize(s));
     return GV_OK;
   }
-  const bool relabel = snapshot == 1;
+  const bool relabel = snapshot == 1 || snapshot == 3;
   if (snapshot) {
-    st = pr_snapshot(in, deg_out, relabel, w, s);
+    st = pr_snapshot(in, deg_out, relabel, w, s, 0);
     if (st) return st;
   }
   // with a relabelled snapshot every per-vertex array below is in the new

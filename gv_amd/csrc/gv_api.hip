@@ -409,8 +409,8 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
     set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
     return GV_E_INVALID;
   }
-  if (cfg.pr_layout < 0 || cfg.pr_layout > 3) {
-    set_error("pr_layout must be 0, 1, 2 or 3");
+  if (cfg.pr_layout < 0 || cfg.pr_layout > 4) {
+    set_error("pr_layout must be 0, 1, 2, 3 or 4");
     return GV_E_INVALID;
   }
   hipError_t e = hipSetDevice(cfg.device);
@@ -765,7 +765,11 @@ static gv_status pagerank_common(gv_graph *g, const gv_pagerank_opts *po, float 
                           g->stream, iter_ms);
   } else {
     rc = pagerank_run(in, g->st[0].deg, bins, o, dout, iters, residual, g->pr, g->stream,
-                      iter_ms, g->cfg.pr_layout == 1 ? 1 : g->cfg.pr_layout == 3 ? 2 : 0);
+                      iter_ms,
+                      g->cfg.pr_layout == 1   ? 1
+                      : g->cfg.pr_layout == 3 ? 2
+                      : g->cfg.pr_layout == 4 ? 3
+                                              : 0);
   }
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
@@ -1094,8 +1098,8 @@ gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg) {
     set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
     return GV_E_INVALID;
   }
-  if (cfg->pr_layout < 0 || cfg->pr_layout > 3) {
-    set_error("pr_layout must be 0, 1, 2 or 3");
+  if (cfg->pr_layout < 0 || cfg->pr_layout > 4) {
+    set_error("pr_layout must be 0, 1, 2, 3 or 4");
     return GV_E_INVALID;
   }
   g->cfg.compact_threshold = cfg->compact_threshold;

@@ -71,7 +71,7 @@ def test_pagerank_closed_forms():
     assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2, 3])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("directed", [True, False])
 def test_pagerank_config1_after_updates(directed, layout):
     V = 1 << 14
@@ -91,7 +91,7 @@ def test_pagerank_config1_after_updates(directed, layout):
     check_pagerank(g, o)
 
 
-@pytest.mark.parametrize("layout", [1, 2, 3])
+@pytest.mark.parametrize("layout", [1, 2, 3, 4])
 def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     # small thresholds push vertices through every bin incl. chunked hubs
     V = 3000
@@ -114,18 +114,20 @@ def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     check_pagerank(g, o)
 
 
+@pytest.mark.parametrize("layout", [1, 4])
 @pytest.mark.parametrize("V", [40000, 1 << 17])
 @pytest.mark.parametrize("directed", [False, True])
-def test_pagerank_staged_hot_and_cold_sources(V, directed):
+def test_pagerank_staged_hot_and_cold_sources(V, directed, layout):
     # more vertices than the 32768 contributions staged in LDS, so gathers
-    # take both the LDS and the global path; small bin limits put vertices in
-    # every bin, and a star hub is split into many chunks
+    # take both the LDS and the global path; the star hub's row spans many
+    # stream tiles (layout 1) or chunks (layout 4, with small bin limits that
+    # put vertices in every bin); directed graphs have empty in-rows
     s, d = O.gen_rmat(O.stream_key(5, O.STREAM_RMAT), 17, 12 * V)
     s, d = s % V, d % V
     s = np.concatenate([s, np.full(V - 1, 7, np.uint32)])
     d = np.concatenate([d, np.arange(V, dtype=np.uint32)[np.arange(V) != 7]])
     g, o = build_pair(V, directed, s, d, pool=24 * V, pr_thread_max=4, pr_group_max=48,
-                      pr_wave_max=1024, pr_layout=1)
+                      pr_wave_max=1024, pr_layout=layout)
     check_pagerank(g, o)
     x1, it1, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
     x2, it2, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
