@@ -15,11 +15,7 @@
 // a fixed order by a one-block kernel, which also sets the done flag, so
 // iterations need no host round trip.
 //
-// Triangle counting (static_tc, tc.cu:3-27, tc_kernel.cu:3-74): degree-ordered
-// orientation (rank = (degree, id)), sorted oriented lists, one wavefront per
-// root vertex intersecting N+(v) for all v in N+(u) against N+(u) staged in
-// LDS.  Each triangle is found once; per-vertex counts attribute it to its
-// minimum vertex ID.
+// Triangle counting (static_tc, tc.cu:3-27) lives in gv_tc.hip.
 //
 // BC (bc, bc.cu:3-88): level-synchronous Brandes.  Forward: frontier expansion
 // with CAS on dist and FP64 atomic sigma (integer-valued, so order-free and
@@ -979,234 +975,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   return GV_OK;
 }
 
-// ===================================================== triangle counting ==
-constexpr uint32_t TC_HASH = 2048;  // LDS hash slots per wavefront (8 KiB)
-
-__device__ __forceinline__ bool tc_higher(uint32_t dw, uint32_t w, uint32_t du, uint32_t u) {
-  return dw > du || (dw == du && w > u);
-}
-
-__global__ __launch_bounds__(256) void k_tc_deg(Store st, uint64_t *dplus) {
-  const uint32_t lane = lane_id();
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < st.V;
-       u += nwaves) {
-    const uint32_t us = st.used[u], du = st.deg[u], tbl = st.tbl_off[u];
-    uint32_t cnt = 0;
-    for (uint32_t j = lane; j < us; j += WAVE) {
-      uint32_t w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
-      if (w != TOMB && tc_higher(st.deg[w], w, du, (uint32_t)u)) ++cnt;
-    }
-    cnt = wave_sum(cnt);
-    if (lane == 0) dplus[u] = cnt;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_tc_fill(Store st, const uint64_t *off, uint32_t bV,
-                                                 uint64_t *keys) {
-  const uint32_t lane = lane_id();
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; u < st.V;
-       u += nwaves) {
-    const uint32_t us = st.used[u], du = st.deg[u], tbl = st.tbl_off[u];
-    const uint64_t o = off[u], cap = off[u + 1] - off[u], tag = u << bV;
-    uint32_t wpos = 0;
-    for (uint32_t base = 0; base < us; base += WAVE) {
-      uint32_t j = base + lane;
-      bool keep = false;
-      uint32_t w = TOMB;
-      if (j < us) {
-        w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
-        keep = w != TOMB && tc_higher(st.deg[w], w, du, (uint32_t)u);
-      }
-      unsigned long long m = __ballot(keep);
-      uint32_t r = wpos + mask_lt_count(m);
-      if (keep && r < cap) keys[o + r] = tag | w;
-      wpos += (uint32_t)__popcll(m);
-    }
-  }
-}
-
-__device__ __forceinline__ bool tc_find(const uint32_t *a, uint32_t n, uint32_t x) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo < n && a[lo] == x;
-}
-
-template <bool PV>
-__device__ __forceinline__ void tc_hit(uint32_t u, uint32_t v, uint32_t w, unsigned long long &cnt,
-                                       unsigned long long &own, unsigned long long *pv) {
-  ++cnt;
-  if (PV) {
-    uint32_t mn = u < v ? u : v;
-    mn = mn < w ? mn : w;
-    if (mn == u) ++own;
-    else atomicAdd(&pv[mn], 1ull);
-  }
-}
-
-__device__ __forceinline__ uint32_t tc_hash(uint32_t w, uint32_t mask) {
-  return (w * 0x9E3779B1u) >> 7 & mask;
-}
-
-// One wavefront per root u with |N+(u)| >= 2.  N+(u) is held as the
-// intersection set: sorted in registers (<= 64 entries, six-step ds_bpermute
-// search), in an LDS open-addressing hash (<= TC_HASH/2 entries), or searched
-// in the sorted global list.  The wedges (v in N+(u), w in N+(v)) are
-// flattened 64 at a time; each lane walks its own cursor through the 64-entry
-// prefix of |N+(v)|, so the mapping needs no search.
-template <bool PV>
-__global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uint32_t *cols,
-                                                  uint32_t V, unsigned long long *pv,
-                                                  unsigned long long *total) {
-  __shared__ uint32_t s_tab[4][TC_HASH];
-  __shared__ uint32_t s_pre[4][WAVE];
-  __shared__ uint32_t s_v[4][WAVE];
-  __shared__ uint64_t s_off[4][WAVE];
-  const uint32_t lane = lane_id();
-  const uint32_t wid = threadIdx.x / WAVE;
-  uint32_t *tab = s_tab[wid];
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  unsigned long long cnt = 0;
-  for (uint64_t uu = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; uu < V;
-       uu += nwaves) {
-    const uint32_t u = (uint32_t)uu;
-    const uint64_t ou = off[u];
-    const uint32_t du = (uint32_t)(off[u + 1] - ou);
-    if (du < 2) continue;
-    const uint32_t *nu = cols + ou;
-    // mode 0: registers, 1: LDS hash, 2: global binary search
-    const int mode = du <= WAVE ? 0 : (du <= TC_HASH / 2 ? 1 : 2);
-    uint32_t sv = 0xFFFFFFFFu, hmask = 0;
-    if (mode == 0) {
-      sv = lane < du ? nu[lane] : 0xFFFFFFFFu;
-    } else if (mode == 1) {
-      uint32_t hsz = 1u << (32 - __clz(2 * du - 1));
-      hmask = hsz - 1;
-      for (uint32_t j = lane; j < hsz; j += WAVE) tab[j] = 0xFFFFFFFFu;
-      for (uint32_t j = lane; j < du; j += WAVE) {
-        uint32_t x = nu[j];
-        uint32_t h = tc_hash(x, hmask);
-        while (atomicCAS(&tab[h], 0xFFFFFFFFu, x) != 0xFFFFFFFFu) h = (h + 1) & hmask;
-      }
-    }
-    unsigned long long own = 0;
-    for (uint32_t c0 = 0; c0 < du; c0 += WAVE) {
-      uint32_t j = c0 + lane;
-      uint32_t v = 0, len = 0;
-      uint64_t ov = 0;
-      if (j < du) {
-        v = nu[j];
-        ov = off[v];
-        len = (uint32_t)(off[v + 1] - ov);
-      }
-      uint32_t pre = wave_incl_scan<uint32_t>(len);
-      const uint32_t W = __shfl(pre, WAVE - 1, WAVE);
-      s_pre[wid][lane] = pre;
-      s_v[wid][lane] = v;
-      s_off[wid][lane] = ov;
-      uint32_t k = 0;  // this lane's current v index (advances monotonically)
-      for (uint32_t fb = 0; fb < W; fb += WAVE) {
-        const uint32_t f = fb + lane;
-        const bool act = f < W;
-        uint32_t wv = 0xFFFFFFFFu, vk = 0;
-        if (act) {
-          while (s_pre[wid][k] <= f) ++k;
-          const uint32_t start = k ? s_pre[wid][k - 1] : 0u;
-          vk = s_v[wid][k];
-          wv = cols[s_off[wid][k] + (f - start)];
-        }
-        bool found = false;
-        if (mode == 0) {
-          // both shuffles run on every lane: a ds_bpermute issued under a
-          // partial exec mask reads zero from the disabled source lanes
-          uint32_t p = wave_lower_bound(sv, wv);
-          uint32_t hit = __shfl(sv, (int)p, WAVE);
-          found = act && hit == wv;
-        } else if (act) {
-          if (mode == 1) {
-            uint32_t h = tc_hash(wv, hmask);
-            while (true) {
-              uint32_t e = tab[h];
-              if (e == wv) { found = true; break; }
-              if (e == 0xFFFFFFFFu) break;
-              h = (h + 1) & hmask;
-            }
-          } else {
-            found = tc_find(nu, du, wv);
-          }
-        }
-        if (found) tc_hit<PV>(u, vk, wv, cnt, own, pv);
-      }
-    }
-    if (PV) {
-      own = wave_sum(own);
-      if (lane == 0 && own) atomicAdd(&pv[u], own);
-    }
-  }
-  cnt = wave_sum(cnt);
-  if (lane == 0 && cnt) atomicAdd(total, cnt);
-}
-
-gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,
-                             hipStream_t s) {
-  const uint32_t V = st.V;
-  *total = 0;
-  if (per_vertex_dev) fill_u64(per_vertex_dev, 0, V, s);
-  if (!V) return GV_OK;
-  const uint32_t bV = id_bits(V);
-  uint64_t *off = nullptr, *tot_dev = nullptr;
-  void *tmp = nullptr;
-  GV_HIP(hipMalloc(&off, ((size_t)V + 1) * sizeof(uint64_t)));
-  GV_HIP(hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1)));
-  GV_HIP(hipMalloc(&tot_dev, sizeof(uint64_t)));
-  k_tc_deg<<<grid_for((uint64_t)V * WAVE, 256, 8192), 256, 0, s>>>(st, off);
-  scan_u64(off, off, V, nullptr, tmp, s);
-  uint64_t E = 0;
-  GV_HIP(hipMemcpyAsync(&E, off + V, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  GV_HIP(hipStreamSynchronize(s));
-  uint64_t *k0 = nullptr, *k1 = nullptr;
-  uint32_t *cols = nullptr;
-  void *stmp = nullptr;
-  gv_status rc = GV_OK;
-  do {
-    if (hipMalloc(&k0, (E ? E : 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&k1, (E ? E : 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&cols, (E ? E : 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&stmp, sort_tmp_bytes(E)) != hipSuccess) {
-      set_error("triangle count: device allocation failed");
-      rc = GV_E_NOMEM;
-      break;
-    }
-    k_tc_fill<<<grid_for((uint64_t)V * WAVE, 256, 8192), 256, 0, s>>>(st, off, bV, k0);
-    uint64_t *sorted = sort_keys(k0, k1, E, 2 * bV, stmp, s);
-    keys_to_cols(sorted, E, bV, cols, s);
-    fill_u64(tot_dev, 0, 1, s);
-    unsigned grid = grid_for((uint64_t)V * WAVE, 256, 8192);
-    if (per_vertex_dev)
-      k_tc_count<true><<<grid, 256, 0, s>>>(off, cols, V, (unsigned long long *)per_vertex_dev,
-                                            (unsigned long long *)tot_dev);
-    else
-      k_tc_count<false><<<grid, 256, 0, s>>>(off, cols, V, nullptr,
-                                             (unsigned long long *)tot_dev);
-    if (hipMemcpyAsync(total, tot_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      set_error("triangle count: HIP error");
-      rc = GV_E_HIP;
-    }
-  } while (0);
-  hipFree(k0);
-  hipFree(k1);
-  hipFree(cols);
-  hipFree(stmp);
-  hipFree(off);
-  hipFree(tmp);
-  hipFree(tot_dev);
-  return rc;
-}
+// Triangle counting: gv_tc.hip.
 
 // ========================================================= betweenness ==
 __global__ void k_bc_init(int *dist, double *sigma, float *delta, uint32_t V) {
