@@ -64,6 +64,10 @@ def parse():
     p.add_argument("--prec-scale", type=int, default=24)
     p.add_argument("--serve-scale", type=int, default=26)
     p.add_argument("--serve-seconds", type=float, default=20.0)
+    p.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DB", "WRITE_DB", "OUT_JSON"),
+                   help="turn two rocprofv3 --pmc result databases (FETCH_SIZE, WRITE_SIZE) of "
+                        "this bench into the per-iteration HBM traffic summary of the PageRank "
+                        "iteration, calibrated on the bench's 256 MiB copy, and exit")
     return p.parse_args()
 
 
@@ -80,6 +84,9 @@ def skey(seed, stream):
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        summarize_pmc(args, *args.summarize_pmc)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU sharding is a design note, not built)", file=sys.stderr)
@@ -220,14 +227,18 @@ def run_rmat(args):
             "gteps": pr_edges / (per_iter_ms / 1e3) / 1e9,
             "gteps_whole_call": pr_edges * it / (float(np.mean(pr_ms)) / 1e3) / 1e9,
         }
+        traffic, tsrc = load_traffic(args)
         out["roofline"] = {
-            "kernel": "k_pr_iter (PageRank pull iteration)",
+            "kernel": "PageRank flat-stream iteration (k_pf_tiles + k_pf_rows + k_pf_resid), "
+                      "timed per iteration with HIP events on the graph's stream"
+                      if args.pr_layout == 1 else f"PageRank iteration, layout {args.pr_layout}",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": tsrc,
             "algorithmic_bytes_per_launch": bytes_per_iter,
         }
     if tc_ms:
@@ -321,6 +332,67 @@ def cpu_baseline(g, V, args, batches):
                   "graph by OpenMP sorted-CSR rebuild (parallel radix sort + merge)",
         "cpu_model": _cpu_model(),
     }
+
+
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pagerank_flat_rmat22_traffic.json")
+PR_ITER_KERNELS = ("k_pf_tiles", "k_pf_rows", "k_pf_resid")
+
+
+def load_traffic(args):
+    """Measured HBM bytes per PageRank iteration (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE, calibrated) for this workload, if a summary made by
+    --summarize-pmc for the same graph and layout is committed."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    w = d.get("workload", {})
+    if (w.get("scale"), w.get("edge_factor"), w.get("seed"), w.get("pr_layout")) != \
+            (args.scale, args.edge_factor, args.seed, args.pr_layout):
+        return None, None
+    return d["traffic_bytes_per_iteration"], os.path.relpath(TRAFFIC_JSON, ROOT)
+
+
+def summarize_pmc(args, fetch_db, write_db, out_json):
+    import sqlite3
+
+    def per_dispatch(db, counter):
+        c = sqlite3.connect(db)
+        rows = c.execute("select kernel_name, value from counters_collection "
+                         "where counter_name = ?", (counter,)).fetchall()
+        c.close()
+        return rows
+
+    res = {}
+    for db, counter in ((fetch_db, "FETCH_SIZE"), (write_db, "WRITE_SIZE")):
+        rows = per_dispatch(db, counter)
+        copies = [v for k, v in rows if "copyBuffer" in k]
+        if not copies:
+            raise SystemExit(f"{db}: no copy dispatch to calibrate {counter} on")
+        # the largest copy is the bench's 256 MiB device-to-device calibration copy
+        factor = (COPY_CALIB_BYTES / 1024.0) / max(copies)
+        kern = {}
+        for name in PR_ITER_KERNELS:
+            vals = [v for k, v in rows if name in k]
+            # launches issued after convergence (the host polls the done flag
+            # every few iterations) exit at once; they are not iterations
+            vals = [v for v in vals if v >= 0.01 * max(vals)] if vals else vals
+            if vals:
+                kern[name] = {"dispatches": len(vals), "kib_mean": sum(vals) / len(vals),
+                              "bytes_calibrated": sum(vals) / len(vals) * 1024.0 * factor}
+        res[counter] = {"calibration_factor": factor, "copy_kib": max(copies), "kernels": kern}
+    total = sum(k["bytes_calibrated"] for c in res.values() for k in c["kernels"].values())
+    out = {
+        "workload": {"scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed,
+                     "pr_layout": args.pr_layout},
+        "traffic_bytes_per_iteration": total,
+        "counters": res,
+        "made_by": "bench.py --summarize-pmc FETCH_DB WRITE_DB OUT_JSON",
+    }
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({"traffic_bytes_per_iteration": total}))
 
 
 def _cpu_model():

@@ -37,7 +37,7 @@ constexpr int PF_ROW_THREADS = 256;
 constexpr uint32_t PF_ROW_BLOCKS = 1024;
 // rows spanning at least PF_LONG tiles are assembled by a wavefront each
 constexpr uint32_t PF_LONG = 8;
-constexpr uint32_t PF_LONG_BLOCKS = 64;
+constexpr uint32_t PF_LONG_BLOCKS = 512;
 
 // ------------------------------------------------------- layout build --
 __global__ void k_pf_pad(uint32_t *cols, uint64_t E, uint64_t Epad) {
@@ -271,16 +271,24 @@ __global__ __launch_bounds__(PF_THREADS) void k_pf_tiles(PfArgs a) {
   }
 }
 
+__device__ void pf_long_rows(const PfArgs &a, uint32_t blk);
+
 // Row sums assembled in tile order, the update, residual partials per block.
+// Blocks from PF_ROW_BLOCKS on take the long rows (pf_long_rows), so both
+// kinds run in one launch.
 __global__ __launch_bounds__(PF_ROW_THREADS) void k_pf_rows(PfArgs a) {
   if (a.ctl->done) return;
+  if (blockIdx.x >= PF_ROW_BLOCKS) {
+    pf_long_rows(a, blockIdx.x - PF_ROW_BLOCKS);
+    return;
+  }
   double diff = 0.0;
   const uint32_t comp = a.compensated;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t stride = (uint64_t)PF_ROW_BLOCKS * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n_nz; i += stride) {
     const uint32_t e0 = a.nzoff[i], e1 = a.nzoff[i + 1];
     const uint32_t ta = e0 / PF_T, tb = (e1 - 1) / PF_T;
-    if (tb - ta >= PF_LONG) continue;  // k_pf_long
+    if (tb - ta >= PF_LONG) continue;  // pf_long_rows
     float2 p;
     if (ta == tb) {
       const bool last = e1 == (ta + 1) * PF_T || i + 1 == a.n_nz;
@@ -314,13 +322,12 @@ __global__ __launch_bounds__(PF_ROW_THREADS) void k_pf_rows(PfArgs a) {
 // Long rows, one wavefront each: lane l joins the tile partials P[ta+1+l],
 // P[ta+1+l+64], ... in order, the lanes are joined by a fixed butterfly, and
 // the first tile's partial Q[ta] goes in front.
-__global__ __launch_bounds__(256) void k_pf_long(PfArgs a) {
-  if (a.ctl->done) return;
+__device__ void pf_long_rows(const PfArgs &a, uint32_t blk) {
   const uint32_t lane = lane_id();
   const uint32_t comp = a.compensated;
-  const uint32_t nw = gridDim.x * (256 / WAVE);
+  const uint32_t nw = PF_LONG_BLOCKS * (PF_ROW_THREADS / WAVE);
   double diff = 0.0;
-  for (uint32_t j = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE; j < a.n_long; j += nw) {
+  for (uint32_t j = blk * (PF_ROW_THREADS / WAVE) + threadIdx.x / WAVE; j < a.n_long; j += nw) {
     const uint32_t i = a.longrows[j];
     const uint32_t ta = a.nzoff[i] / PF_T, tb = (a.nzoff[i + 1] - 1) / PF_T;
     float s = 0.f, c = 0.f;
@@ -351,8 +358,8 @@ __global__ __launch_bounds__(256) void k_pf_long(PfArgs a) {
       diff += fabs((double)xv - (double)a.x[v]);
     }
   }
-  const double bs = block_sum<double, 256>(diff);
-  if (threadIdx.x == 0) a.partials[PF_ROW_BLOCKS + blockIdx.x] = bs;
+  const double bs = block_sum<double, PF_ROW_THREADS>(diff);
+  if (threadIdx.x == 0) a.partials[PF_ROW_BLOCKS + blk] = bs;
 }
 
 // Fixed-order FP64 residual, then the stopping rule (pagerank.cu:56): done
@@ -513,8 +520,7 @@ gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOp
     a.xn = w.x[(t + 1) & 1];
     a.cn = w.contrib[(t + 1) & 1];
     if (n_tiles) k_pf_tiles<<<tile_grid, PF_THREADS, 0, s>>>(a);
-    k_pf_rows<<<PF_ROW_BLOCKS, PF_ROW_THREADS, 0, s>>>(a);
-    k_pf_long<<<PF_LONG_BLOCKS, 256, 0, s>>>(a);
+    k_pf_rows<<<PF_ROW_BLOCKS + PF_LONG_BLOCKS, PF_ROW_THREADS, 0, s>>>(a);
     k_pf_resid<<<1, 1024, 0, s>>>(a, PF_ROW_BLOCKS + PF_LONG_BLOCKS);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
     ++launched;
