@@ -66,6 +66,10 @@ struct PrWork {
   void *rl_tmp = nullptr;
   uint32_t rl_V = 0;
   uint64_t snap_E = 0;  // live entries of the last snapshot
+  // entries the snapshot buffers are sized for at least (a caller that must
+  // not reallocate while other streams run presizes with headroom)
+  uint64_t snap_min_cap = 0;
+  uint32_t pf_grid = 0;  // persistent workgroups of the flat iteration (0 = one per CU)
   // flat iteration (per snapshot): per tile, the nonempty row in progress
   // before the tile and the partial sums of its first and last rows; per
   // nonempty row, its label, stream offset and in-tile sum; empty rows' labels
@@ -75,6 +79,10 @@ struct PrWork {
   float2 *pf_P = nullptr, *pf_Q = nullptr, *pf_S = nullptr;
   uint32_t *pf_nzlabel = nullptr, *pf_nzoff = nullptr, *pf_empty = nullptr, *pf_nzidx = nullptr;
   uint32_t *pf_lidx = nullptr, *pf_long = nullptr;  // rows spanning many tiles
+  // sizes of the prepared flat layout (pagerank_flat_prepare)
+  int pf_ready = 0;
+  uint32_t pf_prepV = 0, pf_n_nz = 0, pf_n_long = 0;
+  uint64_t pf_E = 0, pf_n_tiles = 0;
   double *pf_partials = nullptr;
 };
 
@@ -132,6 +140,13 @@ void pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out,
 gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOpts &o,
                             float *out_dev, int *iters, double *residual, PrWork &w,
                             hipStream_t s, double *iter_ms);
+// The same in two steps: the snapshot and its layout, ordered on stream s
+// after everything before it (the store is not read again afterwards), then
+// the iterations over w alone, on any stream ordered after the first step.
+gv_status pagerank_flat_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
+                                hipStream_t s);
+gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
+                                double *residual, hipStream_t s, double *iter_ms);
 void pr_release(PrWork &w);
 // Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
@@ -149,9 +164,25 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
 gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,
                              hipStream_t s);
 
-// Brandes BC from the given device source list; out_dev[V] receives BC.
+// Brandes BC from the given host source list; out_dev[V] receives BC.
 gv_status bc_run(const Store &st, const uint32_t *sources_host, uint32_t n_sources,
                  float *out_dev, hipStream_t s);
+// Workspace of single-source BC, reusable across calls.
+struct BcWork {
+  uint32_t V = 0;
+  int *dist = nullptr;
+  double *sigma = nullptr;
+  float *delta = nullptr, *bcc = nullptr;
+  uint32_t *queue = nullptr;
+  unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  uint32_t max_blocks = 0;  // grid cap of the level kernels (0 = 8192)
+};
+gv_status bc_reserve(BcWork &w, uint32_t V);
+void bc_release(BcWork &w);
+// The same over a CSR snapshot of the out-adjacency (offsets[V+1], cols).
+gv_status bc_run_csr(const uint64_t *off, const uint32_t *cols, uint32_t V,
+                     const uint32_t *sources_host, uint32_t n_sources, float *out_dev,
+                     hipStream_t s, BcWork &w);
 
 // Personalized PageRank for ncols <= 64 seeds (device array); out_dev is
 // row-major V x ncols.  max_residual is the largest per-column L1 residual of
@@ -162,5 +193,12 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
 // Brandes BC from host source list, 32 sources per batch.
 gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
                        float *out_dev, hipStream_t s);
+
+// Request-loop hooks over a graph handle (gv_api.hip): the relabelled flat
+// PageRank snapshot of the handle's in-adjacency, and a CSR snapshot of its
+// out-adjacency (w.snap_off / w.snap_cols) for BC, each built on the
+// handle's stream.
+gv_status graph_pr_prepare(gv_graph *g, PrWork &w);
+gv_status graph_bc_prepare(gv_graph *g, PrWork &w);
 
 }  // namespace gv

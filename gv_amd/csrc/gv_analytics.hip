@@ -94,7 +94,12 @@ void pr_release(PrWork &w) {
   hipFree(w.pf_lidx);
   hipFree(w.pf_long);
   hipFree(w.pf_partials);
+  // sizing settings of the caller outlive the buffers
+  const uint64_t min_cap = w.snap_min_cap;
+  const uint32_t grid = w.pf_grid;
   w = PrWork();
+  w.snap_min_cap = min_cap;
+  w.pf_grid = grid;
 }
 
 __device__ __forceinline__ int pr_bin(uint32_t used, PrBins b) {
@@ -419,7 +424,8 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
     hipFree(w.snap_cols);
     w.snap_cols = nullptr;
     w.snap_cap = 0;
-    const uint64_t cap = need + need / 8 + PF_T;
+    uint64_t cap = need + need / 8 + PF_T;
+    if (cap < w.snap_min_cap + PF_T) cap = w.snap_min_cap + PF_T;
     GV_HIP(hipMalloc(&w.snap_cols, cap * sizeof(uint32_t)));
     w.snap_cap = cap;
   }
@@ -995,7 +1001,10 @@ __global__ void k_bc_src(int *dist, double *sigma, uint32_t *queue, uint32_t src
   *cursor = 0;
 }
 
-__global__ __launch_bounds__(256) void k_bc_fwd(Store st, int *dist, double *sigma,
+// One wavefront per frontier vertex; Adj is the block store (tombstones
+// skipped) or a CSR snapshot.
+template <class Adj>
+__global__ __launch_bounds__(256) void k_bc_fwd(Adj adj, int *dist, double *sigma,
                                                 uint32_t *queue, uint32_t qs, uint32_t qe, int d,
                                                 unsigned int *cursor) {
   const uint32_t lane = lane_id();
@@ -1004,13 +1013,14 @@ __global__ __launch_bounds__(256) void k_bc_fwd(Store st, int *dist, double *sig
   for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < nf; i += nwaves) {
     const uint32_t u = queue[qs + i];
     const double su = sigma[u];
-    const uint32_t us = st.used[u], tbl = st.tbl_off[u];
+    const uint32_t us = adj.len(u);
+    const uint64_t b = adj.base(u);
     for (uint32_t base = 0; base < us; base += WAVE) {
       uint32_t j = base + lane;
       bool fresh = false;
       uint32_t w = TOMB;
       if (j < us) {
-        w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
+        w = adj.at(b, j);
         if (w != TOMB) {
           // dist moves from -1 to a level once, so a plain read of any other
           // value is final; only a -1 needs the CAS to decide.
@@ -1032,7 +1042,8 @@ __global__ __launch_bounds__(256) void k_bc_fwd(Store st, int *dist, double *sig
   }
 }
 
-__global__ __launch_bounds__(256) void k_bc_bwd(Store st, const int *dist, const double *sigma,
+template <class Adj>
+__global__ __launch_bounds__(256) void k_bc_bwd(Adj adj, const int *dist, const double *sigma,
                                                 float *delta, const uint32_t *queue, uint32_t qs,
                                                 uint32_t qe, int d, uint32_t src, float *bc,
                                                 float *bcc) {
@@ -1042,10 +1053,11 @@ __global__ __launch_bounds__(256) void k_bc_bwd(Store st, const int *dist, const
   for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < nf; i += nwaves) {
     const uint32_t v = queue[qs + i];
     const double sv = sigma[v];
-    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
+    const uint32_t us = adj.len(v);
+    const uint64_t b = adj.base(v);
     float s = 0.f, c = 0.f;
     for (uint32_t j = lane; j < us; j += WAVE) {
-      uint32_t w = st.slots[slot_index(st.arena, tbl, j, st.log2B, st.B)];
+      uint32_t w = adj.at(b, j);
       if (w != TOMB && dist[w] == d + 1) {
         float coef = (float)(sv / sigma[w]);
         kahan_add(s, c, coef * (1.0f + delta[w]));
@@ -1069,73 +1081,94 @@ __global__ __launch_bounds__(256) void k_bc_bwd(Store st, const int *dist, const
   }
 }
 
+gv_status bc_reserve(BcWork &w, uint32_t V) {
+  if (w.V >= V && w.dist) return GV_OK;
+  bc_release(w);
+  const size_t n = (size_t)(V ? V : 1);
+  GV_HIP(hipMalloc(&w.dist, n * sizeof(int)));
+  GV_HIP(hipMalloc(&w.sigma, n * sizeof(double)));
+  GV_HIP(hipMalloc(&w.delta, n * sizeof(float)));
+  GV_HIP(hipMalloc(&w.bcc, n * sizeof(float)));
+  GV_HIP(hipMalloc(&w.queue, n * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&w.cursor, sizeof(unsigned int)));
+  GV_HIP(hipHostMalloc(&w.h_cursor, sizeof(unsigned int), hipHostMallocDefault));
+  w.V = V;
+  return GV_OK;
+}
+
+void bc_release(BcWork &w) {
+  hipFree(w.dist);
+  hipFree(w.sigma);
+  hipFree(w.delta);
+  hipFree(w.bcc);
+  hipFree(w.queue);
+  hipFree(w.cursor);
+  if (w.h_cursor) hipHostFree(w.h_cursor);
+  const uint32_t mb = w.max_blocks;
+  w = BcWork();
+  w.max_blocks = mb;
+}
+
+// Level-synchronous Brandes from each source in turn; BC accumulates into
+// out_dev (FP32, Kahan-compensated per vertex).
+template <class Adj>
+static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
+                           uint32_t n_sources, float *out_dev, hipStream_t s, BcWork &w) {
+  if (!V) return GV_OK;
+  gv_status rc = bc_reserve(w, V);
+  if (rc) return rc;
+  const unsigned cap = w.max_blocks ? w.max_blocks : 8192u;
+  fill_f32(out_dev, 0.f, V, s);
+  fill_f32(w.bcc, 0.f, V, s);
+  for (uint32_t si = 0; si < n_sources && rc == GV_OK; ++si) {
+    const uint32_t src = sources[si];
+    k_bc_init<<<grid_for(V), 256, 0, s>>>(w.dist, w.sigma, w.delta, V);
+    k_bc_src<<<1, 1, 0, s>>>(w.dist, w.sigma, w.queue, src, w.cursor);
+    std::vector<uint32_t> lvl{0u, 1u};
+    int d = 0;
+    while (true) {
+      uint32_t qs = lvl[d], qe = lvl[d + 1];
+      if (qe == qs) break;
+      fill_u32(w.cursor, 0, 1, s);
+      k_bc_fwd<Adj><<<grid_for((uint64_t)(qe - qs) * WAVE, 256, cap), 256, 0, s>>>(
+          adj, w.dist, w.sigma, w.queue, qs, qe, d, w.cursor);
+      if (hipMemcpyAsync(w.h_cursor, w.cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("bc: HIP error");
+        rc = GV_E_HIP;
+        break;
+      }
+      lvl.push_back(qe + *w.h_cursor);
+      ++d;
+    }
+    // levels 0..D-1 are non-empty; level D-1 has no successors
+    const int D = (int)lvl.size() - 2;
+    for (int lev = D - 2; lev >= 0 && rc == GV_OK; --lev) {
+      uint32_t qs = lvl[lev], qe = lvl[lev + 1];
+      k_bc_bwd<Adj><<<grid_for((uint64_t)(qe - qs) * WAVE, 256, cap), 256, 0, s>>>(
+          adj, w.dist, w.sigma, w.delta, w.queue, qs, qe, lev, src, out_dev, w.bcc);
+    }
+  }
+  if (rc == GV_OK && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("bc: HIP error");
+    rc = GV_E_HIP;
+  }
+  return rc;
+}
+
 gv_status bc_run(const Store &st, const uint32_t *sources, uint32_t n_sources, float *out_dev,
                  hipStream_t s) {
-  const uint32_t V = st.V;
-  if (!V) return GV_OK;
-  int *dist = nullptr;
-  double *sigma = nullptr;
-  float *delta = nullptr, *bcc = nullptr;
-  uint32_t *queue = nullptr;
-  unsigned int *cursor = nullptr, *h_cursor = nullptr;
-  gv_status rc = GV_OK;
-  do {
-    if (hipMalloc(&dist, (size_t)V * sizeof(int)) != hipSuccess ||
-        hipMalloc(&sigma, (size_t)V * sizeof(double)) != hipSuccess ||
-        hipMalloc(&delta, (size_t)V * sizeof(float)) != hipSuccess ||
-        hipMalloc(&bcc, (size_t)V * sizeof(float)) != hipSuccess ||
-        hipMalloc(&queue, (size_t)V * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&cursor, sizeof(unsigned int)) != hipSuccess ||
-        hipHostMalloc(&h_cursor, sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
-      set_error("bc: device allocation failed");
-      rc = GV_E_NOMEM;
-      break;
-    }
-    fill_f32(out_dev, 0.f, V, s);
-    fill_f32(bcc, 0.f, V, s);
-    for (uint32_t si = 0; si < n_sources && rc == GV_OK; ++si) {
-      const uint32_t src = sources[si];
-      k_bc_init<<<grid_for(V), 256, 0, s>>>(dist, sigma, delta, V);
-      k_bc_src<<<1, 1, 0, s>>>(dist, sigma, queue, src, cursor);
-      std::vector<uint32_t> lvl{0u, 1u};
-      int d = 0;
-      while (true) {
-        uint32_t qs = lvl[d], qe = lvl[d + 1];
-        if (qe == qs) break;
-        fill_u32(cursor, 0, 1, s);
-        k_bc_fwd<<<grid_for((uint64_t)(qe - qs) * WAVE, 256, 8192), 256, 0, s>>>(
-            st, dist, sigma, queue, qs, qe, d, cursor);
-        if (hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s) !=
-                hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-          set_error("bc: HIP error");
-          rc = GV_E_HIP;
-          break;
-        }
-        lvl.push_back(qe + *h_cursor);
-        ++d;
-      }
-      // levels 0..D-1 are non-empty; level D-1 has no successors
-      const int D = (int)lvl.size() - 2;
-      for (int lev = D - 2; lev >= 0 && rc == GV_OK; --lev) {
-        uint32_t qs = lvl[lev], qe = lvl[lev + 1];
-        k_bc_bwd<<<grid_for((uint64_t)(qe - qs) * WAVE, 256, 8192), 256, 0, s>>>(
-            st, dist, sigma, delta, queue, qs, qe, lev, src, out_dev, bcc);
-      }
-    }
-    if (rc == GV_OK && hipStreamSynchronize(s) != hipSuccess) {
-      set_error("bc: HIP error");
-      rc = GV_E_HIP;
-    }
-  } while (0);
-  hipFree(dist);
-  hipFree(sigma);
-  hipFree(delta);
-  hipFree(bcc);
-  hipFree(queue);
-  hipFree(cursor);
-  if (h_cursor) hipHostFree(h_cursor);
+  BcWork w;
+  gv_status rc = bc_levels(StoreAdj{st}, st.V, sources, n_sources, out_dev, s, w);
+  bc_release(w);
   return rc;
+}
+
+gv_status bc_run_csr(const uint64_t *off, const uint32_t *cols, uint32_t V,
+                     const uint32_t *sources, uint32_t n_sources, float *out_dev, hipStream_t s,
+                     BcWork &w) {
+  return bc_levels(CsrAdj{off, cols}, V, sources, n_sources, out_dev, s, w);
 }
 
 }  // namespace gv

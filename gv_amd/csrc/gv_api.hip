@@ -48,6 +48,12 @@ struct gv_graph {
   std::vector<void *> cap_pinned;   // pinned host copies read by recorded copies
   uint64_t buf_gen = 0;             // bumped whenever a buffer a capture may use moves
   std::vector<gv_capture *> captures;  // live captures (detached by gv_destroy)
+  // Neighbour-query buffers, grown on demand and kept (a free per call would
+  // wait for all device work, including other streams').
+  uint64_t *nb_cnt = nullptr, *nb_k0 = nullptr, *nb_k1 = nullptr;
+  uint32_t *nb_dvs = nullptr, *nb_dout = nullptr;
+  void *nb_tmp = nullptr;
+  size_t nb_cnt_cap = 0, nb_k_cap = 0, nb_dvs_cap = 0, nb_dout_cap = 0, nb_tmp_cap = 0;
   // Chung-Lu cumulative table on the device, cached by its parameters.
   uint64_t *cl_cum = nullptr;
   uint32_t cl_V = 0;
@@ -372,6 +378,30 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
   return take_abort(g);
 }
 
+namespace gv {
+
+// Snapshots for analytics that run off the graph's stream (the request loop):
+// both are ordered on the graph's stream after every call issued before them
+// and read the store only there.
+gv_status graph_pr_prepare(gv_graph *g, PrWork &w) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (g->directed && !g->has_in) {
+    set_error("PageRank on a directed graph needs maintain_in_edges");
+    return GV_E_UNSUPPORTED;
+  }
+  const Store &in = g->has_in ? g->st[1] : g->st[0];
+  return pagerank_flat_prepare(in, g->st[0].deg, w, g->stream);
+}
+
+gv_status graph_bc_prepare(gv_graph *g, PrWork &w) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  return pr_snapshot(g->st[0], g->st[0].deg, false, w, g->stream, 0);
+}
+
+}  // namespace gv
+
 extern "C" {
 
 // A failure while the stream is being captured poisons the capture.
@@ -472,6 +502,12 @@ gv_status gv_destroy(gv_graph *g) {
   for (void *p : g->cap_pinned) hipHostFree(p);
   for (gv_capture *c : g->captures) c->g = nullptr;
   hipFree(g->cl_cum);
+  hipFree(g->nb_cnt);
+  hipFree(g->nb_k0);
+  hipFree(g->nb_k1);
+  hipFree(g->nb_dvs);
+  hipFree(g->nb_dout);
+  hipFree(g->nb_tmp);
   if (g->stream) hipStreamSynchronize(g->stream);
   for (int k = 0; k < 2; ++k) {
     store_release(g->st[k]);
@@ -646,6 +682,32 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
   return rc;
 }
 
+// Grow-only device buffer: reallocated (with headroom) only when too small.
+static hipError_t nb_grow(void **p, size_t &cap, size_t need) {
+  if (*p && need <= cap) return hipSuccess;
+  hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  const size_t c = need + need / 2 + 256;
+  hipError_t e = hipMalloc(p, c);
+  if (e == hipSuccess) cap = c;
+  return e;
+}
+
+// The two key buffers of the neighbour sort share one capacity.
+static hipError_t nb_grow_keys(gv_graph *g, size_t need) {
+  if (g->nb_k0 && g->nb_k1 && need <= g->nb_k_cap) return hipSuccess;
+  hipFree(g->nb_k0);
+  hipFree(g->nb_k1);
+  g->nb_k0 = g->nb_k1 = nullptr;
+  g->nb_k_cap = 0;
+  const size_t c = need + need / 2 + 256;
+  hipError_t e = hipMalloc(&g->nb_k0, c);
+  if (e == hipSuccess) e = hipMalloc(&g->nb_k1, c);
+  if (e == hipSuccess) g->nb_k_cap = c;
+  return e;
+}
+
 gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs, uint32_t *out,
                        int mem) {
   gv_status rc = check_graph(g);
@@ -664,62 +726,43 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
         return GV_E_RANGE;
       }
   const hipMemcpyKind out_kind = mem == GV_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  uint32_t *dvs = nullptr, *dout = nullptr;
-  uint64_t *cnt = nullptr, *k0 = nullptr, *k1 = nullptr;
-  void *tmp = nullptr;
   uint64_t T = 0;
-  do {
-    hipError_t e;
-    if ((e = hipMalloc(&cnt, ((size_t)n + 1) * 8)) != hipSuccess ||
-        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)n + 1))) != hipSuccess) {
-      rc = hip_fail(e, "neighbors alloc");
-      break;
-    }
-    const uint32_t *qv = vs;
-    if (mem != GV_MEM_DEVICE && n) {
-      if ((e = hipMalloc(&dvs, (size_t)n * 4)) != hipSuccess ||
-          (e = hipMemcpyAsync(dvs, vs, (size_t)n * 4, hipMemcpyHostToDevice, s)) != hipSuccess) {
-        rc = hip_fail(e, "neighbors");
-        break;
-      }
-      qv = dvs;
-    }
-    neighbor_counts(g->st[0], qv, n, cnt, s);
-    scan_u64(cnt, cnt, n, nullptr, tmp, s);
-    if ((e = hipMemcpyAsync(&T, cnt + n, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-        (e = hipMemcpyAsync(offs, cnt, ((size_t)n + 1) * 8, out_kind, s)) != hipSuccess ||
-        (e = hipStreamSynchronize(s)) != hipSuccess) {
-      rc = hip_fail(e, "neighbors");
-      break;
-    }
-    if (out && T) {
-      hipFree(tmp);
-      tmp = nullptr;
-      if ((e = hipMalloc(&k0, T * 8)) != hipSuccess || (e = hipMalloc(&k1, T * 8)) != hipSuccess ||
-          (e = hipMalloc(&tmp, sort_tmp_bytes(T))) != hipSuccess ||
-          (mem != GV_MEM_DEVICE && (e = hipMalloc(&dout, T * 4)) != hipSuccess)) {
-        rc = hip_fail(e, "neighbors alloc");
-        break;
-      }
-      neighbor_keys(g->st[0], qv, n, cnt, qb, k0, s);
-      uint64_t *sorted = sort_keys(k0, k1, T, qb + bV, tmp, s);
-      uint32_t *cdst = mem == GV_MEM_DEVICE ? out : dout;
-      keys_to_cols(sorted, T, bV, cdst, s);
-      if (mem != GV_MEM_DEVICE &&
-          (e = hipMemcpyAsync(out, dout, T * 4, hipMemcpyDeviceToHost, s)) != hipSuccess) {
-        rc = hip_fail(e, "neighbors");
-        break;
-      }
-    }
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) rc = hip_fail(e, "neighbors");
-  } while (0);
-  hipFree(dvs);
-  hipFree(dout);
-  hipFree(cnt);
-  hipFree(k0);
-  hipFree(k1);
-  hipFree(tmp);
-  return rc;
+  hipError_t e;
+  const uint64_t *cnt = nullptr;
+  if ((e = nb_grow((void **)&g->nb_cnt, g->nb_cnt_cap, ((size_t)n + 1) * 8)) != hipSuccess ||
+      (e = nb_grow(&g->nb_tmp, g->nb_tmp_cap, scan_tmp_bytes((uint64_t)n + 1))) != hipSuccess)
+    return hip_fail(e, "neighbors alloc");
+  cnt = g->nb_cnt;
+  const uint32_t *qv = vs;
+  if (mem != GV_MEM_DEVICE && n) {
+    if ((e = nb_grow((void **)&g->nb_dvs, g->nb_dvs_cap, (size_t)n * 4)) != hipSuccess ||
+        (e = hipMemcpyAsync(g->nb_dvs, vs, (size_t)n * 4, hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+      return hip_fail(e, "neighbors");
+    qv = g->nb_dvs;
+  }
+  neighbor_counts(g->st[0], qv, n, g->nb_cnt, s);
+  scan_u64(g->nb_cnt, g->nb_cnt, n, nullptr, g->nb_tmp, s);
+  if ((e = hipMemcpyAsync(&T, cnt + n, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(offs, cnt, ((size_t)n + 1) * 8, out_kind, s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
+    return hip_fail(e, "neighbors");
+  if (out && T) {
+    if ((e = nb_grow_keys(g, T * 8)) != hipSuccess ||
+        (e = nb_grow(&g->nb_tmp, g->nb_tmp_cap, sort_tmp_bytes(T))) != hipSuccess ||
+        (mem != GV_MEM_DEVICE &&
+         (e = nb_grow((void **)&g->nb_dout, g->nb_dout_cap, T * 4)) != hipSuccess))
+      return hip_fail(e, "neighbors alloc");
+    neighbor_keys(g->st[0], qv, n, g->nb_cnt, qb, g->nb_k0, s);
+    uint64_t *sorted = sort_keys(g->nb_k0, g->nb_k1, T, qb + bV, g->nb_tmp, s);
+    uint32_t *cdst = mem == GV_MEM_DEVICE ? out : g->nb_dout;
+    keys_to_cols(sorted, T, bV, cdst, s);
+    if (mem != GV_MEM_DEVICE &&
+        (e = hipMemcpyAsync(out, g->nb_dout, T * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return hip_fail(e, "neighbors");
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "neighbors");
+  return GV_OK;
 }
 
 static gv_status pagerank_common(gv_graph *g, const gv_pagerank_opts *po, float *out, int mem,

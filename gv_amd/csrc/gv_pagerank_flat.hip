@@ -424,7 +424,8 @@ static gv_status pf_reserve(PrWork &w, uint32_t V, uint64_t n_tiles) {
     w.pf_P = w.pf_Q = nullptr;
     w.pf_tbase = nullptr;
     w.pf_tiles_cap = 0;
-    const uint64_t cap = n_tiles + n_tiles / 8 + 16;
+    uint64_t cap = n_tiles + n_tiles / 8 + 16;
+    if (cap < cdiv(w.snap_min_cap, PF_T) + 16) cap = cdiv(w.snap_min_cap, PF_T) + 16;
     GV_HIP(hipMalloc(&w.pf_P, cap * sizeof(float2)));
     GV_HIP(hipMalloc(&w.pf_Q, cap * sizeof(float2)));
     GV_HIP(hipMalloc(&w.pf_tbase, cap * sizeof(int32_t)));
@@ -433,16 +434,12 @@ static gv_status pf_reserve(PrWork &w, uint32_t V, uint64_t n_tiles) {
   return GV_OK;
 }
 
-gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOpts &o,
-                            float *out_dev, int *iters, double *residual, PrWork &w,
-                            hipStream_t s, double *iter_ms) {
+gv_status pagerank_flat_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
+                                hipStream_t s) {
   const uint32_t V = in.V;
-  if (iters) *iters = 0;
-  if (residual) *residual = 0.0;
-  if (!V) return GV_OK;
+  w.pf_ready = 0;
   // stream positions and buffer offsets are 32-bit; labels keep bit 31 free
-  if (V >= PF_PAD || (uint64_t)V * 4u > PF_OOB) return GV_E_UNSUPPORTED;
-  if (o.max_iter <= 0) return GV_E_UNSUPPORTED;  // handled by the row form
+  if (!V || V >= PF_PAD || (uint64_t)V * 4u > PF_OOB) return GV_E_UNSUPPORTED;
   gv_status st = pr_reserve(w, V);
   if (st) return st;
   st = pr_snapshot(in, deg_out, true, w, s, 1);
@@ -452,7 +449,6 @@ gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOp
   if (n_tiles * PF_T >= 0xFFFFFFFFull) return GV_E_UNSUPPORTED;
   st = pf_reserve(w, V, n_tiles);
   if (st) return st;
-  const uint32_t *dout = w.rl_dout;  // out-degrees in label order
   if (n_tiles * PF_T > E)
     k_pf_pad<<<1, 1024, 0, s>>>(w.snap_cols, E, n_tiles * PF_T);
   k_pf_nzflag<<<grid_for(V), 256, 0, s>>>(w.snap_off, V, w.pf_nzidx);
@@ -471,11 +467,31 @@ gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOp
     k_pf_longfill<<<grid_for(n_nz), 256, 0, s>>>(w.pf_nzidx, V, w.pf_lidx, w.pf_long);
     GV_HIP(hipMemcpyAsync(&n_long, w.pf_lidx + n_nz, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           s));
+    GV_HIP(hipStreamSynchronize(s));
   }
+  w.pf_E = E;
+  w.pf_n_tiles = n_tiles;
+  w.pf_n_nz = n_nz;
+  w.pf_n_long = n_long;
+  w.pf_prepV = V;
+  w.pf_ready = 1;
+  return GV_OK;
+}
+
+gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
+                                double *residual, hipStream_t s, double *iter_ms) {
+  if (iters) *iters = 0;
+  if (residual) *residual = 0.0;
+  if (!w.pf_ready || o.max_iter <= 0) {
+    set_error("pagerank: no prepared snapshot, or max_iter <= 0");
+    return GV_E_INVALID;
+  }
+  const uint32_t V = w.pf_prepV;
+  const uint64_t n_tiles = w.pf_n_tiles;
+  const uint32_t *dout = w.rl_dout;  // out-degrees in label order
   const float x0 = 1.0f / (float)V;
   pr_ctl_reset(w.ctl, s);
   pr_init_vectors(w.x[0], w.contrib[0], dout, V, x0, o.init, w.rl_order, s);
-  GV_HIP(hipStreamSynchronize(s));
 
   PfArgs a;
   a.cols = w.snap_cols;
@@ -490,19 +506,19 @@ gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOp
   a.longrows = w.pf_long;
   a.partials = w.pf_partials;
   a.ctl = w.ctl;
-  a.E = E;
+  a.E = w.pf_E;
   a.V = V;
   a.nh = V < PF_HOT ? V : PF_HOT;
   a.n_tiles = (uint32_t)n_tiles;
-  a.n_nz = n_nz;
-  a.n_empty = V - n_nz;
-  a.n_long = n_long;
+  a.n_nz = w.pf_n_nz;
+  a.n_empty = V - w.pf_n_nz;
+  a.n_long = w.pf_n_long;
   a.compensated = o.compensated ? 1u : 0u;
   a.max_iter = (uint32_t)o.max_iter;
   a.alpha = o.alpha;
   a.base = (1.0f - o.alpha) / (float)V;
   a.eps = o.eps;
-  const uint32_t tile_grid = (uint32_t)pf_device_cus();
+  const uint32_t tile_grid = w.pf_grid ? w.pf_grid : (uint32_t)pf_device_cus();
   if (iter_ms) {
     while (w.ev.size() < 2 * (size_t)o.max_iter) {
       hipEvent_t e;
@@ -546,6 +562,18 @@ gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOp
     *iter_ms = tot;
   }
   return GV_OK;
+}
+
+gv_status pagerank_flat_run(const Store &in, const uint32_t *deg_out, const PrOpts &o,
+                            float *out_dev, int *iters, double *residual, PrWork &w,
+                            hipStream_t s, double *iter_ms) {
+  if (iters) *iters = 0;
+  if (residual) *residual = 0.0;
+  if (!in.V) return GV_OK;
+  if (o.max_iter <= 0) return GV_E_UNSUPPORTED;  // handled by the row form
+  gv_status st = pagerank_flat_prepare(in, deg_out, w, s);
+  if (st) return st;
+  return pagerank_flat_iterate(o, w, out_dev, iters, residual, s, iter_ms);
 }
 
 }  // namespace gv

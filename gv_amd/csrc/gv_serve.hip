@@ -1,10 +1,12 @@
 // gv_serve.hip -- request loop over one graph handle (include/gv_serve.h).
 //
-// Host orchestration only: a worker thread drains an arrival-ordered queue in
-// steps (a run of update requests, a run of neighbour lookups, a run of
+// Host orchestration only.  The update worker drains an arrival-ordered queue
+// in steps (a run of update requests, a run of neighbour lookups, a run of
 // PageRank refreshes, or one BC request), issuing the store's own entry
 // points on the graph's stream and stamping completion after the device work
-// of the step has finished.
+// of the step has finished.  A PageRank or BC step only takes its snapshot
+// there; the analytics worker runs it on a second stream, so it overlaps the
+// updates and lookups that arrive after it.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -14,8 +16,10 @@
 #include <thread>
 #include <vector>
 
+#include "gv_analytics.h"
 #include "gv_internal.h"
 #include "gv_serve.h"
+#include "gv_store.h"
 
 using namespace gv;
 
@@ -30,6 +34,26 @@ struct Item {
 };
 
 enum class Cls { Update, Neighbors, Pagerank, Bc };
+
+// Analytics snapshot slots: a PageRank or BC request holds one from the
+// moment its snapshot is taken on the graph's stream until its result is out.
+constexpr int AN_SLOTS = 2;
+
+struct AnSlot {
+  PrWork w;
+  hipEvent_t ready = nullptr;
+  bool busy = false;
+};
+
+// A PageRank run (consecutive refreshes share it) or one BC, handed from the
+// update worker to the analytics worker with its snapshot.
+struct AnJob {
+  Cls cls;
+  std::vector<Item> items;
+  int slot;
+  double t_start;
+  uint64_t step;
+};
 
 Cls cls_of(int kind) {
   switch (kind) {
@@ -59,8 +83,20 @@ struct gv_server {
   bool stop = false;
   std::thread worker;
 
-  // worker-only state
+  // analytics worker: own stream and thread, jobs in capture order
+  hipStream_t astream = nullptr;
+  uint32_t an_cus = 0;      // compute units the analytics kernels may occupy
+  std::thread aworker;
+  std::deque<AnJob> ajobs;  // guarded by mu
+  std::condition_variable cv_ajob, cv_slot;
+  bool astop = false;       // guarded by mu
+  AnSlot slots[AN_SLOTS];   // busy flags guarded by mu
+  BcWork bcw;               // analytics worker only
+
+  // update-worker state
   uint32_t *stage_src = nullptr, *stage_dst = nullptr;
+  // PageRank vectors (analytics worker; the update worker only in the
+  // fallback path, when no analytics job is outstanding)
   float *pr[2] = {nullptr, nullptr};
   int pr_cur = -1;  // index of the last PageRank vector, -1 = none yet
   float *bc_out = nullptr;
@@ -214,21 +250,175 @@ static void serve_pagerank(gv_server *s, std::vector<Item> &step, std::vector<gv
   }
 }
 
-static void serve_bc(gv_server *s, Item &it, gv_response &out) {
-  const gv_request &r = it.req;
-  if (r.source >= s->V) {
-    out.status = GV_E_RANGE;
-    return;
+// Record the responses of one served step (worker threads).
+static void finish(gv_server *s, const std::vector<Item> &items, std::vector<gv_response> &out,
+                   double t_start, double t_done, uint64_t sid, uint64_t applied) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.update_pairs_applied += applied;
+    s->stats.free_blocks = s->w_free;
+    s->stats.total_blocks = s->w_total;
+    for (size_t i = 0; i < items.size(); ++i) {
+      gv_response &r = out[i];
+      r.kind = items[i].req.kind;
+      r.arrival_us = items[i].arrival_us;
+      r.start_us = t_start;
+      r.done_us = t_done;
+      r.step = sid;
+      r.step_size = (uint32_t)items.size();
+      s->resp[items[i].ticket] = r;
+      s->finished[items[i].ticket] = 1;
+      ++s->n_done;
+      ++s->stats.requests_done;
+      if (r.status) ++s->stats.refused;
+    }
   }
-  hipError_t e = hipMemcpy(s->bc_src, &r.source, 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    out.status = hip_fail(e, "bc source");
-    return;
+  s->cv_done.notify_all();
+}
+
+static void release_slot(gv_server *s, int k) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->slots[k].busy = false;
   }
-  out.status = gv_bc(s->g, s->bc_src, 1, s->bc_out, GV_MEM_DEVICE);
-  if (!out.status && r.x) {
-    e = hipMemcpy(r.x, s->bc_out, (size_t)s->V * 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) out.status = hip_fail(e, "bc copy");
+  s->cv_slot.notify_all();
+}
+
+// Update worker, PageRank or BC step: take a snapshot slot (waiting while
+// both are held: admission by snapshot memory), build the snapshot on the
+// graph's stream -- after every update that arrived before the step -- and
+// hand the step to the analytics worker.  Returns false when the step was
+// answered here instead (a refused request, or a graph the flat snapshot
+// cannot hold, served synchronously).
+static bool capture_analytics(gv_server *s, Cls c, std::vector<Item> &step,
+                              std::vector<gv_response> &out, double t_start, uint64_t sid) {
+  if (c == Cls::Bc && step[0].req.source >= s->V) {
+    out[0].status = GV_E_RANGE;
+    return false;
+  }
+  int k = -1;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_slot.wait(lk, [&] {
+      for (int i = 0; i < AN_SLOTS; ++i)
+        if (!s->slots[i].busy) return true;
+      return false;
+    });
+    for (int i = 0; i < AN_SLOTS; ++i)
+      if (!s->slots[i].busy) {
+        k = i;
+        break;
+      }
+    s->slots[k].busy = true;
+  }
+  AnSlot &sl = s->slots[k];
+  gv_status rc = c == Cls::Pagerank ? graph_pr_prepare(s->g, sl.w) : graph_bc_prepare(s->g, sl.w);
+  if (rc == GV_OK && hipEventRecord(sl.ready, s->stream) != hipSuccess)
+    rc = hip_fail(hipErrorUnknown, "serve event");
+  if (rc) {
+    release_slot(s, k);
+    if (rc == GV_E_UNSUPPORTED && c == Cls::Pagerank) {
+      // wait for the analytics worker to finish its PageRank jobs (the
+      // vectors are shared), then run this one here
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_slot.wait(lk, [&] {
+        for (int i = 0; i < AN_SLOTS; ++i)
+          if (s->slots[i].busy) return false;
+        return true;
+      });
+      lk.unlock();
+      serve_pagerank(s, step, out);
+    } else {
+      for (auto &o : out) o.status = rc;
+    }
+    return false;
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->ajobs.push_back(AnJob{c, step, k, t_start, sid});
+  }
+  s->cv_ajob.notify_one();
+  return true;
+}
+
+// Analytics worker: each job waits for its snapshot event on the analytics
+// stream and runs there, concurrently with later updates on the graph's
+// stream.
+static void analytics_loop(gv_server *s) {
+  {
+    gv_config cfg;
+    if (gv_get_config(s->g, &cfg) == GV_OK) hipSetDevice(cfg.device);
+  }
+  std::vector<gv_response> out;
+  for (;;) {
+    AnJob job;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_ajob.wait(lk, [&] { return s->astop || !s->ajobs.empty(); });
+      if (s->ajobs.empty()) break;
+      job = std::move(s->ajobs.front());
+      s->ajobs.pop_front();
+    }
+    AnSlot &sl = s->slots[job.slot];
+    out.assign(job.items.size(), gv_response{});
+    gv_status rc = GV_OK;
+    if (hipStreamWaitEvent(s->astream, sl.ready, 0) != hipSuccess)
+      rc = hip_fail(hipErrorUnknown, "serve wait");
+    if (job.cls == Cls::Pagerank) {
+      PrOpts o;
+      o.alpha = s->o.pr_alpha;
+      o.eps = s->o.pr_eps;
+      o.max_iter = s->o.pr_max_iter;
+      o.compensated = 1;
+      const int nxt = s->pr_cur == 0 ? 1 : 0;
+      if (s->pr_cur >= 0) o.init = s->pr[s->pr_cur];
+      int iters = 0;
+      double res = 0.0;
+      if (!rc) {
+        if (o.max_iter <= 0) {
+          // zero iterations: the starting vector itself
+          const float x0 = 1.0f / (float)s->V;
+          if (o.init) {
+            if (hipMemcpyAsync(s->pr[nxt], o.init, (size_t)s->V * 4, hipMemcpyDeviceToDevice,
+                               s->astream) != hipSuccess)
+              rc = hip_fail(hipErrorUnknown, "pagerank copy");
+          } else {
+            fill_f32(s->pr[nxt], x0, s->V, s->astream);
+          }
+        } else {
+          rc = pagerank_flat_iterate(o, sl.w, s->pr[nxt], &iters, &res, s->astream, nullptr);
+        }
+      }
+      if (!rc) s->pr_cur = nxt;
+      for (size_t i = 0; i < job.items.size(); ++i) {
+        out[i].status = rc;
+        if (rc) continue;
+        out[i].iters = iters;
+        out[i].residual = res;
+        if (job.items[i].req.x) {
+          hipError_t e = hipMemcpyAsync(job.items[i].req.x, s->pr[nxt], (size_t)s->V * 4,
+                                        hipMemcpyDeviceToHost, s->astream);
+          if (e == hipSuccess) e = hipStreamSynchronize(s->astream);
+          if (e != hipSuccess) out[i].status = hip_fail(e, "pagerank copy");
+        }
+      }
+    } else {
+      const gv_request &r = job.items[0].req;
+      if (!rc)
+        rc = bc_run_csr(sl.w.snap_off, sl.w.snap_cols, s->V, &r.source, 1, s->bc_out, s->astream,
+                        s->bcw);
+      if (!rc && r.x) {
+        hipError_t e = hipMemcpyAsync(r.x, s->bc_out, (size_t)s->V * 4, hipMemcpyDeviceToHost,
+                                      s->astream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->astream);
+        if (e != hipSuccess) rc = hip_fail(e, "bc copy");
+      }
+      out[0].status = rc;
+    }
+    hipStreamSynchronize(s->astream);
+    const double t_done = s->now_us();
+    release_slot(s, job.slot);
+    finish(s, job.items, out, job.t_start, t_done, job.step, 0);
   }
 }
 
@@ -242,6 +432,7 @@ static void worker_loop(gv_server *s) {
   std::vector<gv_response> out;
   for (;;) {
     step.clear();
+    uint64_t sid = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
@@ -277,40 +468,22 @@ static void worker_loop(gv_server *s) {
           s->queue.pop_front();
         }
       }
+      sid = s->stats.steps++;
     }
     const double t_start = s->now_us();
     out.assign(step.size(), gv_response{});
     uint64_t applied = 0;
-    switch (cls_of(step[0].req.kind)) {
+    const Cls c = cls_of(step[0].req.kind);
+    switch (c) {
       case Cls::Update: applied = serve_updates(s, step, out); break;
       case Cls::Neighbors: serve_neighbors(s, step, out); break;
-      case Cls::Pagerank: serve_pagerank(s, step, out); break;
-      case Cls::Bc: serve_bc(s, step[0], out[0]); break;
+      case Cls::Pagerank:
+      case Cls::Bc:
+        if (capture_analytics(s, c, step, out, t_start, sid)) continue;  // answered later
+        break;
     }
     hipStreamSynchronize(s->stream);
-    const double t_done = s->now_us();
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      const uint64_t sid = s->stats.steps++;
-      s->stats.update_pairs_applied += applied;
-      s->stats.free_blocks = s->w_free;
-      s->stats.total_blocks = s->w_total;
-      for (size_t i = 0; i < step.size(); ++i) {
-        gv_response &r = out[i];
-        r.kind = step[i].req.kind;
-        r.arrival_us = step[i].arrival_us;
-        r.start_us = t_start;
-        r.done_us = t_done;
-        r.step = sid;
-        r.step_size = (uint32_t)step.size();
-        s->resp[step[i].ticket] = r;
-        s->finished[step[i].ticket] = 1;
-        ++s->n_done;
-        ++s->stats.requests_done;
-        if (r.status) ++s->stats.refused;
-      }
-    }
-    s->cv_done.notify_all();
+    finish(s, step, out, t_start, s->now_us(), sid, applied);
   }
 }
 
@@ -326,7 +499,24 @@ gv_status gv_server_opts_default(gv_server_opts *o) {
   o->pr_alpha = 0.85f;
   o->pr_eps = 1e-6;
   o->pr_max_iter = 100;
+  o->analytics_cu_share = 0.75f;
   return GV_OK;
+}
+
+// The analytics stream, and the share of the compute units its kernels may
+// occupy: the persistent PageRank grid gets analytics_cu_share of the CUs
+// (one workgroup each, all of its LDS), the BC level kernels a grid of 8
+// workgroups per such CU, so the rest of the device stays free for the
+// short kernels of update steps.
+static hipError_t create_analytics_stream(gv_server *s) {
+  gv_config cfg;
+  int dev = 0, ncu = 0;
+  if (gv_get_config(s->g, &cfg) == GV_OK) dev = cfg.device;
+  hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  uint32_t n = (uint32_t)((float)ncu * s->o.analytics_cu_share + 0.5f);
+  s->an_cus = n ? n : 1u;
+  return hipStreamCreateWithFlags(&s->astream, hipStreamNonBlocking);
 }
 
 gv_status gv_server_create(gv_graph *g, const gv_server_opts *opts, gv_server **out) {
@@ -339,6 +529,10 @@ gv_status gv_server_create(gv_graph *g, const gv_server_opts *opts, gv_server **
   if (opts) o = *opts; else gv_server_opts_default(&o);
   if (o.max_step_pairs == 0 || o.max_step_requests == 0 || o.pr_max_iter < 0) {
     set_error("max_step_pairs and max_step_requests must be positive");
+    return GV_E_INVALID;
+  }
+  if (!(o.analytics_cu_share > 0.f && o.analytics_cu_share <= 1.f)) {
+    set_error("analytics_cu_share must be in (0, 1]");
     return GV_E_INVALID;
   }
   gv_server *s = new gv_server();
@@ -360,10 +554,39 @@ gv_status gv_server_create(gv_graph *g, const gv_server_opts *opts, gv_server **
         (e = hipMalloc(&s->pr[0], vb)) != hipSuccess ||
         (e = hipMalloc(&s->pr[1], vb)) != hipSuccess ||
         (e = hipMalloc(&s->bc_out, vb)) != hipSuccess ||
-        (e = hipMalloc(&s->bc_src, 4)) != hipSuccess)
+        (e = hipMalloc(&s->bc_src, 4)) != hipSuccess ||
+        (e = create_analytics_stream(s)) != hipSuccess)
       rc = hip_fail(e, "server alloc");
+    for (int k = 0; k < AN_SLOTS && !rc; ++k)
+      if ((e = hipEventCreateWithFlags(&s->slots[k].ready, hipEventDisableTiming)) != hipSuccess)
+        rc = hip_fail(e, "server event");
   }
+  // Everything the request path would otherwise allocate is sized now: a
+  // device free waits for the whole device, so a reallocation in the middle
+  // of the loop would stall updates behind a running PageRank or BC.
+  //  - update staging for the largest step;
+  if (!rc) rc = gv_reserve(g, o.max_step_pairs, o.max_step_requests);
+  //  - both analytics slots, for either snapshot kind, with room for the
+  //    graph to grow by half; BC workspace.
+  uint64_t E = 0;
+  if (!rc) rc = gv_num_edges(g, &E);
+  for (int k = 0; k < AN_SLOTS && !rc; ++k) {
+    s->slots[k].w.snap_min_cap = E + E / 2 + (1u << 20);
+    s->slots[k].w.pf_grid = s->an_cus;
+    gv_status r1 = graph_pr_prepare(g, s->slots[k].w);
+    if (r1 && r1 != GV_E_UNSUPPORTED) rc = r1;
+    if (!rc) rc = graph_bc_prepare(g, s->slots[k].w);
+  }
+  if (!rc) rc = bc_reserve(s->bcw, s->V);
+  s->bcw.max_blocks = 8 * s->an_cus;
+  if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = hip_fail(hipErrorUnknown, "server");
   if (rc) {
+    for (int k = 0; k < AN_SLOTS; ++k) {
+      pr_release(s->slots[k].w);
+      if (s->slots[k].ready) hipEventDestroy(s->slots[k].ready);
+    }
+    bc_release(s->bcw);
+    if (s->astream) hipStreamDestroy(s->astream);
     hipFree(s->stage_src);
     hipFree(s->stage_dst);
     hipFree(s->pr[0]);
@@ -375,6 +598,7 @@ gv_status gv_server_create(gv_graph *g, const gv_server_opts *opts, gv_server **
   }
   s->t0 = Clock::now();
   s->worker = std::thread(worker_loop, s);
+  s->aworker = std::thread(analytics_loop, s);
   *out = s;
   return GV_OK;
 }
@@ -476,7 +700,20 @@ gv_status gv_server_destroy(gv_server *s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->astop = true;  // the update worker has handed over its last job
+  }
+  s->cv_ajob.notify_all();
+  if (s->aworker.joinable()) s->aworker.join();
   hipStreamSynchronize(s->stream);
+  hipStreamSynchronize(s->astream);
+  for (int k = 0; k < AN_SLOTS; ++k) {
+    pr_release(s->slots[k].w);
+    hipEventDestroy(s->slots[k].ready);
+  }
+  bc_release(s->bcw);
+  hipStreamDestroy(s->astream);
   hipFree(s->stage_src);
   hipFree(s->stage_dst);
   hipFree(s->pr[0]);

@@ -66,6 +66,7 @@ class GvServerOpts(ctypes.Structure):
         ("pr_alpha", ctypes.c_float),
         ("pr_eps", ctypes.c_double),
         ("pr_max_iter", ctypes.c_int),
+        ("analytics_cu_share", ctypes.c_float),
     ]
 
 

@@ -22,8 +22,18 @@
  *   - consecutive PageRank refreshes share one run, warm-started from the
  *     server's previous PageRank vector (a cold start the first time);
  *   - each BC request (one source) runs on its own.
+ * PageRank and BC do not hold up the requests behind them: at its place in
+ * the arrival order such a step only takes a snapshot of the graph (the
+ * relabelled in-adjacency, or the out-adjacency as CSR) on the graph's
+ * stream and hands it to a second worker, which computes on its own stream
+ * while later updates and lookups go on.  A snapshot occupies one of two
+ * slots until its result is out; while both are held the next PageRank or
+ * BC step waits for one (admission by snapshot memory).  The analytics
+ * kernels are sized to occupy analytics_cu_share of the compute units, so
+ * the many short kernels of an update step do not queue behind a
+ * whole-chip analytics kernel.
  * A query sees every update that arrived before it and none after it.
- * Latency = done_us - arrival_us, where done_us is taken after the step's
+ * Latency = done_us - arrival_us, where done_us is taken after the request's
  * device work has finished.
  *
  * While a server runs it owns its graph: no other thread may call into that
@@ -82,6 +92,9 @@ typedef struct gv_server_opts {
   float pr_alpha;             /* PageRank refresh: damping */
   double pr_eps;              /*   L1 residual stop */
   int pr_max_iter;            /*   iteration cap */
+  float analytics_cu_share;   /* share of the compute units PageRank / BC kernels occupy (the
+                                 persistent PageRank grid, the BC level grids), so update
+                                 kernels always find free CUs; 1 = all */
 } gv_server_opts;
 
 typedef struct gv_server_stats {
