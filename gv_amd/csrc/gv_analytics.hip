@@ -1,18 +1,18 @@
-// gv_analytics.hip -- PageRank, triangle counting and betweenness centrality
-// over the device-resident block store.
+// gv_analytics.hip -- PageRank (row forms and the shared snapshot builder)
+// and betweenness centrality over the device-resident block store.
 //
 // PageRank (static_pagerank, pagerank.cu:3-68): pull form over the
 // in-adjacency so every vertex's sum is computed by one fixed lane schedule
-// (deterministic, no FP32 atomics).  By default each call first copies the
-// live in-adjacency into a CSR snapshot (one pass over the slots), so the
-// iterations stream plain rows instead of walking block tables and skipping
-// tombstones.  Vertices are binned by row length: one lane, 8 lanes, one
-// wavefront, or chunks of wave_max entries on many wavefronts whose partials
-// a follow-up kernel combines in chunk order (a kernel boundary instead of a
-// device-scope fence, which on a multi-XCD part would write back L2 per
-// chunk).  Each bin is its own launch with its own load depth.  Per-lane
-// Kahan sums are merged with TwoSum.  The L1 residual is reduced in FP64 in
-// a fixed order by a one-block kernel, which also sets the done flag, so
+// (deterministic, no FP32 atomics).  The default form iterates a flat tiled
+// stream (gv_pagerank_flat.hip) over the snapshot built here: a copy of the
+// live in-adjacency, relabelled by descending out-degree, written in one pass
+// over the slots.  The row forms in this file serve the other layouts:
+// vertices binned by row length (one lane, 8 lanes, one wavefront, or chunks
+// on many wavefronts whose partials a follow-up kernel joins in chunk order),
+// over the relabelled snapshot with LDS-staged hot contributions (k_pr_hot),
+// over CSR rows in vertex order, or straight over the block store.  Per-lane
+// Kahan sums are merged with TwoSum; the L1 residual is reduced in FP64 in a
+// fixed order by a one-block kernel that also sets the done flag, so
 // iterations need no host round trip.
 //
 // Triangle counting (static_tc, tc.cu:3-27) lives in gv_tc.hip.
