@@ -368,71 +368,114 @@ __device__ __forceinline__ void probe_hit(const Store &st, const ProbeArgs &a, u
 }
 
 constexpr int PROBE_ROUNDS = PROBE_CHUNK / WAVE;
+// An item this small is probed by one lane on its own.
+constexpr uint32_t PROBE_SMALL_SLOTS = 16;
+constexpr uint32_t PROBE_SMALL_KEYS = 4;
 
-// One wavefront per work item (PROBE_CHUNK slots of one source).  All eight
-// 64-slot rounds are loaded before any test so the slot reads overlap; the
-// source's sorted batch segment is held in registers (one key per lane) when
-// it has at most 64 keys, otherwise searched in memory.
+// Work items (PROBE_CHUNK slots of one source) are dealt to lanes: wave w of
+// W, step r, lane l takes item (r * 64 + l) * W + w, so the items of one hub
+// spread over many wavefronts.  A small item (<= PROBE_SMALL_SLOTS slots,
+// <= PROBE_SMALL_KEYS batch keys -- most touched vertices of an R-MAT batch)
+// is probed by its lane alone: 64 of them share one chain of dependent loads
+// instead of paying it one after another.  The other items of the step are
+// then probed by the whole wavefront, one at a time, their metadata broadcast
+// from the owning lane: all eight 64-slot rounds are loaded before any test,
+// and the source's sorted batch segment is held in registers (one key per
+// lane) when it has at most 64 keys, otherwise searched in memory.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t bV, Ctl *ctl) {
   if (MODE != PROBE_INSERT && ctl->abort) return;
   const uint32_t nitems = ctl->n_items;
   const uint32_t lane = lane_id();
   const uint64_t mask = (1ull << bV) - 1;
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < nitems;
-       it += nwaves) {
-    const uint32_t sidx = a.item_seg[it];
-    const uint32_t c = (uint32_t)it - a.item_start[sidx];
-    const uint32_t u = a.seg_src[sidx];
-    const uint32_t uo = a.seg_used[sidx];
-    const uint32_t tbl = st.tbl_off[u];
-    const uint32_t ks = a.seg_start[sidx], ke = a.seg_start[sidx + 1];
-    const uint32_t k = ke - ks;
-    uint32_t idx[PROBE_ROUNDS], w[PROBE_ROUNDS];
-#pragma unroll
-    for (int r = 0; r < PROBE_ROUNDS; ++r) {
-      uint32_t pos = c * PROBE_CHUNK + r * WAVE + lane;
-      idx[r] = pos < uo ? slot_index(st.arena, tbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
+  const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  for (uint64_t r = 0; r * WAVE * W + w < nitems; ++r) {
+    const uint64_t it = (r * WAVE + lane) * W + w;
+    const bool valid = it < nitems;
+    uint32_t sidx = 0, lo = 0, hi = 0, u = 0, tbl = 0, ks = 0, ke = 0, uo = 0;
+    if (valid) {
+      sidx = a.item_seg[it];
+      const uint32_t c = (uint32_t)it - a.item_start[sidx];
+      u = a.seg_src[sidx];
+      uo = a.seg_used[sidx];
+      tbl = st.tbl_off[u];
+      ks = a.seg_start[sidx];
+      ke = a.seg_start[sidx + 1];
+      lo = c * PROBE_CHUNK;
+      hi = lo + PROBE_CHUNK < uo ? lo + PROBE_CHUNK : uo;
     }
+    const bool small = valid && hi - lo <= PROBE_SMALL_SLOTS && ke - ks <= PROBE_SMALL_KEYS;
+    if (small) {
+      uint64_t kv[PROBE_SMALL_KEYS];
 #pragma unroll
-    for (int r = 0; r < PROBE_ROUNDS; ++r) w[r] = idx[r] != 0xFFFFFFFFu ? st.slots[idx[r]] : TOMB;
-    uint32_t cnt = 0;
-    if (k <= (uint32_t)WAVE) {
-      const uint32_t sv = lane < k ? (uint32_t)(a.keys[ks + lane] & mask) : 0xFFFFFFFFu;
+      for (uint32_t j = 0; j < PROBE_SMALL_KEYS; ++j)
+        kv[j] = ks + j < ke ? (a.keys[ks + j] & mask) : ~0ull;
+      uint32_t cnt = 0;
+      for (uint32_t pos = lo; pos < hi; ++pos) {
+        const uint32_t idx = slot_index(st.arena, tbl, pos, st.log2B, st.B);
+        const uint32_t x = st.slots[idx];
+        if (x == TOMB) continue;
+        // first matching key, as the lower bound of the wavefront path
+        uint32_t p = PROBE_SMALL_KEYS;
 #pragma unroll
-      for (int r = 0; r < PROBE_ROUNDS; ++r) {
-        uint32_t p = wave_lower_bound(sv, w[r]);
-        uint32_t hit = __shfl(sv, (int)p, WAVE);
-        if (w[r] != TOMB && hit == w[r]) probe_hit<MODE>(st, a, idx[r], ks + p, cnt);
+        for (uint32_t j = PROBE_SMALL_KEYS; j-- > 0;)
+          if (kv[j] == x) p = j;
+        if (p < PROBE_SMALL_KEYS) probe_hit<MODE>(st, a, idx, ks + p, cnt);
       }
-    } else {
-      const uint64_t ukey = (uint64_t)u << bV;
+      if (MODE != PROBE_INSERT && cnt) atomicAdd(&a.seg_del[sidx], cnt);
+    }
+    unsigned long long big = __ballot(valid && !small);
+    while (big) {
+      const int src = __ffsll(big) - 1;
+      big &= big - 1;
+      const uint32_t bsidx = __shfl(sidx, src, WAVE), blo = __shfl(lo, src, WAVE);
+      const uint32_t bhi = __shfl(hi, src, WAVE), bu = __shfl(u, src, WAVE);
+      const uint32_t btbl = __shfl(tbl, src, WAVE);
+      const uint32_t bks = __shfl(ks, src, WAVE), bke = __shfl(ke, src, WAVE);
+      const uint32_t k = bke - bks;
+      uint32_t idx[PROBE_ROUNDS], wv[PROBE_ROUNDS];
 #pragma unroll
-      for (int r = 0; r < PROBE_ROUNDS; ++r) {
-        if (w[r] == TOMB) continue;
-        uint64_t key = ukey | w[r];
-        uint32_t lo = ks, hi = ke;
-        while (lo < hi) {
-          uint32_t mid = (lo + hi) >> 1;
-          if (a.keys[mid] < key) lo = mid + 1; else hi = mid;
+      for (int q = 0; q < PROBE_ROUNDS; ++q) {
+        const uint32_t pos = blo + q * WAVE + lane;
+        idx[q] = pos < bhi ? slot_index(st.arena, btbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int q = 0; q < PROBE_ROUNDS; ++q)
+        wv[q] = idx[q] != 0xFFFFFFFFu ? st.slots[idx[q]] : TOMB;
+      uint32_t cnt = 0;
+      if (k <= (uint32_t)WAVE) {
+        const uint32_t sv = lane < k ? (uint32_t)(a.keys[bks + lane] & mask) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int q = 0; q < PROBE_ROUNDS; ++q) {
+          const uint32_t p = wave_lower_bound(sv, wv[q]);
+          const uint32_t hit = __shfl(sv, (int)p, WAVE);
+          if (wv[q] != TOMB && hit == wv[q]) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
         }
-        if (lo < ke && a.keys[lo] == key) probe_hit<MODE>(st, a, idx[r], lo, cnt);
+      } else {
+        const uint64_t ukey = (uint64_t)bu << bV;
+#pragma unroll
+        for (int q = 0; q < PROBE_ROUNDS; ++q) {
+          if (wv[q] == TOMB) continue;
+          const uint64_t key = ukey | wv[q];
+          uint32_t l2 = bks, h2 = bke;
+          while (l2 < h2) {
+            const uint32_t mid = (l2 + h2) >> 1;
+            if (a.keys[mid] < key) l2 = mid + 1; else h2 = mid;
+          }
+          if (l2 < bke && a.keys[l2] == key) probe_hit<MODE>(st, a, idx[q], l2, cnt);
+        }
       }
-    }
-    if (MODE != PROBE_INSERT) {
-      cnt = wave_sum(cnt);
-      if (lane == 0 && cnt) atomicAdd(&a.seg_del[sidx], cnt);
+      if (MODE != PROBE_INSERT) {
+        cnt = wave_sum(cnt);
+        if (lane == 0 && cnt) atomicAdd(&a.seg_del[bsidx], cnt);
+      }
     }
   }
 }
 
-// Grid of a grid-stride wavefront loop over at most items_bound items: one
-// block (4 waves) per 4 items, capped at 2048 blocks -- 8 resident blocks on
-// each of the 256 CUs -- so an over-estimated bound does not dispatch
-// thousands of blocks that exit at once.
 static unsigned probe_grid(uint64_t items_bound) {
-  uint64_t g = cdiv(items_bound ? items_bound : 1, 4);  // 4 waves per block
+  uint64_t g = cdiv(items_bound ? items_bound : 1, 4 * 16);  // 4 waves per block, >= 16 items each
   return (unsigned)(g < 2048 ? g : 2048);
 }
 
