@@ -135,6 +135,9 @@ def run_rmat(args):
     setup_s = time.time() - t_setup
 
     pr_out = DeviceArray(V, np.float32)
+    # degrees and batch sources before the timed batches, for the update
+    # path's algorithmic bytes (touched adjacency read by the probes)
+    deg0 = None
 
     def run_batch(b):
         ds, dd, is_, id_ = batches[b]
@@ -146,6 +149,7 @@ def run_rmat(args):
         run_batch(b)
     g.sync()
     synchronize()
+    deg0 = g.degrees().astype(np.int64)
 
     # Plain device-to-device copy of a known size (the runtime's copy kernel):
     # the calibration point for HBM byte counters collected with --pmc.
@@ -243,12 +247,59 @@ def run_rmat(args):
         }
     if tc_ms:
         out["triangle_count"] = {"ms_mean": float(np.mean(tc_ms)), "total_last": int(tc_total[-1])}
+    out["rooflines"] = other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms)
 
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
     g.close()
     if args.batched:
         out["batched"] = run_batched(args, G, skey, V, n_raw, batches, E_end, value)
+    return out
+
+
+def _roof(nbytes, ms, what):
+    gbs = nbytes / (ms / 1e3) / 1e9
+    return {"what": what, "algorithmic_bytes": nbytes, "ms": ms, "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
+def other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms):
+    """Algorithmic-byte rooflines of the update path, triangle counting and
+    BC on the same graph (secondary to the PageRank line):
+      update batch: 12 k + sum over touched sources of 4 d(u) + 16 |T| for
+        each half (k directed entries), with d from the degrees before the
+        timed batches;
+      TC: sum over oriented edges (u, v) of 4 (d+(u) + d+(v));
+      BC: 24 E + 24 V per source, 16 sources."""
+    out = {}
+    nb = 0
+    for b in range(args.warmup, args.warmup + len(upd_ms)):
+        for s_arr, d_arr in (batches[b][0:2], batches[b][2:4]):
+            su, sd = s_arr.numpy().astype(np.int64), d_arr.numpy().astype(np.int64)
+            src = np.concatenate([su, sd])  # undirected: both directions
+            t = np.unique(src)
+            nb += 12 * len(src) + 4 * int(deg0[t].sum()) + 16 * len(t)
+    out["update_batch"] = _roof(nb / len(upd_ms), float(np.mean(upd_ms)),
+                                "plain update batch (delete half, insert half, compaction)")
+    off, cols = g.export_csr()
+    deg = np.diff(off).astype(np.int64)
+    if tc_ms:
+        u = np.repeat(np.arange(V, dtype=np.int64), deg)
+        w = cols.astype(np.int64)
+        keep = (deg[w] > deg[u]) | ((deg[w] == deg[u]) & (w > u))
+        u, w = u[keep], w[keep]
+        dplus = np.bincount(u, minlength=V).astype(np.int64)
+        nbytes = 4 * int((dplus[u] + dplus[w]).sum())
+        out["triangle_count"] = _roof(nbytes, float(tc_ms[-1]), "exact triangle count, last graph")
+        del u, w, keep
+    E = int(off[-1])
+    srcs = np.random.default_rng(args.seed).integers(0, V, 16).astype(np.uint32)
+    G.bc(g, srcs[:1])
+    t0 = time.perf_counter()
+    G.bc(g, srcs)
+    bc_ms = (time.perf_counter() - t0) * 1e3
+    out["bc_16_sources"] = _roof(16 * (24 * E + 24 * V), bc_ms,
+                                 "Brandes BC from 16 sources (batched), host-timed")
     return out
 
 
