@@ -93,6 +93,10 @@ struct PrOpts {
   int max_iter = 100;
   const float *init = nullptr;  // device vector to start from (warm start), or null = 1/V
   int compensated = 1;          // per-vertex Kahan sums
+  // Called before each iteration is issued (the iterations then run one at
+  // a time): lets a caller hold further work back while other work waits.
+  void (*yield)(void *) = nullptr;
+  void *yield_arg = nullptr;
 };
 
 struct PrBins {
@@ -176,6 +180,9 @@ struct BcWork {
   uint32_t *queue = nullptr;
   unsigned int *cursor = nullptr, *h_cursor = nullptr;
   uint32_t max_blocks = 0;  // grid cap of the level kernels (0 = 8192)
+  // Called before each level is issued, as PrOpts::yield.
+  void (*yield)(void *) = nullptr;
+  void *yield_arg = nullptr;
 };
 gv_status bc_reserve(BcWork &w, uint32_t V);
 void bc_release(BcWork &w);

@@ -1003,10 +1003,13 @@ __global__ void k_bc_src(int *dist, double *sigma, uint32_t *queue, uint32_t src
 
 // One wavefront per frontier vertex; Adj is the block store (tombstones
 // skipped) or a CSR snapshot.
+// Frontier vertices queue[qs, qe); newly reached vertices are appended at
+// queue[qapp + ...] (qapp = the end of the whole level when a level is
+// issued in several launches).
 template <class Adj>
 __global__ __launch_bounds__(256) void k_bc_fwd(Adj adj, int *dist, double *sigma,
-                                                uint32_t *queue, uint32_t qs, uint32_t qe, int d,
-                                                unsigned int *cursor) {
+                                                uint32_t *queue, uint32_t qs, uint32_t qe,
+                                                uint32_t qapp, int d, unsigned int *cursor) {
   const uint32_t lane = lane_id();
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   const uint32_t nf = qe - qs;
@@ -1037,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_bc_fwd(Adj adj, int *dist, double *sigm
       uint32_t pos = 0;
       if (lane == 0 && m) pos = atomicAdd(cursor, (unsigned)__popcll(m));
       pos = __shfl(pos, 0, WAVE);
-      if (fresh) queue[qe + pos + mask_lt_count(m)] = w;
+      if (fresh) queue[qapp + pos + mask_lt_count(m)] = w;
     }
   }
 }
@@ -1118,6 +1121,7 @@ static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
   gv_status rc = bc_reserve(w, V);
   if (rc) return rc;
   const unsigned cap = w.max_blocks ? w.max_blocks : 8192u;
+  const uint32_t ch = w.yield ? (1u << 20) : 0xFFFFFFFFu;
   fill_f32(out_dev, 0.f, V, s);
   fill_f32(w.bcc, 0.f, V, s);
   for (uint32_t si = 0; si < n_sources && rc == GV_OK; ++si) {
@@ -1130,8 +1134,22 @@ static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
       uint32_t qs = lvl[d], qe = lvl[d + 1];
       if (qe == qs) break;
       fill_u32(w.cursor, 0, 1, s);
-      k_bc_fwd<Adj><<<grid_for((uint64_t)(qe - qs) * WAVE, 256, cap), 256, 0, s>>>(
-          adj, w.dist, w.sigma, w.queue, qs, qe, d, w.cursor);
+      // with a yield hook a level is issued in pieces of at most ch vertices
+      // and the caller may hold back the next piece
+      for (uint64_t c0 = qs; c0 < qe && rc == GV_OK; c0 += ch) {
+        const uint32_t c1 = qe - c0 > ch ? (uint32_t)(c0 + ch) : qe;
+        if (w.yield) {
+          if (c0 > qs && hipStreamSynchronize(s) != hipSuccess) {
+            set_error("bc: HIP error");
+            rc = GV_E_HIP;
+            break;
+          }
+          w.yield(w.yield_arg);
+        }
+        k_bc_fwd<Adj><<<grid_for((uint64_t)(c1 - c0) * WAVE, 256, cap), 256, 0, s>>>(
+            adj, w.dist, w.sigma, w.queue, (uint32_t)c0, c1, qe, d, w.cursor);
+      }
+      if (rc) break;
       if (hipMemcpyAsync(w.h_cursor, w.cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s) !=
               hipSuccess ||
           hipStreamSynchronize(s) != hipSuccess) {
@@ -1145,9 +1163,21 @@ static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
     // levels 0..D-1 are non-empty; level D-1 has no successors
     const int D = (int)lvl.size() - 2;
     for (int lev = D - 2; lev >= 0 && rc == GV_OK; --lev) {
-      uint32_t qs = lvl[lev], qe = lvl[lev + 1];
-      k_bc_bwd<Adj><<<grid_for((uint64_t)(qe - qs) * WAVE, 256, cap), 256, 0, s>>>(
-          adj, w.dist, w.sigma, w.delta, w.queue, qs, qe, lev, src, out_dev, w.bcc);
+      const uint32_t qs = lvl[lev], qe = lvl[lev + 1];
+      for (uint64_t c0 = qs; c0 < qe && rc == GV_OK; c0 += ch) {
+        const uint32_t c1 = qe - c0 > ch ? (uint32_t)(c0 + ch) : qe;
+        if (w.yield) {
+          // one piece at a time: finish the previous one before yielding
+          if (hipStreamSynchronize(s) != hipSuccess) {
+            set_error("bc: HIP error");
+            rc = GV_E_HIP;
+            break;
+          }
+          w.yield(w.yield_arg);
+        }
+        k_bc_bwd<Adj><<<grid_for((uint64_t)(c1 - c0) * WAVE, 256, cap), 256, 0, s>>>(
+            adj, w.dist, w.sigma, w.delta, w.queue, (uint32_t)c0, c1, lev, src, out_dev, w.bcc);
+      }
     }
   }
   if (rc == GV_OK && hipStreamSynchronize(s) != hipSuccess) {

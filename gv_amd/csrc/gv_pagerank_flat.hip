@@ -527,9 +527,11 @@ gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int 
     }
   }
   int launched = 0;
-  const int check_every = 8;
+  // with a yield hook each iteration is checked before the next is issued
+  const int check_every = o.yield ? 1 : 8;
   while (launched < o.max_iter) {
     const int t = launched;
+    if (o.yield) o.yield(o.yield_arg);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
     a.x = w.x[t & 1];
     a.cb = w.contrib[t & 1];

@@ -88,7 +88,8 @@ struct gv_server {
   uint32_t an_cus = 0;      // compute units the analytics kernels may occupy
   std::thread aworker;
   std::deque<AnJob> ajobs;  // guarded by mu
-  std::condition_variable cv_ajob, cv_slot;
+  std::condition_variable cv_ajob, cv_slot, cv_idle;
+  bool u_busy = false;      // the update worker is serving a step (guarded by mu)
   bool astop = false;       // guarded by mu
   AnSlot slots[AN_SLOTS];   // busy flags guarded by mu
   BcWork bcw;               // analytics worker only
@@ -341,6 +342,17 @@ static bool capture_analytics(gv_server *s, Cls c, std::vector<Item> &step,
   return true;
 }
 
+// Before each PageRank iteration or BC level the analytics worker steps
+// aside while update-side requests wait or run (for at most 5 ms a time):
+// updates and lookups then wait for one analytics kernel, not for a whole
+// job, and the job takes the device time they leave.
+static void analytics_yield(void *arg) {
+  gv_server *s = static_cast<gv_server *>(arg);
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_idle.wait_for(lk, std::chrono::milliseconds(5),
+                      [&] { return s->stop || (!s->u_busy && s->queue.empty()); });
+}
+
 // Analytics worker: each job waits for its snapshot event on the analytics
 // stream and runs there, concurrently with later updates on the graph's
 // stream.
@@ -370,6 +382,8 @@ static void analytics_loop(gv_server *s) {
       o.eps = s->o.pr_eps;
       o.max_iter = s->o.pr_max_iter;
       o.compensated = 1;
+      o.yield = analytics_yield;
+      o.yield_arg = s;
       const int nxt = s->pr_cur == 0 ? 1 : 0;
       if (s->pr_cur >= 0) o.init = s->pr[s->pr_cur];
       int iters = 0;
@@ -422,6 +436,14 @@ static void analytics_loop(gv_server *s) {
   }
 }
 
+static void set_idle(gv_server *s) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->u_busy = false;
+  }
+  s->cv_idle.notify_all();
+}
+
 static void worker_loop(gv_server *s) {
   hipSetDevice(0);
   {
@@ -469,6 +491,7 @@ static void worker_loop(gv_server *s) {
         }
       }
       sid = s->stats.steps++;
+      s->u_busy = true;
     }
     const double t_start = s->now_us();
     out.assign(step.size(), gv_response{});
@@ -479,12 +502,17 @@ static void worker_loop(gv_server *s) {
       case Cls::Neighbors: serve_neighbors(s, step, out); break;
       case Cls::Pagerank:
       case Cls::Bc:
-        if (capture_analytics(s, c, step, out, t_start, sid)) continue;  // answered later
+        if (capture_analytics(s, c, step, out, t_start, sid)) {  // answered later
+          set_idle(s);
+          continue;
+        }
         break;
     }
     hipStreamSynchronize(s->stream);
     finish(s, step, out, t_start, s->now_us(), sid, applied);
+    set_idle(s);
   }
+  set_idle(s);
 }
 
 extern "C" {
@@ -579,6 +607,8 @@ gv_status gv_server_create(gv_graph *g, const gv_server_opts *opts, gv_server **
   }
   if (!rc) rc = bc_reserve(s->bcw, s->V);
   s->bcw.max_blocks = 8 * s->an_cus;
+  s->bcw.yield = analytics_yield;
+  s->bcw.yield_arg = s;
   if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = hip_fail(hipErrorUnknown, "server");
   if (rc) {
     for (int k = 0; k < AN_SLOTS; ++k) {

@@ -216,25 +216,36 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
         uint32_t h = tc_slot(x, shift);
         while (atomicCAS(&tab[h], TC_EMPTY, x) != TC_EMPTY) h = (h + 1) & hmask;
       }
+      // two wedges per lane and step (f and f + 64), so each lane has two
+      // independent loads in flight before it probes
       uint32_t k = 0;  // this lane's current block entry (advances monotonically)
-      for (uint32_t fb = 0; fb < nwed; fb += WAVE) {
-        const uint32_t f = fb + lane;
-        if (f < nwed) {
-          while (s_pre[wid][k] <= f) ++k;
-          const uint32_t start = k ? s_pre[wid][k - 1] : 0u;
-          const uint32_t wv = cols[s_off[wid][k] + (f - start)];
-          uint32_t h = tc_slot(wv, shift);
+      for (uint32_t fb = 0; fb < nwed; fb += 2 * WAVE) {
+        uint32_t wv[2] = {TC_EMPTY, TC_EMPTY}, kv[2] = {0, 0};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const uint32_t f = fb + r * WAVE + lane;
+          if (f < nwed) {
+            while (s_pre[wid][k] <= f) ++k;
+            const uint32_t start = k ? s_pre[wid][k - 1] : 0u;
+            kv[r] = k;
+            wv[r] = cols[s_off[wid][k] + (f - start)];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          if (wv[r] == TC_EMPTY) continue;
+          uint32_t h = tc_slot(wv[r], shift);
           bool found = false;
           while (true) {
             const uint32_t e = tab[h];
-            if (e == wv) {
+            if (e == wv[r]) {
               found = true;
               break;
             }
             if (e == TC_EMPTY) break;
             h = (h + 1) & hmask;
           }
-          if (found) tc_hit<PV>(u, s_v[wid][k], wv, cnt, own, pv);
+          if (found) tc_hit<PV>(u, s_v[wid][kv[r]], wv[r], cnt, own, pv);
         }
       }
     }

@@ -222,6 +222,18 @@ def test_bc_config1_16_sources():
     check_bc(gu, ou, src)
 
 
+@pytest.mark.parametrize("directed", [True, False])
+def test_bc_single_source_calls(directed):
+    # one source per call: the level-synchronous kernels (not the 32-source
+    # batches), each level's frontier issued and extended in one piece
+    V = 1 << 14
+    s, d = O.gen_rmat(O.stream_key(1, O.STREAM_RMAT), 14, 16 * V)
+    g, o = build_pair(V, directed, s, d, pool=6 * V)
+    rng = np.random.default_rng(7)
+    for src in rng.choice(V, 4, replace=False).astype(np.uint32):
+        check_bc(g, o, np.array([src], np.uint32))
+
+
 def test_bc_rejects_bad_source():
     g, _ = build_pair(10, False, [0, 1], [1, 2])
     with pytest.raises(G.GraphVineError) as e:
