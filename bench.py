@@ -251,6 +251,8 @@ def run_rmat(args):
 
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
+    if not args.no_tc:
+        out["tc_update"] = tc_update(g, G, V, args)
     g.close()
     if args.batched:
         out["batched"] = run_batched(args, G, skey, V, n_raw, batches, E_end, value)
@@ -301,6 +303,50 @@ def other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms):
     out["bc_16_sources"] = _roof(16 * (24 * E + 24 * V), bc_ms,
                                  "Brandes BC from 16 sources (batched), host-timed")
     return out
+
+
+def tc_update(g, G, V, args):
+    """Exact triangle-count update (gv_triangle_count_update) over one more
+    batch of the same kind (a delete half sampled from the current graph, then
+    a uniform insert half), per-vertex counts kept on the device, against the
+    full recount; the updated counts are checked against a recount after."""
+    from gv_amd.hip import DeviceArray, Event
+    STREAM_INS, STREAM_DEL = 2, 3
+    half = args.batch // 2
+    off, cols, _ = G.export_csr_device(g)
+    ds, dd = G.gen_sample(g, skey(args.seed, STREAM_DEL * 1000 + 999), V, off, cols, half)
+    is_, id_ = G.gen_uniform(g, skey(args.seed, STREAM_INS * 1000 + 999), V, half)
+    off.free()
+    cols.free()
+    pv, pv2 = DeviceArray(V, np.uint64), DeviceArray(V, np.uint64)
+    tot, tot2 = ctypes.c_uint64(), ctypes.c_uint64()
+    L = G.lib()
+    stream = g.stream()
+    ev0, ev1 = Event(), Event()
+    ev0.record(stream)
+    G._check(L.gv_triangle_count(g.handle, pv.ptr, G.GV_MEM_DEVICE, ctypes.byref(tot)))
+    ev1.record(stream)
+    ev1.synchronize()
+    res = {"full_recount_ms": ev0.elapsed_ms(ev1)}
+    for name, s_arr, d_arr, kind in (("delete", ds, dd, G.GV_OP_DELETE),
+                                     ("insert", is_, id_, G.GV_OP_INSERT)):
+        ev0.record(stream)
+        G._check(L.gv_triangle_count_update(g.handle, kind, s_arr.ptr, d_arr.ptr, half,
+                                            G.GV_MEM_DEVICE, pv.ptr, G.GV_MEM_DEVICE,
+                                            ctypes.byref(tot), None))
+        ev1.record(stream)
+        ev1.synchronize()
+        res[f"{name}_half_ms"] = ev0.elapsed_ms(ev1)
+    G._check(L.gv_triangle_count(g.handle, pv2.ptr, G.GV_MEM_DEVICE, ctypes.byref(tot2)))
+    res["exact"] = bool(tot.value == tot2.value and np.array_equal(pv.numpy(), pv2.numpy()))
+    res["speedup_vs_recount"] = res["full_recount_ms"] / (res["delete_half_ms"]
+                                                          + res["insert_half_ms"])
+    res["what"] = (f"one {args.batch}-edge batch (delete half, insert half), each applied with "
+                   "its triangle-count update; times include the update itself and the "
+                   "host round trips of the call")
+    for a in (ds, dd, is_, id_, pv, pv2):
+        a.free()
+    return res
 
 
 def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):

@@ -143,6 +143,8 @@ def lib():
         "gv_pagerank_personalized": [P, ctypes.c_float, ctypes.c_double, i32, P, u32, P, i32,
                                      ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)],
         "gv_triangle_count": [P, P, i32, ctypes.POINTER(u64)],
+        "gv_triangle_count_update": [P, i32, P, P, u64, i32, P, i32, ctypes.POINTER(u64),
+                                     ctypes.POINTER(GvBatchStats)],
         "gv_bc": [P, P, u32, P, i32],
         "gv_gen_rmat": [P, u64, i32, u64, u32, u32, u32, P, P],
         "gv_gen_uniform": [P, u64, u32, u64, P, P],
@@ -443,6 +445,34 @@ def static_tc(graph, per_vertex=True):
                                    pv.ctypes.data_as(ctypes.c_void_p) if per_vertex else None,
                                    GV_MEM_HOST, ctypes.byref(tot)))
     return pv, int(tot.value)
+
+
+# reference: dynamic_tc (tc.cu:29-64); the batch is explicit (the reference
+# reads the affected set its last batchInsert left behind)
+def dynamic_tc(graph, src, dst, per_vertex, total, delete=False, stats=False):
+    """Apply one insert (or, delete=True, delete) batch to an undirected graph
+    and update counts taken before it: per_vertex (uint64 host array of V,
+    updated in place; may be None) and total.  Returns (total, stats)."""
+    ps, ks, ms = _ptr_u32(src)
+    pd, kd, md = _ptr_u32(dst)
+    if ms != md:
+        raise ValueError("src and dst must both be host or both be device arrays")
+    n = int(len(src))
+    if int(len(dst)) != n:
+        raise ValueError("src and dst lengths differ")
+    ppv = None
+    if per_vertex is not None:
+        if (not isinstance(per_vertex, np.ndarray) or per_vertex.dtype != np.uint64
+                or per_vertex.shape != (graph.num_vertices,)
+                or not per_vertex.flags.c_contiguous):
+            raise TypeError("per_vertex must be a contiguous uint64 array of V entries")
+        ppv = per_vertex.ctypes.data_as(ctypes.c_void_p)
+    tot = ctypes.c_uint64(int(total))
+    st = GvBatchStats()
+    _check(lib().gv_triangle_count_update(graph.handle, GV_OP_DELETE if delete else GV_OP_INSERT,
+                                          ps, pd, n, ms, ppv, GV_MEM_HOST, ctypes.byref(tot),
+                                          ctypes.byref(st) if stats else None))
+    return int(tot.value), (st.as_dict() if stats else None)
 
 
 # reference: bc (bc.cu:3-88); sources are explicit (the reference ignores its

@@ -168,6 +168,25 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
 gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t *total,
                              hipStream_t s);
 
+// Grow-only workspace of tc_delta.
+struct TcdWork {
+  uint32_t *cnt = nullptr, *start = nullptr, *iseg = nullptr;
+  void *tmp = nullptr;
+  unsigned long long *dev = nullptr;  // [0] total change, [1] block-level items
+  size_t cnt_bytes = 0, start_bytes = 0, iseg_bytes = 0, tmp_bytes = 0, dev_bytes = 0;
+};
+void tcd_release(TcdWork &w);
+// Change of the triangle counts made by the staged batch in sc (keys: its m
+// sorted keys).  Changed edges are the first copies of keys whose flag equals
+// sflag: new edges after an insert (sflag 1, counted in the graph after it)
+// or present edges before a delete (sflag 0 after a membership probe, counted
+// before the delete, neg = 1).  per_vertex_dev (may be null) is updated in
+// place; *delta receives the change of the total.  Skips everything when
+// the store holds an abort code.
+gv_status tc_delta(const Store &st, const Scratch &sc, const uint64_t *keys, uint64_t m,
+                   uint32_t sflag, int neg, uint64_t *per_vertex_dev, TcdWork &w,
+                   long long *delta, hipStream_t s);
+
 // Brandes BC from the given host source list; out_dev[V] receives BC.
 gv_status bc_run(const Store &st, const uint32_t *sources_host, uint32_t n_sources,
                  float *out_dev, hipStream_t s);

@@ -35,6 +35,7 @@ struct gv_graph {
   uint32_t *vlist = nullptr;
   PrWork pr;
   PsWork ps;
+  TcdWork tcd;
   Ctl *h_ctl = nullptr;  // pinned, 2 entries
   uint32_t *in_src = nullptr, *in_dst = nullptr;
   uint64_t in_cap = 0;
@@ -515,6 +516,7 @@ gv_status gv_destroy(gv_graph *g) {
   }
   pr_release(g->pr);
   ps_release(g->ps);
+  tcd_release(g->tcd);
   hipFree(g->vlist);
   hipFree(g->in_src);
   hipFree(g->in_dst);
@@ -896,6 +898,91 @@ gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t
     if (e != hipSuccess) rc = hip_fail(e, "tc copy");
   }
   if (per_vertex && mem != GV_MEM_DEVICE) hipFree(dpv);
+  return rc;
+}
+
+gv_status gv_triangle_count_update(gv_graph *g, int kind, const uint32_t *src, const uint32_t *dst,
+                                   uint64_t n, int mem, uint64_t *per_vertex, int pv_mem,
+                                   uint64_t *total, gv_batch_stats *stats) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!total || (n && (!src || !dst))) {
+    set_error("null pointer argument");
+    return GV_E_INVALID;
+  }
+  if (kind != GV_OP_INSERT && kind != GV_OP_DELETE) {
+    set_error("kind must be GV_OP_INSERT or GV_OP_DELETE");
+    return GV_E_INVALID;
+  }
+  if (per_vertex && pv_mem != GV_MEM_HOST && pv_mem != GV_MEM_DEVICE) {
+    set_error("pv_mem must be GV_MEM_HOST or GV_MEM_DEVICE");
+    return GV_E_INVALID;
+  }
+  if (g->directed) {
+    set_error("triangle counting is defined on undirected graphs");
+    return GV_E_UNSUPPORTED;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return GV_OK;
+  if (n > 0x7FFFFFFFull) {
+    set_error("batch larger than 2^31 pairs; split it");
+    return GV_E_INVALID;
+  }
+  const uint32_t *ds = nullptr, *dd = nullptr;
+  rc = device_inputs(g, src, dst, n, mem, &ds, &dd);
+  if (rc) return rc;
+  rc = reserve_keys(g, 0, 2 * n);
+  if (rc) return rc;
+  hipStream_t s = g->stream;
+  uint64_t *dpv = per_vertex;
+  if (per_vertex && pv_mem == GV_MEM_HOST) {
+    hipError_t e = hipMalloc(&dpv, (size_t)g->V * 8);
+    if (e != hipSuccess) return hip_fail(e, "tc update alloc");
+    e = hipMemcpyAsync(dpv, per_vertex, (size_t)g->V * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      hipFree(dpv);
+      return hip_fail(e, "tc update copy");
+    }
+  }
+  Store &st = g->st[0];
+  Scratch &sc = g->sc[0];
+  uint64_t m = 0;
+  long long delta = 0;
+  op_begin(st, s);
+  uint64_t *keys = stage(st, sc, ds, dd, n, 1, 0, m, s);
+  segment(st, sc, keys, m, s);
+  if (kind == GV_OP_INSERT) {
+    // the new edges are the keys the membership probe left flagged
+    insert_plan(st, sc, keys, m, s);
+    commit_pair(st, sc, nullptr, nullptr, s);
+    insert_write(st, sc, keys, m, s);
+    rc = tc_delta(st, sc, keys, m, 1, 0, dpv, g->tcd, &delta, s);
+  } else {
+    // removed triangles are counted before the tombstones go in
+    probe_present(st, sc, keys, m, s);
+    rc = tc_delta(st, sc, keys, m, 0, 1, dpv, g->tcd, &delta, s);
+    if (!rc) delete_apply(st, sc, keys, m, s);
+  }
+  if (!rc) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "tc update");
+  }
+  if (!rc) rc = read_ctl(g);
+  if (!rc) {
+    if (stats) fill_stats(g, n, stats);
+    rc = take_abort(g);
+  }
+  if (!rc) *total += (uint64_t)delta;
+  if (per_vertex && pv_mem == GV_MEM_HOST) {
+    if (!rc) {
+      hipError_t e = hipMemcpyAsync(per_vertex, dpv, (size_t)g->V * 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) rc = hip_fail(e, "tc update copy");
+    } else {
+      hipStreamSynchronize(s);
+    }
+    hipFree(dpv);
+  }
   return rc;
 }
 

@@ -43,6 +43,8 @@ void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream
 void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Survivor scan + per-segment block/table plan (after an insert probe).
 void insert_alloc(Store &st, Scratch &sc, uint64_t m, hipStream_t s);
+// Membership: clears sc.flag of the staged keys already in the store.
+void probe_present(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Insert: probe existing slots, plan block/table allocation.
 void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Joint all-or-nothing pool check over one or two stores (b may be null).

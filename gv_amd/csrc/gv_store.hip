@@ -607,12 +607,17 @@ __global__ void k_ins_write(Store st, WriteArgs a, uint32_t bV, Ctl *ctl) {
   }
 }
 
-void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+void probe_present(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, sc.flag,
                nullptr, nullptr};
   k_probe<PROBE_INSERT><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
+}
+
+void insert_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  probe_present(st, sc, keys, m, s);
   insert_alloc(st, sc, m, s);
 }
 

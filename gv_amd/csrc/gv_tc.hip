@@ -145,15 +145,65 @@ __global__ __launch_bounds__(256) void k_tc_wkeys(const uint64_t *off, const uin
 }
 
 // ------------------------------------------------------------ counting --
+// Per-block LDS cache of per-vertex count increments.  The minimum vertex of
+// a triangle is concentrated on few IDs (low IDs, hubs), and global atomics
+// on one address serialise; a vertex's increments go to an LDS counter once
+// it holds a slot (first come, at most PVC_PROBE probes, never evicted) and
+// to the global count otherwise.  pvc_flush adds the block's counters to the
+// global counts.  Increments are taken mod 2^64 (a decrement is ~0).
+constexpr uint32_t PVC_N = 256;
+constexpr uint32_t PVC_PROBE = 4;
+constexpr uint32_t PVC_EMPTY = 0xFFFFFFFFu;
+
+struct PvCache {
+  uint32_t key[PVC_N];
+  unsigned long long cnt[PVC_N];
+};
+
+template <int NT>
+__device__ __forceinline__ void pvc_init(PvCache &c) {
+  for (uint32_t i = threadIdx.x; i < PVC_N; i += NT) {
+    c.key[i] = PVC_EMPTY;
+    c.cnt[i] = 0;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void pvc_add(PvCache &c, unsigned long long *pv, uint32_t id,
+                                        unsigned long long inc) {
+  uint32_t h = (id * 0x9E3779B1u) >> 24;  // 8 bits = log2(PVC_N)
+  for (uint32_t p = 0; p < PVC_PROBE; ++p) {
+    uint32_t k = c.key[h];
+    if (k == PVC_EMPTY) k = atomicCAS(&c.key[h], PVC_EMPTY, id);
+    if (k == PVC_EMPTY || k == id) {
+      atomicAdd(&c.cnt[h], inc);
+      return;
+    }
+    h = (h + 1) & (PVC_N - 1);
+  }
+  atomicAdd(&pv[id], inc);
+}
+
+template <int NT>
+__device__ __forceinline__ void pvc_flush(PvCache &c, unsigned long long *pv) {
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < PVC_N; i += NT)
+    if (c.key[i] != PVC_EMPTY && c.cnt[i]) atomicAdd(&pv[c.key[i]], c.cnt[i]);
+}
+
+// Triangle (u, v, w) closed at root u through oriented edge v -> w, stored
+// at cols[pos].  Its minimum vertex is u (counted by the wave) or else
+// min(v, w), which the edge alone fixes: the edge's counter takes it and
+// k_tc_edge_flush moves every edge counter to its minimum endpoint, one add
+// per edge instead of one contended atomic per triangle.
 template <bool PV>
-__device__ __forceinline__ void tc_hit(uint32_t u, uint32_t v, uint32_t w, unsigned long long &cnt,
-                                       unsigned long long &own, unsigned long long *pv) {
+__device__ __forceinline__ void tc_hit(uint32_t u, uint32_t v, uint32_t w, uint64_t pos,
+                                       unsigned long long &cnt, unsigned long long &own,
+                                       uint32_t *ecnt) {
   ++cnt;
   if (PV) {
-    uint32_t mn = u < v ? u : v;
-    mn = mn < w ? mn : w;
-    if (mn == u) ++own;
-    else atomicAdd(&pv[mn], 1ull);
+    if (u < v && u < w) ++own;
+    else atomicAdd(&ecnt[pos], 1u);
   }
 }
 
@@ -170,7 +220,7 @@ template <bool PV>
 __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uint32_t *cols,
                                                   const uint32_t *order, const uint32_t *wroot,
                                                   const uint32_t *wstart, uint32_t n_work,
-                                                  unsigned long long *pv,
+                                                  unsigned long long *pv, uint32_t *ecnt,
                                                   unsigned long long *total) {
   __shared__ uint32_t s_tab[4][TC_TAB];
   __shared__ uint32_t s_pre[4][WAVE];
@@ -221,6 +271,7 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
       uint32_t k = 0;  // this lane's current block entry (advances monotonically)
       for (uint32_t fb = 0; fb < nwed; fb += 2 * WAVE) {
         uint32_t wv[2] = {TC_EMPTY, TC_EMPTY}, kv[2] = {0, 0};
+        uint64_t pv_pos[2] = {0, 0};
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
           const uint32_t f = fb + r * WAVE + lane;
@@ -228,7 +279,8 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
             while (s_pre[wid][k] <= f) ++k;
             const uint32_t start = k ? s_pre[wid][k - 1] : 0u;
             kv[r] = k;
-            wv[r] = cols[s_off[wid][k] + (f - start)];
+            pv_pos[r] = s_off[wid][k] + (f - start);
+            wv[r] = cols[pv_pos[r]];
           }
         }
 #pragma unroll
@@ -245,7 +297,7 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
             if (e == TC_EMPTY) break;
             h = (h + 1) & hmask;
           }
-          if (found) tc_hit<PV>(u, s_v[wid][kv[r]], wv[r], cnt, own, pv);
+          if (found) tc_hit<PV>(u, s_v[wid][kv[r]], wv[r], pv_pos[r], cnt, own, ecnt);
         }
       }
     }
@@ -256,6 +308,28 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
   }
   cnt = wave_sum(cnt);
   if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
+// Edge counters to per-vertex counts: one wavefront per oriented row v; the
+// edges whose minimum endpoint is v are summed in the wave.
+__global__ __launch_bounds__(256) void k_tc_edge_flush(const uint64_t *off, const uint32_t *cols,
+                                                       const uint32_t *ecnt, uint32_t V,
+                                                       unsigned long long *pv) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; v < V; v += nwaves) {
+    const uint64_t o0 = off[v], o1 = off[v + 1];
+    unsigned long long mine = 0;
+    for (uint64_t p = o0 + lane; p < o1; p += WAVE) {
+      const uint32_t c = ecnt[p];
+      if (!c) continue;
+      const uint32_t w = cols[p];
+      if (v < w) mine += c;
+      else atomicAdd(&pv[w], (unsigned long long)c);
+    }
+    mine = wave_sum(mine);
+    if (lane == 0 && mine) atomicAdd(&pv[v], mine);
+  }
 }
 
 static int tc_device_cus() {
@@ -280,7 +354,7 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
   }
   uint32_t *istart = nullptr, *ivert = nullptr, *nb = nullptr, *wroot = nullptr;
   uint64_t *ipos = nullptr, *off = nullptr, *keys[2] = {nullptr, nullptr}, *tot_dev = nullptr;
-  uint32_t *cols = nullptr;
+  uint32_t *cols = nullptr, *ecnt = nullptr;
   void *tmp = nullptr, *stmp = nullptr;
   gv_status rc = GV_OK;
   auto fail = [&](hipError_t e, const char *what) {
@@ -312,6 +386,10 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
         fail(hipStreamSynchronize(s), "sync"))
       break;
     if (fail(hipMalloc(&cols, (E ? E : 1) * sizeof(uint32_t)), "alloc")) break;
+    if (per_vertex_dev) {
+      if (fail(hipMalloc(&ecnt, (E ? E : 1) * sizeof(uint32_t)), "alloc")) break;
+      fill_u32(ecnt, 0, E, s);
+    }
     k_tc_put<<<grid_for(icap * WAVE, 256, 4096), 256, 0, s>>>(st, istart, ivert, ipos, cols);
     // work items
     k_tc_nblk<<<grid_for(V), 256, 0, s>>>(off, V, nb);
@@ -336,13 +414,16 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
       uint32_t *order = reinterpret_cast<uint32_t *>(sorted == keys[0] ? keys[1] : keys[0]);
       keys_to_cols(sorted, n_work, 32, order, s);
       const unsigned grid = (unsigned)tc_device_cus() * 4;
-      if (per_vertex_dev)
+      if (per_vertex_dev) {
         k_tc_count<true><<<grid, 256, 0, s>>>(off, cols, order, wroot, nb, n_work,
-                                              (unsigned long long *)per_vertex_dev,
+                                              (unsigned long long *)per_vertex_dev, ecnt,
                                               (unsigned long long *)tot_dev);
-      else
+        k_tc_edge_flush<<<grid_for((uint64_t)V * WAVE, 256, 4096), 256, 0, s>>>(
+            off, cols, ecnt, V, (unsigned long long *)per_vertex_dev);
+      } else {
         k_tc_count<false><<<grid, 256, 0, s>>>(off, cols, order, wroot, nb, n_work, nullptr,
-                                               (unsigned long long *)tot_dev);
+                                               nullptr, (unsigned long long *)tot_dev);
+      }
     }
     if (fail(hipMemcpyAsync(total, tot_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, s), "copy") ||
         fail(hipStreamSynchronize(s), "sync"))
@@ -358,10 +439,324 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
   hipFree(keys[0]);
   hipFree(keys[1]);
   hipFree(cols);
+  hipFree(ecnt);
   hipFree(tmp);
   hipFree(stmp);
   hipFree(tot_dev);
   return rc;
+}
+
+// ------------------------------------------------------------ TC delta --
+// Exact per-batch change of the counts above (the reference's dynamic_tc,
+// tc.cu:29-64, recounts only the endpoints of updated edges, which misses
+// triangles whose minimum vertex is the third one).
+//
+// A batch changes exactly the triangles that contain at least one changed
+// edge: a new edge for an insert (counted in the graph after it), a removed
+// edge for a delete (counted in the graph before it).  Each changed edge
+// {x, y} is taken at its owner y, the endpoint with more slots (ties: higher
+// ID), and every w in N(x) is looked up in a hash of N(y).  A triangle with
+// several changed edges is counted only at the smallest of them (canonical
+// key (min << bV) | max), so each one moves t[min(x, y, w)] by exactly one.
+//
+// Owners are the source segments of the staged batch (every changed edge is
+// staged in both directions).  Owners of at most TCD_SMALL slots: one
+// wavefront each, N(y) in a wave-private LDS hash.  Larger owners: items
+// (owner, part of N(y) of TCD_PART slots, chunk of TCD_KCH staged keys), one
+// 1024-thread block each, the part in a 128 KiB LDS hash and the scans of the
+// chunk's N(x) flattened over the block -- a hub's adjacency is hashed once
+// per batch (per part), not once per changed edge.
+constexpr uint32_t TCD_SMALL = 1024;
+constexpr uint32_t TCD_TAB = 2048;     // wave-private hash slots (8 KiB)
+constexpr uint32_t TCD_PART = 16384;   // owner slots per block-level part
+constexpr uint32_t TCD_BTAB = 32768;   // block hash slots (128 KiB)
+constexpr uint32_t TCD_KCH = WAVE;     // staged keys per block-level item
+constexpr int TCD_BLOCK = 1024;
+
+struct TcdArgs {
+  const uint64_t *keys;        // staged sorted keys (u << bV) | v, both directions
+  const uint32_t *flag;        // per key; the first copy of a key with flag == sflag changed
+  const uint32_t *seg_start, *seg_src;
+  uint32_t sflag;
+  uint32_t neg;                // 1: removed triangles (delete)
+  unsigned long long *pv;      // per-vertex counts, updated in place (may be null)
+  unsigned long long *delta;   // change of the total (two's complement)
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool tcd_changed(const TcdArgs &a, uint32_t nv, uint64_t key) {
+  uint32_t lo = 0, hi = nv;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nv && a.keys[lo] == key && a.flag[lo] == a.sflag;
+}
+
+__device__ __forceinline__ uint64_t tcd_canon(uint32_t p, uint32_t q, uint32_t bV) {
+  return p < q ? ((uint64_t)p << bV) | q : ((uint64_t)q << bV) | p;
+}
+
+// Triangle {x, y, w} through changed edge {x, y}: count it here unless a
+// smaller edge of it also changed.
+__device__ __forceinline__ void tcd_hit(const TcdArgs &a, uint32_t nv, uint32_t bV, uint32_t x,
+                                        uint32_t y, uint32_t w, unsigned long long &cnt,
+                                        PvCache &pc) {
+  const uint64_t e0 = tcd_canon(x, y, bV);
+  const uint64_t e1 = tcd_canon(x, w, bV), e2 = tcd_canon(y, w, bV);
+  if (e1 < e0 && tcd_changed(a, nv, e1)) return;
+  if (e2 < e0 && tcd_changed(a, nv, e2)) return;
+  ++cnt;
+  if (a.pv) {
+    uint32_t mn = x < y ? x : y;
+    mn = w < mn ? w : mn;
+    pvc_add(pc, a.pv, mn, a.neg ? ~0ull : 1ull);
+  }
+}
+
+__device__ __forceinline__ bool tcd_lookup(const uint32_t *tab, uint32_t w, uint32_t shift,
+                                           uint32_t hmask) {
+  uint32_t h = tc_slot(w, shift);
+  while (true) {
+    const uint32_t e = tab[h];
+    if (e == w) return true;
+    if (e == TC_EMPTY) return false;
+    h = (h + 1) & hmask;
+  }
+}
+
+// Lane view of staged key kk of owner y (of uy slots): x and its slot count
+// when the key is a changed edge owned by y, else 0.
+__device__ __forceinline__ uint32_t tcd_key(const Store &st, const TcdArgs &a, uint64_t mask,
+                                            uint32_t kk, uint32_t k1, uint32_t y, uint32_t uy,
+                                            uint32_t &x, uint32_t &tx) {
+  x = 0;
+  tx = 0;
+  if (kk >= k1) return 0;
+  const uint64_t key = a.keys[kk];
+  x = (uint32_t)(key & mask);
+  if (a.flag[kk] != a.sflag || (kk > 0 && a.keys[kk - 1] == key)) return 0;
+  const uint32_t ux = st.used[x];
+  if (!(uy > ux || (uy == ux && y > x))) return 0;
+  tx = st.tbl_off[x];
+  return ux;
+}
+
+__global__ __launch_bounds__(256) void k_tcd_small(Store st, TcdArgs a, uint32_t bV, Ctl *ctl) {
+  __shared__ uint32_t s_tab[4][TCD_TAB];
+  __shared__ uint32_t s_pre[4][WAVE], s_x[4][WAVE], s_tx[4][WAVE];
+  __shared__ PvCache s_pc;
+  if (ctl->abort) return;
+  pvc_init<256>(s_pc);
+  const uint32_t T = ctl->n_segs, nv = ctl->n_valid;
+  const uint64_t mask = (1ull << bV) - 1;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  uint32_t *tab = s_tab[wid];
+  const uint32_t W = gridDim.x * 4;
+  unsigned long long cnt = 0;
+  for (uint32_t sidx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wid); sidx < T;
+       sidx += W) {
+    const uint32_t y = a.seg_src[sidx], uy = st.used[y];
+    if (uy > TCD_SMALL) continue;
+    const uint32_t k0 = a.seg_start[sidx], k1 = a.seg_start[sidx + 1];
+    const uint32_t ty = st.tbl_off[y];
+    bool built = false;
+    uint32_t shift = 0, hmask = 0;
+    for (uint32_t kb = k0; kb < k1; kb += WAVE) {
+      uint32_t x, tx;
+      const uint32_t ux = tcd_key(st, a, mask, kb + lane, k1, y, uy, x, tx);
+      const uint32_t pre = wave_incl_scan<uint32_t>(ux);
+      const uint32_t tot = __shfl(pre, WAVE - 1, WAVE);
+      if (tot == 0) continue;
+      if (!built) {
+        // >= 2 uy slots, >= 64 (uy >= 1 here: y has the changed edge)
+        uint32_t lg = uy > 32 ? 32 - __clz(2 * uy - 1) : 6;
+        const uint32_t hsz = 1u << lg;
+        shift = 32 - lg;
+        hmask = hsz - 1;
+        for (uint32_t i = lane; i < hsz; i += WAVE) tab[i] = TC_EMPTY;
+        wave_lds_sync();
+        for (uint32_t i = lane; i < uy; i += WAVE) {
+          const uint32_t v = st.slots[slot_index(st.arena, ty, i, st.log2B, st.B)];
+          if (v == TOMB) continue;
+          uint32_t h = tc_slot(v, shift);
+          while (atomicCAS(&tab[h], TC_EMPTY, v) != TC_EMPTY) h = (h + 1) & hmask;
+        }
+        built = true;
+      }
+      s_pre[wid][lane] = pre;
+      s_x[wid][lane] = x;
+      s_tx[wid][lane] = tx;
+      wave_lds_sync();
+      uint32_t k = 0;  // this lane's key (advances monotonically with f)
+      for (uint32_t f = lane; f < tot; f += WAVE) {
+        while (s_pre[wid][k] <= f) ++k;
+        const uint32_t pos = f - (k ? s_pre[wid][k - 1] : 0u);
+        const uint32_t w = st.slots[slot_index(st.arena, s_tx[wid][k], pos, st.log2B, st.B)];
+        if (w != TOMB && tcd_lookup(tab, w, shift, hmask))
+          tcd_hit(a, nv, bV, s_x[wid][k], y, w, cnt, s_pc);
+      }
+      wave_lds_sync();
+    }
+  }
+  if (a.pv) pvc_flush<256>(s_pc, a.pv);
+  cnt = wave_sum(cnt);
+  if (lane == 0 && cnt) atomicAdd(a.delta, a.neg ? 0ull - cnt : cnt);
+}
+
+// Block-level items per owner: parts(uy) x chunks(keys), 0 for small owners.
+__global__ void k_tcd_items(Store st, TcdArgs a, uint32_t *cnt, Ctl *ctl) {
+  const uint32_t T = ctl->n_segs;
+  for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx < T;
+       sidx += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t uy = st.used[a.seg_src[sidx]];
+    const uint32_t nk = a.seg_start[sidx + 1] - a.seg_start[sidx];
+    cnt[sidx] = (ctl->abort || uy <= TCD_SMALL)
+                    ? 0u
+                    : ((uy + TCD_PART - 1) / TCD_PART) * ((nk + TCD_KCH - 1) / TCD_KCH);
+  }
+}
+
+__global__ void k_tcd_expand(const uint32_t *start, uint32_t *iseg, Ctl *ctl) {
+  const uint32_t T = ctl->n_segs;
+  for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx < T;
+       sidx += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = start[sidx]; i < start[sidx + 1]; ++i) iseg[i] = (uint32_t)sidx;
+}
+
+__global__ void k_tcd_total(const uint32_t *start, uint32_t *n_items, Ctl *ctl) {
+  *n_items = start[ctl->n_segs];
+}
+
+__global__ __launch_bounds__(TCD_BLOCK) void k_tcd_big(Store st, TcdArgs a, const uint32_t *start,
+                                                      const uint32_t *iseg, uint32_t n_items,
+                                                      uint32_t bV, Ctl *ctl) {
+  __shared__ uint32_t s_tab[TCD_BTAB];
+  __shared__ uint32_t s_pre[WAVE], s_x[WAVE], s_tx[WAVE];
+  __shared__ PvCache s_pc;
+  if (ctl->abort) return;
+  pvc_init<TCD_BLOCK>(s_pc);
+  const uint32_t nv = ctl->n_valid;
+  const uint64_t mask = (1ull << bV) - 1;
+  const uint32_t lane = lane_id();
+  unsigned long long cnt = 0;
+  for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const uint32_t sidx = iseg[it], li = it - start[sidx];
+    const uint32_t y = a.seg_src[sidx], uy = st.used[y];
+    const uint32_t k0 = a.seg_start[sidx], k1 = a.seg_start[sidx + 1];
+    const uint32_t nch = (k1 - k0 + TCD_KCH - 1) / TCD_KCH;
+    const uint32_t p = li / nch, c = li - p * nch;
+    if (threadIdx.x < WAVE) {
+      const uint32_t kb = k0 + c * TCD_KCH;
+      const uint32_t ke = kb + TCD_KCH < k1 ? kb + TCD_KCH : k1;
+      uint32_t x, tx;
+      const uint32_t ux = tcd_key(st, a, mask, kb + lane, ke, y, uy, x, tx);
+      s_pre[lane] = wave_incl_scan<uint32_t>(ux);
+      s_x[lane] = x;
+      s_tx[lane] = tx;
+    }
+    __syncthreads();
+    const uint32_t tot = s_pre[WAVE - 1];
+    if (tot) {
+      const uint32_t p0 = p * TCD_PART;
+      const uint32_t pn = uy - p0 < TCD_PART ? uy - p0 : TCD_PART;
+      const uint32_t lg = pn > 32 ? 32 - __clz(2 * pn - 1) : 6;
+      const uint32_t hsz = 1u << lg, shift = 32 - lg, hmask = hsz - 1;
+      for (uint32_t i = threadIdx.x; i < hsz; i += TCD_BLOCK) s_tab[i] = TC_EMPTY;
+      __syncthreads();
+      const uint32_t ty = st.tbl_off[y];
+      for (uint32_t i = threadIdx.x; i < pn; i += TCD_BLOCK) {
+        const uint32_t v = st.slots[slot_index(st.arena, ty, p0 + i, st.log2B, st.B)];
+        if (v == TOMB) continue;
+        uint32_t h = tc_slot(v, shift);
+        while (atomicCAS(&s_tab[h], TC_EMPTY, v) != TC_EMPTY) h = (h + 1) & hmask;
+      }
+      __syncthreads();
+      for (uint32_t f = threadIdx.x; f < tot; f += TCD_BLOCK) {
+        uint32_t lo = 0, hi = WAVE - 1;  // first k with s_pre[k] > f
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_pre[mid] <= f) lo = mid + 1;
+          else hi = mid;
+        }
+        const uint32_t pos = f - (lo ? s_pre[lo - 1] : 0u);
+        const uint32_t w = st.slots[slot_index(st.arena, s_tx[lo], pos, st.log2B, st.B)];
+        if (w != TOMB && tcd_lookup(s_tab, w, shift, hmask))
+          tcd_hit(a, nv, bV, s_x[lo], y, w, cnt, s_pc);
+      }
+    }
+    __syncthreads();
+  }
+  if (a.pv) pvc_flush<TCD_BLOCK>(s_pc, a.pv);
+  cnt = wave_sum(cnt);
+  if (lane == 0 && cnt) atomicAdd(a.delta, a.neg ? 0ull - cnt : cnt);
+}
+
+void tcd_release(TcdWork &w) {
+  hipFree(w.cnt);
+  hipFree(w.start);
+  hipFree(w.iseg);
+  hipFree(w.tmp);
+  hipFree(w.dev);
+  w = TcdWork{};
+}
+
+static gv_status tcd_grow(void **p, size_t &cap, size_t need, const char *what) {
+  if (need <= cap) return GV_OK;
+  hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(p, need);
+  if (e != hipSuccess) return hip_fail(e, what);
+  cap = need;
+  return GV_OK;
+}
+
+gv_status tc_delta(const Store &st, const Scratch &sc, const uint64_t *keys, uint64_t m,
+                   uint32_t sflag, int neg, uint64_t *per_vertex_dev, TcdWork &w,
+                   long long *delta, hipStream_t s) {
+  *delta = 0;
+  if (!m) return GV_OK;
+  gv_status rc = GV_OK;
+  if ((rc = tcd_grow((void **)&w.cnt, w.cnt_bytes, (m + 1) * sizeof(uint32_t), "tc delta alloc")) ||
+      (rc = tcd_grow((void **)&w.start, w.start_bytes, (m + 1) * sizeof(uint32_t),
+                     "tc delta alloc")) ||
+      (rc = tcd_grow(&w.tmp, w.tmp_bytes, scan_tmp_bytes(m + 1), "tc delta alloc")) ||
+      (rc = tcd_grow((void **)&w.dev, w.dev_bytes, 2 * sizeof(unsigned long long),
+                     "tc delta alloc")))
+    return rc;
+  const uint32_t bV = id_bits(st.V);
+  TcdArgs a{keys, sc.flag, sc.seg_start, sc.seg_src, sflag, neg ? 1u : 0u,
+            (unsigned long long *)per_vertex_dev, w.dev};
+  fill_u64((uint64_t *)w.dev, 0, 2, s);
+  k_tcd_items<<<grid_for(m), 256, 0, s>>>(st, a, w.cnt, st.ctl);
+  scan_u32(w.cnt, w.start, m, &st.ctl->n_segs, w.tmp, s);
+  k_tcd_total<<<1, 1, 0, s>>>(w.start, (uint32_t *)(w.dev + 1), st.ctl);
+  k_tcd_small<<<grid_for(cdiv(m, 4) * 256, 256, 2048), 256, 0, s>>>(st, a, bV, st.ctl);
+  uint32_t n_items = 0;
+  GV_HIP(hipMemcpyAsync(&n_items, w.dev + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  if (n_items) {
+    if ((rc = tcd_grow((void **)&w.iseg, w.iseg_bytes, (size_t)n_items * sizeof(uint32_t),
+                       "tc delta alloc")))
+      return rc;
+    k_tcd_expand<<<grid_for(m), 256, 0, s>>>(w.start, w.iseg, st.ctl);
+    unsigned grid = (unsigned)tc_device_cus();
+    if (grid > n_items) grid = n_items;
+    k_tcd_big<<<grid, TCD_BLOCK, 0, s>>>(st, a, w.start, w.iseg, n_items, bV, st.ctl);
+  }
+  unsigned long long d = 0;
+  GV_HIP(hipGetLastError());
+  GV_HIP(hipMemcpyAsync(&d, w.dev, sizeof(d), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  *delta = (long long)d;
+  return GV_OK;
 }
 
 }  // namespace gv

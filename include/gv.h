@@ -20,6 +20,7 @@
  *   GraphVine::getVertexSize()        include/dynamicStructure/graphvine.cuh:56
  *   static_pagerank(...)              include/algorithm/pagerank/pagerank.cuh:11
  *   static_tc(...)                    include/algorithm/tc/tc.cuh:10
+ *   dynamic_tc(...)                   src/algorithm/tc/tc.cu:29-64
  *   bc(GraphVine*, int)               include/algorithm/bc/bc.cuh:8
  *   EdgePreallocatedQueue (pool)      include/dynamicStructure/graphStructures.cuh:103-108
  */
@@ -171,6 +172,21 @@ gv_status gv_pagerank_personalized(gv_graph *g, float alpha, double eps, int max
 /* static_tc on an undirected graph: exact uint64 counts; per_vertex (may be
  * NULL) gets t[u] = triangles whose minimum vertex ID is u. */
 gv_status gv_triangle_count(gv_graph *g, uint64_t *per_vertex, int mem, uint64_t *total);
+/* Apply one batch of kind GV_OP_INSERT or GV_OP_DELETE to an undirected
+ * graph, exactly as gv_batch_insert / gv_batch_delete do, and bring triangle
+ * counts of the graph before it up to date: per_vertex (V entries in pv_mem,
+ * may be NULL) and *total are read and rewritten so that they equal what
+ * gv_triangle_count returns after the batch.  Only the triangles through
+ * edges the batch really adds or removes are visited (duplicates, self-loops,
+ * inserts of present and deletes of absent edges change nothing).  On an
+ * error (a bad ID, an exhausted pool) the batch is not applied and the
+ * counts are left as they were.  Replaces dynamic_tc (tc.cu:29-64, with the
+ * affected set of find_affected_nodes, insert_kernels.cu:182-198), whose
+ * recount of the updated edges' endpoints misses triangles whose minimum
+ * vertex is the third vertex. */
+gv_status gv_triangle_count_update(gv_graph *g, int kind, const uint32_t *src, const uint32_t *dst,
+                                   uint64_t n, int mem, uint64_t *per_vertex, int pv_mem,
+                                   uint64_t *total, gv_batch_stats *stats);
 /* bc: Brandes betweenness from the given sources (unnormalised, not halved);
  * sigma in FP64, dependencies in FP32 with compensated sums.  Several sources
  * run 32 at a time in one level-synchronous sweep. */

@@ -3,6 +3,7 @@
 //   class GraphVine            include/dynamicStructure/graphvine.cuh:29-105
 //   static_pagerank(...)       include/algorithm/pagerank/pagerank.cuh:11
 //   static_tc(...)             include/algorithm/tc/tc.cuh:10
+//   dynamic_tc(...)            src/algorithm/tc/tc.cu:29-64
 //   bc(GraphVine*, ...)        include/algorithm/bc/bc.cuh:8
 // Differences: vertex IDs are 0-based, batches are COO host vectors, results
 // are returned in host vectors, and failures throw gv::Error (a
@@ -93,6 +94,20 @@ inline uint64_t static_tc(GraphVine *g, std::vector<uint64_t> &per_vertex) {
   uint64_t total = 0;
   check(gv_triangle_count(g->handle(), per_vertex.data(), GV_MEM_HOST, &total));
   return total;
+}
+
+// dynamic_tc (tc.cu:29-64): applies one insert or delete batch and updates
+// counts taken by static_tc before it (per_vertex and total) to the counts
+// after it.  The batch is explicit here; the reference reads the affected
+// set its last batchInsert left behind.
+inline void dynamic_tc(GraphVine *g, bool insert, const std::vector<uint32_t> &src,
+                       const std::vector<uint32_t> &dst, std::vector<uint64_t> &per_vertex,
+                       uint64_t &total) {
+  if (src.size() != dst.size()) throw Error(GV_E_INVALID, "src and dst sizes differ");
+  per_vertex.resize(g->getVertexSize());
+  check(gv_triangle_count_update(g->handle(), insert ? GV_OP_INSERT : GV_OP_DELETE, src.data(),
+                                 dst.data(), src.size(), GV_MEM_HOST, per_vertex.data(),
+                                 GV_MEM_HOST, &total, nullptr));
 }
 
 // bc: Brandes from the given sources (the reference ignores its source
