@@ -171,9 +171,11 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
 // Grow-only workspace of tc_delta.
 struct TcdWork {
   uint32_t *cnt = nullptr, *start = nullptr, *iseg = nullptr;
+  uint64_t *hset = nullptr;           // changed-edge set
   void *tmp = nullptr;
   unsigned long long *dev = nullptr;  // [0] total change, [1] block-level items
-  size_t cnt_bytes = 0, start_bytes = 0, iseg_bytes = 0, tmp_bytes = 0, dev_bytes = 0;
+  size_t cnt_bytes = 0, start_bytes = 0, iseg_bytes = 0, hset_bytes = 0, tmp_bytes = 0,
+         dev_bytes = 0;
 };
 void tcd_release(TcdWork &w);
 // Change of the triangle counts made by the staged batch in sc (keys: its m

@@ -472,11 +472,15 @@ constexpr uint32_t TCD_PART = 16384;   // owner slots per block-level part
 constexpr uint32_t TCD_BTAB = 32768;   // block hash slots (128 KiB)
 constexpr uint32_t TCD_KCH = WAVE;     // staged keys per block-level item
 constexpr int TCD_BLOCK = 1024;
+constexpr int TCD_UNROLL = 4;          // block-level slot loads per thread in flight
 
 struct TcdArgs {
   const uint64_t *keys;        // staged sorted keys (u << bV) | v, both directions
   const uint32_t *flag;        // per key; the first copy of a key with flag == sflag changed
   const uint32_t *seg_start, *seg_src;
+  const uint64_t *hset;        // open-addressing set of the changed canonical keys
+  uint64_t hmask;              // its size - 1 (a power of two)
+  uint32_t hshift;             // 64 - log2(size)
   uint32_t sflag;
   uint32_t neg;                // 1: removed triangles (delete)
   unsigned long long *pv;      // per-vertex counts, updated in place (may be null)
@@ -489,35 +493,56 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ bool tcd_changed(const TcdArgs &a, uint32_t nv, uint64_t key) {
-  uint32_t lo = 0, hi = nv;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (a.keys[mid] < key) lo = mid + 1;
-    else hi = mid;
+__device__ __forceinline__ uint64_t tcd_hash(const TcdArgs &a, uint64_t key) {
+  return (key * 0x9E3779B97F4A7C15ull) >> a.hshift;  // multiplicative, top bits
+}
+
+// Is canonical key `key` a changed edge?  e is the set's entry at
+// tcd_hash(key), already loaded.
+__device__ __forceinline__ bool tcd_changed(const TcdArgs &a, uint64_t key, uint64_t h, uint64_t e) {
+  while (true) {
+    if (e == key) return true;
+    if (e == KEY_SENT) return false;
+    h = (h + 1) & a.hmask;
+    e = a.hset[h];
   }
-  return lo < nv && a.keys[lo] == key && a.flag[lo] == a.sflag;
 }
 
 __device__ __forceinline__ uint64_t tcd_canon(uint32_t p, uint32_t q, uint32_t bV) {
   return p < q ? ((uint64_t)p << bV) | q : ((uint64_t)q << bV) | p;
 }
 
-// Triangle {x, y, w} through changed edge {x, y}: count it here unless a
-// smaller edge of it also changed.
-__device__ __forceinline__ void tcd_hit(const TcdArgs &a, uint32_t nv, uint32_t bV, uint32_t x,
+// Triangle {x, y, w} through changed edge {x, y} owned by y: count it here
+// unless a smaller edge of it also changed (a probe of the changed-edge set
+// for each smaller edge).  Triangles whose minimum is the
+// owner are counted in own, added once per owner by the caller.
+__device__ __forceinline__ void tcd_hit(const TcdArgs &a, uint32_t bV, uint32_t x,
                                         uint32_t y, uint32_t w, unsigned long long &cnt,
-                                        PvCache &pc) {
+                                        unsigned long long &own, PvCache &pc) {
   const uint64_t e0 = tcd_canon(x, y, bV);
   const uint64_t e1 = tcd_canon(x, w, bV), e2 = tcd_canon(y, w, bV);
-  if (e1 < e0 && tcd_changed(a, nv, e1)) return;
-  if (e2 < e0 && tcd_changed(a, nv, e2)) return;
+  const bool c1 = e1 < e0, c2 = e2 < e0;
+  if (c1 || c2) {
+    // both probes issued before either is used
+    const uint64_t h1 = tcd_hash(a, e1), h2 = tcd_hash(a, e2);
+    const uint64_t s1 = c1 ? a.hset[h1] : KEY_SENT, s2 = c2 ? a.hset[h2] : KEY_SENT;
+    if (c1 && tcd_changed(a, e1, h1, s1)) return;
+    if (c2 && tcd_changed(a, e2, h2, s2)) return;
+  }
   ++cnt;
   if (a.pv) {
-    uint32_t mn = x < y ? x : y;
-    mn = w < mn ? w : mn;
-    pvc_add(pc, a.pv, mn, a.neg ? ~0ull : 1ull);
+    if (y < x && y < w) ++own;
+    else pvc_add(pc, a.pv, x < w ? x : w, a.neg ? ~0ull : 1ull);
   }
+}
+
+// Add a wavefront's owner-minimum triangles of owner y.
+__device__ __forceinline__ void tcd_own_flush(const TcdArgs &a, PvCache &pc, uint32_t y,
+                                              unsigned long long &own) {
+  if (!a.pv) return;
+  const unsigned long long t = wave_sum(own);
+  own = 0;
+  if (lane_id() == 0 && t) pvc_add(pc, a.pv, y, a.neg ? 0ull - t : t);
 }
 
 __device__ __forceinline__ bool tcd_lookup(const uint32_t *tab, uint32_t w, uint32_t shift,
@@ -554,12 +579,12 @@ __global__ __launch_bounds__(256) void k_tcd_small(Store st, TcdArgs a, uint32_t
   __shared__ PvCache s_pc;
   if (ctl->abort) return;
   pvc_init<256>(s_pc);
-  const uint32_t T = ctl->n_segs, nv = ctl->n_valid;
+  const uint32_t T = ctl->n_segs;
   const uint64_t mask = (1ull << bV) - 1;
   const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
   uint32_t *tab = s_tab[wid];
   const uint32_t W = gridDim.x * 4;
-  unsigned long long cnt = 0;
+  unsigned long long cnt = 0, own = 0;
   for (uint32_t sidx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wid); sidx < T;
        sidx += W) {
     const uint32_t y = a.seg_src[sidx], uy = st.used[y];
@@ -595,15 +620,26 @@ __global__ __launch_bounds__(256) void k_tcd_small(Store st, TcdArgs a, uint32_t
       s_tx[wid][lane] = tx;
       wave_lds_sync();
       uint32_t k = 0;  // this lane's key (advances monotonically with f)
-      for (uint32_t f = lane; f < tot; f += WAVE) {
-        while (s_pre[wid][k] <= f) ++k;
-        const uint32_t pos = f - (k ? s_pre[wid][k - 1] : 0u);
-        const uint32_t w = st.slots[slot_index(st.arena, s_tx[wid][k], pos, st.log2B, st.B)];
-        if (w != TOMB && tcd_lookup(tab, w, shift, hmask))
-          tcd_hit(a, nv, bV, s_x[wid][k], y, w, cnt, s_pc);
+      for (uint32_t fb = 0; fb < tot; fb += 2 * WAVE) {
+        uint32_t w[2] = {TOMB, TOMB}, kx[2] = {0, 0};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const uint32_t f = fb + r * WAVE + lane;
+          if (f < tot) {
+            while (s_pre[wid][k] <= f) ++k;
+            const uint32_t pos = f - (k ? s_pre[wid][k - 1] : 0u);
+            kx[r] = k;
+            w[r] = st.slots[slot_index(st.arena, s_tx[wid][k], pos, st.log2B, st.B)];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+          if (w[r] != TOMB && tcd_lookup(tab, w[r], shift, hmask))
+            tcd_hit(a, bV, s_x[wid][kx[r]], y, w[r], cnt, own, s_pc);
       }
       wave_lds_sync();
     }
+    tcd_own_flush(a, s_pc, y, own);
   }
   if (a.pv) pvc_flush<256>(s_pc, a.pv);
   cnt = wave_sum(cnt);
@@ -630,6 +666,25 @@ __global__ void k_tcd_expand(const uint32_t *start, uint32_t *iseg, Ctl *ctl) {
     for (uint32_t i = start[sidx]; i < start[sidx + 1]; ++i) iseg[i] = (uint32_t)sidx;
 }
 
+// Insert the changed edges (the first copy of each u < v key) into the set.
+__global__ void k_tcd_hset(TcdArgs a, uint64_t *hset, uint32_t bV, Ctl *ctl) {
+  if (ctl->abort) return;
+  const uint32_t nv = ctl->n_valid;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = a.keys[i];
+    if ((key >> bV) >= (key & ((1ull << bV) - 1))) continue;
+    if (a.flag[i] != a.sflag || (i > 0 && a.keys[i - 1] == key)) continue;
+    uint64_t h = tcd_hash(a, key);
+    while (true) {
+      const uint64_t prev = atomicCAS((unsigned long long *)&hset[h], (unsigned long long)KEY_SENT,
+                                      (unsigned long long)key);
+      if (prev == KEY_SENT || prev == key) break;
+      h = (h + 1) & a.hmask;
+    }
+  }
+}
+
 __global__ void k_tcd_total(const uint32_t *start, uint32_t *n_items, Ctl *ctl) {
   *n_items = start[ctl->n_segs];
 }
@@ -642,10 +697,9 @@ __global__ __launch_bounds__(TCD_BLOCK) void k_tcd_big(Store st, TcdArgs a, cons
   __shared__ PvCache s_pc;
   if (ctl->abort) return;
   pvc_init<TCD_BLOCK>(s_pc);
-  const uint32_t nv = ctl->n_valid;
   const uint64_t mask = (1ull << bV) - 1;
   const uint32_t lane = lane_id();
-  unsigned long long cnt = 0;
+  unsigned long long cnt = 0, own = 0;
   for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
     const uint32_t sidx = iseg[it], li = it - start[sidx];
     const uint32_t y = a.seg_src[sidx], uy = st.used[y];
@@ -671,25 +725,47 @@ __global__ __launch_bounds__(TCD_BLOCK) void k_tcd_big(Store st, TcdArgs a, cons
       for (uint32_t i = threadIdx.x; i < hsz; i += TCD_BLOCK) s_tab[i] = TC_EMPTY;
       __syncthreads();
       const uint32_t ty = st.tbl_off[y];
-      for (uint32_t i = threadIdx.x; i < pn; i += TCD_BLOCK) {
-        const uint32_t v = st.slots[slot_index(st.arena, ty, p0 + i, st.log2B, st.B)];
-        if (v == TOMB) continue;
-        uint32_t h = tc_slot(v, shift);
-        while (atomicCAS(&s_tab[h], TC_EMPTY, v) != TC_EMPTY) h = (h + 1) & hmask;
+      // TCD_UNROLL independent slot loads per thread in flight, then inserts
+      for (uint32_t ib = 0; ib < pn; ib += TCD_UNROLL * TCD_BLOCK) {
+        uint32_t v[TCD_UNROLL];
+#pragma unroll
+        for (int r = 0; r < TCD_UNROLL; ++r) {
+          const uint32_t i = ib + r * TCD_BLOCK + threadIdx.x;
+          v[r] = i < pn ? st.slots[slot_index(st.arena, ty, p0 + i, st.log2B, st.B)] : TOMB;
+        }
+#pragma unroll
+        for (int r = 0; r < TCD_UNROLL; ++r) {
+          if (v[r] == TOMB) continue;
+          uint32_t h = tc_slot(v[r], shift);
+          while (atomicCAS(&s_tab[h], TC_EMPTY, v[r]) != TC_EMPTY) h = (h + 1) & hmask;
+        }
       }
       __syncthreads();
-      for (uint32_t f = threadIdx.x; f < tot; f += TCD_BLOCK) {
-        uint32_t lo = 0, hi = WAVE - 1;  // first k with s_pre[k] > f
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_pre[mid] <= f) lo = mid + 1;
-          else hi = mid;
+      for (uint32_t fb = 0; fb < tot; fb += TCD_UNROLL * TCD_BLOCK) {
+        uint32_t w[TCD_UNROLL], kx[TCD_UNROLL];
+#pragma unroll
+        for (int r = 0; r < TCD_UNROLL; ++r) {
+          const uint32_t f = fb + r * TCD_BLOCK + threadIdx.x;
+          w[r] = TOMB;
+          kx[r] = 0;
+          if (f < tot) {
+            uint32_t lo = 0, hi = WAVE - 1;  // first k with s_pre[k] > f
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (s_pre[mid] <= f) lo = mid + 1;
+              else hi = mid;
+            }
+            const uint32_t pos = f - (lo ? s_pre[lo - 1] : 0u);
+            kx[r] = lo;
+            w[r] = st.slots[slot_index(st.arena, s_tx[lo], pos, st.log2B, st.B)];
+          }
         }
-        const uint32_t pos = f - (lo ? s_pre[lo - 1] : 0u);
-        const uint32_t w = st.slots[slot_index(st.arena, s_tx[lo], pos, st.log2B, st.B)];
-        if (w != TOMB && tcd_lookup(s_tab, w, shift, hmask))
-          tcd_hit(a, nv, bV, s_x[lo], y, w, cnt, s_pc);
+#pragma unroll
+        for (int r = 0; r < TCD_UNROLL; ++r)
+          if (w[r] != TOMB && tcd_lookup(s_tab, w[r], shift, hmask))
+            tcd_hit(a, bV, s_x[kx[r]], y, w[r], cnt, own, s_pc);
       }
+      tcd_own_flush(a, s_pc, y, own);
     }
     __syncthreads();
   }
@@ -702,6 +778,7 @@ void tcd_release(TcdWork &w) {
   hipFree(w.cnt);
   hipFree(w.start);
   hipFree(w.iseg);
+  hipFree(w.hset);
   hipFree(w.tmp);
   hipFree(w.dev);
   w = TcdWork{};
@@ -729,12 +806,19 @@ gv_status tc_delta(const Store &st, const Scratch &sc, const uint64_t *keys, uin
                      "tc delta alloc")) ||
       (rc = tcd_grow(&w.tmp, w.tmp_bytes, scan_tmp_bytes(m + 1), "tc delta alloc")) ||
       (rc = tcd_grow((void **)&w.dev, w.dev_bytes, 2 * sizeof(unsigned long long),
+                     "tc delta alloc")) ||
+      (rc = tcd_grow((void **)&w.hset, w.hset_bytes,
+                     ((size_t)1 << ceil_log2(m < 64 ? 64 : m)) * sizeof(uint64_t),
                      "tc delta alloc")))
     return rc;
   const uint32_t bV = id_bits(st.V);
-  TcdArgs a{keys, sc.flag, sc.seg_start, sc.seg_src, sflag, neg ? 1u : 0u,
-            (unsigned long long *)per_vertex_dev, w.dev};
+  // at most m/2 changed canonical edges in a set of >= m slots: load <= 1/2
+  const uint32_t lgh = ceil_log2(m < 64 ? 64 : m);
+  TcdArgs a{keys, sc.flag, sc.seg_start, sc.seg_src, w.hset, ((uint64_t)1 << lgh) - 1,
+            64 - lgh, sflag, neg ? 1u : 0u, (unsigned long long *)per_vertex_dev, w.dev};
   fill_u64((uint64_t *)w.dev, 0, 2, s);
+  fill_u64(w.hset, KEY_SENT, (uint64_t)1 << lgh, s);
+  k_tcd_hset<<<grid_for(m), 256, 0, s>>>(a, w.hset, bV, st.ctl);
   k_tcd_items<<<grid_for(m), 256, 0, s>>>(st, a, w.cnt, st.ctl);
   scan_u32(w.cnt, w.start, m, &st.ctl->n_segs, w.tmp, s);
   k_tcd_total<<<1, 1, 0, s>>>(w.start, (uint32_t *)(w.dev + 1), st.ctl);
