@@ -171,7 +171,9 @@ __device__ __forceinline__ T block_excl_scan(T v, T &total) {
 }
 
 // Lower bound of x in the ascending values held one per lane (sv, padded with
-// 0xFFFFFFFF past the segment): six uniform ds_bpermute steps, no divergence.
+// 0xFFFFFFFF past the segment): the number of lanes whose sv is < x, capped
+// at 63 (all 64 below x also gives 63).  Six uniform ds_bpermute steps, no
+// divergence.
 __device__ __forceinline__ uint32_t wave_lower_bound(uint32_t sv, uint32_t x) {
   uint32_t pos = 0;
 #pragma unroll

@@ -429,23 +429,47 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
     while (big) {
       const int src = __ffsll(big) - 1;
       big &= big - 1;
-      const uint32_t bsidx = __shfl(sidx, src, WAVE), blo = __shfl(lo, src, WAVE);
-      const uint32_t bhi = __shfl(hi, src, WAVE), bu = __shfl(u, src, WAVE);
-      const uint32_t btbl = __shfl(tbl, src, WAVE);
-      const uint32_t bks = __shfl(ks, src, WAVE), bke = __shfl(ke, src, WAVE);
+      // the item's metadata is wave-uniform: scalar registers
+      const uint32_t bsidx = __builtin_amdgcn_readfirstlane(__shfl(sidx, src, WAVE));
+      const uint32_t blo = __builtin_amdgcn_readfirstlane(__shfl(lo, src, WAVE));
+      const uint32_t bhi = __builtin_amdgcn_readfirstlane(__shfl(hi, src, WAVE));
+      const uint32_t bu = __builtin_amdgcn_readfirstlane(__shfl(u, src, WAVE));
+      const uint32_t btbl = __builtin_amdgcn_readfirstlane(__shfl(tbl, src, WAVE));
+      const uint32_t bks = __builtin_amdgcn_readfirstlane(__shfl(ks, src, WAVE));
+      const uint32_t bke = __builtin_amdgcn_readfirstlane(__shfl(ke, src, WAVE));
       const uint32_t k = bke - bks;
+      // batch keys first, independent of the slot chain: the whole segment
+      // (one key per lane) when it has <= 64 keys, else 64 fences, lane l
+      // holding key bks + l*k/64
+      uint32_t sv = 0xFFFFFFFFu;
+      if (k <= (uint32_t)WAVE) {
+        if (lane < k) sv = (uint32_t)(a.keys[bks + lane] & mask);
+      } else {
+        sv = (uint32_t)(a.keys[bks + (uint32_t)(((uint64_t)lane * k) >> 6)] & mask);
+      }
       uint32_t idx[PROBE_ROUNDS], wv[PROBE_ROUNDS];
+      if (st.log2B >= 6) {
+        // blo is a multiple of PROBE_CHUNK, so each 64-slot round lies in one
+        // block: one uniform block-table load per round
 #pragma unroll
-      for (int q = 0; q < PROBE_ROUNDS; ++q) {
-        const uint32_t pos = blo + q * WAVE + lane;
-        idx[q] = pos < bhi ? slot_index(st.arena, btbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
+        for (int q = 0; q < PROBE_ROUNDS; ++q) {
+          const uint32_t base = blo + q * WAVE;
+          const uint32_t blk = base < bhi ? st.arena[btbl + (base >> st.log2B)] : 0u;
+          const uint32_t pos = base + lane;
+          idx[q] = pos < bhi ? blk * st.B + (pos & (st.B - 1)) : 0xFFFFFFFFu;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < PROBE_ROUNDS; ++q) {
+          const uint32_t pos = blo + q * WAVE + lane;
+          idx[q] = pos < bhi ? slot_index(st.arena, btbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
+        }
       }
 #pragma unroll
       for (int q = 0; q < PROBE_ROUNDS; ++q)
         wv[q] = idx[q] != 0xFFFFFFFFu ? st.slots[idx[q]] : TOMB;
       uint32_t cnt = 0;
       if (k <= (uint32_t)WAVE) {
-        const uint32_t sv = lane < k ? (uint32_t)(a.keys[bks + lane] & mask) : 0xFFFFFFFFu;
 #pragma unroll
         for (int q = 0; q < PROBE_ROUNDS; ++q) {
           const uint32_t p = wave_lower_bound(sv, wv[q]);
@@ -453,12 +477,18 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
           if (wv[q] != TOMB && hit == wv[q]) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
         }
       } else {
+        // the fences bracket the lower bound: keys[f(p-1)] < x <= keys[f(p)]
+        // with f(l) = bks + l*k/64, so memory is searched over ~k/64 keys
         const uint64_t ukey = (uint64_t)bu << bV;
+        const uint32_t last = __shfl(sv, WAVE - 1, WAVE);
 #pragma unroll
         for (int q = 0; q < PROBE_ROUNDS; ++q) {
+          uint32_t p = wave_lower_bound(sv, wv[q]);  // at most 63: all 64 fences < x gives 64
+          if (p == WAVE - 1 && last < wv[q]) p = WAVE;
           if (wv[q] == TOMB) continue;
           const uint64_t key = ukey | wv[q];
-          uint32_t l2 = bks, h2 = bke;
+          uint32_t l2 = p ? bks + (uint32_t)(((uint64_t)(p - 1) * k) >> 6) + 1 : bks;
+          uint32_t h2 = p < (uint32_t)WAVE ? bks + (uint32_t)(((uint64_t)p * k) >> 6) : bke;
           while (l2 < h2) {
             const uint32_t mid = (l2 + h2) >> 1;
             if (a.keys[mid] < key) l2 = mid + 1; else h2 = mid;

@@ -179,6 +179,32 @@ def test_hub_beyond_reference_degree_limit():
     g.close()
 
 
+@pytest.mark.parametrize("block_slots", [32, 64, 128])
+def test_hub_batch_segments_of_many_keys(block_slots):
+    # one source with 65..4000 keys in a batch: its slots are searched through
+    # 64 fence keys, including keys past the last fence and absent keys
+    V = 20000
+    rng = np.random.default_rng(block_slots)
+    nb = rng.choice(np.arange(1, V, dtype=np.uint32), 12000, replace=False)
+    g = make(V, True, pool=8 * V, block_slots=block_slots)
+    o = O.OracleGraph(V, True)
+    g.bulkBuild(np.zeros(len(nb), np.uint32), nb)
+    o.bulk_build(np.zeros(len(nb), np.uint32), nb)
+    live = np.sort(nb)
+    for k in (65, 127, 1000, 4000):
+        pick = rng.choice(live, k, replace=False)
+        pick = np.concatenate([pick, live[-3:], rng.integers(1, V, 50).astype(np.uint32)])
+        src = np.zeros(len(pick), np.uint32)
+        assert g.batchDelete(src, pick)["applied"] == o.delete(src, pick)
+        assert_same(g, o, f"delete of {k} keys")
+        live = o.export_csr()[1]
+        back = rng.choice(pick, len(pick) // 2, replace=False)
+        src = np.zeros(len(back), np.uint32)
+        assert g.batchInsert(src, back)["applied"] == o.insert(src, back)
+        assert_same(g, o, f"re-insert after {k}")
+        live = o.export_csr()[1]
+
+
 def test_neighbors_query_matches_oracle():
     V = 4000
     g = make(V, False, pool=8 * V)
