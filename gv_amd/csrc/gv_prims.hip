@@ -3,9 +3,14 @@
 // graphvine.cu:197,291,329 and its host COO->CSR placement at csr.cu:55-78).
 //
 // Scan: reduce-then-scan over 2048-element tiles (256 threads x 8).
-// Sort: 8-bit LSD passes over 4096-key tiles; per pass one tile-histogram
-// kernel, one scan of the digit-major histogram, one stable scatter that
-// ranks keys with 64-lane ballot matching.
+// Sort: 8-bit LSD passes over 4096-key tiles, stable scatter that ranks keys
+// with 64-lane ballot matching.  Up to 2^24 keys the sort is single-sweep
+// (Merrill and Adinets, "Onesweep", 2022): one kernel counts every digit of
+// every pass (global counts do not depend on key order), then each pass is
+// one scatter kernel whose tiles find their offsets by decoupled look-back
+// over the tiles before them -- 2 + passes launches instead of 4 per pass.
+// Larger sorts take one tile-histogram kernel, one scan of the digit-major
+// histogram and one scatter per pass.
 #include "gv_internal.h"
 
 namespace gv {
@@ -260,10 +265,179 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
   }
 }
 
+// ------------------------------------------------- single-sweep sort --
+// Tile status word per (pass, tile, digit): bits 31..30 = OS_AGG (the tile's
+// own count is in bits 29..0) or OS_PREFIX (the count of the digit in all
+// tiles up to and including this one); 0 = not yet published.
+constexpr uint32_t OS_AGG = 1u << 30;
+constexpr uint32_t OS_PREFIX = 2u << 30;
+constexpr uint32_t OS_VAL = OS_AGG - 1;
+// Counts must fit 30 bits; past 2^24 keys launches no longer dominate and
+// the classic path's workspace (1/8 of this one's) is used.
+constexpr uint64_t OS_MAX_KEYS = 1ull << 24;
+constexpr int OS_MAX_PASSES = 8;
+constexpr uint32_t OS_SPIN_LIMIT = 1u << 26;  // a bug shows as wrong output, not a hang
+constexpr int OS_LOOK = 8;                    // predecessors read per look-back round trip
+
+// Single-sweep workspace (32-bit words): ghist[8][256], tile counters[8],
+// error word, then status[passes][nt][256].
+struct OsTmp {
+  uint32_t *ghist, *ctr, *err, *status;
+};
+static OsTmp os_tmp(void *tmp) {
+  uint32_t *w = (uint32_t *)tmp;
+  return OsTmp{w, w + OS_MAX_PASSES * 256, w + OS_MAX_PASSES * 256 + OS_MAX_PASSES,
+               w + OS_MAX_PASSES * 256 + OS_MAX_PASSES + 8};
+}
+static uint64_t os_words(uint64_t nt, uint32_t passes) {
+  return OS_MAX_PASSES * 256 + OS_MAX_PASSES + 8 + (uint64_t)passes * nt * 256;
+}
+
+__global__ void k_os_clear(uint32_t *w, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    w[i] = 0;
+}
+
+// Global digit counts of every pass, from the keys in any order.
+__global__ __launch_bounds__(SORT_THREADS) void k_os_hist(const uint64_t *keys, uint64_t n,
+                                                          uint32_t lo, uint32_t passes,
+                                                          uint32_t *ghist) {
+  __shared__ uint32_t cnt[OS_MAX_PASSES][256];
+  for (uint32_t i = threadIdx.x; i < OS_MAX_PASSES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&cnt[p][(uint32_t)(k >> (lo + 8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t c = cnt[p][threadIdx.x];
+    if (c) atomicAdd(&ghist[p * 256 + threadIdx.x], c);
+  }
+}
+
+__device__ __forceinline__ void os_publish(uint32_t *w, uint32_t v) {
+  __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t os_peek(uint32_t *w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One pass.  A block takes the next tile in start order (atomic counter), so
+// it only ever waits on tiles whose blocks are already running.
+__global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in, uint64_t *out,
+                                                             uint64_t n, uint32_t shift,
+                                                             const uint32_t *ghist, uint32_t *ctr,
+                                                             uint32_t *status, uint32_t *err) {
+  constexpr int NW = SORT_THREADS / WAVE;
+  __shared__ uint32_t wcnt[NW][256];
+  __shared__ uint32_t s_tile, s_bad;
+  for (int i = threadIdx.x; i < NW * 256; i += SORT_THREADS) (&wcnt[0][0])[i] = 0;
+  if (threadIdx.x == 0) {
+    s_tile = atomicAdd(ctr, 1u);
+    s_bad = 0;
+  }
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int w = threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  const uint64_t base = (uint64_t)tile * SORT_TILE + (uint64_t)w * (WAVE * SORT_ITEMS);
+  uint64_t key[SORT_ITEMS];
+  uint32_t rnk[SORT_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * WAVE + lane;
+    bool valid = i < n;
+    uint64_t k = valid ? in[i] : 0;
+    uint32_t d = (uint32_t)(k >> shift) & 255u;
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      unsigned long long bb = __ballot(valid && ((d >> b) & 1u));
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t old = 0;
+    if (valid) old = wcnt[w][d];
+    uint32_t below = mask_lt_count(peers);
+    if (valid && below == 0) wcnt[w][d] = old + (uint32_t)__popcll(peers);
+    key[j] = k;
+    rnk[j] = old + below;
+  }
+  __syncthreads();
+  {
+    const uint32_t d = threadIdx.x;  // SORT_THREADS == 256 digits
+    uint32_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) mine += wcnt[i][d];
+    uint32_t *st = status + (uint64_t)tile * 256 + d;
+    uint32_t excl = 0;
+    if (tile == 0) {
+      os_publish(st, OS_PREFIX | mine);
+    } else {
+      os_publish(st, OS_AGG | mine);
+      // look back OS_LOOK tiles per round trip, newest first; tile 0 always
+      // holds a prefix, so the walk ends there at the latest
+      uint32_t back = 1;  // next predecessor to read: tile - back
+      uint32_t spins = 0;
+      bool done = false;
+      while (!done) {
+        uint32_t v[OS_LOOK];
+#pragma unroll
+        for (int r = 0; r < OS_LOOK; ++r)
+          v[r] = back + r <= tile ? os_peek(st - (uint64_t)(back + r) * 256) : 0u;
+        bool stalled = false;
+#pragma unroll
+        for (int r = 0; r < OS_LOOK; ++r) {
+          if (done || stalled || back > tile) continue;
+          if (v[r] == 0) {
+            stalled = true;  // not published yet: read it again
+            continue;
+          }
+          excl += v[r] & OS_VAL;
+          ++back;
+          if (v[r] & OS_PREFIX) done = true;
+        }
+        if (stalled && !done) {
+          if (++spins > OS_SPIN_LIMIT) {
+            atomicOr(err, 1u);
+            s_bad = 1;  // offsets unknown: this tile writes nothing
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      os_publish(st, OS_PREFIX | (excl + mine));
+    }
+    // digit base: counts of all smaller digits in this pass
+    uint32_t all;
+    uint32_t acc = block_excl_scan<uint32_t, SORT_THREADS>(ghist[d], all) + excl;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      uint32_t c = wcnt[i][d];
+      wcnt[i][d] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  if (s_bad) return;
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * WAVE + lane;
+    if (i < n) {
+      uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+      out[wcnt[w][d] + rnk[j]] = key[j];
+    }
+  }
+}
+
 size_t sort_tmp_bytes(uint64_t n) {
   uint64_t nt = cdiv(n ? n : 1, SORT_TILE);
   uint64_t h = 256 * nt + 1;
-  return h * sizeof(uint32_t) + 256 + scan_tmp_bytes(h);
+  size_t classic = h * sizeof(uint32_t) + 256 + scan_tmp_bytes(h);
+  size_t sweep = n <= OS_MAX_KEYS ? os_words(nt, OS_MAX_PASSES) * sizeof(uint32_t) : 0;
+  return classic > sweep ? classic : sweep;
 }
 
 uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp,
@@ -275,6 +449,26 @@ uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uin
                           void *tmp, hipStream_t s) {
   if (n <= 1) return a;
   uint64_t nt = cdiv(n, SORT_TILE);
+  const uint32_t passes = (hi - lo + 7) / 8;
+  if (n <= OS_MAX_KEYS && passes <= (uint32_t)OS_MAX_PASSES) {
+    const OsTmp t = os_tmp(tmp);
+    const uint64_t words = os_words(nt, passes);
+    k_os_clear<<<dim3((unsigned)(cdiv(words, 256) < 1024 ? cdiv(words, 256) : 1024)), 256, 0, s>>>(
+        t.ghist, words);
+    const uint64_t hb = cdiv(n, SORT_THREADS * 16);
+    k_os_hist<<<dim3((unsigned)(hb < 1024 ? hb : 1024)), dim3(SORT_THREADS), 0, s>>>(a, n, lo,
+                                                                                     passes, t.ghist);
+    uint64_t *src = a, *dst = b;
+    for (uint32_t p = 0; p < passes; ++p) {
+      k_os_scatter<<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(
+          src, dst, n, lo + 8 * p, t.ghist + p * 256, t.ctr + p, t.status + (uint64_t)p * nt * 256,
+          t.err);
+      uint64_t *x = src;
+      src = dst;
+      dst = x;
+    }
+    return src;
+  }
   uint64_t h = 256 * nt;
   uint32_t *hist = (uint32_t *)tmp;
   void *stmp = (void *)(((uintptr_t)(hist + h + 1) + 255) & ~(uintptr_t)255);
