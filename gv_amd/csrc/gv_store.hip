@@ -369,7 +369,7 @@ __device__ __forceinline__ void probe_hit(const Store &st, const ProbeArgs &a, u
 
 constexpr int PROBE_ROUNDS = PROBE_CHUNK / WAVE;
 // An item this small is probed by one lane on its own.
-constexpr uint32_t PROBE_SMALL_SLOTS = 16;
+constexpr uint32_t PROBE_SMALL_SLOTS = 16;  // <= the smallest block (32 slots), multiple of 4
 constexpr uint32_t PROBE_SMALL_KEYS = 4;
 
 // Work items (PROBE_CHUNK slots of one source) are dealt to lanes: wave w of
@@ -412,18 +412,47 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
       for (uint32_t j = 0; j < PROBE_SMALL_KEYS; ++j)
         kv[j] = ks + j < ke ? (a.keys[ks + j] & mask) : ~0ull;
       uint32_t cnt = 0;
-      for (uint32_t pos = lo; pos < hi; ++pos) {
-        const uint32_t idx = slot_index(st.arena, tbl, pos, st.log2B, st.B);
-        const uint32_t x = st.slots[idx];
-        if (x == TOMB) continue;
-        // first matching key, as the lower bound of the wavefront path
-        uint32_t p = PROBE_SMALL_KEYS;
+      if (MODE == PROBE_INSERT) {
+        // lo is a multiple of PROBE_CHUNK and the item has <= 16 slots, so it
+        // lies at the start of one block (B >= 32): one block-table load,
+        // then its slots as four 16-byte loads issued together
+        const uint32_t n = hi - lo;
+        const uint32_t b0 = n ? st.arena[tbl + (lo >> st.log2B)] * st.B : 0u;
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(st.slots + b0);
+        uint32_t x[PROBE_SMALL_SLOTS];
 #pragma unroll
-        for (uint32_t j = PROBE_SMALL_KEYS; j-- > 0;)
-          if (kv[j] == x) p = j;
-        if (p < PROBE_SMALL_KEYS) probe_hit<MODE>(st, a, idx, ks + p, cnt);
+        for (uint32_t r = 0; r < PROBE_SMALL_SLOTS / 4; ++r) {
+          const uint4 q = 4 * r < n ? p4[r] : make_uint4(TOMB, TOMB, TOMB, TOMB);
+          x[4 * r] = q.x;
+          x[4 * r + 1] = q.y;
+          x[4 * r + 2] = q.z;
+          x[4 * r + 3] = q.w;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < PROBE_SMALL_SLOTS; ++j) {
+          if (j >= n || x[j] == TOMB) continue;
+          // first matching key, as the lower bound of the wavefront path
+          uint32_t p = PROBE_SMALL_KEYS;
+#pragma unroll
+          for (uint32_t i = PROBE_SMALL_KEYS; i-- > 0;)
+            if (kv[i] == x[j]) p = i;
+          if (p < PROBE_SMALL_KEYS) probe_hit<MODE>(st, a, b0 + j, ks + p, cnt);
+        }
+      } else {
+        // a delete has few small items; the short loop keeps registers (and
+        // occupancy) for the wavefront-wide items
+        for (uint32_t pos = lo; pos < hi; ++pos) {
+          const uint32_t idx = slot_index(st.arena, tbl, pos, st.log2B, st.B);
+          const uint32_t x = st.slots[idx];
+          if (x == TOMB) continue;
+          uint32_t p = PROBE_SMALL_KEYS;
+#pragma unroll
+          for (uint32_t j = PROBE_SMALL_KEYS; j-- > 0;)
+            if (kv[j] == x) p = j;
+          if (p < PROBE_SMALL_KEYS) probe_hit<MODE>(st, a, idx, ks + p, cnt);
+        }
+        if (cnt) atomicAdd(&a.seg_del[sidx], cnt);
       }
-      if (MODE != PROBE_INSERT && cnt) atomicAdd(&a.seg_del[sidx], cnt);
     }
     unsigned long long big = __ballot(valid && !small);
     while (big) {
