@@ -4,7 +4,7 @@
 
 namespace gv {
 
-constexpr uint32_t PROBE_CHUNK = 512;  // slots per membership/tombstone work item
+constexpr uint32_t PROBE_CHUNK = 256;  // slots per membership/tombstone work item
 
 // Operation of a key in a merged group of batches.
 constexpr uint32_t OP_INSERT = 0;
