@@ -219,8 +219,23 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
                   const uint32_t *seeds_dev, uint32_t ncols, float *out_dev, int *iters,
                   double *max_residual, double *iter_ms, hipStream_t s);
 // Brandes BC from host source list, 32 sources per batch.
+// Grow-only workspace of multi-source BC, kept with the graph handle so
+// repeated calls do not allocate (or free, which waits for the device).
+struct BcmWork {
+  uint32_t V = 0;
+  uint64_t items = 0;  // capacity of part, in items
+  int *dist = nullptr;
+  double *sigma = nullptr;
+  float *delta = nullptr, *bcc = nullptr;
+  float2 *part = nullptr;
+  uint32_t *stamp = nullptr, *queue = nullptr, *cnt = nullptr, *coff = nullptr, *dsrc = nullptr;
+  unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  unsigned long long *used_tot = nullptr;
+  void *tmp = nullptr;
+};
+void bcm_release(BcmWork &w);
 gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
-                       float *out_dev, hipStream_t s);
+                       float *out_dev, BcmWork &w, hipStream_t s);
 
 // Request-loop hooks over a graph handle (gv_api.hip): the relabelled flat
 // PageRank snapshot of the handle's in-adjacency, and a CSR snapshot of its

@@ -36,6 +36,7 @@ struct gv_graph {
   PrWork pr;
   PsWork ps;
   TcdWork tcd;
+  BcmWork bcm;
   Ctl *h_ctl = nullptr;  // pinned, 2 entries
   uint32_t *in_src = nullptr, *in_dst = nullptr;
   uint64_t in_cap = 0;
@@ -517,6 +518,7 @@ gv_status gv_destroy(gv_graph *g) {
   pr_release(g->pr);
   ps_release(g->ps);
   tcd_release(g->tcd);
+  bcm_release(g->bcm);
   hipFree(g->vlist);
   hipFree(g->in_src);
   hipFree(g->in_dst);
@@ -1008,7 +1010,7 @@ gv_status gv_bc(gv_graph *g, const uint32_t *sources, uint32_t n_sources, float 
     hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
     if (e != hipSuccess) return hip_fail(e, "bc alloc");
   }
-  rc = n_sources > 1 ? bc_multi_run(g->st[0], hs.data(), n_sources, dout, g->stream)
+  rc = n_sources > 1 ? bc_multi_run(g->st[0], hs.data(), n_sources, dout, g->bcm, g->stream)
                      : bc_run(g->st[0], hs.data(), n_sources, dout, g->stream);
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);

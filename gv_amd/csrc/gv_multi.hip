@@ -475,29 +475,88 @@ __global__ void k_used_sum(Store st, unsigned long long *tot) {
   if (threadIdx.x == 0 && acc) atomicAdd(tot, acc);
 }
 
+void bcm_release(BcmWork &w) {
+  hipFree(w.dist);
+  hipFree(w.sigma);
+  hipFree(w.delta);
+  hipFree(w.bcc);
+  hipFree(w.part);
+  hipFree(w.stamp);
+  hipFree(w.queue);
+  hipFree(w.cnt);
+  hipFree(w.coff);
+  hipFree(w.tmp);
+  hipFree(w.dsrc);
+  hipFree(w.cursor);
+  hipFree(w.used_tot);
+  if (w.h_cursor) hipHostFree(w.h_cursor);
+  w = BcmWork{};
+}
+
+// Per-vertex state for V vertices, level items for max_items.
+static gv_status bcm_reserve(BcmWork &w, uint32_t V, uint64_t max_items) {
+  hipError_t e = hipSuccess;
+  if (!w.used_tot) {
+    if ((e = hipMalloc(&w.used_tot, sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&w.dsrc, BCM_S * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&w.cursor, sizeof(unsigned int))) != hipSuccess ||
+        (e = hipHostMalloc(&w.h_cursor, sizeof(unsigned int), hipHostMallocDefault)) != hipSuccess)
+      return hip_fail(e, "bc alloc");
+  }
+  if (V > w.V) {
+    hipFree(w.dist);
+    hipFree(w.sigma);
+    hipFree(w.delta);
+    hipFree(w.bcc);
+    hipFree(w.stamp);
+    hipFree(w.queue);
+    hipFree(w.cnt);
+    hipFree(w.coff);
+    hipFree(w.tmp);
+    w.dist = nullptr;
+    w.sigma = nullptr;
+    w.delta = w.bcc = nullptr;
+    w.stamp = w.queue = w.cnt = w.coff = nullptr;
+    w.tmp = nullptr;
+    w.V = 0;
+    const uint64_t n2 = (uint64_t)V * BCM_S;
+    if ((e = hipMalloc(&w.dist, n2 * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&w.sigma, n2 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&w.delta, n2 * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&w.bcc, (size_t)V * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&w.stamp, (size_t)V * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&w.queue, n2 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&w.cnt, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&w.coff, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&w.tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess)
+      return hip_fail(e, "bc alloc");
+    w.V = V;
+  }
+  if (max_items > w.items) {
+    hipFree(w.part);
+    w.part = nullptr;
+    w.items = 0;
+    const uint64_t cap = max_items + max_items / 4;  // room for graphs that grow
+    if ((e = hipMalloc(&w.part, cap * BCM_S * sizeof(float2))) != hipSuccess)
+      return hip_fail(e, "bc alloc");
+    w.items = cap;
+  }
+  return GV_OK;
+}
+
 gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
-                       float *out_dev, hipStream_t s) {
+                       float *out_dev, BcmWork &w, hipStream_t s) {
   const uint32_t V = st.V;
   if (!V) return GV_OK;
-  int *dist = nullptr;
-  double *sigma = nullptr;
-  float *delta = nullptr, *bcc = nullptr;
-  float2 *part = nullptr;
-  uint32_t *stamp = nullptr, *queue = nullptr, *dsrc = nullptr, *cnt = nullptr, *coff = nullptr;
-  unsigned int *cursor = nullptr, *h_cursor = nullptr;
-  unsigned long long *used_tot = nullptr, h_used = 0;
-  void *tmp = nullptr;
+  unsigned long long h_used = 0;
   gv_status rc = GV_OK;
   const uint64_t n2 = (uint64_t)V * BCM_S;
   do {
     hipError_t e = hipSuccess;
-    if ((e = hipMalloc(&used_tot, sizeof(unsigned long long))) != hipSuccess) {
-      rc = hip_fail(e, "bc alloc");
-      break;
-    }
-    fill_u64((uint64_t *)used_tot, 0, 1, s);
-    k_used_sum<<<grid_for(V, 256, 1024), 256, 0, s>>>(st, used_tot);
-    if ((e = hipMemcpyAsync(&h_used, used_tot, sizeof(h_used), hipMemcpyDeviceToHost, s)) !=
+    if ((rc = bcm_reserve(w, V, 0))) break;
+    fill_u64((uint64_t *)w.used_tot, 0, 1, s);
+    k_used_sum<<<grid_for(V, 256, 1024), 256, 0, s>>>(st, w.used_tot);
+    if ((e = hipMemcpyAsync(&h_used, w.used_tot, sizeof(h_used), hipMemcpyDeviceToHost, s)) !=
             hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess) {
       rc = hip_fail(e, "bc");
@@ -505,27 +564,19 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
     }
     // items of one level: at most one per frontier vertex plus one per BCM_CH slots
     const uint64_t max_items = (uint64_t)V + h_used / BCM_CH + 1;
-    if ((e = hipMalloc(&dist, n2 * sizeof(int))) != hipSuccess ||
-        (e = hipMalloc(&sigma, n2 * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc(&delta, n2 * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&bcc, (size_t)V * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&stamp, (size_t)V * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&queue, n2 * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&cnt, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&coff, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&part, max_items * BCM_S * sizeof(float2))) != hipSuccess ||
-        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess ||
-        (e = hipMalloc(&dsrc, BCM_S * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&cursor, sizeof(unsigned int))) != hipSuccess ||
-        (e = hipHostMalloc(&h_cursor, sizeof(unsigned int), hipHostMallocDefault)) != hipSuccess) {
-      rc = hip_fail(e, "bc alloc");
-      break;
-    }
     if (max_items > 0xFFFFFFFFull) {
       set_error("bc: too many work items for 32-bit offsets");
       rc = GV_E_INVALID;
       break;
     }
+    if ((rc = bcm_reserve(w, V, max_items))) break;
+    int *dist = w.dist;
+    double *sigma = w.sigma;
+    float *delta = w.delta, *bcc = w.bcc;
+    float2 *part = w.part;
+    uint32_t *stamp = w.stamp, *queue = w.queue, *cnt = w.cnt, *coff = w.coff, *dsrc = w.dsrc;
+    unsigned int *cursor = w.cursor, *h_cursor = w.h_cursor;
+    void *tmp = w.tmp;
     // item offsets of the frontier segment [qs, qe); a level's frontier holds
     // each vertex at most once, so nf <= V
     auto level_items = [&](uint32_t qs, uint32_t qe) {
@@ -590,20 +641,6 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
     }
     if (rc == GV_OK && (hipStreamSynchronize(s) != hipSuccess)) rc = hip_fail(hipGetLastError(), "bc");
   } while (0);
-  hipFree(dist);
-  hipFree(sigma);
-  hipFree(delta);
-  hipFree(bcc);
-  hipFree(part);
-  hipFree(stamp);
-  hipFree(queue);
-  hipFree(cnt);
-  hipFree(coff);
-  hipFree(tmp);
-  hipFree(dsrc);
-  hipFree(cursor);
-  hipFree(used_tot);
-  if (h_cursor) hipHostFree(h_cursor);
   return rc;
 }
 
