@@ -219,6 +219,7 @@ void scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *n_d
               hipStream_t s);
 void scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, const uint32_t *n_dev, void *tmp,
               hipStream_t s);
+// Workspace of any sort of at most n keys.
 size_t sort_tmp_bytes(uint64_t n);
 // LSD radix sort of n keys on their low `bits` bits; returns the buffer that
 // holds the result (a or b).

@@ -432,11 +432,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
   }
 }
 
+// Workspace for any sort of at most n keys: callers size it once and sort
+// fewer keys (a sort of <= 2^24 keys takes the single-sweep path even when
+// the workspace was sized for more).
 size_t sort_tmp_bytes(uint64_t n) {
   uint64_t nt = cdiv(n ? n : 1, SORT_TILE);
   uint64_t h = 256 * nt + 1;
   size_t classic = h * sizeof(uint32_t) + 256 + scan_tmp_bytes(h);
-  size_t sweep = n <= OS_MAX_KEYS ? os_words(nt, OS_MAX_PASSES) * sizeof(uint32_t) : 0;
+  const uint64_t ns = n < OS_MAX_KEYS ? n : OS_MAX_KEYS;
+  size_t sweep = os_words(cdiv(ns ? ns : 1, SORT_TILE), OS_MAX_PASSES) * sizeof(uint32_t);
   return classic > sweep ? classic : sweep;
 }
 

@@ -158,6 +158,21 @@ def test_pagerank_segmented_snapshot_bins_and_odd_sizes(V):
         g.close()
 
 
+def test_pagerank_cycle_of_2_pow_24_vertices():
+    # the relabelling sort runs on exactly 2^24 keys (the largest single-sweep
+    # sort) with a workspace sized for V + 1 keys, and the build stages 2^24
+    # keys; a directed cycle has x = 1/V at every iteration
+    V = 1 << 24
+    s = np.arange(V, dtype=np.uint32)
+    d = (s + 1) % V
+    g = make(V, True, pool=V + 4096, block_slots=32)
+    g.bulkBuild(s, d, stats=False)
+    x, it, r = G.static_pagerank(g, 0.85, 1e-6, 20)
+    assert it == 1 and r < 1e-6
+    assert np.allclose(x, 1.0 / V, rtol=1e-6)
+    g.close()
+
+
 @pytest.mark.parametrize("n", [3, 7, 40])
 def test_tc_complete_graph(n):
     e = list(itertools.combinations(range(n), 2))
