@@ -228,7 +228,8 @@ struct BcmWork {
   double *sigma = nullptr;
   float *delta = nullptr, *bcc = nullptr;
   float2 *part = nullptr;
-  uint32_t *stamp = nullptr, *queue = nullptr, *cnt = nullptr, *coff = nullptr, *dsrc = nullptr;
+  uint2 *sn = nullptr;  // per vertex: (seen, next) source masks of the forward sweep
+  uint32_t *queue = nullptr, *cnt = nullptr, *coff = nullptr, *dsrc = nullptr;
   unsigned int *cursor = nullptr, *h_cursor = nullptr;
   unsigned long long *used_tot = nullptr;
   void *tmp = nullptr;

@@ -184,6 +184,14 @@ __device__ __forceinline__ uint32_t wave_lower_bound(uint32_t sv, uint32_t x) {
   return pos;
 }
 
+// Orders a wavefront's LDS writes before its later LDS reads (lanes of one
+// wavefront exchanging data through LDS without a block barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave-aggregated atomic add: every lane of the wave calls it (inc may be 0);
 // one atomic per wave; returns this lane's exclusive base.
 __device__ __forceinline__ unsigned long long wave_atomic_add(unsigned long long *ctr,

@@ -239,25 +239,41 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
 constexpr int BCM_S = 32;   // sources per batch = lanes per frontier vertex
 constexpr int BCM_ILP = 8;  // neighbour rows requested before the first is used
 
-__global__ void k_bcm_init(int *dist, double *sigma, float *delta, uint32_t *stamp, uint32_t V) {
+// Per-vertex source masks of the forward sweep: seen (x) = sources that
+// reached the vertex at an earlier level, next (y) = sources reaching it at
+// the level being expanded.
+__global__ void k_bcm_init(int *dist, double *sigma, float *delta, uint2 *sn, uint32_t V) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)V * BCM_S;
        i += (uint64_t)gridDim.x * blockDim.x) {
     dist[i] = -1;
     sigma[i] = 0.0;
     delta[i] = 0.f;
-    if (i < V) stamp[i] = 0;
+    if (i < V) sn[i] = make_uint2(0u, 0u);
   }
 }
 
-// Sources of the batch enter level 0 (stamp = level + 1 marks queue entry).
-__global__ void k_bcm_src(const uint32_t *src, uint32_t ns, int *dist, double *sigma,
-                          uint32_t *stamp, uint32_t *queue, unsigned int *cursor) {
+// Sources of the batch enter level 0; a vertex holding several sources is
+// queued once.
+__global__ void k_bcm_src(const uint32_t *src, uint32_t ns, int *dist, double *sigma, uint2 *sn,
+                          uint32_t *queue, unsigned int *cursor) {
   const uint32_t s = threadIdx.x;
   if (s >= ns) return;
   const uint32_t v = src[s];
   dist[(uint64_t)v * BCM_S + s] = 0;
   sigma[(uint64_t)v * BCM_S + s] = 1.0;
-  if (atomicMax(&stamp[v], 1u) < 1u) queue[atomicAdd(cursor, 1u)] = v;
+  if (atomicOr(&sn[v].x, 1u << s) == 0u) queue[atomicAdd(cursor, 1u)] = v;
+}
+
+// After a level: the new frontier's level bits join its seen mask.
+__global__ void k_bcm_next(const uint32_t *queue, uint32_t qe, const unsigned int *cursor,
+                           uint2 *sn) {
+  const uint32_t n = *cursor;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = queue[qe + i];
+    const uint2 m = sn[w];
+    sn[w] = make_uint2(m.x | m.y, 0u);
+  }
 }
 
 // Work items: a frontier vertex with more than BCM_CH slots is split into
@@ -310,57 +326,87 @@ __device__ __forceinline__ BcmItem bcm_item(const Store &st, const int *dist, co
   return it;
 }
 
-__global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *sigma,
-                                                 uint32_t *stamp, uint32_t *queue, uint32_t qs,
-                                                 uint32_t qe, int d, unsigned int *cursor,
-                                                 const uint32_t *coff) {
-  const uint32_t lane = lane_id();
-  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;  // two items per wavefront
-  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
+// Forward, one wavefront per item with its lanes over the item's slots
+// (bit-parallel multi-source BFS, Then et al., VLDB 2014): F = the sources
+// for which v is at level d (dist[v][s] == d); for a neighbour w, cand =
+// F & ~seen[w] are exactly the sources whose shortest-path DAG holds v -> w,
+// so sigma[w][s] += sigma[v][s] for each.  The first claim of a bit at this
+// level (atomicOr on next[w]) sets dist[w][s] = d + 1, and the claim that
+// makes next[w] non-zero queues w.  One 8-byte mask load per edge instead
+// of a 32-source distance row.
+__global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *sigma, uint2 *sn,
+                                                 uint32_t *queue, uint32_t qs, uint32_t qe, int d,
+                                                 unsigned int *cursor, const uint32_t *coff) {
+  __shared__ double s_sig[4][BCM_S];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   const uint32_t nf = qe - qs;
   const uint32_t nitems = coff[nf];
-  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
-       g0 < nitems; g0 += ngroups) {
-    const uint64_t gi = g0 + sub;
-    const BcmItem it = bcm_item(st, dist, sigma, queue, qs, coff, nf, gi, gi < nitems, s);
-    const bool act = it.dv == d;
-    const uint32_t span = it.c1 - it.c0;
-    // both half-waves loop to the larger span; shuffles stay uniform
-    const uint32_t span_max = max(span, __shfl_xor(span, BCM_S, WAVE));
-    for (uint32_t base = 0; base < span_max; base += BCM_S) {
-      uint32_t slot = TOMB;
-      if (base + s < span)
-        slot = st.slots[slot_index(st.arena, it.tbl, it.c0 + base + s, st.log2B, st.B)];
-      for (uint32_t t0 = 0; t0 < (uint32_t)BCM_S; t0 += BCM_ILP) {
-        // BCM_ILP neighbours' distance rows are requested before any is used
-        uint32_t w[BCM_ILP];
-        int cur[BCM_ILP];
-#pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k) w[k] = __shfl(slot, (int)(t0 + k), BCM_S);
-#pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k)
-          cur[k] = (act && w[k] != TOMB) ? dist[(uint64_t)w[k] * BCM_S + s] : -2;
-#pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k) {
-          // dist moves from -1 to a level once per batch, so a plain read of
-          // any other value is final; only a -1 needs the CAS to decide.
-          bool fresh = false;
-          const uint64_t wi = (uint64_t)w[k] * BCM_S + s;
-          if (cur[k] == -1) {
-            const int old = atomicCAS(&dist[wi], -1, d + 1);
-            fresh = old == -1;
-            cur[k] = fresh ? d + 1 : old;
+  for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < nitems;
+       it += nwaves) {
+    const uint32_t j = bcm_owner(coff, nf, (uint32_t)it);
+    const uint32_t v = queue[qs + j];
+    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
+    const uint32_t c0 = ((uint32_t)it - coff[j]) * BCM_CH;
+    const uint32_t c1 = c0 + BCM_CH < us ? c0 + BCM_CH : us;
+    const int dv = lane < (uint32_t)BCM_S ? dist[(uint64_t)v * BCM_S + lane] : -1;
+    const uint32_t F = (uint32_t)__ballot(lane < (uint32_t)BCM_S && dv == d);
+    if (F == 0u) continue;
+    if (lane < (uint32_t)BCM_S) s_sig[wid][lane] = sigma[(uint64_t)v * BCM_S + lane];
+    wave_lds_sync();
+    for (uint32_t base = c0; base < c1; base += WAVE) {
+      const uint32_t pos = base + lane;
+      const uint32_t w =
+          pos < c1 ? st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)] : TOMB;
+      bool enq = false;
+      if (w != TOMB) {
+        const uint2 m = sn[w];
+        const uint32_t cand = F & ~m.x;
+        if (cand) {
+          uint32_t claimed = 0u;
+          if (cand & ~m.y) {  // some bit not claimed yet at this level
+            const uint32_t old = atomicOr(&sn[w].y, cand);
+            claimed = cand & ~old;
+            enq = old == 0u;
           }
-          if (cur[k] == d + 1) atomicAdd(&sigma[wi], it.sv);
-          const unsigned long long m = __ballot(fresh);
-          const unsigned long long mine = (m >> (sub * BCM_S)) & 0xFFFFFFFFull;
-          if (s == 0 && mine) {
-            if (atomicMax(&stamp[w[k]], (uint32_t)(d + 2)) < (uint32_t)(d + 2))
-              queue[qe + atomicAdd(cursor, 1u)] = w[k];
+          for (uint32_t b = claimed; b; b &= b - 1)
+            dist[(uint64_t)w * BCM_S + (__ffs(b) - 1)] = d + 1;
+          for (uint32_t b = cand; b; b &= b - 1) {
+            const uint32_t s = __ffs(b) - 1;
+            atomicAdd(&sigma[(uint64_t)w * BCM_S + s], s_sig[wid][s]);
           }
         }
       }
+      const unsigned long long qm = __ballot(enq);
+      if (qm) {
+        uint32_t qb = 0;
+        if (lane == 0) qb = atomicAdd(cursor, (unsigned int)__popcll(qm));
+        qb = __shfl(qb, 0, WAVE);
+        if (enq) queue[qe + qb + mask_lt_count(qm)] = w;
+      }
     }
+    wave_lds_sync();
+  }
+}
+
+// Backward level masks: next[w] (free after the forward sweep) = the sources
+// for which w is at level lev, for the queue entries [qs, qe) of that level
+// (set), or 0 again (clear).  Two entries per wavefront, lane = source.
+__global__ __launch_bounds__(256) void k_bcm_lmask(const uint32_t *queue, uint32_t qs,
+                                                   uint32_t qe, int lev, const int *dist,
+                                                   uint2 *sn, int set) {
+  const uint32_t lane = lane_id();
+  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
+  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
+  const uint32_t n = qe - qs;
+  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
+       g0 < n; g0 += ngroups) {
+    const uint64_t j = g0 + sub;
+    const bool have = j < n;
+    const uint32_t w = have ? queue[qs + j] : 0u;
+    const bool bit = have && set && dist[(uint64_t)w * BCM_S + s] == lev;
+    const uint32_t m = (uint32_t)((__ballot(bit) >> (sub * BCM_S)) & 0xFFFFFFFFull);
+    if (have && s == 0) sn[w].y = m;
   }
 }
 
@@ -374,10 +420,13 @@ __device__ __forceinline__ void bcm_merge(float &s, float &c, float s2, float c2
 
 // Backward, per item: Kahan sum over the item's successors of
 // sigma_v/sigma_w (1 + delta_w), written as a (sum, compensation) partial.
+// Successors at level d + 1 are found through the level masks (next[w],
+// set by k_bcm_lmask): one 4-byte mask per neighbour, shuffled to the 32
+// source lanes, instead of a 32-source distance row.
 __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, const double *sigma,
-                                                 const float *delta, const uint32_t *queue,
-                                                 uint32_t qs, uint32_t qe, int d,
-                                                 const uint32_t *coff, float2 *part) {
+                                                 const float *delta, const uint2 *sn,
+                                                 const uint32_t *queue, uint32_t qs, uint32_t qe,
+                                                 int d, const uint32_t *coff, float2 *part) {
   const uint32_t lane = lane_id();
   const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
   const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
@@ -396,6 +445,7 @@ __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, cons
       uint32_t slot = TOMB;
       if (base + s < span)
         slot = st.slots[slot_index(st.arena, it.tbl, it.c0 + base + s, st.log2B, st.B)];
+      const uint32_t lm = slot != TOMB ? sn[slot].y : 0u;  // this slot's level-(d+1) sources
       for (uint32_t t0 = 0; t0 < (uint32_t)BCM_S; t0 += BCM_ILP) {
         // rows of BCM_ILP neighbours in flight; sums stay in neighbour order
         uint32_t w[BCM_ILP];
@@ -403,10 +453,12 @@ __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, cons
         double sw[BCM_ILP];
         float dw[BCM_ILP];
 #pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k) w[k] = __shfl(slot, (int)(t0 + k), BCM_S);
+        for (int k = 0; k < BCM_ILP; ++k) {
+          w[k] = __shfl(slot, (int)(t0 + k), BCM_S);
+          hit[k] = (__shfl(lm, (int)(t0 + k), BCM_S) >> s) & 1u;
+        }
 #pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k)
-          hit[k] = act && w[k] != TOMB && dist[(uint64_t)w[k] * BCM_S + s] == d + 1;
+        for (int k = 0; k < BCM_ILP; ++k) hit[k] = hit[k] && act && w[k] != TOMB;
 #pragma unroll
         for (int k = 0; k < BCM_ILP; ++k) {
           const uint64_t wi = (uint64_t)w[k] * BCM_S + s;
@@ -481,7 +533,7 @@ void bcm_release(BcmWork &w) {
   hipFree(w.delta);
   hipFree(w.bcc);
   hipFree(w.part);
-  hipFree(w.stamp);
+  hipFree(w.sn);
   hipFree(w.queue);
   hipFree(w.cnt);
   hipFree(w.coff);
@@ -508,7 +560,7 @@ static gv_status bcm_reserve(BcmWork &w, uint32_t V, uint64_t max_items) {
     hipFree(w.sigma);
     hipFree(w.delta);
     hipFree(w.bcc);
-    hipFree(w.stamp);
+    hipFree(w.sn);
     hipFree(w.queue);
     hipFree(w.cnt);
     hipFree(w.coff);
@@ -516,7 +568,8 @@ static gv_status bcm_reserve(BcmWork &w, uint32_t V, uint64_t max_items) {
     w.dist = nullptr;
     w.sigma = nullptr;
     w.delta = w.bcc = nullptr;
-    w.stamp = w.queue = w.cnt = w.coff = nullptr;
+    w.sn = nullptr;
+    w.queue = w.cnt = w.coff = nullptr;
     w.tmp = nullptr;
     w.V = 0;
     const uint64_t n2 = (uint64_t)V * BCM_S;
@@ -524,7 +577,7 @@ static gv_status bcm_reserve(BcmWork &w, uint32_t V, uint64_t max_items) {
         (e = hipMalloc(&w.sigma, n2 * sizeof(double))) != hipSuccess ||
         (e = hipMalloc(&w.delta, n2 * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&w.bcc, (size_t)V * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&w.stamp, (size_t)V * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&w.sn, (size_t)V * sizeof(uint2))) != hipSuccess ||
         (e = hipMalloc(&w.queue, n2 * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&w.cnt, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&w.coff, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
@@ -574,7 +627,8 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
     double *sigma = w.sigma;
     float *delta = w.delta, *bcc = w.bcc;
     float2 *part = w.part;
-    uint32_t *stamp = w.stamp, *queue = w.queue, *cnt = w.cnt, *coff = w.coff, *dsrc = w.dsrc;
+    uint2 *sn = w.sn;
+    uint32_t *queue = w.queue, *cnt = w.cnt, *coff = w.coff, *dsrc = w.dsrc;
     unsigned int *cursor = w.cursor, *h_cursor = w.h_cursor;
     void *tmp = w.tmp;
     // item offsets of the frontier segment [qs, qe); a level's frontier holds
@@ -588,6 +642,10 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
       const uint64_t bound = (uint64_t)nf + h_used / BCM_CH + 1;
       return grid_for(bound * BCM_S, 256, 8192);
     };
+    auto fwd_grid = [&](uint32_t nf) {  // one wavefront per item
+      const uint64_t bound = (uint64_t)nf + h_used / BCM_CH + 1;
+      return grid_for(bound * WAVE, 256, 8192);
+    };
     fill_f32(out_dev, 0.f, V, s);
     fill_f32(bcc, 0.f, V, s);
     for (uint32_t b0 = 0; b0 < n_sources && rc == GV_OK; b0 += BCM_S) {
@@ -597,9 +655,9 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
         rc = hip_fail(e, "bc");
         break;
       }
-      k_bcm_init<<<grid_for(n2, 256, 16384), 256, 0, s>>>(dist, sigma, delta, stamp, V);
+      k_bcm_init<<<grid_for(n2, 256, 16384), 256, 0, s>>>(dist, sigma, delta, sn, V);
       fill_u32(cursor, 0, 1, s);
-      k_bcm_src<<<1, BCM_S, 0, s>>>(dsrc, ns, dist, sigma, stamp, queue, cursor);
+      k_bcm_src<<<1, BCM_S, 0, s>>>(dsrc, ns, dist, sigma, sn, queue, cursor);
       if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
               hipSuccess ||
           (e = hipStreamSynchronize(s)) != hipSuccess) {
@@ -613,8 +671,10 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
         if (qe == qs) break;
         fill_u32(cursor, 0, 1, s);
         level_items(qs, qe);
-        k_bcm_fwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, stamp, queue, qs, qe, d,
-                                                     cursor, coff);
+        k_bcm_fwd<<<fwd_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, sn, queue, qs, qe, d, cursor,
+                                                    coff);
+        k_bcm_next<<<grid_for(qe - qs + h_used / BCM_CH + 1, 256, 4096), 256, 0, s>>>(queue, qe,
+                                                                                   cursor, sn);
         if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
                 hipSuccess ||
             (e = hipStreamSynchronize(s)) != hipSuccess) {
@@ -627,14 +687,22 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
       if (rc) break;
       // queue[lvl[k], lvl[k+1]) = union frontier of level k: every vertex
       // that some source of the batch reaches at depth k (a vertex appears
-      // once per distinct depth, stamp-deduplicated).  The deepest level has
+      // once per distinct depth).  The deepest level has
       // no successors, so the backward sweep starts one level above it.
       const int D = (int)lvl.size() - 2;
+      auto lmask = [&](int lev, int set) {
+        const uint32_t a = lvl[lev], b = lvl[lev + 1];
+        if (b > a)
+          k_bcm_lmask<<<grid_for((uint64_t)(b - a) * BCM_S, 256, 8192), 256, 0, s>>>(
+              queue, a, b, lev, dist, sn, set);
+      };
       for (int lev = D - 2; lev >= 0; --lev) {
         const uint32_t qs = lvl[lev], qe = lvl[lev + 1];
         level_items(qs, qe);
-        k_bcm_bwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, delta, queue, qs, qe, lev,
-                                                     coff, part);
+        lmask(lev + 1, 1);
+        k_bcm_bwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, delta, sn, queue, qs, qe,
+                                                     lev, coff, part);
+        lmask(lev + 1, 0);
         k_bcm_fin<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
             queue, qs, qe, lev, dist, delta, part, coff, dsrc, ns, out_dev, bcc);
       }

@@ -487,12 +487,6 @@ struct TcdArgs {
   unsigned long long *delta;   // change of the total (two's complement)
 };
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ uint64_t tcd_hash(const TcdArgs &a, uint64_t key) {
   return (key * 0x9E3779B97F4A7C15ull) >> a.hshift;  // multiplicative, top bits
 }
