@@ -249,6 +249,7 @@ def run_rmat(args):
         out["triangle_count"] = {"ms_mean": float(np.mean(tc_ms)), "total_last": int(tc_total[-1])}
     out["rooflines"] = other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms)
 
+    out["h2d_inclusive"] = h2d_inclusive(g, G, V, args)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
     if not args.no_tc:
@@ -316,6 +317,43 @@ def other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms):
                                  "kernels), HIP events on the graph's stream around the call "
                                  "(host round trips between levels included)")
     return out
+
+
+def h2d_inclusive(g, G, V, args):
+    """The plain form with the batch handed over in host memory (the boundary's
+    GV_MEM_HOST path: each call copies its pairs host->device on the graph's
+    stream), host-timed per batch around delete half + insert half +
+    compaction.  Fresh batches of the same kind (deletes sampled from the
+    current graph, uniform inserts), generated on the device and copied to
+    host arrays before timing."""
+    STREAM_INS, STREAM_DEL = 2, 3
+    nb = 3
+    half = args.batch // 2
+    off, cols, _ = G.export_csr_device(g)
+    host = []
+    for b in range(nb):
+        ds, dd = G.gen_sample(g, skey(args.seed, STREAM_DEL * 1000 + 900 + b), V, off, cols, half)
+        is_, id_ = G.gen_uniform(g, skey(args.seed, STREAM_INS * 1000 + 900 + b), V, half)
+        host.append((ds.numpy(), dd.numpy(), is_.numpy(), id_.numpy()))
+        for a in (ds, dd, is_, id_):
+            a.free()
+    off.free()
+    cols.free()
+    g.sync()
+    ms = []
+    for ds, dd, is_, id_ in host:
+        t0 = time.perf_counter()
+        g.batchDelete(ds, dd, stats=False)
+        g.batchInsert(is_, id_, stats=False)
+        G.lib().gv_compact(g.handle, 0.25, None)
+        g.sync()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    per = float(np.mean(ms))
+    return {"value": args.batch / (per / 1e3), "unit": "edge updates/s", "ms_per_batch": per,
+            "batches": nb,
+            "what": "fresh 1M-edge batches of the same kind passed as host arrays through the "
+                    "C ABI (GV_MEM_HOST, pageable memory), host-timed; deletes sampled from the "
+                    "graph as it stands after the timed batches"}
 
 
 def tc_update(g, G, V, args):
