@@ -471,7 +471,7 @@ def cpu_baseline(g, V, args, batches):
         off, cols = O.omp_apply_batch(V, False, off, cols, si, di, delete=False)
         t_total += time.perf_counter() - t0
         done += args.batch
-    return {
+    out = {
         "value": done / t_total,
         "unit": "edge updates/s",
         "cores": O.omp_threads(),
@@ -480,6 +480,31 @@ def cpu_baseline(g, V, args, batches):
                   "graph by OpenMP sorted-CSR rebuild (parallel radix sort + merge)",
         "cpu_model": _cpu_model(),
     }
+    out["sequential_oracle"] = cpu_sequential(g, V, args, batches)
+    return out
+
+
+def cpu_sequential(g, V, args, batches):
+    """The host-sequential oracle (one core: sorted adjacency arrays, batch
+    radix-sorted and merged per touched vertex) loaded with the same graph
+    (not timed), then one of the same 1M-edge batches timed."""
+    from oracle import oracle as O
+    off, cols = g.export_csr()
+    u = np.repeat(np.arange(V, dtype=np.uint32), np.diff(off).astype(np.int64))
+    keep = u < cols
+    o = O.OracleGraph(V, False)
+    o.bulk_build(u[keep], cols[keep])
+    del u, keep, off, cols
+    ds, dd, is_, id_ = batches[0]
+    sd, dd_h, si, di = ds.numpy(), dd.numpy(), is_.numpy(), id_.numpy()
+    t0 = time.perf_counter()
+    o.delete(sd, dd_h)
+    o.insert(si, di)
+    t = time.perf_counter() - t0
+    o.close()
+    return {"value": args.batch / t, "unit": "edge updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 of the same 1M-edge batches (delete half, insert half) on the "
+                      f"scale-{args.scale} graph, oracle/gv_oracle.c on one core"}
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pagerank_flat_rmat22_traffic.json")
