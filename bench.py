@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--analytics-every", type=int, default=1,
                    help="run PageRank + TC after every k-th timed batch (0 = never)")
     p.add_argument("--no-tc", action="store_true")
+    p.add_argument("--no-parity", action="store_true",
+                   help="skip the full-size neighbour-set comparison with the oracle")
     p.add_argument("--cpu-baseline", type=int, default=1, help="1 = time the CPU baseline")
     p.add_argument("--cpu-batches", type=int, default=2)
     p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
@@ -249,6 +251,8 @@ def run_rmat(args):
         out["triangle_count"] = {"ms_mean": float(np.mean(tc_ms)), "total_last": int(tc_total[-1])}
     out["rooflines"] = other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms)
 
+    if not args.no_parity:
+        out["parity_full_size"] = full_size_parity(g, V, args, batches, n_raw)
     out["h2d_inclusive"] = h2d_inclusive(g, G, V, args)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
@@ -317,6 +321,33 @@ def other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms):
                                  "kernels), HIP events on the graph's stream around the call "
                                  "(host round trips between levels included)")
     return out
+
+
+def full_size_parity(g, V, args, batches, n_raw):
+    """Parity gate on the benchmark graph itself: the oracle is built from the
+    host R-MAT generator (the same stream as the device one) and given the
+    same warm-up and timed batches; degrees and every sorted neighbour set
+    must equal the device store's after the timed batches."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    s, d = O.gen_rmat(skey(args.seed, 1), args.scale, n_raw)
+    o = O.OracleGraph(V, False)
+    o.bulk_build(s, d)
+    del s, d
+    for b in range(args.warmup + args.steps):
+        ds, dd, is_, id_ = batches[b]
+        o.delete(ds.numpy(), dd.numpy())
+        o.insert(is_.numpy(), id_.numpy())
+    goff, gcols = g.export_csr()
+    ooff, ocols = o.export_csr()
+    o.close()
+    deg_eq = bool(np.array_equal(goff, ooff))
+    sets_eq = deg_eq and bool(np.array_equal(gcols, ocols))
+    return {"degrees_equal": deg_eq, "neighbour_sets_equal": sets_eq,
+            "directed_edges": int(goff[-1]), "check_s": round(time.perf_counter() - t0, 1),
+            "what": "sequential oracle built from the same R-MAT stream and the same "
+                    f"{args.warmup + args.steps} batches, compared with the device store's "
+                    "sorted export after the timed batches"}
 
 
 def h2d_inclusive(g, G, V, args):
