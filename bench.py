@@ -236,7 +236,8 @@ def run_rmat(args):
         traffic, tsrc = load_traffic(args)
         out["roofline"] = {
             "kernel": "PageRank flat-stream iteration (k_pf_tiles + k_pf_rows + k_pf_resid), "
-                      "timed per iteration with HIP events on the graph's stream"
+                      "timed with HIP events on the graph's stream around the counted "
+                      "iterations (launch gaps included)"
                       if args.pr_layout == 1 else f"PageRank iteration, layout {args.pr_layout}",
             "bound": "hbm",
             "achieved": achieved,

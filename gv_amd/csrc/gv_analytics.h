@@ -46,8 +46,9 @@ struct PrWork {
   double *partials = nullptr;    // per block residual partials, then per hub
   uint64_t chunk_cap = 0, part_cap = 0;
   PrCtl *ctl = nullptr;
-  PrCtl *h_ctl = nullptr;        // pinned host mirror
-  std::vector<hipEvent_t> ev;    // per-iteration start/stop events (timed runs)
+  PrCtl *h_ctl = nullptr;        // pinned host mirror: [0] final, [1..2] per iteration group
+  std::vector<hipEvent_t> ev;    // timed runs: start, then one event after each iteration
+  hipEvent_t ev_grp[2] = {nullptr, nullptr};  // end of an iteration group (flat form)
   // CSR snapshot of the in-adjacency (live slots only), rebuilt per call.
   uint64_t *snap_off = nullptr;   // [V+1]
   uint32_t *snap_cols = nullptr;  // [E]

@@ -53,7 +53,7 @@ gv_status pr_reserve(PrWork &w, uint32_t V) {
   GV_HIP(hipMalloc(&w.list, n * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&w.hub_base, n * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&w.ctl, sizeof(PrCtl)));
-  GV_HIP(hipHostMalloc(&w.h_ctl, sizeof(PrCtl), hipHostMallocDefault));
+  GV_HIP(hipHostMalloc(&w.h_ctl, 3 * sizeof(PrCtl), hipHostMallocDefault));
   return GV_OK;
 }
 
@@ -70,6 +70,11 @@ void pr_release(PrWork &w) {
   hipFree(w.ctl);
   if (w.h_ctl) hipHostFree(w.h_ctl);
   for (hipEvent_t e : w.ev) hipEventDestroy(e);
+  w.ev.clear();
+  for (hipEvent_t &e : w.ev_grp) {
+    if (e) hipEventDestroy(e);
+    e = nullptr;
+  }
   hipFree(w.snap_off);
   hipFree(w.snap_cols);
   hipFree(w.snap_istart);

@@ -138,27 +138,124 @@ __device__ __forceinline__ void pf_load(const uint32_t *cols, uint32_t t, uint32
   for (uint32_t k = 0; k < PF_K; ++k) e[k] = __builtin_nontemporal_load(p + k * 64);
 }
 
+// Stages the contributions of labels below nh in LDS (every thread of the
+// block takes part; the caller then waits at a block barrier).
+__device__ __forceinline__ void pf_stage_hot(const PfArgs &a, float *s_hot) {
+  constexpr int PER = PF_HOT / 4 / PF_THREADS;
+  const uint32_t n4 = a.nh / 4;
+  const float4 *cb4 = reinterpret_cast<const float4 *>(a.cb);
+  float4 *s4 = reinterpret_cast<float4 *>(s_hot);
+  float4 tv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t i = threadIdx.x + k * PF_THREADS;
+    tv[k] = i < n4 ? cb4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t i = threadIdx.x + k * PF_THREADS;
+    if (i < n4) s4[i] = tv[k];
+  }
+  for (uint32_t i = n4 * 4 + threadIdx.x; i < a.nh; i += PF_THREADS) s_hot[i] = a.cb[i];
+}
+
+// Cold gathers of one tile: raw buffer loads of the contributions of labels
+// at or above nh; LDS-served and padding entries get an out-of-range offset
+// (the load returns 0 without touching L1/L2).
+__device__ __forceinline__ void pf_gather(__amdgpu_buffer_rsrc_t rcb, uint32_t nh,
+                                          const uint32_t (&e)[PF_K], float (&gv)[PF_K]) {
+#pragma unroll
+  for (uint32_t k = 0; k < PF_K; ++k) {
+    const uint32_t w = e[k] & ~PF_FLAG;
+    const uint32_t off = w < nh || w == PF_PAD ? PF_OOB : w * 4u;
+    gv[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rcb, (int)off, 0, 0));
+  }
+}
+
+// Row sums of tile t from its entries (cur), their LDS-staged values (hv,
+// used for labels below nh) and their gathered values (gv): rows begun and
+// ended in the tile go to S, the tile's first-row and last-row partials to
+// P[t] and Q[t].
+__device__ __forceinline__ void pf_tile_sums(const PfArgs &a, uint32_t t, int32_t tb_cur,
+                                             const uint32_t (&cur)[PF_K],
+                                             const float (&hv)[PF_K], const float (&gv)[PF_K],
+                                             uint32_t lane, uint32_t comp) {
+  uint32_t fm = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < PF_K; ++k) fm |= (cur[k] >> 31) << k;
+  const uint32_t nst = (uint32_t)__popc(fm);
+  const uint32_t incl = wave_incl_scan<uint32_t>(nst);
+  // nonempty row in progress at the start of this lane's entries
+  int32_t row = tb_cur + (int32_t)(incl - nst);
+  float s = 0.f, c = 0.f, hs = 0.f, hc = 0.f;
+  bool seen = false;
+#pragma unroll
+  for (uint32_t k = 0; k < PF_K; ++k) {
+    if ((fm >> k) & 1u) {
+      if (seen) {
+        a.S[row] = make_float2(s, c);  // row begun and ended in this lane
+      } else {
+        hs = s;
+        hc = c;
+      }
+      ++row;
+      s = 0.f;
+      c = 0.f;
+      seen = true;
+    }
+    const uint32_t w = cur[k] & ~PF_FLAG;
+    const float g = w < a.nh ? hv[k] : gv[k];
+    if (w != PF_PAD) {
+      if (comp) kahan_add(s, c, g);
+      else s += g;
+    }
+  }
+  if (!seen) {
+    hs = s;
+    hc = c;
+  }
+  // segmented inclusive scan over lanes of the value carried out of each
+  // lane: its last row's partial (if a row starts in it) or its whole sum
+  float vs = seen ? s : hs, vc = seen ? c : hc;
+  uint32_t vf = seen ? 1u : 0u;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const float os = __shfl_up(vs, d, WAVE), oc = __shfl_up(vc, d, WAVE);
+    const uint32_t of = __shfl_up(vf, d, WAVE);
+    if (lane >= (uint32_t)d && !vf) {
+      float ns = os, nc = oc;
+      pf_join(ns, nc, vs, vc, comp);
+      vs = ns;
+      vc = nc;
+      vf = of;
+    }
+  }
+  float cs = __shfl_up(vs, 1, WAVE), cc = __shfl_up(vc, 1, WAVE);
+  uint32_t cf = __shfl_up(vf, 1, WAVE);
+  if (lane == 0) {
+    cs = 0.f;
+    cc = 0.f;
+    cf = 0u;
+  }
+  if (seen) {
+    // the row in progress at this lane's start ends before its first flag
+    pf_join(cs, cc, hs, hc, comp);
+    const int32_t r0 = tb_cur + (int32_t)(incl - nst);
+    if (cf) a.S[r0] = make_float2(cs, cc);        // begun in an earlier lane
+    else a.P[t] = make_float2(cs, cc);            // the tile's first row
+  }
+  const float ls = __shfl(vs, WAVE - 1, WAVE), lc = __shfl(vc, WAVE - 1, WAVE);
+  const uint32_t lf = __shfl(vf, WAVE - 1, WAVE);
+  if (lane == 0) {
+    a.Q[t] = make_float2(ls, lc);
+    if (!lf) a.P[t] = make_float2(ls, lc);  // no row starts in this tile
+  }
+}
+
 __global__ __launch_bounds__(PF_THREADS) void k_pf_tiles(PfArgs a) {
   __shared__ float s_hot[PF_HOT];
   if (a.ctl->done) return;
-  {
-    constexpr int PER = PF_HOT / 4 / PF_THREADS;
-    const uint32_t n4 = a.nh / 4;
-    const float4 *cb4 = reinterpret_cast<const float4 *>(a.cb);
-    float4 *s4 = reinterpret_cast<float4 *>(s_hot);
-    float4 tv[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const uint32_t i = threadIdx.x + k * PF_THREADS;
-      tv[k] = i < n4 ? cb4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const uint32_t i = threadIdx.x + k * PF_THREADS;
-      if (i < n4) s4[i] = tv[k];
-    }
-    for (uint32_t i = n4 * 4 + threadIdx.x; i < a.nh; i += PF_THREADS) s_hot[i] = a.cb[i];
-  }
+  pf_stage_hot(a, s_hot);
   __syncthreads();
   // gfx9 buffer descriptor: raw (stride 0), range V*4 bytes
   const __amdgpu_buffer_rsrc_t rcb =
@@ -178,11 +275,10 @@ __global__ __launch_bounds__(PF_THREADS) void k_pf_tiles(PfArgs a) {
   int32_t tb_cur = a.tbase[gw];
   for (uint32_t t = gw; t < a.n_tiles; t += W) {
     float hv[PF_K], gv[PF_K];
+    pf_gather(rcb, a.nh, cur, gv);
 #pragma unroll
     for (uint32_t k = 0; k < PF_K; ++k) {
       const uint32_t w = cur[k] & ~PF_FLAG;
-      const uint32_t off = w < a.nh || w == PF_PAD ? PF_OOB : w * 4u;
-      gv[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rcb, (int)off, 0, 0));
       hv[k] = s_hot[w < PF_HOT ? w : PF_HOT - 1];
     }
     // the next tile's entries are requested after this tile's gathers:
@@ -195,83 +291,14 @@ __global__ __launch_bounds__(PF_THREADS) void k_pf_tiles(PfArgs a) {
     pf_load(a.cols, tn, lane, nxt);
     const int32_t tb_nxt = a.tbase[tn];
     __builtin_amdgcn_sched_barrier(0);
-    uint32_t fm = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < PF_K; ++k) fm |= (cur[k] >> 31) << k;
-    const uint32_t nst = (uint32_t)__popc(fm);
-    const uint32_t incl = wave_incl_scan<uint32_t>(nst);
-    // nonempty row in progress at the start of this lane's entries
-    int32_t row = tb_cur + (int32_t)(incl - nst);
-    float s = 0.f, c = 0.f, hs = 0.f, hc = 0.f;
-    bool seen = false;
-#pragma unroll
-    for (uint32_t k = 0; k < PF_K; ++k) {
-      if ((fm >> k) & 1u) {
-        if (seen) {
-          a.S[row] = make_float2(s, c);  // row begun and ended in this lane
-        } else {
-          hs = s;
-          hc = c;
-        }
-        ++row;
-        s = 0.f;
-        c = 0.f;
-        seen = true;
-      }
-      const uint32_t w = cur[k] & ~PF_FLAG;
-      const float g = w < a.nh ? hv[k] : gv[k];
-      if (w != PF_PAD) {
-        if (comp) kahan_add(s, c, g);
-        else s += g;
-      }
-    }
-    if (!seen) {
-      hs = s;
-      hc = c;
-    }
-    // segmented inclusive scan over lanes of the value carried out of each
-    // lane: its last row's partial (if a row starts in it) or its whole sum
-    float vs = seen ? s : hs, vc = seen ? c : hc;
-    uint32_t vf = seen ? 1u : 0u;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      const float os = __shfl_up(vs, d, WAVE), oc = __shfl_up(vc, d, WAVE);
-      const uint32_t of = __shfl_up(vf, d, WAVE);
-      if (lane >= (uint32_t)d && !vf) {
-        float ns = os, nc = oc;
-        pf_join(ns, nc, vs, vc, comp);
-        vs = ns;
-        vc = nc;
-        vf = of;
-      }
-    }
-    float cs = __shfl_up(vs, 1, WAVE), cc = __shfl_up(vc, 1, WAVE);
-    uint32_t cf = __shfl_up(vf, 1, WAVE);
-    if (lane == 0) {
-      cs = 0.f;
-      cc = 0.f;
-      cf = 0u;
-    }
-    if (seen) {
-      // the row in progress at this lane's start ends before its first flag
-      pf_join(cs, cc, hs, hc, comp);
-      const int32_t r0 = tb_cur + (int32_t)(incl - nst);
-      if (cf) a.S[r0] = make_float2(cs, cc);        // begun in an earlier lane
-      else a.P[t] = make_float2(cs, cc);            // the tile's first row
-    }
-    const float ls = __shfl(vs, WAVE - 1, WAVE), lc = __shfl(vc, WAVE - 1, WAVE);
-    const uint32_t lf = __shfl(vf, WAVE - 1, WAVE);
-    if (lane == 0) {
-      a.Q[t] = make_float2(ls, lc);
-      if (!lf) a.P[t] = make_float2(ls, lc);  // no row starts in this tile
-    }
+    pf_tile_sums(a, t, tb_cur, cur, hv, gv, lane, comp);
 #pragma unroll
     for (uint32_t k = 0; k < PF_K; ++k) cur[k] = nxt[k];
     tb_cur = tb_nxt;
   }
 }
 
-__device__ void pf_long_rows(const PfArgs &a, uint32_t blk);
+__device__ double pf_long_rows(const PfArgs &a, uint32_t blk);
 
 // Row sums assembled in tile order, the update, residual partials per block.
 // Blocks from PF_ROW_BLOCKS on take the long rows (pf_long_rows), so both
@@ -279,7 +306,8 @@ __device__ void pf_long_rows(const PfArgs &a, uint32_t blk);
 __global__ __launch_bounds__(PF_ROW_THREADS) void k_pf_rows(PfArgs a) {
   if (a.ctl->done) return;
   if (blockIdx.x >= PF_ROW_BLOCKS) {
-    pf_long_rows(a, blockIdx.x - PF_ROW_BLOCKS);
+    const double bs = pf_long_rows(a, blockIdx.x - PF_ROW_BLOCKS);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
     return;
   }
   double diff = 0.0;
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(PF_ROW_THREADS) void k_pf_rows(PfArgs a) {
 // Long rows, one wavefront each: lane l joins the tile partials P[ta+1+l],
 // P[ta+1+l+64], ... in order, the lanes are joined by a fixed butterfly, and
 // the first tile's partial Q[ta] goes in front.
-__device__ void pf_long_rows(const PfArgs &a, uint32_t blk) {
+__device__ double pf_long_rows(const PfArgs &a, uint32_t blk) {
   const uint32_t lane = lane_id();
   const uint32_t comp = a.compensated;
   const uint32_t nw = PF_LONG_BLOCKS * (PF_ROW_THREADS / WAVE);
@@ -358,8 +386,7 @@ __device__ void pf_long_rows(const PfArgs &a, uint32_t blk) {
       diff += fabs((double)xv - (double)a.x[v]);
     }
   }
-  const double bs = block_sum<double, PF_ROW_THREADS>(diff);
-  if (threadIdx.x == 0) a.partials[PF_ROW_BLOCKS + blk] = bs;
+  return block_sum<double, PF_ROW_THREADS>(diff);
 }
 
 // Fixed-order FP64 residual, then the stopping rule (pagerank.cu:56): done
@@ -520,19 +547,30 @@ gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int 
   a.eps = o.eps;
   const uint32_t tile_grid = w.pf_grid ? w.pf_grid : (uint32_t)pf_device_cus();
   if (iter_ms) {
-    while (w.ev.size() < 2 * (size_t)o.max_iter) {
+    // ev[0] before the first iteration, ev[t + 1] after iteration t
+    while (w.ev.size() < (size_t)o.max_iter + 1) {
       hipEvent_t e;
       GV_HIP(hipEventCreate(&e));
       w.ev.push_back(e);
     }
+    GV_HIP(hipEventRecord(w.ev[0], s));
+  }
+  if (w.ev_grp[0] == nullptr) {
+    for (int i = 0; i < 2; ++i)
+      GV_HIP(hipEventCreateWithFlags(&w.ev_grp[i], hipEventDisableTiming));
   }
   int launched = 0;
-  // with a yield hook each iteration is checked before the next is issued
-  const int check_every = o.yield ? 1 : 8;
+  // Without a yield hook, iterations go out in groups of PF_GROUP; after each
+  // group the done flag is copied to a host slot, and the host reads the
+  // previous group's copy, so the device always has a group queued while the
+  // host checks.  With a yield hook each iteration is checked before the
+  // next is issued.
+  constexpr int PF_GROUP = 8;
+  const int group = o.yield ? 1 : PF_GROUP;
+  int pending = -1;  // host slot of a group copy not yet read
   while (launched < o.max_iter) {
     const int t = launched;
     if (o.yield) o.yield(o.yield_arg);
-    if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t], s));
     a.x = w.x[t & 1];
     a.cb = w.contrib[t & 1];
     a.xn = w.x[(t + 1) & 1];
@@ -540,12 +578,22 @@ gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int 
     if (n_tiles) k_pf_tiles<<<tile_grid, PF_THREADS, 0, s>>>(a);
     k_pf_rows<<<PF_ROW_BLOCKS + PF_LONG_BLOCKS, PF_ROW_THREADS, 0, s>>>(a);
     k_pf_resid<<<1, 1024, 0, s>>>(a, PF_ROW_BLOCKS + PF_LONG_BLOCKS);
-    if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
+    if (iter_ms) GV_HIP(hipEventRecord(w.ev[t + 1], s));
     ++launched;
-    if (launched % check_every == 0 && launched < o.max_iter) {
-      GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
-      GV_HIP(hipStreamSynchronize(s));
-      if (w.h_ctl->done) break;
+    if (launched % group == 0 && launched < o.max_iter) {
+      const int slot = 1 + (launched / group & 1);
+      GV_HIP(hipMemcpyAsync(&w.h_ctl[slot], w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+      GV_HIP(hipEventRecord(w.ev_grp[slot - 1], s));
+      if (o.yield) {
+        GV_HIP(hipEventSynchronize(w.ev_grp[slot - 1]));
+        if (w.h_ctl[slot].done) break;
+      } else {
+        if (pending > 0) {
+          GV_HIP(hipEventSynchronize(w.ev_grp[pending - 1]));
+          if (w.h_ctl[pending].done) break;
+        }
+        pending = slot;
+      }
     }
   }
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
@@ -555,13 +603,11 @@ gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int 
   if (iters) *iters = (int)it;
   if (residual) *residual = w.h_ctl->residual;
   if (iter_ms) {
-    double tot = 0.0;
-    for (uint32_t t = 0; t < it && (int)t < launched; ++t) {
-      float ms = 0.f;
-      GV_HIP(hipEventElapsedTime(&ms, w.ev[2 * t], w.ev[2 * t + 1]));
-      tot += ms;
-    }
-    *iter_ms = tot;
+    // the counted iterations, launch gaps between them included
+    float ms = 0.f;
+    const int last = (int)it < launched ? (int)it : launched;
+    if (last > 0) GV_HIP(hipEventElapsedTime(&ms, w.ev[0], w.ev[last]));
+    *iter_ms = ms;
   }
   return GV_OK;
 }

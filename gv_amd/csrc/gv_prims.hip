@@ -278,6 +278,9 @@ constexpr uint64_t OS_MAX_KEYS = 1ull << 24;
 constexpr int OS_MAX_PASSES = 8;
 constexpr uint32_t OS_SPIN_LIMIT = 1u << 26;  // a bug shows as wrong output, not a hang
 constexpr int OS_LOOK = 8;                    // predecessors read per look-back round trip
+// keys per thread of a single-sweep tile
+constexpr int OS_ITEMS = 16;
+constexpr uint64_t OS_TILE = SORT_THREADS * OS_ITEMS;
 
 // Single-sweep workspace (32-bit words): ghist[8][256], tile counters[8],
 // error word, then status[passes][nt][256].
@@ -299,17 +302,29 @@ __global__ void k_os_clear(uint32_t *w, uint64_t n) {
     w[i] = 0;
 }
 
-// Global digit counts of every pass, from the keys in any order.
+// Global digit counts of every pass, from the keys in any order.  A block
+// takes chunks of SORT_THREADS * OS_ITEMS keys and loads a chunk's keys
+// before counting them, so their latencies overlap.
 __global__ __launch_bounds__(SORT_THREADS) void k_os_hist(const uint64_t *keys, uint64_t n,
                                                           uint32_t lo, uint32_t passes,
                                                           uint32_t *ghist) {
   __shared__ uint32_t cnt[OS_MAX_PASSES][256];
   for (uint32_t i = threadIdx.x; i < OS_MAX_PASSES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
   __syncthreads();
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = keys[i];
-    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&cnt[p][(uint32_t)(k >> (lo + 8 * p)) & 255u], 1u);
+  for (uint64_t base = (uint64_t)blockIdx.x * OS_TILE; base < n;
+       base += (uint64_t)gridDim.x * OS_TILE) {
+    uint64_t kk[OS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < OS_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * SORT_THREADS + threadIdx.x;
+      kk[j] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < OS_ITEMS; ++j) {
+      if (base + (uint64_t)j * SORT_THREADS + threadIdx.x >= n) continue;
+      for (uint32_t p = 0; p < passes; ++p)
+        atomicAdd(&cnt[p][(uint32_t)(kk[j] >> (lo + 8 * p)) & 255u], 1u);
+    }
   }
   __syncthreads();
   for (uint32_t p = 0; p < passes; ++p) {
@@ -343,14 +358,22 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
   const uint32_t tile = s_tile;
   const int w = threadIdx.x / WAVE;
   const uint32_t lane = lane_id();
-  const uint64_t base = (uint64_t)tile * SORT_TILE + (uint64_t)w * (WAVE * SORT_ITEMS);
-  uint64_t key[SORT_ITEMS];
-  uint32_t rnk[SORT_ITEMS];
+  const uint64_t base = (uint64_t)tile * OS_TILE + (uint64_t)w * (WAVE * OS_ITEMS);
+  // this thread's digit total and the tile's keys, all requested before the
+  // ranking so their latencies overlap
+  const uint32_t gd = ghist[threadIdx.x];
+  uint64_t key[OS_ITEMS];
+  uint32_t rnk[OS_ITEMS];
 #pragma unroll
-  for (int j = 0; j < SORT_ITEMS; ++j) {
+  for (int j = 0; j < OS_ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * WAVE + lane;
+    key[j] = i < n ? in[i] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < OS_ITEMS; ++j) {
     uint64_t i = base + (uint64_t)j * WAVE + lane;
     bool valid = i < n;
-    uint64_t k = valid ? in[i] : 0;
+    uint64_t k = key[j];
     uint32_t d = (uint32_t)(k >> shift) & 255u;
     unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -362,7 +385,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
     if (valid) old = wcnt[w][d];
     uint32_t below = mask_lt_count(peers);
     if (valid && below == 0) wcnt[w][d] = old + (uint32_t)__popcll(peers);
-    key[j] = k;
     rnk[j] = old + below;
   }
   __syncthreads();
@@ -412,7 +434,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
     }
     // digit base: counts of all smaller digits in this pass
     uint32_t all;
-    uint32_t acc = block_excl_scan<uint32_t, SORT_THREADS>(ghist[d], all) + excl;
+    uint32_t acc = block_excl_scan<uint32_t, SORT_THREADS>(gd, all) + excl;
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
       uint32_t c = wcnt[i][d];
@@ -423,7 +445,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
   __syncthreads();
   if (s_bad) return;
 #pragma unroll
-  for (int j = 0; j < SORT_ITEMS; ++j) {
+  for (int j = 0; j < OS_ITEMS; ++j) {
     uint64_t i = base + (uint64_t)j * WAVE + lane;
     if (i < n) {
       uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
@@ -440,7 +462,7 @@ size_t sort_tmp_bytes(uint64_t n) {
   uint64_t h = 256 * nt + 1;
   size_t classic = h * sizeof(uint32_t) + 256 + scan_tmp_bytes(h);
   const uint64_t ns = n < OS_MAX_KEYS ? n : OS_MAX_KEYS;
-  size_t sweep = os_words(cdiv(ns ? ns : 1, SORT_TILE), OS_MAX_PASSES) * sizeof(uint32_t);
+  size_t sweep = os_words(cdiv(ns ? ns : 1, OS_TILE), OS_MAX_PASSES) * sizeof(uint32_t);
   return classic > sweep ? classic : sweep;
 }
 
@@ -452,14 +474,14 @@ uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *t
 uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
                           void *tmp, hipStream_t s) {
   if (n <= 1) return a;
-  uint64_t nt = cdiv(n, SORT_TILE);
   const uint32_t passes = (hi - lo + 7) / 8;
   if (n <= OS_MAX_KEYS && passes <= (uint32_t)OS_MAX_PASSES) {
+    const uint64_t nt = cdiv(n, OS_TILE);
     const OsTmp t = os_tmp(tmp);
     const uint64_t words = os_words(nt, passes);
     k_os_clear<<<dim3((unsigned)(cdiv(words, 256) < 1024 ? cdiv(words, 256) : 1024)), 256, 0, s>>>(
         t.ghist, words);
-    const uint64_t hb = cdiv(n, SORT_THREADS * 16);
+    const uint64_t hb = cdiv(n, OS_TILE);
     k_os_hist<<<dim3((unsigned)(hb < 1024 ? hb : 1024)), dim3(SORT_THREADS), 0, s>>>(a, n, lo,
                                                                                      passes, t.ghist);
     uint64_t *src = a, *dst = b;
@@ -473,6 +495,7 @@ uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uin
     }
     return src;
   }
+  const uint64_t nt = cdiv(n, SORT_TILE);
   uint64_t h = 256 * nt;
   uint32_t *hist = (uint32_t *)tmp;
   void *stmp = (void *)(((uintptr_t)(hist + h + 1) + 255) & ~(uintptr_t)255);
