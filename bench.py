@@ -462,6 +462,9 @@ def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):
         hip().hipMemcpy(ctypes.c_void_p(cd.ptr.value + o), d.ptr, half * 4, D2D)
         offsets.append(offsets[-1] + half)
     kinds = [p[2] for p in parts]
+    # staging sized once for the group, as the plain form's is by its warm-up
+    # batches (a batching caller reserves once; captured pipelines must)
+    g2.reserve(n, len(parts))
     g2.sync()
     synchronize()
     stream = g2.stream()

@@ -259,7 +259,10 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
   const uint32_t bV = id_bits(st.V), sb = id_bits(nb);
   k_stage_seq<<<grid_for(n), 256, 0, s>>>(src, dst, n, boff, nb, st.V, bV, sb, mirror, swap,
                                           sc.k0, st.ctl);
-  uint64_t *sorted = sort_keys(sc.k0, sc.k1, m, 2 * bV + sb, sc.tmp, s);
+  // the keys are staged in batch order, so a stable sort on the (u, v) bits
+  // alone leaves equal edges in batch order: the same array as a sort on the
+  // whole key, with one pass fewer when the tag crosses a digit boundary
+  uint64_t *sorted = sort_keys_range(sc.k0, sc.k1, m, sb, 2 * bV + sb, sc.tmp, s);
   uint64_t *other = sorted == sc.k0 ? sc.k1 : sc.k0;
   k_last_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.segof);
   scan_u32(sc.segof, sc.rank, m, nullptr, sc.tmp, s);
