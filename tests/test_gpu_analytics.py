@@ -91,6 +91,31 @@ def test_pagerank_config1_after_updates(directed, layout):
     check_pagerank(g, o)
 
 
+def test_pagerank_iteration_limits_and_repeated_calls():
+    """The flat form issues iterations in groups of 8 and reads the done flag
+    one group behind: the iteration count must still stop exactly at max_iter
+    on either side of a group boundary, and the launches queued after a stop
+    must leave the next call on the same handle unaffected."""
+    V = 1 << 12
+    s, d = O.gen_rmat(O.stream_key(5, O.STREAM_RMAT), 12, 16 * V)
+    g, o = build_pair(V, False, s, d)
+    for max_iter in (1, 7, 8, 9, 15, 16, 17, 25):
+        # eps 0 never converges: every call runs to its limit
+        x, it, _ = G.static_pagerank(g, 0.85, 0.0, max_iter)
+        assert it == max_iter
+        xr, _, _ = o.pagerank(0.85, 0.0, max_iter, fixed_iters=max_iter)
+        assert float(np.abs(x.astype(np.float64) - xr).sum()) <= PR_L1
+    # converging calls back to back: same answer every time
+    first = None
+    for _ in range(3):
+        l1, it = check_pagerank(g, o)
+        x, it2, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+        assert it2 == it
+        if first is None:
+            first = x
+        assert np.array_equal(x, first)
+
+
 @pytest.mark.parametrize("layout", [1, 2, 3, 4])
 def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     # small thresholds push vertices through every bin incl. chunked hubs
