@@ -133,9 +133,10 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
  * max_iter updates.  out[V] receives x. */
 gv_status gv_pagerank(gv_graph *g, float alpha, double eps, int max_iter, float *out, int mem,
                       int *iters, double *residual);
-/* gv_pagerank plus measurement: kernel_ms is the summed device time of the
- * iteration kernels (HIP events on the handle's stream); edges is the live
- * in-adjacency size each iteration reads. */
+/* gv_pagerank plus measurement: kernel_ms is the device time of the counted
+ * iterations (HIP events on the handle's stream; in the flat layout one span
+ * from the first iteration's launch to the end of the last counted one, launch
+ * gaps included); edges is the live in-adjacency size each iteration reads. */
 typedef struct gv_pagerank_stats {
   int iters;
   double residual;
