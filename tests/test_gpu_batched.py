@@ -106,3 +106,47 @@ def test_group_rejects_bad_ids_atomically():
     assert e.value.status == G.GV_E_RANGE
     assert_same(g, o)
     g.close()
+
+
+def test_group_above_single_sweep_limit_equals_sequential():
+    """A group of more than 2^24 directed keys takes the classic radix path
+    and sorts only the (source, destination) bits: equal edges must still
+    come out in batch order, so the last operation on an edge wins.  Inserts
+    re-add edges deleted by the batch before, deletes hit edges inserted by
+    the batch before."""
+    V = 1 << 16
+    s, d = O.gen_rmat(O.stream_key(9, O.STREAM_RMAT), 16, 16 * V)
+    o = O.OracleGraph(V, False)
+    g = make(V, False, pool=64 * V)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    rng = np.random.default_rng(9)
+    srcs, dsts, offs, kinds = [], [], [0], []
+    prev = None
+    n = 300_000
+    for b in range(40):
+        if b % 2 == 0:
+            bs, bd = O.gen_uniform(O.stream_key(9, 30_000 + b), V, n)
+            if prev is not None:
+                k = len(prev[0]) // 4
+                bs[:k], bd[:k] = prev[0][:k], prev[1][:k]
+            kinds.append(G.GV_OP_INSERT)
+            o.insert(bs, bd)
+        else:
+            eo, ec = o.export_csr()
+            bs, bd = O.gen_sample(O.stream_key(9, 40_000 + b), V, eo, ec, n // 2)
+            k = min(len(bs), n // 8)
+            pick = rng.choice(len(prev[0]), k, replace=False)
+            bs = np.concatenate([bs, prev[0][pick]])
+            bd = np.concatenate([bd, prev[1][pick]])
+            kinds.append(G.GV_OP_DELETE)
+            o.delete(bs, bd)
+        prev = (bs.copy(), bd.copy())
+        srcs.append(bs)
+        dsts.append(bd)
+        offs.append(offs[-1] + len(bs))
+    assert 2 * offs[-1] > (1 << 24)
+    g.applyBatches(np.concatenate(srcs), np.concatenate(dsts), np.array(offs, np.uint64), kinds,
+                   compact_fraction=0.25)
+    assert_same(g, o, "group above the single-sweep limit")
+    g.close()
