@@ -229,6 +229,7 @@ struct BcmWork {
   double *sigma = nullptr;
   float *delta = nullptr, *bcc = nullptr;
   float2 *part = nullptr;
+  uint32_t *owner = nullptr;  // per item of a level: its frontier index (capacity items)
   uint2 *sn = nullptr;  // per vertex: (seen, next) source masks of the forward sweep
   uint32_t *queue = nullptr, *cnt = nullptr, *coff = nullptr, *dsrc = nullptr;
   unsigned int *cursor = nullptr, *h_cursor = nullptr;

@@ -290,14 +290,12 @@ __global__ void k_bcm_count(Store st, const uint32_t *q, uint32_t nf, uint32_t *
   }
 }
 
-// Frontier index j of item i: the largest j < nf with coff[j] <= i.
-__device__ __forceinline__ uint32_t bcm_owner(const uint32_t *coff, uint32_t nf, uint32_t i) {
-  uint32_t lo = 0, hi = nf;
-  while (hi - lo > 1) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (coff[mid] <= i) lo = mid; else hi = mid;
-  }
-  return lo;
+// owner[i] = frontier index j of item i (coff[j] <= i < coff[j + 1]), so a
+// wavefront finds its item's vertex with one load.
+__global__ void k_bcm_owner(const uint32_t *coff, uint32_t nf, uint32_t *owner) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nf;
+       j += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = coff[j], e = coff[j + 1]; i < e; ++i) owner[i] = (uint32_t)j;
 }
 
 // One item per lane group: the group's vertex, its slot range [c0, c1), and
@@ -310,11 +308,11 @@ struct BcmItem {
 
 __device__ __forceinline__ BcmItem bcm_item(const Store &st, const int *dist, const double *sigma,
                                             const uint32_t *queue, uint32_t qs,
-                                            const uint32_t *coff, uint32_t nf, uint64_t gi,
-                                            bool have, uint32_t s) {
+                                            const uint32_t *coff, const uint32_t *owner,
+                                            uint64_t gi, bool have, uint32_t s) {
   BcmItem it{0, 0, 0, 0, -2, 0.0};
   if (have) {
-    const uint32_t j = bcm_owner(coff, nf, (uint32_t)gi);
+    const uint32_t j = owner[gi];
     it.v = queue[qs + j];
     const uint32_t us = st.used[it.v];
     it.tbl = st.tbl_off[it.v];
@@ -336,7 +334,8 @@ __device__ __forceinline__ BcmItem bcm_item(const Store &st, const int *dist, co
 // of a 32-source distance row.
 __global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *sigma, uint2 *sn,
                                                  uint32_t *queue, uint32_t qs, uint32_t qe, int d,
-                                                 unsigned int *cursor, const uint32_t *coff) {
+                                                 unsigned int *cursor, const uint32_t *coff,
+                                                 const uint32_t *owner) {
   __shared__ double s_sig[4][BCM_S];
   const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
@@ -344,7 +343,7 @@ __global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *si
   const uint32_t nitems = coff[nf];
   for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < nitems;
        it += nwaves) {
-    const uint32_t j = bcm_owner(coff, nf, (uint32_t)it);
+    const uint32_t j = owner[it];
     const uint32_t v = queue[qs + j];
     const uint32_t us = st.used[v], tbl = st.tbl_off[v];
     const uint32_t c0 = ((uint32_t)it - coff[j]) * BCM_CH;
@@ -426,7 +425,8 @@ __device__ __forceinline__ void bcm_merge(float &s, float &c, float s2, float c2
 __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, const double *sigma,
                                                  const float *delta, const uint2 *sn,
                                                  const uint32_t *queue, uint32_t qs, uint32_t qe,
-                                                 int d, const uint32_t *coff, float2 *part) {
+                                                 int d, const uint32_t *coff,
+                                                 const uint32_t *owner, float2 *part) {
   const uint32_t lane = lane_id();
   const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
   const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, cons
        g0 < nitems; g0 += ngroups) {
     const uint64_t gi = g0 + sub;
     const bool have = gi < nitems;
-    const BcmItem it = bcm_item(st, dist, sigma, queue, qs, coff, nf, gi, have, s);
+    const BcmItem it = bcm_item(st, dist, sigma, queue, qs, coff, owner, gi, have, s);
     const bool act = it.dv == d;
     float ds = 0.f, dc = 0.f;
     const uint32_t span = it.c1 - it.c0;
@@ -533,6 +533,7 @@ void bcm_release(BcmWork &w) {
   hipFree(w.delta);
   hipFree(w.bcc);
   hipFree(w.part);
+  hipFree(w.owner);
   hipFree(w.sn);
   hipFree(w.queue);
   hipFree(w.cnt);
@@ -587,10 +588,13 @@ static gv_status bcm_reserve(BcmWork &w, uint32_t V, uint64_t max_items) {
   }
   if (max_items > w.items) {
     hipFree(w.part);
+    hipFree(w.owner);
     w.part = nullptr;
+    w.owner = nullptr;
     w.items = 0;
     const uint64_t cap = max_items + max_items / 4;  // room for graphs that grow
-    if ((e = hipMalloc(&w.part, cap * BCM_S * sizeof(float2))) != hipSuccess)
+    if ((e = hipMalloc(&w.part, cap * BCM_S * sizeof(float2))) != hipSuccess ||
+        (e = hipMalloc(&w.owner, cap * sizeof(uint32_t))) != hipSuccess)
       return hip_fail(e, "bc alloc");
     w.items = cap;
   }
@@ -628,7 +632,7 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
     float *delta = w.delta, *bcc = w.bcc;
     float2 *part = w.part;
     uint2 *sn = w.sn;
-    uint32_t *queue = w.queue, *cnt = w.cnt, *coff = w.coff, *dsrc = w.dsrc;
+    uint32_t *queue = w.queue, *cnt = w.cnt, *coff = w.coff, *dsrc = w.dsrc, *owner = w.owner;
     unsigned int *cursor = w.cursor, *h_cursor = w.h_cursor;
     void *tmp = w.tmp;
     // item offsets of the frontier segment [qs, qe); a level's frontier holds
@@ -637,6 +641,7 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
       const uint32_t nf = qe - qs;
       k_bcm_count<<<grid_for(nf), 256, 0, s>>>(st, queue + qs, nf, cnt);
       scan_u32(cnt, coff, nf, nullptr, tmp, s);
+      k_bcm_owner<<<grid_for(nf), 256, 0, s>>>(coff, nf, owner);
     };
     auto item_grid = [&](uint32_t nf) {
       const uint64_t bound = (uint64_t)nf + h_used / BCM_CH + 1;
@@ -672,7 +677,7 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
         fill_u32(cursor, 0, 1, s);
         level_items(qs, qe);
         k_bcm_fwd<<<fwd_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, sn, queue, qs, qe, d, cursor,
-                                                    coff);
+                                                    coff, owner);
         k_bcm_next<<<grid_for(qe - qs + h_used / BCM_CH + 1, 256, 4096), 256, 0, s>>>(queue, qe,
                                                                                    cursor, sn);
         if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
@@ -701,7 +706,7 @@ gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sour
         level_items(qs, qe);
         lmask(lev + 1, 1);
         k_bcm_bwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, delta, sn, queue, qs, qe,
-                                                     lev, coff, part);
+                                                     lev, coff, owner, part);
         lmask(lev + 1, 0);
         k_bcm_fin<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
             queue, qs, qe, lev, dist, delta, part, coff, dsrc, ns, out_dev, bcc);
