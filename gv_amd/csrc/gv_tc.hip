@@ -266,14 +266,14 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
         uint32_t h = tc_slot(x, shift);
         while (atomicCAS(&tab[h], TC_EMPTY, x) != TC_EMPTY) h = (h + 1) & hmask;
       }
-      // two wedges per lane and step (f and f + 64), so each lane has two
-      // independent loads in flight before it probes
+      // four wedges per lane and step (f, f + 64, ...), so each lane has
+      // four independent loads in flight before it probes (two: 2% slower)
       uint32_t k = 0;  // this lane's current block entry (advances monotonically)
-      for (uint32_t fb = 0; fb < nwed; fb += 2 * WAVE) {
-        uint32_t wv[2] = {TC_EMPTY, TC_EMPTY}, kv[2] = {0, 0};
-        uint64_t pv_pos[2] = {0, 0};
+      for (uint32_t fb = 0; fb < nwed; fb += 4 * WAVE) {
+        uint32_t wv[4] = {TC_EMPTY, TC_EMPTY, TC_EMPTY, TC_EMPTY}, kv[4] = {0, 0, 0, 0};
+        uint64_t pv_pos[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < 4; ++r) {
           const uint32_t f = fb + r * WAVE + lane;
           if (f < nwed) {
             while (s_pre[wid][k] <= f) ++k;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
           }
         }
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < 4; ++r) {
           if (wv[r] == TC_EMPTY) continue;
           uint32_t h = tc_slot(wv[r], shift);
           bool found = false;
