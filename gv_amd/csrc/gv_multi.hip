@@ -29,6 +29,11 @@ namespace gv {
 constexpr int PPR_COLS = 64;
 constexpr int PPR_THREADS = 256;
 constexpr uint32_t PPR_GRID = 2048;
+// Rows longer than PPR_HUB slots are summed in chunks of PPR_HUB by separate
+// wavefronts and merged in chunk order: with one wavefront per row, the
+// iteration waited for the wavefront holding the largest hub.
+constexpr uint32_t PPR_HUB = 4096;
+constexpr uint32_t PPR_FIN_GRID = 256;
 
 struct PprArgs {
   Store in;
@@ -67,6 +72,7 @@ __global__ __launch_bounds__(PPR_THREADS) void k_ppr_iter(PprArgs a, const float
     const uint64_t v = v0 + wid;
     if (v >= V) continue;  // uniform per wavefront
     const uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+    if (us > PPR_HUB) continue;  // chunked: k_ppr_hub, k_ppr_hubfin
     float s = 0.f, c = 0.f;
     for (uint32_t base = 0; base < us; base += WAVE) {
       const uint32_t n = us - base < (uint32_t)WAVE ? us - base : (uint32_t)WAVE;
@@ -105,6 +111,102 @@ __global__ __launch_bounds__(PPR_THREADS) void k_ppr_iter(PprArgs a, const float
   }
 }
 
+// Chunk counts of the rows longer than PPR_HUB slots (0 for the others).
+__global__ void k_ppr_hubcnt(Store in, uint32_t *cnt) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < in.V;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = in.used[v];
+    cnt[v] = u > PPR_HUB ? (u + PPR_HUB - 1) / PPR_HUB : 0u;
+  }
+}
+
+// iv[i] = row of chunk item i (coff = exclusive scan of the chunk counts).
+__global__ void k_ppr_hubitems(const uint32_t *coff, uint32_t V, uint32_t *iv) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t i = coff[v], e = coff[v + 1]; i < e; ++i) iv[i] = (uint32_t)v;
+}
+
+// One wavefront per chunk item, lane = column: the Kahan sum of the chunk's
+// contribution rows, in slot order.
+__global__ __launch_bounds__(PPR_THREADS) void k_ppr_hub(PprArgs a, const float *CB,
+                                                         const uint32_t *iv, const uint32_t *coff,
+                                                         uint32_t n_items, float2 *part) {
+  if (a.ctl->done) return;
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < n_items;
+       it += nwaves) {
+    const uint32_t v = iv[it];
+    const uint32_t us = a.in.used[v], tbl = a.in.tbl_off[v];
+    const uint32_t lo = ((uint32_t)it - coff[v]) * PPR_HUB;
+    const uint32_t hi = lo + PPR_HUB < us ? lo + PPR_HUB : us;
+    float s = 0.f, c = 0.f;
+    for (uint32_t base = lo; base < hi; base += WAVE) {
+      const uint32_t n = hi - base < (uint32_t)WAVE ? hi - base : (uint32_t)WAVE;
+      uint32_t slot = TOMB;
+      if (lane < n) slot = a.in.slots[slot_index(a.in.arena, tbl, base + lane, a.in.log2B, a.in.B)];
+      for (uint32_t t = 0; t < n; t += 8) {
+        uint32_t u[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) u[k] = __shfl(slot, (int)((t + k) & (WAVE - 1)), WAVE);
+        float g[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          g[k] = (t + k < n && u[k] != TOMB) ? CB[(uint64_t)u[k] * PPR_COLS + lane] : 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (t + k < n && u[k] != TOMB) kahan_add(s, c, g[k]);
+      }
+    }
+    part[it * PPR_COLS + lane] = make_float2(s, c);
+  }
+}
+
+// One wavefront per chunked row (its first chunk item): the chunk sums
+// joined in chunk order (TwoSum), the update, and per-column residual
+// partials in rows roff + blockIdx.x of a.partials.
+__global__ __launch_bounds__(PPR_THREADS) void k_ppr_hubfin(PprArgs a, const float *X, float *Xn,
+                                                            float *CBn, const uint32_t *iv,
+                                                            const uint32_t *coff, uint32_t n_items,
+                                                            const float2 *part, uint32_t roff) {
+  if (a.ctl->done) return;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  double diff = 0.0;
+  for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < n_items;
+       it += nwaves) {
+    const uint32_t v = iv[it];
+    if ((uint32_t)it != coff[v]) continue;  // not the row's first chunk
+    float s = 0.f, c = 0.f;
+    for (uint32_t i = coff[v], e = coff[v + 1]; i < e; ++i) {
+      const float2 q = part[(uint64_t)i * PPR_COLS + lane];
+      const float t = s + q.x;
+      const float bp = t - s;
+      const float er = (s - (t - bp)) + (q.x - bp);
+      s = t;
+      c = c + q.y - er;
+    }
+    const uint32_t seed = lane < a.ncols ? a.seeds[lane] : 0xFFFFFFFFu;
+    float xv = a.alpha * (s - c) + (v == seed ? 1.0f - a.alpha : 0.0f);
+    if (lane >= a.ncols) xv = 0.f;
+    const uint64_t idx = (uint64_t)v * PPR_COLS + lane;
+    const uint32_t d = a.deg_out[v];
+    diff += fabs((double)xv - (double)X[idx]);
+    Xn[idx] = xv;
+    CBn[idx] = d ? xv / (float)d : 0.f;
+  }
+  __shared__ double red[PPR_THREADS / WAVE][WAVE];
+  red[wid][lane] = diff;
+  __syncthreads();
+  if (wid == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < PPR_THREADS / WAVE; ++w) t += red[w][lane];
+    a.partials[(uint64_t)(roff + blockIdx.x) * PPR_COLS + lane] = t;
+  }
+}
+
 __global__ __launch_bounds__(PPR_COLS) void k_ppr_resid(PprArgs a, uint32_t nblocks) {
   if (a.ctl->done) return;
   const uint32_t c = threadIdx.x;
@@ -140,6 +242,9 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
   float *X[2] = {nullptr, nullptr}, *CB[2] = {nullptr, nullptr};
   double *partials = nullptr, *col_res = nullptr;
   PrCtl *ctl = nullptr, *h_ctl = nullptr;
+  uint32_t *hcnt = nullptr, *hoff = nullptr, *hiv = nullptr;
+  float2 *hpart = nullptr;
+  void *htmp = nullptr;
   std::vector<hipEvent_t> ev;
   gv_status rc = GV_OK;
   do {
@@ -148,7 +253,11 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
       e = hipMalloc(&X[i], blk);
       if (e == hipSuccess) e = hipMalloc(&CB[i], blk);
     }
-    if (e == hipSuccess) e = hipMalloc(&partials, (size_t)PPR_GRID * PPR_COLS * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMalloc(&partials, (size_t)(PPR_GRID + PPR_FIN_GRID) * PPR_COLS * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&hcnt, ((size_t)V + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&hoff, ((size_t)V + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&htmp, scan_tmp_bytes((uint64_t)V + 1));
     if (e == hipSuccess) e = hipMalloc(&col_res, PPR_COLS * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctl, sizeof(PrCtl));
     if (e == hipSuccess) e = hipHostMalloc(&h_ctl, sizeof(PrCtl), hipHostMallocDefault);
@@ -156,6 +265,27 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
       rc = hip_fail(e, "personalized PageRank alloc");
       break;
     }
+    // chunk items of the rows longer than PPR_HUB slots
+    k_ppr_hubcnt<<<grid_for(V), 256, 0, s>>>(in, hcnt);
+    scan_u32(hcnt, hoff, V, nullptr, htmp, s);
+    uint32_t n_items = 0;
+    if ((e = hipMemcpyAsync(&n_items, hoff + V, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      rc = hip_fail(e, "personalized PageRank");
+      break;
+    }
+    if (n_items) {
+      if ((e = hipMalloc(&hiv, (size_t)n_items * sizeof(uint32_t))) != hipSuccess ||
+          (e = hipMalloc(&hpart, (size_t)n_items * PPR_COLS * sizeof(float2))) != hipSuccess) {
+        rc = hip_fail(e, "personalized PageRank alloc");
+        break;
+      }
+      k_ppr_hubitems<<<grid_for(V), 256, 0, s>>>(hoff, V, hiv);
+    }
+    const uint32_t hub_grid = (uint32_t)std::min<uint64_t>(cdiv(n_items, PPR_THREADS / WAVE), 4096);
+    const uint32_t fin_grid =
+        (uint32_t)std::min<uint64_t>(cdiv(n_items, PPR_THREADS / WAVE), PPR_FIN_GRID);
     pr_ctl_reset(ctl, s);
     k_ppr_init<<<grid_for((uint64_t)V * PPR_COLS, 256, 16384), 256, 0, s>>>(X[0], CB[0], deg_out,
                                                                            seeds_dev, ncols, V);
@@ -184,7 +314,13 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
       }
       k_ppr_iter<<<grid, PPR_THREADS, 0, s>>>(a, X[t & 1], CB[t & 1], X[(t + 1) & 1],
                                               CB[(t + 1) & 1]);
-      k_ppr_resid<<<1, PPR_COLS, 0, s>>>(a, grid);
+      if (n_items) {
+        k_ppr_hub<<<hub_grid, PPR_THREADS, 0, s>>>(a, CB[t & 1], hiv, hoff, n_items, hpart);
+        k_ppr_hubfin<<<fin_grid, PPR_THREADS, 0, s>>>(a, X[t & 1], X[(t + 1) & 1],
+                                                      CB[(t + 1) & 1], hiv, hoff, n_items, hpart,
+                                                      grid);
+      }
+      k_ppr_resid<<<1, PPR_COLS, 0, s>>>(a, grid + fin_grid);
       if (iter_ms) hipEventRecord(ev.back(), s);
       if ((launched + 1) % 8 == 0) {
         if (hipMemcpyAsync(h_ctl, ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -231,6 +367,11 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
   hipFree(partials);
   hipFree(col_res);
   hipFree(ctl);
+  hipFree(hcnt);
+  hipFree(hoff);
+  hipFree(hiv);
+  hipFree(hpart);
+  hipFree(htmp);
   if (h_ctl) hipHostFree(h_ctl);
   return rc;
 }

@@ -36,6 +36,30 @@ def test_personalized_pagerank_block(directed, ncols):
     g.close()
 
 
+@pytest.mark.parametrize("directed", [False, True])
+def test_personalized_pagerank_chunked_hub_rows(directed):
+    """Rows longer than 4096 slots are summed in chunks by separate
+    wavefronts and merged in chunk order: two hubs of 5000 and 13000
+    in-neighbours (2 and 4 chunks), one of them a seed, over an R-MAT graph."""
+    V = 20000
+    rng = np.random.default_rng(3)
+    s, d = O.gen_rmat(O.stream_key(22, O.STREAM_RMAT), 14, 60000)
+    s, d = s % V, d % V
+    hub_src = [rng.choice(np.arange(2, V), 5000, replace=False).astype(np.uint32),
+               rng.choice(np.arange(2, V), 13000, replace=False).astype(np.uint32)]
+    s = np.concatenate([s, hub_src[0], hub_src[1]]).astype(np.uint32)
+    d = np.concatenate([d, np.zeros(5000, np.uint32), np.ones(13000, np.uint32)]).astype(np.uint32)
+    g, o = build_pair(V, directed, s, d)
+    seeds = np.concatenate([[1], rng.choice(np.arange(2, V), 63, replace=False)]).astype(np.uint32)
+    X, it, r = G.pagerank_personalized(g, seeds, 0.85, 1e-6, 100)
+    Xo, ito, _ = o.pagerank_personalized(seeds, 0.85, 1e-6, 100)
+    assert abs(it - ito) <= 1
+    Xr, _, _ = o.pagerank_personalized(seeds, 0.85, 1e-6, 100, fixed_iters=it)
+    l1 = np.abs(X.astype(np.float64) - Xr).sum(axis=0)
+    assert l1.max() <= 1e-6, l1.max()
+    g.close()
+
+
 def test_personalized_matches_global_when_averaged():
     V = 64
     s, d = O.gen_rmat(O.stream_key(22, O.STREAM_RMAT), 6, 500)
