@@ -201,6 +201,12 @@ struct BcWork {
   float *delta = nullptr, *bcc = nullptr;
   uint32_t *queue = nullptr;
   unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  // rows longer than BC_HUB slots of the level piece in flight, spread over
+  // the whole grid; backward partials per (hub, block)
+  uint32_t *hubs = nullptr;
+  unsigned int *nhub = nullptr;
+  float2 *hpart = nullptr;
+  uint64_t hpart_cap = 0;  // hubs hpart holds
   uint32_t max_blocks = 0;  // grid cap of the level kernels (0 = 8192)
   // Called before each level is issued, as PrOpts::yield.
   void (*yield)(void *) = nullptr;

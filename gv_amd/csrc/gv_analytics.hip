@@ -989,6 +989,12 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
 // Triangle counting: gv_tc.hip.
 
 // ========================================================= betweenness ==
+// Frontier rows longer than BC_HUB slots are taken out of the one-wavefront-
+// per-row level kernels and spread over a whole grid (BC_HUB_GRID blocks in
+// the backward sweep, one partial each).
+constexpr uint32_t BC_HUB = 8192;
+constexpr uint32_t BC_HUB_GRID = 128;
+
 __global__ void k_bc_init(int *dist, double *sigma, float *delta, uint32_t V) {
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (uint64_t)gridDim.x * blockDim.x) {
@@ -1014,14 +1020,19 @@ __global__ void k_bc_src(int *dist, double *sigma, uint32_t *queue, uint32_t src
 template <class Adj>
 __global__ __launch_bounds__(256) void k_bc_fwd(Adj adj, int *dist, double *sigma,
                                                 uint32_t *queue, uint32_t qs, uint32_t qe,
-                                                uint32_t qapp, int d, unsigned int *cursor) {
+                                                uint32_t qapp, int d, unsigned int *cursor,
+                                                uint32_t *hubs, unsigned int *nhub) {
   const uint32_t lane = lane_id();
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   const uint32_t nf = qe - qs;
   for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < nf; i += nwaves) {
     const uint32_t u = queue[qs + i];
-    const double su = sigma[u];
     const uint32_t us = adj.len(u);
+    if (us > BC_HUB) {  // k_bc_fwd_hub
+      if (lane == 0) hubs[atomicAdd(nhub, 1u)] = u;
+      continue;
+    }
+    const double su = sigma[u];
     const uint64_t b = adj.base(u);
     for (uint32_t base = 0; base < us; base += WAVE) {
       uint32_t j = base + lane;
@@ -1054,14 +1065,18 @@ template <class Adj>
 __global__ __launch_bounds__(256) void k_bc_bwd(Adj adj, const int *dist, const double *sigma,
                                                 float *delta, const uint32_t *queue, uint32_t qs,
                                                 uint32_t qe, int d, uint32_t src, float *bc,
-                                                float *bcc) {
+                                                float *bcc, uint32_t *hubs, unsigned int *nhub) {
   const uint32_t lane = lane_id();
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   const uint32_t nf = qe - qs;
   for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < nf; i += nwaves) {
     const uint32_t v = queue[qs + i];
-    const double sv = sigma[v];
     const uint32_t us = adj.len(v);
+    if (us > BC_HUB) {  // k_bc_bwd_hub, k_bc_bwd_hubfin
+      if (lane == 0) hubs[atomicAdd(nhub, 1u)] = v;
+      continue;
+    }
+    const double sv = sigma[v];
     const uint64_t b = adj.base(v);
     float s = 0.f, c = 0.f;
     for (uint32_t j = lane; j < us; j += WAVE) {
@@ -1089,6 +1104,128 @@ __global__ __launch_bounds__(256) void k_bc_bwd(Adj adj, const int *dist, const 
   }
 }
 
+// Hub rows of a forward level piece (listed by k_bc_fwd), one after another,
+// each spread over the whole grid in 64-slot rounds: with one wavefront per
+// row, a level holding a hub of 10^5-10^6 slots waited for that wavefront.
+template <class Adj>
+__global__ __launch_bounds__(256) void k_bc_fwd_hub(Adj adj, int *dist, double *sigma,
+                                                    uint32_t *queue, uint32_t qapp, int d,
+                                                    unsigned int *cursor, const uint32_t *hubs,
+                                                    const unsigned int *nhub) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nh = *nhub;
+  for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t u = hubs[h];
+    const double su = sigma[u];
+    const uint32_t us = adj.len(u);
+    const uint64_t b = adj.base(u);
+    for (uint64_t base = wv * WAVE; base < us; base += nwaves * WAVE) {
+      const uint32_t j = (uint32_t)base + lane;
+      bool fresh = false;
+      uint32_t w = TOMB;
+      if (j < us) {
+        w = adj.at(b, j);
+        if (w != TOMB) {
+          int cur = dist[w];
+          if (cur == -1) {
+            const int old = atomicCAS(&dist[w], -1, d + 1);
+            fresh = old == -1;
+            cur = fresh ? d + 1 : old;
+          }
+          if (cur == d + 1) atomicAdd(&sigma[w], su);
+        }
+      }
+      unsigned long long m = __ballot(fresh);
+      uint32_t pos = 0;
+      if (lane == 0 && m) pos = atomicAdd(cursor, (unsigned)__popcll(m));
+      pos = __shfl(pos, 0, WAVE);
+      if (fresh) queue[qapp + pos + mask_lt_count(m)] = w;
+    }
+  }
+}
+
+// Backward hub rows: block k of BC_HUB_GRID sums the successors in its slot
+// range [k*span, (k+1)*span) of each listed row (Kahan, then a fixed-order
+// join over its lanes and wavefronts) into hpart[h * BC_HUB_GRID + k].
+template <class Adj>
+__global__ __launch_bounds__(256) void k_bc_bwd_hub(Adj adj, const int *dist, const double *sigma,
+                                                    const float *delta, int d,
+                                                    const uint32_t *hubs, const unsigned int *nhub,
+                                                    float2 *hpart) {
+  __shared__ float2 s_w[256 / WAVE];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const uint32_t nh = *nhub;
+  for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t v = hubs[h];
+    const double sv = sigma[v];
+    const uint32_t us = adj.len(v);
+    const uint64_t b = adj.base(v);
+    const uint32_t span = (uint32_t)(((us + BC_HUB_GRID - 1) / BC_HUB_GRID + WAVE - 1) / WAVE * WAVE);
+    const uint32_t lo = blockIdx.x * span;
+    const uint32_t hi = lo + span < us ? lo + span : us;
+    float s = 0.f, c = 0.f;
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+      const uint32_t w = adj.at(b, j);
+      if (w != TOMB && dist[w] == d + 1) {
+        const float coef = (float)(sv / sigma[w]);
+        kahan_add(s, c, coef * (1.0f + delta[w]));
+      }
+    }
+#pragma unroll
+    for (int k = WAVE / 2; k >= 1; k >>= 1) {
+      float s2 = __shfl_xor(s, k, WAVE), c2 = __shfl_xor(c, k, WAVE);
+      two_sum_merge(s, c, s2, c2);
+    }
+    if (lane == 0) s_w[wid] = make_float2(s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float2 t = s_w[0];
+      for (int k = 1; k < 256 / WAVE; ++k) two_sum_merge(t.x, t.y, s_w[k].x, s_w[k].y);
+      hpart[(uint64_t)h * BC_HUB_GRID + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// One wavefront per listed hub row: its BC_HUB_GRID partials joined in block
+// order, then delta and BC as in k_bc_bwd.
+__global__ __launch_bounds__(256) void k_bc_bwd_hubfin(float *delta, const uint32_t *hubs,
+                                                       const unsigned int *nhub,
+                                                       const float2 *hpart, uint32_t src,
+                                                       float *bc, float *bcc) {
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint32_t nh = *nhub;
+  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; h < nh;
+       h += nwaves) {
+    if (lane != 0) continue;
+    float s = 0.f, c = 0.f;
+    for (uint32_t k = 0; k < BC_HUB_GRID; ++k) {
+      const float2 q = hpart[h * BC_HUB_GRID + k];
+      two_sum_merge(s, c, q.x, q.y);
+    }
+    const uint32_t v = hubs[h];
+    const float dv = s - c;
+    delta[v] = dv;
+    if (v != src) {
+      float bv = bc[v], bcomp = bcc[v];
+      kahan_add(bv, bcomp, dv);
+      bc[v] = bv;
+      bcc[v] = bcomp;
+    }
+  }
+}
+
+// Rows longer than BC_HUB slots in the whole graph (sizes hpart).
+template <class Adj>
+__global__ void k_bc_hubcount(Adj adj, uint32_t V, unsigned int *n) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    if (adj.len((uint32_t)v) > BC_HUB) atomicAdd(n, 1u);
+}
+
 gv_status bc_reserve(BcWork &w, uint32_t V) {
   if (w.V >= V && w.dist) return GV_OK;
   bc_release(w);
@@ -1100,6 +1237,13 @@ gv_status bc_reserve(BcWork &w, uint32_t V) {
   GV_HIP(hipMalloc(&w.queue, n * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&w.cursor, sizeof(unsigned int)));
   GV_HIP(hipHostMalloc(&w.h_cursor, sizeof(unsigned int), hipHostMallocDefault));
+  GV_HIP(hipMalloc(&w.hubs, n * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&w.nhub, sizeof(unsigned int)));
+  // hub partials presized (a serving loop reserves once and must not
+  // allocate later); a graph with more hub rows grows them in bc_levels
+  const uint64_t hc = n < 16384 ? n : 16384;
+  GV_HIP(hipMalloc(&w.hpart, hc * BC_HUB_GRID * sizeof(float2)));
+  w.hpart_cap = hc;
   w.V = V;
   return GV_OK;
 }
@@ -1111,6 +1255,9 @@ void bc_release(BcWork &w) {
   hipFree(w.bcc);
   hipFree(w.queue);
   hipFree(w.cursor);
+  hipFree(w.hubs);
+  hipFree(w.nhub);
+  hipFree(w.hpart);
   if (w.h_cursor) hipHostFree(w.h_cursor);
   const uint32_t mb = w.max_blocks;
   w = BcWork();
@@ -1127,6 +1274,24 @@ static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
   if (rc) return rc;
   const unsigned cap = w.max_blocks ? w.max_blocks : 8192u;
   const uint32_t ch = w.yield ? (1u << 20) : 0xFFFFFFFFu;
+  // capacity of the backward hub partials: every row longer than BC_HUB
+  fill_u32(w.nhub, 0, 1, s);
+  k_bc_hubcount<Adj><<<grid_for(V, 256, 4096), 256, 0, s>>>(adj, V, w.nhub);
+  if (hipMemcpyAsync(w.h_cursor, w.nhub, sizeof(unsigned), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("bc: HIP error");
+    return GV_E_HIP;
+  }
+  const uint64_t nh_all = *w.h_cursor;
+  if (nh_all > w.hpart_cap) {
+    hipFree(w.hpart);
+    w.hpart = nullptr;
+    w.hpart_cap = 0;
+    GV_HIP(hipMalloc(&w.hpart, (nh_all + nh_all / 4 + 1) * BC_HUB_GRID * sizeof(float2)));
+    w.hpart_cap = nh_all + nh_all / 4 + 1;
+  }
+  const unsigned hub_grid = cap < 1024u ? cap : 1024u;
   fill_f32(out_dev, 0.f, V, s);
   fill_f32(w.bcc, 0.f, V, s);
   for (uint32_t si = 0; si < n_sources && rc == GV_OK; ++si) {
@@ -1151,8 +1316,12 @@ static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
           }
           w.yield(w.yield_arg);
         }
+        fill_u32(w.nhub, 0, 1, s);
         k_bc_fwd<Adj><<<grid_for((uint64_t)(c1 - c0) * WAVE, 256, cap), 256, 0, s>>>(
-            adj, w.dist, w.sigma, w.queue, (uint32_t)c0, c1, qe, d, w.cursor);
+            adj, w.dist, w.sigma, w.queue, (uint32_t)c0, c1, qe, d, w.cursor, w.hubs, w.nhub);
+        if (nh_all)
+          k_bc_fwd_hub<Adj><<<hub_grid, 256, 0, s>>>(adj, w.dist, w.sigma, w.queue, qe, d,
+                                                     w.cursor, w.hubs, w.nhub);
       }
       if (rc) break;
       if (hipMemcpyAsync(w.h_cursor, w.cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s) !=
@@ -1180,8 +1349,16 @@ static gv_status bc_levels(const Adj &adj, uint32_t V, const uint32_t *sources,
           }
           w.yield(w.yield_arg);
         }
+        fill_u32(w.nhub, 0, 1, s);
         k_bc_bwd<Adj><<<grid_for((uint64_t)(c1 - c0) * WAVE, 256, cap), 256, 0, s>>>(
-            adj, w.dist, w.sigma, w.delta, w.queue, (uint32_t)c0, c1, lev, src, out_dev, w.bcc);
+            adj, w.dist, w.sigma, w.delta, w.queue, (uint32_t)c0, c1, lev, src, out_dev, w.bcc,
+            w.hubs, w.nhub);
+        if (nh_all) {
+          k_bc_bwd_hub<Adj><<<BC_HUB_GRID, 256, 0, s>>>(adj, w.dist, w.sigma, w.delta, lev,
+                                                        w.hubs, w.nhub, w.hpart);
+          k_bc_bwd_hubfin<<<64, 256, 0, s>>>(w.delta, w.hubs, w.nhub, w.hpart, src, out_dev,
+                                             w.bcc);
+        }
       }
     }
   }

@@ -274,6 +274,25 @@ def test_bc_single_source_calls(directed):
         check_bc(g, o, np.array([src], np.uint32))
 
 
+@pytest.mark.parametrize("directed", [False, True])
+def test_bc_single_source_hub_rows(directed):
+    """Frontier rows longer than 8192 slots are spread over the whole grid in
+    both sweeps (partials joined in block order backward): two hubs of 9000
+    and 30000 neighbours over an R-MAT graph, sources on and off the hubs."""
+    V = 40000
+    rng = np.random.default_rng(8)
+    s, d = O.gen_rmat(O.stream_key(23, O.STREAM_RMAT), 15, 120000)
+    s, d = s % V, d % V
+    n0 = rng.choice(np.arange(2, V), 9000, replace=False).astype(np.uint32)
+    n1 = rng.choice(np.arange(2, V), 30000, replace=False).astype(np.uint32)
+    s = np.concatenate([s, np.zeros(9000, np.uint32), np.ones(30000, np.uint32), n1[:500]])
+    d = np.concatenate([d, n0, n1, np.zeros(500, np.uint32)]).astype(np.uint32)
+    g, o = build_pair(V, directed, s.astype(np.uint32), d, pool=16 * V)
+    for src in ([0], [1], [int(n1[7])], [5, 1]):
+        check_bc(g, o, np.array(src, np.uint32))
+    g.close()
+
+
 def test_bc_rejects_bad_source():
     g, _ = build_pair(10, False, [0, 1], [1, 2])
     with pytest.raises(G.GraphVineError) as e:
