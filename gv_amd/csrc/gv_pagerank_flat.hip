@@ -335,7 +335,11 @@ __global__ __launch_bounds__(PF_ROW_THREADS) void k_pf_rows(PfArgs a) {
     a.cn[v] = d ? xv / (float)d : 0.f;
     diff += fabs((double)xv - (double)a.x[v]);
   }
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < a.n_empty; j += stride) {
+  // rows with no entries take x' = base in every iteration: once both
+  // buffers hold it (after iterations 0 and 1) they change nothing and add
+  // exactly 0 to the residual, so later iterations skip them
+  const uint64_t n_empty = a.ctl->iters < 2 ? a.n_empty : 0;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_empty; j += stride) {
     const uint32_t v = a.empty[j];
     const float xv = a.base;
     a.xn[v] = xv;
