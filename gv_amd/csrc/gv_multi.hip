@@ -34,6 +34,9 @@ constexpr uint32_t PPR_GRID = 2048;
 // iteration waited for the wavefront holding the largest hub.
 constexpr uint32_t PPR_HUB = 4096;
 constexpr uint32_t PPR_FIN_GRID = 256;
+// contribution rows (256 bytes each) requested per wavefront before the first
+// is added
+constexpr int PPR_ILP = 16;
 
 struct PprArgs {
   Store in;
@@ -78,16 +81,16 @@ __global__ __launch_bounds__(PPR_THREADS) void k_ppr_iter(PprArgs a, const float
       const uint32_t n = us - base < (uint32_t)WAVE ? us - base : (uint32_t)WAVE;
       uint32_t slot = TOMB;
       if (lane < n) slot = a.in.slots[slot_index(a.in.arena, tbl, base + lane, a.in.log2B, a.in.B)];
-      for (uint32_t t = 0; t < n; t += 8) {
-        uint32_t u[8];
+      for (uint32_t t = 0; t < n; t += PPR_ILP) {
+        uint32_t u[PPR_ILP];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) u[k] = __shfl(slot, (int)((t + k) & (WAVE - 1)), WAVE);
-        float g[8];
+        for (int k = 0; k < PPR_ILP; ++k) u[k] = __shfl(slot, (int)((t + k) & (WAVE - 1)), WAVE);
+        float g[PPR_ILP];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < PPR_ILP; ++k)
           g[k] = (t + k < n && u[k] != TOMB) ? CB[(uint64_t)u[k] * PPR_COLS + lane] : 0.f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < PPR_ILP; ++k)
           if (t + k < n && u[k] != TOMB) kahan_add(s, c, g[k]);
       }
     }
@@ -146,16 +149,16 @@ __global__ __launch_bounds__(PPR_THREADS) void k_ppr_hub(PprArgs a, const float 
       const uint32_t n = hi - base < (uint32_t)WAVE ? hi - base : (uint32_t)WAVE;
       uint32_t slot = TOMB;
       if (lane < n) slot = a.in.slots[slot_index(a.in.arena, tbl, base + lane, a.in.log2B, a.in.B)];
-      for (uint32_t t = 0; t < n; t += 8) {
-        uint32_t u[8];
+      for (uint32_t t = 0; t < n; t += PPR_ILP) {
+        uint32_t u[PPR_ILP];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) u[k] = __shfl(slot, (int)((t + k) & (WAVE - 1)), WAVE);
-        float g[8];
+        for (int k = 0; k < PPR_ILP; ++k) u[k] = __shfl(slot, (int)((t + k) & (WAVE - 1)), WAVE);
+        float g[PPR_ILP];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < PPR_ILP; ++k)
           g[k] = (t + k < n && u[k] != TOMB) ? CB[(uint64_t)u[k] * PPR_COLS + lane] : 0.f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < PPR_ILP; ++k)
           if (t + k < n && u[k] != TOMB) kahan_add(s, c, g[k]);
       }
     }
