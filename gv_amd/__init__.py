@@ -44,7 +44,7 @@ EXPORTED = [
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_gen_chung_lu",
     "gv_get_config", "gv_set_tuning", "gv_reserve", "gv_capture_begin", "gv_capture_end",
     "gv_capture_launch", "gv_capture_destroy", "gv_read_matrix_market", "gv_sync",
-    "gv_get_stream",
+    "gv_get_stream", "gv_test_set_sort_spin_limit",
     "gv_last_error",
 ]
 
@@ -162,6 +162,7 @@ def lib():
                                   P, ctypes.POINTER(i32)],
         "gv_sync": [P],
         "gv_get_stream": [P, ctypes.POINTER(P)],
+        "gv_test_set_sort_spin_limit": [u32],
     }
     for name, args in sig.items():
         f = getattr(L, name)

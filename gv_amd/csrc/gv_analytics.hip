@@ -344,6 +344,7 @@ __global__ void k_rl_perm(const uint64_t *sorted, uint32_t V, uint32_t bV, const
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
        r += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = (uint32_t)(sorted[r] & mask);
+    if (v >= V) continue;  // only after a failed sort, which the caller reports
     order[r] = v;
     newid[v] = (uint32_t)r;
     len_new[r] = rowlen[v];
@@ -422,6 +423,10 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
   uint64_t E = 0;
   GV_HIP(hipMemcpyAsync(&E, w.snap_off + V, sizeof(E), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
+  if (relabel) {
+    gv_status src = sort_check(w.rl_tmp, V, 0, id_bits(V) + id_bits(V + 1), s);
+    if (src) return src;
+  }
   w.snap_E = E;
   // the flat layout pads the stream to whole tiles
   const uint64_t need = flat ? cdiv(E, PF_T) * PF_T : E;

@@ -75,6 +75,11 @@ extern "C" {
 
 const char *gv_last_error(void) { return g_err.c_str(); }
 
+gv_status gv_test_set_sort_spin_limit(uint32_t limit) {
+  sort_set_spin_limit(limit);
+  return GV_OK;
+}
+
 gv_status gv_config_default(gv_config *cfg) {
   if (!cfg) return GV_E_INVALID;
   std::memset(cfg, 0, sizeof(*cfg));
@@ -135,6 +140,10 @@ static gv_status take_abort(gv_graph *g) {
   if (ab == ABORT_POOL) {
     set_error("edge-block pool exhausted; batch not applied");
     return GV_E_POOL;
+  }
+  if (ab == ABORT_SORT) {
+    set_error("radix sort of the staged keys failed (a tile's look-back gave up); nothing applied");
+    return GV_E_HIP;
   }
   set_error("device abort");
   return GV_E_HIP;
@@ -252,6 +261,9 @@ static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *d
     const int mirror = g->directed ? 0 : 1;
     op_begin(g->st[k], s);
     keys[k] = stage(g->st[k], g->sc[k], ds, dd, n, mirror, k == 1 ? 1 : 0, m[k], s);
+  }
+  if (nst > 1) abort_join(g->st[0], g->st[1], s);
+  for (int k = 0; k < nst; ++k) {
     segment(g->st[k], g->sc[k], keys[k], m[k], s);
     if (op == Op::Insert) insert_plan(g->st[k], g->sc[k], keys[k], m[k], s);
     else delete_apply(g->st[k], g->sc[k], keys[k], m[k], s);
@@ -352,6 +364,9 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
       op_begin(g->st[k], s);
       keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, g->grp_off, g->grp_kind, nb, mirror,
                             k == 1 ? 1 : 0, m[k], s);
+    }
+    if (nst > 1) abort_join(g->st[0], g->st[1], s);
+    for (int k = 0; k < nst; ++k) {
       segment(g->st[k], g->sc[k], keys[k], m[k], s, g->sc[k].op);
       mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
     }
@@ -654,6 +669,13 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
       rc = hip_fail(e, "export");
       break;
     }
+    if (cols && E > SORT_MAX_KEYS) {
+      // the export sorts all E entries at once, and a sort's offsets are 32-bit
+      set_error("gv_export_csr: more than 2^32-1 live edges; export a vertex range with "
+                "gv_neighbors instead");
+      rc = GV_E_UNSUPPORTED;
+      break;
+    }
     if (cols && E) {
       hipFree(tmp);
       tmp = nullptr;
@@ -668,6 +690,7 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
       }
       export_keys(g->st[0], off, k0, s);
       uint64_t *sorted = sort_keys(k0, k1, E, 2 * bV, tmp, s);
+      if ((rc = sort_check(tmp, E, 0, 2 * bV, s)) != GV_OK) break;
       uint32_t *cdst = mem == GV_MEM_DEVICE ? cols : dcols;
       keys_to_cols(sorted, E, bV, cdst, s);
       if (mem != GV_MEM_DEVICE &&
@@ -751,6 +774,10 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
       (e = hipMemcpyAsync(offs, cnt, ((size_t)n + 1) * 8, out_kind, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
     return hip_fail(e, "neighbors");
+  if (out && T > SORT_MAX_KEYS) {
+    set_error("gv_neighbors: more than 2^32-1 neighbours in one call; split the query");
+    return GV_E_UNSUPPORTED;
+  }
   if (out && T) {
     if ((e = nb_grow_keys(g, T * 8)) != hipSuccess ||
         (e = nb_grow(&g->nb_tmp, g->nb_tmp_cap, sort_tmp_bytes(T))) != hipSuccess ||
@@ -759,6 +786,7 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
       return hip_fail(e, "neighbors alloc");
     neighbor_keys(g->st[0], qv, n, g->nb_cnt, qb, g->nb_k0, s);
     uint64_t *sorted = sort_keys(g->nb_k0, g->nb_k1, T, qb + bV, g->nb_tmp, s);
+    if ((rc = sort_check(g->nb_tmp, T, 0, qb + bV, s)) != GV_OK) return rc;
     uint32_t *cdst = mem == GV_MEM_DEVICE ? out : g->nb_dout;
     keys_to_cols(sorted, T, bV, cdst, s);
     if (mem != GV_MEM_DEVICE &&

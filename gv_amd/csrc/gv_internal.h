@@ -229,11 +229,24 @@ void scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, const uint32_t *n_d
               hipStream_t s);
 // Workspace of any sort of at most n keys.
 size_t sort_tmp_bytes(uint64_t n);
-// LSD radix sort of n keys on their low `bits` bits; returns the buffer that
-// holds the result (a or b).
-uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp, hipStream_t s);
+// Largest key count a sort takes (its histograms and offsets are 32-bit).
+constexpr uint64_t SORT_MAX_KEYS = 0xFFFFFFFFull;
+// Abort code a sort leaves in the caller's abort word when a tile of the
+// single-sweep path could not find its offsets (the output is then invalid).
+constexpr uint32_t ABORT_SORT = 5;
+// LSD radix sort of n <= SORT_MAX_KEYS keys on their low `bits` bits;
+// returns the buffer that holds the result (a or b).  abort (device word, may
+// be null) receives ABORT_SORT if the sort failed.
+uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp, hipStream_t s,
+                    uint32_t *abort = nullptr);
 // Same, ordering on bits [lo, hi) only (stable for keys equal there).
 uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
-                          void *tmp, hipStream_t s);
+                          void *tmp, hipStream_t s, uint32_t *abort = nullptr);
+// After a sort of n keys on bits [lo, hi) without an abort word: waits for
+// the stream and returns GV_E_HIP (error text set) if the sort failed.
+gv_status sort_check(const void *tmp, uint64_t n, uint32_t lo, uint32_t hi, hipStream_t s);
+// Bound on the look-back polls of the single-sweep sort (tests lower it to
+// exercise the failure path; 0 makes every tile after the first give up).
+void sort_set_spin_limit(uint32_t limit);
 
 }  // namespace gv

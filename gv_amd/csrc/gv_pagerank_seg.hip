@@ -413,6 +413,7 @@ gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBin
     }
     k_ps_emit<<<grid_for((uint64_t)V, 256, 8192), 256, 0, s>>>(in, off_in, g, k0);
     uint64_t *sorted = sort_keys_range(k0, k1, E, g.bV, 2 * g.bV + 3, tmp, s);
+    if ((rc = sort_check(tmp, E, g.bV, 2 * g.bV + 3, s)) != GV_OK) break;
     if (E) {
       k_ps_split<<<grid_for(E), 256, 0, s>>>(sorted, E, g, w.src, w.off);
     } else {

@@ -271,12 +271,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
 // tiles up to and including this one); 0 = not yet published.
 constexpr uint32_t OS_AGG = 1u << 30;
 constexpr uint32_t OS_PREFIX = 2u << 30;
+// A tile whose look-back gave up publishes OS_BAD: its offsets are unknown,
+// so every tile after it gives up too instead of trusting a wrong prefix.
+constexpr uint32_t OS_BAD = OS_AGG | OS_PREFIX;
 constexpr uint32_t OS_VAL = OS_AGG - 1;
 // Counts must fit 30 bits; past 2^24 keys launches no longer dominate and
 // the classic path's workspace (1/8 of this one's) is used.
 constexpr uint64_t OS_MAX_KEYS = 1ull << 24;
 constexpr int OS_MAX_PASSES = 8;
-constexpr uint32_t OS_SPIN_LIMIT = 1u << 26;  // a bug shows as wrong output, not a hang
+// Bounded wait: a look-back that sees no progress for this many polls gives
+// up, and the sort reports the failure through the caller's abort word
+// instead of hanging.
+constexpr uint32_t OS_SPIN_LIMIT = 1u << 26;
 constexpr int OS_LOOK = 8;                    // predecessors read per look-back round trip
 // keys per thread of a single-sweep tile
 constexpr int OS_ITEMS = 16;
@@ -342,10 +348,14 @@ __device__ __forceinline__ uint32_t os_peek(uint32_t *w) {
 
 // One pass.  A block takes the next tile in start order (atomic counter), so
 // it only ever waits on tiles whose blocks are already running.
+// spin_limit bounds the polls of one look-back (OS_SPIN_LIMIT unless a test
+// lowers it; 0 gives up at the first predecessor, so every tile after tile 0
+// fails and the caller's error path runs while every kernel still exits).
 __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in, uint64_t *out,
                                                              uint64_t n, uint32_t shift,
                                                              const uint32_t *ghist, uint32_t *ctr,
-                                                             uint32_t *status, uint32_t *err) {
+                                                             uint32_t *status, uint32_t *err,
+                                                             uint32_t spin_limit) {
   constexpr int NW = SORT_THREADS / WAVE;
   __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t s_tile, s_bad;
@@ -403,8 +413,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
       // holds a prefix, so the walk ends there at the latest
       uint32_t back = 1;  // next predecessor to read: tile - back
       uint32_t spins = 0;
-      bool done = false;
-      while (!done) {
+      bool done = false, bad = spin_limit == 0;
+      while (!done && !bad) {
         uint32_t v[OS_LOOK];
 #pragma unroll
         for (int r = 0; r < OS_LOOK; ++r)
@@ -412,25 +422,36 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
         bool stalled = false;
 #pragma unroll
         for (int r = 0; r < OS_LOOK; ++r) {
-          if (done || stalled || back > tile) continue;
+          if (done || bad || stalled || back > tile) continue;
           if (v[r] == 0) {
             stalled = true;  // not published yet: read it again
+            continue;
+          }
+          if ((v[r] & OS_BAD) == OS_BAD) {
+            bad = true;  // a predecessor gave up: its count is unknown
             continue;
           }
           excl += v[r] & OS_VAL;
           ++back;
           if (v[r] & OS_PREFIX) done = true;
         }
-        if (stalled && !done) {
-          if (++spins > OS_SPIN_LIMIT) {
-            atomicOr(err, 1u);
-            s_bad = 1;  // offsets unknown: this tile writes nothing
-            break;
+        if (stalled && !done && !bad) {
+          if (++spins > spin_limit) {
+            bad = true;
+            continue;
           }
           __builtin_amdgcn_s_sleep(1);
         }
       }
-      os_publish(st, OS_PREFIX | (excl + mine));
+      if (bad) {
+        // offsets unknown: this tile writes nothing, and its status tells
+        // the tiles after it to give up as well
+        atomicOr(err, 1u);
+        s_bad = 1;
+        os_publish(st, OS_BAD);
+      } else {
+        os_publish(st, OS_PREFIX | (excl + mine));
+      }
     }
     // digit base: counts of all smaller digits in this pass
     uint32_t all;
@@ -454,6 +475,30 @@ __global__ __launch_bounds__(SORT_THREADS) void k_os_scatter(const uint64_t *in,
   }
 }
 
+// After the passes of a single-sweep sort: a look-back that gave up leaves
+// err set; the caller's sticky abort word then takes ABORT_SORT (unless it
+// already holds a code), so the operation fails as a whole.
+__global__ void k_os_fold(const uint32_t *err, uint32_t *abort) {
+  if (*err) atomicCAS(abort, 0u, ABORT_SORT);
+}
+
+static uint32_t g_sort_spin_limit = OS_SPIN_LIMIT;
+void sort_set_spin_limit(uint32_t limit) { g_sort_spin_limit = limit; }
+
+gv_status sort_check(const void *tmp, uint64_t n, uint32_t lo, uint32_t hi, hipStream_t s) {
+  const uint32_t passes = (hi - lo + 7) / 8;
+  if (n <= 1 || n > OS_MAX_KEYS || passes > (uint32_t)OS_MAX_PASSES) return GV_OK;
+  uint32_t err = 0;
+  GV_HIP(hipMemcpyAsync(&err, os_tmp(const_cast<void *>(tmp)).err, sizeof(err),
+                        hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  if (err) {
+    set_error("radix sort failed: a tile's look-back gave up; results discarded");
+    return GV_E_HIP;
+  }
+  return GV_OK;
+}
+
 // Workspace for any sort of at most n keys: callers size it once and sort
 // fewer keys (a sort of <= 2^24 keys takes the single-sweep path even when
 // the workspace was sized for more).
@@ -467,12 +512,12 @@ size_t sort_tmp_bytes(uint64_t n) {
 }
 
 uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp,
-                    hipStream_t s) {
-  return sort_keys_range(a, b, n, 0, bits, tmp, s);
+                    hipStream_t s, uint32_t *abort) {
+  return sort_keys_range(a, b, n, 0, bits, tmp, s, abort);
 }
 
 uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
-                          void *tmp, hipStream_t s) {
+                          void *tmp, hipStream_t s, uint32_t *abort) {
   if (n <= 1) return a;
   const uint32_t passes = (hi - lo + 7) / 8;
   if (n <= OS_MAX_KEYS && passes <= (uint32_t)OS_MAX_PASSES) {
@@ -488,11 +533,12 @@ uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uin
     for (uint32_t p = 0; p < passes; ++p) {
       k_os_scatter<<<dim3((unsigned)nt), dim3(SORT_THREADS), 0, s>>>(
           src, dst, n, lo + 8 * p, t.ghist + p * 256, t.ctr + p, t.status + (uint64_t)p * nt * 256,
-          t.err);
+          t.err, g_sort_spin_limit);
       uint64_t *x = src;
       src = dst;
       dst = x;
     }
+    if (abort) k_os_fold<<<1, 1, 0, s>>>(t.err, abort);
     return src;
   }
   const uint64_t nt = cdiv(n, SORT_TILE);

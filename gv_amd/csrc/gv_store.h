@@ -24,6 +24,8 @@ void scratch_release(Scratch &sc);
 
 // Per-op counter reset (keeps the sticky abort code).
 void op_begin(Store &st, hipStream_t s);
+// Copy an abort code set in either store into the other.
+void abort_join(Store &a, Store &b, hipStream_t s);
 
 // Stage n (src,dst) pairs as sorted keys in sc; mirror adds (dst,src) too,
 // swap stores (dst,src) instead.  Returns the sorted key buffer; m = keys.

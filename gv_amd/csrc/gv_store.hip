@@ -185,7 +185,7 @@ uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst
   if (!n) return sc.k0;
   const uint32_t bV = id_bits(st.V);
   k_stage<<<grid_for(n), 256, 0, s>>>(src, dst, n, st.V, bV, mirror, swap, sc.k0, st.ctl);
-  return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s);
+  return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s, &st.ctl->abort);
 }
 
 // Group staging: key = ((u << bV | v) << sb) | batch, so after the sort the
@@ -227,22 +227,26 @@ __device__ __forceinline__ bool last_of_edge(const uint64_t *keys, uint64_t m, u
   return i + 1 == m || (keys[i + 1] >> sb) != (k >> sb);
 }
 
-__global__ void k_last_mark(const uint64_t *keys, uint64_t m, uint32_t sb, uint32_t *flag) {
+// After an abort (a failed sort leaves arbitrary keys) no key is kept.
+__global__ void k_last_mark(const uint64_t *keys, uint64_t m, uint32_t sb, uint32_t *flag,
+                            const Ctl *ctl) {
+  const bool ab = ctl->abort != 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x)
-    flag[i] = last_of_edge(keys, m, i, sb) ? 1u : 0u;
+    flag[i] = !ab && last_of_edge(keys, m, i, sb) ? 1u : 0u;
 }
 
 // Keep the last entry of every edge (without its batch tag) and its op; pad
 // the tail with the sentinel so the array stays m long.
 __global__ void k_last_compact(const uint64_t *keys, uint64_t m, uint32_t sb,
                                const uint32_t *pos, const uint32_t *kinds, uint64_t *out,
-                               uint32_t *op) {
+                               uint32_t *op, const Ctl *ctl) {
   const uint32_t total = pos[m];
   const uint64_t smask = (1ull << sb) - 1;
+  const bool ab = ctl->abort != 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    if (last_of_edge(keys, m, i, sb)) {
+    if (!ab && last_of_edge(keys, m, i, sb)) {
       uint64_t k = keys[i];
       out[pos[i]] = k >> sb;
       op[pos[i]] = kinds[k & smask];
@@ -262,23 +266,36 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
   // the keys are staged in batch order, so a stable sort on the (u, v) bits
   // alone leaves equal edges in batch order: the same array as a sort on the
   // whole key, with one pass fewer when the tag crosses a digit boundary
-  uint64_t *sorted = sort_keys_range(sc.k0, sc.k1, m, sb, 2 * bV + sb, sc.tmp, s);
+  uint64_t *sorted =
+      sort_keys_range(sc.k0, sc.k1, m, sb, 2 * bV + sb, sc.tmp, s, &st.ctl->abort);
   uint64_t *other = sorted == sc.k0 ? sc.k1 : sc.k0;
-  k_last_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.segof);
+  k_last_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.segof, st.ctl);
   scan_u32(sc.segof, sc.rank, m, nullptr, sc.tmp, s);
-  k_last_compact<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.rank, kinds, other, sc.op);
+  k_last_compact<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.rank, kinds, other, sc.op,
+                                             st.ctl);
   return other;
 }
 
+// Both stores of a directed graph stage the same batch; an abort in either
+// (a failed sort) must stop both before anything is written.
+__global__ void k_abort_join(Ctl *a, Ctl *b) {
+  const unsigned ab = a->abort ? a->abort : b->abort;
+  if (!a->abort) a->abort = ab;
+  if (!b->abort) b->abort = ab;
+}
+void abort_join(Store &a, Store &b, hipStream_t s) { k_abort_join<<<1, 1, 0, s>>>(a.ctl, b.ctl); }
+
 // ---------------------------------------------------------- segmentation --
 // op (may be null): per-key operation of a merged group, OP_DELETE keys are
-// never insert candidates.
+// never insert candidates.  After an abort nothing is segmented: the keys of
+// a failed sort are arbitrary, and no later step may index with them.
 __global__ void k_seg_mark(const uint64_t *keys, uint64_t m, uint32_t bV, uint32_t *head,
                            uint32_t *flag, const uint32_t *op, Ctl *ctl) {
+  const bool ab = ctl->abort != 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t k = keys[i];
-    bool valid = k != KEY_SENT;
+    bool valid = !ab && k != KEY_SENT;
     uint64_t prev = i ? keys[i - 1] : KEY_SENT;
     bool h = valid && (i == 0 || (prev >> bV) != (k >> bV));
     bool dup = valid && i > 0 && prev == k;

@@ -413,6 +413,13 @@ gv_status triangle_count_run(const Store &st, uint64_t *per_vertex_dev, uint64_t
       k_tc_wkeys<<<grid_for((uint64_t)n_work * WAVE, 256, 4096), 256, 0, s>>>(off, cols, wroot,
                                                                               nb, n_work, keys[0]);
       const uint64_t *sorted = sort_keys_range(keys[0], keys[1], n_work, 32, 52, stmp, s);
+      {
+        const gv_status src = sort_check(stmp, n_work, 32, 52, s);
+        if (src) {
+          rc = src;
+          break;
+        }
+      }
       // the item order is the low word of the sorted keys
       uint32_t *order = reinterpret_cast<uint32_t *>(sorted == keys[0] ? keys[1] : keys[0]);
       keys_to_cols(sorted, n_work, 32, order, s);

@@ -121,10 +121,13 @@ gv_status gv_num_edges(gv_graph *g, uint64_t *live_edges);
 gv_status gv_degrees(gv_graph *g, uint32_t *deg, int mem);
 gv_status gv_pool_stats(gv_graph *g, uint64_t *free_blocks, uint64_t *total_blocks);
 
-/* Sorted CSR export: offsets[V+1], cols[offsets[V]] (ascending per vertex). */
+/* Sorted CSR export: offsets[V+1], cols[offsets[V]] (ascending per vertex).
+ * With cols, a graph of more than 2^32-1 live directed edges is refused
+ * (GV_E_UNSUPPORTED): the export sorts every entry at once. */
 gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem);
 /* 1-hop lookups: offs[n+1]; out[offs[n]] sorted per query (out may be NULL
- * to size it first). */
+ * to size it first; with out, more than 2^32-1 neighbours in one call is
+ * GV_E_UNSUPPORTED). */
 gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs, uint32_t *out,
                        int mem);
 
@@ -268,6 +271,13 @@ gv_status gv_capture_begin(gv_graph *g);
 gv_status gv_capture_end(gv_graph *g, gv_capture **out);
 gv_status gv_capture_launch(gv_capture *c, gv_batch_stats *stats);
 gv_status gv_capture_destroy(gv_capture *c);
+
+/* Test hook: bound on the polls of one look-back wait inside the staging
+ * radix sort (default 2^26).  0 makes every sort tile after the first give
+ * up at once, so a test can check that the failure is reported (GV_E_HIP,
+ * nothing applied) while every kernel still exits normally.  Process-wide;
+ * not for production use.  No reference counterpart. */
+gv_status gv_test_set_sort_spin_limit(uint32_t limit);
 
 /* Stream plumbing. */
 gv_status gv_sync(gv_graph *g);

@@ -140,6 +140,44 @@ def test_pool_exhaustion_rejects_and_recovers():
     g.close()
 
 
+@pytest.mark.parametrize("directed", [True, False])
+def test_failed_sort_is_reported_and_nothing_applied(directed):
+    """The staging sort's bounded look-back wait, forced to give up at once
+    from the host (every tile after the first), must fail the call with
+    GV_E_HIP and leave the graph as it was -- for inserts, deletes and
+    groups, and in both stores of a directed graph.  Every kernel exits
+    normally; the wait bound is restored afterwards."""
+    V = 1 << 12
+    g = make(V, directed)
+    o = O.OracleGraph(V, directed)
+    s, d = O.gen_rmat(O.stream_key(21, O.STREAM_RMAT), 12, 8 * V)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    si, di = O.gen_uniform(O.stream_key(21, 1), V, 12000)  # > 2 sort tiles of keys
+    off, cols = o.export_csr()
+    sd, dd = O.gen_sample(O.stream_key(21, 2), V, off, cols, 12000)
+    L = G.lib()
+    try:
+        assert L.gv_test_set_sort_spin_limit(0) == 0
+        for fn in (g.batchInsert, g.batchDelete):
+            with pytest.raises(G.GraphVineError) as e:
+                fn(si, di) if fn == g.batchInsert else fn(sd, dd)
+            assert e.value.status == G.GV_E_HIP
+            assert "sort" in str(e.value)
+        with pytest.raises(G.GraphVineError) as e:
+            g.applyBatches(np.concatenate([sd, si]), np.concatenate([dd, di]),
+                           [0, len(sd), len(sd) + len(si)], [G.GV_OP_DELETE, G.GV_OP_INSERT],
+                           compact_fraction=0.25)
+        assert e.value.status == G.GV_E_HIP
+    finally:
+        L.gv_test_set_sort_spin_limit(1 << 26)
+    assert_same(g, o, "after failed sorts")
+    assert g.batchDelete(sd, dd)["applied"] == o.delete(sd, dd)
+    assert g.batchInsert(si, di)["applied"] == o.insert(si, di)
+    assert_same(g, o, "after the same batches with the sort restored")
+    g.close()
+
+
 def test_delete_all_then_compact_returns_every_block():
     V = 3000
     g = make(V, False, pool=6 * V)
