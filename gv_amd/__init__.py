@@ -38,7 +38,8 @@ GV_OP_DELETE = 1
 EXPORTED = [
     "gv_config_default", "gv_create", "gv_destroy", "gv_bulk_build", "gv_batch_insert",
     "gv_batch_delete", "gv_batch_apply", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
-    "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_pagerank", "gv_pagerank_ex",
+    "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_neighbors_noalloc", "gv_pagerank",
+    "gv_pagerank_ex",
     "gv_pagerank_opts_default", "gv_pagerank_run", "gv_pagerank_personalized",
     "gv_triangle_count",
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_gen_chung_lu",
@@ -133,6 +134,7 @@ def lib():
         "gv_pool_stats": [P, ctypes.POINTER(u64), ctypes.POINTER(u64)],
         "gv_export_csr": [P, P, P, i32],
         "gv_neighbors": [P, P, u32, P, P, i32],
+        "gv_neighbors_noalloc": [P, P, u32, P, P],
         "gv_pagerank": [P, ctypes.c_float, ctypes.c_double, i32, P, i32, ctypes.POINTER(i32),
                         ctypes.POINTER(ctypes.c_double)],
         "gv_pagerank_ex": [P, ctypes.c_float, ctypes.c_double, i32, P, i32,

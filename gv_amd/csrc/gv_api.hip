@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -709,6 +710,47 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
   return rc;
 }
 
+// ----------------------------------------------- allocation-free lookups --
+// gv_neighbors_noalloc works in one file-scope device array instead of
+// buffers it allocates: the key ping-pong pair of the sort and the sort's
+// workspace.  Every GPU the library runs on holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name in host code
+// is a host-side stand-in, not a device address).
+constexpr uint32_t NA_MAX_QUERIES = 1u << 16;
+constexpr uint64_t NA_MAX_NEIGHBORS = 1ull << 21;
+constexpr size_t NA_KEY_BYTES = NA_MAX_NEIGHBORS * sizeof(uint64_t);
+constexpr size_t NA_TMP_BYTES = 5u << 20;  // >= sort_tmp_bytes(NA_MAX_NEIGHBORS), scan of 2^16+1
+constexpr size_t NA_SCRATCH_BYTES = 2 * NA_KEY_BYTES + NA_TMP_BYTES;
+
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static uint8_t *g_na_ptr[NA_MAX_DEVICES];
+
+// Device address of this device's scratch (the current device is set by
+// check_graph); looked up once per device.
+static gv_status noalloc_scratch(int dev, uint8_t **out) {
+  if (dev < 0 || dev >= NA_MAX_DEVICES) {
+    set_error("gv_neighbors_noalloc: device ordinal out of range");
+    return GV_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) {
+      set_error("gv_neighbors_noalloc: no device address for the scratch array");
+      return GV_E_HIP;
+    }
+    g_na_ptr[dev] = (uint8_t *)p;
+  }
+  *out = g_na_ptr[dev];
+  return GV_OK;
+}
+
 // Grow-only device buffer: reallocated (with headroom) only when too small.
 static hipError_t nb_grow(void **p, size_t &cap, size_t need) {
   if (*p && need <= cap) return hipSuccess;
@@ -733,6 +775,56 @@ static hipError_t nb_grow_keys(gv_graph *g, size_t need) {
   if (e == hipSuccess) e = hipMalloc(&g->nb_k1, c);
   if (e == hipSuccess) g->nb_k_cap = c;
   return e;
+}
+
+gv_status gv_neighbors_noalloc(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs,
+                               uint32_t *out) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!offs || (n && !vs)) {
+    set_error("gv_neighbors_noalloc: null pointer");
+    return GV_E_INVALID;
+  }
+  if (n > NA_MAX_QUERIES) {
+    set_error("gv_neighbors_noalloc: more than 65536 queries in one call (the no-alloc maximum)");
+    return GV_E_INVALID;
+  }
+  hipStream_t s = g->stream;
+  const uint32_t V = g->V, bV = id_bits(V), qb = id_bits(n ? n : 1);
+  if (qb + bV > 64) {
+    set_error("too many queries for one call");
+    return GV_E_INVALID;
+  }
+  if (sort_tmp_bytes(NA_MAX_NEIGHBORS) > NA_TMP_BYTES ||
+      scan_tmp_bytes((uint64_t)NA_MAX_QUERIES + 1) > NA_TMP_BYTES) {
+    set_error("gv_neighbors_noalloc: scratch layout too small for its limits");
+    return GV_E_HIP;
+  }
+  uint8_t *base = nullptr;
+  if ((rc = noalloc_scratch(g->cfg.device, &base)) != GV_OK) return rc;
+  uint64_t *k0 = (uint64_t *)base;
+  uint64_t *k1 = (uint64_t *)(base + NA_KEY_BYTES);
+  void *tmp = base + 2 * NA_KEY_BYTES;
+  // counts go straight into the caller's offsets, scanned in place
+  neighbor_counts(g->st[0], vs, n, offs, s);
+  scan_u64(offs, offs, n, nullptr, tmp, s);
+  uint64_t T = 0;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(&T, offs + n, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess)
+    return hip_fail(e, "neighbors_noalloc");
+  if (!out || !T) return GV_OK;
+  if (T > NA_MAX_NEIGHBORS) {
+    set_error("gv_neighbors_noalloc: " + std::to_string(T) +
+              " neighbours exceed the no-alloc maximum of 2097152 per call; split the query");
+    return GV_E_INVALID;
+  }
+  neighbor_keys(g->st[0], vs, n, offs, qb, k0, s);
+  uint64_t *sorted = sort_keys(k0, k1, T, qb + bV, tmp, s);
+  if ((rc = sort_check(tmp, T, 0, qb + bV, s)) != GV_OK) return rc;
+  keys_to_cols(sorted, T, bV, out, s);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "neighbors_noalloc");
+  return GV_OK;
 }
 
 gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs, uint32_t *out,

@@ -131,6 +131,19 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
 gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs, uint32_t *out,
                        int mem);
 
+/* gv_neighbors without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * vs[n], offs[n+1] and out are device pointers (out may be NULL to size the
+ * answer first); results equal gv_neighbors'.  Working memory is one
+ * file-scope device array of the library (36 MiB, held on every GPU the
+ * library runs on until the process ends), so limits apply: n <= 65536
+ * queries and offs[n] <= 2097152 neighbours per call (GV_E_INVALID beyond,
+ * nothing written to out).  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (other threads, other handles) race. */
+gv_status gv_neighbors_noalloc(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *offs,
+                               uint32_t *out);
+
 /* static_pagerank: FP32 values, per-vertex compensated pull sums, FP64 L1
  * residual; stops after the first update with residual < eps or after
  * max_iter updates.  out[V] receives x. */
