@@ -66,10 +66,17 @@ def parse():
     p.add_argument("--prec-scale", type=int, default=24)
     p.add_argument("--serve-scale", type=int, default=26)
     p.add_argument("--serve-seconds", type=float, default=20.0)
-    p.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DB", "WRITE_DB", "OUT_JSON"),
+    p.add_argument("--summarize-pmc", nargs=4,
+                   metavar=("FETCH_DB", "WRITE_DB", "BENCH_JSON", "OUT_JSON"),
                    help="turn two rocprofv3 --pmc result databases (FETCH_SIZE, WRITE_SIZE) of "
-                        "this bench into the per-iteration HBM traffic summary of the PageRank "
-                        "iteration, calibrated on the bench's 256 MiB copy, and exit")
+                        "this bench, run with the same flags as given here, into the "
+                        "per-iteration HBM traffic summary of the PageRank iteration, "
+                        "calibrated on the bench's 256 MiB copy (BENCH_JSON: the JSON line "
+                        "that run printed, for its final edge count), and exit")
+    p.add_argument("--serve-leg", type=int, default=1,
+                   help="rmat22: 1 = also run a bounded serving leg (request loop, p50/p99)")
+    p.add_argument("--serve-leg-scale", type=int, default=24)
+    p.add_argument("--serve-leg-seconds", type=float, default=8.0)
     return p.parse_args()
 
 
@@ -233,7 +240,7 @@ def run_rmat(args):
             "gteps": pr_edges / (per_iter_ms / 1e3) / 1e9,
             "gteps_whole_call": pr_edges * it / (float(np.mean(pr_ms)) / 1e3) / 1e9,
         }
-        traffic, tsrc = load_traffic(args)
+        traffic, tsrc = load_traffic(args, E_end)
         out["roofline"] = {
             "kernel": "PageRank flat-stream iteration (k_pf_tiles + k_pf_rows + k_pf_resid), "
                       "timed with HIP events on the graph's stream around the counted "
@@ -251,18 +258,55 @@ def run_rmat(args):
     if tc_ms:
         out["triangle_count"] = {"ms_mean": float(np.mean(tc_ms)), "total_last": int(tc_total[-1])}
     out["rooflines"] = other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms)
+    if pr_ms:
+        it = float(np.mean(pr_iters))
+        out["rooflines"]["pagerank_whole_call"] = _roof(
+            it * (8.0 * pr_edges + 16.0 * V), float(np.mean(pr_ms)),
+            "whole PageRank call (snapshot of the live in-adjacency + iterations + host "
+            "waits), HIP events on the graph's stream; bytes = iterations x (8 E + 16 V)")
 
+    # the plain run's final graph (sorted CSR), the reference for every
+    # full-size check below and for the batched form's final graph
+    final_csr = start_csr = None
     if not args.no_parity:
-        out["parity_full_size"] = full_size_parity(g, V, args, batches, n_raw)
-    out["h2d_inclusive"] = h2d_inclusive(g, G, V, args)
+        par, final_csr, start_csr = full_size_parity(g, G, V, args, batches, n_raw,
+                                                     pr_out if pr_ms else None,
+                                                     pr_iters[-1] if pr_ms else 0)
+        out["parity_full_size"] = par
     if args.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(g, V, args, batches)
+        out["cpu_baseline"] = cpu_baseline(g, V, args, batches, final_csr, start_csr)
+    del start_csr
+    out["h2d_inclusive"] = h2d_inclusive(g, G, V, args)
     if not args.no_tc:
         out["tc_update"] = tc_update(g, G, V, args)
     g.close()
     if args.batched:
-        out["batched"] = run_batched(args, G, skey, V, n_raw, batches, E_end, value)
+        out["batched"] = run_batched(args, G, skey, V, n_raw, batches, E_end, value, final_csr)
+    del final_csr
+    if args.serve_leg:
+        out["serving"] = serve_leg(args)
     return out
+
+
+def serve_leg(args):
+    """Bounded serving leg of the default run: the request loop on directed
+    R-MAT (scale --serve-leg-scale, default 24) over a --serve-leg-seconds
+    trace of Poisson update arrivals (2000/s, sizes log-uniform [100, 100k],
+    70/30) and queries (100/s) with a heavy share that exercises the
+    analytics: 97.5% 1-hop lookups of 1000 vertices, 1% warm-started PageRank
+    refreshes, 1.5% single-source BC.  p50/p99 latency per request kind."""
+    from gv_amd.workloads import run_serve
+
+    class A:
+        serve_scale = args.serve_leg_scale
+        serve_seconds = args.serve_leg_seconds
+        serve_query_mix = (0.975, 0.01, 0.015)
+    r = run_serve(A)
+    keep = ("config", "latency_ms", "service_ms", "p50_ms", "p99_ms",
+            "sustained_edge_updates_per_s", "offered_edge_updates", "trace_span_s",
+            "steps_served", "refused", "pagerank_refresh_iters_mean", "device_busy_fraction",
+            "setup_s")
+    return {k: r[k] for k in keep if k in r}
 
 
 def _roof(nbytes, ms, what):
@@ -324,18 +368,26 @@ def other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms):
     return out
 
 
-def full_size_parity(g, V, args, batches, n_raw):
-    """Parity gate on the benchmark graph itself: the oracle is built from the
-    host R-MAT generator (the same stream as the device one) and given the
-    same warm-up and timed batches; degrees and every sorted neighbour set
-    must equal the device store's after the timed batches."""
+def full_size_parity(g, G, V, args, batches, n_raw, pr_x, pr_iters):
+    """Parity gate on the benchmark graph itself, after the timed batches.
+    The sequential oracle is built from the host R-MAT generator (the same
+    stream as the device one) and given the same warm-up and timed batches;
+    degrees and every sorted neighbour set must equal the device store's.
+    On that CSR the FP64 OpenMP PageRank (pinned to the oracle) runs as many
+    updates as the device's last PageRank call did, and the OpenMP triangle
+    count (pinned to the oracle per vertex) is compared with a device count
+    per vertex and in total.  Returns (summary, sorted CSR of the final
+    graph, sorted CSR of the graph the timed batches start from)."""
     from oracle import oracle as O
     t0 = time.perf_counter()
     s, d = O.gen_rmat(skey(args.seed, 1), args.scale, n_raw)
     o = O.OracleGraph(V, False)
     o.bulk_build(s, d)
     del s, d
+    start_csr = None
     for b in range(args.warmup + args.steps):
+        if b == args.warmup:
+            start_csr = o.export_csr()  # the graph the timed batches start from
         ds, dd, is_, id_ = batches[b]
         o.delete(ds.numpy(), dd.numpy())
         o.insert(is_.numpy(), id_.numpy())
@@ -344,11 +396,34 @@ def full_size_parity(g, V, args, batches, n_raw):
     o.close()
     deg_eq = bool(np.array_equal(goff, ooff))
     sets_eq = deg_eq and bool(np.array_equal(gcols, ocols))
-    return {"degrees_equal": deg_eq, "neighbour_sets_equal": sets_eq,
-            "directed_edges": int(goff[-1]), "check_s": round(time.perf_counter() - t0, 1),
-            "what": "sequential oracle built from the same R-MAT stream and the same "
-                    f"{args.warmup + args.steps} batches, compared with the device store's "
-                    "sorted export after the timed batches"}
+    del goff, gcols
+    res = {"degrees_equal": deg_eq, "neighbour_sets_equal": sets_eq,
+           "directed_edges": int(ooff[-1]),
+           "what": "sequential oracle built from the same R-MAT stream and the same "
+                   f"{args.warmup + args.steps} batches, compared with the device store's "
+                   "sorted export after the timed batches; PageRank and triangle counts of "
+                   "that graph against the OpenMP FP64 / exact references on the oracle's CSR"}
+    if pr_x is not None:
+        x = pr_x.numpy().astype(np.float64)
+        xo, ito, _ = O.omp_pagerank_sym(V, ooff, ocols, fixed_iters=pr_iters, residual=True)
+        err = np.abs(x - xo)
+        res["pagerank"] = {"iters": int(pr_iters), "l1": float(err.sum()),
+                           "linf": float(err.max()), "l1_bound": 1e-6,
+                           "within_bound": bool(err.sum() <= 1e-6),
+                           "reference": "oracle/gv_omp.c omp_pagerank_sym (FP64, x0 = 1/V) at "
+                                        "the device's iteration count"}
+    if not args.no_tc:
+        pv, tot = G.static_tc(g, per_vertex=True)
+        t1 = time.perf_counter()
+        pvo, toto = O.omp_triangle_count(V, ooff, ocols)
+        res["triangles"] = {"total": int(tot), "total_reference": int(toto),
+                            "per_vertex_equal": bool(np.array_equal(pv, pvo)),
+                            "equal": bool(tot == toto and np.array_equal(pv, pvo)),
+                            "reference_s": round(time.perf_counter() - t1, 2),
+                            "reference": "oracle/gv_omp.c omp_triangle_count (exact, min-ID "
+                                         "per vertex)"}
+    res["check_s"] = round(time.perf_counter() - t0, 1)
+    return res, (ooff, ocols), start_csr
 
 
 def h2d_inclusive(g, G, V, args):
@@ -401,8 +476,8 @@ def tc_update(g, G, V, args):
     is_, id_ = G.gen_uniform(g, skey(args.seed, STREAM_INS * 1000 + 999), V, half)
     off.free()
     cols.free()
-    pv, pv2 = DeviceArray(V, np.uint64), DeviceArray(V, np.uint64)
-    tot, tot2 = ctypes.c_uint64(), ctypes.c_uint64()
+    pv = DeviceArray(V, np.uint64)
+    tot = ctypes.c_uint64()
     L = G.lib()
     stream = g.stream()
     ev0, ev1 = Event(), Event()
@@ -420,19 +495,25 @@ def tc_update(g, G, V, args):
         ev1.record(stream)
         ev1.synchronize()
         res[f"{name}_half_ms"] = ev0.elapsed_ms(ev1)
-    G._check(L.gv_triangle_count(g.handle, pv2.ptr, G.GV_MEM_DEVICE, ctypes.byref(tot2)))
-    res["exact"] = bool(tot.value == tot2.value and np.array_equal(pv.numpy(), pv2.numpy()))
+    # the updated counts against the OpenMP exact count of the graph after
+    # the batch (its sorted export), per vertex and in total
+    from oracle import oracle as O
+    goff, gcols = g.export_csr()
+    pvo, toto = O.omp_triangle_count(V, goff, gcols)
+    del goff, gcols
+    res["exact"] = bool(tot.value == toto and np.array_equal(pv.numpy(), pvo))
+    res["reference"] = "oracle/gv_omp.c omp_triangle_count on the graph after the batch"
     res["speedup_vs_recount"] = res["full_recount_ms"] / (res["delete_half_ms"]
                                                           + res["insert_half_ms"])
     res["what"] = (f"one {args.batch}-edge batch (delete half, insert half), each applied with "
                    "its triangle-count update; times include the update itself and the "
                    "host round trips of the call")
-    for a in (ds, dd, is_, id_, pv, pv2):
+    for a in (ds, dd, is_, id_, pv):
         a.free()
     return res
 
 
-def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):
+def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value, final_csr=None):
     """Batched form: the same K timed batches (each a delete half then an
     insert half) applied as one group by gv_batch_apply, followed by one
     compaction at 25% tombstones, on a fresh graph brought to the same state
@@ -476,6 +557,11 @@ def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):
     g2.sync()
     ms = ev0.elapsed_ms(ev1)
     E2 = g2.num_edges()
+    sets_eq = None
+    if final_csr is not None:
+        boff, bcols = g2.export_csr()
+        sets_eq = bool(np.array_equal(boff, final_csr[0]) and np.array_equal(bcols, final_csr[1]))
+        del boff, bcols
     g2.close()
     val = args.batch * args.steps / (ms / 1e3)
     return {
@@ -485,85 +571,157 @@ def run_batched(args, G, skey, V, n_raw, batches, E_plain_end, plain_value):
         "batches_in_group": len(parts),
         "speedup_vs_plain": val / plain_value,
         "final_edges_equal_plain": bool(E2 == E_plain_end),
+        "final_neighbour_sets_equal_plain": sets_eq,
         "form": "one gv_batch_apply over the same timed batches (delete half, insert half "
                 "each) + one compaction at 25% tombstones",
     }
 
 
-def cpu_baseline(g, V, args, batches):
-    """OpenMP CSR baseline (sorted CSR rebuilt per batch by parallel
-    sort-merge) on the same graph and the first batches, timed on this host."""
-    from oracle import oracle as O
-    off, cols = g.export_csr()
-    nbat = min(args.cpu_batches, len(batches))
-    t_total = 0.0
-    done = 0
-    for b in range(nbat):
-        ds, dd, is_, id_ = batches[b]
-        sd, dd_h, si, di = ds.numpy(), dd.numpy(), is_.numpy(), id_.numpy()
+def _median_timed(fn, runs=5, warmup=1, budget_s=None):
+    """Median wall time (s) of `runs` calls of fn after `warmup` untimed ones;
+    with budget_s, stops early once the timed calls exceed it (at least one).
+    Returns (median_s, timed_runs)."""
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
         t0 = time.perf_counter()
-        off, cols = O.omp_apply_batch(V, False, off, cols, sd, dd_h, delete=True)
-        off, cols = O.omp_apply_batch(V, False, off, cols, si, di, delete=False)
-        t_total += time.perf_counter() - t0
-        done += args.batch
+        fn()
+        ts.append(time.perf_counter() - t0)
+        if budget_s is not None and sum(ts) > budget_s:
+            break
+    return float(np.median(ts)), len(ts)
+
+
+def cpu_baseline(g, V, args, batches, final_csr=None, start_csr=None):
+    """CPU baselines on this host, on the same graph and the same batches:
+      updates: the OpenMP CSR baseline (sorted CSR rebuilt per batch by
+        parallel sort-merge) from the graph the timed batches start from,
+        over the first timed batches in order: one untimed, then five timed,
+        median per batch (without the parity check's CSRs it starts from
+        the device graph as it stands);
+      analytics (OpenMP, all host threads, on the plain run's final graph):
+        pull PageRank per iteration, exact triangle count, Brandes BC from
+        the same 16 sources as the device's BC line, each the median of 5
+        after 1 warm-up (the triangle count stops early past 30 s);
+      sequential oracle (one core): the same update batches, median of 5
+        after 1."""
+    from oracle import oracle as O
+    if final_csr is None:
+        final_csr = g.export_csr()
+    if start_csr is None:
+        start_csr, first = final_csr, 0
+    else:
+        first = args.warmup
+    nbat = min(6, len(batches) - first)
+    state = {"off": start_csr[0], "cols": start_csr[1], "b": 0}
+    host = [tuple(a.numpy() for a in batches[first + b]) for b in range(nbat)]
+
+    def one_batch():
+        sd, dd_h, si, di = host[state["b"] % nbat]
+        state["b"] += 1
+        off, cols = O.omp_apply_batch(V, False, state["off"], state["cols"], sd, dd_h,
+                                      delete=True)
+        state["off"], state["cols"] = O.omp_apply_batch(V, False, off, cols, si, di,
+                                                        delete=False)
+
+    t_upd, n_upd = _median_timed(one_batch, runs=max(nbat - 1, 1))
+    del state
     out = {
-        "value": done / t_total,
+        "value": args.batch / t_upd,
         "unit": "edge updates/s",
         "cores": O.omp_threads(),
         "kind": "port",
-        "sample": f"{nbat} of the same 1M-edge batches applied to the same scale-{args.scale} "
-                  "graph by OpenMP sorted-CSR rebuild (parallel radix sort + merge)",
+        "sample": f"1M-edge batches of the same stream (delete half, insert half) applied to "
+                  f"the scale-{args.scale} graph by OpenMP sorted-CSR rebuild (parallel radix "
+                  f"sort + merge): 1 untimed, then the median of {n_upd}",
         "cpu_model": _cpu_model(),
     }
-    out["sequential_oracle"] = cpu_sequential(g, V, args, batches)
+    off, cols = final_csr
+    it = 3
+    t_pr, n_pr = _median_timed(lambda: O.omp_pagerank_sym(V, off, cols, fixed_iters=it))
+    E = int(off[-1])
+    an = {"pagerank_ms_per_iter": t_pr / it * 1e3,
+          "pagerank_gteps": E / (t_pr / it) / 1e9,
+          "pagerank_runs": n_pr}
+    if not args.no_tc:
+        t_tc, n_tc = _median_timed(lambda: O.omp_triangle_count(V, off, cols, per_vertex=False),
+                                   budget_s=30.0)
+        an["triangle_count_ms"] = t_tc * 1e3
+        an["triangle_count_runs"] = n_tc
+    srcs = np.random.default_rng(args.seed).integers(0, V, 16).astype(np.uint32)
+    t_bc, n_bc = _median_timed(lambda: O.omp_bc_dir(V, off, cols, srcs), budget_s=30.0)
+    an["bc_16_sources_ms"] = t_bc * 1e3
+    an["bc_runs"] = n_bc
+    an["what"] = ("OpenMP CSR analytics on the plain run's final graph, all host threads: pull "
+                  "PageRank (FP64), exact triangle count (degree-oriented lists), Brandes BC "
+                  "from the 16 sources of the device's BC line; median of the runs after 1 "
+                  "warm-up")
+    out["analytics"] = an
+    out["sequential_oracle"] = cpu_sequential(V, args, batches, start_csr[0], start_csr[1], first)
     return out
 
 
-def cpu_sequential(g, V, args, batches):
+def cpu_sequential(V, args, batches, off, cols, first):
     """The host-sequential oracle (one core: sorted adjacency arrays, batch
-    radix-sorted and merged per touched vertex) loaded with the same graph
-    (not timed), then one of the same 1M-edge batches timed."""
+    radix-sorted and merged per touched vertex) loaded with the graph the
+    timed batches start from (not timed), then the first timed batches in
+    order: 1 untimed, the median of 5."""
     from oracle import oracle as O
-    off, cols = g.export_csr()
     u = np.repeat(np.arange(V, dtype=np.uint32), np.diff(off).astype(np.int64))
     keep = u < cols
     o = O.OracleGraph(V, False)
     o.bulk_build(u[keep], cols[keep])
-    del u, keep, off, cols
-    ds, dd, is_, id_ = batches[0]
-    sd, dd_h, si, di = ds.numpy(), dd.numpy(), is_.numpy(), id_.numpy()
-    t0 = time.perf_counter()
-    o.delete(sd, dd_h)
-    o.insert(si, di)
-    t = time.perf_counter() - t0
+    del u, keep
+    nbat = min(6, len(batches) - first)
+    host = [tuple(a.numpy() for a in batches[first + b]) for b in range(nbat)]
+    state = {"b": 0}
+
+    def one_batch():
+        sd, dd_h, si, di = host[state["b"] % nbat]
+        state["b"] += 1
+        o.delete(sd, dd_h)
+        o.insert(si, di)
+
+    t, n = _median_timed(one_batch, runs=max(nbat - 1, 1))
     o.close()
     return {"value": args.batch / t, "unit": "edge updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 of the same 1M-edge batches (delete half, insert half) on the "
-                      f"scale-{args.scale} graph, oracle/gv_oracle.c on one core"}
+            "sample": f"1M-edge batches of the same stream (delete half, insert half) on the "
+                      f"scale-{args.scale} graph, oracle/gv_oracle.c on one core: 1 untimed, "
+                      f"then the median of {n}"}
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pagerank_flat_rmat22_traffic.json")
 PR_ITER_KERNELS = ("k_pf_tiles", "k_pf_rows", "k_pf_resid")
 
 
-def load_traffic(args):
+def _traffic_key(args, edges_final):
+    return {"scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed,
+            "pr_layout": args.pr_layout, "steps": args.steps, "warmup": args.warmup,
+            "batch": args.batch, "directed_edges_final": int(edges_final)}
+
+
+def load_traffic(args, edges_final):
     """Measured HBM bytes per PageRank iteration (rocprofv3 FETCH_SIZE +
     WRITE_SIZE, calibrated) for this workload, if a summary made by
-    --summarize-pmc for the same graph and layout is committed."""
+    --summarize-pmc of a run of the same shape is committed: same graph,
+    layout, warm-up and timed batch counts, batch size and final edge count
+    (the graph the last PageRank call ran on).  Anything else is refused."""
     try:
         with open(TRAFFIC_JSON) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None, None
-    w = d.get("workload", {})
-    if (w.get("scale"), w.get("edge_factor"), w.get("seed"), w.get("pr_layout")) != \
-            (args.scale, args.edge_factor, args.seed, args.pr_layout):
+    if d.get("workload") != _traffic_key(args, edges_final):
         return None, None
     return d["traffic_bytes_per_iteration"], os.path.relpath(TRAFFIC_JSON, ROOT)
 
 
-def summarize_pmc(args, fetch_db, write_db, out_json):
+def summarize_pmc(args, fetch_db, write_db, bench_json, out_json):
     import sqlite3
+    with open(bench_json) as f:
+        line = [x for x in f if x.lstrip().startswith("{")][-1]
+    edges_final = json.loads(line)["config"]["directed_edges_final"]
 
     def per_dispatch(db, counter):
         c = sqlite3.connect(db)
@@ -592,11 +750,11 @@ def summarize_pmc(args, fetch_db, write_db, out_json):
         res[counter] = {"calibration_factor": factor, "copy_kib": max(copies), "kernels": kern}
     total = sum(k["bytes_calibrated"] for c in res.values() for k in c["kernels"].values())
     out = {
-        "workload": {"scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed,
-                     "pr_layout": args.pr_layout},
+        "workload": _traffic_key(args, edges_final),
         "traffic_bytes_per_iteration": total,
         "counters": res,
-        "made_by": "bench.py --summarize-pmc FETCH_DB WRITE_DB OUT_JSON",
+        "made_by": "bench.py [the profiled run's flags] --summarize-pmc FETCH_DB WRITE_DB "
+                   "BENCH_JSON OUT_JSON",
     }
     with open(out_json, "w") as f:
         json.dump(out, f, indent=1)

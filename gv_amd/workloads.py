@@ -268,7 +268,8 @@ def run_serve(args):
     g = GraphVine(V, directed=True)
     build_rmat_chunked(g, scale, seed)
     E0 = g.num_edges()
-    ev = serve_trace(V, seconds, seed)
+    mix = tuple(getattr(args, "serve_query_mix", (0.995, 0.002, 0.003)))
+    ev = serve_trace(V, seconds, seed, query_mix=mix)
     upd = [(i, e) for i, e in enumerate(ev) if e[1] in ("insert", "delete")]
     total = sum(e[2] for _, e in upd)
     us, ud = DeviceArray(max(total, 1), np.uint32), DeviceArray(max(total, 1), np.uint32)
@@ -343,8 +344,9 @@ def run_serve(args):
             "workload": f"directed R-MAT scale {scale}, edge factor 16, seed {seed}, in-edge "
                         f"mirror; {seconds:.0f} s trace: updates Poisson 2000/s, sizes "
                         "log-uniform [100, 100k], 70% insert / 30% delete; queries Poisson "
-                        "100/s: 99.5% 1-hop lookups of 1000 vertices, 0.2% PageRank refresh "
-                        "(warm start, L1 1e-6), 0.3% single-source BC",
+                        f"100/s: {100 * mix[0]:.1f}% 1-hop lookups of 1000 vertices, "
+                        f"{100 * mix[1]:.1f}% PageRank refresh (warm start, L1 1e-6), "
+                        f"{100 * mix[2]:.1f}% single-source BC",
             "vertices": V,
             "directed_edges_initial": int(E0),
             "directed_edges_final": int(E_end),

@@ -136,16 +136,21 @@ uint64_t omp_apply_batch(uint32_t V, int directed, const uint64_t *off, const ui
   return s;
 }
 
-/* Pull PageRank over a symmetric CSR (in = out), same rule as the oracle;
- * FP64.  Returns the number of updates. */
+/* Pull PageRank over a symmetric CSR (in = out), same rule as the oracle
+ * (orc_pagerank: pagerank.cu:17-59); FP64.  x0 (may be NULL = 1/V) is the
+ * start vector; fixed_iters > 0 runs exactly that many updates.  Returns the
+ * number of updates; *residual (may be NULL) gets the last L1 residual. */
 int omp_pagerank_sym(uint32_t V, const uint64_t *off, const uint32_t *cols, double alpha, double eps,
-                     int max_iter, double *x) {
+                     int max_iter, int fixed_iters, const double *x0, double *x,
+                     double *residual) {
   double *c = (double *)malloc(sizeof(double) * (V ? V : 1));
   double *xn = (double *)malloc(sizeof(double) * (V ? V : 1));
-  for (uint32_t v = 0; v < V; ++v) x[v] = 1.0 / V;
+  for (uint32_t v = 0; v < V; ++v) x[v] = x0 ? x0[v] : 1.0 / V;
   double base = (1.0 - alpha) / V;
+  const int limit = fixed_iters > 0 ? fixed_iters : max_iter;
   int it = 0;
-  while (it < max_iter) {
+  double rlast = 0.0;
+  while (it < limit) {
 #pragma omp parallel for schedule(static)
     for (uint32_t v = 0; v < V; ++v) {
       uint64_t d = off[v + 1] - off[v];
@@ -161,11 +166,86 @@ int omp_pagerank_sym(uint32_t V, const uint64_t *off, const uint32_t *cols, doub
     }
     memcpy(x, xn, sizeof(double) * V);
     ++it;
-    if (r < eps) break;
+    rlast = r;
+    if (fixed_iters <= 0 && r < eps) break;
   }
+  if (residual) *residual = rlast;
   free(c);
   free(xn);
   return it;
+}
+
+/* Exact triangle count of an undirected graph given as a sorted symmetric
+ * CSR, with the oracle's per-vertex definition (tc_kernel.cu:27-73):
+ * t[u] = triangles whose minimum vertex ID is u.  Each triangle is found
+ * once over degree-oriented lists (u -> v when (deg u, u) < (deg v, v)):
+ * a thread marks N+(u) in its own byte map, then every w in N+(v) for
+ * v in N+(u) that is marked closes {u, v, w}.  The count goes to
+ * min(u, v, w); each thread keeps its own per-vertex array and the arrays
+ * are added in thread order, so the result is exact and independent of
+ * scheduling.  per_vertex may be NULL.  Returns the total. */
+uint64_t omp_triangle_count(uint32_t V, const uint64_t *off, const uint32_t *cols,
+                            uint64_t *per_vertex) {
+  const int nt = omp_get_max_threads();
+  const uint64_t E = off[V];
+  uint64_t *poff = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)V + 1));
+  uint32_t *pcols = (uint32_t *)malloc(sizeof(uint32_t) * (E ? E : 1));
+#define GV_BEFORE(a, b)                                                                        \
+  ((off[(a) + 1] - off[a]) < (off[(b) + 1] - off[b]) ||                                      \
+   ((off[(a) + 1] - off[a]) == (off[(b) + 1] - off[b]) && (a) < (b)))
+#pragma omp parallel for schedule(static)
+  for (uint32_t u = 0; u < V; ++u) {
+    uint64_t c = 0;
+    for (uint64_t j = off[u]; j < off[u + 1]; ++j)
+      if (GV_BEFORE(u, cols[j])) ++c;
+    poff[u + 1] = c;
+  }
+  poff[0] = 0;
+  for (uint32_t u = 0; u < V; ++u) poff[u + 1] += poff[u];
+#pragma omp parallel for schedule(static)
+  for (uint32_t u = 0; u < V; ++u) {
+    uint64_t o = poff[u];
+    for (uint64_t j = off[u]; j < off[u + 1]; ++j)
+      if (GV_BEFORE(u, cols[j])) pcols[o++] = cols[j];
+  }
+#undef GV_BEFORE
+  uint64_t *part = NULL;
+  if (per_vertex) part = (uint64_t *)calloc((size_t)nt * (V ? V : 1), sizeof(uint64_t));
+  uint64_t total = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : total)
+  {
+    uint64_t *t = per_vertex ? part + (size_t)omp_get_thread_num() * V : NULL;
+    uint8_t *mark = (uint8_t *)calloc(V ? V : 1, 1);
+#pragma omp for schedule(dynamic, 64)
+    for (uint32_t u = 0; u < V; ++u) {
+      for (uint64_t j = poff[u]; j < poff[u + 1]; ++j) mark[pcols[j]] = 1;
+      for (uint64_t j = poff[u]; j < poff[u + 1]; ++j) {
+        const uint32_t v = pcols[j];
+        const uint32_t muv = u < v ? u : v;
+        for (uint64_t k = poff[v]; k < poff[v + 1]; ++k) {
+          const uint32_t w = pcols[k];
+          if (mark[w]) {
+            ++total;
+            if (t) t[w < muv ? w : muv]++;
+          }
+        }
+      }
+      for (uint64_t j = poff[u]; j < poff[u + 1]; ++j) mark[pcols[j]] = 0;
+    }
+    free(mark);
+  }
+  if (per_vertex) {
+#pragma omp parallel for schedule(static)
+    for (uint32_t v = 0; v < V; ++v) {
+      uint64_t s = 0;
+      for (int k = 0; k < nt; ++k) s += part[(size_t)k * V + v];
+      per_vertex[v] = s;
+    }
+    free(part);
+  }
+  free(poff);
+  free(pcols);
+  return total;
 }
 
 /* In-lists of a CSR of out-lists: in_off[V+1], in_cols[E], each in-list in

@@ -318,13 +318,16 @@ int orc_batch_delete(orc_graph *g, const uint32_t *src, const uint32_t *dst, uin
  *   x0 = 1/V; x'[v] = (1-a)/V + a * sum_{u: v in N(u)} x[u]/|N(u)|;
  *   r = sum |x'-x|; stop after the first update with r < eps or after
  *   max_iter updates.  Dangling mass is dropped (pr_kernel.cu:40).
- * fixed_iters > 0 runs exactly that many updates (replay of a GPU run).
- * Returns the number of updates performed. */
+ * x0 (may be NULL) replaces the 1/V start: the warm start of
+ * dynamic_pagerank (pagerank.cu:70-132), which iterates the same update
+ * from the previous vector (its affected-set heuristic, pr_kernel.cu:88, is
+ * not reproduced).  fixed_iters > 0 runs exactly that many updates (replay
+ * of a GPU run).  Returns the number of updates performed. */
 int orc_pagerank(const orc_graph *g, double alpha, double eps, int max_iter, int fixed_iters,
-                 double *x, double *residual) {
+                 const double *x0, double *x, double *residual) {
   uint32_t V = g->V;
   double *acc = (double *)malloc(sizeof(double) * (V ? V : 1));
-  for (uint32_t i = 0; i < V; ++i) x[i] = 1.0 / (double)V;
+  for (uint32_t i = 0; i < V; ++i) x[i] = x0 ? x0[i] : 1.0 / (double)V;
   double base = (1.0 - alpha) / (double)V, r = 0.0;
   int limit = fixed_iters > 0 ? fixed_iters : max_iter, it = 0;
   while (it < limit) {

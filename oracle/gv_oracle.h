@@ -58,8 +58,9 @@ void orc_degrees(const orc_graph *g, uint32_t *deg);
 void orc_export_csr(const orc_graph *g, uint64_t *offsets, uint32_t *cols);
 
 /* analytics */
+/* x0: start vector (NULL = 1/V everywhere) */
 int orc_pagerank(const orc_graph *g, double alpha, double eps, int max_iter, int fixed_iters,
-                 double *x, double *residual);
+                 const double *x0, double *x, double *residual);
 int orc_pagerank_personalized(const orc_graph *g, double alpha, double eps, int max_iter,
                               int fixed_iters, const uint32_t *seeds, uint32_t ncols, double *x,
                               double *max_residual);

@@ -56,12 +56,13 @@ def lib():
         "orc_num_vertices": ([P], u32),
         "orc_degrees": ([P, P], None),
         "orc_export_csr": ([P, P, P], None),
-        "orc_pagerank": ([P, f64, f64, i32, i32, P, ctypes.POINTER(f64)], i32),
+        "orc_pagerank": ([P, f64, f64, i32, i32, P, P, ctypes.POINTER(f64)], i32),
         "orc_pagerank_personalized": ([P, f64, f64, i32, i32, P, u32, P, ctypes.POINTER(f64)], i32),
         "orc_triangle_count": ([P, P], u64),
         "orc_bc": ([P, P, u32, P], None),
         "omp_apply_batch": ([u32, i32, P, P, P, P, u64, i32, P, P], u64),
-        "omp_pagerank_sym": ([u32, P, P, f64, f64, i32, P], i32),
+        "omp_pagerank_sym": ([u32, P, P, f64, f64, i32, i32, P, P, ctypes.POINTER(f64)], i32),
+        "omp_triangle_count": ([u32, P, P, P], u64),
         "omp_threads": ([], i32),
         "omp_transpose": ([u32, P, P, P, P], None),
         "omp_pagerank_dir": ([u32, P, P, P, f64, f64, i32, i32, P, ctypes.POINTER(f64)], i32),
@@ -175,10 +176,18 @@ class OracleGraph:
         lib().orc_export_csr(self._g, _p(off), _p(cols))
         return off, cols[: int(off[-1])]
 
-    def pagerank(self, alpha=0.85, eps=1e-6, max_iter=100, fixed_iters=0):
+    def pagerank(self, alpha=0.85, eps=1e-6, max_iter=100, fixed_iters=0, x0=None):
+        """x0: start vector (default 1/V), e.g. the float32 result of an
+        earlier run for a warm-started refresh."""
         x = np.empty(self.V, np.float64)
         r = ctypes.c_double()
-        it = lib().orc_pagerank(self._g, alpha, eps, max_iter, fixed_iters, _p(x), ctypes.byref(r))
+        start = None
+        if x0 is not None:
+            start = np.ascontiguousarray(x0, dtype=np.float64)
+            if len(start) != self.V:
+                raise ValueError("x0 must have V entries")
+        it = lib().orc_pagerank(self._g, alpha, eps, max_iter, fixed_iters,
+                                _p(start) if start is not None else None, _p(x), ctypes.byref(r))
         return x, int(it), float(r.value)
 
     def pagerank_personalized(self, seeds, alpha=0.85, eps=1e-6, max_iter=100, fixed_iters=0):
@@ -214,12 +223,32 @@ def omp_apply_batch(V, directed, off, cols, src, dst, delete):
     return off_out, cols_out[:n]
 
 
-def omp_pagerank_sym(V, off, cols, alpha=0.85, eps=1e-6, max_iter=100):
+def omp_pagerank_sym(V, off, cols, alpha=0.85, eps=1e-6, max_iter=100, fixed_iters=0, x0=None,
+                     residual=False):
+    """FP64 pull PageRank of a symmetric CSR (the oracle's rule); x0: start
+    vector (default 1/V); fixed_iters > 0 runs exactly that many updates.
+    Returns (x, iters) or, with residual=True, (x, iters, last residual)."""
     x = np.empty(V, np.float64)
     off = np.ascontiguousarray(off, np.uint64)
     cols = u32(cols) if len(cols) else np.zeros(1, np.uint32)
-    it = lib().omp_pagerank_sym(V, _p(off), _p(cols), alpha, eps, max_iter, _p(x))
-    return x, int(it)
+    start = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+    if start is not None and len(start) != V:
+        raise ValueError("x0 must have V entries")
+    r = ctypes.c_double()
+    it = lib().omp_pagerank_sym(V, _p(off), _p(cols), alpha, eps, max_iter, fixed_iters,
+                                _p(start) if start is not None else None, _p(x), ctypes.byref(r))
+    return (x, int(it), float(r.value)) if residual else (x, int(it))
+
+
+def omp_triangle_count(V, off, cols, per_vertex=True):
+    """Exact triangle count of a sorted symmetric CSR (OpenMP, degree-oriented
+    lists, one marked neighbour set per thread); returns (per-vertex min-ID
+    counts or None, total)."""
+    off = np.ascontiguousarray(off, np.uint64)
+    cols = u32(cols) if len(cols) else np.zeros(1, np.uint32)
+    pv = np.empty(V, np.uint64) if per_vertex else None
+    tot = lib().omp_triangle_count(V, _p(off), _p(cols), _p(pv) if pv is not None else None)
+    return pv, int(tot)
 
 
 def omp_transpose(V, off, cols):

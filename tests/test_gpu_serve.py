@@ -25,6 +25,30 @@ def assert_same(g, o, where=""):
     assert np.array_equal(cols, ec), f"neighbour sets differ {where}"
 
 
+PR_TOL = 1e-6  # PageRank L1 bar, against the oracle at equal iterations and mix vs alone
+
+
+def oracle_from_csr(V, directed, off, cols):
+    g = O.OracleGraph(V, directed)
+    g.bulk_build(np.repeat(np.arange(V, dtype=np.uint32), np.diff(off).astype(np.int64)), cols)
+    return g
+
+
+def pagerank_starts(reqs, expect):
+    """Start vector of each refresh in the mix: a refresh warm-starts from the
+    result of the previous serving step that ran PageRank (1/V for the first);
+    requests served by one step share its run."""
+    starts, prev, prev_step, cur = {}, None, None, None
+    order = sorted((r.response["step"], i) for i, (r, e) in enumerate(zip(reqs, expect))
+                   if e[0] == "pr")
+    for step, i in order:
+        if step != prev_step:
+            prev, prev_step = cur, step
+        starts[i] = prev
+        cur = reqs[i].vector().astype(np.float64)
+    return starts
+
+
 def make_trace(o, V, seed, n_req=120, burst=20):
     """Requests with expected answers computed on the oracle at each request's
     point in the arrival order."""
@@ -53,9 +77,10 @@ def make_trace(o, V, seed, n_req=120, burst=20):
             reqs.append(S.neighbors(vs, out_cap=sum(len(x) for x in lists) + 16))
             expect.append(("nbr", vs, lists))
         elif u < 0.95:
-            x, _, _ = o.pagerank(eps=1e-12, max_iter=1000)
+            # the graph at this point; the start vector (the previous
+            # refresh's result) is known only after the replay
             reqs.append(S.pagerank(V))
-            expect.append(("pr", x))
+            expect.append(("pr",) + o.export_csr())
         else:
             src = int(rng.integers(0, V))
             reqs.append(S.bc(src, V))
@@ -81,19 +106,27 @@ def test_mix_equals_alone(directed):
     assert st["requests_done"] == len(reqs) and st["steps"] < len(reqs)
     for r in resp:
         assert r["done_us"] >= r["start_us"] >= r["arrival_us"] - 1.0
-    # in the mix vs the oracle at the same point of the arrival order
-    for req, exp in zip(reqs, expect):
+    # in the mix vs the oracle at the same point of the arrival order; a
+    # refresh against the oracle from the same start at equal iterations
+    starts = pagerank_starts(reqs, expect)
+    assert sum(e[0] == "pr" for e in expect) >= 2
+    for i, (req, exp) in enumerate(zip(reqs, expect)):
         if exp[0] == "nbr":
             offs, out = req.neighbours()
             for k, lst in enumerate(exp[2]):
                 assert np.array_equal(out[offs[k]:offs[k + 1]], lst)
         elif exp[0] == "pr":
-            assert np.abs(req.vector().astype(np.float64) - exp[1]).sum() < 2e-5
+            og = oracle_from_csr(V, directed, exp[1], exp[2])
+            it = req.response["iters"]
+            xo, _, _ = og.pagerank(fixed_iters=it, x0=starts[i])
+            assert np.abs(req.vector().astype(np.float64) - xo).sum() <= PR_TOL
         elif exp[0] == "bc":
             b, bo = req.vector().astype(np.float64), exp[2]
             assert (np.abs(b - bo) / np.maximum(np.abs(bo), 1.0)).max() <= 1e-5
     assert_same(g, o, "after the mix")
-    # alone: each request on its own, in arrival order, on a second graph
+    # alone: each request on its own, in arrival order, on a second graph; a
+    # PageRank refresh alone warm-starts from the previous refresh alone
+    prev_alone = None
     for req, exp in zip(reqs, expect):
         if exp[0] == "ins":
             ga.batchInsert(exp[1], exp[2])
@@ -104,8 +137,9 @@ def test_mix_equals_alone(directed):
             offs, out = req.neighbours()
             assert np.array_equal(offs_a, offs) and np.array_equal(out_a, out)
         elif exp[0] == "pr":
-            xa, _, _ = G.pagerank(ga)
-            assert np.abs(xa.astype(np.float64) - req.vector()).sum() < 2e-5
+            xa, _, _ = G.pagerank(ga, init=prev_alone)
+            prev_alone = xa
+            assert np.abs(xa.astype(np.float64) - req.vector()).sum() <= PR_TOL
         else:
             ba = G.bc(ga, np.array([exp[1]], np.uint32)).astype(np.float64)
             b = req.vector().astype(np.float64)

@@ -212,6 +212,34 @@ def test_pagerank_cycle_is_uniform_every_iteration():
         assert it == k and np.allclose(x, 1.0 / V, rtol=0, atol=1e-15) and r < 1e-15
 
 
+def test_pagerank_start_vector():
+    """x0 = 1/V reproduces the default start bit for bit; on a directed cycle
+    (v -> v+1) from an arbitrary start the iterate has the closed form
+    x_t[v] = (1-a)/V * sum_{k<t} a^k + a^t * x0[v-t mod V], which pins both the
+    start and the direction of the shift; and a start at the fixed point stops
+    after one update."""
+    a = 0.85
+    g = O.OracleGraph(40, False)
+    s, d = O.gen_rmat(O.stream_key(5, O.STREAM_RMAT), 6, 300)
+    g.bulk_build(s % 40, d % 40)
+    x_def, it_def, r_def = g.pagerank(eps=1e-9, max_iter=500)
+    x_one, it_one, r_one = g.pagerank(eps=1e-9, max_iter=500, x0=np.full(40, 1.0 / 40))
+    assert it_one == it_def and r_one == r_def and np.array_equal(x_one, x_def)
+    V = 9
+    c = O.OracleGraph(V, True)
+    c.bulk_build(np.arange(V), (np.arange(V) + 1) % V)
+    x0 = np.random.default_rng(3).random(V)
+    for t in range(1, 12):
+        x, it, _ = c.pagerank(alpha=a, fixed_iters=t, x0=x0)
+        want = (1 - a) / V * sum(a ** k for k in range(t)) + a ** t * np.roll(x0, t)
+        assert it == t and np.allclose(x, want, rtol=0, atol=1e-15)
+    xf, _, _ = g.pagerank(eps=1e-15, max_iter=5000)
+    x2, it2, r2 = g.pagerank(eps=1e-9, max_iter=500, x0=xf)
+    assert it2 == 1 and r2 < 1e-9 and np.abs(x2 - xf).max() < 1e-15
+    with pytest.raises(ValueError):
+        g.pagerank(x0=np.ones(3))
+
+
 def test_pagerank_two_vertex_dangling_dropped():
     a = 0.85
     g = O.OracleGraph(2, True)
@@ -431,6 +459,48 @@ def test_omp_baseline_matches_oracle():
     x, it = O.omp_pagerank_sym(V, off, cols)
     xo, ito, _ = g.pagerank()
     assert it == ito and np.abs(x - xo).sum() < 1e-12
+
+
+def test_omp_pagerank_start_vector_and_fixed_iterations():
+    V = 1500
+    s, d = O.gen_rmat(O.stream_key(7, O.STREAM_RMAT), 11, 12000)
+    g = O.OracleGraph(V, False)
+    g.bulk_build(s % V, d % V)
+    off, cols = g.export_csr()
+    x0 = np.random.default_rng(7).random(V) / V
+    for fixed in (1, 6):
+        x, it, r = O.omp_pagerank_sym(V, off, cols, fixed_iters=fixed, x0=x0, residual=True)
+        xo, ito, ro = g.pagerank(fixed_iters=fixed, x0=x0)
+        assert it == ito == fixed and np.abs(x - xo).max() <= 1e-15
+        assert abs(r - ro) <= 1e-12 * ro
+    x, it, r = O.omp_pagerank_sym(V, off, cols, eps=1e-9, max_iter=300, x0=x0, residual=True)
+    xo, ito, ro = g.pagerank(eps=1e-9, max_iter=300, x0=x0)
+    assert it == ito and r < 1e-9 and np.abs(x - xo).max() <= 1e-15
+
+
+@pytest.mark.parametrize("seed,V,n", [(8, 300, 3000), (9, 2048, 40000), (10, 40, 700)])
+def test_omp_triangle_count_matches_oracle(seed, V, n):
+    """The OpenMP triangle count (full-size reference of the
+    benchmark's triangle parity) equals the sequential oracle per vertex --
+    itself pinned by closed forms and brute force above -- on R-MAT graphs
+    with hubs (orientation by degree puts hubs last), and on K_n."""
+    s, d = O.gen_rmat(O.stream_key(seed, O.STREAM_RMAT), 12, n)
+    g = O.OracleGraph(V, False)
+    g.bulk_build(s % V, d % V)
+    off, cols = g.export_csr()
+    pv, tot = O.omp_triangle_count(V, off, cols)
+    pvo, toto = g.triangles()
+    assert tot == toto > 0 and np.array_equal(pv, pvo)
+    _, tot2 = O.omp_triangle_count(V, off, cols, per_vertex=False)
+    assert tot2 == tot
+    k = 9
+    kn = O.OracleGraph(k, False)
+    iu, ju = np.triu_indices(k, 1)
+    kn.bulk_build(iu.astype(np.uint32), ju.astype(np.uint32))
+    off, cols = kn.export_csr()
+    pv, tot = O.omp_triangle_count(k, off, cols)
+    assert tot == k * (k - 1) * (k - 2) // 6
+    assert pv.tolist() == [(k - 1 - i) * (k - 2 - i) // 2 for i in range(k)]
 
 
 @pytest.mark.parametrize("directed", [True, False])
