@@ -48,10 +48,11 @@ def parse():
     p.add_argument("--cpu-baseline", type=int, default=1, help="1 = time the CPU baseline")
     p.add_argument("--cpu-batches", type=int, default=2)
     p.add_argument("--batched", type=int, default=1, help="1 = also time the batched form")
-    p.add_argument("--pr-layout", type=int, default=1,
-                   help="PageRank layout: 1 = degree-relabelled CSR snapshot (default), "
-                        "3 = snapshot in vertex order, 2 = pull over the store, "
-                        "0 = XCD-segmented snapshot")
+    p.add_argument("--pr-layout", type=int, default=5,
+                   help="PageRank layout: 5 = degree-relabelled snapshot, hot stream + "
+                        "propagation-blocked cold stream (default), 1 = the same snapshot as "
+                        "one flat stream, 3 = snapshot in vertex order, 2 = pull over the "
+                        "store, 0 = XCD-segmented snapshot")
     p.add_argument("--workload", default="rmat22",
                    choices=["rmat22", "chunglu", "precision", "serve"],
                    help="rmat22: plain vs batched update stream on undirected R-MAT (default); "
@@ -242,10 +243,12 @@ def run_rmat(args):
         }
         traffic, tsrc = load_traffic(args, E_end)
         out["roofline"] = {
-            "kernel": "PageRank flat-stream iteration (k_pf_tiles + k_pf_rows + k_pf_resid), "
-                      "timed with HIP events on the graph's stream around the counted "
-                      "iterations (launch gaps included)"
-                      if args.pr_layout == 1 else f"PageRank iteration, layout {args.pr_layout}",
+            "kernel": ("PageRank iteration, propagation-blocked layout (k_pb_tiles + k_pb_rows "
+                       "+ k_pb_expand)" if args.pr_layout == 5 else
+                       "PageRank flat-stream iteration (k_pf_tiles + k_pf_rows + k_pf_resid)"
+                       if args.pr_layout == 1 else f"PageRank iteration, layout {args.pr_layout}")
+                      + ", timed with HIP events on the graph's stream around the counted "
+                        "iterations (launch gaps included)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -692,7 +695,8 @@ def cpu_sequential(V, args, batches, off, cols, first):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pagerank_flat_rmat22_traffic.json")
-PR_ITER_KERNELS = ("k_pf_tiles", "k_pf_rows", "k_pf_resid")
+PR_ITER_KERNELS = ("k_pf_tiles", "k_pf_rows", "k_pf_resid", "k_pb_tiles", "k_pb_rows",
+                   "k_pb_expand")
 
 
 def _traffic_key(args, edges_final):

@@ -35,6 +35,13 @@ struct PrCtl {
   unsigned int pad;
 };
 
+// Capacities (bytes) of the propagation-blocked layout's buffers.
+struct PbCaps {
+  size_t hitem = 0, hoff = 0, coff = 0, lidxr = 0, lng = 0, Sh = 0, Sc = 0, partials = 0;
+  size_t hot = 0, cold = 0, csrc = 0, lidx = 0, vals = 0, htb = 0, ctb = 0, PQh = 0, PQc = 0;
+  size_t cnt = 0, ngrp = 0, gaddr = 0, nit = 0, tmp = 0, items = 0;
+};
+
 struct PrWork {
   uint32_t V = 0;
   float *x[2] = {nullptr, nullptr};
@@ -85,6 +92,28 @@ struct PrWork {
   uint32_t pf_prepV = 0, pf_n_nz = 0, pf_n_long = 0;
   uint64_t pf_E = 0, pf_n_tiles = 0;
   double *pf_partials = nullptr;
+  // propagation-blocked layout (gv_pagerank_pb.hip), built per snapshot:
+  // hot and cold 2-byte streams, their row offsets (label order), tile bases
+  // and row partials; the expansion stream of cold source offsets, the
+  // values it fills, the (source block, bin) table, per bin the offset of its
+  // image groups and per image group its place in the values, the expansion
+  // work items
+  uint16_t *pb_hot = nullptr, *pb_cold = nullptr, *pb_lidx = nullptr;
+  uint32_t *pb_csrc = nullptr, *pb_hoff = nullptr, *pb_coff = nullptr;
+  int32_t *pb_htb = nullptr, *pb_ctb = nullptr;
+  float2 *pb_Sh = nullptr, *pb_Sc = nullptr, *pb_PQh = nullptr, *pb_PQc = nullptr;
+  float *pb_vals = nullptr;
+  uint32_t *pb_hitem = nullptr, *pb_cnt = nullptr, *pb_ngrp = nullptr, *pb_nit = nullptr;
+  uint32_t *pb_gaddr = nullptr;
+  uint4 *pb_items = nullptr;
+  uint32_t *pb_lidxr = nullptr, *pb_long = nullptr;
+  void *pb_tmp = nullptr;
+  double *pb_partials = nullptr;
+  PbCaps pb_caps;
+  int pb_ready = 0;
+  uint32_t pb_NS = 0, pb_NB = 0, pb_nht = 0, pb_nct = 0, pb_n_long = 0, pb_prepV = 0;
+  uint32_t pb_bin_tiles = 0;
+  uint64_t pb_E = 0;
 };
 
 // Run parameters of one PageRank call.
@@ -153,8 +182,25 @@ gv_status pagerank_flat_prepare(const Store &in, const uint32_t *deg_out, PrWork
 gv_status pagerank_flat_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
                                 double *residual, hipStream_t s, double *iter_ms);
 void pr_release(PrWork &w);
+// Propagation-blocked PageRank over the relabelled snapshot (same contract
+// as the flat form): prepare builds the snapshot and both streams on s and
+// returns GV_E_UNSUPPORTED when the graph exceeds the layout's limits (more
+// than 16384 cold source blocks, 32-bit stream positions); iterate runs the
+// iterations over w alone.
+gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &w, hipStream_t s);
+gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
+                              double *residual, hipStream_t s, double *iter_ms);
+void pb_release(PrWork &w);
+// Snapshot + layout for PageRank by layout id (5: propagation-blocked, with
+// the flat stream as fallback; otherwise the flat stream), then iterations
+// over whichever was prepared.
+gv_status pagerank_snap_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
+                                hipStream_t s, int layout);
+gv_status pagerank_snap_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
+                                double *residual, hipStream_t s, double *iter_ms);
 // Pull PageRank over `in` (in-adjacency; the out-store itself for undirected
 // graphs) with out-degrees deg_out.  On return out_dev[V] holds x.
+// snapshot = 5: the propagation-blocked layout (falling back to 1);
 // snapshot = 1: iterate over a copy of the live in-adjacency built at the
 // start of the call, vertices relabelled by descending out-degree, as the
 // flat tiled stream (falling back to the per-row form of 3 when the graph

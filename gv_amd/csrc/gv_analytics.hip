@@ -88,6 +88,7 @@ void pr_release(PrWork &w) {
   hipFree(w.rl_keys[0]);
   hipFree(w.rl_keys[1]);
   hipFree(w.rl_tmp);
+  pb_release(w);
   hipFree(w.pf_tbase);
   hipFree(w.pf_P);
   hipFree(w.pf_Q);
@@ -838,6 +839,13 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   if (!V) return GV_OK;
   gv_status st = pr_reserve(w, V);
   if (st) return st;
+  if (snapshot == 5 && max_iter > 0) {
+    st = pagerank_pb_prepare(in, deg_out, w, s);
+    if (st == GV_OK) return pagerank_pb_iterate(o, w, out_dev, iters, residual, s, iter_ms);
+    if (st != GV_E_UNSUPPORTED) return st;
+    snapshot = 1;
+  }
+  if (snapshot == 5) snapshot = 1;
   if (snapshot == 1) {
     st = pagerank_flat_run(in, deg_out, o, out_dev, iters, residual, w, s, iter_ms);
     if (st != GV_E_UNSUPPORTED) return st;

@@ -93,7 +93,7 @@ gv_status gv_config_default(gv_config *cfg) {
   cfg->pr_wave_max = 4096;
   cfg->maintain_in_edges = 1;
   cfg->device = 0;
-  cfg->pr_layout = 1;
+  cfg->pr_layout = 5;
   return GV_OK;
 }
 
@@ -409,7 +409,7 @@ gv_status graph_pr_prepare(gv_graph *g, PrWork &w) {
     return GV_E_UNSUPPORTED;
   }
   const Store &in = g->has_in ? g->st[1] : g->st[0];
-  return pagerank_flat_prepare(in, g->st[0].deg, w, g->stream);
+  return pagerank_snap_prepare(in, g->st[0].deg, w, g->stream, g->cfg.pr_layout);
 }
 
 gv_status graph_bc_prepare(gv_graph *g, PrWork &w) {
@@ -457,8 +457,8 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
     set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
     return GV_E_INVALID;
   }
-  if (cfg.pr_layout < 0 || cfg.pr_layout > 4) {
-    set_error("pr_layout must be 0, 1, 2, 3 or 4");
+  if (cfg.pr_layout < 0 || cfg.pr_layout > 5) {
+    set_error("pr_layout must be 0, 1, 2, 3, 4 or 5");
     return GV_E_INVALID;
   }
   hipError_t e = hipSetDevice(cfg.device);
@@ -933,7 +933,8 @@ static gv_status pagerank_common(gv_graph *g, const gv_pagerank_opts *po, float 
   } else {
     rc = pagerank_run(in, g->st[0].deg, bins, o, dout, iters, residual, g->pr, g->stream,
                       iter_ms,
-                      g->cfg.pr_layout == 1   ? 1
+                      g->cfg.pr_layout == 5   ? 5
+                      : g->cfg.pr_layout == 1 ? 1
                       : g->cfg.pr_layout == 3 ? 2
                       : g->cfg.pr_layout == 4 ? 3
                                               : 0);
@@ -1350,8 +1351,8 @@ gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg) {
     set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
     return GV_E_INVALID;
   }
-  if (cfg->pr_layout < 0 || cfg->pr_layout > 4) {
-    set_error("pr_layout must be 0, 1, 2, 3 or 4");
+  if (cfg->pr_layout < 0 || cfg->pr_layout > 5) {
+    set_error("pr_layout must be 0, 1, 2, 3, 4 or 5");
     return GV_E_INVALID;
   }
   g->cfg.compact_threshold = cfg->compact_threshold;

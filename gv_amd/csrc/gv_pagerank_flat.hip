@@ -465,10 +465,26 @@ static gv_status pf_reserve(PrWork &w, uint32_t V, uint64_t n_tiles) {
   return GV_OK;
 }
 
+gv_status pagerank_snap_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
+                                hipStream_t s, int layout) {
+  if (layout == 5) {
+    const gv_status st = pagerank_pb_prepare(in, deg_out, w, s);
+    if (st != GV_E_UNSUPPORTED) return st;
+  }
+  return pagerank_flat_prepare(in, deg_out, w, s);
+}
+
+gv_status pagerank_snap_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
+                                double *residual, hipStream_t s, double *iter_ms) {
+  if (w.pb_ready) return pagerank_pb_iterate(o, w, out_dev, iters, residual, s, iter_ms);
+  return pagerank_flat_iterate(o, w, out_dev, iters, residual, s, iter_ms);
+}
+
 gv_status pagerank_flat_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
                                 hipStream_t s) {
   const uint32_t V = in.V;
   w.pf_ready = 0;
+  w.pb_ready = 0;
   // stream positions and buffer offsets are 32-bit; labels keep bit 31 free
   if (!V || V >= PF_PAD || (uint64_t)V * 4u > PF_OOB) return GV_E_UNSUPPORTED;
   gv_status st = pr_reserve(w, V);

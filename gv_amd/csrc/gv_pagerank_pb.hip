@@ -1,0 +1,1129 @@
+// gv_pagerank_pb.hip -- PageRank iterations with propagation blocking over
+// the relabelled in-adjacency snapshot (same semantics and stopping rule as
+// static_pagerank, pagerank.cu:3-68, pr_kernel.cu:21-45).
+//
+// Why: in the flat stream (gv_pagerank_flat.hip) every entry whose source
+// is not among the LDS-staged highest out-degrees gathers one 4-byte
+// contribution from a 16 MiB-plus vector, one L2 request per entry; at
+// R-MAT scale 22 after the update stream that is 58% of the entries, and the
+// per-CU rate of such divergent requests bounds the iteration.  Here those
+// cold entries never gather from global memory (propagation blocking,
+// Beamer, Asanovic and Patterson, IPDPS 2017, arranged for deterministic
+// sums):
+//   - expansion: cold sources are cut into blocks of PB_SB consecutive
+//     labels; for each block, its contributions are staged in LDS and
+//     copied out once per out-entry into a values array, in an order that
+//     groups the block's entries by destination bin -- sequential writes;
+//   - accumulation: the cold entries, in destination-major order, are cut
+//     into bins of whole tiles; a bin's values (one contiguous segment
+//     per source block) are loaded into LDS, and each entry holds the 15-bit
+//     LDS index of its value.  Row sums are then formed exactly as in the
+//     flat stream (tiles of 1024 entries, per-lane Kahan sums in stream
+//     order, segmented TwoSum scan), so the order of every sum is fixed.
+// Entries with a source label below PB_H (the highest out-degrees) form a
+// second, hot stream whose entries hold the label itself: the hot
+// contributions are staged in LDS once per workgroup and iteration, as in
+// the flat stream.  Both streams store 2-byte entries: a 15-bit index and
+// the row-start flag.  Every vertex has a row in both streams (a padding
+// entry, index PB_PAD, whose LDS slot holds 0, stands in for an empty one),
+// so a stream's row number is the vertex label.
+//
+// Per iteration: k_pb_tiles (hot tiles on some workgroups, cold bins on the
+// rest) -> k_pb_rows (joins the hot and cold row sums, the update, residual
+// partials) -> k_pb_expand (the FP64 residual and the stopping rule on one
+// block, the next iteration's expansion on the others).
+#include "gv_analytics.h"
+#include "gv_store.h"
+
+namespace gv {
+
+// LDS image of a workgroup: the hot contributions, or one cold bin's values;
+// its last slot, PB_PAD, holds 0 and is the index of every padding entry.
+// With the work-unit word it fills exactly 80 KiB: two workgroups per CU.
+constexpr uint32_t PB_IMG = 20479;            // floats
+constexpr uint32_t PB_PAD = PB_IMG - 1;
+constexpr uint32_t PB_F16 = 0x8000u;          // row-start flag of a stream entry
+constexpr uint32_t PB_IDX = 0x7FFFu;          // index field of a stream entry
+// hot source labels: [0, PB_H); a multiple of 4, so the expansion windows
+// (PB_H + k * PB_SB) start on 16-byte boundaries
+constexpr uint32_t PB_H = PB_PAD & ~3u;
+constexpr uint32_t PB_SB = 16384;             // cold sources per expansion block (64 KiB)
+// A bin's image holds its cold values in source-block order, each block's
+// segment padded to whole groups of 4 (16 B): at most bin_tiles * PF_T +
+// 3 * NS slots, which must stay below PB_PAD.  bin_tiles is chosen per graph
+// as the largest multiple of the workgroup's wavefront count that fits: more
+// tiles per bin make longer segments and more work per image load.
+constexpr uint32_t PB_MIN_BIN_TILES = 8;
+constexpr uint32_t PB_MAX_NS = 4095;
+static_assert(PB_MIN_BIN_TILES * PF_T + 3 * PB_MAX_NS < PB_PAD, "bin image overflows");
+__host__ __device__ inline uint32_t pb_bin_tiles(uint32_t NS) {
+  const uint32_t room = (PB_PAD - 1 - 3 * NS) / PF_T;
+  return room / PB_MIN_BIN_TILES * PB_MIN_BIN_TILES;
+}
+constexpr uint32_t PB_HOT_UNIT = 32;          // hot tiles per work unit (4 per wavefront)
+constexpr uint32_t PB_CH = 1u << 17;          // entries per expansion work item (multiple of 4)
+constexpr int PB_THREADS = 512;               // tile workgroups: 8 wavefronts
+constexpr int PB_BUILD_THREADS = 1024;
+constexpr uint32_t PB_CSRC_FLAG = 0x80000000u;
+constexpr uint32_t PB_CSRC_PAD = 0x7FFFFFFFu;
+constexpr int PB_ROW_THREADS = 256;
+constexpr uint32_t PB_ROW_BLOCKS = 1024;
+constexpr uint32_t PB_LONG_BLOCKS = 256;
+constexpr uint32_t PB_PARTS = PB_ROW_BLOCKS + PB_LONG_BLOCKS;
+// rows spanning at least PB_LONG tiles of a stream are joined by a wavefront
+constexpr uint32_t PB_LONG = 8;
+
+// Tiled layout of a 2-byte stream: lane l of a wavefront owns entries
+// [16 l, 16 l + 16) of a tile; entries 8 j .. 8 j + 7 of every lane are
+// stored contiguously for the wavefront (lane-major), so each of a lane's
+// two 16-byte loads is one contiguous 1 KiB read for the wavefront.
+__host__ __device__ inline uint64_t pb_phys(uint64_t p) {
+  const uint64_t t = p / PF_T, r = p % PF_T;
+  const uint64_t l = r / PF_K, kk = r % PF_K;
+  return t * PF_T + (kk >> 3) * 512 + l * 8 + (kk & 7);
+}
+
+__host__ __device__ inline uint32_t pb_pad4(uint32_t c) { return (c + 3u) & ~3u; }
+
+// ------------------------------------------------------- layout build --
+// Hot entries of every snapshot item (one wavefront per item).  Item i of
+// old vertex v covers live entries [ipos[i], ipos[i+1]) of v's row, which is
+// row newid[v] of the relabelled snapshot (cols, row-major, offsets off).
+__global__ __launch_bounds__(256) void k_pb_itemhot(const uint32_t *istart, const uint32_t *ivert,
+                                                    const uint64_t *ipos, uint32_t V,
+                                                    const uint32_t *newid, const uint64_t *off,
+                                                    const uint32_t *cols, uint32_t *hitem) {
+  const uint32_t ni = istart[V];
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
+       i += nwaves) {
+    const uint32_t v = ivert[i];
+    const uint64_t len = ipos[i + 1] - ipos[i];
+    const uint64_t base = off[newid[v]] + (ipos[i] - ipos[istart[v]]);
+    uint32_t cnt = 0;
+    for (uint64_t p = 0; p < len; p += WAVE) {
+      const bool hot = p + lane < len && cols[base + p + lane] < PB_H;
+      cnt += (uint32_t)__popcll(__ballot(hot));
+    }
+    if (lane == 0) hitem[i] = cnt;
+  }
+}
+
+// Row lengths of both streams in label order (a padding entry for an empty
+// row).  hscan: exclusive scan of the items' hot counts.
+__global__ void k_pb_rowlen(const uint32_t *istart, const uint32_t *hscan, const uint32_t *order,
+                            const uint64_t *off, uint32_t V, uint32_t *hl, uint32_t *cl) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = order[r];
+    const uint32_t h = hscan[istart[v + 1]] - hscan[istart[v]];
+    const uint32_t c = (uint32_t)(off[r + 1] - off[r]) - h;
+    hl[r] = h ? h : 1u;
+    cl[r] = c ? c : 1u;
+  }
+}
+
+// Splits each item's entries into the hot stream (label and flag, tiled) and
+// the cold source list (label, flag in bit 31, stream order).
+__global__ __launch_bounds__(256) void k_pb_split(const uint32_t *istart, const uint32_t *ivert,
+                                                  const uint64_t *ipos, uint32_t V,
+                                                  const uint32_t *newid, const uint64_t *off,
+                                                  const uint32_t *cols, const uint32_t *hscan,
+                                                  const uint32_t *hoff, const uint32_t *coff,
+                                                  uint16_t *hot, uint32_t *csrc) {
+  const uint32_t ni = istart[V];
+  const uint32_t lane = lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
+       i += nwaves) {
+    const uint32_t v = ivert[i];
+    const uint32_t r = newid[v];
+    const uint64_t len = ipos[i + 1] - ipos[i];
+    const uint64_t inrow = ipos[i] - ipos[istart[v]];
+    const uint64_t base = off[r] + inrow;
+    const uint32_t hin = hscan[i] - hscan[istart[v]];
+    uint32_t hb = hoff[r] + hin;
+    uint32_t cb = coff[r] + (uint32_t)inrow - hin;
+    const uint32_t h0 = hoff[r], c0 = coff[r];
+    for (uint64_t p = 0; p < len; p += WAVE) {
+      const bool valid = p + lane < len;
+      const uint32_t e = valid ? cols[base + p + lane] : 0u;
+      const bool ishot = valid && e < PB_H;
+      const bool iscold = valid && !ishot;
+      const unsigned long long mh = __ballot(ishot), mc = __ballot(iscold);
+      if (ishot) {
+        const uint32_t q = hb + mask_lt_count(mh);
+        hot[pb_phys(q)] = (uint16_t)(e | (q == h0 ? PB_F16 : 0u));
+      }
+      if (iscold) {
+        const uint32_t q = cb + mask_lt_count(mc);
+        csrc[q] = e | (q == c0 ? PB_CSRC_FLAG : 0u);
+      }
+      hb += (uint32_t)__popcll(mh);
+      cb += (uint32_t)__popcll(mc);
+    }
+  }
+}
+
+// Padding entries for rows empty in a stream, and after the last entry up to
+// the end of the last tile.
+__global__ void k_pb_pads(const uint32_t *istart, const uint32_t *hscan, const uint32_t *order,
+                          const uint64_t *off, uint32_t V, const uint32_t *hoff,
+                          const uint32_t *coff, uint16_t *hot, uint32_t *csrc) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = order[r];
+    const uint32_t h = hscan[istart[v + 1]] - hscan[istart[v]];
+    const uint32_t c = (uint32_t)(off[r + 1] - off[r]) - h;
+    if (!h) hot[pb_phys(hoff[r])] = (uint16_t)(PB_PAD | PB_F16);
+    if (!c) csrc[coff[r]] = PB_CSRC_PAD | PB_CSRC_FLAG;
+  }
+}
+
+__global__ void k_pb_tail(const uint32_t *hoff, const uint32_t *coff, uint32_t V, uint64_t hend,
+                          uint64_t cend, uint16_t *hot, uint32_t *csrc) {
+  const uint64_t h0 = hoff[V], c0 = coff[V];
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = h0 + i0; p < hend; p += st) hot[pb_phys(p)] = (uint16_t)PB_PAD;
+  for (uint64_t p = c0 + i0; p < cend; p += st) csrc[p] = PB_CSRC_PAD;
+}
+
+// tbase[t] = label of the row in progress at the start of tile t (the last
+// row whose first entry lies before t * PF_T), -1 for tile 0.
+__global__ void k_pb_tbase(const uint32_t *off, uint32_t V, uint32_t n_tiles, int32_t *tbase) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = t * PF_T;
+    uint32_t lo = 0, hi = V;  // first r with off[r] >= e
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (off[mid] < e) lo = mid + 1; else hi = mid;
+    }
+    tbase[t] = (int32_t)lo - 1;
+  }
+}
+
+__device__ __forceinline__ uint32_t pb_span(const uint32_t *off, uint32_t r) {
+  return (off[r + 1] - 1) / PF_T - off[r] / PF_T;
+}
+
+// Rows spanning at least PB_LONG tiles in either stream, in label order.
+__global__ void k_pb_longflag(const uint32_t *hoff, const uint32_t *coff, uint32_t V,
+                              uint32_t *flag) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
+       r += (uint64_t)gridDim.x * blockDim.x)
+    flag[r] = pb_span(hoff, (uint32_t)r) >= PB_LONG || pb_span(coff, (uint32_t)r) >= PB_LONG
+                  ? 1u : 0u;
+}
+
+__global__ void k_pb_longfill(const uint32_t *lidx, uint32_t V, uint32_t *longrows) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
+       r += (uint64_t)gridDim.x * blockDim.x)
+    if (lidx[r + 1] > lidx[r]) longrows[lidx[r]] = (uint32_t)r;
+}
+
+// Cold entries of bin b per source block k, padded to groups of 4:
+// cnt[k * NB + b] (k-major, so an exclusive scan gives the expansion
+// order: source block, then bin); the bin's image size in groups goes to
+// ngrp[b].
+__global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_bincnt(const uint32_t *csrc,
+                                                                uint64_t cend, uint32_t NS,
+                                                                uint32_t NB, uint32_t bin,
+                                                                uint32_t *cnt, uint32_t *ngrp) {
+  __shared__ uint32_t hist[PB_MAX_NS];
+  const uint32_t b = blockIdx.x;
+  for (uint32_t k = threadIdx.x; k < NS; k += PB_BUILD_THREADS) hist[k] = 0;
+  __syncthreads();
+  const uint64_t q0 = (uint64_t)b * bin;
+  const uint64_t q1 = q0 + bin < cend ? q0 + bin : cend;
+  for (uint64_t q = q0 + threadIdx.x; q < q1; q += PB_BUILD_THREADS) {
+    const uint32_t src = csrc[q] & ~PB_CSRC_FLAG;
+    if (src != PB_CSRC_PAD) atomicAdd(&hist[(src - PB_H) / PB_SB], 1u);
+  }
+  __syncthreads();
+  uint32_t ng = 0;
+  for (uint32_t k = threadIdx.x; k < NS; k += PB_BUILD_THREADS) {
+    const uint32_t c = pb_pad4(hist[k]);
+    cnt[(uint64_t)k * NB + b] = c;
+    ng += c / 4;
+  }
+  ng = block_sum<uint32_t, PB_BUILD_THREADS>(ng);
+  if (threadIdx.x == 0) ngrp[b] = ng;
+}
+
+// Expansion order and image index of every cold entry of bin b.
+// seg: exclusive scan of the padded counts (k-major), so segment (k, b) of
+// the expansion stream starts at seg[k * NB + b] and spans seg[k * NB + b +
+// 1] - seg[k * NB + b] slots (a multiple of 4; the padding slots expand
+// harmlessly from window offset 0).  In the bin's image the segments follow
+// one another in k order (lpre[k] = slots of blocks before k); gaddr lists,
+// per group of 4 image slots, where that group lies in the values array (in
+// units of 4 floats).  Within a segment entries take places by LDS atomics,
+// so that order is not fixed -- but each entry's index names its own value
+// and the sums run in stream order, so the results do not depend on it.
+__global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
+    const uint32_t *csrc, uint64_t cend, uint32_t NS, uint32_t NB, uint32_t bin,
+    const uint32_t *seg, const uint32_t *gbase, uint32_t *gaddr, uint16_t *lidx,
+    uint16_t *cold) {
+  __shared__ uint32_t lpre[PB_MAX_NS];
+  __shared__ uint32_t ctr[PB_MAX_NS];
+  const uint32_t b = blockIdx.x;
+  uint32_t carry = 0;
+  const uint32_t g0 = gbase[b];
+  for (uint32_t k0 = 0; k0 < NS; k0 += PB_BUILD_THREADS) {
+    const uint32_t k = k0 + threadIdx.x;
+    const uint64_t at = (uint64_t)k * NB + b;
+    const uint32_t c = k < NS ? seg[at + 1] - seg[at] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<uint32_t, PB_BUILD_THREADS>(c, tot) + carry;
+    if (k < NS) {
+      lpre[k] = ex;
+      ctr[k] = 0;
+      const uint32_t s0 = seg[at];
+      for (uint32_t g = 0; g < c / 4; ++g) gaddr[g0 + ex / 4 + g] = s0 / 4 + g;
+    }
+    carry += tot;
+  }
+  __syncthreads();
+  const uint64_t q0 = (uint64_t)b * bin;
+  const uint64_t q1 = q0 + bin < cend ? q0 + bin : cend;
+  for (uint64_t q = q0 + threadIdx.x; q < q1; q += PB_BUILD_THREADS) {
+    const uint32_t raw = csrc[q];
+    const uint32_t f = raw & PB_CSRC_FLAG ? PB_F16 : 0u;
+    const uint32_t src = raw & ~PB_CSRC_FLAG;
+    uint32_t idx = PB_PAD;
+    if (src != PB_CSRC_PAD) {
+      const uint32_t k = (src - PB_H) / PB_SB;
+      const uint32_t pos = atomicAdd(&ctr[k], 1u);
+      lidx[seg[(uint64_t)k * NB + b] + pos] = (uint16_t)(src - PB_H - k * PB_SB);
+      idx = lpre[k] + pos;
+    }
+    cold[pb_phys(q)] = (uint16_t)(idx | f);
+  }
+  __syncthreads();
+  // padding slots of each segment expand from window offset 0
+  for (uint32_t k = threadIdx.x; k < NS; k += PB_BUILD_THREADS) {
+    const uint64_t at = (uint64_t)k * NB + b;
+    const uint32_t s0 = seg[at], c = seg[at + 1] - s0;
+    for (uint32_t p = ctr[k]; p < c; ++p) lidx[s0 + p] = 0;
+  }
+}
+
+// Expansion work items: the slots of source block k (seg[k * NB] ..
+// seg[(k + 1) * NB]) in pieces of at most PB_CH.
+__global__ void k_pb_itemcnt(const uint32_t *seg, uint32_t NS, uint32_t NB, uint32_t *nit) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < NS;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t n = seg[(k + 1) * NB] - seg[k * NB];
+    nit[k] = (n + PB_CH - 1) / PB_CH;
+  }
+}
+
+__global__ void k_pb_itemfill(const uint32_t *seg, uint32_t NS, uint32_t NB, const uint32_t *nit,
+                              uint4 *items) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < NS;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s0 = seg[k * NB], s1 = seg[(k + 1) * NB];
+    uint32_t j = nit[k];
+    for (uint32_t a = s0; a < s1; a += PB_CH, ++j)
+      items[j] = make_uint4((uint32_t)k, a, a + PB_CH < s1 ? a + PB_CH : s1, 0u);
+  }
+}
+
+// ---------------------------------------------------------- iteration --
+struct PbArgs {
+  const uint4 *hot, *cold;        // tiled 2-byte streams, 16-byte units
+  const int32_t *htbase, *ctbase;
+  float2 *Ph, *Qh, *Sh, *Pc, *Qc, *Sc;
+  const uint32_t *hoff, *coff;
+  const uint32_t *gaddr, *gbase;
+  const uint16_t *lidx;
+  float *vals;
+  const uint4 *items;
+  const uint32_t *n_items;
+  const float *x, *cb;
+  float *xn, *cn;
+  const uint32_t *deg_out, *longrows;
+  double *partials;
+  PrCtl *ctl;
+  uint32_t V, nh, n_htiles, n_ctiles, n_hot_units, n_bins, bin_tiles, n_long, compensated,
+      max_iter;
+  float alpha, base;
+  double eps;
+};
+
+// left := left (+) right, left first; compensated or plain
+__device__ __forceinline__ void pb_join(float &s, float &c, float s2, float c2, uint32_t comp) {
+  if (comp) {
+    const float t = s + s2;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (s2 - bp);
+    s = t;
+    c = c + c2 - e;
+  } else {
+    s = s + s2;
+  }
+}
+
+typedef unsigned int pb_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void pb_load(const uint4 *s4, uint32_t t, uint32_t lane,
+                                        uint4 (&q)[2]) {
+  const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(s4 + (uint64_t)t * (PF_T / 8) + lane);
+  const pb_u32x4 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 64);
+  q[0] = make_uint4(a.x, a.y, a.z, a.w);
+  q[1] = make_uint4(b.x, b.y, b.z, b.w);
+}
+
+// Entry kk (0..15) of a lane's tile part.
+__device__ __forceinline__ uint32_t pb_entry(const uint4 (&q)[2], uint32_t kk) {
+  const uint4 v = q[kk >> 3];
+  const uint32_t e = kk & 7;
+  const uint32_t w = e < 2 ? v.x : e < 4 ? v.y : e < 6 ? v.z : v.w;
+  return (e & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+// Cross-lane moves for the scans below, on the DPP network (no LDS): a lane
+// whose source lies outside the pattern, or in a row the row mask leaves
+// out, reads 0 -- the identity of every combine used with them.
+template <int CTRL, int RM>
+__device__ __forceinline__ float pb_dpp(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, RM, 0xF, true));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ uint32_t pb_dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xF, true);
+}
+constexpr int DPP_ROW_SHR = 0x110;    // + n: row_shr:n (within rows of 16 lanes)
+constexpr int DPP_BCAST15 = 0x142;    // lane 15 of each row to the next row
+constexpr int DPP_BCAST31 = 0x143;    // lane 31 to rows 2 and 3
+constexpr int DPP_WAVE_SHR1 = 0x138;  // whole wavefront, one lane up
+
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ uint32_t pb_wave_incl(uint32_t v) {
+  v += pb_dpp<DPP_ROW_SHR + 1, 0xF>(v);
+  v += pb_dpp<DPP_ROW_SHR + 2, 0xF>(v);
+  v += pb_dpp<DPP_ROW_SHR + 4, 0xF>(v);
+  v += pb_dpp<DPP_ROW_SHR + 8, 0xF>(v);
+  v += pb_dpp<DPP_BCAST15, 0xA>(v);
+  v += pb_dpp<DPP_BCAST31, 0xC>(v);
+  return v;
+}
+
+// One step of the segmented scan: a lane whose value carries no row start
+// takes the value on its left joined in front (left first) and that value's
+// row-start flag.
+template <int CTRL, int RM, bool COMP>
+__device__ __forceinline__ void pb_seg_step(float &vs, float &vc, uint32_t &vf) {
+  constexpr uint32_t comp = COMP ? 1u : 0u;
+  const float os = pb_dpp<CTRL, RM>(vs), oc = pb_dpp<CTRL, RM>(vc);
+  const uint32_t of = pb_dpp<CTRL, RM>(vf);
+  float ns = os, nc = oc;
+  pb_join(ns, nc, vs, vc, comp);
+  vs = vf ? vs : ns;
+  vc = vf ? vc : nc;
+  vf = vf ? vf : of;
+}
+
+// Row sums of tile t of a stream (rows numbered by label) from the lane's
+// entries and their image values: rows begun and ended in the tile go to S,
+// the tile's first-row and last-row partials to P[t] and Q[t].  Padding
+// entries read the image slot holding 0, and Kahan-adding 0 leaves the
+// represented sum unchanged, so they need no test.
+template <bool COMP>
+__device__ __forceinline__ void pb_tile_sums(float2 *S, float2 *P, float2 *Q, uint32_t t,
+                                             int32_t tb, const uint4 (&q)[2],
+                                             const float *lds, uint32_t lane, uint32_t nrows) {
+  constexpr uint32_t comp = COMP ? 1u : 0u;
+  uint32_t fm = 0;
+  float g[PF_K];
+#pragma unroll
+  for (uint32_t k = 0; k < PF_K; ++k) {
+    const uint32_t e = pb_entry(q, k);
+    fm |= (e >> 15) << k;
+    g[k] = lds[e & PB_IDX];
+  }
+  const uint32_t nst = (uint32_t)__popc(fm);
+  const uint32_t incl = pb_wave_incl(nst);
+  // label of the row in progress at this lane's start
+  const int32_t r0 = tb + (int32_t)(incl - nst);
+  int32_t row = r0;
+  float s = 0.f, c = 0.f, hs = 0.f, hc = 0.f;
+  bool seen = false;
+#pragma unroll
+  for (uint32_t k = 0; k < PF_K; ++k) {
+    const bool f = (fm >> k) & 1u;
+    // a row begun in this lane ends here
+    if (f && seen && (uint32_t)row < nrows) S[row] = make_float2(s, c);
+    hs = f && !seen ? s : hs;
+    hc = f && !seen ? c : hc;
+    row += f ? 1 : 0;
+    s = f ? 0.f : s;
+    c = f ? 0.f : c;
+    seen = seen || f;
+    if (COMP) kahan_add(s, c, g[k]);
+    else s += g[k];
+  }
+  if (!seen) {
+    hs = s;
+    hc = c;
+  }
+  // segmented inclusive scan over lanes of the value carried out of each
+  // lane: its last row's partial (if a row starts in it) or its whole sum
+  float vs = seen ? s : hs, vc = seen ? c : hc;
+  uint32_t vf = seen ? 1u : 0u;
+  pb_seg_step<DPP_ROW_SHR + 1, 0xF, COMP>(vs, vc, vf);
+  pb_seg_step<DPP_ROW_SHR + 2, 0xF, COMP>(vs, vc, vf);
+  pb_seg_step<DPP_ROW_SHR + 4, 0xF, COMP>(vs, vc, vf);
+  pb_seg_step<DPP_ROW_SHR + 8, 0xF, COMP>(vs, vc, vf);
+  pb_seg_step<DPP_BCAST15, 0xA, COMP>(vs, vc, vf);
+  pb_seg_step<DPP_BCAST31, 0xC, COMP>(vs, vc, vf);
+  // what the lanes before this one carry into it (0 for lane 0)
+  float cs = pb_dpp<DPP_WAVE_SHR1, 0xF>(vs), cc = pb_dpp<DPP_WAVE_SHR1, 0xF>(vc);
+  const uint32_t cf = pb_dpp<DPP_WAVE_SHR1, 0xF>(vf);
+  if (seen) {
+    // the row in progress at this lane's start ends before its first flag
+    pb_join(cs, cc, hs, hc, comp);
+    if (cf) {
+      if ((uint32_t)r0 < nrows) S[r0] = make_float2(cs, cc);  // begun in an earlier lane
+    } else {
+      P[t] = make_float2(cs, cc);  // the tile's first row
+    }
+  }
+  const float ls = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vs), 63));
+  const float lc = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vc), 63));
+  const uint32_t lf = (uint32_t)__builtin_amdgcn_readlane((int)vf, 63);
+  if (lane == 0) {
+    Q[t] = make_float2(ls, lc);
+    if (!lf) P[t] = make_float2(ls, lc);  // no row starts in this tile
+  }
+}
+
+// Work units from one counter per iteration: first the hot units (PB_HOT_UNIT
+// tiles of the hot stream, staged hot contributions), then the cold bins (a
+// bin's values loaded into the image, then its bin_tiles tiles, dealt to the
+// wavefronts in turn).  A workgroup stages the hot contributions once, at its first
+// hot unit; once it has taken a bin it never takes a hot unit again.
+template <bool COMP>
+__global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
+  __shared__ float s_img[PB_IMG];
+  __shared__ uint32_t s_unit;
+  if (a.ctl->done) return;
+  const uint32_t lane = lane_id();
+  const uint32_t wv = threadIdx.x / WAVE;
+  unsigned int *cursor = &a.ctl->cursor[a.ctl->iters & 1u];
+  const uint32_t n_units = a.n_hot_units + a.n_bins;
+  bool staged = false;
+  // the next unit is requested before the current one is worked on, so the
+  // counter's round trip overlaps the work
+  uint32_t next = 0;
+  if (threadIdx.x == 0) next = atomicAdd(cursor, 1u);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_unit = next;
+      next = atomicAdd(cursor, 1u);
+    }
+    __syncthreads();
+    const uint32_t u = s_unit;
+    __syncthreads();
+    if (u >= n_units) break;
+    if (u < a.n_hot_units) {
+      if (!staged) {
+        // every load of the stage is issued before the first LDS write
+        constexpr uint32_t NQ = (PB_IMG / 4 + PB_THREADS - 1) / PB_THREADS;
+        const float4 *c4 = reinterpret_cast<const float4 *>(a.cb);
+        float4 hv[NQ];
+#pragma unroll
+        for (uint32_t j = 0; j < NQ; ++j) {
+          const uint32_t i = threadIdx.x + j * PB_THREADS;
+          hv[j] = 4 * i + 3 < a.nh ? c4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        float4 *img4 = reinterpret_cast<float4 *>(s_img);
+#pragma unroll
+        for (uint32_t j = 0; j < NQ; ++j) {
+          const uint32_t i = threadIdx.x + j * PB_THREADS;
+          if (4 * i + 3 < PB_IMG) img4[i] = hv[j];
+        }
+        // a partial last group of hot labels, and the image's last slots
+        for (uint32_t i = (a.nh & ~3u) + threadIdx.x; i < PB_IMG; i += PB_THREADS)
+          if (i >= (PB_IMG & ~3u) || i < a.nh) s_img[i] = i < a.nh ? a.cb[i] : 0.f;
+        __syncthreads();
+        staged = true;
+      }
+      // the wavefront's tiles of the unit, the next one's entries loaded
+      // before the current one is summed
+      constexpr uint32_t STEP = PB_THREADS / WAVE;
+      const uint32_t t0 = u * PB_HOT_UNIT + wv;
+      uint4 cur[2], nxt[2];
+      pb_load(a.hot, t0 < a.n_htiles ? t0 : 0u, lane, cur);
+      int32_t tb_cur = a.htbase[t0 < a.n_htiles ? t0 : 0u];
+      for (uint32_t t = t0; t < (u + 1) * PB_HOT_UNIT && t < a.n_htiles; t += STEP) {
+        const uint32_t tn = t + STEP < (u + 1) * PB_HOT_UNIT && t + STEP < a.n_htiles ? t + STEP : t;
+        pb_load(a.hot, tn, lane, nxt);
+        const int32_t tb_nxt = a.htbase[tn];
+        pb_tile_sums<COMP>(a.Sh, a.Ph, a.Qh, t, tb_cur, cur, s_img, lane, a.V);
+        cur[0] = nxt[0];
+        cur[1] = nxt[1];
+        tb_cur = tb_nxt;
+      }
+      continue;
+    }
+    // cold bin: its image, group by group (16 B each), then its tiles
+    const uint32_t b = u - a.n_hot_units;
+    const uint32_t t0 = b * a.bin_tiles + wv;
+    const uint32_t t1 = (b + 1) * a.bin_tiles < a.n_ctiles ? (b + 1) * a.bin_tiles : a.n_ctiles;
+    uint4 ent[2];
+    pb_load(a.cold, t0 < t1 ? t0 : 0u, lane, ent);
+    int32_t tb = a.ctbase[t0 < t1 ? t0 : 0u];
+    const uint32_t g0 = a.gbase[b], ng = a.gbase[b + 1] - g0;
+    const float4 *v4 = reinterpret_cast<const float4 *>(a.vals);
+    float4 *img4 = reinterpret_cast<float4 *>(s_img);
+    // all loads of a round are issued before any image write (addresses of
+    // groups past the bin's end are clamped to a valid one, their values
+    // dropped)
+    constexpr uint32_t NG = 10;
+    for (uint32_t gb = 0; gb < ng; gb += NG * PB_THREADS) {
+      uint32_t ga[NG];
+#pragma unroll
+      for (uint32_t i = 0; i < NG; ++i) {
+        const uint32_t g = gb + threadIdx.x + i * PB_THREADS;
+        ga[i] = a.gaddr[g0 + (g < ng ? g : 0u)];
+      }
+      float4 v[NG];
+#pragma unroll
+      for (uint32_t i = 0; i < NG; ++i) v[i] = v4[ga[i]];
+#pragma unroll
+      for (uint32_t i = 0; i < NG; ++i) {
+        const uint32_t g = gb + threadIdx.x + i * PB_THREADS;
+        if (g < ng && g < PB_PAD / 4) img4[g] = v[i];
+      }
+    }
+    if (threadIdx.x == 0) s_img[PB_PAD] = 0.f;
+    staged = false;
+    __syncthreads();
+    constexpr uint32_t STEP = PB_THREADS / WAVE;
+    for (uint32_t t = t0; t < t1; t += STEP) {
+      uint4 nxt[2];
+      const uint32_t tn = t + STEP < t1 ? t + STEP : t;
+      pb_load(a.cold, tn, lane, nxt);
+      const int32_t tb_nxt = a.ctbase[tn];
+      pb_tile_sums<COMP>(a.Sc, a.Pc, a.Qc, t, tb, ent, s_img, lane, a.V);
+      ent[0] = nxt[0];
+      ent[1] = nxt[1];
+      tb = tb_nxt;
+    }
+  }
+}
+
+// Sum of row r in one stream (not a long row): S[r] when it begins and ends
+// inside a tile without being the tile's last row, else Q of its first tile
+// joined with P of the following tiles in order.
+__device__ __forceinline__ float2 pb_part(const uint32_t *off, const float2 *S, const float2 *P,
+                                          const float2 *Q, uint32_t r, uint32_t V,
+                                          uint32_t comp) {
+  const uint32_t e0 = off[r], e1 = off[r + 1];
+  const uint32_t ta = e0 / PF_T, tb = (e1 - 1) / PF_T;
+  if (ta == tb) {
+    const bool last = e1 == (ta + 1) * PF_T || r + 1 == V;
+    return last ? Q[ta] : S[r];
+  }
+  float2 p = Q[ta];
+  for (uint32_t t = ta + 1; t <= tb; ++t) {
+    const float2 x = P[t];
+    pb_join(p.x, p.y, x.x, x.y, comp);
+  }
+  return p;
+}
+
+__device__ __forceinline__ double pb_finish(const PbArgs &a, uint32_t r, float2 h, float2 c) {
+  pb_join(h.x, h.y, c.x, c.y, a.compensated);
+  const float xv = a.base + a.alpha * (h.x - h.y);
+  a.xn[r] = xv;
+  const uint32_t d = a.deg_out[r];
+  a.cn[r] = d ? xv / (float)d : 0.f;
+  return fabs((double)xv - (double)a.x[r]);
+}
+
+// A long row's part in one stream, by one wavefront: lane l joins P[ta+1+l],
+// P[ta+1+l+64], ... in order, the lanes are joined by a fixed butterfly, and
+// Q[ta] goes in front.  Rows not long in this stream take the short path.
+__device__ __forceinline__ float2 pb_long_part(const uint32_t *off, const float2 *S,
+                                               const float2 *P, const float2 *Q, uint32_t r,
+                                               uint32_t V, uint32_t comp, uint32_t lane) {
+  const uint32_t ta = off[r] / PF_T, tb = (off[r + 1] - 1) / PF_T;
+  if (tb - ta < PB_LONG) return pb_part(off, S, P, Q, r, V, comp);
+  float s = 0.f, c = 0.f;
+  for (uint32_t t = ta + 1 + lane; t <= tb; t += WAVE) {
+    const float2 q = P[t];
+    pb_join(s, c, q.x, q.y, comp);
+  }
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const float s2 = __shfl_xor(s, d, WAVE), c2 = __shfl_xor(c, d, WAVE);
+    if (lane & d) {
+      float ns = s2, nc = c2;
+      pb_join(ns, nc, s, c, comp);
+      s = ns;
+      c = nc;
+    } else {
+      pb_join(s, c, s2, c2, comp);
+    }
+  }
+  float2 p = Q[ta];
+  pb_join(p.x, p.y, s, c, comp);
+  return p;
+}
+
+// Row offsets of both streams for row r: whether either spans PB_LONG
+// tiles or more (a long row, joined by a wavefront), else the row's S/P/Q
+// reads are those of pb_part.
+__global__ __launch_bounds__(PB_ROW_THREADS) void k_pb_rows(PbArgs a) {
+  if (a.ctl->done) return;
+  // the next iteration's tile kernel deals its units from the other counter
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[(a.ctl->iters + 1) & 1u] = 0;
+  const uint32_t comp = a.compensated;
+  double diff = 0.0;
+  if (blockIdx.x >= PB_ROW_BLOCKS) {
+    const uint32_t lane = lane_id();
+    const uint32_t nw = PB_LONG_BLOCKS * (PB_ROW_THREADS / WAVE);
+    for (uint32_t j = (blockIdx.x - PB_ROW_BLOCKS) * (PB_ROW_THREADS / WAVE) + threadIdx.x / WAVE;
+         j < a.n_long; j += nw) {
+      const uint32_t r = a.longrows[j];
+      const float2 h = pb_long_part(a.hoff, a.Sh, a.Ph, a.Qh, r, a.V, comp, lane);
+      const float2 c = pb_long_part(a.coff, a.Sc, a.Pc, a.Qc, r, a.V, comp, lane);
+      if (lane == 0) diff += pb_finish(a, r, h, c);
+    }
+  } else {
+    // R rows per thread and step, each step's loads issued together: the
+    // offsets, then the row partials they select, then x and degrees
+    constexpr uint32_t R = 4;
+    const uint64_t stride = (uint64_t)PB_ROW_BLOCKS * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < a.V;
+         i0 += R * stride) {
+      uint32_t h0[R], h1[R], c0[R], c1[R];
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) {
+        const uint64_t i = i0 + k * stride;
+        const uint32_t r = i < a.V ? (uint32_t)i : 0u;
+        h0[k] = a.hoff[r];
+        h1[k] = a.hoff[r + 1];
+        c0[k] = a.coff[r];
+        c1[k] = a.coff[r + 1];
+      }
+      float2 hp[R], cp[R];
+      float xo[R];
+      uint32_t dg[R];
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) {
+        const uint64_t i = i0 + k * stride;
+        const uint32_t r = i < a.V ? (uint32_t)i : 0u;
+        const uint32_t hta = h0[k] / PF_T, htb = (h1[k] - 1) / PF_T;
+        const uint32_t cta = c0[k] / PF_T, ctb = (c1[k] - 1) / PF_T;
+        // the row's first read in each stream: S[r] when it is not its tile's
+        // last row, else Q of its first tile (further tiles are joined below)
+        const bool hs = hta == htb && !(h1[k] == (hta + 1) * PF_T || r + 1 == a.V);
+        const bool cs = cta == ctb && !(c1[k] == (cta + 1) * PF_T || r + 1 == a.V);
+        hp[k] = hs ? a.Sh[r] : a.Qh[hta];
+        cp[k] = cs ? a.Sc[r] : a.Qc[cta];
+        xo[k] = a.x[r];
+        dg[k] = a.deg_out[r];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) {
+        const uint64_t i = i0 + k * stride;
+        if (i >= a.V) continue;
+        const uint32_t r = (uint32_t)i;
+        const uint32_t hta = h0[k] / PF_T, htb = (h1[k] - 1) / PF_T;
+        const uint32_t cta = c0[k] / PF_T, ctb = (c1[k] - 1) / PF_T;
+        if (htb - hta >= PB_LONG || ctb - cta >= PB_LONG) continue;  // a long row
+        float2 h = hp[k], c = cp[k];
+        for (uint32_t t = hta + 1; t <= htb; ++t) {
+          const float2 x = a.Ph[t];
+          pb_join(h.x, h.y, x.x, x.y, comp);
+        }
+        for (uint32_t t = cta + 1; t <= ctb; ++t) {
+          const float2 x = a.Pc[t];
+          pb_join(c.x, c.y, x.x, x.y, comp);
+        }
+        pb_join(h.x, h.y, c.x, c.y, comp);
+        const float xv = a.base + a.alpha * (h.x - h.y);
+        a.xn[r] = xv;
+        a.cn[r] = dg[k] ? xv / (float)dg[k] : 0.f;
+        diff += fabs((double)xv - (double)xo[k]);
+      }
+    }
+  }
+  const double bs = block_sum<double, PB_ROW_THREADS>(diff);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+}
+
+// Block 0 (with resid): fixed-order FP64 residual of the iteration just
+// finished and the stopping rule (pagerank.cu:56).  Other blocks: the
+// expansion of the contributions cb for the next iteration, 4 slots (one
+// 8-byte load of window offsets, one 16-byte store of values) per thread
+// and step.
+__global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_expand(PbArgs a, uint32_t resid) {
+  __shared__ float s_win[PB_SB];
+  if (resid) {
+    if (blockIdx.x == 0) {
+      if (a.ctl->done) return;
+      double r = 0.0;
+      for (uint32_t i = threadIdx.x; i < PB_PARTS; i += PB_BUILD_THREADS) r += a.partials[i];
+      r = block_sum<double, PB_BUILD_THREADS>(r);
+      if (threadIdx.x == 0) {
+        a.ctl->residual = r;
+        const unsigned it = a.ctl->iters + 1;
+        a.ctl->iters = it;
+        if (r < a.eps || it >= a.max_iter) a.ctl->done = 1;
+      }
+      return;
+    }
+  }
+  if (a.ctl->done) return;
+  const uint32_t nb = gridDim.x - resid, bid = blockIdx.x - resid;
+  const uint32_t n_items = *a.n_items;
+  const uint2 *l4 = reinterpret_cast<const uint2 *>(a.lidx);
+  float4 *v4 = reinterpret_cast<float4 *>(a.vals);
+  for (uint32_t it = bid; it < n_items; it += nb) {
+    const uint4 item = a.items[it];
+    const uint32_t s0 = PB_H + item.x * PB_SB;
+    const uint32_t ns = a.V - s0 < PB_SB ? a.V - s0 : PB_SB;
+    {
+      // the window's loads all issued before its LDS writes (s0 is a multiple
+      // of 4, so whole groups are 16-byte loads)
+      constexpr uint32_t NW = PB_SB / 4 / PB_BUILD_THREADS;
+      const float4 *c4 = reinterpret_cast<const float4 *>(a.cb + s0);
+      float4 wv[NW];
+#pragma unroll
+      for (uint32_t j = 0; j < NW; ++j) {
+        const uint32_t i = threadIdx.x + j * PB_BUILD_THREADS;
+        wv[j] = 4 * i + 3 < ns ? c4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      float4 *w4 = reinterpret_cast<float4 *>(s_win);
+#pragma unroll
+      for (uint32_t j = 0; j < NW; ++j) w4[threadIdx.x + j * PB_BUILD_THREADS] = wv[j];
+      for (uint32_t i = (ns & ~3u) + threadIdx.x; i < ns; i += PB_BUILD_THREADS)
+        s_win[i] = a.cb[s0 + i];
+    }
+    __syncthreads();
+    constexpr uint32_t U = 4;
+    const uint32_t ga = item.y / 4, gb = item.z / 4;
+    for (uint32_t g = ga + threadIdx.x; g < gb; g += U * PB_BUILD_THREADS) {
+      uint2 l[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t q = g + u * PB_BUILD_THREADS;
+        l[u] = q < gb ? l4[q] : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t q = g + u * PB_BUILD_THREADS;
+        if (q < gb)
+          v4[q] = make_float4(s_win[l[u].x & 0xFFFFu], s_win[l[u].x >> 16],
+                              s_win[l[u].y & 0xFFFFu], s_win[l[u].y >> 16]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ workspace --
+static int pb_device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+// Grow-only device buffer (headroom of an eighth, at least `floor` bytes).
+static gv_status pb_grow(void **p, size_t &cap, size_t need, size_t floor = 0) {
+  if (*p && need <= cap) return GV_OK;
+  hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  size_t c = need + need / 8 + 256;
+  if (c < floor) c = floor;
+  GV_HIP(hipMalloc(p, c));
+  cap = c;
+  return GV_OK;
+}
+
+void pb_release(PrWork &w) {
+  void **bufs[] = {(void **)&w.pb_hot,   (void **)&w.pb_cold,  (void **)&w.pb_csrc,
+                   (void **)&w.pb_lidx,  (void **)&w.pb_vals,  (void **)&w.pb_hoff,
+                   (void **)&w.pb_coff,  (void **)&w.pb_htb,   (void **)&w.pb_ctb,
+                   (void **)&w.pb_Sh,    (void **)&w.pb_Sc,    (void **)&w.pb_PQh,
+                   (void **)&w.pb_PQc,   (void **)&w.pb_hitem, (void **)&w.pb_cnt,
+                   (void **)&w.pb_ngrp,  (void **)&w.pb_gaddr, (void **)&w.pb_items,
+                   (void **)&w.pb_nit,   (void **)&w.pb_lidxr, (void **)&w.pb_long,
+                   (void **)&w.pb_tmp,   (void **)&w.pb_partials};
+  for (void **b : bufs) {
+    hipFree(*b);
+    *b = nullptr;
+  }
+  w.pb_caps = PbCaps{};
+  w.pb_ready = 0;
+}
+
+gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
+                              hipStream_t s) {
+  const uint32_t V = in.V;
+  w.pb_ready = 0;
+  w.pf_ready = 0;
+  if (!V) return GV_E_UNSUPPORTED;
+  const uint32_t NS = V > PB_H ? (uint32_t)cdiv(V - PB_H, PB_SB) : 0u;
+  if (NS > PB_MAX_NS) return GV_E_UNSUPPORTED;
+  gv_status st = pr_reserve(w, V);
+  if (st) return st;
+  st = pr_snapshot(in, deg_out, true, w, s, 0);
+  if (st) return st;
+  const uint64_t E = w.snap_E;
+  // stream positions, expansion positions and the segment table are 32-bit
+  const uint32_t BT = pb_bin_tiles(NS);
+  const uint32_t bin = BT * PF_T;
+  if (E + 2ull * V + 2ull * bin >= 0x7FFFFFFFull) return GV_E_UNSUPPORTED;
+  PbCaps &cp = w.pb_caps;
+  const uint32_t ni_cap = (uint32_t)w.snap_icap;
+  const size_t vfloor = (size_t)(V + 1) * 4;
+  if ((st = pb_grow((void **)&w.pb_hitem, cp.hitem, ((size_t)ni_cap + 1) * 4)) ||
+      (st = pb_grow((void **)&w.pb_hoff, cp.hoff, vfloor)) ||
+      (st = pb_grow((void **)&w.pb_coff, cp.coff, vfloor)) ||
+      (st = pb_grow((void **)&w.pb_lidxr, cp.lidxr, vfloor)) ||
+      (st = pb_grow((void **)&w.pb_long, cp.lng, vfloor)) ||
+      (st = pb_grow((void **)&w.pb_Sh, cp.Sh, (size_t)V * 8)) ||
+      (st = pb_grow((void **)&w.pb_Sc, cp.Sc, (size_t)V * 8)) ||
+      (st = pb_grow((void **)&w.pb_partials, cp.partials, PB_PARTS * sizeof(double))))
+    return st;
+  // hot counts per item, scanned; row lengths of both streams, scanned
+  k_pb_itemhot<<<grid_for((uint64_t)ni_cap * WAVE, 256, 2048), 256, 0, s>>>(
+      w.snap_istart, w.snap_ivert, w.snap_ipos, V, w.rl_newid, w.snap_off, w.snap_cols,
+      w.pb_hitem);
+  scan_u32(w.pb_hitem, w.pb_hitem, ni_cap, w.snap_istart + V, w.snap_tmp, s);
+  k_pb_rowlen<<<grid_for(V), 256, 0, s>>>(w.snap_istart, w.pb_hitem, w.rl_order, w.snap_off, V,
+                                          w.pb_hoff, w.pb_coff);
+  scan_u32(w.pb_hoff, w.pb_hoff, V, nullptr, w.snap_tmp, s);
+  scan_u32(w.pb_coff, w.pb_coff, V, nullptr, w.snap_tmp, s);
+  uint32_t tot[2] = {0, 0};
+  GV_HIP(hipMemcpyAsync(&tot[0], w.pb_hoff + V, 4, hipMemcpyDeviceToHost, s));
+  GV_HIP(hipMemcpyAsync(&tot[1], w.pb_coff + V, 4, hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  const uint64_t Eh = tot[0], Ec = tot[1];
+  const uint64_t nht = cdiv(Eh, PF_T), nct = cdiv(Ec, PF_T);
+  const uint64_t hend = nht * PF_T, cend = nct * PF_T;
+  const uint32_t NB = (uint32_t)cdiv(nct, BT);
+  const uint64_t table = (uint64_t)NS * NB;
+  if (table >= (1ull << 29)) return GV_E_UNSUPPORTED;
+  // the streams are sized for the snapshot buffer's capacity, so a caller
+  // that presized it (the request loop) does not reallocate here either
+  const uint64_t ecap = std::max<uint64_t>(E, w.snap_min_cap) + V;
+  const uint64_t tcap = cdiv(ecap, PF_T) + 2;
+  const uint64_t bcap = cdiv(tcap, PB_MIN_BIN_TILES) + 1;
+  if ((st = pb_grow((void **)&w.pb_hot, cp.hot, hend * 2, tcap * PF_T * 2)) ||
+      (st = pb_grow((void **)&w.pb_cold, cp.cold, cend * 2, tcap * PF_T * 2)) ||
+      (st = pb_grow((void **)&w.pb_csrc, cp.csrc, cend * 4, tcap * PF_T * 4)) ||
+      (st = pb_grow((void **)&w.pb_htb, cp.htb, (nht + 1) * 4, tcap * 4)) ||
+      (st = pb_grow((void **)&w.pb_ctb, cp.ctb, (nct + 1) * 4, tcap * 4)) ||
+      (st = pb_grow((void **)&w.pb_PQh, cp.PQh, (nht + 1) * 16, tcap * 16)) ||
+      (st = pb_grow((void **)&w.pb_PQc, cp.PQc, (nct + 1) * 16, tcap * 16)) ||
+      (st = pb_grow((void **)&w.pb_cnt, cp.cnt, (table + 1) * 4)) ||
+      (st = pb_grow((void **)&w.pb_ngrp, cp.ngrp, ((size_t)NB + 1) * 4, (bcap + 1) * 4)) ||
+      (st = pb_grow((void **)&w.pb_nit, cp.nit, ((size_t)NS + 1) * 4 + 8)) ||
+      (st = pb_grow((void **)&w.pb_tmp, cp.tmp,
+                    std::max(scan_tmp_bytes(table + 1), scan_tmp_bytes((uint64_t)bcap + 1)))))
+    return st;
+  // both streams
+  k_pb_split<<<grid_for((uint64_t)ni_cap * WAVE, 256, 2048), 256, 0, s>>>(
+      w.snap_istart, w.snap_ivert, w.snap_ipos, V, w.rl_newid, w.snap_off, w.snap_cols,
+      w.pb_hitem, w.pb_hoff, w.pb_coff, w.pb_hot, w.pb_csrc);
+  k_pb_pads<<<grid_for(V), 256, 0, s>>>(w.snap_istart, w.pb_hitem, w.rl_order, w.snap_off, V,
+                                        w.pb_hoff, w.pb_coff, w.pb_hot, w.pb_csrc);
+  k_pb_tail<<<4, 256, 0, s>>>(w.pb_hoff, w.pb_coff, V, hend, cend, w.pb_hot, w.pb_csrc);
+  if (nht) k_pb_tbase<<<grid_for(nht), 256, 0, s>>>(w.pb_hoff, V, (uint32_t)nht, w.pb_htb);
+  if (nct) k_pb_tbase<<<grid_for(nct), 256, 0, s>>>(w.pb_coff, V, (uint32_t)nct, w.pb_ctb);
+  k_pb_longflag<<<grid_for(V), 256, 0, s>>>(w.pb_hoff, w.pb_coff, V, w.pb_lidxr);
+  scan_u32(w.pb_lidxr, w.pb_lidxr, V, nullptr, w.snap_tmp, s);
+  k_pb_longfill<<<grid_for(V), 256, 0, s>>>(w.pb_lidxr, V, w.pb_long);
+  // cold bins: padded (source block, bin) counts, expansion positions, image
+  // group offsets
+  if (NS) {
+    k_pb_bincnt<<<NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, cend, NS, NB, bin, w.pb_cnt,
+                                                w.pb_ngrp);
+  } else {
+    fill_u32(w.pb_ngrp, 0, NB, s);
+  }
+  scan_u32(w.pb_cnt, w.pb_cnt, table, nullptr, w.pb_tmp, s);
+  scan_u32(w.pb_ngrp, w.pb_ngrp, NB, nullptr, w.pb_tmp, s);
+  uint32_t n_long = 0, ptot[2] = {0, 0};
+  GV_HIP(hipMemcpyAsync(&n_long, w.pb_lidxr + V, 4, hipMemcpyDeviceToHost, s));
+  GV_HIP(hipMemcpyAsync(&ptot[0], w.pb_cnt + table, 4, hipMemcpyDeviceToHost, s));
+  GV_HIP(hipMemcpyAsync(&ptot[1], w.pb_ngrp + NB, 4, hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  const uint64_t P1 = ptot[0], G = ptot[1];
+  const uint64_t pcap = std::max<uint64_t>(P1, ecap + ecap / 8) + 8;
+  if ((st = pb_grow((void **)&w.pb_lidx, cp.lidx, (P1 + 8) * 2, pcap * 2)) ||
+      (st = pb_grow((void **)&w.pb_vals, cp.vals, (P1 + 8) * 4, pcap * 4)) ||
+      (st = pb_grow((void **)&w.pb_gaddr, cp.gaddr, (G + 8) * 4, pcap)))
+    return st;
+  if (NS) {
+    k_pb_scatter<<<NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, cend, NS, NB, bin, w.pb_cnt,
+                                                 w.pb_ngrp, w.pb_gaddr, w.pb_lidx, w.pb_cold);
+    k_pb_itemcnt<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, NB, w.pb_nit);
+    scan_u32(w.pb_nit, w.pb_nit, NS, nullptr, w.pb_tmp, s);
+    const uint64_t icap = cdiv(std::max<uint64_t>(pcap, 1), PB_CH) + NS + 1;
+    if ((st = pb_grow((void **)&w.pb_items, cp.items, icap * 16))) return st;
+    k_pb_itemfill<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, NB, w.pb_nit, w.pb_items);
+  } else {
+    // every cold entry is padding: the stream's entries all take index PB_PAD
+    k_pb_scatter<<<NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, cend, 0, NB, bin, w.pb_cnt,
+                                                 w.pb_ngrp, w.pb_gaddr, w.pb_lidx, w.pb_cold);
+    fill_u32(w.pb_nit, 0, 1, s);
+  }
+  GV_HIP(hipGetLastError());
+  w.pb_NS = NS;
+  w.pb_NB = NB;
+  w.pb_bin_tiles = BT;
+  w.pb_nht = (uint32_t)nht;
+  w.pb_nct = (uint32_t)nct;
+  w.pb_n_long = n_long;
+  w.pb_prepV = V;
+  w.pb_E = E;
+  w.pb_ready = 1;
+  return GV_OK;
+}
+
+gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
+                              double *residual, hipStream_t s, double *iter_ms) {
+  if (iters) *iters = 0;
+  if (residual) *residual = 0.0;
+  if (!w.pb_ready || o.max_iter <= 0) {
+    set_error("pagerank: no prepared snapshot, or max_iter <= 0");
+    return GV_E_INVALID;
+  }
+  const uint32_t V = w.pb_prepV;
+  const uint32_t *dout = w.rl_dout;  // out-degrees in label order
+  const float x0 = 1.0f / (float)V;
+  pr_ctl_reset(w.ctl, s);
+  pr_init_vectors(w.x[0], w.contrib[0], dout, V, x0, o.init, w.rl_order, s);
+
+  PbArgs a;
+  a.hot = reinterpret_cast<const uint4 *>(w.pb_hot);
+  a.cold = reinterpret_cast<const uint4 *>(w.pb_cold);
+  a.htbase = w.pb_htb;
+  a.ctbase = w.pb_ctb;
+  const uint64_t tcap_h = w.pb_caps.PQh / 16, tcap_c = w.pb_caps.PQc / 16;
+  a.Ph = w.pb_PQh;
+  a.Qh = w.pb_PQh + tcap_h;
+  a.Pc = w.pb_PQc;
+  a.Qc = w.pb_PQc + tcap_c;
+  a.Sh = w.pb_Sh;
+  a.Sc = w.pb_Sc;
+  a.hoff = w.pb_hoff;
+  a.coff = w.pb_coff;
+  a.gaddr = w.pb_gaddr;
+  a.gbase = w.pb_ngrp;
+  a.lidx = w.pb_lidx;
+  a.vals = w.pb_vals;
+  a.items = w.pb_items;
+  a.n_items = w.pb_nit + w.pb_NS;
+  a.deg_out = dout;
+  a.longrows = w.pb_long;
+  a.partials = w.pb_partials;
+  a.ctl = w.ctl;
+  a.V = V;
+  a.nh = V < PB_H ? V : PB_H;
+  a.n_htiles = w.pb_nht;
+  a.n_ctiles = w.pb_nct;
+  a.n_hot_units = (uint32_t)cdiv(w.pb_nht, PB_HOT_UNIT);
+  a.n_bins = w.pb_NB;
+  a.bin_tiles = w.pb_bin_tiles;
+  a.n_long = w.pb_n_long;
+  a.compensated = o.compensated ? 1u : 0u;
+  a.max_iter = (uint32_t)o.max_iter;
+  a.alpha = o.alpha;
+  a.base = (1.0f - o.alpha) / (float)V;
+  a.eps = o.eps;
+  // two tile workgroups per CU (80 KiB of LDS each); the request loop's
+  // CU share scales the grid through pf_grid (workgroups per CU pair)
+  const uint32_t cus = w.pf_grid ? w.pf_grid : (uint32_t)pb_device_cus();
+  const uint32_t grid = 2 * cus;
+  const uint32_t egrid = 2 * cus + 1;
+  a.x = w.x[0];
+  a.cb = w.contrib[0];
+  if (w.pb_NS) k_pb_expand<<<egrid - 1, PB_BUILD_THREADS, 0, s>>>(a, 0u);
+  if (iter_ms) {
+    while (w.ev.size() < (size_t)o.max_iter + 1) {
+      hipEvent_t e;
+      GV_HIP(hipEventCreate(&e));
+      w.ev.push_back(e);
+    }
+    GV_HIP(hipEventRecord(w.ev[0], s));
+  }
+  if (w.ev_grp[0] == nullptr) {
+    for (int i = 0; i < 2; ++i)
+      GV_HIP(hipEventCreateWithFlags(&w.ev_grp[i], hipEventDisableTiming));
+  }
+  int launched = 0;
+  // Iterations go out in groups (one at a time with a yield hook); after each
+  // group the done flag is copied to a host slot and the host reads the
+  // previous group's copy, so the device always has a group queued.
+  constexpr int PB_GROUP = 8;
+  const int group = o.yield ? 1 : PB_GROUP;
+  int pending = -1;
+  while (launched < o.max_iter) {
+    const int t = launched;
+    if (o.yield) o.yield(o.yield_arg);
+    a.x = w.x[t & 1];
+    a.cb = w.contrib[t & 1];
+    a.xn = w.x[(t + 1) & 1];
+    a.cn = w.contrib[(t + 1) & 1];
+    if (a.compensated) k_pb_tiles<true><<<grid, PB_THREADS, 0, s>>>(a);
+    else k_pb_tiles<false><<<grid, PB_THREADS, 0, s>>>(a);
+    k_pb_rows<<<PB_PARTS, PB_ROW_THREADS, 0, s>>>(a);
+    // the residual, and the expansion of the contributions just written
+    PbArgs ae = a;
+    ae.cb = a.cn;
+    k_pb_expand<<<w.pb_NS ? egrid : 1, PB_BUILD_THREADS, 0, s>>>(ae, 1u);
+    if (iter_ms) GV_HIP(hipEventRecord(w.ev[t + 1], s));
+    ++launched;
+    if (launched % group == 0 && launched < o.max_iter) {
+      const int slot = 1 + (launched / group & 1);
+      GV_HIP(hipMemcpyAsync(&w.h_ctl[slot], w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+      GV_HIP(hipEventRecord(w.ev_grp[slot - 1], s));
+      if (o.yield) {
+        GV_HIP(hipEventSynchronize(w.ev_grp[slot - 1]));
+        if (w.h_ctl[slot].done) break;
+      } else {
+        if (pending > 0) {
+          GV_HIP(hipEventSynchronize(w.ev_grp[pending - 1]));
+          if (w.h_ctl[pending].done) break;
+        }
+        pending = slot;
+      }
+    }
+  }
+  GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  const uint32_t it = w.h_ctl->iters;
+  pr_unpermute(w.x[it & 1], w.rl_newid, V, out_dev, s);
+  if (iters) *iters = (int)it;
+  if (residual) *residual = w.h_ctl->residual;
+  if (iter_ms) {
+    float ms = 0.f;
+    const int last = (int)it < launched ? (int)it : launched;
+    if (last > 0) GV_HIP(hipEventElapsedTime(&ms, w.ev[0], w.ev[last]));
+    *iter_ms = ms;
+  }
+  return GV_OK;
+}
+
+}  // namespace gv

@@ -400,7 +400,7 @@ static void analytics_loop(gv_server *s) {
             fill_f32(s->pr[nxt], x0, s->V, s->astream);
           }
         } else {
-          rc = pagerank_flat_iterate(o, sl.w, s->pr[nxt], &iters, &res, s->astream, nullptr);
+          rc = pagerank_snap_iterate(o, sl.w, s->pr[nxt], &iters, &res, s->astream, nullptr);
         }
       }
       if (!rc) s->pr_cur = nxt;

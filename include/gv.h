@@ -60,12 +60,14 @@ typedef struct gv_config {
   uint32_t pr_wave_max;       /*   <= this -> 1 wavefront; larger -> chunked */
   int maintain_in_edges;      /* directed graphs: keep the in-adjacency mirror */
   int device;                 /* HIP device ordinal */
-  int pr_layout;              /* PageRank: 1 = snapshot of the live in-adjacency, rebuilt per
-                                 call, vertices relabelled by descending out-degree, iterated
-                                 as a flat stream of equal tiles (default); 4 = the same
-                                 relabelled snapshot iterated row by row; 3 = CSR snapshot in
-                                 the original vertex order; 2 = pull over the block store;
-                                 0 = XCD-segmented relabelled snapshot */
+  int pr_layout;              /* PageRank: 5 = snapshot of the live in-adjacency, rebuilt per
+                                 call, vertices relabelled by descending out-degree, split into
+                                 a hot stream (sources among the highest out-degrees, staged in
+                                 LDS) and a propagation-blocked cold stream (default; graphs
+                                 beyond its limits use 1); 1 = the same snapshot iterated as one
+                                 flat stream of equal tiles; 4 = the relabelled snapshot row by
+                                 row; 3 = CSR snapshot in the original vertex order; 2 = pull
+                                 over the block store; 0 = XCD-segmented relabelled snapshot */
 } gv_config;
 
 /* Result of one update call. */

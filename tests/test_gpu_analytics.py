@@ -71,7 +71,7 @@ def test_pagerank_closed_forms():
     assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("directed", [True, False])
 def test_pagerank_config1_after_updates(directed, layout):
     V = 1 << 14
@@ -116,7 +116,7 @@ def test_pagerank_iteration_limits_and_repeated_calls():
         assert np.array_equal(x, first)
 
 
-@pytest.mark.parametrize("layout", [1, 2, 3, 4])
+@pytest.mark.parametrize("layout", [1, 2, 3, 4, 5])
 def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     # small thresholds push vertices through every bin incl. chunked hubs
     V = 3000
@@ -139,7 +139,7 @@ def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     check_pagerank(g, o)
 
 
-@pytest.mark.parametrize("layout", [1, 4])
+@pytest.mark.parametrize("layout", [1, 4, 5])
 @pytest.mark.parametrize("V", [40000, 1 << 17])
 @pytest.mark.parametrize("directed", [False, True])
 def test_pagerank_staged_hot_and_cold_sources(V, directed, layout):
@@ -161,6 +161,42 @@ def test_pagerank_staged_hot_and_cold_sources(V, directed, layout):
     sd, dd = O.gen_sample(O.stream_key(5, 2000), V, off, cols, V // 4)
     g.batchDelete(sd, dd)  # tombstones: the snapshot keeps live entries only
     o.delete(sd, dd)
+    check_pagerank(g, o)
+    g.close()
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_pagerank_propagation_blocked_many_blocks_and_bins(directed):
+    """Layout 5 at a size with many cold source blocks (16384 labels each,
+    past the 32767 hot labels) and many 31-tile bins: rows empty in one
+    stream or both (padding entries), a hub row spanning many tiles of the
+    cold stream and hub sources in many hot rows, a warm start, plain sums,
+    and the result after deletes and inserts, all at L1 <= 1e-6 against the
+    oracle at equal iterations; reruns are bit-identical."""
+    V = 300_000
+    s, d = O.gen_rmat(O.stream_key(52, O.STREAM_RMAT), 19, 10 * V)
+    s, d = s % V, d % V
+    hub = 12345
+    s = np.concatenate([s, np.full(60000, hub, np.uint32)])
+    d = np.concatenate([d, np.arange(100000, 160000, dtype=np.uint32)])
+    g, o = build_pair(V, directed, s, d, pool=24 * V, pr_layout=5)
+    check_pagerank(g, o)
+    x1, it1, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+    x2, it2, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
+    assert it1 == it2 and np.array_equal(x1, x2)
+    xw, itw, _ = G.pagerank(g, init=x1 * 1.5)
+    xo, _, _ = o.pagerank(fixed_iters=itw, x0=(x1 * 1.5).astype(np.float64))
+    assert np.abs(xw.astype(np.float64) - xo).sum() <= PR_L1
+    xp, itp, _ = G.pagerank(g, compensated=False)
+    xo, _, _ = o.pagerank(fixed_iters=itp)
+    assert np.abs(xp.astype(np.float64) - xo).sum() <= PR_L1
+    off, cols = o.export_csr()
+    sd, dd = O.gen_sample(O.stream_key(52, 2000), V, off, cols, V)
+    g.batchDelete(sd, dd)
+    o.delete(sd, dd)
+    si, di = O.gen_uniform(O.stream_key(52, 3000), V, V)
+    g.batchInsert(si, di)
+    o.insert(si, di)
     check_pagerank(g, o)
     g.close()
 
