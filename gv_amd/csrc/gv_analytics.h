@@ -162,9 +162,12 @@ gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBin
 
 gv_status pr_reserve(PrWork &w, uint32_t V);
 // CSR snapshot of the live in-adjacency (relabelled by descending out-degree
-// when relabel); flat != 0 writes the tiled stream layout above.
+// when relabel); flat != 0 writes the tiled stream layout above.  hitem (may
+// be null, pr_snapshot_icap(in) entries): per chunk item of the snapshot, the
+// entries it wrote whose (relabelled) source is below hot_below.
+uint64_t pr_snapshot_icap(const Store &in);
 gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
-                      hipStream_t s, int flat);
+                      hipStream_t s, int flat, uint32_t *hitem = nullptr, uint32_t hot_below = 0);
 void pr_init_vectors(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0,
                      const float *init, const uint32_t *order, hipStream_t s);
 void pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out, hipStream_t s);

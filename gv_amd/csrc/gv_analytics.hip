@@ -296,10 +296,12 @@ __global__ void k_snap_off(const uint32_t *istart, uint32_t V, const uint64_t *i
 // writes the tiled stream layout of the flat PageRank iteration instead:
 // entry p of the row-major stream is stored at pf_phys(p) and the first
 // entry of every row carries PF_FLAG.
+// hitem (may be null): per item, the entries it wrote below hot_below.
 __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *istart,
                                                     const uint32_t *ivert, const uint64_t *ipos,
                                                     const uint32_t *newid, const uint64_t *off,
-                                                    uint32_t *cols, uint32_t flat) {
+                                                    uint32_t *cols, uint32_t flat,
+                                                    uint32_t *hitem, uint32_t hot_below) {
   const uint32_t ni = istart[st.V];
   const uint32_t lane = lane_id();
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
@@ -312,19 +314,23 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
     uint64_t dst = ipos[i];
     const uint64_t row0 = newid ? off[newid[v]] : ipos[istart[v]];
     if (newid) dst = row0 + (dst - ipos[istart[v]]);
+    uint32_t hot = 0;
     for (uint32_t p = lo; p < hi; p += WAVE) {
       uint32_t w = TOMB;
       if (p + lane < hi) w = st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)];
       const bool live = w != TOMB;
       const unsigned long long m = __ballot(live);
+      uint32_t val = 0;
       if (live) {
         const uint64_t q = dst + mask_lt_count(m);
-        const uint32_t val = newid ? newid[w] : w;
+        val = newid ? newid[w] : w;
         if (flat) cols[pf_phys(q)] = val | (q == row0 ? PF_FLAG : 0u);
         else cols[q] = val;
       }
+      if (hitem) hot += (uint32_t)__popcll(__ballot(live && val < hot_below));
       dst += (uint64_t)__popcll(m);
     }
+    if (hitem && lane == 0) hitem[i] = hot;
   }
 }
 
@@ -354,11 +360,15 @@ __global__ void k_rl_perm(const uint64_t *sorted, uint32_t V, uint32_t bV, const
   }
 }
 
-gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
-                      hipStream_t s, int flat) {
-  const uint32_t V = in.V;
+uint64_t pr_snapshot_icap(const Store &in) {
   // items: at most one per vertex plus one per SNAP_CH slots of the pool
-  const uint64_t icap = (uint64_t)V + in.P * (uint64_t)in.B / SNAP_CH + 1;
+  return (uint64_t)in.V + in.P * (uint64_t)in.B / SNAP_CH + 1;
+}
+
+gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
+                      hipStream_t s, int flat, uint32_t *hitem, uint32_t hot_below) {
+  const uint32_t V = in.V;
+  const uint64_t icap = pr_snapshot_icap(in);
   if (icap > 0xFFFFFFFFull) {
     set_error("pagerank snapshot: too many chunk items");
     return GV_E_INVALID;
@@ -442,7 +452,7 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
   }
   k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(
       in, w.snap_istart, w.snap_ivert, w.snap_ipos, relabel ? w.rl_newid : nullptr, w.snap_off,
-      w.snap_cols, flat ? 1u : 0u);
+      w.snap_cols, flat ? 1u : 0u, hitem, hot_below);
   return GV_OK;
 }
 

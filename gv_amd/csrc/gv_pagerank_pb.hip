@@ -60,8 +60,14 @@ __host__ __device__ inline uint32_t pb_bin_tiles(uint32_t NS) {
   const uint32_t room = (PB_PAD - 1 - 3 * NS) / PF_T;
   return room / PB_MIN_BIN_TILES * PB_MIN_BIN_TILES;
 }
+// The image is filled in PB_ROUNDS rounds of PB_PER_ROUND 16-byte loads per
+// thread, each round's loads issued before its LDS writes (two rounds keep
+// the loaded values within the register budget).
+constexpr uint32_t PB_PER_ROUND = 5;
+constexpr uint32_t PB_ROUNDS = 2;
+static_assert(PB_ROUNDS * PB_PER_ROUND * 512 >= PB_PAD / 4, "the rounds cover the image");
 constexpr uint32_t PB_HOT_UNIT = 32;          // hot tiles per work unit (4 per wavefront)
-constexpr uint32_t PB_CH = 1u << 17;          // entries per expansion work item (multiple of 4)
+constexpr uint32_t PB_CH = 1u << 15;          // entries per expansion work item (multiple of 4)
 constexpr int PB_THREADS = 512;               // tile workgroups: 8 wavefronts
 constexpr int PB_BUILD_THREADS = 1024;
 constexpr uint32_t PB_CSRC_FLAG = 0x80000000u;
@@ -86,30 +92,6 @@ __host__ __device__ inline uint64_t pb_phys(uint64_t p) {
 __host__ __device__ inline uint32_t pb_pad4(uint32_t c) { return (c + 3u) & ~3u; }
 
 // ------------------------------------------------------- layout build --
-// Hot entries of every snapshot item (one wavefront per item).  Item i of
-// old vertex v covers live entries [ipos[i], ipos[i+1]) of v's row, which is
-// row newid[v] of the relabelled snapshot (cols, row-major, offsets off).
-__global__ __launch_bounds__(256) void k_pb_itemhot(const uint32_t *istart, const uint32_t *ivert,
-                                                    const uint64_t *ipos, uint32_t V,
-                                                    const uint32_t *newid, const uint64_t *off,
-                                                    const uint32_t *cols, uint32_t *hitem) {
-  const uint32_t ni = istart[V];
-  const uint32_t lane = lane_id();
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
-       i += nwaves) {
-    const uint32_t v = ivert[i];
-    const uint64_t len = ipos[i + 1] - ipos[i];
-    const uint64_t base = off[newid[v]] + (ipos[i] - ipos[istart[v]]);
-    uint32_t cnt = 0;
-    for (uint64_t p = 0; p < len; p += WAVE) {
-      const bool hot = p + lane < len && cols[base + p + lane] < PB_H;
-      cnt += (uint32_t)__popcll(__ballot(hot));
-    }
-    if (lane == 0) hitem[i] = cnt;
-  }
-}
-
 // Row lengths of both streams in label order (a padding entry for an empty
 // row).  hscan: exclusive scan of the items' hot counts.
 __global__ void k_pb_rowlen(const uint32_t *istart, const uint32_t *hscan, const uint32_t *order,
@@ -502,6 +484,36 @@ __device__ __forceinline__ void pb_tile_sums(float2 *S, float2 *P, float2 *Q, ui
   }
 }
 
+// One round of an image fill: groups g = threadIdx.x + (R * 5 + i) *
+// PB_THREADS, i < 5, of a source of n groups (through the group map gmap
+// when given, else directly); all five loads are issued before the LDS
+// writes.  Groups past n reload the last one and write image slots no entry
+// names; no group reaches the zero slot.
+template <uint32_t R>
+__device__ __forceinline__ void pb_fill_round(float4 *img4, const float4 *src,
+                                              const uint32_t *gmap, uint32_t n) {
+  static_assert(PB_PER_ROUND == 5, "the round below is written out for five loads");
+  const uint32_t b = threadIdx.x + R * PB_PER_ROUND * PB_THREADS;
+  const uint32_t g0 = b, g1 = b + PB_THREADS, g2 = b + 2 * PB_THREADS, g3 = b + 3 * PB_THREADS,
+                 g4 = b + 4 * PB_THREADS;
+  uint32_t a0 = g0 < n ? g0 : n - 1, a1 = g1 < n ? g1 : n - 1, a2 = g2 < n ? g2 : n - 1,
+           a3 = g3 < n ? g3 : n - 1, a4 = g4 < n ? g4 : n - 1;
+  if (gmap) {
+    a0 = gmap[a0];
+    a1 = gmap[a1];
+    a2 = gmap[a2];
+    a3 = gmap[a3];
+    a4 = gmap[a4];
+  }
+  const float4 v0 = src[a0], v1 = src[a1], v2 = src[a2], v3 = src[a3], v4 = src[a4];
+  __builtin_amdgcn_sched_barrier(0);
+  img4[g0] = v0;
+  img4[g1] = v1;
+  img4[g2] = v2;
+  img4[g3] = v3;
+  if (R + 1 < PB_ROUNDS || g4 < PB_PAD / 4) img4[g4] = v4;
+}
+
 // Work units from one counter per iteration: first the hot units (PB_HOT_UNIT
 // tiles of the hot stream, staged hot contributions), then the cold bins (a
 // bin's values loaded into the image, then its bin_tiles tiles, dealt to the
@@ -532,24 +544,19 @@ __global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
     if (u >= n_units) break;
     if (u < a.n_hot_units) {
       if (!staged) {
-        // every load of the stage is issued before the first LDS write
-        constexpr uint32_t NQ = (PB_IMG / 4 + PB_THREADS - 1) / PB_THREADS;
-        const float4 *c4 = reinterpret_cast<const float4 *>(a.cb);
-        float4 hv[NQ];
-#pragma unroll
-        for (uint32_t j = 0; j < NQ; ++j) {
-          const uint32_t i = threadIdx.x + j * PB_THREADS;
-          hv[j] = 4 * i + 3 < a.nh ? c4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // every load of the stage is issued before the first LDS write; the
+        // vector part covers whole groups of 4 labels below nh (groups past
+        // them reload the last one, writing image slots no hot entry names)
+        const uint32_t nh4 = a.nh / 4;
+        if (nh4) {
+          const float4 *c4 = reinterpret_cast<const float4 *>(a.cb);
+          float4 *img4 = reinterpret_cast<float4 *>(s_img);
+          pb_fill_round<0>(img4, c4, nullptr, nh4);
+          pb_fill_round<1>(img4, c4, nullptr, nh4);
         }
-        float4 *img4 = reinterpret_cast<float4 *>(s_img);
-#pragma unroll
-        for (uint32_t j = 0; j < NQ; ++j) {
-          const uint32_t i = threadIdx.x + j * PB_THREADS;
-          if (4 * i + 3 < PB_IMG) img4[i] = hv[j];
-        }
-        // a partial last group of hot labels, and the image's last slots
-        for (uint32_t i = (a.nh & ~3u) + threadIdx.x; i < PB_IMG; i += PB_THREADS)
-          if (i >= (PB_IMG & ~3u) || i < a.nh) s_img[i] = i < a.nh ? a.cb[i] : 0.f;
+        // the last partial group of hot labels, then the zero slot
+        for (uint32_t i = nh4 * 4 + threadIdx.x; i < a.nh; i += PB_THREADS) s_img[i] = a.cb[i];
+        if (threadIdx.x == 0) s_img[PB_PAD] = 0.f;
         __syncthreads();
         staged = true;
       }
@@ -581,25 +588,12 @@ __global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
     const uint32_t g0 = a.gbase[b], ng = a.gbase[b + 1] - g0;
     const float4 *v4 = reinterpret_cast<const float4 *>(a.vals);
     float4 *img4 = reinterpret_cast<float4 *>(s_img);
-    // all loads of a round are issued before any image write (addresses of
-    // groups past the bin's end are clamped to a valid one, their values
-    // dropped)
-    constexpr uint32_t NG = 10;
-    for (uint32_t gb = 0; gb < ng; gb += NG * PB_THREADS) {
-      uint32_t ga[NG];
-#pragma unroll
-      for (uint32_t i = 0; i < NG; ++i) {
-        const uint32_t g = gb + threadIdx.x + i * PB_THREADS;
-        ga[i] = a.gaddr[g0 + (g < ng ? g : 0u)];
-      }
-      float4 v[NG];
-#pragma unroll
-      for (uint32_t i = 0; i < NG; ++i) v[i] = v4[ga[i]];
-#pragma unroll
-      for (uint32_t i = 0; i < NG; ++i) {
-        const uint32_t g = gb + threadIdx.x + i * PB_THREADS;
-        if (g < ng && g < PB_PAD / 4) img4[g] = v[i];
-      }
+    // all image loads are issued before any image write; groups past the
+    // bin's end reload its last group and write image slots past its image
+    // (never named by an entry, and below the zero slot)
+    if (ng) {
+      pb_fill_round<0>(img4, v4, a.gaddr + g0, ng);
+      pb_fill_round<1>(img4, v4, a.gaddr + g0, ng);
     }
     if (threadIdx.x == 0) s_img[PB_PAD] = 0.f;
     staged = false;
@@ -787,11 +781,18 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_expand(PbArgs a, uint32
   const uint32_t n_items = *a.n_items;
   const uint2 *l4 = reinterpret_cast<const uint2 *>(a.lidx);
   float4 *v4 = reinterpret_cast<float4 *>(a.vals);
-  for (uint32_t it = bid; it < n_items; it += nb) {
+  // a contiguous run of items per workgroup: consecutive items mostly share
+  // their source block, whose window is then staged once
+  const uint32_t i0 = (uint32_t)((uint64_t)n_items * bid / nb);
+  const uint32_t i1 = (uint32_t)((uint64_t)n_items * (bid + 1) / nb);
+  uint32_t staged_k = 0xFFFFFFFFu;
+  for (uint32_t it = i0; it < i1; ++it) {
     const uint4 item = a.items[it];
     const uint32_t s0 = PB_H + item.x * PB_SB;
     const uint32_t ns = a.V - s0 < PB_SB ? a.V - s0 : PB_SB;
-    {
+    if (item.x != staged_k) {
+      if (staged_k != 0xFFFFFFFFu) __syncthreads();  // the previous window's last reads
+      staged_k = item.x;
       // the window's loads all issued before its LDS writes (s0 is a multiple
       // of 4, so whole groups are 16-byte loads)
       constexpr uint32_t NW = PB_SB / 4 / PB_BUILD_THREADS;
@@ -802,13 +803,14 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_expand(PbArgs a, uint32
         const uint32_t i = threadIdx.x + j * PB_BUILD_THREADS;
         wv[j] = 4 * i + 3 < ns ? c4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
+      __builtin_amdgcn_sched_barrier(0);
       float4 *w4 = reinterpret_cast<float4 *>(s_win);
 #pragma unroll
       for (uint32_t j = 0; j < NW; ++j) w4[threadIdx.x + j * PB_BUILD_THREADS] = wv[j];
       for (uint32_t i = (ns & ~3u) + threadIdx.x; i < ns; i += PB_BUILD_THREADS)
         s_win[i] = a.cb[s0 + i];
+      __syncthreads();
     }
-    __syncthreads();
     constexpr uint32_t U = 4;
     const uint32_t ga = item.y / 4, gb = item.z / 4;
     for (uint32_t g = ga + threadIdx.x; g < gb; g += U * PB_BUILD_THREADS) {
@@ -826,7 +828,6 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_expand(PbArgs a, uint32
                               s_win[l[u].y & 0xFFFFu], s_win[l[u].y >> 16]);
       }
     }
-    __syncthreads();
   }
 }
 
@@ -885,18 +886,19 @@ gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &
   if (NS > PB_MAX_NS) return GV_E_UNSUPPORTED;
   gv_status st = pr_reserve(w, V);
   if (st) return st;
-  st = pr_snapshot(in, deg_out, true, w, s, 0);
+  PbCaps &cp = w.pb_caps;
+  // hot entries per snapshot item, counted while the snapshot is written
+  if ((st = pb_grow((void **)&w.pb_hitem, cp.hitem, (pr_snapshot_icap(in) + 1) * 4))) return st;
+  st = pr_snapshot(in, deg_out, true, w, s, 0, w.pb_hitem, PB_H);
   if (st) return st;
   const uint64_t E = w.snap_E;
   // stream positions, expansion positions and the segment table are 32-bit
   const uint32_t BT = pb_bin_tiles(NS);
   const uint32_t bin = BT * PF_T;
   if (E + 2ull * V + 2ull * bin >= 0x7FFFFFFFull) return GV_E_UNSUPPORTED;
-  PbCaps &cp = w.pb_caps;
   const uint32_t ni_cap = (uint32_t)w.snap_icap;
   const size_t vfloor = (size_t)(V + 1) * 4;
-  if ((st = pb_grow((void **)&w.pb_hitem, cp.hitem, ((size_t)ni_cap + 1) * 4)) ||
-      (st = pb_grow((void **)&w.pb_hoff, cp.hoff, vfloor)) ||
+  if ((st = pb_grow((void **)&w.pb_hoff, cp.hoff, vfloor)) ||
       (st = pb_grow((void **)&w.pb_coff, cp.coff, vfloor)) ||
       (st = pb_grow((void **)&w.pb_lidxr, cp.lidxr, vfloor)) ||
       (st = pb_grow((void **)&w.pb_long, cp.lng, vfloor)) ||
@@ -904,10 +906,7 @@ gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &
       (st = pb_grow((void **)&w.pb_Sc, cp.Sc, (size_t)V * 8)) ||
       (st = pb_grow((void **)&w.pb_partials, cp.partials, PB_PARTS * sizeof(double))))
     return st;
-  // hot counts per item, scanned; row lengths of both streams, scanned
-  k_pb_itemhot<<<grid_for((uint64_t)ni_cap * WAVE, 256, 2048), 256, 0, s>>>(
-      w.snap_istart, w.snap_ivert, w.snap_ipos, V, w.rl_newid, w.snap_off, w.snap_cols,
-      w.pb_hitem);
+  // hot counts per item scanned; row lengths of both streams, scanned
   scan_u32(w.pb_hitem, w.pb_hitem, ni_cap, w.snap_istart + V, w.snap_tmp, s);
   k_pb_rowlen<<<grid_for(V), 256, 0, s>>>(w.snap_istart, w.pb_hitem, w.rl_order, w.snap_off, V,
                                           w.pb_hoff, w.pb_coff);
