@@ -694,7 +694,7 @@ def cpu_sequential(V, args, batches, off, cols, first):
                       f"then the median of {n}"}
 
 
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pagerank_flat_rmat22_traffic.json")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pagerank_pb_rmat22_traffic.json")
 PR_ITER_KERNELS = ("k_pf_tiles", "k_pf_rows", "k_pf_resid", "k_pb_tiles", "k_pb_rows",
                    "k_pb_expand")
 
