@@ -360,4 +360,62 @@ void omp_bc_dir(uint32_t V, const uint64_t *off, const uint32_t *cols, const uin
   free(part);
 }
 
+/* Personalized PageRank of the out-adjacency CSR, ncols columns (seed s_c
+ * each), the oracle's rule and arithmetic (orc_pagerank_personalized,
+ * gv_oracle.c): push x_c[u]/|N(u)| along every out-edge in (u, j) order,
+ * then x_c' = a*acc + (1-a)[v == s_c], residual summed over v in order, stop
+ * on the largest per-column residual below eps.  Columns are independent, so
+ * threads take whole columns (held column-major, one V-array each) and every
+ * column sees exactly the oracle's operations in the oracle's order: the
+ * result equals the oracle's bit for bit.  x is row-major V x ncols. */
+int omp_pagerank_personalized(uint32_t V, const uint64_t *off, const uint32_t *cols,
+                              double alpha, double eps, int max_iter, int fixed_iters,
+                              const uint32_t *seeds, uint32_t ncols, double *x,
+                              double *max_residual) {
+  size_t n = (size_t)V * ncols;
+  double *xc = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double *acc = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double *r = (double *)malloc(sizeof(double) * (ncols ? ncols : 1));
+  int limit = fixed_iters > 0 ? fixed_iters : max_iter, it = 0;
+  double mr = 0.0;
+#pragma omp parallel for schedule(static, 1)
+  for (uint32_t c = 0; c < ncols; ++c) {
+    double *xv = xc + (size_t)c * V;
+    for (uint32_t v = 0; v < V; ++v) xv[v] = 0.0;
+    xv[seeds[c]] = 1.0;
+  }
+  while (it < limit) {
+#pragma omp parallel for schedule(static, 1)
+    for (uint32_t c = 0; c < ncols; ++c) {
+      double *xv = xc + (size_t)c * V, *av = acc + (size_t)c * V;
+      memset(av, 0, sizeof(double) * V);
+      for (uint32_t u = 0; u < V; ++u) {
+        uint64_t b = off[u], e = off[u + 1];
+        if (b == e) continue;
+        double q = xv[u] / (double)(e - b);
+        for (uint64_t j = b; j < e; ++j) av[cols[j]] += q;
+      }
+      double rc = 0.0;
+      for (uint32_t v = 0; v < V; ++v) {
+        double xn = alpha * av[v] + (seeds[c] == v ? 1.0 - alpha : 0.0);
+        rc += fabs(xn - xv[v]);
+        xv[v] = xn;
+      }
+      r[c] = rc;
+    }
+    mr = 0.0;
+    for (uint32_t c = 0; c < ncols; ++c) mr = r[c] > mr ? r[c] : mr;
+    ++it;
+    if (fixed_iters <= 0 && mr < eps) break;
+  }
+#pragma omp parallel for schedule(static)
+  for (uint32_t v = 0; v < V; ++v)
+    for (uint32_t c = 0; c < ncols; ++c) x[(size_t)v * ncols + c] = xc[(size_t)c * V + v];
+  if (max_residual) *max_residual = mr;
+  free(xc);
+  free(acc);
+  free(r);
+  return it;
+}
+
 int omp_threads(void) { return omp_get_max_threads(); }

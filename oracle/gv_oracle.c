@@ -365,6 +365,9 @@ int orc_pagerank_personalized(const orc_graph *g, double alpha, double eps, int 
   size_t n = (size_t)V * ncols;
   double *acc = (double *)malloc(sizeof(double) * (n ? n : 1));
   double *r = (double *)malloc(sizeof(double) * (ncols ? ncols : 1));
+  /* x_c[u]/|N(u)| of the current source row, computed once per row (the
+   * same quotients every edge of the row would compute) */
+  double *q = (double *)malloc(sizeof(double) * (ncols ? ncols : 1));
   for (size_t i = 0; i < n; ++i) x[i] = 0.0;
   for (uint32_t c = 0; c < ncols; ++c) x[(size_t)seeds[c] * ncols + c] = 1.0;
   int limit = fixed_iters > 0 ? fixed_iters : max_iter, it = 0;
@@ -374,10 +377,10 @@ int orc_pagerank_personalized(const orc_graph *g, double alpha, double eps, int 
     for (uint32_t u = 0; u < V; ++u) {
       uint32_t d = g->out[u].n;
       if (!d) continue;
+      for (uint32_t c = 0; c < ncols; ++c) q[c] = x[(size_t)u * ncols + c] / (double)d;
       for (uint32_t j = 0; j < d; ++j) {
-        uint32_t v = g->out[u].a[j];
-        for (uint32_t c = 0; c < ncols; ++c)
-          acc[(size_t)v * ncols + c] += x[(size_t)u * ncols + c] / (double)d;
+        double *av = acc + (size_t)g->out[u].a[j] * ncols;
+        for (uint32_t c = 0; c < ncols; ++c) av[c] += q[c];
       }
     }
     for (uint32_t c = 0; c < ncols; ++c) r[c] = 0.0;
@@ -396,6 +399,7 @@ int orc_pagerank_personalized(const orc_graph *g, double alpha, double eps, int 
   if (max_residual) *max_residual = mr;
   free(acc);
   free(r);
+  free(q);
   return it;
 }
 

@@ -67,6 +67,8 @@ def lib():
         "omp_transpose": ([u32, P, P, P, P], None),
         "omp_pagerank_dir": ([u32, P, P, P, f64, f64, i32, i32, P, ctypes.POINTER(f64)], i32),
         "omp_bc_dir": ([u32, P, P, P, u32, P], None),
+        "omp_pagerank_personalized": ([u32, P, P, f64, f64, i32, i32, P, u32, P,
+                                       ctypes.POINTER(f64)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -238,6 +240,22 @@ def omp_pagerank_sym(V, off, cols, alpha=0.85, eps=1e-6, max_iter=100, fixed_ite
     it = lib().omp_pagerank_sym(V, _p(off), _p(cols), alpha, eps, max_iter, fixed_iters,
                                 _p(start) if start is not None else None, _p(x), ctypes.byref(r))
     return (x, int(it), float(r.value)) if residual else (x, int(it))
+
+
+def omp_pagerank_personalized(V, off, cols, seeds, alpha=0.85, eps=1e-6, max_iter=100,
+                              fixed_iters=0):
+    """The oracle's personalized PageRank (OracleGraph.pagerank_personalized)
+    on the out-adjacency CSR, columns spread over OpenMP threads: same
+    operations in the same order per column, so the same bits.  Returns
+    (X[V, ncols] FP64, iters, largest per-column residual)."""
+    sd = u32(seeds)
+    off = np.ascontiguousarray(off, np.uint64)
+    cols = u32(cols) if len(cols) else np.zeros(1, np.uint32)
+    X = np.empty((V, len(sd)), np.float64)
+    r = ctypes.c_double()
+    it = lib().omp_pagerank_personalized(V, _p(off), _p(cols), alpha, eps, max_iter, fixed_iters,
+                                         _p(sd), len(sd), _p(X), ctypes.byref(r))
+    return X, int(it), float(r.value)
 
 
 def omp_triangle_count(V, off, cols, per_vertex=True):

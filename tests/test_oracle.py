@@ -525,3 +525,26 @@ def test_omp_fp64_references_match_the_oracle(directed):
     ioff, icols = O.omp_transpose(V, off, cols)
     assert ioff[-1] == off[-1]
     assert np.array_equal(np.diff(ioff), np.bincount(cols, minlength=V))
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_omp_personalized_pagerank_equals_the_oracle_bitwise(directed):
+    """The OpenMP personalized PageRank (the full-size reference of the
+    64-column block) runs the oracle's per-column operations in the oracle's
+    order, so both its stopping iteration and every value equal the
+    oracle's exactly: on a graph with dangling vertices and repeated seeds,
+    converging and at a fixed iteration count."""
+    V = 1500
+    s, d = O.gen_rmat(O.stream_key(62, O.STREAM_RMAT), 11, 6 * V)
+    o = O.OracleGraph(V, directed)
+    o.bulk_build(s % V, d % V)
+    off, cols = o.export_csr()
+    assert directed is False or (np.diff(off) == 0).any()
+    seeds = np.array([0, 7, 7, 1499, 3, 100, 42], np.uint32)
+    X, it, r = O.omp_pagerank_personalized(V, off, cols, seeds, 0.85, 1e-6, 200)
+    Xo, ito, ro = o.pagerank_personalized(seeds, 0.85, 1e-6, 200)
+    assert it == ito and r == ro and 1 < it < 200
+    assert np.array_equal(X, Xo)
+    X3, it3, r3 = O.omp_pagerank_personalized(V, off, cols, seeds, fixed_iters=3)
+    Xo3, ito3, ro3 = o.pagerank_personalized(seeds, fixed_iters=3)
+    assert it3 == ito3 == 3 and r3 == ro3 and np.array_equal(X3, Xo3)
