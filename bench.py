@@ -53,6 +53,9 @@ def parse():
                         "propagation-blocked cold stream (default), 1 = the same snapshot as "
                         "one flat stream, 3 = snapshot in vertex order, 2 = pull over the "
                         "store, 0 = XCD-segmented snapshot")
+    p.add_argument("--bc-bu-ratio", type=float, default=None,
+                   help="BC: bottom-up when a level's candidate in-slots are below this "
+                        "multiple of its frontier's out-slots (default: the library's)")
     p.add_argument("--workload", default="rmat22",
                    choices=["rmat22", "chunglu", "precision", "serve"],
                    help="rmat22: plain vs batched update stream on undirected R-MAT (default); "
@@ -125,7 +128,8 @@ def run_rmat(args):
     n_raw = args.edge_factor * V
     half = args.batch // 2
     t_setup = time.time()
-    g = G.GraphVine(V, directed=False, pr_layout=args.pr_layout)
+    kw = {} if args.bc_bu_ratio is None else {"bc_bu_ratio": args.bc_bu_ratio}
+    g = G.GraphVine(V, directed=False, pr_layout=args.pr_layout, **kw)
     stream = g.stream()
     src, dst = G.gen_rmat(g, skey(args.seed, STREAM_RMAT), args.scale, n_raw)
     st = g.bulkBuild(src, dst)

@@ -62,6 +62,7 @@ class GvConfig(ctypes.Structure):
         ("maintain_in_edges", ctypes.c_int),
         ("device", ctypes.c_int),
         ("pr_layout", ctypes.c_int),
+        ("bc_bu_ratio", ctypes.c_float),
     ]
 
 

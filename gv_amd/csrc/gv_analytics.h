@@ -274,26 +274,31 @@ gv_status bc_run_csr(const uint64_t *off, const uint32_t *cols, uint32_t V,
 gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double eps, int max_iter,
                   const uint32_t *seeds_dev, uint32_t ncols, float *out_dev, int *iters,
                   double *max_residual, double *iter_ms, hipStream_t s);
-// Brandes BC from host source list, 32 sources per batch.
 // Grow-only workspace of multi-source BC, kept with the graph handle so
 // repeated calls do not allocate (or free, which waits for the device).
+// n_* are capacities in elements.
 struct BcmWork {
-  uint32_t V = 0;
-  uint64_t items = 0;  // capacity of part, in items
-  int *dist = nullptr;
   double *sigma = nullptr;
-  float *delta = nullptr, *bcc = nullptr;
-  float2 *part = nullptr;
-  uint32_t *owner = nullptr;  // per item of a level: its frontier index (capacity items)
-  uint2 *sn = nullptr;  // per vertex: (seen, next) source masks of the forward sweep
-  uint32_t *queue = nullptr, *cnt = nullptr, *coff = nullptr, *dsrc = nullptr;
-  unsigned int *cursor = nullptr, *h_cursor = nullptr;
+  float *coef = nullptr, *bcc = nullptr;
+  void *sn = nullptr, *fl = nullptr;  // uint2 per vertex
+  uint32_t *queue = nullptr, *qmask = nullptr, *qpart = nullptr, *longl = nullptr;
+  uint32_t *dsrc = nullptr;
+  void *items = nullptr, *cand = nullptr, *part = nullptr;  // uint2, uint2, float2
+  void *lv = nullptr, *ctl = nullptr, *h_ctl = nullptr;
   unsigned long long *used_tot = nullptr;
-  void *tmp = nullptr;
+  uint64_t n_sigma = 0, n_coef = 0, n_bcc = 0, n_sn = 0, n_fl = 0, n_queue = 0, n_qmask = 0,
+           n_qpart = 0, n_items = 0, n_longl = 0, n_cand = 0, n_part = 0, n_dsrc = 0, n_ctl = 0,
+           n_used_tot = 0, lvcap = 0;
 };
 void bcm_release(BcmWork &w);
-gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
-                       float *out_dev, BcmWork &w, hipStream_t s);
+// Brandes BC from a host source list in sweeps of 16 (at most 16 sources) or
+// 32 sources.  out: adjacency; in: in-adjacency for bottom-up levels (the
+// same store for undirected graphs; null disables bottom-up); bu_ratio: a
+// level goes bottom-up when its candidates' in-slots are below bu_ratio
+// times its frontier's out-slots (0: always top-down).
+gv_status bc_multi_run(const Store &out, const Store *in, const uint32_t *sources,
+                       uint32_t n_sources, float *out_dev, BcmWork &w, float bu_ratio,
+                       hipStream_t s);
 
 // Request-loop hooks over a graph handle (gv_api.hip): the relabelled flat
 // PageRank snapshot of the handle's in-adjacency, and a CSR snapshot of its

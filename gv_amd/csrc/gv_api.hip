@@ -94,6 +94,7 @@ gv_status gv_config_default(gv_config *cfg) {
   cfg->maintain_in_edges = 1;
   cfg->device = 0;
   cfg->pr_layout = 5;
+  cfg->bc_bu_ratio = 3.0f;
   return GV_OK;
 }
 
@@ -459,6 +460,10 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
   }
   if (cfg.pr_layout < 0 || cfg.pr_layout > 5) {
     set_error("pr_layout must be 0, 1, 2, 3, 4 or 5");
+    return GV_E_INVALID;
+  }
+  if (!(cfg.bc_bu_ratio >= 0.f) || !std::isfinite(cfg.bc_bu_ratio)) {
+    set_error("bc_bu_ratio must be finite and >= 0");
     return GV_E_INVALID;
   }
   hipError_t e = hipSetDevice(cfg.device);
@@ -1131,7 +1136,11 @@ gv_status gv_bc(gv_graph *g, const uint32_t *sources, uint32_t n_sources, float 
     hipError_t e = hipMalloc(&dout, (size_t)(g->V ? g->V : 1) * 4);
     if (e != hipSuccess) return hip_fail(e, "bc alloc");
   }
-  rc = n_sources > 1 ? bc_multi_run(g->st[0], hs.data(), n_sources, dout, g->bcm, g->stream)
+  // bottom-up levels pull over the in-adjacency: the store itself when
+  // undirected, its mirror when directed (none without maintain_in_edges)
+  const Store *in = !g->directed ? &g->st[0] : (g->has_in ? &g->st[1] : nullptr);
+  rc = n_sources > 1 ? bc_multi_run(g->st[0], in, hs.data(), n_sources, dout, g->bcm,
+                                    g->cfg.bc_bu_ratio, g->stream)
                      : bc_run(g->st[0], hs.data(), n_sources, dout, g->stream);
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
@@ -1355,11 +1364,16 @@ gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg) {
     set_error("pr_layout must be 0, 1, 2, 3, 4 or 5");
     return GV_E_INVALID;
   }
+  if (!(cfg->bc_bu_ratio >= 0.f) || !std::isfinite(cfg->bc_bu_ratio)) {
+    set_error("bc_bu_ratio must be finite and >= 0");
+    return GV_E_INVALID;
+  }
   g->cfg.compact_threshold = cfg->compact_threshold;
   g->cfg.pr_thread_max = cfg->pr_thread_max;
   g->cfg.pr_group_max = cfg->pr_group_max;
   g->cfg.pr_wave_max = cfg->pr_wave_max;
   g->cfg.pr_layout = cfg->pr_layout;
+  g->cfg.bc_bu_ratio = cfg->bc_bu_ratio;
   return GV_OK;
 }
 

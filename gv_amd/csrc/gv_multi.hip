@@ -380,177 +380,414 @@ gv_status ppr_run(const Store &in, const uint32_t *deg_out, float alpha, double 
 }
 
 // ================================================================ BC ==
-constexpr int BCM_S = 32;   // sources per batch = lanes per frontier vertex
-constexpr int BCM_ILP = 8;  // neighbour rows requested before the first is used
+// Brandes BC from a sweep of up to 32 sources at once (bit-parallel
+// multi-source BFS, Then et al., VLDB 2014), one lane per source.
+//
+// Forward, level by level.  An expansion only marks claims: next[w] |= the
+// sources that reach w at level d + 1, and sigma.  Opening level d + 1 then
+// scans the vertices once: every vertex with a claim is appended to the
+// level queue with its source mask (qmask) and one work item per BCM_CH
+// slots of its adjacency, each workgroup reserving its range's entries with
+// one atomic per counter (appends from the expansion itself would put
+// millions of atomics on one cursor).  A level is expanded either top-down
+// (the frontier's items push sigma along their edges with FP64 atomics) or
+// bottom-up (Beamer, Asanovic, Patterson, SC 2012): the vertices still
+// unreached by a source that is active at level d pull sigma from their
+// level-d neighbours, with plain stores.  Both are launched for every level
+// and the one the level's slot counts do not pick exits at once, so the
+// level loop needs no host decision; the host checks for the end of the
+// sweep once per BCM_GROUP levels.
+//
+// Backward, deepest level first: each level-d vertex sums, over its
+// successors w (level d + 1 for that source), coef_w = (1 + delta_w) /
+// sigma_w; delta_v = sigma_v * sum, and coef_v is stored for the level
+// above.  Sources of the sweep with no level-d entry contribute nothing.
+constexpr int BCM_S = 32;                      // sources per sweep (at most)
+constexpr int BCM_ILP = 8;                     // neighbour rows requested before the first is used
+constexpr uint32_t BCM_CH = 256;               // adjacency slots per work item
+constexpr uint32_t BCM_MULTI = 0x80000000u;    // item flag: its vertex has several items
+constexpr uint32_t BCM_NOLEV = 0xFFFFFFFFu;    // no level yet
+constexpr int BCM_GROUP = 8;                   // forward levels per host check
+constexpr int BCM_THREADS = 256;
 
-// Per-vertex source masks of the forward sweep: seen (x) = sources that
-// reached the vertex at an earlier level, next (y) = sources reaching it at
-// the level being expanded.
-__global__ void k_bcm_init(int *dist, double *sigma, float *delta, uint2 *sn, uint32_t V) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)V * BCM_S;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    dist[i] = -1;
-    sigma[i] = 0.0;
-    delta[i] = 0.f;
-    if (i < V) sn[i] = make_uint2(0u, 0u);
-  }
-}
-
-// Sources of the batch enter level 0; a vertex holding several sources is
-// queued once.
-__global__ void k_bcm_src(const uint32_t *src, uint32_t ns, int *dist, double *sigma, uint2 *sn,
-                          uint32_t *queue, unsigned int *cursor) {
-  const uint32_t s = threadIdx.x;
-  if (s >= ns) return;
-  const uint32_t v = src[s];
-  dist[(uint64_t)v * BCM_S + s] = 0;
-  sigma[(uint64_t)v * BCM_S + s] = 1.0;
-  if (atomicOr(&sn[v].x, 1u << s) == 0u) queue[atomicAdd(cursor, 1u)] = v;
-}
-
-// After a level: the new frontier's level bits join its seen mask.
-__global__ void k_bcm_next(const uint32_t *queue, uint32_t qe, const unsigned int *cursor,
-                           uint2 *sn) {
-  const uint32_t n = *cursor;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t w = queue[qe + i];
-    const uint2 m = sn[w];
-    sn[w] = make_uint2(m.x | m.y, 0u);
-  }
-}
-
-// Work items: a frontier vertex with more than BCM_CH slots is split into
-// chunks of BCM_CH slots, each its own item, so a hub's adjacency is walked
-// by many lane groups at once.  cnt/coff hold the per-vertex item counts and
-// their exclusive scan (coff[nf] = items of the level).
-constexpr uint32_t BCM_CH = 256;
-
-__global__ void k_bcm_count(Store st, const uint32_t *q, uint32_t nf, uint32_t *cnt) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t u = st.used[q[i]];
-    cnt[i] = u > BCM_CH ? (u + BCM_CH - 1) / BCM_CH : 1u;
-  }
-}
-
-// owner[i] = frontier index j of item i (coff[j] <= i < coff[j + 1]), so a
-// wavefront finds its item's vertex with one load.
-__global__ void k_bcm_owner(const uint32_t *coff, uint32_t nf, uint32_t *owner) {
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nf;
-       j += (uint64_t)gridDim.x * blockDim.x)
-    for (uint32_t i = coff[j], e = coff[j + 1]; i < e; ++i) owner[i] = (uint32_t)j;
-}
-
-// One item per lane group: the group's vertex, its slot range [c0, c1), and
-// the lane's (dist, sigma) for source s.
-struct BcmItem {
-  uint32_t v, tbl, c0, c1;
-  int dv;
-  double sv;
+// Level counters (device).  qb/ib/lb: first queue entry, item and
+// multi-item entry of the level; pn: partial rows taken by the level's
+// multi-item vertices; amask: sources with a level-d vertex; ncand:
+// bottom-up items of the level's multi-item candidates; done: workgroups
+// of the level's open kernel that have finished; fslots/cslots: adjacency
+// slots of the level's vertices and of the bottom-up candidates.
+struct BcmLevel {
+  uint32_t qb, ib, lb, pn, amask, ncand, done, pad;
+  unsigned long long fslots, cslots;
 };
 
-__device__ __forceinline__ BcmItem bcm_item(const Store &st, const int *dist, const double *sigma,
-                                            const uint32_t *queue, uint32_t qs,
-                                            const uint32_t *coff, const uint32_t *owner,
-                                            uint64_t gi, bool have, uint32_t s) {
-  BcmItem it{0, 0, 0, 0, -2, 0.0};
-  if (have) {
-    const uint32_t j = owner[gi];
-    it.v = queue[qs + j];
-    const uint32_t us = st.used[it.v];
-    it.tbl = st.tbl_off[it.v];
-    it.c0 = ((uint32_t)gi - coff[j]) * BCM_CH;
-    it.c1 = it.c0 + BCM_CH < us ? it.c0 + BCM_CH : us;
-    it.dv = dist[(uint64_t)it.v * BCM_S + s];
-    it.sv = sigma[(uint64_t)it.v * BCM_S + s];
-  }
-  return it;
+// Append cursors (queue entries in the low 32 bits of qi and items in the
+// high 32 bits, so one atomic reserves both), the first empty level (the
+// depth) and the overflow flag (a queue, item or multi-item buffer was too
+// small: the sweep is rerun with larger buffers).
+struct BcmCtl {
+  unsigned long long qi;
+  uint32_t lcur, depth, overflow, pad;
+};
+
+struct BcmArgs {
+  Store out, in;       // adjacency (successors) and in-adjacency (predecessors)
+  double *sigma;       // [V][RS] shortest-path counts
+  float *coef;         // [V][RS] (1 + delta) / sigma of processed (vertex, source) pairs
+  float *bc, *bcc;     // result and its compensation
+  uint2 *sn;           // forward: (seen, next) masks; backward: (mask, level) of level d + 1
+  uint2 *fl;           // forward: (mask, level) of each vertex's latest level
+  uint32_t *queue, *qmask, *qpart;
+  uint2 *items;        // {queue entry, chunk | BCM_MULTI}
+  uint32_t *longl;     // queue entries of multi-item vertices
+  uint2 *cand;         // bottom-up items {vertex, chunk | BCM_MULTI}
+  float2 *part;        // [pn][RS] per-item partial sums of multi-item vertices
+  BcmLevel *lv;
+  BcmCtl *ctl;
+  const uint32_t *src;
+  uint32_t ns, V;
+  uint32_t qcap, icap, lcap;
+  float bu_ratio;      // bottom-up when cand slots < bu_ratio * frontier slots (0: never)
+};
+
+__device__ __forceinline__ uint32_t bcm_chunks(uint32_t used) {
+  return used > BCM_CH ? (used + BCM_CH - 1) / BCM_CH : 1u;
 }
 
-// Forward, one wavefront per item with its lanes over the item's slots
-// (bit-parallel multi-source BFS, Then et al., VLDB 2014): F = the sources
-// for which v is at level d (dist[v][s] == d); for a neighbour w, cand =
-// F & ~seen[w] are exactly the sources whose shortest-path DAG holds v -> w,
-// so sigma[w][s] += sigma[v][s] for each.  The first claim of a bit at this
-// level (atomicOr on next[w]) sets dist[w][s] = d + 1, and the claim that
-// makes next[w] non-zero queues w.  One 8-byte mask load per edge instead
-// of a 32-source distance row.
-__global__ __launch_bounds__(256) void k_bcm_fwd(Store st, int *dist, double *sigma, uint2 *sn,
-                                                 uint32_t *queue, uint32_t qs, uint32_t qe, int d,
-                                                 unsigned int *cursor, const uint32_t *coff,
-                                                 const uint32_t *owner) {
-  __shared__ double s_sig[4][BCM_S];
+__device__ __forceinline__ bool bcm_live(const BcmArgs &a) {
+  return a.ctl->overflow == 0u && a.ctl->depth == BCM_NOLEV;
+}
+
+__device__ __forceinline__ bool bcm_bottom_up(const BcmArgs &a, const BcmLevel &L) {
+  return a.bu_ratio > 0.f && (double)L.cslots < (double)a.bu_ratio * (double)L.fslots;
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int d = WAVE / 2; d >= 1; d >>= 1) v |= __shfl_xor(v, d, WAVE);
+  return v;
+}
+
+// Per sweep: sigma rows and masks cleared, level counters reset.
+template <int RS>
+__global__ void k_bcm_init(BcmArgs a, uint32_t lvcap) {
+  const uint64_t ns = (uint64_t)a.V * RS;
+  const uint64_t n = ns > lvcap ? ns : lvcap;
+  const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) {
+    if (i < ns) a.sigma[i] = 0.0;
+    if (i < a.V) {
+      a.sn[i] = make_uint2(0u, 0u);
+      a.fl[i] = make_uint2(0u, BCM_NOLEV);
+    }
+    if (i < lvcap) a.lv[i] = BcmLevel{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i == 0) *a.ctl = BcmCtl{0, 0, BCM_NOLEV, 0, 0};
+  }
+}
+
+// The sweep's sources claim level 0.  One wavefront.
+template <int RS>
+__global__ void k_bcm_src(BcmArgs a) {
+  const uint32_t s = lane_id();
+  if (s < a.ns) {
+    const uint32_t v = a.src[s];
+    a.sigma[(uint64_t)v * RS + s] = 1.0;
+    atomicOr(&a.sn[v].y, 1u << s);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T block_or(T v) {
+  __shared__ T wo[BCM_THREADS / WAVE];
+#pragma unroll
+  for (int d = WAVE / 2; d >= 1; d >>= 1) v |= __shfl_xor(v, d, WAVE);
+  if (lane_id() == 0) wo[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  T t = 0;
+#pragma unroll
+  for (int i = 0; i < BCM_THREADS / WAVE; ++i) t |= wo[i];
+  __syncthreads();
+  return t;
+}
+
+// Opens level d from the claims of the previous expansion (or of the
+// sources): each vertex with a non-empty next mask m is appended to the
+// level queue with its items, m joins its seen mask and is recorded per
+// queue entry (qmask) and per vertex (fl).  amask, fslots and the bottom-up
+// candidates of the level are collected on the way: a candidate has
+// in-adjacency and an unreached source among those active at level d - 1
+// (a superset of those active at level d, which the bottom-up kernel uses);
+// candidates with more than BCM_CH slots are listed as items.  Each
+// workgroup owns a contiguous vertex range: it counts, reserves with one
+// atomic per counter, then writes.  The last workgroup to finish fixes the
+// level's bounds and, if the level is empty, the depth.
+template <int RS>
+__global__ __launch_bounds__(BCM_THREADS) void k_bcm_open(BcmArgs a, uint32_t d) {
+  __shared__ uint32_t s_live, s_last, s_q, s_i, s_l, s_p, s_c;
+  if (threadIdx.x == 0)
+    s_live = a.ctl->depth == BCM_NOLEV && a.ctl->overflow == 0u;  // read once per workgroup
+  __syncthreads();
+  if (!s_live) return;
+  const uint32_t amp = d ? a.lv[d - 1].amask : (a.ns >= 32 ? 0xFFFFFFFFu : (1u << a.ns) - 1u);
+  const bool bu = a.bu_ratio > 0.f;
+  const uint32_t per = (uint32_t)(((uint64_t)a.V + gridDim.x - 1) / gridDim.x + BCM_THREADS - 1) /
+                       BCM_THREADS * BCM_THREADS;
+  const uint32_t v0 = (uint32_t)std::min<uint64_t>((uint64_t)blockIdx.x * per, a.V);
+  const uint32_t v1 = (uint32_t)std::min<uint64_t>((uint64_t)v0 + per, a.V);
+  // pass 1: counts
+  uint32_t nq = 0, ni = 0, nl = 0, np = 0, nc = 0, am = 0;
+  unsigned long long fs = 0, cs = 0;
+  for (uint32_t v = v0 + threadIdx.x; v < v1; v += BCM_THREADS) {
+    const uint2 x = a.sn[v];
+    if (x.y) {
+      const uint32_t used = a.out.used[v], k = bcm_chunks(used);
+      ++nq;
+      ni += k;
+      if (k > 1u) {
+        ++nl;
+        np += k;
+      }
+      am |= x.y;
+      fs += used;
+    }
+    if (bu) {
+      const uint32_t iu = a.in.used[v];
+      if (iu && (amp & ~(x.x | x.y))) {
+        cs += iu;
+        if (iu > BCM_CH) nc += bcm_chunks(iu);
+      }
+    }
+  }
+  nq = block_sum<uint32_t, BCM_THREADS>(nq);
+  ni = block_sum<uint32_t, BCM_THREADS>(ni);
+  nl = block_sum<uint32_t, BCM_THREADS>(nl);
+  np = block_sum<uint32_t, BCM_THREADS>(np);
+  nc = block_sum<uint32_t, BCM_THREADS>(nc);
+  am = block_or(am);
+  fs = block_sum<unsigned long long, BCM_THREADS>(fs);
+  cs = block_sum<unsigned long long, BCM_THREADS>(cs);
+  if (threadIdx.x == 0) {
+    uint32_t ok = 1;
+    if (nq) {
+      const unsigned long long qi =
+          atomicAdd(&a.ctl->qi, (unsigned long long)nq | ((unsigned long long)ni << 32));
+      s_q = (uint32_t)qi;
+      s_i = (uint32_t)(qi >> 32);
+      if ((uint64_t)s_q + nq > a.qcap || (uint64_t)s_i + ni > a.icap) ok = 0;
+      if (nl) {
+        s_l = atomicAdd(&a.ctl->lcur, nl);
+        s_p = atomicAdd(&a.lv[d].pn, np);
+        if ((uint64_t)s_l + nl > a.lcap) ok = 0;
+      }
+      atomicOr(&a.lv[d].amask, am);
+      atomicAdd(&a.lv[d].fslots, fs);
+    }
+    if (nc) s_c = atomicAdd(&a.lv[d].ncand, nc);
+    if (cs) atomicAdd(&a.lv[d].cslots, cs);
+    if (!ok) atomicOr(&a.ctl->overflow, 1u);
+    s_live = ok;
+  }
+  __syncthreads();
+  // pass 2: writes, positions from workgroup-local cursors in LDS
+  if (s_live && (nq || nc)) {
+    for (uint32_t b = v0; b < v1; b += BCM_THREADS) {
+      const uint32_t v = b + threadIdx.x;
+      uint2 x = make_uint2(0u, 0u);
+      uint32_t k = 0, kc = 0;
+      if (v < v1) {
+        x = a.sn[v];
+        if (x.y) k = bcm_chunks(a.out.used[v]);
+        if (bu) {
+          const uint32_t iu = a.in.used[v];
+          if (iu > BCM_CH && (amp & ~(x.x | x.y))) kc = bcm_chunks(iu);
+        }
+      }
+      const bool q = k != 0u, multi = k > 1u;
+      // wave-aggregated reservations from the workgroup's cursors
+      const unsigned long long mq = __ballot(q), mm = __ballot(multi);
+      const uint32_t kinc = wave_incl_scan<uint32_t>(k);
+      const uint32_t kminc = wave_incl_scan<uint32_t>(multi ? k : 0u);
+      const uint32_t kcinc = wave_incl_scan<uint32_t>(kc);
+      uint32_t bq = 0, bi = 0, bl = 0, bp = 0, bc = 0;
+      if (lane_id() == WAVE - 1) {
+        if (mq) {
+          bq = atomicAdd(&s_q, (uint32_t)__popcll(mq));
+          bi = atomicAdd(&s_i, kinc);
+        }
+        if (mm) {
+          bl = atomicAdd(&s_l, (uint32_t)__popcll(mm));
+          bp = atomicAdd(&s_p, kminc);
+        }
+        if (kcinc) bc = atomicAdd(&s_c, kcinc);
+      }
+      bq = __shfl(bq, WAVE - 1, WAVE);
+      bi = __shfl(bi, WAVE - 1, WAVE);
+      bl = __shfl(bl, WAVE - 1, WAVE);
+      bp = __shfl(bp, WAVE - 1, WAVE);
+      bc = __shfl(bc, WAVE - 1, WAVE);
+      if (q) {
+        const uint32_t j = bq + mask_lt_count(mq);
+        const uint32_t i0 = bi + kinc - k;
+        a.queue[j] = v;
+        a.qmask[j] = x.y;
+        a.qpart[j] = bp + kminc - (multi ? k : 0u);
+        const uint32_t fm = multi ? BCM_MULTI : 0u;
+        for (uint32_t c = 0; c < k; ++c) a.items[i0 + c] = make_uint2(j, c | fm);
+        if (multi) a.longl[bl + mask_lt_count(mm)] = j;
+        a.sn[v] = make_uint2(x.x | x.y, 0u);
+        a.fl[v] = make_uint2(x.y, d);
+      }
+      for (uint32_t c = 0; c < kc; ++c) a.cand[bc + kcinc - kc + c] = make_uint2(v, c | BCM_MULTI);
+    }
+  }
+  // The last workgroup fixes the level's bounds.  It reads only the append
+  // counters, through atomics, and every workgroup's reservations returned
+  // before its done increment was issued, so no fence is needed (the queue
+  // writes reach the next kernels at the kernel boundary); a device-scope
+  // fence per workgroup would write back the L2 each time.
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&a.lv[d].done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    const unsigned long long qi = atomicAdd(&a.ctl->qi, 0ull);
+    a.lv[d + 1].qb = (uint32_t)qi;
+    a.lv[d + 1].ib = (uint32_t)(qi >> 32);
+    a.lv[d + 1].lb = atomicAdd(&a.ctl->lcur, 0u);
+    if ((uint32_t)qi == a.lv[d].qb) a.ctl->depth = d;
+  }
+}
+
+// Top-down expansion of level d, one wavefront per item, lanes over its
+// slots: for a neighbour w, cand = F & ~seen[w] are exactly the sources for
+// which v -> w is a shortest-path DAG edge, so sigma[w][s] += sigma[v][s]
+// for each, and the sources join next[w].
+template <int RS>
+__global__ __launch_bounds__(BCM_THREADS) void k_bcm_td(BcmArgs a, uint32_t d) {
+  __shared__ double s_sig[BCM_THREADS / WAVE][RS];
+  if (!bcm_live(a)) return;
+  const BcmLevel L = a.lv[d];
+  if (bcm_bottom_up(a, L)) return;
   const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  const uint32_t nf = qe - qs;
-  const uint32_t nitems = coff[nf];
-  for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; it < nitems;
-       it += nwaves) {
-    const uint32_t j = owner[it];
-    const uint32_t v = queue[qs + j];
-    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
-    const uint32_t c0 = ((uint32_t)it - coff[j]) * BCM_CH;
+  const uint32_t ib = L.ib, ie = a.lv[d + 1].ib;
+  const uint32_t nw = gridDim.x * (BCM_THREADS / WAVE);
+  for (uint32_t i = ib + blockIdx.x * (BCM_THREADS / WAVE) + wid; i < ie; i += nw) {
+    const uint2 it = a.items[i];
+    const uint32_t v = a.queue[it.x], F = a.qmask[it.x];
+    const uint32_t us = a.out.used[v], tbl = a.out.tbl_off[v];
+    const uint32_t c0 = (it.y & ~BCM_MULTI) * BCM_CH;
     const uint32_t c1 = c0 + BCM_CH < us ? c0 + BCM_CH : us;
-    const int dv = lane < (uint32_t)BCM_S ? dist[(uint64_t)v * BCM_S + lane] : -1;
-    const uint32_t F = (uint32_t)__ballot(lane < (uint32_t)BCM_S && dv == d);
-    if (F == 0u) continue;
-    if (lane < (uint32_t)BCM_S) s_sig[wid][lane] = sigma[(uint64_t)v * BCM_S + lane];
+    if (lane < (uint32_t)RS) s_sig[wid][lane] = a.sigma[(uint64_t)v * RS + lane];
     wave_lds_sync();
     for (uint32_t base = c0; base < c1; base += WAVE) {
       const uint32_t pos = base + lane;
       const uint32_t w =
-          pos < c1 ? st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)] : TOMB;
-      bool enq = false;
+          pos < c1 ? a.out.slots[slot_index(a.out.arena, tbl, pos, a.out.log2B, a.out.B)] : TOMB;
       if (w != TOMB) {
-        const uint2 m = sn[w];
+        const uint2 m = a.sn[w];
         const uint32_t cand = F & ~m.x;
         if (cand) {
-          uint32_t claimed = 0u;
-          if (cand & ~m.y) {  // some bit not claimed yet at this level
-            const uint32_t old = atomicOr(&sn[w].y, cand);
-            claimed = cand & ~old;
-            enq = old == 0u;
-          }
-          for (uint32_t b = claimed; b; b &= b - 1)
-            dist[(uint64_t)w * BCM_S + (__ffs(b) - 1)] = d + 1;
+          if (cand & ~m.y) atomicOr(&a.sn[w].y, cand);
           for (uint32_t b = cand; b; b &= b - 1) {
             const uint32_t s = __ffs(b) - 1;
-            atomicAdd(&sigma[(uint64_t)w * BCM_S + s], s_sig[wid][s]);
+            atomicAdd(&a.sigma[(uint64_t)w * RS + s], s_sig[wid][s]);
           }
         }
-      }
-      const unsigned long long qm = __ballot(enq);
-      if (qm) {
-        uint32_t qb = 0;
-        if (lane == 0) qb = atomicAdd(cursor, (unsigned int)__popcll(qm));
-        qb = __shfl(qb, 0, WAVE);
-        if (enq) queue[qe + qb + mask_lt_count(qm)] = w;
       }
     }
     wave_lds_sync();
   }
 }
 
-// Backward level masks: next[w] (free after the forward sweep) = the sources
-// for which w is at level lev, for the queue entries [qs, qe) of that level
-// (set), or 0 again (clear).  Two entries per wavefront, lane = source.
-__global__ __launch_bounds__(256) void k_bcm_lmask(const uint32_t *queue, uint32_t qs,
-                                                   uint32_t qe, int lev, const int *dist,
-                                                   uint2 *sn, int set) {
+// Bottom-up expansion of level d, RS lanes (one per source) per candidate
+// item: lane s sums sigma[v][s] over the item's in-neighbours v at level d
+// for source s (fl[v] = (mask, d)), in neighbour order.  A vertex with one
+// item owns its row and writes it; items of a multi-item vertex add
+// atomically.
+template <int RS>
+__global__ __launch_bounds__(BCM_THREADS) void k_bcm_bu(BcmArgs a, uint32_t d) {
+  if (!bcm_live(a)) return;
+  const BcmLevel L = a.lv[d];
+  if (!bcm_bottom_up(a, L)) return;
   const uint32_t lane = lane_id();
-  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
-  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
-  const uint32_t n = qe - qs;
-  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
+  const uint32_t s = lane % RS, sub = lane / RS;
+  const uint32_t nl = L.ncand, n = nl + a.V;
+  const uint32_t per_wave = WAVE / RS;
+  const uint32_t ngroups = gridDim.x * (BCM_THREADS / WAVE) * per_wave;
+  for (uint32_t g0 = (blockIdx.x * (BCM_THREADS / WAVE) + threadIdx.x / WAVE) * per_wave;
        g0 < n; g0 += ngroups) {
-    const uint64_t j = g0 + sub;
-    const bool have = j < n;
-    const uint32_t w = have ? queue[qs + j] : 0u;
-    const bool bit = have && set && dist[(uint64_t)w * BCM_S + s] == lev;
-    const uint32_t m = (uint32_t)((__ballot(bit) >> (sub * BCM_S)) & 0xFFFFFFFFull);
-    if (have && s == 0) sn[w].y = m;
+    const uint32_t gi = g0 + sub;
+    bool have = false, multi = false;
+    uint32_t w = 0, U = 0, c0 = 0, c1 = 0, tbl = 0;
+    if (gi < nl) {  // an item of a listed multi-item candidate
+      const uint2 it = a.cand[gi];
+      w = it.x;
+      multi = have = true;
+      U = L.amask & ~a.sn[w].x;
+      const uint32_t us = a.in.used[w];
+      tbl = a.in.tbl_off[w];
+      c0 = (it.y & ~BCM_MULTI) * BCM_CH;
+      c1 = c0 + BCM_CH < us ? c0 + BCM_CH : us;
+    } else if (gi < n) {  // a vertex: a candidate if it has one item's worth of slots
+      w = gi - nl;
+      const uint32_t us = a.in.used[w];
+      if (us && us <= BCM_CH) {
+        U = L.amask & ~a.sn[w].x;
+        if (U) {
+          have = true;
+          tbl = a.in.tbl_off[w];
+          c1 = us;
+        }
+      }
+    }
+    const uint32_t span = c1 - c0;
+    uint32_t span_max = span;
+#pragma unroll
+    for (int x = RS; x < WAVE; x <<= 1) span_max = max(span_max, __shfl_xor(span_max, x, WAVE));
+    double acc = 0.0;
+    for (uint32_t base = 0; base < span_max; base += RS) {
+      uint32_t v = TOMB, m = 0;
+      if (base + s < span) {
+        v = a.in.slots[slot_index(a.in.arena, tbl, c0 + base + s, a.in.log2B, a.in.B)];
+        if (v != TOMB) {
+          const uint2 f = a.fl[v];
+          m = f.y == d ? (f.x & U) : 0u;
+        }
+      }
+      for (uint32_t t0 = 0; t0 < (uint32_t)RS; t0 += BCM_ILP) {
+        uint32_t vv[BCM_ILP];
+        bool hit[BCM_ILP];
+        double sg[BCM_ILP];
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) {
+          vv[k] = __shfl(v, (int)(t0 + k), RS);
+          hit[k] = (__shfl(m, (int)(t0 + k), RS) >> s) & 1u;
+        }
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) sg[k] = hit[k] ? a.sigma[(uint64_t)vv[k] * RS + s] : 0.0;
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) acc += sg[k];
+      }
+    }
+    const uint32_t got = (uint32_t)((__ballot(acc != 0.0) >> (sub * RS)) & ((RS == 32) ? 0xFFFFFFFFull : 0xFFFFull));
+    if (have && got) {
+      if (!multi) {
+        if (acc != 0.0) a.sigma[(uint64_t)w * RS + s] = acc;
+        if (s == 0) a.sn[w].y = got;
+      } else {
+        if (acc != 0.0) atomicAdd(&a.sigma[(uint64_t)w * RS + s], acc);
+        if (s == 0) atomicOr(&a.sn[w].y, got);
+      }
+    }
   }
+}
+
+// Backward level masks: for the queue entries of level t, sn[w] = (mask, t).
+template <int RS>
+__global__ __launch_bounds__(BCM_THREADS) void k_bcm_lmask(BcmArgs a, uint32_t t) {
+  const uint32_t qb = a.lv[t].qb, qe = a.lv[t + 1].qb;
+  const uint32_t st = gridDim.x * blockDim.x;
+  for (uint32_t j = qb + blockIdx.x * blockDim.x + threadIdx.x; j < qe; j += st)
+    a.sn[a.queue[j]] = make_uint2(a.qmask[j], t);
 }
 
 __device__ __forceinline__ void bcm_merge(float &s, float &c, float s2, float c2) {
@@ -561,104 +798,133 @@ __device__ __forceinline__ void bcm_merge(float &s, float &c, float s2, float c2
   c = c + c2 - e;
 }
 
-// Backward, per item: Kahan sum over the item's successors of
-// sigma_v/sigma_w (1 + delta_w), written as a (sum, compensation) partial.
-// Successors at level d + 1 are found through the level masks (next[w],
-// set by k_bcm_lmask): one 4-byte mask per neighbour, shuffled to the 32
-// source lanes, instead of a 32-source distance row.
-__global__ __launch_bounds__(256) void k_bcm_bwd(Store st, const int *dist, const double *sigma,
-                                                 const float *delta, const uint2 *sn,
-                                                 const uint32_t *queue, uint32_t qs, uint32_t qe,
-                                                 int d, const uint32_t *coff,
-                                                 const uint32_t *owner, float2 *part) {
-  const uint32_t lane = lane_id();
-  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
-  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
-  const uint32_t nf = qe - qs;
-  const uint32_t nitems = coff[nf];
-  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
-       g0 < nitems; g0 += ngroups) {
-    const uint64_t gi = g0 + sub;
-    const bool have = gi < nitems;
-    const BcmItem it = bcm_item(st, dist, sigma, queue, qs, coff, owner, gi, have, s);
-    const bool act = it.dv == d;
-    float ds = 0.f, dc = 0.f;
-    const uint32_t span = it.c1 - it.c0;
-    const uint32_t span_max = max(span, __shfl_xor(span, BCM_S, WAVE));
-    for (uint32_t base = 0; base < span_max; base += BCM_S) {
-      uint32_t slot = TOMB;
-      if (base + s < span)
-        slot = st.slots[slot_index(st.arena, it.tbl, it.c0 + base + s, st.log2B, st.B)];
-      const uint32_t lm = slot != TOMB ? sn[slot].y : 0u;  // this slot's level-(d+1) sources
-      for (uint32_t t0 = 0; t0 < (uint32_t)BCM_S; t0 += BCM_ILP) {
-        // rows of BCM_ILP neighbours in flight; sums stay in neighbour order
-        uint32_t w[BCM_ILP];
-        bool hit[BCM_ILP];
-        double sw[BCM_ILP];
-        float dw[BCM_ILP];
+// Finishes vertex v for the RS source lanes of its group: delta = sigma_v *
+// S (S = the compensated successor sum), coef for the level above, and the
+// dependencies of the sources that are not v itself added to BC[v] in a
+// fixed lane order.
+template <int RS>
+__device__ __forceinline__ void bcm_finish(const BcmArgs &a, bool have, uint32_t v, bool act,
+                                           double sv, float S, uint32_t s) {
+  float dl = 0.f;
+  if (have && act) {
+    dl = (float)(sv * (double)S);
+    a.coef[(uint64_t)v * RS + s] = (float)((1.0 + (double)dl) / sv);
+  }
+  const uint32_t my_src = s < a.ns ? a.src[s] : 0xFFFFFFFFu;
+  float bs = (have && act && v != my_src) ? dl : 0.f, bcomp = 0.f;
 #pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k) {
-          w[k] = __shfl(slot, (int)(t0 + k), BCM_S);
-          hit[k] = (__shfl(lm, (int)(t0 + k), BCM_S) >> s) & 1u;
-        }
-#pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k) hit[k] = hit[k] && act && w[k] != TOMB;
-#pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k) {
-          const uint64_t wi = (uint64_t)w[k] * BCM_S + s;
-          sw[k] = hit[k] ? sigma[wi] : 1.0;
-          dw[k] = hit[k] ? delta[wi] : 0.f;
-        }
-#pragma unroll
-        for (int k = 0; k < BCM_ILP; ++k)
-          if (hit[k]) kahan_add(ds, dc, (float)(it.sv / sw[k]) * (1.0f + dw[k]));
-      }
-    }
-    if (have) part[gi * BCM_S + s] = make_float2(ds, dc);
+  for (int k = RS / 2; k >= 1; k >>= 1) {
+    float s2 = __shfl_xor(bs, k, RS), c2 = __shfl_xor(bcomp, k, RS);
+    bcm_merge(bs, bcomp, s2, c2);
+  }
+  if (have && s == 0) {
+    float b = a.bc[v], bcv = a.bcc[v];
+    kahan_add(b, bcv, bs - bcomp);
+    a.bc[v] = b;
+    a.bcc[v] = bcv;
   }
 }
 
-// Per frontier vertex: merge its items' partials in chunk order, store
-// delta, then add the S dependencies to BC[v] in a fixed lane order.
-__global__ __launch_bounds__(256) void k_bcm_fin(const uint32_t *queue, uint32_t qs, uint32_t qe,
-                                                 int d, const int *dist, float *delta,
-                                                 const float2 *part, const uint32_t *coff,
-                                                 const uint32_t *src, uint32_t ns, float *bc,
-                                                 float *bcc) {
+// Backward of level d, RS lanes per item: Kahan sum over the item's
+// successors (sn[w] = (mask, d + 1)) of coef[w][s].  Single-item vertices
+// finish at once; multi-item vertices leave (sum, compensation) partials.
+template <int RS>
+__global__ __launch_bounds__(BCM_THREADS) void k_bcm_bwd(BcmArgs a, uint32_t d, float2 *part) {
   const uint32_t lane = lane_id();
-  const uint32_t s = lane % BCM_S, sub = lane / BCM_S;
-  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / BCM_S;
-  const uint32_t nf = qe - qs;
-  const uint32_t my_src = s < ns ? src[s] : 0xFFFFFFFFu;
-  for (uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE * (WAVE / BCM_S);
-       g0 < nf; g0 += ngroups) {
-    const uint64_t j = g0 + sub;
-    const bool have = j < nf;
+  const uint32_t s = lane % RS, sub = lane / RS;
+  const uint32_t ib = a.lv[d].ib, n = a.lv[d + 1].ib - ib;
+  const uint32_t per_wave = WAVE / RS;
+  const uint32_t ngroups = gridDim.x * (BCM_THREADS / WAVE) * per_wave;
+  for (uint32_t g0 = (blockIdx.x * (BCM_THREADS / WAVE) + threadIdx.x / WAVE) * per_wave;
+       g0 < n; g0 += ngroups) {
+    const uint32_t gi = g0 + sub;
+    const bool have = gi < n;
+    uint32_t v = 0, tbl = 0, c0 = 0, c1 = 0, j = 0, chunk = 0;
+    bool act = false, multi = false;
+    double sv = 0.0;
+    if (have) {
+      const uint2 it = a.items[ib + gi];
+      j = it.x;
+      chunk = it.y & ~BCM_MULTI;
+      multi = (it.y & BCM_MULTI) != 0u;
+      v = a.queue[j];
+      act = (a.qmask[j] >> s) & 1u;
+      if (act) sv = a.sigma[(uint64_t)v * RS + s];
+      const uint32_t us = a.out.used[v];
+      tbl = a.out.tbl_off[v];
+      c0 = chunk * BCM_CH;
+      c1 = c0 + BCM_CH < us ? c0 + BCM_CH : us;
+    }
+    const uint32_t span = c1 - c0;
+    uint32_t span_max = span;
+#pragma unroll
+    for (int x = RS; x < WAVE; x <<= 1) span_max = max(span_max, __shfl_xor(span_max, x, WAVE));
+    float ds = 0.f, dc = 0.f;
+    for (uint32_t base = 0; base < span_max; base += RS) {
+      uint32_t w = TOMB, lm = 0;
+      if (base + s < span) {
+        w = a.out.slots[slot_index(a.out.arena, tbl, c0 + base + s, a.out.log2B, a.out.B)];
+        if (w != TOMB) {
+          const uint2 b = a.sn[w];
+          lm = b.y == d + 1 ? b.x : 0u;
+        }
+      }
+      for (uint32_t t0 = 0; t0 < (uint32_t)RS; t0 += BCM_ILP) {
+        uint32_t ww[BCM_ILP];
+        bool hit[BCM_ILP];
+        float cw[BCM_ILP];
+        // every lane takes part in the shuffles (a lane left out of a
+        // cross-lane read gives 0 to the lanes reading it)
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) {
+          ww[k] = __shfl(w, (int)(t0 + k), RS);
+          hit[k] = (__shfl(lm, (int)(t0 + k), RS) >> s) & 1u;
+        }
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) hit[k] = hit[k] && act;
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k) cw[k] = hit[k] ? a.coef[(uint64_t)ww[k] * RS + s] : 0.f;
+#pragma unroll
+        for (int k = 0; k < BCM_ILP; ++k)
+          if (hit[k]) kahan_add(ds, dc, cw[k]);
+      }
+    }
+    if (have && multi) part[(uint64_t)(a.qpart[j] + chunk) * RS + s] = make_float2(ds, dc);
+    // groups of a wavefront finish together; multi-item groups pass have = false
+    bcm_finish<RS>(a, have && !multi, v, act, sv, ds - dc, s);
+  }
+}
+
+// Multi-item vertices of level d: merge their items' partials in chunk
+// order, then finish.
+template <int RS>
+__global__ __launch_bounds__(BCM_THREADS) void k_bcm_fin(BcmArgs a, uint32_t d,
+                                                         const float2 *part) {
+  const uint32_t lane = lane_id();
+  const uint32_t s = lane % RS, sub = lane / RS;
+  const uint32_t lb = a.lv[d].lb, n = a.lv[d + 1].lb - lb;
+  const uint32_t per_wave = WAVE / RS;
+  const uint32_t ngroups = gridDim.x * (BCM_THREADS / WAVE) * per_wave;
+  for (uint32_t g0 = (blockIdx.x * (BCM_THREADS / WAVE) + threadIdx.x / WAVE) * per_wave;
+       g0 < n; g0 += ngroups) {
+    const uint32_t gi = g0 + sub;
+    const bool have = gi < n;
     uint32_t v = 0;
     bool act = false;
+    double sv = 0.0;
     float ds = 0.f, dc = 0.f;
     if (have) {
-      v = queue[qs + j];
-      act = dist[(uint64_t)v * BCM_S + s] == d;
-      for (uint32_t i = coff[j]; i < coff[j + 1]; ++i) {
-        const float2 p = part[(uint64_t)i * BCM_S + s];
+      const uint32_t j = a.longl[lb + gi];
+      v = a.queue[j];
+      act = (a.qmask[j] >> s) & 1u;
+      if (act) sv = a.sigma[(uint64_t)v * RS + s];
+      const uint32_t k = bcm_chunks(a.out.used[v]), pb = a.qpart[j];
+      for (uint32_t c = 0; c < k; ++c) {
+        const float2 p = part[(uint64_t)(pb + c) * RS + s];
         bcm_merge(ds, dc, p.x, p.y);
       }
     }
-    const float dvl = act ? ds - dc : 0.f;
-    if (act) delta[(uint64_t)v * BCM_S + s] = dvl;
-    float bs = (act && v != my_src) ? dvl : 0.f, bcomp = 0.f;
-#pragma unroll
-    for (int k = BCM_S / 2; k >= 1; k >>= 1) {
-      float s2 = __shfl_xor(bs, k, BCM_S), c2 = __shfl_xor(bcomp, k, BCM_S);
-      bcm_merge(bs, bcomp, s2, c2);
-    }
-    if (have && s == 0) {
-      float b = bc[v], bcv = bcc[v];
-      kahan_add(b, bcv, bs - bcomp);
-      bc[v] = b;
-      bcc[v] = bcv;
-    }
+    bcm_finish<RS>(a, have, v, act, sv, ds - dc, s);
   }
 }
 
@@ -672,193 +938,212 @@ __global__ void k_used_sum(Store st, unsigned long long *tot) {
 }
 
 void bcm_release(BcmWork &w) {
-  hipFree(w.dist);
   hipFree(w.sigma);
-  hipFree(w.delta);
+  hipFree(w.coef);
   hipFree(w.bcc);
-  hipFree(w.part);
-  hipFree(w.owner);
   hipFree(w.sn);
+  hipFree(w.fl);
   hipFree(w.queue);
-  hipFree(w.cnt);
-  hipFree(w.coff);
-  hipFree(w.tmp);
+  hipFree(w.qmask);
+  hipFree(w.qpart);
+  hipFree(w.items);
+  hipFree(w.longl);
+  hipFree(w.cand);
+  hipFree(w.part);
+  hipFree(w.lv);
+  hipFree(w.ctl);
   hipFree(w.dsrc);
-  hipFree(w.cursor);
   hipFree(w.used_tot);
-  if (w.h_cursor) hipHostFree(w.h_cursor);
+  if (w.h_ctl) hipHostFree(w.h_ctl);
   w = BcmWork{};
 }
 
-// Per-vertex state for V vertices, level items for max_items.
-static gv_status bcm_reserve(BcmWork &w, uint32_t V, uint64_t max_items) {
-  hipError_t e = hipSuccess;
-  if (!w.used_tot) {
-    if ((e = hipMalloc(&w.used_tot, sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc(&w.dsrc, BCM_S * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&w.cursor, sizeof(unsigned int))) != hipSuccess ||
-        (e = hipHostMalloc(&w.h_cursor, sizeof(unsigned int), hipHostMallocDefault)) != hipSuccess)
-      return hip_fail(e, "bc alloc");
+// (Re)allocates buf for cap elements of sz bytes when it holds fewer.
+static hipError_t bcm_grow(void **buf, uint64_t &have, uint64_t cap, size_t sz) {
+  if (cap <= have && *buf) return hipSuccess;
+  hipFree(*buf);
+  *buf = nullptr;
+  have = 0;
+  hipError_t e = hipMalloc(buf, (size_t)(cap ? cap : 1) * sz);
+  if (e == hipSuccess) have = cap;
+  return e;
+}
+
+// Level counter array for at least n levels, keeping the first keep entries.
+static gv_status bcm_levels(BcmWork &w, uint64_t n, uint64_t keep, hipStream_t s) {
+  if (n <= w.lvcap) return GV_OK;
+  uint64_t cap = w.lvcap ? w.lvcap : 64;
+  while (cap < n) cap *= 2;
+  void *nb = nullptr;
+  hipError_t e = hipMalloc(&nb, cap * sizeof(BcmLevel));
+  if (e == hipSuccess && keep)
+    e = hipMemcpyAsync(nb, w.lv, keep * sizeof(BcmLevel), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) {
+    // counters of the levels not yet reached start at zero
+    static_assert(sizeof(BcmLevel) % 4 == 0, "level counters are cleared as 32-bit words");
+    fill_u32((uint32_t *)nb + keep * (sizeof(BcmLevel) / 4), 0u,
+             (cap - keep) * (sizeof(BcmLevel) / 4), s);
+    e = hipStreamSynchronize(s);
   }
-  if (V > w.V) {
-    hipFree(w.dist);
-    hipFree(w.sigma);
-    hipFree(w.delta);
-    hipFree(w.bcc);
-    hipFree(w.sn);
-    hipFree(w.queue);
-    hipFree(w.cnt);
-    hipFree(w.coff);
-    hipFree(w.tmp);
-    w.dist = nullptr;
-    w.sigma = nullptr;
-    w.delta = w.bcc = nullptr;
-    w.sn = nullptr;
-    w.queue = w.cnt = w.coff = nullptr;
-    w.tmp = nullptr;
-    w.V = 0;
-    const uint64_t n2 = (uint64_t)V * BCM_S;
-    if ((e = hipMalloc(&w.dist, n2 * sizeof(int))) != hipSuccess ||
-        (e = hipMalloc(&w.sigma, n2 * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc(&w.delta, n2 * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&w.bcc, (size_t)V * sizeof(float))) != hipSuccess ||
-        (e = hipMalloc(&w.sn, (size_t)V * sizeof(uint2))) != hipSuccess ||
-        (e = hipMalloc(&w.queue, n2 * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&w.cnt, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&w.coff, ((size_t)V + 1) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&w.tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess)
-      return hip_fail(e, "bc alloc");
-    w.V = V;
+  if (e != hipSuccess) {
+    hipFree(nb);
+    return hip_fail(e, "bc levels");
   }
-  if (max_items > w.items) {
-    hipFree(w.part);
-    hipFree(w.owner);
-    w.part = nullptr;
-    w.owner = nullptr;
-    w.items = 0;
-    const uint64_t cap = max_items + max_items / 4;  // room for graphs that grow
-    if ((e = hipMalloc(&w.part, cap * BCM_S * sizeof(float2))) != hipSuccess ||
-        (e = hipMalloc(&w.owner, cap * sizeof(uint32_t))) != hipSuccess)
-      return hip_fail(e, "bc alloc");
-    w.items = cap;
-  }
+  hipFree(w.lv);
+  w.lv = nb;
+  w.lvcap = cap;
   return GV_OK;
 }
 
-gv_status bc_multi_run(const Store &st, const uint32_t *sources, uint32_t n_sources,
-                       float *out_dev, BcmWork &w, hipStream_t s) {
-  const uint32_t V = st.V;
-  if (!V) return GV_OK;
-  unsigned long long h_used = 0;
+template <int RS>
+static void bcm_forward_group(const BcmArgs &a, uint32_t d0, hipStream_t s) {
+  const unsigned gv = grid_for(a.V, BCM_THREADS, 2048), ge = 2048;
+  for (uint32_t d = d0; d < d0 + BCM_GROUP; ++d) {
+    k_bcm_open<RS><<<gv, BCM_THREADS, 0, s>>>(a, d);
+    k_bcm_td<RS><<<ge, BCM_THREADS, 0, s>>>(a, d);
+    k_bcm_bu<RS><<<ge, BCM_THREADS, 0, s>>>(a, d);
+  }
+}
+
+template <int RS>
+static void bcm_backward(const BcmArgs &a, uint32_t D, hipStream_t s) {
+  const unsigned gq = 1024, ge = 2048;
+  // level 0 holds only the sweep's sources, whose own dependencies are not
+  // counted, so the sweep stops at level 1
+  for (uint32_t d = D; d-- > 1;) {
+    if (d + 1 < D) k_bcm_lmask<RS><<<gq, BCM_THREADS, 0, s>>>(a, d + 1);
+    k_bcm_bwd<RS><<<ge, BCM_THREADS, 0, s>>>(a, d, a.part);
+    k_bcm_fin<RS><<<gq, BCM_THREADS, 0, s>>>(a, d, a.part);
+  }
+}
+
+template <int RS>
+static gv_status bcm_sweeps(const Store &out, const Store *in, const uint32_t *sources,
+                            uint32_t n_sources, float *out_dev, BcmWork &w, float bu_ratio,
+                            uint64_t h_out, uint64_t h_in, hipStream_t s) {
+  const uint32_t V = out.V;
+  const uint64_t ch_out = h_out / BCM_CH + 1, ch_in = h_in / BCM_CH + 1;
+  // bounds: a vertex is queued at most once per level and at most RS times
+  // (once per source); per level its items are at most chunks of its slots
+  const uint64_t qmax = (uint64_t)V * RS, imax = (uint64_t)RS * (V + ch_out),
+                 lmax = (uint64_t)RS * ch_out;
+  if (imax + V + ch_out > 0xFFFFFFFFull) {
+    set_error("bc: graph too large for 32-bit work item positions");
+    return GV_E_INVALID;
+  }
+  hipError_t e = hipSuccess;
+  if ((e = bcm_grow((void **)&w.sigma, w.n_sigma, (uint64_t)V * RS, sizeof(double))) ||
+      (e = bcm_grow((void **)&w.coef, w.n_coef, (uint64_t)V * RS, sizeof(float))) ||
+      (e = bcm_grow((void **)&w.bcc, w.n_bcc, V, sizeof(float))) ||
+      (e = bcm_grow((void **)&w.sn, w.n_sn, V, sizeof(uint2))) ||
+      (e = bcm_grow((void **)&w.fl, w.n_fl, V, sizeof(uint2))) ||
+      (e = bcm_grow((void **)&w.part, w.n_part, (2 * ch_out + 2) * RS, sizeof(float2))) ||
+      (e = bcm_grow((void **)&w.cand, w.n_cand, in ? 2 * ch_in + 2 : 1, sizeof(uint2))) ||
+      (e = bcm_grow((void **)&w.dsrc, w.n_dsrc, BCM_S, sizeof(uint32_t))) ||
+      (e = bcm_grow((void **)&w.ctl, w.n_ctl, 1, sizeof(BcmCtl))))
+    return hip_fail(e, "bc alloc");
+  if (!w.h_ctl && (e = hipHostMalloc(&w.h_ctl, sizeof(BcmCtl), hipHostMallocDefault)))
+    return hip_fail(e, "bc alloc");
+  uint64_t qcap = std::min<uint64_t>(qmax, std::max<uint64_t>(w.n_queue, 4ull * V + 64));
+  uint64_t icap = std::min<uint64_t>(imax, std::max<uint64_t>(w.n_items, 4ull * (V + ch_out) + 64));
+  uint64_t lcap = std::min<uint64_t>(lmax, std::max<uint64_t>(w.n_longl, 2 * ch_out + 64));
   gv_status rc = GV_OK;
-  const uint64_t n2 = (uint64_t)V * BCM_S;
-  do {
-    hipError_t e = hipSuccess;
-    if ((rc = bcm_reserve(w, V, 0))) break;
-    fill_u64((uint64_t *)w.used_tot, 0, 1, s);
-    k_used_sum<<<grid_for(V, 256, 1024), 256, 0, s>>>(st, w.used_tot);
-    if ((e = hipMemcpyAsync(&h_used, w.used_tot, sizeof(h_used), hipMemcpyDeviceToHost, s)) !=
-            hipSuccess ||
-        (e = hipStreamSynchronize(s)) != hipSuccess) {
-      rc = hip_fail(e, "bc");
-      break;
+  BcmArgs a{};
+  a.out = out;
+  a.in = in ? *in : out;
+  a.bc = out_dev;
+  a.ns = 0;
+  a.V = V;
+  a.bu_ratio = in ? bu_ratio : 0.f;
+  fill_f32(out_dev, 0.f, V, s);
+  fill_f32(w.bcc, 0.f, V, s);
+  for (uint32_t b0 = 0; b0 < n_sources && rc == GV_OK;) {
+    const uint32_t ns = std::min<uint32_t>(RS, n_sources - b0);
+    if ((e = bcm_grow((void **)&w.queue, w.n_queue, qcap, 4)) ||
+        (e = bcm_grow((void **)&w.qmask, w.n_qmask, qcap, 4)) ||
+        (e = bcm_grow((void **)&w.qpart, w.n_qpart, qcap, 4)) ||
+        (e = bcm_grow((void **)&w.items, w.n_items, icap, sizeof(uint2))) ||
+        (e = bcm_grow((void **)&w.longl, w.n_longl, lcap, 4)))
+      return hip_fail(e, "bc alloc");
+    if ((rc = bcm_levels(w, BCM_GROUP + 2, 0, s))) return rc;
+    a.sigma = w.sigma;
+    a.coef = w.coef;
+    a.bcc = w.bcc;
+    a.sn = (uint2 *)w.sn;
+    a.fl = (uint2 *)w.fl;
+    a.queue = w.queue;
+    a.qmask = w.qmask;
+    a.qpart = w.qpart;
+    a.items = (uint2 *)w.items;
+    a.longl = w.longl;
+    a.cand = (uint2 *)w.cand;
+    a.part = (float2 *)w.part;
+    a.ctl = (BcmCtl *)w.ctl;
+    a.src = w.dsrc;
+    a.ns = ns;
+    a.qcap = (uint32_t)std::min<uint64_t>(w.n_queue, 0xFFFFFFFFull);
+    a.icap = (uint32_t)std::min<uint64_t>(w.n_items, 0xFFFFFFFFull);
+    a.lcap = (uint32_t)std::min<uint64_t>(w.n_longl, 0xFFFFFFFFull);
+    if ((e = hipMemcpyAsync(w.dsrc, sources + b0, ns * sizeof(uint32_t), hipMemcpyHostToDevice,
+                            s)))
+      return hip_fail(e, "bc");
+    a.lv = (BcmLevel *)w.lv;
+    {
+      const uint64_t n = std::max<uint64_t>((uint64_t)V * RS, w.lvcap);
+      k_bcm_init<RS><<<grid_for(n, 256, 16384), 256, 0, s>>>(a, (uint32_t)w.lvcap);
     }
-    // items of one level: at most one per frontier vertex plus one per BCM_CH slots
-    const uint64_t max_items = (uint64_t)V + h_used / BCM_CH + 1;
-    if (max_items > 0xFFFFFFFFull) {
-      set_error("bc: too many work items for 32-bit offsets");
-      rc = GV_E_INVALID;
-      break;
-    }
-    if ((rc = bcm_reserve(w, V, max_items))) break;
-    int *dist = w.dist;
-    double *sigma = w.sigma;
-    float *delta = w.delta, *bcc = w.bcc;
-    float2 *part = w.part;
-    uint2 *sn = w.sn;
-    uint32_t *queue = w.queue, *cnt = w.cnt, *coff = w.coff, *dsrc = w.dsrc, *owner = w.owner;
-    unsigned int *cursor = w.cursor, *h_cursor = w.h_cursor;
-    void *tmp = w.tmp;
-    // item offsets of the frontier segment [qs, qe); a level's frontier holds
-    // each vertex at most once, so nf <= V
-    auto level_items = [&](uint32_t qs, uint32_t qe) {
-      const uint32_t nf = qe - qs;
-      k_bcm_count<<<grid_for(nf), 256, 0, s>>>(st, queue + qs, nf, cnt);
-      scan_u32(cnt, coff, nf, nullptr, tmp, s);
-      k_bcm_owner<<<grid_for(nf), 256, 0, s>>>(coff, nf, owner);
-    };
-    auto item_grid = [&](uint32_t nf) {
-      const uint64_t bound = (uint64_t)nf + h_used / BCM_CH + 1;
-      return grid_for(bound * BCM_S, 256, 8192);
-    };
-    auto fwd_grid = [&](uint32_t nf) {  // one wavefront per item
-      const uint64_t bound = (uint64_t)nf + h_used / BCM_CH + 1;
-      return grid_for(bound * WAVE, 256, 8192);
-    };
-    fill_f32(out_dev, 0.f, V, s);
-    fill_f32(bcc, 0.f, V, s);
-    for (uint32_t b0 = 0; b0 < n_sources && rc == GV_OK; b0 += BCM_S) {
-      const uint32_t ns = std::min<uint32_t>(BCM_S, n_sources - b0);
-      if ((e = hipMemcpyAsync(dsrc, sources + b0, ns * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              s)) != hipSuccess) {
-        rc = hip_fail(e, "bc");
+    k_bcm_src<RS><<<1, WAVE, 0, s>>>(a);
+    uint32_t D = BCM_NOLEV;
+    bool overflow = false;
+    for (uint32_t d0 = 0; D == BCM_NOLEV; d0 += BCM_GROUP) {
+      if ((rc = bcm_levels(w, (uint64_t)d0 + BCM_GROUP + 2, d0 + 1, s))) return rc;
+      a.lv = (BcmLevel *)w.lv;
+      bcm_forward_group<RS>(a, d0, s);
+      if ((e = hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(BcmCtl), hipMemcpyDeviceToHost, s)) ||
+          (e = hipStreamSynchronize(s)))
+        return hip_fail(e, "bc");
+      const BcmCtl *hc = (const BcmCtl *)w.h_ctl;
+      if (hc->overflow) {
+        overflow = true;
         break;
       }
-      k_bcm_init<<<grid_for(n2, 256, 16384), 256, 0, s>>>(dist, sigma, delta, sn, V);
-      fill_u32(cursor, 0, 1, s);
-      k_bcm_src<<<1, BCM_S, 0, s>>>(dsrc, ns, dist, sigma, sn, queue, cursor);
-      if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
-              hipSuccess ||
-          (e = hipStreamSynchronize(s)) != hipSuccess) {
-        rc = hip_fail(e, "bc");
-        break;
-      }
-      std::vector<uint32_t> lvl{0u, *h_cursor};
-      int d = 0;
-      while (true) {
-        const uint32_t qs = lvl[d], qe = lvl[d + 1];
-        if (qe == qs) break;
-        fill_u32(cursor, 0, 1, s);
-        level_items(qs, qe);
-        k_bcm_fwd<<<fwd_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, sn, queue, qs, qe, d, cursor,
-                                                    coff, owner);
-        k_bcm_next<<<grid_for(qe - qs + h_used / BCM_CH + 1, 256, 4096), 256, 0, s>>>(queue, qe,
-                                                                                   cursor, sn);
-        if ((e = hipMemcpyAsync(h_cursor, cursor, sizeof(unsigned), hipMemcpyDeviceToHost, s)) !=
-                hipSuccess ||
-            (e = hipStreamSynchronize(s)) != hipSuccess) {
-          rc = hip_fail(e, "bc");
-          break;
-        }
-        lvl.push_back(qe + *h_cursor);
-        ++d;
-      }
-      if (rc) break;
-      // queue[lvl[k], lvl[k+1]) = union frontier of level k: every vertex
-      // that some source of the batch reaches at depth k (a vertex appears
-      // once per distinct depth).  The deepest level has
-      // no successors, so the backward sweep starts one level above it.
-      const int D = (int)lvl.size() - 2;
-      auto lmask = [&](int lev, int set) {
-        const uint32_t a = lvl[lev], b = lvl[lev + 1];
-        if (b > a)
-          k_bcm_lmask<<<grid_for((uint64_t)(b - a) * BCM_S, 256, 8192), 256, 0, s>>>(
-              queue, a, b, lev, dist, sn, set);
-      };
-      for (int lev = D - 2; lev >= 0; --lev) {
-        const uint32_t qs = lvl[lev], qe = lvl[lev + 1];
-        level_items(qs, qe);
-        lmask(lev + 1, 1);
-        k_bcm_bwd<<<item_grid(qe - qs), 256, 0, s>>>(st, dist, sigma, delta, sn, queue, qs, qe,
-                                                     lev, coff, owner, part);
-        lmask(lev + 1, 0);
-        k_bcm_fin<<<grid_for((uint64_t)(qe - qs) * BCM_S, 256, 8192), 256, 0, s>>>(
-            queue, qs, qe, lev, dist, delta, part, coff, dsrc, ns, out_dev, bcc);
-      }
+      D = hc->depth;
     }
-    if (rc == GV_OK && (hipStreamSynchronize(s) != hipSuccess)) rc = hip_fail(hipGetLastError(), "bc");
-  } while (0);
+    if (overflow) {  // rerun this sweep with larger buffers
+      if (qcap >= qmax && icap >= imax && lcap >= lmax) {
+        set_error("bc: work buffers at their bound overflowed");
+        return GV_E_HIP;
+      }
+      qcap = std::min(qmax, 2 * qcap);
+      icap = std::min(imax, 2 * icap);
+      lcap = std::min(lmax, 2 * lcap);
+      continue;
+    }
+    bcm_backward<RS>(a, D, s);
+    b0 += ns;
+  }
+  if (rc == GV_OK && (e = hipStreamSynchronize(s))) rc = hip_fail(e, "bc");
   return rc;
+}
+
+gv_status bc_multi_run(const Store &out, const Store *in, const uint32_t *sources,
+                       uint32_t n_sources, float *out_dev, BcmWork &w, float bu_ratio,
+                       hipStream_t s) {
+  const uint32_t V = out.V;
+  if (!V) return GV_OK;
+  hipError_t e = hipSuccess;
+  if ((e = bcm_grow((void **)&w.used_tot, w.n_used_tot, 2, sizeof(unsigned long long))))
+    return hip_fail(e, "bc alloc");
+  fill_u64((uint64_t *)w.used_tot, 0, 2, s);
+  k_used_sum<<<grid_for(V, 256, 1024), 256, 0, s>>>(out, w.used_tot);
+  if (in) k_used_sum<<<grid_for(V, 256, 1024), 256, 0, s>>>(*in, w.used_tot + 1);
+  unsigned long long h[2] = {0, 0};
+  if ((e = hipMemcpyAsync(h, w.used_tot, sizeof(h), hipMemcpyDeviceToHost, s)) ||
+      (e = hipStreamSynchronize(s)))
+    return hip_fail(e, "bc");
+  return n_sources <= 16
+             ? bcm_sweeps<16>(out, in, sources, n_sources, out_dev, w, bu_ratio, h[0], h[1], s)
+             : bcm_sweeps<32>(out, in, sources, n_sources, out_dev, w, bu_ratio, h[0], h[1], s);
 }
 
 }  // namespace gv

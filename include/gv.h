@@ -68,6 +68,11 @@ typedef struct gv_config {
                                  flat stream of equal tiles; 4 = the relabelled snapshot row by
                                  row; 3 = CSR snapshot in the original vertex order; 2 = pull
                                  over the block store; 0 = XCD-segmented relabelled snapshot */
+  float bc_bu_ratio;          /* multi-source BC: a BFS level is expanded bottom-up (unreached
+                                 vertices pull from the level over their in-adjacency) when
+                                 the candidates' in-slots are below this multiple of the
+                                 level's out-slots, else top-down; 0 = always top-down.
+                                 Directed graphs need maintain_in_edges for bottom-up. */
 } gv_config;
 
 /* Result of one update call. */
@@ -252,8 +257,8 @@ gv_status gv_read_matrix_market(const char *path, uint32_t *num_vertices, uint64
                                 uint32_t *src, uint32_t *dst, int *symmetric);
 
 /* Tuning.  gv_set_tuning changes, on a live handle, the fields that do not
- * move memory: compact_threshold, pr_thread_max, pr_group_max, pr_wave_max
- * and pr_layout (the other fields of cfg are ignored).  Used by the
+ * move memory: compact_threshold, pr_thread_max, pr_group_max, pr_wave_max,
+ * pr_layout and bc_bu_ratio (the other fields of cfg are ignored).  Used by the
  * autotuner; no reference counterpart (the reference's block size and
  * launch shapes are compile-time constants, constants.cuh:4-17). */
 gv_status gv_get_config(gv_graph *g, gv_config *cfg);

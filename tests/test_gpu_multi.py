@@ -71,18 +71,55 @@ def test_personalized_matches_global_when_averaged():
     g.close()
 
 
-@pytest.mark.parametrize("nsrc", [2, 33, 70])
-def test_multi_source_bc(nsrc):
+# bc_bu_ratio: 0 = every level top-down, 1e9 = bottom-up wherever the
+# in-adjacency allows (every level but the sources'), 1 = the default rule
+@pytest.mark.parametrize("ratio", [0.0, 1.0, 1e9])
+@pytest.mark.parametrize("nsrc", [2, 16, 33, 70])
+def test_multi_source_bc(nsrc, ratio):
     V = 1 << 12
     s, d = O.gen_rmat(O.stream_key(23, O.STREAM_RMAT), 12, 16 * V)
     for directed in (True, False):
-        g, o = build_pair(V, directed, s, d)
+        g, o = build_pair(V, directed, s, d, bc_bu_ratio=ratio)
         src = np.random.default_rng(nsrc).choice(V, nsrc, replace=False).astype(np.uint32)
         b = G.bc(g, src).astype(np.float64)
         bo = o.bc(src)
         err = np.abs(b - bo) / np.maximum(np.abs(bo), 1.0)
         assert err.max() <= 1e-5, err.max()
+        # the result does not depend on the direction chosen per level
+        b2 = G.bc(g, src).astype(np.float64)
+        assert np.array_equal(b, b2)
         g.close()
+
+
+@pytest.mark.parametrize("ratio", [0.0, 1e9])
+@pytest.mark.parametrize("directed", [True, False])
+def test_multi_source_bc_hub_items_both_directions(directed, ratio):
+    """Vertices with many work items in every role: a hub H reached at level
+    2 through 700 level-1 vertices (a bottom-up candidate with 700 in-slots,
+    three 256-slot items that add atomically), whose 1500 out-neighbours form
+    level 3 (a top-down vertex with six items and a backward vertex whose
+    partials are merged); sources at the hub's in-neighbours, the hub and a
+    far vertex; a chain hangs off level 3 so the sweep runs past 8 levels."""
+    n1, n3, chain = 700, 1500, 12
+    H = 1 + n1
+    V = H + 1 + n3 + chain + 5
+    s = [0] * n1 + list(range(1, n1 + 1))
+    d = list(range(1, n1 + 1)) + [H] * n1
+    lvl3 = list(range(H + 1, H + 1 + n3))
+    s += [H] * n3
+    d += lvl3
+    c0 = H + 1 + n3
+    s += [lvl3[0]] + list(range(c0, c0 + chain - 1))
+    d += list(range(c0, c0 + chain))
+    g, o = build_pair(V, directed, np.array(s, np.uint32), np.array(d, np.uint32),
+                      bc_bu_ratio=ratio)
+    for src in ([0], [0, 5], [0, 5, H, c0 + 3, V - 1]):
+        src = np.array(src + [0], np.uint32)  # at least two sources (the multi-source path)
+        b = G.bc(g, src).astype(np.float64)
+        bo = o.bc(src)
+        err = np.abs(b - bo) / np.maximum(np.abs(bo), 1.0)
+        assert err.max() <= 1e-5, (src, err.max())
+    g.close()
 
 
 def test_multi_source_bc_repeated_source_and_path():
