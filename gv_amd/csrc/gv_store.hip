@@ -391,6 +391,9 @@ constexpr int PROBE_ROUNDS = PROBE_CHUNK / WAVE;
 // An item this small is probed by one lane on its own.
 constexpr uint32_t PROBE_SMALL_SLOTS = 16;  // <= the smallest block (32 slots), multiple of 4
 constexpr uint32_t PROBE_SMALL_KEYS = 4;
+// A wavefront-wide item whose source has at most this many batch keys
+// compares each slot with every key.
+constexpr uint32_t PROBE_FEW_KEYS = 8;
 
 // Work items (PROBE_CHUNK slots of one source) are dealt to lanes: wave w of
 // W, step r, lane l takes item (r * 64 + l) * W + w, so the items of one hub
@@ -518,7 +521,22 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
       for (int q = 0; q < PROBE_ROUNDS; ++q)
         wv[q] = idx[q] != 0xFFFFFFFFu ? st.slots[idx[q]] : TOMB;
       uint32_t cnt = 0;
-      if (k <= (uint32_t)WAVE) {
+      if (k <= PROBE_FEW_KEYS) {
+        // few keys (most touched vertices): wave-uniform copies, one compare
+        // per key and slot instead of a search over the lanes
+        uint32_t kv[PROBE_FEW_KEYS];
+#pragma unroll
+        for (uint32_t j = 0; j < PROBE_FEW_KEYS; ++j)
+          kv[j] = __builtin_amdgcn_readlane(sv, (int)j);
+#pragma unroll
+        for (int q = 0; q < PROBE_ROUNDS; ++q) {
+          uint32_t p = PROBE_FEW_KEYS;
+#pragma unroll
+          for (uint32_t j = PROBE_FEW_KEYS; j-- > 0;)
+            if (kv[j] == wv[q]) p = j;
+          if (wv[q] != TOMB && p < k) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
+        }
+      } else if (k <= (uint32_t)WAVE) {
 #pragma unroll
         for (int q = 0; q < PROBE_ROUNDS; ++q) {
           const uint32_t p = wave_lower_bound(sv, wv[q]);
