@@ -554,7 +554,10 @@ __global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
           pb_fill_round<0>(img4, c4, nullptr, nh4);
           pb_fill_round<1>(img4, c4, nullptr, nh4);
         }
-        // the last partial group of hot labels, then the zero slot
+        // the last partial group of hot labels, then the zero slot; the
+        // vector rounds above rewrite that group's slots with the reloaded
+        // last whole group, so its labels are written after a barrier
+        if (a.nh & 3u) __syncthreads();
         for (uint32_t i = nh4 * 4 + threadIdx.x; i < a.nh; i += PB_THREADS) s_img[i] = a.cb[i];
         if (threadIdx.x == 0) s_img[PB_PAD] = 0.f;
         __syncthreads();
@@ -807,6 +810,9 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_expand(PbArgs a, uint32
       float4 *w4 = reinterpret_cast<float4 *>(s_win);
 #pragma unroll
       for (uint32_t j = 0; j < NW; ++j) w4[threadIdx.x + j * PB_BUILD_THREADS] = wv[j];
+      // a partial last group was written as zeros above: its labels go in
+      // after a barrier
+      if (ns & 3u) __syncthreads();
       for (uint32_t i = (ns & ~3u) + threadIdx.x; i < ns; i += PB_BUILD_THREADS)
         s_win[i] = a.cb[s0 + i];
       __syncthreads();
