@@ -51,8 +51,7 @@ def parse():
     p.add_argument("--pr-layout", type=int, default=5,
                    help="PageRank layout: 5 = degree-relabelled snapshot, hot stream + "
                         "propagation-blocked cold stream (default), 1 = the same snapshot as "
-                        "one flat stream, 3 = snapshot in vertex order, 2 = pull over the "
-                        "store, 0 = XCD-segmented snapshot")
+                        "one flat stream, 4 = the same snapshot row by row")
     p.add_argument("--bc-bu-ratio", type=float, default=None,
                    help="BC: bottom-up when a level's candidate in-slots are below this "
                         "multiple of its frontier's out-slots (default: the library's)")

@@ -313,11 +313,11 @@ class GraphVine:
 
     def set_tuning(self, **fields):
         """Change compact_threshold / pr_thread_max / pr_group_max /
-        pr_wave_max / pr_layout of the live handle."""
+        pr_wave_max / pr_layout / bc_bu_ratio of the live handle."""
         cfg = self.get_config()
         for k, v in fields.items():
             if k not in ("compact_threshold", "pr_thread_max", "pr_group_max", "pr_wave_max",
-                         "pr_layout"):
+                         "pr_layout", "bc_bu_ratio"):
                 raise ValueError(f"{k} is not a live tuning field")
             setattr(cfg, k, v)
         _check(lib().gv_set_tuning(self.handle, ctypes.byref(cfg)))

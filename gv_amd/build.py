@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgv.so")
 
-SOURCES = ["gv_prims.hip", "gv_store.hip", "gv_analytics.hip", "gv_pagerank_seg.hip", "gv_pagerank_flat.hip", "gv_pagerank_pb.hip", "gv_tc.hip",
+SOURCES = ["gv_prims.hip", "gv_store.hip", "gv_analytics.hip", "gv_pagerank_flat.hip", "gv_pagerank_pb.hip", "gv_tc.hip",
            "gv_multi.hip", "gv_api.hip", "gv_serve.hip", "gv_io.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -133,32 +133,7 @@ struct PrBins {
   uint32_t thread_max, group_max, wave_max;
 };
 
-// Workspace of the XCD-segmented PageRank snapshot (gv_pagerank_seg.hip).
-struct PsWork {
-  uint32_t V = 0;
-  uint64_t E_cap = 0, part_cap = 0;
-  uint32_t *src = nullptr;   // sources (relabelled), grouped by (segment, vertex)
-  uint32_t *off = nullptr;   // [8 * V + 1] offsets into src
-  uint32_t *list = nullptr;  // relabelled vertices grouped by degree bin
-  float *P = nullptr;        // [8 * V] per-(segment, vertex) partial sums
-  float *xp[2] = {nullptr, nullptr};
-  float *cp[2] = {nullptr, nullptr};
-  uint32_t *din = nullptr, *dout = nullptr;
-  uint32_t *heavy = nullptr;  // bit per relabelled vertex: handled by its own unit
-  double *partials = nullptr;
-  std::vector<hipEvent_t> ev;
-};
-
-// Bin helpers shared by both PageRank layouts.
 void pr_ctl_reset(PrCtl *ctl, hipStream_t s);
-void pr_bins_count(const uint32_t *deg, uint32_t V, const PrBins &b, PrCtl *ctl, hipStream_t s);
-void pr_bins_place(const uint32_t *deg, uint32_t V, const PrBins &b, uint32_t *list, PrCtl *ctl,
-                   hipStream_t s);
-void ps_release(PsWork &w);
-// PageRank on the XCD-segmented snapshot; same contract as pagerank_run.
-gv_status pagerank_seg_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
-                           const PrOpts &o, float *out_dev, int *iters, double *residual,
-                           PrWork &pw, PsWork &w, hipStream_t s, double *iter_ms = nullptr);
 
 gv_status pr_reserve(PrWork &w, uint32_t V);
 // CSR snapshot of the live in-adjacency (relabelled by descending out-degree
@@ -206,10 +181,9 @@ gv_status pagerank_snap_iterate(const PrOpts &o, PrWork &w, float *out_dev, int 
 // snapshot = 5: the propagation-blocked layout (falling back to 1);
 // snapshot = 1: iterate over a copy of the live in-adjacency built at the
 // start of the call, vertices relabelled by descending out-degree, as the
-// flat tiled stream (falling back to the per-row form of 3 when the graph
-// exceeds its 32-bit positions); 3: the same relabelled copy as CSR rows,
-// one row per work item; 2: CSR rows in the original vertex order; 0:
-// iterate over the block store.
+// flat tiled stream (falling back to the row form of 3 when the graph
+// exceeds its 32-bit positions); 3: the same relabelled copy as CSR rows
+// cut into degree bins (bins), one row or row chunk per work item.
 gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &bins,
                        const PrOpts &o, float *out_dev, int *iters, double *residual, PrWork &w,
                        hipStream_t s, double *iter_ms = nullptr, int snapshot = 1);

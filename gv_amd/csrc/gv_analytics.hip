@@ -214,13 +214,6 @@ void pr_init_vectors(float *x, float *cb, const uint32_t *deg_out, uint32_t V, f
 void pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out, hipStream_t s) {
   k_pr_unpermute<<<grid_for(V), 256, 0, s>>>(x, newid, V, out);
 }
-void pr_bins_count(const uint32_t *deg, uint32_t V, const PrBins &b, PrCtl *ctl, hipStream_t s) {
-  k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(deg, V, b, ctl);
-}
-void pr_bins_place(const uint32_t *deg, uint32_t V, const PrBins &b, uint32_t *list, PrCtl *ctl,
-                   hipStream_t s) {
-  k_pr_place<<<grid_for(V, 256, 512), 256, 0, s>>>(deg, V, b, list, ctl);
-}
 
 // Chunk lists of the vertices in the chunked bin.  Chunk c of hub h lives at
 // hub_base[h] + c, so partials combine in chunk order whatever the schedule.
@@ -861,6 +854,8 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     if (st != GV_E_UNSUPPORTED) return st;
     st = GV_OK;
   }
+  // the row form (also where the stream layouts do not apply: max_iter <= 0,
+  // or graphs past their 32-bit positions)
   const float x0 = 1.0f / (float)V;
   k_pr_reset<<<1, 1, 0, s>>>(w.ctl);
   if (max_iter <= 0) {
@@ -869,19 +864,15 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     GV_HIP(hipStreamSynchronize(s));
     return GV_OK;
   }
-  const bool relabel = snapshot == 1 || snapshot == 3;
-  if (snapshot) {
-    st = pr_snapshot(in, deg_out, relabel, w, s, 0);
-    if (st) return st;
-  }
-  // with a relabelled snapshot every per-vertex array below is in the new
-  // vertex order; the result is permuted back at the end
-  if (relabel) deg_out = w.rl_dout;
+  st = pr_snapshot(in, deg_out, true, w, s, 0);
+  if (st) return st;
+  // every per-vertex array below is in the relabelled vertex order; the
+  // result is permuted back at the end
+  deg_out = w.rl_dout;
   k_pr_init<<<grid_for(V), 256, 0, s>>>(w.x[0], w.contrib[0], deg_out, V, x0, o.init,
-                                        relabel ? w.rl_order : nullptr);
-  // row lengths the bins are cut by: live entries for the snapshot, slots
-  // (tombstones included) for the store
-  const uint32_t *rowlen = relabel ? w.rl_len : snapshot ? in.deg : in.used;
+                                        w.rl_order);
+  // row lengths the bins are cut by: live entries of the snapshot
+  const uint32_t *rowlen = w.rl_len;
   k_pr_count<<<grid_for(V, 256, 512), 256, 0, s>>>(rowlen, V, bins, w.ctl);
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
@@ -928,8 +919,8 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   a.ib1 = a.ib2 + n2;
   a.ib0 = a.ib1 + (uint32_t)cdiv(n1, WAVE / PR_GROUP);
   a.n_items = a.ib0 + (uint32_t)cdiv(n0, WAVE);
-  // staged form: one persistent workgroup per CU, relabelled snapshot only
-  const bool staged = relabel && V <= PRH_OOB / 4;
+  // staged form: one persistent workgroup per CU
+  const bool staged = V <= PRH_OOB / 4;
   const uint32_t hot_grid = (uint32_t)device_cus();
   const uint32_t grid = staged ? hot_grid : a.U ? a.U : 1;
   if (n3)
@@ -949,7 +940,6 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
       w.ev.push_back(e);
     }
   }
-  const StoreAdj sadj{in};
   const CsrAdj cadj{w.snap_off, w.snap_cols};
   int launched = 0;
   const int check_every = 8;
@@ -961,7 +951,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
     if (staged) {
       k_pr_hot<16, 8, 8><<<hot_grid, PRH_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
       k_pr_hotfin<<<1, 1024, 0, s>>>(a, cadj, x, xn, cn, hot_grid);
-    } else if (snapshot) {
+    } else {
       if (a.U3) {
         k_pr_iter<CsrAdj, 32, 3><<<a.U3, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
         k_pr_hubfin<CsrAdj><<<grid_for(a.n3), 256, 0, s>>>(a, cadj, x, xn, cn);
@@ -969,14 +959,6 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
       if (a.U2) k_pr_iter<CsrAdj, 32, 2><<<a.U2, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
       if (a.U1) k_pr_iter<CsrAdj, 16, 1><<<a.U1, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
       if (a.U0) k_pr_iter<CsrAdj, 8, 0><<<a.U0, PR_THREADS, 0, s>>>(a, cadj, x, cb, xn, cn);
-    } else {
-      if (a.U3) {
-        k_pr_iter<StoreAdj, 8, 3><<<a.U3, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
-        k_pr_hubfin<StoreAdj><<<grid_for(a.n3), 256, 0, s>>>(a, sadj, x, xn, cn);
-      }
-      if (a.U2) k_pr_iter<StoreAdj, 8, 2><<<a.U2, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
-      if (a.U1) k_pr_iter<StoreAdj, 8, 1><<<a.U1, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
-      if (a.U0) k_pr_iter<StoreAdj, 8, 0><<<a.U0, PR_THREADS, 0, s>>>(a, sadj, x, cb, xn, cn);
     }
     if (!staged) k_pr_resid<<<1, 1024, 0, s>>>(a);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[2 * t + 1], s));
@@ -990,11 +972,7 @@ gv_status pagerank_run(const Store &in, const uint32_t *deg_out, const PrBins &b
   GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
   GV_HIP(hipStreamSynchronize(s));
   const uint32_t it = w.h_ctl->iters;
-  if (relabel)
-    k_pr_unpermute<<<grid_for(V), 256, 0, s>>>(w.x[it & 1], w.rl_newid, V, out_dev);
-  else
-    GV_HIP(hipMemcpyAsync(out_dev, w.x[it & 1], (size_t)V * sizeof(float),
-                          hipMemcpyDeviceToDevice, s));
+  k_pr_unpermute<<<grid_for(V), 256, 0, s>>>(w.x[it & 1], w.rl_newid, V, out_dev);
   if (iters) *iters = (int)it;
   if (residual) *residual = w.h_ctl->residual;
   if (iter_ms) {

@@ -35,7 +35,6 @@ struct gv_graph {
   Scratch sc[2];
   uint32_t *vlist = nullptr;
   PrWork pr;
-  PsWork ps;
   TcdWork tcd;
   BcmWork bcm;
   Ctl *h_ctl = nullptr;  // pinned, 2 entries
@@ -458,8 +457,8 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
     set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
     return GV_E_INVALID;
   }
-  if (cfg.pr_layout < 0 || cfg.pr_layout > 5) {
-    set_error("pr_layout must be 0, 1, 2, 3, 4 or 5");
+  if (cfg.pr_layout != 1 && cfg.pr_layout != 4 && cfg.pr_layout != 5) {
+    set_error("pr_layout must be 1, 4 or 5");
     return GV_E_INVALID;
   }
   if (!(cfg.bc_bu_ratio >= 0.f) || !std::isfinite(cfg.bc_bu_ratio)) {
@@ -537,7 +536,6 @@ gv_status gv_destroy(gv_graph *g) {
     scratch_release(g->sc[k]);
   }
   pr_release(g->pr);
-  ps_release(g->ps);
   tcd_release(g->tcd);
   bcm_release(g->bcm);
   hipFree(g->vlist);
@@ -931,19 +929,8 @@ static gv_status pagerank_common(gv_graph *g, const gv_pagerank_opts *po, float 
       o.init = dinit;
     }
   }
-  if (g->cfg.pr_layout == 0) {
-    PrBins sbins{256, 256, 16384};
-    rc = pagerank_seg_run(in, g->st[0].deg, sbins, o, dout, iters, residual, g->pr, g->ps,
-                          g->stream, iter_ms);
-  } else {
-    rc = pagerank_run(in, g->st[0].deg, bins, o, dout, iters, residual, g->pr, g->stream,
-                      iter_ms,
-                      g->cfg.pr_layout == 5   ? 5
-                      : g->cfg.pr_layout == 1 ? 1
-                      : g->cfg.pr_layout == 3 ? 2
-                      : g->cfg.pr_layout == 4 ? 3
-                                              : 0);
-  }
+  rc = pagerank_run(in, g->st[0].deg, bins, o, dout, iters, residual, g->pr, g->stream, iter_ms,
+                    g->cfg.pr_layout == 4 ? 3 : g->cfg.pr_layout);
   if (!rc && mem != GV_MEM_DEVICE) {
     hipError_t e = hipMemcpyAsync(out, dout, (size_t)g->V * 4, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
@@ -1360,8 +1347,8 @@ gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg) {
     set_error("PageRank bins must satisfy thread_max <= group_max <= wave_max, wave_max > 0");
     return GV_E_INVALID;
   }
-  if (cfg->pr_layout < 0 || cfg->pr_layout > 5) {
-    set_error("pr_layout must be 0, 1, 2, 3, 4 or 5");
+  if (cfg->pr_layout != 1 && cfg->pr_layout != 4 && cfg->pr_layout != 5) {
+    set_error("pr_layout must be 1, 4 or 5");
     return GV_E_INVALID;
   }
   if (!(cfg->bc_bu_ratio >= 0.f) || !std::isfinite(cfg->bc_bu_ratio)) {

@@ -59,7 +59,7 @@ def run_chunglu(args):
     columns and BC from 256 sources (32 per pass) run on the result."""
     from . import (GV_MEM_DEVICE, GV_OP_INSERT, GraphVine, _check, default_config,
                    export_csr_device, gen_chung_lu, lib, stream_key)
-    from .autotune import tune
+    from .autotune import overrides, tune
     from .hip import DeviceArray
 
     V = int(args.cl_vertices)
@@ -100,10 +100,10 @@ def run_chunglu(args):
     tuned = None
     cfg_kw = {}
     if getattr(args, "autotune", 1):
-        tuned = tune(V, False, build, batched)
-        b = tuned["best"]
-        cfg_kw = {"block_slots": b["block_slots"], "pr_thread_max": b["pr_thread_max"],
-                  "pr_group_max": b["pr_group_max"], "pr_wave_max": b["pr_wave_max"]}
+        # BC timed from 32 sources (one sweep), PageRank on the updated graph
+        bsrc = np.random.default_rng(seed + 200).choice(V, 32, replace=False)
+        tuned = tune(V, False, build, batched, bc_sources=bsrc)
+        cfg_kw = overrides(tuned["best"])
 
     def fresh():
         h = GraphVine(V, directed=False, config=default_config(**cfg_kw))
@@ -197,6 +197,7 @@ def run_chunglu(args):
             "directed_edges_final": int(E_end),
             "updates_total": total,
             "block_slots": cfg_kw.get("block_slots", 64),
+            "tuned": cfg_kw or None,
         },
         "forms": forms,
         "speedup_batched_vs_plain": forms["batched"]["edge_updates_per_s"]

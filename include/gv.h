@@ -55,7 +55,8 @@ typedef struct gv_config {
   uint64_t pool_blocks;       /* edge blocks in the pool; 0 = sized from free HBM */
   double pool_fraction;       /* share of free HBM used when pool_blocks == 0 */
   float compact_threshold;    /* default tombstone fraction for gv_compact(g, -1) */
-  uint32_t pr_thread_max;     /* PageRank degree bins: <= this -> 1 lane per vertex */
+  uint32_t pr_thread_max;     /* degree bins of the PageRank row form (layout 4): a row with
+                                 <= this entries -> 1 lane per vertex */
   uint32_t pr_group_max;      /*   <= this -> 8 lanes per vertex */
   uint32_t pr_wave_max;       /*   <= this -> 1 wavefront; larger -> chunked */
   int maintain_in_edges;      /* directed graphs: keep the in-adjacency mirror */
@@ -65,9 +66,9 @@ typedef struct gv_config {
                                  a hot stream (sources among the highest out-degrees, staged in
                                  LDS) and a propagation-blocked cold stream (default; graphs
                                  beyond its limits use 1); 1 = the same snapshot iterated as one
-                                 flat stream of equal tiles; 4 = the relabelled snapshot row by
-                                 row; 3 = CSR snapshot in the original vertex order; 2 = pull
-                                 over the block store; 0 = XCD-segmented relabelled snapshot */
+                                 flat stream of equal tiles (graphs beyond its 32-bit positions
+                                 use 4); 4 = the relabelled snapshot row by row, rows cut into
+                                 the degree bins below */
   float bc_bu_ratio;          /* multi-source BC: a BFS level is expanded bottom-up (unreached
                                  vertices pull from the level over their in-adjacency) when
                                  the candidates' in-slots are below this multiple of the

@@ -71,7 +71,7 @@ def test_pagerank_closed_forms():
     assert it == 3 and abs(x[0] - xa) < 1e-7 and abs(x[1] - xb) < 1e-7
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("layout", [1, 4, 5])
 @pytest.mark.parametrize("directed", [True, False])
 def test_pagerank_config1_after_updates(directed, layout):
     V = 1 << 14
@@ -116,7 +116,7 @@ def test_pagerank_iteration_limits_and_repeated_calls():
         assert np.array_equal(x, first)
 
 
-@pytest.mark.parametrize("layout", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("layout", [1, 4, 5])
 def test_pagerank_all_degree_bins_and_chunked_hubs(layout):
     # small thresholds push vertices through every bin incl. chunked hubs
     V = 3000
@@ -201,17 +201,20 @@ def test_pagerank_propagation_blocked_many_blocks_and_bins(directed):
     g.close()
 
 
-@pytest.mark.parametrize("V", [5, 9, 20001, 70000])
-def test_pagerank_segmented_snapshot_bins_and_odd_sizes(V):
-    # hub with > 16384 in-edges (one workgroup per segment), vertex counts
-    # that are not powers of two (cycle-walked relabelling), tiny graphs
+@pytest.mark.parametrize("layout", [1, 4, 5])
+@pytest.mark.parametrize("V", [5, 9, 20001, 36863, 70000])
+def test_pagerank_odd_sizes_and_large_hub(V, layout):
+    # hub with > 16384 in-edges, vertex counts that are not powers of two,
+    # tiny graphs; for layout 5, 20001 leaves a partial last group of 4 in
+    # the LDS-staged hot labels and 36863 a partial last group in the last
+    # source block's expansion window (3 labels)
     rng = np.random.default_rng(V)
     s = rng.integers(0, V, 6 * V).astype(np.uint32)
     d = rng.integers(0, V, 6 * V).astype(np.uint32)
     s = np.concatenate([s, np.zeros(V - 1, np.uint32)])
     d = np.concatenate([d, np.arange(1, V, dtype=np.uint32)])
     for directed in (False, True):
-        g, o = build_pair(V, directed, s, d, pool=16 * V + 4096, pr_layout=0)
+        g, o = build_pair(V, directed, s, d, pool=16 * V + 4096, pr_layout=layout)
         check_pagerank(g, o)
         x1, it1, _ = G.static_pagerank(g, 0.85, 1e-6, 100)
         x2, it2, _ = G.static_pagerank(g, 0.85, 1e-6, 100)

@@ -198,8 +198,8 @@ def test_set_tuning_keeps_pagerank():
     g.bulkBuild(s, d)
     o = O.OracleGraph(V, False)
     o.bulk_build(s, d)
-    for tm, gm, wm, layout in [(1, 1, 64, 1), (32, 512, 100_000, 1), (8, 96, 4096, 0),
-                               (8, 96, 4096, 2), (1, 1, 64, 2), (8, 96, 4096, 1)]:
+    for tm, gm, wm, layout in [(1, 1, 64, 1), (32, 512, 100_000, 1), (8, 96, 4096, 4),
+                               (1, 1, 64, 4), (32, 512, 100_000, 4), (8, 96, 4096, 5)]:
         g.set_tuning(pr_thread_max=tm, pr_group_max=gm, pr_wave_max=wm, pr_layout=layout)
         cfg = g.get_config()
         assert (cfg.pr_thread_max, cfg.pr_group_max, cfg.pr_wave_max, cfg.pr_layout) == \
@@ -209,6 +209,15 @@ def test_set_tuning_keeps_pagerank():
         assert np.abs(x - xo).sum() <= 1e-6
     with pytest.raises(G.GraphVineError):
         g.set_tuning(pr_thread_max=100, pr_group_max=10)
+    for bad in (0, 2, 3, 6):
+        with pytest.raises(G.GraphVineError):
+            g.set_tuning(pr_layout=bad)
+    for r in (0.0, 2.5):
+        g.set_tuning(bc_bu_ratio=r)
+        assert g.get_config().bc_bu_ratio == r
+    for bad in (-1.0, float("inf"), float("nan")):
+        with pytest.raises(G.GraphVineError):
+            g.set_tuning(bc_bu_ratio=bad)
     with pytest.raises(ValueError):
         g.set_tuning(block_slots=32)
     g.close()

@@ -23,7 +23,7 @@ def build(scale, directed, seed):
     return g, V
 
 
-@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("layout", [1, 4, 5])
 def test_pagerank_rounding_and_convergence_error(layout):
     g, V = build(16, True, 71)
     g.set_tuning(pr_layout=layout)
