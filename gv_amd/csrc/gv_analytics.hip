@@ -250,28 +250,46 @@ __global__ void k_snap_expand(const uint32_t *istart, uint32_t V, uint32_t *iver
     for (uint32_t i = istart[v]; i < istart[v + 1]; ++i) ivert[i] = (uint32_t)v;
 }
 
-// One wavefront per item: live slots of the item's chunk.
+// Live slots of each item's chunk.  Items are dealt to lanes: wave w of W,
+// step r, lane l takes item (r * 64 + l) * W + w, so the items of one hub
+// spread over many wavefronts.  A lane settles its item alone when no slot
+// needs reading: the vertex has no tombstones (the chunk's slot count), or it
+// has one item (its live degree).  The chunks of hubs with tombstones are
+// then counted by the whole wavefront, one after another.
 __global__ __launch_bounds__(256) void k_snap_count(Store st, const uint32_t *istart,
                                                     const uint32_t *ivert, uint64_t *ipos) {
   const uint32_t ni = istart[st.V];
   const uint32_t lane = lane_id();
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
-       i += nwaves) {
-    const uint32_t v = ivert[i];
-    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
-    const uint32_t lo = ((uint32_t)i - istart[v]) * SNAP_CH;
-    const uint32_t hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
-    uint32_t cnt = hi - lo;
-    if (us != st.deg[v]) {
-      cnt = 0;
-      for (uint32_t p = lo; p < hi; p += WAVE) {
-        const bool live = p + lane < hi &&
-                          st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)] != TOMB;
-        cnt += (uint32_t)__popcll(__ballot(live));
+  const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  for (uint64_t r = 0; r * WAVE * W + w < ni; ++r) {
+    const uint64_t i = (r * WAVE + lane) * W + w;
+    uint32_t lo = 0, hi = 0, tbl = 0;
+    bool scan = false;
+    if (i < ni) {
+      const uint32_t v = ivert[i];
+      const uint32_t us = st.used[v], ndeg = st.deg[v];
+      lo = ((uint32_t)i - istart[v]) * SNAP_CH;
+      hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
+      if (us == ndeg) ipos[i] = hi - lo;
+      else if (us <= SNAP_CH) ipos[i] = ndeg;  // the vertex's only item
+      else {
+        scan = true;
+        tbl = st.tbl_off[v];
       }
     }
-    if (lane == 0) ipos[i] = cnt;
+    for (unsigned long long m = __ballot(scan); m; m &= m - 1) {
+      const int j = __ffsll(m) - 1;
+      const uint32_t jlo = __builtin_amdgcn_readlane(lo, j), jhi = __builtin_amdgcn_readlane(hi, j);
+      const uint32_t jtbl = __builtin_amdgcn_readlane(tbl, j);
+      uint32_t cnt = 0;
+      for (uint32_t p = jlo; p < jhi; p += WAVE) {
+        const bool live = p + lane < jhi &&
+                          st.slots[slot_index(st.arena, jtbl, p + lane, st.log2B, st.B)] != TOMB;
+        cnt += (uint32_t)__popcll(__ballot(live));
+      }
+      if (lane == (uint32_t)j) ipos[i] = cnt;
+    }
   }
 }
 
