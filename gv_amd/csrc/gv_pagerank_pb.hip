@@ -106,8 +106,15 @@ __global__ void k_pb_rowlen(const uint32_t *istart, const uint32_t *hscan, const
   }
 }
 
-// Splits each item's entries into the hot stream (label and flag, tiled) and
-// the cold source list (label, flag in bit 31, stream order).
+// Both streams from the snapshot: each entry of an item (a row chunk) goes to
+// the hot stream when its source label is below PB_H, else to the cold
+// source list, at the positions its row and the hot counts before it give.
+// Items are dealt to lanes (wave w of W, step r, lane l takes item
+// (r * 64 + l) * W + w); an item of at most PB_SPLIT_LANE entries is split by
+// its lane alone, four entries per load round, the longer ones by the whole
+// wavefront, one after another.
+constexpr uint32_t PB_SPLIT_LANE = 64;
+
 __global__ __launch_bounds__(256) void k_pb_split(const uint32_t *istart, const uint32_t *ivert,
                                                   const uint64_t *ipos, uint32_t V,
                                                   const uint32_t *newid, const uint64_t *off,
@@ -116,34 +123,67 @@ __global__ __launch_bounds__(256) void k_pb_split(const uint32_t *istart, const 
                                                   uint16_t *hot, uint32_t *csrc) {
   const uint32_t ni = istart[V];
   const uint32_t lane = lane_id();
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
-       i += nwaves) {
-    const uint32_t v = ivert[i];
-    const uint32_t r = newid[v];
-    const uint64_t len = ipos[i + 1] - ipos[i];
-    const uint64_t inrow = ipos[i] - ipos[istart[v]];
-    const uint64_t base = off[r] + inrow;
-    const uint32_t hin = hscan[i] - hscan[istart[v]];
-    uint32_t hb = hoff[r] + hin;
-    uint32_t cb = coff[r] + (uint32_t)inrow - hin;
-    const uint32_t h0 = hoff[r], c0 = coff[r];
-    for (uint64_t p = 0; p < len; p += WAVE) {
-      const bool valid = p + lane < len;
-      const uint32_t e = valid ? cols[base + p + lane] : 0u;
-      const bool ishot = valid && e < PB_H;
-      const bool iscold = valid && !ishot;
-      const unsigned long long mh = __ballot(ishot), mc = __ballot(iscold);
-      if (ishot) {
-        const uint32_t q = hb + mask_lt_count(mh);
-        hot[pb_phys(q)] = (uint16_t)(e | (q == h0 ? PB_F16 : 0u));
+  const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  for (uint64_t r = 0; r * WAVE * W + w < ni; ++r) {
+    const uint64_t i = (r * WAVE + lane) * W + w;
+    uint32_t len = 0, hb = 0, cb = 0, h0 = 0, c0 = 0;
+    uint64_t base = 0;
+    if (i < ni) {
+      const uint32_t v = ivert[i];
+      const uint32_t rw = newid[v];
+      const uint64_t rs = ipos[istart[v]];
+      len = (uint32_t)(ipos[i + 1] - ipos[i]);
+      const uint64_t inrow = ipos[i] - rs;
+      base = off[rw] + inrow;
+      const uint32_t hin = hscan[i] - hscan[istart[v]];
+      h0 = hoff[rw];
+      c0 = coff[rw];
+      hb = h0 + hin;
+      cb = c0 + (uint32_t)inrow - hin;
+    }
+    if (len <= PB_SPLIT_LANE) {
+      for (uint32_t p = 0; p < len; p += 4) {
+        uint32_t e[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) e[k] = p + k < len ? cols[base + p + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          if (p + k >= len) break;
+          if (e[k] < PB_H) {
+            hot[pb_phys(hb)] = (uint16_t)(e[k] | (hb == h0 ? PB_F16 : 0u));
+            ++hb;
+          } else {
+            csrc[cb] = e[k] | (cb == c0 ? PB_CSRC_FLAG : 0u);
+            ++cb;
+          }
+        }
       }
-      if (iscold) {
-        const uint32_t q = cb + mask_lt_count(mc);
-        csrc[q] = e | (q == c0 ? PB_CSRC_FLAG : 0u);
+    }
+    for (unsigned long long m = __ballot(len > PB_SPLIT_LANE); m; m &= m - 1) {
+      const int j = __ffsll(m) - 1;
+      const uint32_t jlen = __builtin_amdgcn_readlane(len, j);
+      const uint64_t jbase = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(base >> 32), j) << 32) |
+                             __builtin_amdgcn_readlane((uint32_t)base, j);
+      uint32_t jhb = __builtin_amdgcn_readlane(hb, j), jcb = __builtin_amdgcn_readlane(cb, j);
+      const uint32_t jh0 = __builtin_amdgcn_readlane(h0, j), jc0 = __builtin_amdgcn_readlane(c0, j);
+      for (uint32_t p = 0; p < jlen; p += WAVE) {
+        const bool valid = p + lane < jlen;
+        const uint32_t e = valid ? cols[jbase + p + lane] : 0u;
+        const bool ishot = valid && e < PB_H;
+        const bool iscold = valid && !ishot;
+        const unsigned long long mh = __ballot(ishot), mc = __ballot(iscold);
+        if (ishot) {
+          const uint32_t q = jhb + mask_lt_count(mh);
+          hot[pb_phys(q)] = (uint16_t)(e | (q == jh0 ? PB_F16 : 0u));
+        }
+        if (iscold) {
+          const uint32_t q = jcb + mask_lt_count(mc);
+          csrc[q] = e | (q == jc0 ? PB_CSRC_FLAG : 0u);
+        }
+        jhb += (uint32_t)__popcll(mh);
+        jcb += (uint32_t)__popcll(mc);
       }
-      hb += (uint32_t)__popcll(mh);
-      cb += (uint32_t)__popcll(mc);
     }
   }
 }
