@@ -300,14 +300,26 @@ __global__ void k_snap_off(const uint32_t *istart, uint32_t V, const uint64_t *i
     off[v] = ipos[istart[v]];
 }
 
-// One wavefront per item: live slots packed, in slot order, at the item's
-// output position.  With a relabelling (newid non-null) the row of old vertex
-// v is row newid[v] of the snapshot (offsets off) and every entry w is
-// written as newid[w]; the entry order within the row is kept.  flat != 0
-// writes the tiled stream layout of the flat PageRank iteration instead:
-// entry p of the row-major stream is stored at pf_phys(p) and the first
-// entry of every row carries PF_FLAG.
-// hitem (may be null): per item, the entries it wrote below hot_below.
+// Live slots of each item packed, in slot order, at the item's output
+// position.  With a relabelling (newid non-null) the row of old vertex v is
+// row newid[v] of the snapshot (offsets off) and every entry w is written as
+// newid[w]; the entry order within the row is kept.  flat != 0 writes the
+// tiled stream layout of the flat PageRank iteration instead: entry p of the
+// row-major stream is stored at pf_phys(p) and the first entry of every row
+// carries PF_FLAG.  hitem (may be null): per item, the entries it wrote below
+// hot_below.
+// Items are dealt to lanes (wave w of W, step r, lane l takes item
+// (r * 64 + l) * W + w); an item of at most SNAP_LANE slots is written by its
+// lane alone, four slots (and their new labels) in flight at a time, the
+// longer ones by the whole wavefront, one after another.
+constexpr uint32_t SNAP_LANE = 64;
+
+__device__ __forceinline__ void snap_put(uint32_t *cols, uint32_t flat, uint64_t q, uint64_t row0,
+                                         uint32_t val) {
+  if (flat) cols[pf_phys(q)] = val | (q == row0 ? PF_FLAG : 0u);
+  else cols[q] = val;
+}
+
 __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *istart,
                                                     const uint32_t *ivert, const uint64_t *ipos,
                                                     const uint32_t *newid, const uint64_t *off,
@@ -315,33 +327,68 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
                                                     uint32_t *hitem, uint32_t hot_below) {
   const uint32_t ni = istart[st.V];
   const uint32_t lane = lane_id();
-  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < ni;
-       i += nwaves) {
-    const uint32_t v = ivert[i];
-    const uint32_t us = st.used[v], tbl = st.tbl_off[v];
-    const uint32_t lo = ((uint32_t)i - istart[v]) * SNAP_CH;
-    const uint32_t hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
-    uint64_t dst = ipos[i];
-    const uint64_t row0 = newid ? off[newid[v]] : ipos[istart[v]];
-    if (newid) dst = row0 + (dst - ipos[istart[v]]);
-    uint32_t hot = 0;
-    for (uint32_t p = lo; p < hi; p += WAVE) {
-      uint32_t w = TOMB;
-      if (p + lane < hi) w = st.slots[slot_index(st.arena, tbl, p + lane, st.log2B, st.B)];
-      const bool live = w != TOMB;
-      const unsigned long long m = __ballot(live);
-      uint32_t val = 0;
-      if (live) {
-        const uint64_t q = dst + mask_lt_count(m);
-        val = newid ? newid[w] : w;
-        if (flat) cols[pf_phys(q)] = val | (q == row0 ? PF_FLAG : 0u);
-        else cols[q] = val;
-      }
-      if (hitem) hot += (uint32_t)__popcll(__ballot(live && val < hot_below));
-      dst += (uint64_t)__popcll(m);
+  const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  for (uint64_t r = 0; r * WAVE * W + w < ni; ++r) {
+    const uint64_t i = (r * WAVE + lane) * W + w;
+    uint32_t lo = 0, hi = 0, tbl = 0;
+    uint64_t dst = 0, row0 = 0;
+    if (i < ni) {
+      const uint32_t v = ivert[i];
+      const uint32_t us = st.used[v];
+      tbl = st.tbl_off[v];
+      lo = ((uint32_t)i - istart[v]) * SNAP_CH;
+      hi = lo + SNAP_CH < us ? lo + SNAP_CH : us;
+      dst = ipos[i];
+      const uint64_t rs = ipos[istart[v]];
+      row0 = newid ? off[newid[v]] : rs;
+      if (newid) dst = row0 + (dst - rs);
     }
-    if (hitem && lane == 0) hitem[i] = hot;
+    if (hi - lo <= SNAP_LANE) {
+      uint32_t hot = 0;
+      for (uint32_t p = lo; p < hi; p += 4) {
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+          x[k] = p + k < hi ? st.slots[slot_index(st.arena, tbl, p + k, st.log2B, st.B)] : TOMB;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+          if (newid && x[k] != TOMB) x[k] = newid[x[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          if (x[k] == TOMB) continue;
+          snap_put(cols, flat, dst, row0, x[k]);
+          ++dst;
+          hot += x[k] < hot_below ? 1u : 0u;
+        }
+      }
+      if (hitem && i < ni) hitem[i] = hot;
+    }
+    for (unsigned long long m = __ballot(hi - lo > SNAP_LANE); m; m &= m - 1) {
+      const int j = __ffsll(m) - 1;
+      const uint32_t jlo = __builtin_amdgcn_readlane(lo, j), jhi = __builtin_amdgcn_readlane(hi, j);
+      const uint32_t jtbl = __builtin_amdgcn_readlane(tbl, j);
+      uint64_t jdst = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(dst >> 32), j) << 32) |
+                      __builtin_amdgcn_readlane((uint32_t)dst, j);
+      const uint64_t jrow0 =
+          ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(row0 >> 32), j) << 32) |
+          __builtin_amdgcn_readlane((uint32_t)row0, j);
+      uint32_t hot = 0;
+      for (uint32_t p = jlo; p < jhi; p += WAVE) {
+        uint32_t x = TOMB;
+        if (p + lane < jhi) x = st.slots[slot_index(st.arena, jtbl, p + lane, st.log2B, st.B)];
+        const bool live = x != TOMB;
+        const unsigned long long lm = __ballot(live);
+        uint32_t val = 0;
+        if (live) {
+          val = newid ? newid[x] : x;
+          snap_put(cols, flat, jdst + mask_lt_count(lm), jrow0, val);
+        }
+        if (hitem) hot += (uint32_t)__popcll(__ballot(live && val < hot_below));
+        jdst += (uint64_t)__popcll(lm);
+      }
+      if (hitem && lane == (uint32_t)j) hitem[i] = hot;
+    }
   }
 }
 
