@@ -291,6 +291,7 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
     uint16_t *cold) {
   __shared__ uint32_t lpre[PB_MAX_NS];
   __shared__ uint32_t ctr[PB_MAX_NS];
+  __shared__ uint32_t sbase[PB_MAX_NS];  // segment (k, b)'s first slot
   const uint32_t b = blockIdx.x;
   uint32_t carry = 0;
   const uint32_t g0 = gbase[b];
@@ -304,6 +305,7 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
       lpre[k] = ex;
       ctr[k] = 0;
       const uint32_t s0 = seg[at];
+      sbase[k] = s0;
       for (uint32_t g = 0; g < c / 4; ++g) gaddr[g0 + ex / 4 + g] = s0 / 4 + g;
     }
     carry += tot;
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
     if (src != PB_CSRC_PAD) {
       const uint32_t k = (src - PB_H) / PB_SB;
       const uint32_t pos = atomicAdd(&ctr[k], 1u);
-      lidx[seg[(uint64_t)k * NB + b] + pos] = (uint16_t)(src - PB_H - k * PB_SB);
+      lidx[sbase[k] + pos] = (uint16_t)(src - PB_H - k * PB_SB);
       idx = lpre[k] + pos;
     }
     cold[pb_phys(q)] = (uint16_t)(idx | f);
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
   // padding slots of each segment expand from window offset 0
   for (uint32_t k = threadIdx.x; k < NS; k += PB_BUILD_THREADS) {
     const uint64_t at = (uint64_t)k * NB + b;
-    const uint32_t s0 = seg[at], c = seg[at + 1] - s0;
+    const uint32_t s0 = sbase[k], c = seg[at + 1] - s0;
     for (uint32_t p = ctr[k]; p < c; ++p) lidx[s0 + p] = 0;
   }
 }
