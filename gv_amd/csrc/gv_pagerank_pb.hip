@@ -47,7 +47,7 @@ constexpr uint32_t PB_IDX = 0x7FFFu;          // index field of a stream entry
 // hot source labels: [0, PB_H); a multiple of 4, so the expansion windows
 // (PB_H + k * PB_SB) start on 16-byte boundaries
 constexpr uint32_t PB_H = PB_PAD & ~3u;
-constexpr uint32_t PB_SB = 16384;             // cold sources per expansion block (64 KiB)
+constexpr uint32_t PB_SB = 32768;             // cold sources per expansion block (128 KiB)
 // A bin's image holds its cold values in source-block order, each block's
 // segment padded to whole groups of 4 (16 B): at most bin_tiles * PF_T +
 // 3 * NS slots, which must stay below PB_PAD.  bin_tiles is chosen per graph
