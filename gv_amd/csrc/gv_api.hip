@@ -474,7 +474,8 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
   g->has_in = (directed && cfg.maintain_in_edges) ? 1 : 0;
   const int nst = g->has_in ? 2 : 1;
   uint64_t P = cfg.pool_blocks;
-  const uint64_t bytes_per_block = (uint64_t)cfg.block_slots * 4 + 4 + 16;
+  // slots, free-stack entry, table entries, fences (one word per 16 slots)
+  const uint64_t bytes_per_block = (uint64_t)cfg.block_slots * 4 + 4 + 16 + cfg.block_slots / 4;
   if (P == 0) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)1 << 32;

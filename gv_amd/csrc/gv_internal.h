@@ -10,6 +10,16 @@
 //   used[V]         slots in use (live + tombstones); slots [0, used) valid.
 //   tbl_off[V]      offset of v's block table in arena[].
 //   tbl_cap[V]      capacity of that table (power of two, 0 = none).
+//   sorted[V]       slots [0, sorted[v]) of v hold ascending values, tombstones
+//                   aside (the sorted prefix: the first insert run into an
+//                   empty vertex, e.g. a bulk build, less later tombstones;
+//                   later runs are appended after it, compaction keeps its
+//                   order).
+//   fence[P*B/16]   fence[idx >> 4], for a slot idx at a position that is a
+//                   multiple of 16 below sorted[v]: the value that slot held
+//                   when the prefix was formed.  Prefix chunk c of 16 slots
+//                   then holds only values in [fence of c, fence of c + 1),
+//                   so a key can only be in one chunk of the prefix.
 //   arena[A]        block tables: arena[tbl_off[v] + i] = i-th block of v in
 //                   level order.  The reference's complete binary tree of
 //                   blocks (children 2i+1, 2i+2; device_functions.cu:42-100)
@@ -61,6 +71,7 @@ struct Store {
   uint32_t *slots = nullptr;
   uint32_t *free_ids = nullptr;
   uint32_t *deg = nullptr, *used = nullptr, *tbl_off = nullptr, *tbl_cap = nullptr;
+  uint32_t *sorted = nullptr, *fence = nullptr;
   uint32_t *arena = nullptr;
   Ctl *ctl = nullptr;
 };

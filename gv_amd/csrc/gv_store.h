@@ -5,6 +5,20 @@
 namespace gv {
 
 constexpr uint32_t PROBE_CHUNK = 256;  // slots per membership/tombstone work item
+constexpr uint32_t FENCE_SLOTS = 16;   // slots per fence chunk of a sorted prefix
+constexpr uint32_t FENCE_ITEM = 64 * FENCE_SLOTS;  // prefix slots per fence item (a chunk per lane)
+
+// Work items of a source with `used` slots and sorted prefix `sorted`: the
+// prefix up to a multiple of PROBE_CHUNK (pf) in fence items of FENCE_ITEM
+// slots, then [pf, used) in scan items of PROBE_CHUNK slots.  Never more
+// than ceil(used / PROBE_CHUNK).
+__host__ __device__ inline uint32_t probe_pf(uint32_t sorted) {
+  return sorted / PROBE_CHUNK * PROBE_CHUNK;
+}
+__host__ __device__ inline uint32_t probe_items(uint32_t used, uint32_t sorted) {
+  const uint32_t pf = probe_pf(sorted);
+  return (pf + FENCE_ITEM - 1) / FENCE_ITEM + (used - pf + PROBE_CHUNK - 1) / PROBE_CHUNK;
+}
 
 // Operation of a key in a merged group of batches.
 constexpr uint32_t OP_INSERT = 0;

@@ -60,6 +60,8 @@ gv_status store_alloc(Store &st, uint32_t V, uint32_t B, uint64_t P, uint64_t A)
   GV_HIP(hipMalloc(&st.used, (size_t)(V ? V : 1) * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&st.tbl_off, (size_t)(V ? V : 1) * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&st.tbl_cap, (size_t)(V ? V : 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.sorted, (size_t)(V ? V : 1) * sizeof(uint32_t)));
+  GV_HIP(hipMalloc(&st.fence, P * B / FENCE_SLOTS * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&st.arena, A * sizeof(uint32_t)));
   GV_HIP(hipMalloc(&st.ctl, sizeof(Ctl)));
   return GV_OK;
@@ -72,13 +74,16 @@ void store_release(Store &st) {
   hipFree(st.used);
   hipFree(st.tbl_off);
   hipFree(st.tbl_cap);
+  hipFree(st.sorted);
+  hipFree(st.fence);
   hipFree(st.arena);
   hipFree(st.ctl);
   st = Store();
 }
 
 __global__ void k_store_reset(uint32_t *free_ids, uint64_t P, uint32_t *deg, uint32_t *used,
-                              uint32_t *tbl_off, uint32_t *tbl_cap, uint32_t V, Ctl *ctl) {
+                              uint32_t *tbl_off, uint32_t *tbl_cap, uint32_t *sorted, uint32_t V,
+                              Ctl *ctl) {
   uint64_t n = P > V ? P : V;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -88,6 +93,7 @@ __global__ void k_store_reset(uint32_t *free_ids, uint64_t P, uint32_t *deg, uin
       used[i] = 0;
       tbl_off[i] = 0;
       tbl_cap[i] = 0;
+      sorted[i] = 0;
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -100,7 +106,7 @@ __global__ void k_store_reset(uint32_t *free_ids, uint64_t P, uint32_t *deg, uin
 void store_reset(Store &st, hipStream_t s) {
   uint64_t n = st.P > st.V ? st.P : st.V;
   k_store_reset<<<grid_for(n), 256, 0, s>>>(st.free_ids, st.P, st.deg, st.used, st.tbl_off,
-                                            st.tbl_cap, st.V, st.ctl);
+                                            st.tbl_cap, st.sorted, st.V, st.ctl);
 }
 
 bool scratch_fits(const Scratch &sc, const Store &st, uint64_t m) {
@@ -313,7 +319,7 @@ struct SegArgs {
 };
 
 __global__ void k_seg_build(const uint64_t *keys, uint64_t m, uint32_t bV, SegArgs a,
-                            const uint32_t *used, Ctl *ctl) {
+                            const uint32_t *used, const uint32_t *sorted, Ctl *ctl) {
   const uint32_t nv = ctl->n_valid;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -327,7 +333,7 @@ __global__ void k_seg_build(const uint64_t *keys, uint64_t m, uint32_t bV, SegAr
       a.seg_start[sidx] = (uint32_t)i;
       a.seg_src[sidx] = u;
       a.seg_used[sidx] = uo;
-      a.seg_cnt[sidx] = (uo + PROBE_CHUNK - 1) / PROBE_CHUNK;
+      a.seg_cnt[sidx] = probe_items(uo, sorted[u]);
       a.seg_blk[sidx] = 0;
     }
     if (i == 0) {
@@ -356,7 +362,7 @@ void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream
   k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, op, st.ctl);
   scan_u32(sc.segof, sc.segof, m, nullptr, sc.tmp, s);
   SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
-  k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.ctl);
+  k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.sorted, st.ctl);
   scan_u32(sc.seg_cnt, sc.item_start, m, &st.ctl->n_segs, sc.tmp, s);
   k_item_expand<<<grid_for(m), 256, 0, s>>>(sc.seg_cnt, sc.item_start, sc.item_seg, st.ctl);
 }
@@ -388,25 +394,210 @@ __device__ __forceinline__ void probe_hit(const Store &st, const ProbeArgs &a, u
 }
 
 constexpr int PROBE_ROUNDS = PROBE_CHUNK / WAVE;
-// An item this small is probed by one lane on its own.
-constexpr uint32_t PROBE_SMALL_SLOTS = 16;  // <= the smallest block (32 slots), multiple of 4
-constexpr uint32_t PROBE_SMALL_KEYS = 4;
+// A scan item of at most PROBE_LANE_SLOTS slots whose source has at most
+// PROBE_SMALL_KEYS batch keys is probed by one lane on its own, 16 slots per
+// round (a multiple of 16, at most PROBE_CHUNK).
+constexpr uint32_t PROBE_LANE_SLOTS = 128;
+constexpr uint32_t PROBE_SMALL_KEYS = 4;  // key indices packed in two bits
 // A wavefront-wide item whose source has at most this many batch keys
 // compares each slot with every key.
 constexpr uint32_t PROBE_FEW_KEYS = 8;
+// A fence item looks its keys' chunks up when its source has at most one
+// batch key per PROBE_SPARSE slots of sorted prefix; denser, it is scanned.
+constexpr uint32_t PROBE_SPARSE = 32;
 
-// Work items (PROBE_CHUNK slots of one source) are dealt to lanes: wave w of
+// First key of the source's sorted batch segment (keys[bks, bks + k), k > 64)
+// that is >= value x, relative to bks: p = the number of the 64 key samples
+// (lane l of the wavefront holding keys[bks + l*k/64]) below x brackets it,
+// then memory is searched over ~k/64 keys.
+__device__ __forceinline__ uint32_t probe_seg_lb(const uint64_t *keys, uint32_t bks, uint32_t k,
+                                                 uint64_t ukey, uint32_t p, uint32_t x) {
+  uint32_t l2 = p ? bks + (uint32_t)(((uint64_t)(p - 1) * k) >> 6) + 1 : bks;
+  uint32_t h2 = p < (uint32_t)WAVE ? bks + (uint32_t)(((uint64_t)p * k) >> 6) : bks + k;
+  const uint64_t key = ukey | x;
+  while (l2 < h2) {
+    const uint32_t mid = (l2 + h2) >> 1;
+    if (keys[mid] < key) l2 = mid + 1; else h2 = mid;
+  }
+  return l2 - bks;
+}
+
+// Key kt against the 16 slots x of a chunk starting at slot index cb: the
+// values in a store adjacency are distinct, so at most one slot matches.
+template <int MODE>
+__device__ __forceinline__ void probe_chunk_hit(const Store &st, const ProbeArgs &a,
+                                                const uint32_t (&x)[FENCE_SLOTS], uint32_t kt,
+                                                uint32_t cb, uint32_t p, uint32_t &cnt) {
+  uint32_t hm = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < FENCE_SLOTS; ++j) hm |= x[j] == kt ? 1u << j : 0u;
+  if (hm) probe_hit<MODE>(st, a, cb + (uint32_t)__ffs(hm) - 1, p, cnt);
+}
+
+// A fence item: prefix slots [blo, bhi) of one source (multiples of
+// FENCE_SLOTS, at most 64 chunks), lane l owning chunk blo/16 + l.  The lane
+// reads its chunk's fence and the next one, which bound the chunk's values;
+// only when a batch key of the source lies in that range does it read the
+// chunk's 16 slots (four 16-byte loads) and compare them with those keys.  sv:
+// the source's keys one per lane (k <= 64, padded with TOMB) or 64 key samples.
+// The first of equal keys acts, as on the scan path.
+template <int MODE>
+__device__ __forceinline__ void probe_fences(const Store &st, const ProbeArgs &a, uint64_t mask,
+                                             uint64_t ukey, uint32_t lane, uint32_t blo,
+                                             uint32_t bhi, uint32_t bso, uint32_t btbl,
+                                             uint32_t bks, uint32_t k, uint32_t sv,
+                                             uint32_t &cnt) {
+  const uint32_t nch = (bhi - blo) / FENCE_SLOTS;
+  const bool on = lane < nch;
+  uint32_t cb = 0, f = TOMB;
+  if (on) {
+    cb = slot_index(st.arena, btbl, blo + lane * FENCE_SLOTS, st.log2B, st.B);
+    f = st.fence[cb / FENCE_SLOTS];
+  }
+  // exclusive upper bound of the chunk's values: the next chunk's fence, or
+  // TOMB (above every ID) past the prefix
+  uint32_t fn = __shfl(f, (int)((lane + 1) & (WAVE - 1)), WAVE);
+  if (lane + 1 == nch) {
+    fn = TOMB;
+    if (bhi < bso) fn = st.fence[slot_index(st.arena, btbl, bhi, st.log2B, st.B) / FENCE_SLOTS];
+  }
+  const uint32_t last = __shfl(sv, WAVE - 1, WAVE);
+  uint32_t p0 = wave_lower_bound(sv, f), p1 = wave_lower_bound(sv, fn);
+  if (p0 == WAVE - 1 && last < f) p0 = WAVE;
+  if (p1 == WAVE - 1 && last < fn) p1 = WAVE;
+  // keys [i0, i1) of the segment (relative to bks) fall in the chunk's range
+  uint32_t i0 = 0, i1 = 0;
+  if (on) {
+    if (k <= (uint32_t)WAVE) {
+      i0 = p0;
+      i1 = p1;
+    } else {
+      i0 = probe_seg_lb(a.keys, bks, k, ukey, p0, f);
+      i1 = fn == TOMB ? k : probe_seg_lb(a.keys, bks, k, ukey, p1, fn);
+    }
+  }
+  const uint32_t nk = i1 - i0;
+  uint32_t mk = nk;
+#pragma unroll
+  for (int d = WAVE / 2; d >= 1; d >>= 1) {
+    const uint32_t o = __shfl_xor(mk, d, WAVE);
+    mk = o > mk ? o : mk;
+  }
+  if (!mk) return;
+  uint32_t x[FENCE_SLOTS];
+  const uint4 *p4 = reinterpret_cast<const uint4 *>(st.slots + cb);
+#pragma unroll
+  for (uint32_t r = 0; r < FENCE_SLOTS / 4; ++r) {
+    const uint4 q = nk ? p4[r] : make_uint4(TOMB, TOMB, TOMB, TOMB);
+    x[4 * r] = q.x;
+    x[4 * r + 1] = q.y;
+    x[4 * r + 2] = q.z;
+    x[4 * r + 3] = q.w;
+  }
+  uint32_t kprev = TOMB;
+  if (k <= (uint32_t)WAVE) {
+    // keys from their lanes: a uniform loop, every lane shuffling
+    for (uint32_t t = 0; t < mk; ++t) {
+      const uint32_t kt = __shfl(sv, (int)((i0 + t) & (WAVE - 1)), WAVE);
+      if (t < nk && kt != kprev) probe_chunk_hit<MODE>(st, a, x, kt, cb, bks + i0 + t, cnt);
+      kprev = kt;
+    }
+  } else {
+    for (uint32_t t = 0; t < nk; ++t) {
+      const uint32_t kt = (uint32_t)(a.keys[bks + i0 + t] & mask);
+      if (kt != kprev) probe_chunk_hit<MODE>(st, a, x, kt, cb, bks + i0 + t, cnt);
+      kprev = kt;
+    }
+  }
+}
+
+// A scan item: slots [blo, bhi) of one source (blo a multiple of
+// PROBE_CHUNK), read in 64-slot rounds, each live slot searched among the
+// source's batch keys (sv as in probe_fences).
+template <int MODE>
+__device__ __forceinline__ void probe_scan(const Store &st, const ProbeArgs &a, uint32_t bV,
+                                           uint32_t bu, uint32_t lane, uint32_t blo,
+                                           uint32_t bhi, uint32_t btbl, uint32_t bks,
+                                           uint32_t bke, uint32_t k, uint32_t sv,
+                                           uint32_t &cnt) {
+  uint32_t idx[PROBE_ROUNDS], wv[PROBE_ROUNDS];
+  if (st.log2B >= 6) {
+    // blo is a multiple of PROBE_CHUNK, so each 64-slot round lies in one
+    // block: one uniform block-table load per round
+#pragma unroll
+    for (int q = 0; q < PROBE_ROUNDS; ++q) {
+      const uint32_t base = blo + q * WAVE;
+      const uint32_t blk = base < bhi ? st.arena[btbl + (base >> st.log2B)] : 0u;
+      const uint32_t pos = base + lane;
+      idx[q] = pos < bhi ? blk * st.B + (pos & (st.B - 1)) : 0xFFFFFFFFu;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < PROBE_ROUNDS; ++q) {
+      const uint32_t pos = blo + q * WAVE + lane;
+      idx[q] = pos < bhi ? slot_index(st.arena, btbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PROBE_ROUNDS; ++q)
+    wv[q] = idx[q] != 0xFFFFFFFFu ? st.slots[idx[q]] : TOMB;
+  if (k <= PROBE_FEW_KEYS) {
+    // few keys (most touched vertices): wave-uniform copies, one compare
+    // per key and slot instead of a search over the lanes
+    uint32_t kv[PROBE_FEW_KEYS];
+#pragma unroll
+    for (uint32_t j = 0; j < PROBE_FEW_KEYS; ++j)
+      kv[j] = __builtin_amdgcn_readlane(sv, (int)j);
+#pragma unroll
+    for (int q = 0; q < PROBE_ROUNDS; ++q) {
+      uint32_t p = PROBE_FEW_KEYS;
+#pragma unroll
+      for (uint32_t j = PROBE_FEW_KEYS; j-- > 0;)
+        if (kv[j] == wv[q]) p = j;
+      if (wv[q] != TOMB && p < k) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
+    }
+  } else if (k <= (uint32_t)WAVE) {
+#pragma unroll
+    for (int q = 0; q < PROBE_ROUNDS; ++q) {
+      const uint32_t p = wave_lower_bound(sv, wv[q]);
+      const uint32_t hit = __shfl(sv, (int)p, WAVE);
+      if (wv[q] != TOMB && hit == wv[q]) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
+    }
+  } else {
+    // the key samples bracket the lower bound: keys[f(p-1)] < x <= keys[f(p)]
+    // with f(l) = bks + l*k/64, so memory is searched over ~k/64 keys
+    const uint64_t ukey = (uint64_t)bu << bV;
+    const uint32_t last = __shfl(sv, WAVE - 1, WAVE);
+#pragma unroll
+    for (int q = 0; q < PROBE_ROUNDS; ++q) {
+      uint32_t p = wave_lower_bound(sv, wv[q]);  // at most 63: all 64 samples < x gives 64
+      if (p == WAVE - 1 && last < wv[q]) p = WAVE;
+      if (wv[q] == TOMB) continue;
+      const uint64_t key = ukey | wv[q];
+      uint32_t l2 = p ? bks + (uint32_t)(((uint64_t)(p - 1) * k) >> 6) + 1 : bks;
+      uint32_t h2 = p < (uint32_t)WAVE ? bks + (uint32_t)(((uint64_t)p * k) >> 6) : bke;
+      while (l2 < h2) {
+        const uint32_t mid = (l2 + h2) >> 1;
+        if (a.keys[mid] < key) l2 = mid + 1; else h2 = mid;
+      }
+      if (l2 < bke && a.keys[l2] == key) probe_hit<MODE>(st, a, idx[q], l2, cnt);
+    }
+  }
+}
+
+// Work items (fence items over a source's sorted prefix, scan items of
+// PROBE_CHUNK slots over the rest) are dealt to lanes: wave w of
 // W, step r, lane l takes item (r * 64 + l) * W + w, so the items of one hub
-// spread over many wavefronts.  A small item (<= PROBE_SMALL_SLOTS slots,
+// spread over many wavefronts.  A small item (<= PROBE_LANE_SLOTS slots,
 // <= PROBE_SMALL_KEYS batch keys -- most touched vertices of an R-MAT batch)
 // is probed by its lane alone: 64 of them share one chain of dependent loads
 // instead of paying it one after another.  The other items of the step are
 // then probed by the whole wavefront, one at a time, their metadata broadcast
-// from the owning lane: all eight 64-slot rounds are loaded before any test,
-// and the source's sorted batch segment is held in registers (one key per
-// lane) when it has at most 64 keys, otherwise searched in memory.
+// from the owning lane (probe_fences, probe_scan); the source's sorted batch
+// segment is held in registers (one key per lane) when it has at most 64
+// keys, otherwise searched in memory.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t bV, Ctl *ctl) {
+__global__ __launch_bounds__(256, 8) void k_probe(Store st, ProbeArgs a, uint32_t bV, Ctl *ctl) {
   if (MODE != PROBE_INSERT && ctl->abort) return;
   const uint32_t nitems = ctl->n_items;
   const uint32_t lane = lane_id();
@@ -416,66 +607,80 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
   for (uint64_t r = 0; r * WAVE * W + w < nitems; ++r) {
     const uint64_t it = (r * WAVE + lane) * W + w;
     const bool valid = it < nitems;
-    uint32_t sidx = 0, lo = 0, hi = 0, u = 0, tbl = 0, ks = 0, ke = 0, uo = 0;
+    uint32_t sidx = 0, lo = 0, hi = 0, u = 0, tbl = 0, ks = 0, ke = 0, uo = 0, so = 0;
+    bool fitem = false;
     if (valid) {
       sidx = a.item_seg[it];
       const uint32_t c = (uint32_t)it - a.item_start[sidx];
       u = a.seg_src[sidx];
       uo = a.seg_used[sidx];
       tbl = st.tbl_off[u];
+      so = st.sorted[u];
       ks = a.seg_start[sidx];
       ke = a.seg_start[sidx + 1];
-      lo = c * PROBE_CHUNK;
-      hi = lo + PROBE_CHUNK < uo ? lo + PROBE_CHUNK : uo;
+      // the item list of probe_items: fence items over [0, pf), then scan
+      // items over [pf, uo)
+      const uint32_t pf = probe_pf(so);
+      const uint32_t nf = (pf + FENCE_ITEM - 1) / FENCE_ITEM;
+      if (c < nf) {
+        fitem = true;
+        lo = c * FENCE_ITEM;
+        hi = lo + FENCE_ITEM < pf ? lo + FENCE_ITEM : pf;
+      } else {
+        lo = pf + (c - nf) * PROBE_CHUNK;
+        hi = lo + PROBE_CHUNK < uo ? lo + PROBE_CHUNK : uo;
+      }
     }
-    const bool small = valid && hi - lo <= PROBE_SMALL_SLOTS && ke - ks <= PROBE_SMALL_KEYS;
+    const bool small =
+        valid && !fitem && hi - lo <= PROBE_LANE_SLOTS && ke - ks <= PROBE_SMALL_KEYS;
     if (small) {
-      uint64_t kv[PROBE_SMALL_KEYS];
+      // the keys' destinations, TOMB past the segment (never equal to a live slot)
+      uint32_t kv[PROBE_SMALL_KEYS];
 #pragma unroll
       for (uint32_t j = 0; j < PROBE_SMALL_KEYS; ++j)
-        kv[j] = ks + j < ke ? (a.keys[ks + j] & mask) : ~0ull;
+        kv[j] = ks + j < ke ? (uint32_t)(a.keys[ks + j] & mask) : TOMB;
       uint32_t cnt = 0;
-      if (MODE == PROBE_INSERT) {
-        // lo is a multiple of PROBE_CHUNK and the item has <= 16 slots, so it
-        // lies at the start of one block (B >= 32): one block-table load,
-        // then its slots as four 16-byte loads issued together
-        const uint32_t n = hi - lo;
-        const uint32_t b0 = n ? st.arena[tbl + (lo >> st.log2B)] * st.B : 0u;
-        const uint4 *p4 = reinterpret_cast<const uint4 *>(st.slots + b0);
-        uint32_t x[PROBE_SMALL_SLOTS];
+      const uint32_t n = hi - lo;
+      // lo is a multiple of PROBE_CHUNK, so each round of 16 slots lies in
+      // one block; the block-table entry is loaded as a round enters a new
+      // block, then the round's slots as four 16-byte loads issued together
+      uint32_t blk = 0;
+#pragma unroll 1
+      for (uint32_t r = 0; 16 * r < n; ++r) {
+        const uint32_t pos = lo + 16 * r;
+        if (r == 0 || (pos & (st.B - 1)) == 0) blk = st.arena[tbl + (pos >> st.log2B)];
+        const uint32_t rb = blk * st.B + (pos & (st.B - 1));
+        const uint32_t nr = n - 16 * r;
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(st.slots + rb);
+        uint32_t x[16];
 #pragma unroll
-        for (uint32_t r = 0; r < PROBE_SMALL_SLOTS / 4; ++r) {
-          const uint4 q = 4 * r < n ? p4[r] : make_uint4(TOMB, TOMB, TOMB, TOMB);
-          x[4 * r] = q.x;
-          x[4 * r + 1] = q.y;
-          x[4 * r + 2] = q.z;
-          x[4 * r + 3] = q.w;
+        for (uint32_t q = 0; q < 4; ++q) {
+          const uint4 v = 4 * q < nr ? p4[q] : make_uint4(TOMB, TOMB, TOMB, TOMB);
+          x[4 * q] = v.x;
+          x[4 * q + 1] = v.y;
+          x[4 * q + 2] = v.z;
+          x[4 * q + 3] = v.w;
         }
+        // matching slots (hm) and, two bits each, their first matching key
+        // (the lower bound, as on the wavefront path); hits are acted on after
+        uint32_t hm = 0, hp = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < PROBE_SMALL_SLOTS; ++j) {
-          if (j >= n || x[j] == TOMB) continue;
-          // first matching key, as the lower bound of the wavefront path
+        for (uint32_t j = 0; j < 16; ++j) {
           uint32_t p = PROBE_SMALL_KEYS;
 #pragma unroll
           for (uint32_t i = PROBE_SMALL_KEYS; i-- > 0;)
             if (kv[i] == x[j]) p = i;
-          if (p < PROBE_SMALL_KEYS) probe_hit<MODE>(st, a, b0 + j, ks + p, cnt);
+          const bool hit = j < nr && x[j] != TOMB && p < PROBE_SMALL_KEYS;
+          hm |= hit ? 1u << j : 0u;
+          hp |= hit ? p << (2 * j) : 0u;
         }
-      } else {
-        // a delete has few small items; the short loop keeps registers (and
-        // occupancy) for the wavefront-wide items
-        for (uint32_t pos = lo; pos < hi; ++pos) {
-          const uint32_t idx = slot_index(st.arena, tbl, pos, st.log2B, st.B);
-          const uint32_t x = st.slots[idx];
-          if (x == TOMB) continue;
-          uint32_t p = PROBE_SMALL_KEYS;
-#pragma unroll
-          for (uint32_t j = PROBE_SMALL_KEYS; j-- > 0;)
-            if (kv[j] == x) p = j;
-          if (p < PROBE_SMALL_KEYS) probe_hit<MODE>(st, a, idx, ks + p, cnt);
+        while (hm) {
+          const uint32_t j = (uint32_t)__ffs(hm) - 1;
+          hm &= hm - 1;
+          probe_hit<MODE>(st, a, rb + j, ks + ((hp >> (2 * j)) & 3u), cnt);
         }
-        if (cnt) atomicAdd(&a.seg_del[sidx], cnt);
       }
+      if (MODE != PROBE_INSERT && cnt) atomicAdd(&a.seg_del[sidx], cnt);
     }
     unsigned long long big = __ballot(valid && !small);
     while (big) {
@@ -489,6 +694,9 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
       const uint32_t btbl = __builtin_amdgcn_readfirstlane(__shfl(tbl, src, WAVE));
       const uint32_t bks = __builtin_amdgcn_readfirstlane(__shfl(ks, src, WAVE));
       const uint32_t bke = __builtin_amdgcn_readfirstlane(__shfl(ke, src, WAVE));
+      const uint32_t bso = __builtin_amdgcn_readfirstlane(__shfl(so, src, WAVE));
+      const uint32_t bfit =
+          __builtin_amdgcn_readfirstlane(__shfl(fitem ? 1u : 0u, src, WAVE));
       const uint32_t k = bke - bks;
       // batch keys first, independent of the slot chain: the whole segment
       // (one key per lane) when it has <= 64 keys, else 64 fences, lane l
@@ -499,69 +707,18 @@ __global__ __launch_bounds__(256) void k_probe(Store st, ProbeArgs a, uint32_t b
       } else {
         sv = (uint32_t)(a.keys[bks + (uint32_t)(((uint64_t)lane * k) >> 6)] & mask);
       }
-      uint32_t idx[PROBE_ROUNDS], wv[PROBE_ROUNDS];
-      if (st.log2B >= 6) {
-        // blo is a multiple of PROBE_CHUNK, so each 64-slot round lies in one
-        // block: one uniform block-table load per round
-#pragma unroll
-        for (int q = 0; q < PROBE_ROUNDS; ++q) {
-          const uint32_t base = blo + q * WAVE;
-          const uint32_t blk = base < bhi ? st.arena[btbl + (base >> st.log2B)] : 0u;
-          const uint32_t pos = base + lane;
-          idx[q] = pos < bhi ? blk * st.B + (pos & (st.B - 1)) : 0xFFFFFFFFu;
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < PROBE_ROUNDS; ++q) {
-          const uint32_t pos = blo + q * WAVE + lane;
-          idx[q] = pos < bhi ? slot_index(st.arena, btbl, pos, st.log2B, st.B) : 0xFFFFFFFFu;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < PROBE_ROUNDS; ++q)
-        wv[q] = idx[q] != 0xFFFFFFFFu ? st.slots[idx[q]] : TOMB;
       uint32_t cnt = 0;
-      if (k <= PROBE_FEW_KEYS) {
-        // few keys (most touched vertices): wave-uniform copies, one compare
-        // per key and slot instead of a search over the lanes
-        uint32_t kv[PROBE_FEW_KEYS];
-#pragma unroll
-        for (uint32_t j = 0; j < PROBE_FEW_KEYS; ++j)
-          kv[j] = __builtin_amdgcn_readlane(sv, (int)j);
-#pragma unroll
-        for (int q = 0; q < PROBE_ROUNDS; ++q) {
-          uint32_t p = PROBE_FEW_KEYS;
-#pragma unroll
-          for (uint32_t j = PROBE_FEW_KEYS; j-- > 0;)
-            if (kv[j] == wv[q]) p = j;
-          if (wv[q] != TOMB && p < k) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
-        }
-      } else if (k <= (uint32_t)WAVE) {
-#pragma unroll
-        for (int q = 0; q < PROBE_ROUNDS; ++q) {
-          const uint32_t p = wave_lower_bound(sv, wv[q]);
-          const uint32_t hit = __shfl(sv, (int)p, WAVE);
-          if (wv[q] != TOMB && hit == wv[q]) probe_hit<MODE>(st, a, idx[q], bks + p, cnt);
-        }
+      if (bfit && (uint64_t)k * PROBE_SPARSE <= bso) {
+        probe_fences<MODE>(st, a, mask, (uint64_t)bu << bV, lane, blo, bhi, bso, btbl, bks, k,
+                           sv, cnt);
+      } else if (bfit) {
+        // keys dense in the prefix: reading every chunk is cheaper than
+        // looking each key's chunk up
+        for (uint32_t q0 = blo; q0 < bhi; q0 += PROBE_CHUNK)
+          probe_scan<MODE>(st, a, bV, bu, lane, q0, q0 + PROBE_CHUNK < bhi ? q0 + PROBE_CHUNK : bhi,
+                           btbl, bks, bke, k, sv, cnt);
       } else {
-        // the fences bracket the lower bound: keys[f(p-1)] < x <= keys[f(p)]
-        // with f(l) = bks + l*k/64, so memory is searched over ~k/64 keys
-        const uint64_t ukey = (uint64_t)bu << bV;
-        const uint32_t last = __shfl(sv, WAVE - 1, WAVE);
-#pragma unroll
-        for (int q = 0; q < PROBE_ROUNDS; ++q) {
-          uint32_t p = wave_lower_bound(sv, wv[q]);  // at most 63: all 64 fences < x gives 64
-          if (p == WAVE - 1 && last < wv[q]) p = WAVE;
-          if (wv[q] == TOMB) continue;
-          const uint64_t key = ukey | wv[q];
-          uint32_t l2 = p ? bks + (uint32_t)(((uint64_t)(p - 1) * k) >> 6) + 1 : bks;
-          uint32_t h2 = p < (uint32_t)WAVE ? bks + (uint32_t)(((uint64_t)p * k) >> 6) : bke;
-          while (l2 < h2) {
-            const uint32_t mid = (l2 + h2) >> 1;
-            if (a.keys[mid] < key) l2 = mid + 1; else h2 = mid;
-          }
-          if (l2 < bke && a.keys[l2] == key) probe_hit<MODE>(st, a, idx[q], l2, cnt);
-        }
+        probe_scan<MODE>(st, a, bV, bu, lane, blo, bhi, btbl, bks, bke, k, sv, cnt);
       }
       if (MODE != PROBE_INSERT) {
         cnt = wave_sum(cnt);
@@ -669,6 +826,9 @@ __global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
     uint32_t nbo = (uo + st.B - 1) >> st.log2B;
     uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
     uint32_t tbl = st.tbl_off[u];
+    // the survivors of an empty vertex are written in key order: a sorted
+    // prefix (a run into a non-empty vertex starts or extends its tail)
+    if (uo == 0) st.sorted[u] = surv;
     const uint64_t pl = a.plan[sidx];
     if (nbn > st.tbl_cap[u]) {
       uint32_t noff = (uint32_t)(ab0 + (pl & 0xFFFFFFFFull));
@@ -700,7 +860,9 @@ __global__ void k_ins_write(Store st, WriteArgs a, uint32_t bV, Ctl *ctl) {
     uint32_t pos = a.seg_used[sidx] + (a.rank[i] - a.rank[a.seg_start[sidx]]);
     uint32_t u = a.seg_src[sidx];
     uint32_t idx = slot_index(st.arena, st.tbl_off[u], pos, st.log2B, st.B);
-    st.slots[idx] = (uint32_t)(a.keys[i] & mask);
+    const uint32_t v = (uint32_t)(a.keys[i] & mask);
+    st.slots[idx] = v;
+    if ((pos & (FENCE_SLOTS - 1)) == 0 && pos < st.sorted[u]) st.fence[idx / FENCE_SLOTS] = v;
   }
 }
 
@@ -827,7 +989,10 @@ __global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist
     const uint32_t u = vlist[i];
     const uint32_t uo = st.used[u];
     const uint32_t tbl = st.tbl_off[u];
-    uint32_t wpos = 0;
+    // the live slots of the sorted prefix stay first and in order: they are
+    // the new prefix, and its chunks get fences from their new first slots
+    const uint32_t so = st.sorted[u];
+    uint32_t wpos = 0, nso = 0;
     for (uint32_t base = 0; base < uo; base += WAVE) {
       uint32_t pos = base + lane;
       uint32_t w = TOMB;
@@ -836,8 +1001,13 @@ __global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist
       unsigned long long m = __ballot(keep);
       if (keep) {
         uint32_t dp = wpos + mask_lt_count(m);
-        st.slots[slot_index(st.arena, tbl, dp, st.log2B, st.B)] = w;
+        const uint32_t didx = slot_index(st.arena, tbl, dp, st.log2B, st.B);
+        st.slots[didx] = w;
+        if (pos < so && (dp & (FENCE_SLOTS - 1)) == 0) st.fence[didx / FENCE_SLOTS] = w;
       }
+      const unsigned long long below =
+          so <= base ? 0ull : (so - base >= (uint32_t)WAVE ? ~0ull : (1ull << (so - base)) - 1);
+      nso += (uint32_t)__popcll(m & below);
       wpos += (uint32_t)__popcll(m);
     }
     const uint32_t nbo = (uo + st.B - 1) >> st.log2B;
@@ -849,6 +1019,7 @@ __global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist
     for (uint32_t j = lane; j < nf; j += WAVE) st.free_ids[fb + j] = st.arena[tbl + nbn + j];
     if (lane == 0) {
       st.used[u] = wpos;
+      st.sorted[u] = nso;
       if (nf) atomicAdd(&ctl->blocks_freed, (unsigned long long)nf);
     }
   }
