@@ -260,10 +260,9 @@ static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *d
   if (g->capturing) g->cap_pairs += n;
   for (int k = 0; k < nst; ++k) {
     const int mirror = g->directed ? 0 : 1;
-    op_begin(g->st[k], s);
     keys[k] = stage(g->st[k], g->sc[k], ds, dd, n, mirror, k == 1 ? 1 : 0, m[k], s);
   }
-  if (nst > 1) abort_join(g->st[0], g->st[1], s);
+  if (nst > 1) abort_join(g->st[0], g->st[1], g->sc[0], g->sc[1], s);
   for (int k = 0; k < nst; ++k) {
     segment(g->st[k], g->sc[k], keys[k], m[k], s);
     if (op == Op::Insert) insert_plan(g->st[k], g->sc[k], keys[k], m[k], s);
@@ -366,7 +365,7 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
       keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, g->grp_off, g->grp_kind, nb, mirror,
                             k == 1 ? 1 : 0, m[k], s);
     }
-    if (nst > 1) abort_join(g->st[0], g->st[1], s);
+    if (nst > 1) abort_join(g->st[0], g->st[1], g->sc[0], g->sc[1], s);
     for (int k = 0; k < nst; ++k) {
       segment(g->st[k], g->sc[k], keys[k], m[k], s, g->sc[k].op);
       mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
@@ -383,8 +382,7 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
         compact_touched(g->st[k], g->sc[k], g->directed ? n : 2 * n, compact_fraction, g->vlist,
                         s);
       } else {
-        op_begin(g->st[k], s);
-        compact(g->st[k], compact_fraction, g->vlist, s);
+        compact(g->st[k], compact_fraction, false, s);
       }
     }
   }
@@ -576,10 +574,8 @@ gv_status gv_compact(gv_graph *g, float frac, uint64_t *vertices_compacted) {
     return g->cap_err = GV_E_INVALID;
   }
   if (frac < 0.f) frac = g->cfg.compact_threshold;
-  for (int k = 0; k < (g->has_in ? 2 : 1); ++k) {
-    op_begin(g->st[k], g->stream);
-    compact(g->st[k], frac, g->vlist, g->stream);
-  }
+  for (int k = 0; k < (g->has_in ? 2 : 1); ++k)
+    compact(g->st[k], frac, vertices_compacted != nullptr, g->stream);
   GV_HIP(hipGetLastError());
   if (!vertices_compacted) return GV_OK;
   rc = read_ctl(g);
@@ -1064,7 +1060,6 @@ gv_status gv_triangle_count_update(gv_graph *g, int kind, const uint32_t *src, c
   Scratch &sc = g->sc[0];
   uint64_t m = 0;
   long long delta = 0;
-  op_begin(st, s);
   uint64_t *keys = stage(st, sc, ds, dd, n, 1, 0, m, s);
   segment(st, sc, keys, m, s);
   if (kind == GV_OP_INSERT) {

@@ -55,7 +55,7 @@ struct Ctl {
   unsigned long long blk_base;    // absolute index into free_ids of this op's blocks
   unsigned long long arena_base;  // absolute arena offset of this op's tables
   unsigned long long applied;
-  unsigned long long blocks_freed;
+  unsigned long long blocks_freed;  // unused (kept for the layout)
   unsigned long long compacted;
   unsigned int n_valid;           // valid (non-sentinel) keys of the staged batch
   unsigned int n_segs;            // source segments of the staged batch
@@ -93,6 +93,7 @@ struct Scratch {
   uint32_t *seg_blk = nullptr;            // per segment: tombstones written (delete)
   uint64_t *plan = nullptr;               // per segment: (new blocks << 32 | table cap), scanned
   uint32_t *op = nullptr;                 // per key: final op of a merged group
+  const uint32_t *sort_err = nullptr;     // failure word of the last staging sort, if not folded
   void *tmp = nullptr;                    // scan/sort partials
   size_t tmp_bytes = 0;
 };
@@ -247,12 +248,24 @@ constexpr uint64_t SORT_MAX_KEYS = 0xFFFFFFFFull;
 constexpr uint32_t ABORT_SORT = 5;
 // LSD radix sort of n <= SORT_MAX_KEYS keys on their low `bits` bits;
 // returns the buffer that holds the result (a or b).  abort (device word, may
-// be null) receives ABORT_SORT if the sort failed.
+// be null) receives ABORT_SORT if the sort failed.  cleared: the caller has
+// already zeroed the words sort_clear_span names (the sort then skips that
+// launch).
 uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp, hipStream_t s,
-                    uint32_t *abort = nullptr);
+                    uint32_t *abort = nullptr, bool cleared = false);
 // Same, ordering on bits [lo, hi) only (stable for keys equal there).
 uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
-                          void *tmp, hipStream_t s, uint32_t *abort = nullptr);
+                          void *tmp, hipStream_t s, uint32_t *abort = nullptr,
+                          bool cleared = false);
+// The workspace words a sort of n keys on bits [lo, hi) needs zeroed before
+// it starts (w[0, words)), and its failure word (nonzero after the sort if a
+// tile gave up); both empty when the sort takes the path that needs neither.
+struct SortClear {
+  uint32_t *w = nullptr;
+  uint64_t words = 0;
+  const uint32_t *err = nullptr;
+};
+SortClear sort_clear_span(void *tmp, uint64_t n, uint32_t lo, uint32_t hi);
 // After a sort of n keys on bits [lo, hi) without an abort word: waits for
 // the stream and returns GV_E_HIP (error text set) if the sort failed.
 gv_status sort_check(const void *tmp, uint64_t n, uint32_t lo, uint32_t hi, hipStream_t s);

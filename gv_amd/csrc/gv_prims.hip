@@ -512,20 +512,28 @@ size_t sort_tmp_bytes(uint64_t n) {
 }
 
 uint64_t *sort_keys(uint64_t *a, uint64_t *b, uint64_t n, uint32_t bits, void *tmp,
-                    hipStream_t s, uint32_t *abort) {
-  return sort_keys_range(a, b, n, 0, bits, tmp, s, abort);
+                    hipStream_t s, uint32_t *abort, bool cleared) {
+  return sort_keys_range(a, b, n, 0, bits, tmp, s, abort, cleared);
+}
+
+SortClear sort_clear_span(void *tmp, uint64_t n, uint32_t lo, uint32_t hi) {
+  const uint32_t passes = (hi - lo + 7) / 8;
+  if (n <= 1 || n > OS_MAX_KEYS || passes > (uint32_t)OS_MAX_PASSES) return SortClear{};
+  const OsTmp t = os_tmp(tmp);
+  return SortClear{t.ghist, os_words(cdiv(n, OS_TILE), passes), t.err};
 }
 
 uint64_t *sort_keys_range(uint64_t *a, uint64_t *b, uint64_t n, uint32_t lo, uint32_t hi,
-                          void *tmp, hipStream_t s, uint32_t *abort) {
+                          void *tmp, hipStream_t s, uint32_t *abort, bool cleared) {
   if (n <= 1) return a;
   const uint32_t passes = (hi - lo + 7) / 8;
   if (n <= OS_MAX_KEYS && passes <= (uint32_t)OS_MAX_PASSES) {
     const uint64_t nt = cdiv(n, OS_TILE);
     const OsTmp t = os_tmp(tmp);
     const uint64_t words = os_words(nt, passes);
-    k_os_clear<<<dim3((unsigned)(cdiv(words, 256) < 1024 ? cdiv(words, 256) : 1024)), 256, 0, s>>>(
-        t.ghist, words);
+    if (!cleared)
+      k_os_clear<<<dim3((unsigned)(cdiv(words, 256) < 1024 ? cdiv(words, 256) : 1024)), 256, 0,
+                   s>>>(t.ghist, words);
     const uint64_t hb = cdiv(n, OS_TILE);
     k_os_hist<<<dim3((unsigned)(hb < 1024 ? hb : 1024)), dim3(SORT_THREADS), 0, s>>>(a, n, lo,
                                                                                      passes, t.ghist);

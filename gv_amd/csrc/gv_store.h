@@ -38,11 +38,15 @@ void scratch_release(Scratch &sc);
 
 // Per-op counter reset (keeps the sticky abort code).
 void op_begin(Store &st, hipStream_t s);
-// Copy an abort code set in either store into the other.
-void abort_join(Store &a, Store &b, hipStream_t s);
+// Fold the staging sorts' failure words into the stores' abort codes, then
+// copy an abort code set in either store into the other.
+void abort_join(Store &a, Store &b, const Scratch &sa, const Scratch &sb, hipStream_t s);
 
-// Stage n (src,dst) pairs as sorted keys in sc; mirror adds (dst,src) too,
-// swap stores (dst,src) instead.  Returns the sorted key buffer; m = keys.
+// Start an operation (the per-op counters reset, as op_begin) and stage n
+// (src,dst) pairs as sorted keys in sc; mirror adds (dst,src) too, swap
+// stores (dst,src) instead.  Returns the sorted key buffer; m = keys.  A
+// failure of the sort is left in sc.sort_err for segment (or abort_join) to
+// fold into the abort code.
 uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst, uint64_t n,
                 int mirror, int swap, uint64_t &m, hipStream_t s);
 // Stage a group of nb batches (entries [boff[b], boff[b+1]) of batch b, op
@@ -68,8 +72,9 @@ void commit_pair(Store &a, Scratch &sa, Store *b, Scratch *sb, hipStream_t s);
 void insert_write(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Delete: tombstone matches and fix degrees.
 void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
-// Compaction of vertices whose tombstones exceed frac of their slots.
-void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s);
+// Compaction of vertices whose tombstones exceed frac of their slots; count:
+// also leave their number in ctl->compacted.
+void compact(Store &st, float frac, bool count, hipStream_t s);
 // Compaction restricted to the sources of the last staged batch of sc (m =
 // staged key bound), for pipelines that compact what they just updated.
 void compact_touched(Store &st, const Scratch &sc, uint64_t m, float frac, uint32_t *vlist,

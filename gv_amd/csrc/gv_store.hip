@@ -147,7 +147,7 @@ void scratch_release(Scratch &sc) {
   sc = Scratch();
 }
 
-__global__ void k_op_begin(Ctl *ctl) {
+__device__ __forceinline__ void op_reset(Ctl *ctl) {
   ctl->need_blocks = 0;
   ctl->need_arena = 0;
   ctl->blk_base = 0;
@@ -159,11 +159,19 @@ __global__ void k_op_begin(Ctl *ctl) {
   ctl->n_segs = 0;
   ctl->n_items = 0;
 }
+__global__ void k_op_begin(Ctl *ctl) { op_reset(ctl); }
 void op_begin(Store &st, hipStream_t s) { k_op_begin<<<1, 1, 0, s>>>(st.ctl); }
 
 // --------------------------------------------------------------- staging --
+// Also starts the operation: the per-op counters are reset (op_begin) and
+// the sort's workspace words clr[0, nclr) are zeroed, saving two launches.
 __global__ void k_stage(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t V,
-                        uint32_t bV, int mirror, int swap, uint64_t *keys, Ctl *ctl) {
+                        uint32_t bV, int mirror, int swap, uint64_t *keys, Ctl *ctl,
+                        uint32_t *clr, uint64_t nclr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) op_reset(ctl);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nclr;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    clr[i] = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t u = src[i], v = dst[i];
@@ -188,10 +196,19 @@ __global__ void k_stage(const uint32_t *src, const uint32_t *dst, uint64_t n, ui
 uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst, uint64_t n,
                 int mirror, int swap, uint64_t &m, hipStream_t s) {
   m = mirror ? 2 * n : n;
-  if (!n) return sc.k0;
+  sc.sort_err = nullptr;
+  if (!n) {
+    op_begin(st, s);
+    return sc.k0;
+  }
   const uint32_t bV = id_bits(st.V);
-  k_stage<<<grid_for(n), 256, 0, s>>>(src, dst, n, st.V, bV, mirror, swap, sc.k0, st.ctl);
-  return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s, &st.ctl->abort);
+  const SortClear c = sort_clear_span(sc.tmp, m, 0, 2 * bV);
+  k_stage<<<grid_for(n), 256, 0, s>>>(src, dst, n, st.V, bV, mirror, swap, sc.k0, st.ctl, c.w,
+                                      c.words);
+  if (!c.err) return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s, &st.ctl->abort);
+  // the sort's failure word is folded into abort by segment (or abort_join)
+  sc.sort_err = c.err;
+  return sort_keys(sc.k0, sc.k1, m, 2 * bV, sc.tmp, s, nullptr, true);
 }
 
 // Group staging: key = ((u << bV | v) << sb) | batch, so after the sort the
@@ -265,6 +282,7 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
                       uint64_t n, const uint64_t *boff, const uint32_t *kinds, uint32_t nb,
                       int mirror, int swap, uint64_t &m, hipStream_t s) {
   m = mirror ? 2 * n : n;
+  sc.sort_err = nullptr;
   if (!n) return sc.k0;
   const uint32_t bV = id_bits(st.V), sb = id_bits(nb);
   k_stage_seq<<<grid_for(n), 256, 0, s>>>(src, dst, n, boff, nb, st.V, bV, sb, mirror, swap,
@@ -284,20 +302,27 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
 
 // Both stores of a directed graph stage the same batch; an abort in either
 // (a failed sort) must stop both before anything is written.
-__global__ void k_abort_join(Ctl *a, Ctl *b) {
+__global__ void k_abort_join(Ctl *a, Ctl *b, const uint32_t *err_a, const uint32_t *err_b) {
+  if (err_a && *err_a && !a->abort) a->abort = ABORT_SORT;
+  if (err_b && *err_b && !b->abort) b->abort = ABORT_SORT;
   const unsigned ab = a->abort ? a->abort : b->abort;
   if (!a->abort) a->abort = ab;
   if (!b->abort) b->abort = ab;
 }
-void abort_join(Store &a, Store &b, hipStream_t s) { k_abort_join<<<1, 1, 0, s>>>(a.ctl, b.ctl); }
+void abort_join(Store &a, Store &b, const Scratch &sa, const Scratch &sb, hipStream_t s) {
+  k_abort_join<<<1, 1, 0, s>>>(a.ctl, b.ctl, sa.sort_err, sb.sort_err);
+}
 
 // ---------------------------------------------------------- segmentation --
 // op (may be null): per-key operation of a merged group, OP_DELETE keys are
 // never insert candidates.  After an abort nothing is segmented: the keys of
 // a failed sort are arbitrary, and no later step may index with them.
+// serr (may be null): the staging sort's failure word, folded into abort here.
 __global__ void k_seg_mark(const uint64_t *keys, uint64_t m, uint32_t bV, uint32_t *head,
-                           uint32_t *flag, const uint32_t *op, Ctl *ctl) {
-  const bool ab = ctl->abort != 0;
+                           uint32_t *flag, const uint32_t *op, Ctl *ctl, const uint32_t *serr) {
+  const bool sfail = serr && *serr;
+  if (sfail && blockIdx.x == 0 && threadIdx.x == 0) atomicCAS(&ctl->abort, 0u, ABORT_SORT);
+  const bool ab = sfail || ctl->abort != 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t k = keys[i];
@@ -359,7 +384,8 @@ void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream
              const uint32_t *op) {
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
-  k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, op, st.ctl);
+  k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, op, st.ctl,
+                                         sc.sort_err);
   scan_u32(sc.segof, sc.segof, m, nullptr, sc.tmp, s);
   SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
   k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.sorted, st.ctl);
@@ -951,78 +977,82 @@ __device__ __forceinline__ bool cmp_wanted(const Store &st, uint32_t v, float fr
   return tomb > 0 && (frac <= 0.f || (float)tomb > frac * (float)us);
 }
 
-// Each block owns a contiguous vertex range: it counts its selections, takes
-// one base with a single atomic, then writes them in vertex order.
-__global__ __launch_bounds__(256) void k_cmp_select(Store st, float frac, uint32_t *vlist,
-                                                    Ctl *ctl) {
-  const uint64_t chunk = cdiv_dev(st.V, gridDim.x);
-  const uint64_t lo = blockIdx.x * chunk;
-  const uint64_t hi = lo + chunk < st.V ? lo + chunk : st.V;
-  uint32_t mine = 0;
-  for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) mine += cmp_wanted(st, (uint32_t)v, frac);
-  uint32_t tot = block_sum<uint32_t, 256>(mine);
-  __shared__ unsigned long long base;
-  if (threadIdx.x == 0) base = tot ? atomicAdd(&ctl->compacted, (unsigned long long)tot) : 0ull;
-  __syncthreads();
-  if (!tot) return;
-  unsigned long long b = base;
-  for (uint64_t t0 = lo; t0 < hi; t0 += blockDim.x) {
-    uint64_t v = t0 + threadIdx.x;
-    uint32_t sel = v < hi ? cmp_wanted(st, (uint32_t)v, frac) : 0u;
-    uint32_t ttot;
-    uint32_t ex = block_excl_scan<uint32_t, 256>(sel, ttot);
-    if (sel) vlist[b + ex] = (uint32_t)v;
-    b += ttot;
-  }
-}
-
-// One wavefront packs one vertex in place: round r reads 64 slots, keeps the
+// One wavefront packs vertex u in place: round r reads 64 slots, keeps the
 // live ones at the running write position (never ahead of the read
 // position), then the emptied tail blocks go back on the free stack.
 // Intent of compactionVertexCentric (delete_kernels.cu:50-261) without its
 // defects D5/D6.
+__device__ __forceinline__ void cmp_vertex(const Store &st, uint32_t u, uint32_t lane, Ctl *ctl) {
+  const uint32_t uo = st.used[u];
+  const uint32_t tbl = st.tbl_off[u];
+  // the live slots of the sorted prefix stay first and in order: they are
+  // the new prefix, and its chunks get fences from their new first slots
+  const uint32_t so = st.sorted[u];
+  uint32_t wpos = 0, nso = 0;
+  for (uint32_t base = 0; base < uo; base += WAVE) {
+    uint32_t pos = base + lane;
+    uint32_t w = TOMB;
+    if (pos < uo) w = st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)];
+    bool keep = w != TOMB;
+    unsigned long long m = __ballot(keep);
+    if (keep) {
+      uint32_t dp = wpos + mask_lt_count(m);
+      const uint32_t didx = slot_index(st.arena, tbl, dp, st.log2B, st.B);
+      st.slots[didx] = w;
+      if (pos < so && (dp & (FENCE_SLOTS - 1)) == 0) st.fence[didx / FENCE_SLOTS] = w;
+    }
+    const unsigned long long below =
+        so <= base ? 0ull : (so - base >= (uint32_t)WAVE ? ~0ull : (1ull << (so - base)) - 1);
+    nso += (uint32_t)__popcll(m & below);
+    wpos += (uint32_t)__popcll(m);
+  }
+  const uint32_t nbo = (uo + st.B - 1) >> st.log2B;
+  const uint32_t nbn = (wpos + st.B - 1) >> st.log2B;
+  const uint32_t nf = nbo - nbn;
+  unsigned long long fb = 0;
+  if (lane == 0 && nf) fb = atomicAdd(&ctl->free_top, (unsigned long long)nf);
+  fb = __shfl(fb, 0, WAVE);
+  for (uint32_t j = lane; j < nf; j += WAVE) st.free_ids[fb + j] = st.arena[tbl + nbn + j];
+  if (lane == 0) {
+    st.used[u] = wpos;
+    st.sorted[u] = nso;
+  }
+}
+
+// Whole-graph compaction in one launch: lane l of a wavefront checks vertex
+// base + l (coalesced reads of used and deg), then the wavefront packs the
+// vertices it selected one after another.  COUNT: their number is also added
+// to ctl->compacted (reset by op_begin), one atomic per block.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_cmp_all(Store st, float frac, Ctl *ctl) {
+  const uint32_t lane = lane_id();
+  const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  uint32_t mine = 0;
+  for (uint64_t base = w * WAVE; base < st.V; base += W * WAVE) {
+    const uint64_t v = base + lane;
+    const bool sel = v < st.V && cmp_wanted(st, (uint32_t)v, frac);
+    unsigned long long m = __ballot(sel);
+    mine += (uint32_t)__popcll(m);
+    while (m) {
+      const int j = __ffsll(m) - 1;
+      m &= m - 1;
+      cmp_vertex(st, (uint32_t)(base + j), lane, ctl);
+    }
+  }
+  if (COUNT) {
+    const uint32_t tot = block_sum<uint32_t, 256>(lane == 0 ? mine : 0u);
+    if (threadIdx.x == 0 && tot) atomicAdd(&ctl->compacted, (unsigned long long)tot);
+  }
+}
+
+// One wavefront per listed vertex (vlist[0, ctl->compacted)).
 __global__ __launch_bounds__(256) void k_cmp_run(Store st, const uint32_t *vlist, Ctl *ctl) {
   const uint64_t n = ctl->compacted;
   const uint32_t lane = lane_id();
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < n; i += nwaves) {
-    const uint32_t u = vlist[i];
-    const uint32_t uo = st.used[u];
-    const uint32_t tbl = st.tbl_off[u];
-    // the live slots of the sorted prefix stay first and in order: they are
-    // the new prefix, and its chunks get fences from their new first slots
-    const uint32_t so = st.sorted[u];
-    uint32_t wpos = 0, nso = 0;
-    for (uint32_t base = 0; base < uo; base += WAVE) {
-      uint32_t pos = base + lane;
-      uint32_t w = TOMB;
-      if (pos < uo) w = st.slots[slot_index(st.arena, tbl, pos, st.log2B, st.B)];
-      bool keep = w != TOMB;
-      unsigned long long m = __ballot(keep);
-      if (keep) {
-        uint32_t dp = wpos + mask_lt_count(m);
-        const uint32_t didx = slot_index(st.arena, tbl, dp, st.log2B, st.B);
-        st.slots[didx] = w;
-        if (pos < so && (dp & (FENCE_SLOTS - 1)) == 0) st.fence[didx / FENCE_SLOTS] = w;
-      }
-      const unsigned long long below =
-          so <= base ? 0ull : (so - base >= (uint32_t)WAVE ? ~0ull : (1ull << (so - base)) - 1);
-      nso += (uint32_t)__popcll(m & below);
-      wpos += (uint32_t)__popcll(m);
-    }
-    const uint32_t nbo = (uo + st.B - 1) >> st.log2B;
-    const uint32_t nbn = (wpos + st.B - 1) >> st.log2B;
-    const uint32_t nf = nbo - nbn;
-    unsigned long long fb = 0;
-    if (lane == 0 && nf) fb = atomicAdd(&ctl->free_top, (unsigned long long)nf);
-    fb = __shfl(fb, 0, WAVE);
-    for (uint32_t j = lane; j < nf; j += WAVE) st.free_ids[fb + j] = st.arena[tbl + nbn + j];
-    if (lane == 0) {
-      st.used[u] = wpos;
-      st.sorted[u] = nso;
-      if (nf) atomicAdd(&ctl->blocks_freed, (unsigned long long)nf);
-    }
-  }
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < n; i += nwaves)
+    cmp_vertex(st, vlist[i], lane, ctl);
 }
 
 // Same selection over the sources of the last staged batch only (segments
@@ -1055,9 +1085,14 @@ __global__ __launch_bounds__(256) void k_cmp_select_touched(Store st, float frac
   }
 }
 
-void compact(Store &st, float frac, uint32_t *vlist, hipStream_t s) {
-  k_cmp_select<<<grid_for(st.V, 256, 512), 256, 0, s>>>(st, frac, vlist, st.ctl);
-  k_cmp_run<<<probe_grid(st.V), 256, 0, s>>>(st, vlist, st.ctl);
+void compact(Store &st, float frac, bool count, hipStream_t s) {
+  const unsigned grid = grid_for(st.V, 256 * 8, 2048);
+  if (count) {
+    op_begin(st, s);
+    k_cmp_all<true><<<grid, 256, 0, s>>>(st, frac, st.ctl);
+  } else {
+    k_cmp_all<false><<<grid, 256, 0, s>>>(st, frac, st.ctl);
+  }
 }
 
 void compact_touched(Store &st, const Scratch &sc, uint64_t m, float frac, uint32_t *vlist,
