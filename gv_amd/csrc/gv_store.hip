@@ -428,8 +428,9 @@ constexpr uint32_t PROBE_SMALL_KEYS = 4;  // key indices packed in two bits
 // A wavefront-wide item whose source has at most this many batch keys
 // compares each slot with every key.
 constexpr uint32_t PROBE_FEW_KEYS = 8;
-// A fence item looks its keys' chunks up when its source has at most one
-// batch key per PROBE_SPARSE slots of sorted prefix; denser, it is scanned.
+// A fence item looks its keys' chunks up when its source has more than 64
+// batch keys (a scan would search memory for every slot) or at most one per
+// PROBE_SPARSE slots of sorted prefix; otherwise it is scanned.
 constexpr uint32_t PROBE_SPARSE = 32;
 
 // First key of the source's sorted batch segment (keys[bks, bks + k), k > 64)
@@ -734,12 +735,12 @@ __global__ __launch_bounds__(256, 8) void k_probe(Store st, ProbeArgs a, uint32_
         sv = (uint32_t)(a.keys[bks + (uint32_t)(((uint64_t)lane * k) >> 6)] & mask);
       }
       uint32_t cnt = 0;
-      if (bfit && (uint64_t)k * PROBE_SPARSE <= bso) {
+      if (bfit && (k > (uint32_t)WAVE || (uint64_t)k * PROBE_SPARSE <= bso)) {
         probe_fences<MODE>(st, a, mask, (uint64_t)bu << bV, lane, blo, bhi, bso, btbl, bks, k,
                            sv, cnt);
       } else if (bfit) {
-        // keys dense in the prefix: reading every chunk is cheaper than
-        // looking each key's chunk up
+        // at most 64 keys (held in registers), dense in the prefix: reading
+        // every chunk costs no more than looking each key's chunk up
         for (uint32_t q0 = blo; q0 < bhi; q0 += PROBE_CHUNK)
           probe_scan<MODE>(st, a, bV, bu, lane, q0, q0 + PROBE_CHUNK < bhi ? q0 + PROBE_CHUNK : bhi,
                            btbl, bks, bke, k, sv, cnt);
