@@ -361,13 +361,12 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
     uint64_t *keys[2] = {nullptr, nullptr};
     for (int k = 0; k < nst; ++k) {
       const int mirror = g->directed ? 0 : 1;
-      op_begin(g->st[k], s);
       keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, g->grp_off, g->grp_kind, nb, mirror,
                             k == 1 ? 1 : 0, m[k], s);
     }
     if (nst > 1) abort_join(g->st[0], g->st[1], g->sc[0], g->sc[1], s);
     for (int k = 0; k < nst; ++k) {
-      segment(g->st[k], g->sc[k], keys[k], m[k], s, g->sc[k].op);
+      segment_marked(g->st[k], g->sc[k], keys[k], m[k], s);
       mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
     }
     commit_pair(g->st[0], g->sc[0], nst > 1 ? &g->st[1] : nullptr, nst > 1 ? &g->sc[1] : nullptr,

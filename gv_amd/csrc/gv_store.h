@@ -49,9 +49,11 @@ void abort_join(Store &a, Store &b, const Scratch &sa, const Scratch &sb, hipStr
 // fold into the abort code.
 uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst, uint64_t n,
                 int mirror, int swap, uint64_t &m, hipStream_t s);
-// Stage a group of nb batches (entries [boff[b], boff[b+1]) of batch b, op
+// Start an operation (as op_begin) and stage a group of nb batches (entries
+// [boff[b], boff[b+1]) of batch b, op
 // kinds[b]; both device arrays) as unique sorted keys with their final op in
-// sc.op.  Returns the key buffer (m keys, sentinel-padded).
+// sc.op and their segment marks (for segment_marked).  Returns the key
+// buffer (m keys, sentinel-padded).
 uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst,
                       uint64_t n, const uint64_t *boff, const uint32_t *kinds, uint32_t nb,
                       int mirror, int swap, uint64_t &m, hipStream_t s);
@@ -59,6 +61,9 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
 // operation of a merged group, or null).
 void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s,
              const uint32_t *op = nullptr);
+// The same after stage_group, which has already written the keys' segment
+// marks (sc.segof heads, sc.flag insert candidates, n_valid).
+void segment_marked(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Merged group: probe (tombstone / mark present), fix degrees, plan inserts.
 void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Survivor scan + per-segment block/table plan (after an insert probe).

@@ -214,15 +214,36 @@ uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst
 // Group staging: key = ((u << bV | v) << sb) | batch, so after the sort the
 // entries of one edge sit together in batch order and the last one holds the
 // edge's final operation.
+// Like k_stage, also starts the operation and clears the sort's workspace.
+constexpr uint32_t STAGE_SEQ_LDS = 2048;
 __global__ void k_stage_seq(const uint32_t *src, const uint32_t *dst, uint64_t n,
                             const uint64_t *boff, uint32_t nb, uint32_t V, uint32_t bV,
-                            uint32_t sb, int mirror, int swap, uint64_t *keys, Ctl *ctl) {
+                            uint32_t sb, int mirror, int swap, uint64_t *keys, Ctl *ctl,
+                            uint32_t *clr, uint64_t nclr) {
+  // the batch offsets in LDS when they fit (the search below runs per key)
+  __shared__ uint64_t s_off[STAGE_SEQ_LDS];
+  const bool in_lds = nb < STAGE_SEQ_LDS;
+  if (in_lds) {
+    for (uint32_t k = threadIdx.x; k <= nb; k += blockDim.x) s_off[k] = boff[k];
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) op_reset(ctl);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nclr;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    clr[i] = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t lo = 0, hi = nb;  // largest b with boff[b] <= i
-    while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (boff[mid] <= i) lo = mid; else hi = mid;
+    if (in_lds) {
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= i) lo = mid; else hi = mid;
+      }
+    } else {
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (boff[mid] <= i) lo = mid; else hi = mid;
+      }
     }
     uint32_t u = src[i], v = dst[i];
     if (swap) {
@@ -260,21 +281,35 @@ __global__ void k_last_mark(const uint64_t *keys, uint64_t m, uint32_t sb, uint3
 }
 
 // Keep the last entry of every edge (without its batch tag) and its op; pad
-// the tail with the sentinel so the array stays m long.
-__global__ void k_last_compact(const uint64_t *keys, uint64_t m, uint32_t sb,
+// the tail with the sentinel so the array stays m long.  The segment marks
+// of the kept keys are written here too (what k_seg_mark would compute from
+// them, for segment_marked): head = first key of its source, flag = an
+// insert (no duplicates are left), and n_valid.
+__global__ void k_last_compact(const uint64_t *keys, uint64_t m, uint32_t sb, uint32_t bV,
                                const uint32_t *pos, const uint32_t *kinds, uint64_t *out,
-                               uint32_t *op, const Ctl *ctl) {
+                               uint32_t *op, uint32_t *head, uint32_t *flag, Ctl *ctl) {
   const uint32_t total = pos[m];
   const uint64_t smask = (1ull << sb) - 1;
   const bool ab = ctl->abort != 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_valid = total;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (!ab && last_of_edge(keys, m, i, sb)) {
-      uint64_t k = keys[i];
-      out[pos[i]] = k >> sb;
-      op[pos[i]] = kinds[k & smask];
+      const uint64_t k = keys[i], e = k >> sb;
+      const uint32_t p = pos[i], o = kinds[k & smask];
+      out[p] = e;
+      op[p] = o;
+      // the entry before this edge's first one is the previous edge's last
+      uint64_t j = i;
+      while (j > 0 && (keys[j - 1] >> sb) == e) --j;
+      head[p] = j == 0 || ((keys[j - 1] >> sb) >> bV) != (e >> bV) ? 1u : 0u;
+      flag[p] = o == OP_INSERT ? 1u : 0u;
     }
-    if (i >= total) out[i] = KEY_SENT;
+    if (i >= total) {
+      out[i] = KEY_SENT;
+      head[i] = 0;
+      flag[i] = 0;
+    }
   }
 }
 
@@ -283,20 +318,24 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
                       int mirror, int swap, uint64_t &m, hipStream_t s) {
   m = mirror ? 2 * n : n;
   sc.sort_err = nullptr;
-  if (!n) return sc.k0;
+  if (!n) {
+    op_begin(st, s);
+    return sc.k0;
+  }
   const uint32_t bV = id_bits(st.V), sb = id_bits(nb);
+  const SortClear c = sort_clear_span(sc.tmp, m, sb, 2 * bV + sb);
   k_stage_seq<<<grid_for(n), 256, 0, s>>>(src, dst, n, boff, nb, st.V, bV, sb, mirror, swap,
-                                          sc.k0, st.ctl);
+                                          sc.k0, st.ctl, c.w, c.words);
   // the keys are staged in batch order, so a stable sort on the (u, v) bits
   // alone leaves equal edges in batch order: the same array as a sort on the
   // whole key, with one pass fewer when the tag crosses a digit boundary
-  uint64_t *sorted =
-      sort_keys_range(sc.k0, sc.k1, m, sb, 2 * bV + sb, sc.tmp, s, &st.ctl->abort);
+  uint64_t *sorted = sort_keys_range(sc.k0, sc.k1, m, sb, 2 * bV + sb, sc.tmp, s,
+                                     &st.ctl->abort, c.words != 0);
   uint64_t *other = sorted == sc.k0 ? sc.k1 : sc.k0;
   k_last_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.segof, st.ctl);
   scan_u32(sc.segof, sc.rank, m, nullptr, sc.tmp, s);
-  k_last_compact<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.rank, kinds, other, sc.op,
-                                             st.ctl);
+  k_last_compact<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, bV, sc.rank, kinds, other, sc.op,
+                                             sc.segof, sc.flag, st.ctl);
   return other;
 }
 
@@ -386,6 +425,12 @@ void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream
   const uint32_t bV = id_bits(st.V);
   k_seg_mark<<<grid_for(m), 256, 0, s>>>(keys, m, bV, sc.segof, sc.flag, op, st.ctl,
                                          sc.sort_err);
+  segment_marked(st, sc, keys, m, s);
+}
+
+void segment_marked(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
+  if (!m) return;
+  const uint32_t bV = id_bits(st.V);
   scan_u32(sc.segof, sc.segof, m, nullptr, sc.tmp, s);
   SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
   k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.sorted, st.ctl);
