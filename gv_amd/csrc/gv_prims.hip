@@ -232,9 +232,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
     rnk[j] = old + below;
   }
   __syncthreads();
+  // the tile is staged in LDS in digit order (local position = digit start
+  // in the tile + the wave's offset within the digit + rank), then written
+  // out in LDS order, so consecutive threads store consecutive addresses of
+  // a digit's run instead of one scattered word each
+  __shared__ uint64_t s_keys[SORT_TILE];
+  __shared__ uint32_t s_ls[256], s_gb[256];
   {
     const uint32_t d = threadIdx.x;  // SORT_THREADS == 256 digits
-    uint32_t acc;
+    uint32_t gbase;
     if (RAW_HIST) {
       uint32_t tot = 0, below = 0;
       for (uint64_t t = 0; t < nt; ++t) {
@@ -243,16 +249,21 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
         if (t < blockIdx.x) below += c;
       }
       uint32_t all;
-      acc = block_excl_scan<uint32_t, SORT_THREADS>(tot, all) + below;
+      gbase = block_excl_scan<uint32_t, SORT_THREADS>(tot, all) + below;
     } else {
-      acc = offs[(uint64_t)d * nt + blockIdx.x];
+      gbase = offs[(uint64_t)d * nt + blockIdx.x];
     }
+    uint32_t c = 0;
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
-      uint32_t c = wcnt[i][d];
-      wcnt[i][d] = acc;
-      acc += c;
+      const uint32_t x = wcnt[i][d];
+      wcnt[i][d] = c;
+      c += x;
     }
+    uint32_t all;
+    const uint32_t ls = block_excl_scan<uint32_t, SORT_THREADS>(c, all);
+    s_ls[d] = ls;
+    s_gb[d] = gbase - ls;  // modulo 2^32: s_gb[d] + local position is the output index
   }
   __syncthreads();
 #pragma unroll
@@ -260,7 +271,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(const uint64_t *
     uint64_t i = base + (uint64_t)j * WAVE + lane;
     if (i < n) {
       uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-      out[wcnt[w][d] + rnk[j]] = key[j];
+      s_keys[s_ls[d] + wcnt[w][d] + rnk[j]] = key[j];
+    }
+  }
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * SORT_TILE;
+  const uint32_t tn = n - t0 < SORT_TILE ? (uint32_t)(n - t0) : (uint32_t)SORT_TILE;
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    const uint32_t p = (uint32_t)j * SORT_THREADS + threadIdx.x;
+    if (p < tn) {
+      const uint64_t k = s_keys[p];
+      out[s_gb[(uint32_t)(k >> shift) & 255u] + p] = k;
     }
   }
 }
