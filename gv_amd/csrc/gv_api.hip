@@ -366,7 +366,7 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
     }
     if (nst > 1) abort_join(g->st[0], g->st[1], g->sc[0], g->sc[1], s);
     for (int k = 0; k < nst; ++k) {
-      segment_marked(g->st[k], g->sc[k], keys[k], m[k], s);
+      segment_items(g->st[k], g->sc[k], m[k], s);
       mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
     }
     commit_pair(g->st[0], g->sc[0], nst > 1 ? &g->st[1] : nullptr, nst > 1 ? &g->sc[1] : nullptr,

@@ -74,6 +74,43 @@ __global__ __launch_bounds__(SPINE_THREADS) void k_scan_spine(T *tile_sums, uint
   if (threadIdx.x == 0) tile_sums[nt] = carry;
 }
 
+// A thread's SCAN_ITEMS consecutive elements through 16-byte accesses when
+// the tile lies wholly inside the array and both arrays are 16-byte aligned
+// (vec); element by element otherwise.
+template <typename T>
+__device__ __forceinline__ void scan_load8(const T *in, uint64_t mine, uint64_t ne, bool vec,
+                                           T (&v)[SCAN_ITEMS]) {
+  static_assert(SCAN_ITEMS * sizeof(T) % 16 == 0, "whole 16-byte groups");
+  if (vec) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + mine);
+#pragma unroll
+    for (int r = 0; r < (int)(SCAN_ITEMS * sizeof(T) / 16); ++r) {
+      const uint4 q = p[r];
+      __builtin_memcpy(reinterpret_cast<uint8_t *>(v) + 16 * r, &q, 16);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) v[j] = mine + j < ne ? in[mine + j] : 0;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void scan_store8(T *out, uint64_t mine, uint64_t ne, bool vec,
+                                            const T (&v)[SCAN_ITEMS]) {
+  if (vec) {
+    uint4 *p = reinterpret_cast<uint4 *>(out + mine);
+#pragma unroll
+    for (int r = 0; r < (int)(SCAN_ITEMS * sizeof(T) / 16); ++r) {
+      uint4 q;
+      __builtin_memcpy(&q, reinterpret_cast<const uint8_t *>(v) + 16 * r, 16);
+      p[r] = q;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+      if (mine + j < ne) out[mine + j] = v[j];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T *in, T *out, uint64_t n,
                                                              const uint32_t *n_dev,
@@ -83,23 +120,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T *in, T *out
   if (blockIdx.x == 0 && threadIdx.x == 0) out[ne] = tile_sums[nt];
   if (base >= ne) return;
   // blocked: thread t owns [base + t*8, base + t*8 + 8)
+  const bool vec = base + SCAN_TILE <= ne && ((((uintptr_t)in) | ((uintptr_t)out)) & 15) == 0;
   T v[SCAN_ITEMS];
-  T s = 0;
   const uint64_t mine = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  scan_load8<T>(in, mine, ne, vec, v);
+  T s = 0;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    uint64_t i = mine + j;
-    v[j] = i < ne ? in[i] : 0;
-    s += v[j];
-  }
+  for (int j = 0; j < SCAN_ITEMS; ++j) s += v[j];
   T tot;
   T ex = block_excl_scan<T, SCAN_THREADS>(s, tot) + tile_sums[blockIdx.x];
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    uint64_t i = mine + j;
-    if (i < ne) out[i] = ex;
-    ex += v[j];
+    const T x = v[j];
+    v[j] = ex;
+    ex += x;
   }
+  scan_store8<T>(out, mine, ne, vec, v);
 }
 
 // Up to SCAN_FUSED_TILES tiles: the apply kernel adds up the tile sums
@@ -127,23 +163,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply_fused(const T *in, 
   all = block_sum<T, SCAN_THREADS>(all);
   if (blockIdx.x == 0 && threadIdx.x == 0) out[ne] = all;
   if (base >= ne) return;
+  const bool vec = base + SCAN_TILE <= ne && ((((uintptr_t)in) | ((uintptr_t)out)) & 15) == 0;
   T v[SCAN_ITEMS];
-  T s = 0;
   const uint64_t mine = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  scan_load8<T>(in, mine, ne, vec, v);
+  T s = 0;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    uint64_t i = mine + j;
-    v[j] = i < ne ? in[i] : 0;
-    s += v[j];
-  }
+  for (int j = 0; j < SCAN_ITEMS; ++j) s += v[j];
   T tot;
   T ex = block_excl_scan<T, SCAN_THREADS>(s, tot) + before;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    uint64_t i = mine + j;
-    if (i < ne) out[i] = ex;
-    ex += v[j];
+    const T x = v[j];
+    v[j] = ex;
+    ex += x;
   }
+  scan_store8<T>(out, mine, ne, vec, v);
 }
 
 size_t scan_tmp_bytes(uint64_t n) { return (cdiv(n ? n : 1, SCAN_TILE) + 1) * sizeof(uint64_t); }

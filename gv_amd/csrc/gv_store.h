@@ -52,8 +52,9 @@ uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst
 // Start an operation (as op_begin) and stage a group of nb batches (entries
 // [boff[b], boff[b+1]) of batch b, op
 // kinds[b]; both device arrays) as unique sorted keys with their final op in
-// sc.op and their segment marks (for segment_marked).  Returns the key
-// buffer (m keys, sentinel-padded).
+// sc.op, insert flags in sc.flag and their source segments built (as
+// segment_marked leaves them; segment_items then lists the probe items).
+// Returns the key buffer (m keys, sentinel-padded).
 uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst,
                       uint64_t n, const uint64_t *boff, const uint32_t *kinds, uint32_t nb,
                       int mirror, int swap, uint64_t &m, hipStream_t s);
@@ -64,6 +65,8 @@ void segment(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream
 // The same after stage_group, which has already written the keys' segment
 // marks (sc.segof heads, sc.flag insert candidates, n_valid).
 void segment_marked(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
+// The probe work list of segments already built (n_segs, seg_cnt).
+void segment_items(Store &st, Scratch &sc, uint64_t m, hipStream_t s);
 // Merged group: probe (tombstone / mark present), fix degrees, plan inserts.
 void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s);
 // Survivor scan + per-segment block/table plan (after an insert probe).

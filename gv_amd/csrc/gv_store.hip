@@ -264,50 +264,173 @@ __global__ void k_stage_seq(const uint32_t *src, const uint32_t *dst, uint64_t n
   }
 }
 
-__device__ __forceinline__ bool last_of_edge(const uint64_t *keys, uint64_t m, uint64_t i,
-                                             uint32_t sb) {
-  uint64_t k = keys[i];
-  if (k == KEY_SENT) return false;
-  return i + 1 == m || (keys[i + 1] >> sb) != (k >> sb);
+// ------------------------------------------ group selection + segments --
+// After the group sort, index i of the sorted keys is kept when it is the
+// last entry of its edge (the edge's final op); a kept key heads a segment
+// when no kept key of its source comes before it.  Two passes over tiles of
+// GS_TILE sorted keys do the selection and the segmentation: k_gs_count
+// counts each tile's kept keys and heads, one scan over the tiles gives
+// their bases, and k_gs_write writes the kept keys (tag removed), their ops,
+// insert flags and segment numbers, and each segment's description.  Row j
+// of wave w of tile t holds indexes t * GS_TILE + j * 256 + w * 64 + lane,
+// so a key's neighbours in index order are its neighbour lanes.
+constexpr int GS_ROWS = 16;
+constexpr uint64_t GS_TILE = 256 * GS_ROWS;
+
+// A tile's keys are loaded before any flag is evaluated; the neighbour keys
+// that the shuffles of a row cannot supply (lane 63: key i + 1, lane 0: key
+// i - 1) come from the first and last keys of every (row, wave) cell,
+// exchanged through LDS, and the keys just outside the tile.
+struct GsEdges {
+  uint64_t first[GS_ROWS * 4], last[GS_ROWS * 4], before, after;
+};
+
+__device__ __forceinline__ void gs_load_tile(const uint64_t *keys, uint64_t m, uint64_t t0,
+                                             uint32_t lane, uint64_t (&k)[GS_ROWS],
+                                             GsEdges &ed) {
+  const uint32_t w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int j = 0; j < GS_ROWS; ++j) {
+    const uint64_t i = t0 + (uint64_t)j * 256 + threadIdx.x;
+    k[j] = i < m ? keys[i] : KEY_SENT;
+  }
+  if (threadIdx.x == 0) ed.before = t0 > 0 && t0 < m ? keys[t0 - 1] : KEY_SENT;
+  if (threadIdx.x == 1) ed.after = t0 + GS_TILE < m ? keys[t0 + GS_TILE] : KEY_SENT;
+#pragma unroll
+  for (int j = 0; j < GS_ROWS; ++j) {
+    if (lane == 0) ed.first[j * 4 + w] = k[j];
+    if (lane == WAVE - 1) ed.last[j * 4 + w] = k[j];
+  }
+  __syncthreads();
 }
 
-// After an abort (a failed sort leaves arbitrary keys) no key is kept.
-__global__ void k_last_mark(const uint64_t *keys, uint64_t m, uint32_t sb, uint32_t *flag,
-                            const Ctl *ctl) {
-  const bool ab = ctl->abort != 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    flag[i] = !ab && last_of_edge(keys, m, i, sb) ? 1u : 0u;
+// The edge-lane neighbour of row j: key i + 1 for lane 63, key i - 1 for lane 0.
+__device__ __forceinline__ uint64_t gs_edge(const GsEdges &ed, int j, uint32_t lane) {
+  const uint32_t c = (uint32_t)j * 4 + threadIdx.x / WAVE;
+  if (lane == WAVE - 1) return c + 1 < GS_ROWS * 4 ? ed.first[c + 1] : ed.after;
+  return c > 0 ? ed.last[c - 1] : ed.before;
 }
 
-// Keep the last entry of every edge (without its batch tag) and its op; pad
-// the tail with the sentinel so the array stays m long.  The segment marks
-// of the kept keys are written here too (what k_seg_mark would compute from
-// them, for segment_marked): head = first key of its source, flag = an
-// insert (no duplicates are left), and n_valid.
-__global__ void k_last_compact(const uint64_t *keys, uint64_t m, uint32_t sb, uint32_t bV,
-                               const uint32_t *pos, const uint32_t *kinds, uint64_t *out,
-                               uint32_t *op, uint32_t *head, uint32_t *flag, Ctl *ctl) {
-  const uint32_t total = pos[m];
-  const uint64_t smask = (1ull << sb) - 1;
-  const bool ab = ctl->abort != 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_valid = total;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    if (!ab && last_of_edge(keys, m, i, sb)) {
-      const uint64_t k = keys[i], e = k >> sb;
-      const uint32_t p = pos[i], o = kinds[k & smask];
-      out[p] = e;
-      op[p] = o;
-      // the entry before this edge's first one is the previous edge's last
-      uint64_t j = i;
-      while (j > 0 && (keys[j - 1] >> sb) == e) --j;
-      head[p] = j == 0 || ((keys[j - 1] >> sb) >> bV) != (e >> bV) ? 1u : 0u;
-      flag[p] = o == OP_INSERT ? 1u : 0u;
+// Flags of index i (every lane of the wavefront calls it): keep = last entry
+// of its edge, head = kept and the entry before the edge's first one has
+// another source.  After an abort (a failed sort leaves arbitrary keys)
+// nothing is kept.
+__device__ __forceinline__ void gs_flags(const uint64_t *keys, uint64_t m, uint64_t i,
+                                         uint32_t sb, uint32_t bV, bool ab, uint32_t lane,
+                                         uint64_t k, uint64_t x, bool &keep, bool &head) {
+  uint64_t nxt = __shfl_down(k, 1, WAVE);
+  if (lane == WAVE - 1) nxt = x;
+  uint64_t prv = __shfl_up(k, 1, WAVE);
+  if (lane == 0) prv = x;
+  const uint64_t e = k >> sb;
+  keep = !ab && i < m && k != KEY_SENT && (i + 1 == m || (nxt >> sb) != e);
+  head = false;
+  if (keep) {
+    // walk back over earlier entries of the same edge (repeats across batches)
+    uint64_t j = i, before = i > 0 ? prv : KEY_SENT;
+    while (j > 0 && (before >> sb) == e) {
+      --j;
+      before = j > 0 ? keys[j - 1] : KEY_SENT;
     }
-    if (i >= total) {
+    head = j == 0 || (before >> (sb + bV)) != (e >> bV);
+  }
+}
+
+// tcnt[t] = (kept keys of tile t) << 32 | (heads of tile t)
+__global__ __launch_bounds__(256) void k_gs_count(const uint64_t *keys, uint64_t m, uint32_t sb,
+                                                  uint32_t bV, const Ctl *ctl, uint64_t *tcnt) {
+  __shared__ GsEdges ed;
+  const bool ab = ctl->abort != 0;
+  const uint32_t lane = lane_id();
+  const uint64_t t0 = (uint64_t)blockIdx.x * GS_TILE, base = t0 + threadIdx.x;
+  uint64_t k[GS_ROWS];
+  gs_load_tile(keys, m, t0, lane, k, ed);
+  uint32_t nk = 0, nh = 0;
+#pragma unroll
+  for (int j = 0; j < GS_ROWS; ++j) {
+    bool keep, head;
+    gs_flags(keys, m, base + (uint64_t)j * 256, sb, bV, ab, lane, k[j], gs_edge(ed, j, lane),
+             keep, head);
+    nk += keep ? 1u : 0u;
+    nh += head ? 1u : 0u;
+  }
+  const unsigned long long t =
+      block_sum<unsigned long long, 256>(((unsigned long long)nk << 32) | nh);
+  if (threadIdx.x == 0) tcnt[blockIdx.x] = t;
+}
+
+struct SegArgs {
+  uint32_t *segof, *seg_start, *seg_src, *seg_used, *seg_cnt, *seg_blk;
+};
+
+// toff: exclusive scan of tcnt over the nt tiles, toff[nt] = the totals.
+// Writes, for the kept key of rank p: out[p] (tag removed), op[p], flag[p]
+// (an insert: no duplicates are left) and segof[p] (its segment); per
+// segment: first key, source, used slots, probe items; n_valid and n_segs.
+// The rest of out is padded with the sentinel.
+__global__ __launch_bounds__(256) void k_gs_write(const uint64_t *keys, uint64_t m, uint32_t sb,
+                                                  uint32_t bV, const uint64_t *toff, uint64_t nt,
+                                                  const uint32_t *kinds, uint64_t *out,
+                                                  uint32_t *op, uint32_t *flag, SegArgs a,
+                                                  const uint32_t *used, const uint32_t *sorted,
+                                                  Ctl *ctl) {
+  __shared__ uint32_t cell[GS_ROWS * 4];  // per (row, wave): kept << 16 | heads
+  __shared__ GsEdges ed;
+  const bool ab = ctl->abort != 0;
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+  const uint64_t t0 = (uint64_t)blockIdx.x * GS_TILE, base = t0 + threadIdx.x;
+  const uint64_t smask = (1ull << sb) - 1;
+  uint64_t key[GS_ROWS];
+  gs_load_tile(keys, m, t0, lane, key, ed);
+  uint32_t bits = 0;  // bit j: row j kept, bit GS_ROWS + j: row j heads a segment
+#pragma unroll
+  for (int j = 0; j < GS_ROWS; ++j) {
+    bool keep, head;
+    gs_flags(keys, m, base + (uint64_t)j * 256, sb, bV, ab, lane, key[j], gs_edge(ed, j, lane),
+             keep, head);
+    bits |= (keep ? 1u << j : 0u) | (head ? 1u << (GS_ROWS + j) : 0u);
+    const unsigned long long mk = __ballot(keep), mh = __ballot(head);
+    if (lane == 0) cell[j * 4 + w] = ((uint32_t)__popcll(mk) << 16) | (uint32_t)__popcll(mh);
+  }
+  __syncthreads();
+  // exclusive prefix of the 64 cells in index order (row-major), lane c = cell c
+  const uint32_t c = cell[lane];
+  const uint32_t cex = wave_incl_scan<uint32_t>(c) - c;
+  const uint64_t tb = toff[blockIdx.x], tot = toff[nt];
+  const uint32_t K = (uint32_t)(tot >> 32), H = (uint32_t)tot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->n_valid = K;
+    ctl->n_segs = H;
+    a.seg_start[H] = K;
+  }
+  // the keys are read again (cache hits) rather than held across the loop
+#pragma unroll 4
+  for (int j = 0; j < GS_ROWS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 256;
+    const uint32_t ex = __shfl(cex, j * 4 + (int)w, WAVE);
+    const bool keep = (bits >> j) & 1u, head = (bits >> (GS_ROWS + j)) & 1u;
+    const unsigned long long mk = __ballot(keep), mh = __ballot(head);
+    if (keep) {
+      const uint32_t pk = (uint32_t)(tb >> 32) + (ex >> 16) + mask_lt_count(mk);
+      const uint32_t ph = (uint32_t)tb + (ex & 0xFFFFu) + mask_lt_count(mh);  // heads before
+      const uint64_t kk = keys[i];
+      const uint64_t e = kk >> sb;
+      const uint32_t o = kinds[kk & smask];
+      out[pk] = e;
+      op[pk] = o;
+      flag[pk] = o == OP_INSERT ? 1u : 0u;
+      a.segof[pk] = head ? ph : ph - 1u;
+      if (head) {
+        const uint32_t u = (uint32_t)(e >> bV), uo = used[u];
+        a.seg_start[ph] = pk;
+        a.seg_src[ph] = u;
+        a.seg_used[ph] = uo;
+        a.seg_cnt[ph] = probe_items(uo, sorted[u]);
+        a.seg_blk[ph] = 0;
+      }
+    }
+    if (i < m && i >= K) {
       out[i] = KEY_SENT;
-      head[i] = 0;
       flag[i] = 0;
     }
   }
@@ -332,10 +455,13 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
   uint64_t *sorted = sort_keys_range(sc.k0, sc.k1, m, sb, 2 * bV + sb, sc.tmp, s,
                                      &st.ctl->abort, c.words != 0);
   uint64_t *other = sorted == sc.k0 ? sc.k1 : sc.k0;
-  k_last_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, sc.segof, st.ctl);
-  scan_u32(sc.segof, sc.rank, m, nullptr, sc.tmp, s);
-  k_last_compact<<<grid_for(m), 256, 0, s>>>(sorted, m, sb, bV, sc.rank, kinds, other, sc.op,
-                                             sc.segof, sc.flag, st.ctl);
+  // tile counts in sc.plan (cap + 1 words; the insert plan is formed later)
+  const uint64_t nt = cdiv(m, GS_TILE);
+  k_gs_count<<<dim3((unsigned)nt), 256, 0, s>>>(sorted, m, sb, bV, st.ctl, sc.plan);
+  scan_u64(sc.plan, sc.plan, nt, nullptr, sc.tmp, s);
+  SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
+  k_gs_write<<<dim3((unsigned)nt), 256, 0, s>>>(sorted, m, sb, bV, sc.plan, nt, kinds, other,
+                                                sc.op, sc.flag, a, st.used, st.sorted, st.ctl);
   return other;
 }
 
@@ -377,10 +503,6 @@ __global__ void k_seg_mark(const uint64_t *keys, uint64_t m, uint32_t bV, uint32
     }
   }
 }
-
-struct SegArgs {
-  uint32_t *segof, *seg_start, *seg_src, *seg_used, *seg_cnt, *seg_blk;
-};
 
 __global__ void k_seg_build(const uint64_t *keys, uint64_t m, uint32_t bV, SegArgs a,
                             const uint32_t *used, const uint32_t *sorted, Ctl *ctl) {
@@ -434,6 +556,11 @@ void segment_marked(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hi
   scan_u32(sc.segof, sc.segof, m, nullptr, sc.tmp, s);
   SegArgs a{sc.segof, sc.seg_start, sc.seg_src, sc.seg_used, sc.seg_cnt, sc.seg_blk};
   k_seg_build<<<grid_for(m), 256, 0, s>>>(keys, m, bV, a, st.used, st.sorted, st.ctl);
+  segment_items(st, sc, m, s);
+}
+
+void segment_items(Store &st, Scratch &sc, uint64_t m, hipStream_t s) {
+  if (!m) return;
   scan_u32(sc.seg_cnt, sc.item_start, m, &st.ctl->n_segs, sc.tmp, s);
   k_item_expand<<<grid_for(m), 256, 0, s>>>(sc.seg_cnt, sc.item_start, sc.item_seg, st.ctl);
 }
@@ -986,7 +1113,7 @@ __global__ void k_del_finish(Store st, const uint32_t *seg_src, const uint32_t *
     if (d) st.deg[seg_src[sidx]] -= d;
     mine += d;
   }
-  // one atomic pair per block (the grid is capped at a few hundred blocks)
+  // one atomic pair per block (the grid is capped at 2048 blocks)
   unsigned long long tot = block_sum<unsigned long long, 256>(mine);
   if (threadIdx.x == 0 && tot) {
     atomicAdd(&ctl->applied, tot);
@@ -1000,7 +1127,7 @@ void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipS
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, nullptr,
                sc.seg_blk, nullptr};
   k_probe<PROBE_DELETE><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
-  k_del_finish<<<grid_for(m, 256, 256), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+  k_del_finish<<<grid_for(m, 256, 2048), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
 }
 
 // Merged group: one probe pass tombstones keys whose last op is a delete and
@@ -1012,7 +1139,7 @@ void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStr
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, sc.flag,
                sc.seg_blk, sc.op};
   k_probe<PROBE_MIXED><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
-  k_del_finish<<<grid_for(m, 256, 256), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+  k_del_finish<<<grid_for(m, 256, 2048), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
   insert_alloc(st, sc, m, s);
 }
 
@@ -1143,7 +1270,7 @@ void compact(Store &st, float frac, bool count, hipStream_t s) {
 
 void compact_touched(Store &st, const Scratch &sc, uint64_t m, float frac, uint32_t *vlist,
                      hipStream_t s) {
-  k_cmp_select_touched<<<grid_for(m, 256, 512), 256, 0, s>>>(st, frac, sc.seg_src, vlist,
+  k_cmp_select_touched<<<grid_for(m, 256, 2048), 256, 0, s>>>(st, frac, sc.seg_src, vlist,
                                                               st.ctl);
   k_cmp_run<<<probe_grid(m), 256, 0, s>>>(st, vlist, st.ctl);
 }
