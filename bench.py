@@ -371,6 +371,18 @@ def other_rooflines(g, G, V, args, batches, deg0, upd_ms, tc_ms):
                                  "Brandes BC from 16 sources (one batch of the 32-source "
                                  "kernels), HIP events on the graph's stream around the call "
                                  "(host round trips between levels included)")
+    # sorted CSR export of the whole graph into device memory
+    off_d, cols_d = DeviceArray(V + 1, np.uint64), DeviceArray(max(E, 1), np.uint32)
+    G._check(L.gv_export_csr(g.handle, off_d.ptr, cols_d.ptr, G.GV_MEM_DEVICE))
+    t0 = time.perf_counter()
+    G._check(L.gv_export_csr(g.handle, off_d.ptr, cols_d.ptr, G.GV_MEM_DEVICE))
+    ex_ms = (time.perf_counter() - t0) * 1e3
+    off_d.free()
+    cols_d.free()
+    out["export_csr"] = {"ms": ex_ms, "directed_edges": E,
+                         "what": "gv_export_csr of the whole graph into device memory (item-"
+                                 "balanced key emission, radix sort of the keys, columns), host "
+                                 "wall time of the call including its workspace allocations"}
     return out
 
 

@@ -403,6 +403,23 @@ def pagerank_device(graph, out_dev, alpha=0.85, epsilon=1e-6, max_iter=100):
     return {f: getattr(st, f) for f, _ in st._fields_}
 
 
+def pagerank_device_run(graph, out_dev, init_dev=None, alpha=0.85, epsilon=1e-6, max_iter=100,
+                        compensated=True):
+    """PageRank into a device float32[V] buffer, warm-started from a device
+    float32[V] vector (init_dev; None = 1/V); returns the stats dict."""
+    o = GvPagerankOpts()
+    _check(lib().gv_pagerank_opts_default(ctypes.byref(o)))
+    o.alpha, o.eps, o.max_iter = float(alpha), float(epsilon), int(max_iter)
+    o.compensated = 1 if compensated else 0
+    if init_dev is not None:
+        o.init = init_dev.ptr
+        o.init_mem = GV_MEM_DEVICE
+    st = GvPagerankStats()
+    _check(lib().gv_pagerank_run(graph.handle, ctypes.byref(o), out_dev.ptr, GV_MEM_DEVICE,
+                                 ctypes.byref(st)))
+    return {f: getattr(st, f) for f, _ in st._fields_}
+
+
 def pagerank(graph, alpha=0.85, epsilon=1e-6, max_iter=100, init=None, compensated=True,
              stats=False):
     """PageRank with a warm start (init: float32[V] host array) and the choice

@@ -54,8 +54,10 @@ struct gv_graph {
   // wait for all device work, including other streams').
   uint64_t *nb_cnt = nullptr, *nb_k0 = nullptr, *nb_k1 = nullptr;
   uint32_t *nb_dvs = nullptr, *nb_dout = nullptr;
+  uint32_t *nb_ist = nullptr, *nb_cur = nullptr, *nb_iq = nullptr;  // key emission items
   void *nb_tmp = nullptr;
   size_t nb_cnt_cap = 0, nb_k_cap = 0, nb_dvs_cap = 0, nb_dout_cap = 0, nb_tmp_cap = 0;
+  size_t nb_ist_cap = 0, nb_cur_cap = 0, nb_iq_cap = 0;
   // Chung-Lu cumulative table on the device, cached by its parameters.
   uint64_t *cl_cum = nullptr;
   uint32_t cl_V = 0;
@@ -527,6 +529,9 @@ gv_status gv_destroy(gv_graph *g) {
   hipFree(g->nb_k1);
   hipFree(g->nb_dvs);
   hipFree(g->nb_dout);
+  hipFree(g->nb_ist);
+  hipFree(g->nb_cur);
+  hipFree(g->nb_iq);
   hipFree(g->nb_tmp);
   if (g->stream) hipStreamSynchronize(g->stream);
   for (int k = 0; k < 2; ++k) {
@@ -646,19 +651,24 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
   hipStream_t s = g->stream;
   const uint32_t V = g->V, bV = id_bits(V);
   uint64_t *off = nullptr, *k0 = nullptr, *k1 = nullptr;
-  uint32_t *dcols = nullptr;
+  uint32_t *dcols = nullptr, *ist = nullptr, *cur = nullptr, *iq = nullptr;
   void *tmp = nullptr;
   uint64_t E = 0;
+  uint32_t items = 0;
   do {
     hipError_t e;
     if ((e = hipMalloc(&off, ((size_t)V + 1) * 8)) != hipSuccess ||
-        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess) {
+        (e = hipMalloc(&tmp, scan_tmp_bytes((uint64_t)V + 1))) != hipSuccess ||
+        (cols && (e = hipMalloc(&ist, ((size_t)V + 1) * 4)) != hipSuccess) ||
+        (cols && (e = hipMalloc(&cur, ((size_t)V + 1) * 4)) != hipSuccess)) {
       rc = hip_fail(e, "export alloc");
       break;
     }
     degrees_u64(g->st[0], off, s);
     scan_u64(off, off, V, nullptr, tmp, s);
+    if (cols) emit_items(g->st[0], nullptr, V, ist, cur, tmp, s);
     if ((e = hipMemcpyAsync(&E, off + V, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (cols && (e = hipMemcpyAsync(&items, ist + V, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
         (e = hipStreamSynchronize(s)) != hipSuccess) {
       rc = hip_fail(e, "export");
       break;
@@ -680,7 +690,8 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
       hipFree(tmp);
       tmp = nullptr;
       if ((e = hipMalloc(&k0, E * 8)) != hipSuccess || (e = hipMalloc(&k1, E * 8)) != hipSuccess ||
-          (e = hipMalloc(&tmp, sort_tmp_bytes(E))) != hipSuccess) {
+          (e = hipMalloc(&tmp, sort_tmp_bytes(E))) != hipSuccess ||
+          (e = hipMalloc(&iq, ((size_t)items + 1) * 4)) != hipSuccess) {
         rc = hip_fail(e, "export alloc");
         break;
       }
@@ -688,7 +699,7 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
         rc = hip_fail(e, "export alloc");
         break;
       }
-      export_keys(g->st[0], off, k0, s);
+      emit_keys(g->st[0], nullptr, V, ist, iq, items, off, cur, k0, s);
       uint64_t *sorted = sort_keys(k0, k1, E, 2 * bV, tmp, s);
       if ((rc = sort_check(tmp, E, 0, 2 * bV, s)) != GV_OK) break;
       uint32_t *cdst = mem == GV_MEM_DEVICE ? cols : dcols;
@@ -706,6 +717,9 @@ gv_status gv_export_csr(gv_graph *g, uint64_t *offsets, uint32_t *cols, int mem)
   hipFree(k1);
   hipFree(dcols);
   hipFree(tmp);
+  hipFree(ist);
+  hipFree(cur);
+  hipFree(iq);
   return rc;
 }
 
@@ -859,9 +873,17 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
       return hip_fail(e, "neighbors");
     qv = g->nb_dvs;
   }
+  if (out && n &&
+      ((e = nb_grow((void **)&g->nb_ist, g->nb_ist_cap, ((size_t)n + 1) * 4)) != hipSuccess ||
+       (e = nb_grow((void **)&g->nb_cur, g->nb_cur_cap, ((size_t)n + 1) * 4)) != hipSuccess))
+    return hip_fail(e, "neighbors alloc");
   neighbor_counts(g->st[0], qv, n, g->nb_cnt, s);
   scan_u64(g->nb_cnt, g->nb_cnt, n, nullptr, g->nb_tmp, s);
+  uint32_t items = 0;
+  if (out && n) emit_items(g->st[0], qv, n, g->nb_ist, g->nb_cur, g->nb_tmp, s);
   if ((e = hipMemcpyAsync(&T, cnt + n, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (out && n &&
+       (e = hipMemcpyAsync(&items, g->nb_ist + n, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
       (e = hipMemcpyAsync(offs, cnt, ((size_t)n + 1) * 8, out_kind, s)) != hipSuccess ||
       (e = hipStreamSynchronize(s)) != hipSuccess)
     return hip_fail(e, "neighbors");
@@ -872,10 +894,11 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
   if (out && T) {
     if ((e = nb_grow_keys(g, T * 8)) != hipSuccess ||
         (e = nb_grow(&g->nb_tmp, g->nb_tmp_cap, sort_tmp_bytes(T))) != hipSuccess ||
+        (e = nb_grow((void **)&g->nb_iq, g->nb_iq_cap, ((size_t)items + 1) * 4)) != hipSuccess ||
         (mem != GV_MEM_DEVICE &&
          (e = nb_grow((void **)&g->nb_dout, g->nb_dout_cap, T * 4)) != hipSuccess))
       return hip_fail(e, "neighbors alloc");
-    neighbor_keys(g->st[0], qv, n, g->nb_cnt, qb, g->nb_k0, s);
+    emit_keys(g->st[0], qv, n, g->nb_ist, g->nb_iq, items, g->nb_cnt, g->nb_cur, g->nb_k0, s);
     uint64_t *sorted = sort_keys(g->nb_k0, g->nb_k1, T, qb + bV, g->nb_tmp, s);
     if ((rc = sort_check(g->nb_tmp, T, 0, qb + bV, s)) != GV_OK) return rc;
     uint32_t *cdst = mem == GV_MEM_DEVICE ? out : g->nb_dout;

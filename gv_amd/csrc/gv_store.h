@@ -90,7 +90,16 @@ void compact_touched(Store &st, const Scratch &sc, uint64_t m, float frac, uint3
 
 // Read-back helpers.
 void degrees_u64(const Store &st, uint64_t *out, hipStream_t s);
-void export_keys(const Store &st, const uint64_t *offsets, uint64_t *keys, hipStream_t s);
+// Balanced key emission for n queries (vertices vs[i], or i when vs is
+// null): emit_items writes the per-query item counts into istart[0, n)
+// (n + 1 words), scans them (istart[n] = items, read back by the caller to
+// size iq) and clears cur[0, n); emit_keys then writes query i's live slots
+// as keys (i << bV | dst) into [offs[i], offs[i + 1]), in no fixed order.
+void emit_items(const Store &st, const uint32_t *vs, uint32_t n, uint32_t *istart,
+                uint32_t *cur, void *tmp, hipStream_t s);
+void emit_keys(const Store &st, const uint32_t *vs, uint32_t n, const uint32_t *istart,
+               uint32_t *iq, uint64_t items, const uint64_t *offs, uint32_t *cur, uint64_t *keys,
+               hipStream_t s);
 void keys_to_cols(const uint64_t *keys, uint64_t n, uint32_t bV, uint32_t *cols, hipStream_t s);
 void neighbor_counts(const Store &st, const uint32_t *vs, uint32_t n, uint64_t *cnt,
                      hipStream_t s);

@@ -1312,8 +1312,126 @@ __global__ __launch_bounds__(256) void k_emit_keys(Store st, const uint32_t *vs,
   }
 }
 
-void export_keys(const Store &st, const uint64_t *offsets, uint64_t *keys, hipStream_t s) {
-  k_emit_keys<<<probe_grid(st.V), 256, 0, s>>>(st, nullptr, st.V, offsets, id_bits(st.V), keys);
+// ------------------------------------------------ balanced key emission --
+// The live slots of query i (vertex vs[i], or i) as keys (i << bV | dst) in
+// [offs[i], offs[i + 1]).  Every query's slots are cut into items of EMIT_CH
+// slots, dealt to lanes (wave w of W, step r, lane l takes item
+// (r * 64 + l) * W + w), so a hub's items spread over many wavefronts; an
+// item of at most EMIT_LANE slots is written by its lane, a longer one by
+// the whole wavefront.  An item of a query without tombstones writes at
+// fixed positions (output position = slot position); a query with
+// tombstones takes each item's range from a per-query cursor after counting
+// its live slots.  Callers sort the keys, so their order within a query's
+// range does not matter.
+constexpr uint32_t EMIT_CH = 1024;
+constexpr uint32_t EMIT_LANE = 64;
+
+__global__ void k_emit_items(Store st, const uint32_t *vs, uint32_t n, uint32_t *icnt,
+                             uint32_t *cur) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = vs ? vs[i] : (uint32_t)i;
+    const uint32_t us = u < st.V ? st.used[u] : 0u;
+    icnt[i] = (us + EMIT_CH - 1) / EMIT_CH;
+    cur[i] = 0;
+  }
+}
+
+__global__ void k_emit_expand(const uint32_t *istart, uint32_t n, uint32_t *iq) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    for (uint32_t j = istart[i]; j < istart[i + 1]; ++j) iq[j] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_emit_chunks(Store st, const uint32_t *vs, uint32_t n,
+                                                     const uint32_t *istart, const uint32_t *iq,
+                                                     const uint64_t *offs, uint32_t bV,
+                                                     uint32_t *cur, uint64_t *keys) {
+  const uint32_t ni = istart[n];
+  const uint32_t lane = lane_id();
+  const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  for (uint64_t r = 0; r * WAVE * W + w < ni; ++r) {
+    const uint64_t it = (r * WAVE + lane) * W + w;
+    uint32_t lo = 0, hi = 0, tbl = 0, q = 0;
+    uint64_t dst = 0;
+    bool tomb = false;
+    if (it < ni) {
+      q = iq[it];
+      const uint32_t u = vs ? vs[q] : q;
+      const uint32_t us = st.used[u];
+      tomb = us != st.deg[u];
+      tbl = st.tbl_off[u];
+      lo = ((uint32_t)it - istart[q]) * EMIT_CH;
+      hi = lo + EMIT_CH < us ? lo + EMIT_CH : us;
+      dst = offs[q] + lo;
+    }
+    if (it < ni && hi - lo <= EMIT_LANE) {
+      const uint64_t tag = (uint64_t)q << bV;
+      if (tomb) {
+        uint32_t cnt = 0;
+        for (uint32_t p = lo; p < hi; ++p)
+          cnt += st.slots[slot_index(st.arena, tbl, p, st.log2B, st.B)] != TOMB ? 1u : 0u;
+        dst = offs[q] + (cnt ? atomicAdd(&cur[q], cnt) : 0u);
+      }
+      for (uint32_t p = lo; p < hi; p += 4) {
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+          x[k] = p + k < hi ? st.slots[slot_index(st.arena, tbl, p + k, st.log2B, st.B)] : TOMB;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          if (x[k] == TOMB) continue;
+          keys[dst++] = tag | x[k];
+        }
+      }
+    }
+    for (unsigned long long m = __ballot(it < ni && hi - lo > EMIT_LANE); m; m &= m - 1) {
+      const int j = __ffsll(m) - 1;
+      const uint32_t jlo = __builtin_amdgcn_readlane(lo, j), jhi = __builtin_amdgcn_readlane(hi, j);
+      const uint32_t jtbl = __builtin_amdgcn_readlane(tbl, j);
+      const uint32_t jq = __builtin_amdgcn_readlane(q, j);
+      const bool jtomb = __builtin_amdgcn_readlane(tomb ? 1u : 0u, j) != 0u;
+      uint64_t jdst = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(dst >> 32), j) << 32) |
+                      __builtin_amdgcn_readlane((uint32_t)dst, j);
+      const uint64_t tag = (uint64_t)jq << bV;
+      if (jtomb) {
+        uint32_t cnt = 0;
+        for (uint32_t p = jlo; p < jhi; p += WAVE) {
+          const bool live = p + lane < jhi &&
+                            st.slots[slot_index(st.arena, jtbl, p + lane, st.log2B, st.B)] != TOMB;
+          cnt += (uint32_t)__popcll(__ballot(live));
+        }
+        uint32_t base = 0;
+        if (lane == 0 && cnt) base = atomicAdd(&cur[jq], cnt);
+        jdst = offs[jq] + __shfl(base, 0, WAVE);
+      }
+      for (uint32_t p = jlo; p < jhi; p += WAVE) {
+        uint32_t x = TOMB;
+        if (p + lane < jhi) x = st.slots[slot_index(st.arena, jtbl, p + lane, st.log2B, st.B)];
+        const bool live = x != TOMB;
+        const unsigned long long lm = __ballot(live);
+        if (live) keys[jdst + mask_lt_count(lm)] = tag | x;
+        jdst += (uint64_t)__popcll(lm);
+      }
+    }
+  }
+}
+
+void emit_items(const Store &st, const uint32_t *vs, uint32_t n, uint32_t *istart,
+                uint32_t *cur, void *tmp, hipStream_t s) {
+  if (!n) return;
+  k_emit_items<<<grid_for(n), 256, 0, s>>>(st, vs, n, istart, cur);
+  scan_u32(istart, istart, n, nullptr, tmp, s);
+}
+
+void emit_keys(const Store &st, const uint32_t *vs, uint32_t n, const uint32_t *istart,
+               uint32_t *iq, uint64_t items, const uint64_t *offs, uint32_t *cur, uint64_t *keys,
+               hipStream_t s) {
+  if (!n || !items) return;
+  k_emit_expand<<<grid_for(n), 256, 0, s>>>(istart, n, iq);
+  k_emit_chunks<<<grid_for(items * WAVE, 256, 2048), 256, 0, s>>>(st, vs, n, istart, iq, offs,
+                                                                  id_bits(st.V), cur, keys);
 }
 
 __global__ void k_keys_to_cols(const uint64_t *keys, uint64_t n, uint64_t mask, uint32_t *cols) {

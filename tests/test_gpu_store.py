@@ -260,6 +260,40 @@ def test_neighbors_query_matches_oracle():
     g.close()
 
 
+def test_neighbors_and_export_over_hub_items_with_tombstones():
+    # hubs span many 1024-slot emission items; tombstones (no compaction)
+    # make their items take output ranges from the per-query cursor; a
+    # vertex is queried more than once
+    V = 50000
+    rng = np.random.default_rng(7)
+    g = make(V, True, pool=16 * V)
+    o = O.OracleGraph(V, True)
+    h0 = rng.choice(np.arange(1, V, dtype=np.uint32), 30000, replace=False)
+    h1 = rng.choice(np.arange(2, V, dtype=np.uint32), 3000, replace=False)
+    rs = rng.integers(2, V, 60000).astype(np.uint32)
+    rd = rng.integers(0, V, 60000).astype(np.uint32)
+    s = np.concatenate([np.zeros(len(h0), np.uint32), np.ones(len(h1), np.uint32), rs])
+    d = np.concatenate([h0, h1, rd])
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    d0 = rng.choice(h0, 5000, replace=False)
+    d1 = rng.choice(h1, 500, replace=False)
+    ds = np.concatenate([np.zeros(len(d0), np.uint32), np.ones(len(d1), np.uint32)])
+    dd = np.concatenate([d0, d1])
+    assert g.batchDelete(ds, dd)["applied"] == o.delete(ds, dd)
+    # more inserts after the tombstones: items past a sorted prefix
+    is_ = np.concatenate([np.zeros(2000, np.uint32), rng.integers(2, V, 5000).astype(np.uint32)])
+    id_ = rng.integers(0, V, len(is_)).astype(np.uint32)
+    assert g.batchInsert(is_, id_)["applied"] == o.insert(is_, id_)
+    assert_same(g, o, "export with tombstones")
+    q = np.concatenate([[0, 1, 0], rng.integers(0, V, 500), [1]]).astype(np.uint32)
+    offs, out = g.neighbors(q)
+    eo, ec = o.export_csr()
+    for i, v in enumerate(q):
+        assert np.array_equal(out[offs[i]:offs[i + 1]], ec[eo[v]:eo[v + 1]]), f"query {i} vertex {v}"
+    g.close()
+
+
 def test_device_generators_match_oracle():
     from gv_amd.hip import DeviceArray
     g = make(1 << 14, True)
