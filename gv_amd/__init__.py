@@ -43,7 +43,8 @@ EXPORTED = [
     "gv_pagerank_opts_default", "gv_pagerank_run", "gv_pagerank_personalized",
     "gv_triangle_count",
     "gv_bc", "gv_gen_rmat", "gv_gen_uniform", "gv_gen_sample", "gv_gen_chung_lu",
-    "gv_get_config", "gv_set_tuning", "gv_reserve", "gv_capture_begin", "gv_capture_end",
+    "gv_get_config", "gv_set_tuning", "gv_reserve", "gv_pagerank_reserve", "gv_pagerank_last",
+    "gv_capture_begin", "gv_capture_end",
     "gv_capture_launch", "gv_capture_destroy", "gv_read_matrix_market", "gv_sync",
     "gv_get_stream", "gv_test_set_sort_spin_limit",
     "gv_last_error",
@@ -157,6 +158,8 @@ def lib():
         "gv_get_config": [P, ctypes.POINTER(GvConfig)],
         "gv_set_tuning": [P, ctypes.POINTER(GvConfig)],
         "gv_reserve": [P, u64, u32],
+        "gv_pagerank_reserve": [P, u64],
+        "gv_pagerank_last": [P, ctypes.POINTER(GvPagerankStats)],
         "gv_capture_begin": [P],
         "gv_capture_end": [P, ctypes.POINTER(P)],
         "gv_capture_launch": [P, ctypes.POINTER(GvBatchStats)],
@@ -327,9 +330,21 @@ class GraphVine:
         """Size staging buffers so captured updates never allocate."""
         _check(lib().gv_reserve(self.handle, int(max_pairs), int(max_batches)))
 
+    def reserve_pagerank(self, max_edges):
+        """Size the default PageRank layout for up to max_edges live edges, so
+        PageRank can be captured (and never allocates up to that size)."""
+        _check(lib().gv_pagerank_reserve(self.handle, int(max_edges)))
+
+    def pagerank_last(self):
+        """Stats dict (iters, residual, edges) of the last default-layout
+        PageRank call, captured replays included."""
+        st = GvPagerankStats()
+        _check(lib().gv_pagerank_last(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
     def capture(self):
-        """Context manager recording the update/compaction calls made inside
-        it; on exit .result holds a Capture that replays them."""
+        """Context manager recording the update/compaction/PageRank calls
+        made inside it; on exit .result holds a Capture that replays them."""
         return _CaptureScope(self)
 
     def stream(self):
@@ -404,9 +419,10 @@ def pagerank_device(graph, out_dev, alpha=0.85, epsilon=1e-6, max_iter=100):
 
 
 def pagerank_device_run(graph, out_dev, init_dev=None, alpha=0.85, epsilon=1e-6, max_iter=100,
-                        compensated=True):
+                        compensated=True, stats=True):
     """PageRank into a device float32[V] buffer, warm-started from a device
-    float32[V] vector (init_dev; None = 1/V); returns the stats dict."""
+    float32[V] vector (init_dev; None = 1/V); returns the stats dict, or
+    None with stats=False (the form a capture records)."""
     o = GvPagerankOpts()
     _check(lib().gv_pagerank_opts_default(ctypes.byref(o)))
     o.alpha, o.eps, o.max_iter = float(alpha), float(epsilon), int(max_iter)
@@ -416,8 +432,8 @@ def pagerank_device_run(graph, out_dev, init_dev=None, alpha=0.85, epsilon=1e-6,
         o.init_mem = GV_MEM_DEVICE
     st = GvPagerankStats()
     _check(lib().gv_pagerank_run(graph.handle, ctypes.byref(o), out_dev.ptr, GV_MEM_DEVICE,
-                                 ctypes.byref(st)))
-    return {f: getattr(st, f) for f, _ in st._fields_}
+                                 ctypes.byref(st) if stats else None))
+    return {f: getattr(st, f) for f, _ in st._fields_} if stats else None
 
 
 def pagerank(graph, alpha=0.85, epsilon=1e-6, max_iter=100, init=None, compensated=True,

@@ -35,6 +35,17 @@ struct PrCtl {
   unsigned int pad;
 };
 
+// Sizes of the propagation-blocked layout of one snapshot, computed on the
+// device (so that a captured PageRank needs no host round trip): stream
+// lengths in whole tiles, tile and bin counts, the (source block, bin)
+// table size, long rows, expansion slots and image groups.  fail != 0 (a
+// snapshot larger than the buffers reserved for a captured call, or a failed
+// relabelling sort) makes every later kernel of the call exit at once.
+struct PbDims {
+  unsigned long long E, hend, cend;
+  unsigned int nht, nct, NB, table, n_long, P1, G, fail;
+};
+
 // Capacities (bytes) of the propagation-blocked layout's buffers.
 struct PbCaps {
   size_t hitem = 0, hoff = 0, coff = 0, lidxr = 0, lng = 0, Sh = 0, Sc = 0, partials = 0;
@@ -114,6 +125,13 @@ struct PrWork {
   uint32_t pb_NS = 0, pb_NB = 0, pb_nht = 0, pb_nct = 0, pb_n_long = 0, pb_prepV = 0;
   uint32_t pb_bin_tiles = 0;
   uint64_t pb_E = 0;
+  PbDims *pb_dims = nullptr;   // device sizes of the prepared layout
+  PbDims *h_pb_dims = nullptr; // pinned host mirror
+  // capture reservation (pagerank_pb_reserve): vertex count and live
+  // in-edges every buffer is sized for; 0 = none
+  uint32_t pb_res_V = 0;
+  uint64_t pb_res_E = 0;
+  uint64_t gen = 0;  // bumped whenever a buffer of this workspace moves
 };
 
 // Run parameters of one PageRank call.
@@ -141,8 +159,13 @@ gv_status pr_reserve(PrWork &w, uint32_t V);
 // be null, pr_snapshot_icap(in) entries): per chunk item of the snapshot, the
 // entries it wrote whose (relabelled) source is below hot_below.
 uint64_t pr_snapshot_icap(const Store &in);
+// fail (capture form, may be null): no host round trip; every buffer must
+// already hold the reserved size (GV_E_INVALID otherwise), and a snapshot
+// larger than the reservation, or a failed relabelling sort, sets *fail
+// instead of being written.
 gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
-                      hipStream_t s, int flat, uint32_t *hitem = nullptr, uint32_t hot_below = 0);
+                      hipStream_t s, int flat, uint32_t *hitem = nullptr, uint32_t hot_below = 0,
+                      uint32_t *fail = nullptr);
 void pr_init_vectors(float *x, float *cb, const uint32_t *deg_out, uint32_t V, float x0,
                      const float *init, const uint32_t *order, hipStream_t s);
 void pr_unpermute(const float *x, const uint32_t *newid, uint32_t V, float *out, hipStream_t s);
@@ -165,9 +188,26 @@ void pr_release(PrWork &w);
 // returns GV_E_UNSUPPORTED when the graph exceeds the layout's limits (more
 // than 16384 cold source blocks, 32-bit stream positions); iterate runs the
 // iterations over w alone.
-gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &w, hipStream_t s);
+// capture: the form a stream capture can record -- no host round trip and
+// no allocation (pagerank_pb_reserve must have sized every buffer for V and
+// at least the snapshot's live entries; a larger snapshot sets
+// w.pb_dims->fail on the device and the call computes nothing), launches
+// sized by capacity, max_iter iteration launches (those after convergence
+// exit at once); iters, residual and iter_ms must be null.
+gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &w, hipStream_t s,
+                              bool capture = false);
 gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
-                              double *residual, hipStream_t s, double *iter_ms);
+                              double *residual, hipStream_t s, double *iter_ms,
+                              bool capture = false);
+// Sizes every buffer of the propagation-blocked layout (and its snapshot)
+// for graphs of V vertices and up to max_edges live in-edges, by preparing
+// the current graph once with that headroom.  GV_E_UNSUPPORTED when the
+// layout does not apply to such graphs.
+gv_status pagerank_pb_reserve(const Store &in, const uint32_t *deg_out, PrWork &w,
+                              uint64_t max_edges, hipStream_t s);
+// Iterations and residual of the last (possibly captured) call; GV_E_INVALID
+// when its snapshot exceeded the reservation (nothing was computed).
+gv_status pagerank_pb_last(PrWork &w, int *iters, double *residual, hipStream_t s);
 void pb_release(PrWork &w);
 // Snapshot + layout for PageRank by layout id (5: propagation-blocked, with
 // the flat stream as fallback; otherwise the flat stream), then iterations

@@ -103,9 +103,11 @@ void pr_release(PrWork &w) {
   // sizing settings of the caller outlive the buffers
   const uint64_t min_cap = w.snap_min_cap;
   const uint32_t grid = w.pf_grid;
+  const uint64_t gen = w.gen;
   w = PrWork();
   w.snap_min_cap = min_cap;
   w.pf_grid = grid;
+  w.gen = gen + 1;
 }
 
 __device__ __forceinline__ int pr_bin(uint32_t used, PrBins b) {
@@ -324,7 +326,9 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
                                                     const uint32_t *ivert, const uint64_t *ipos,
                                                     const uint32_t *newid, const uint64_t *off,
                                                     uint32_t *cols, uint32_t flat,
-                                                    uint32_t *hitem, uint32_t hot_below) {
+                                                    uint32_t *hitem, uint32_t hot_below,
+                                                    const uint32_t *fail) {
+  if (fail && *fail) return;
   const uint32_t ni = istart[st.V];
   const uint32_t lane = lane_id();
   const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
@@ -418,20 +422,32 @@ __global__ void k_rl_perm(const uint64_t *sorted, uint32_t V, uint32_t bV, const
   }
 }
 
+// Capture form: the snapshot's entries must fit the columns buffer.
+__global__ void k_snap_check(const uint64_t *off_end, uint64_t cap, uint32_t *fail) {
+  if (*off_end > cap) atomicCAS(fail, 0u, 1u);
+}
+
 uint64_t pr_snapshot_icap(const Store &in) {
   // items: at most one per vertex plus one per SNAP_CH slots of the pool
   return (uint64_t)in.V + in.P * (uint64_t)in.B / SNAP_CH + 1;
 }
 
 gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, PrWork &w,
-                      hipStream_t s, int flat, uint32_t *hitem, uint32_t hot_below) {
+                      hipStream_t s, int flat, uint32_t *hitem, uint32_t hot_below,
+                      uint32_t *fail) {
   const uint32_t V = in.V;
   const uint64_t icap = pr_snapshot_icap(in);
   if (icap > 0xFFFFFFFFull) {
     set_error("pagerank snapshot: too many chunk items");
     return GV_E_INVALID;
   }
+  if (fail && (!w.snap_istart || icap > w.snap_icap || !w.snap_cols ||
+               w.snap_cap < w.snap_min_cap || (relabel && (w.rl_V < V || !w.rl_order)) || flat)) {
+    set_error("pagerank capture: the snapshot buffers are not reserved for this graph");
+    return GV_E_INVALID;
+  }
   if (!w.snap_istart || icap > w.snap_icap) {
+    ++w.gen;
     hipFree(w.snap_istart);
     hipFree(w.snap_ivert);
     hipFree(w.snap_ipos);
@@ -457,6 +473,7 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
   scan_u64(w.snap_ipos, w.snap_ipos, icap, w.snap_istart + V, w.snap_tmp, s);
   if (relabel) {
     if (w.rl_V < V || !w.rl_order) {
+      ++w.gen;
       hipFree(w.rl_order);
       hipFree(w.rl_newid);
       hipFree(w.rl_len);
@@ -480,14 +497,22 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
     }
     const uint32_t bV = id_bits(V);
     k_rl_keys<<<grid_for(V), 256, 0, s>>>(deg_out, V, bV, w.rl_keys[0]);
+    // capture form: a failed sort is folded into *fail on the device
     const uint64_t *sorted = sort_keys_range(w.rl_keys[0], w.rl_keys[1], V, 0,
-                                             bV + id_bits(V + 1), w.rl_tmp, s);
+                                             bV + id_bits(V + 1), w.rl_tmp, s, fail);
     k_rl_perm<<<grid_for(V), 256, 0, s>>>(sorted, V, bV, in.deg, deg_out, w.rl_order,
                                           w.rl_newid, w.rl_len, w.rl_dout, w.snap_off);
     scan_u64(w.snap_off, w.snap_off, V, nullptr, w.snap_tmp, s);
   } else {
     k_snap_off<<<grid_for((uint64_t)V + 1), 256, 0, s>>>(w.snap_istart, V, w.snap_ipos,
                                                          w.snap_off);
+  }
+  if (fail) {
+    k_snap_check<<<1, 1, 0, s>>>(w.snap_off + V, w.snap_cap, fail);
+    k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(
+        in, w.snap_istart, w.snap_ivert, w.snap_ipos, relabel ? w.rl_newid : nullptr, w.snap_off,
+        w.snap_cols, 0u, hitem, hot_below, fail);
+    return GV_OK;
   }
   uint64_t E = 0;
   GV_HIP(hipMemcpyAsync(&E, w.snap_off + V, sizeof(E), hipMemcpyDeviceToHost, s));
@@ -500,6 +525,7 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
   // the flat layout pads the stream to whole tiles
   const uint64_t need = flat ? cdiv(E, PF_T) * PF_T : E;
   if (!w.snap_cols || need > w.snap_cap) {
+    ++w.gen;
     hipFree(w.snap_cols);
     w.snap_cols = nullptr;
     w.snap_cap = 0;
@@ -510,7 +536,7 @@ gv_status pr_snapshot(const Store &in, const uint32_t *deg_out, bool relabel, Pr
   }
   k_snap_write<<<grid_for(icap * WAVE, 256, 2048), 256, 0, s>>>(
       in, w.snap_istart, w.snap_ivert, w.snap_ipos, relabel ? w.rl_newid : nullptr, w.snap_off,
-      w.snap_cols, flat ? 1u : 0u, hitem, hot_below);
+      w.snap_cols, flat ? 1u : 0u, hitem, hot_below, nullptr);
   return GV_OK;
 }
 

@@ -69,6 +69,7 @@ struct gv_capture {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t gen = 0;
+  uint64_t pr_gen = 0;  // the PageRank workspace's generation at capture
   uint64_t pairs = 0;
   std::vector<void *> pinned;
 };
@@ -911,15 +912,40 @@ gv_status gv_neighbors(gv_graph *g, const uint32_t *vs, uint32_t n, uint64_t *of
   return GV_OK;
 }
 
+// Inside a capture: the default layout's capture form (reserved buffers, no
+// host round trip, no allocation); the result lands in the device vector out
+// when the capture is replayed, gv_pagerank_last reports its iterations.
+static gv_status pagerank_captured(gv_graph *g, const gv_pagerank_opts *po, float *out, int mem) {
+  if (mem != GV_MEM_DEVICE || (po->init && po->init_mem != GV_MEM_DEVICE)) {
+    set_error("a captured PageRank takes device vectors only (GV_MEM_DEVICE)");
+    return GV_E_INVALID;
+  }
+  if (g->cfg.pr_layout != 5 || po->max_iter <= 0) {
+    set_error("a captured PageRank needs the default layout (pr_layout 5) and max_iter > 0");
+    return GV_E_UNSUPPORTED;
+  }
+  const Store &in = g->has_in ? g->st[1] : g->st[0];
+  PrOpts o;
+  o.alpha = po->alpha;
+  o.eps = po->eps;
+  o.max_iter = po->max_iter;
+  o.compensated = po->compensated;
+  o.init = po->init;
+  gv_status rc = pagerank_pb_prepare(in, g->st[0].deg, g->pr, g->stream, true);
+  if (rc) return rc;
+  return pagerank_pb_iterate(o, g->pr, out, nullptr, nullptr, g->stream, nullptr, true);
+}
+
 static gv_status pagerank_common(gv_graph *g, const gv_pagerank_opts *po, float *out, int mem,
                                  int *iters, double *residual, double *iter_ms) {
-  gv_status rc = check_graph(g);
+  gv_status rc = check_graph(g, iters == nullptr && residual == nullptr && iter_ms == nullptr);
   if (rc) return rc;
   if (!out || !po) return GV_E_INVALID;
   if (g->directed && !g->has_in) {
     set_error("PageRank on a directed graph needs maintain_in_edges");
     return GV_E_UNSUPPORTED;
   }
+  if (g->capturing) return pagerank_captured(g, po, out, mem);
   const Store &in = g->has_in ? g->st[1] : g->st[0];
   PrBins bins{g->cfg.pr_thread_max, g->cfg.pr_group_max, g->cfg.pr_wave_max};
   const size_t vb = (size_t)(g->V ? g->V : 1) * 4;
@@ -979,6 +1005,14 @@ gv_status gv_pagerank_run(gv_graph *g, const gv_pagerank_opts *opts, float *out,
                           gv_pagerank_stats *stats) {
   gv_pagerank_stats st;
   std::memset(&st, 0, sizeof(st));
+  if (g && g->capturing) {
+    if (stats) {
+      set_error("a captured PageRank cannot return stats; pass NULL (gv_pagerank_last)");
+      if (!g->cap_err) g->cap_err = GV_E_INVALID;
+      return GV_E_INVALID;
+    }
+    return note_capture(g, pagerank_common(g, opts, out, mem, nullptr, nullptr, nullptr));
+  }
   gv_status rc = pagerank_common(g, opts, out, mem, &st.iters, &st.residual,
                                  stats ? &st.kernel_ms : nullptr);
   if (rc) return rc;
@@ -1259,6 +1293,34 @@ gv_status gv_reserve(gv_graph *g, uint64_t max_pairs, uint32_t max_batches) {
   return reserve_groups(g, max_batches);
 }
 
+gv_status gv_pagerank_reserve(gv_graph *g, uint64_t max_edges) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (g->directed && !g->has_in) {
+    set_error("PageRank on a directed graph needs maintain_in_edges");
+    return GV_E_UNSUPPORTED;
+  }
+  if (g->cfg.pr_layout != 5) {
+    set_error("gv_pagerank_reserve sizes the default layout (pr_layout 5)");
+    return GV_E_UNSUPPORTED;
+  }
+  const Store &in = g->has_in ? g->st[1] : g->st[0];
+  return pagerank_pb_reserve(in, g->st[0].deg, g->pr, max_edges, g->stream);
+}
+
+gv_status gv_pagerank_last(gv_graph *g, gv_pagerank_stats *stats) {
+  gv_status rc = check_graph(g);
+  if (rc) return rc;
+  if (!stats) return GV_E_INVALID;
+  std::memset(stats, 0, sizeof(*stats));
+  rc = pagerank_pb_last(g->pr, &stats->iters, &stats->residual, g->stream);
+  if (rc) return rc;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  stats->edges = g->h_ctl[g->has_in ? 1 : 0].live_edges;
+  return GV_OK;
+}
+
 gv_status gv_capture_begin(gv_graph *g) {
   gv_status rc = check_graph(g);
   if (rc) return rc;
@@ -1303,6 +1365,7 @@ gv_status gv_capture_end(gv_graph *g, gv_capture **out) {
   c->graph = graph;
   c->exec = exec;
   c->gen = g->buf_gen;
+  c->pr_gen = g->pr.gen;
   c->pairs = g->cap_pairs;
   c->pinned.swap(pinned);
   g->captures.push_back(c);
@@ -1322,7 +1385,7 @@ gv_status gv_capture_launch(gv_capture *c, gv_batch_stats *stats) {
   }
   gv_status rc = check_graph(g);
   if (rc) return rc;
-  if (c->gen != g->buf_gen) {
+  if (c->gen != g->buf_gen || c->pr_gen != g->pr.gen) {
     set_error("buffers used by this capture were reallocated after it was made; capture again");
     return GV_E_INVALID;
   }

@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void k_pb_split(const uint32_t *istart, const 
                                                   const uint32_t *newid, const uint64_t *off,
                                                   const uint32_t *cols, const uint32_t *hscan,
                                                   const uint32_t *hoff, const uint32_t *coff,
-                                                  uint16_t *hot, uint32_t *csrc) {
+                                                  uint16_t *hot, uint32_t *csrc,
+                                                  const PbDims *dims) {
+  if (dims->fail) return;
   const uint32_t ni = istart[V];
   const uint32_t lane = lane_id();
   const uint64_t W = (uint64_t)gridDim.x * blockDim.x / WAVE;
@@ -192,7 +194,9 @@ __global__ __launch_bounds__(256) void k_pb_split(const uint32_t *istart, const 
 // the end of the last tile.
 __global__ void k_pb_pads(const uint32_t *istart, const uint32_t *hscan, const uint32_t *order,
                           const uint64_t *off, uint32_t V, const uint32_t *hoff,
-                          const uint32_t *coff, uint16_t *hot, uint32_t *csrc) {
+                          const uint32_t *coff, uint16_t *hot, uint32_t *csrc,
+                          const PbDims *dims) {
+  if (dims->fail) return;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < V;
        r += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = order[r];
@@ -203,8 +207,10 @@ __global__ void k_pb_pads(const uint32_t *istart, const uint32_t *hscan, const u
   }
 }
 
-__global__ void k_pb_tail(const uint32_t *hoff, const uint32_t *coff, uint32_t V, uint64_t hend,
-                          uint64_t cend, uint16_t *hot, uint32_t *csrc) {
+__global__ void k_pb_tail(const uint32_t *hoff, const uint32_t *coff, uint32_t V,
+                          const PbDims *dims, uint16_t *hot, uint32_t *csrc) {
+  if (dims->fail) return;
+  const uint64_t hend = dims->hend, cend = dims->cend;
   const uint64_t h0 = hoff[V], c0 = coff[V];
   const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t st = (uint64_t)gridDim.x * blockDim.x;
@@ -213,8 +219,12 @@ __global__ void k_pb_tail(const uint32_t *hoff, const uint32_t *coff, uint32_t V
 }
 
 // tbase[t] = label of the row in progress at the start of tile t (the last
-// row whose first entry lies before t * PF_T), -1 for tile 0.
-__global__ void k_pb_tbase(const uint32_t *off, uint32_t V, uint32_t n_tiles, int32_t *tbase) {
+// row whose first entry lies before t * PF_T), -1 for tile 0.  cold: the
+// cold stream's tiles, else the hot stream's.
+__global__ void k_pb_tbase(const uint32_t *off, uint32_t V, const PbDims *dims, uint32_t cold,
+                           int32_t *tbase) {
+  if (dims->fail) return;
+  const uint32_t n_tiles = cold ? dims->nct : dims->nht;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles;
        t += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e = t * PF_T;
@@ -250,12 +260,17 @@ __global__ void k_pb_longfill(const uint32_t *lidx, uint32_t V, uint32_t *longro
 // cnt[k * NB + b] (k-major, so an exclusive scan gives the expansion
 // order: source block, then bin); the bin's image size in groups goes to
 // ngrp[b].
+// One workgroup per bin; a grid sized for more bins than the layout has
+// (a captured call) leaves the others idle.
 __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_bincnt(const uint32_t *csrc,
-                                                                uint64_t cend, uint32_t NS,
-                                                                uint32_t NB, uint32_t bin,
-                                                                uint32_t *cnt, uint32_t *ngrp) {
+                                                                const PbDims *dims, uint32_t NS,
+                                                                uint32_t bin, uint32_t *cnt,
+                                                                uint32_t *ngrp) {
   __shared__ uint32_t hist[PB_MAX_NS];
   const uint32_t b = blockIdx.x;
+  const uint32_t NB = dims->NB;
+  const uint64_t cend = dims->cend;
+  if (dims->fail || b >= NB) return;
   for (uint32_t k = threadIdx.x; k < NS; k += PB_BUILD_THREADS) hist[k] = 0;
   __syncthreads();
   const uint64_t q0 = (uint64_t)b * bin;
@@ -286,13 +301,15 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_bincnt(const uint32_t *
 // so that order is not fixed -- but each entry's index names its own value
 // and the sums run in stream order, so the results do not depend on it.
 __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
-    const uint32_t *csrc, uint64_t cend, uint32_t NS, uint32_t NB, uint32_t bin,
-    const uint32_t *seg, const uint32_t *gbase, uint32_t *gaddr, uint16_t *lidx,
-    uint16_t *cold) {
+    const uint32_t *csrc, const PbDims *dims, uint32_t NS, uint32_t bin, const uint32_t *seg,
+    const uint32_t *gbase, uint32_t *gaddr, uint16_t *lidx, uint16_t *cold) {
   __shared__ uint32_t lpre[PB_MAX_NS];
   __shared__ uint32_t ctr[PB_MAX_NS];
   __shared__ uint32_t sbase[PB_MAX_NS];  // segment (k, b)'s first slot
   const uint32_t b = blockIdx.x;
+  const uint32_t NB = dims->NB;
+  const uint64_t cend = dims->cend;
+  if (dims->fail || b >= NB) return;
   uint32_t carry = 0;
   const uint32_t g0 = gbase[b];
   for (uint32_t k0 = 0; k0 < NS; k0 += PB_BUILD_THREADS) {
@@ -337,16 +354,21 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
 
 // Expansion work items: the slots of source block k (seg[k * NB] ..
 // seg[(k + 1) * NB]) in pieces of at most PB_CH.
-__global__ void k_pb_itemcnt(const uint32_t *seg, uint32_t NS, uint32_t NB, uint32_t *nit) {
+// (no items at all after a failure: the expansion then does nothing)
+__global__ void k_pb_itemcnt(const uint32_t *seg, uint32_t NS, const PbDims *dims, uint32_t *nit) {
+  const uint32_t NB = dims->NB;
+  const bool fail = dims->fail != 0;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < NS;
        k += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t n = seg[(k + 1) * NB] - seg[k * NB];
+    const uint32_t n = fail ? 0u : seg[(k + 1) * NB] - seg[k * NB];
     nit[k] = (n + PB_CH - 1) / PB_CH;
   }
 }
 
-__global__ void k_pb_itemfill(const uint32_t *seg, uint32_t NS, uint32_t NB, const uint32_t *nit,
-                              uint4 *items) {
+__global__ void k_pb_itemfill(const uint32_t *seg, uint32_t NS, const PbDims *dims,
+                              const uint32_t *nit, uint4 *items) {
+  if (dims->fail) return;
+  const uint32_t NB = dims->NB;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < NS;
        k += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t s0 = seg[k * NB], s1 = seg[(k + 1) * NB];
@@ -372,11 +394,25 @@ struct PbArgs {
   const uint32_t *deg_out, *longrows;
   double *partials;
   PrCtl *ctl;
+  const PbDims *dims;  // tile, bin and long-row counts of the prepared layout
   uint32_t V, nh, n_htiles, n_ctiles, n_hot_units, n_bins, bin_tiles, n_long, compensated,
       max_iter;
   float alpha, base;
   double eps;
 };
+
+// The counts the layout was prepared with, from the device (the host does
+// not know them when the call is captured); false after a failed prepare.
+__device__ __forceinline__ bool pb_sizes(PbArgs &a) {
+  const PbDims *d = a.dims;
+  if (d->fail) return false;
+  a.n_htiles = d->nht;
+  a.n_ctiles = d->nct;
+  a.n_hot_units = (d->nht + PB_HOT_UNIT - 1) / PB_HOT_UNIT;
+  a.n_bins = d->NB;
+  a.n_long = d->n_long;
+  return true;
+}
 
 // left := left (+) right, left first; compensated or plain
 __device__ __forceinline__ void pb_join(float &s, float &c, float s2, float c2, uint32_t comp) {
@@ -565,7 +601,7 @@ template <bool COMP>
 __global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
   __shared__ float s_img[PB_IMG];
   __shared__ uint32_t s_unit;
-  if (a.ctl->done) return;
+  if (a.ctl->done || !pb_sizes(a)) return;
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x / WAVE;
   unsigned int *cursor = &a.ctl->cursor[a.ctl->iters & 1u];
@@ -720,7 +756,7 @@ __device__ __forceinline__ float2 pb_long_part(const uint32_t *off, const float2
 // tiles or more (a long row, joined by a wavefront), else the row's S/P/Q
 // reads are those of pb_part.
 __global__ __launch_bounds__(PB_ROW_THREADS) void k_pb_rows(PbArgs a) {
-  if (a.ctl->done) return;
+  if (a.ctl->done || !pb_sizes(a)) return;
   // the next iteration's tile kernel deals its units from the other counter
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[(a.ctl->iters + 1) & 1u] = 0;
   const uint32_t comp = a.compensated;
@@ -806,6 +842,7 @@ __global__ __launch_bounds__(PB_ROW_THREADS) void k_pb_rows(PbArgs a) {
 // and step.
 __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_expand(PbArgs a, uint32_t resid) {
   __shared__ float s_win[PB_SB];
+  if (!pb_sizes(a)) return;
   if (resid) {
     if (blockIdx.x == 0) {
       if (a.ctl->done) return;
@@ -895,8 +932,10 @@ static int pb_device_cus() {
 }
 
 // Grow-only device buffer (headroom of an eighth, at least `floor` bytes).
-static gv_status pb_grow(void **p, size_t &cap, size_t need, size_t floor = 0) {
+// A move bumps the workspace's generation (captured calls hold addresses).
+static gv_status pb_grow(PrWork &w, void **p, size_t &cap, size_t need, size_t floor = 0) {
   if (*p && need <= cap) return GV_OK;
+  ++w.gen;
   hipFree(*p);
   *p = nullptr;
   cap = 0;
@@ -915,44 +954,144 @@ void pb_release(PrWork &w) {
                    (void **)&w.pb_PQc,   (void **)&w.pb_hitem, (void **)&w.pb_cnt,
                    (void **)&w.pb_ngrp,  (void **)&w.pb_gaddr, (void **)&w.pb_items,
                    (void **)&w.pb_nit,   (void **)&w.pb_lidxr, (void **)&w.pb_long,
-                   (void **)&w.pb_tmp,   (void **)&w.pb_partials};
+                   (void **)&w.pb_tmp,   (void **)&w.pb_partials, (void **)&w.pb_dims};
   for (void **b : bufs) {
     hipFree(*b);
     *b = nullptr;
   }
+  if (w.h_pb_dims) hipHostFree(w.h_pb_dims);
+  w.h_pb_dims = nullptr;
   w.pb_caps = PbCaps{};
   w.pb_ready = 0;
+  w.pb_res_V = 0;
+  w.pb_res_E = 0;
+  ++w.gen;
 }
 
-gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &w,
-                              hipStream_t s) {
+// Capacities (entries) the device-side size checks of a captured call use;
+// a plain call grows its buffers to the sizes it reads back instead.
+struct PbLimits {
+  unsigned long long hot, cold, htb, ctb, pqh, pqc, cnt, ngrp, lidx, vals, gaddr, items;
+  uint32_t check;
+};
+
+static PbLimits pb_limits(const PrWork &w, bool capture) {
+  PbLimits l{};
+  l.check = capture ? 1u : 0u;
+  if (!capture) return l;
+  const PbCaps &c = w.pb_caps;
+  l.hot = c.hot / 2;
+  l.cold = std::min(c.cold / 2, c.csrc / 4);
+  l.htb = c.htb / 4;
+  l.ctb = c.ctb / 4;
+  l.pqh = c.PQh / 16;
+  l.pqc = c.PQc / 16;
+  l.cnt = c.cnt / 4;
+  l.ngrp = c.ngrp / 4;
+  l.lidx = c.lidx / 2;
+  l.vals = c.vals / 4;
+  l.gaddr = c.gaddr / 4;
+  l.items = c.items / 16;
+  return l;
+}
+
+__global__ void k_pb_dims_reset(PbDims *d) {
+  PbDims z = {};
+  *d = z;
+}
+
+// Stream lengths, tile and bin counts and the table size, once both streams'
+// row offsets are scanned; with l.check, a layout that exceeds the reserved
+// buffers or the layout's 32-bit positions sets fail.
+__global__ void k_pb_dims1(const uint32_t *hoff, const uint32_t *coff, const uint64_t *snap_off,
+                           uint32_t V, uint32_t NS, uint32_t BT, PbLimits l, PbDims *d) {
+  const unsigned long long E = snap_off[V];
+  const unsigned long long nht = ((unsigned long long)hoff[V] + PF_T - 1) / PF_T;
+  const unsigned long long nct = ((unsigned long long)coff[V] + PF_T - 1) / PF_T;
+  const unsigned long long NB = (nct + BT - 1) / BT;
+  d->E = E;
+  d->hend = nht * PF_T;
+  d->cend = nct * PF_T;
+  d->nht = (uint32_t)nht;
+  d->nct = (uint32_t)nct;
+  d->NB = (uint32_t)NB;
+  d->table = (uint32_t)(NS * NB);
+  if (!l.check) return;
+  const bool bad = E + 2ull * V + 2ull * BT * PF_T >= 0x7FFFFFFFull ||
+                   (unsigned long long)NS * NB >= (1ull << 29) || d->hend > l.hot ||
+                   d->cend > l.cold || nht + 1 > l.htb || nct + 1 > l.ctb || nht + 1 > l.pqh ||
+                   nct + 1 > l.pqc || NS * NB + 1 > l.cnt || NB + 1 > l.ngrp;
+  if (bad && !d->fail) d->fail = 2;
+}
+
+// Long rows, expansion slots and image groups, once the table is scanned.
+__global__ void k_pb_dims2(const uint32_t *lidxr, const uint32_t *cnt, const uint32_t *ngrp,
+                           uint32_t V, uint32_t NS, PbLimits l, PbDims *d) {
+  if (d->fail) return;
+  d->n_long = lidxr[V];
+  d->P1 = cnt[d->table];
+  d->G = ngrp[d->NB];
+  if (!l.check) return;
+  const unsigned long long items = ((unsigned long long)d->P1 + PB_CH - 1) / PB_CH + NS + 1;
+  // (no expansion items without cold source blocks)
+  if (d->P1 + 8ull > l.lidx || d->P1 + 8ull > l.vals || d->G + 8ull > l.gaddr ||
+      (NS && items > l.items))
+    d->fail = 3;
+}
+
+// The result in vertex order from whichever buffer the last update wrote
+// (its parity is known only on the device in a captured call).
+__global__ void k_pb_unpermute(const float *x0, const float *x1, const PrCtl *ctl,
+                               const PbDims *dims, const uint32_t *newid, uint32_t V,
+                               float *out) {
+  if (dims->fail) return;
+  const float *x = ctl->iters & 1u ? x1 : x0;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = x[newid[v]];
+}
+
+gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &w, hipStream_t s,
+                              bool capture) {
   const uint32_t V = in.V;
   w.pb_ready = 0;
   w.pf_ready = 0;
   if (!V) return GV_E_UNSUPPORTED;
   const uint32_t NS = V > PB_H ? (uint32_t)cdiv(V - PB_H, PB_SB) : 0u;
   if (NS > PB_MAX_NS) return GV_E_UNSUPPORTED;
+  if (capture && (w.pb_res_V != V || !w.pb_dims || w.V < V || !w.ctl)) {
+    set_error("pagerank capture: call gv_pagerank_reserve for this graph first");
+    return GV_E_INVALID;
+  }
   gv_status st = pr_reserve(w, V);
   if (st) return st;
   PbCaps &cp = w.pb_caps;
+  if (!w.pb_dims) {
+    ++w.gen;
+    GV_HIP(hipMalloc(&w.pb_dims, sizeof(PbDims)));
+    GV_HIP(hipHostMalloc(&w.h_pb_dims, sizeof(PbDims), hipHostMallocDefault));
+  }
+  PbDims *dims = w.pb_dims;
+  k_pb_dims_reset<<<1, 1, 0, s>>>(dims);
   // hot entries per snapshot item, counted while the snapshot is written
-  if ((st = pb_grow((void **)&w.pb_hitem, cp.hitem, (pr_snapshot_icap(in) + 1) * 4))) return st;
-  st = pr_snapshot(in, deg_out, true, w, s, 0, w.pb_hitem, PB_H);
+  if (!capture && (st = pb_grow(w, (void **)&w.pb_hitem, cp.hitem, (pr_snapshot_icap(in) + 1) * 4)))
+    return st;
+  st = pr_snapshot(in, deg_out, true, w, s, 0, w.pb_hitem, PB_H, capture ? &dims->fail : nullptr);
   if (st) return st;
-  const uint64_t E = w.snap_E;
   // stream positions, expansion positions and the segment table are 32-bit
   const uint32_t BT = pb_bin_tiles(NS);
   const uint32_t bin = BT * PF_T;
-  if (E + 2ull * V + 2ull * bin >= 0x7FFFFFFFull) return GV_E_UNSUPPORTED;
+  if (!capture && w.snap_E + 2ull * V + 2ull * bin >= 0x7FFFFFFFull) return GV_E_UNSUPPORTED;
   const uint32_t ni_cap = (uint32_t)w.snap_icap;
   const size_t vfloor = (size_t)(V + 1) * 4;
-  if ((st = pb_grow((void **)&w.pb_hoff, cp.hoff, vfloor)) ||
-      (st = pb_grow((void **)&w.pb_coff, cp.coff, vfloor)) ||
-      (st = pb_grow((void **)&w.pb_lidxr, cp.lidxr, vfloor)) ||
-      (st = pb_grow((void **)&w.pb_long, cp.lng, vfloor)) ||
-      (st = pb_grow((void **)&w.pb_Sh, cp.Sh, (size_t)V * 8)) ||
-      (st = pb_grow((void **)&w.pb_Sc, cp.Sc, (size_t)V * 8)) ||
-      (st = pb_grow((void **)&w.pb_partials, cp.partials, PB_PARTS * sizeof(double))))
+  if (!capture &&
+      ((st = pb_grow(w, (void **)&w.pb_hoff, cp.hoff, vfloor)) ||
+       (st = pb_grow(w, (void **)&w.pb_coff, cp.coff, vfloor)) ||
+       (st = pb_grow(w, (void **)&w.pb_lidxr, cp.lidxr, vfloor)) ||
+       (st = pb_grow(w, (void **)&w.pb_long, cp.lng, vfloor)) ||
+       (st = pb_grow(w, (void **)&w.pb_Sh, cp.Sh, (size_t)V * 8)) ||
+       (st = pb_grow(w, (void **)&w.pb_Sc, cp.Sc, (size_t)V * 8)) ||
+       (st = pb_grow(w, (void **)&w.pb_partials, cp.partials, PB_PARTS * sizeof(double)))))
     return st;
   // hot counts per item scanned; row lengths of both streams, scanned
   scan_u32(w.pb_hitem, w.pb_hitem, ni_cap, w.snap_istart + V, w.snap_tmp, s);
@@ -960,100 +1099,183 @@ gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &
                                           w.pb_hoff, w.pb_coff);
   scan_u32(w.pb_hoff, w.pb_hoff, V, nullptr, w.snap_tmp, s);
   scan_u32(w.pb_coff, w.pb_coff, V, nullptr, w.snap_tmp, s);
-  uint32_t tot[2] = {0, 0};
-  GV_HIP(hipMemcpyAsync(&tot[0], w.pb_hoff + V, 4, hipMemcpyDeviceToHost, s));
-  GV_HIP(hipMemcpyAsync(&tot[1], w.pb_coff + V, 4, hipMemcpyDeviceToHost, s));
-  GV_HIP(hipStreamSynchronize(s));
-  const uint64_t Eh = tot[0], Ec = tot[1];
-  const uint64_t nht = cdiv(Eh, PF_T), nct = cdiv(Ec, PF_T);
-  const uint64_t hend = nht * PF_T, cend = nct * PF_T;
-  const uint32_t NB = (uint32_t)cdiv(nct, BT);
-  const uint64_t table = (uint64_t)NS * NB;
-  if (table >= (1ull << 29)) return GV_E_UNSUPPORTED;
-  // the streams are sized for the snapshot buffer's capacity, so a caller
-  // that presized it (the request loop) does not reallocate here either
-  const uint64_t ecap = std::max<uint64_t>(E, w.snap_min_cap) + V;
-  const uint64_t tcap = cdiv(ecap, PF_T) + 2;
-  const uint64_t bcap = cdiv(tcap, PB_MIN_BIN_TILES) + 1;
-  if ((st = pb_grow((void **)&w.pb_hot, cp.hot, hend * 2, tcap * PF_T * 2)) ||
-      (st = pb_grow((void **)&w.pb_cold, cp.cold, cend * 2, tcap * PF_T * 2)) ||
-      (st = pb_grow((void **)&w.pb_csrc, cp.csrc, cend * 4, tcap * PF_T * 4)) ||
-      (st = pb_grow((void **)&w.pb_htb, cp.htb, (nht + 1) * 4, tcap * 4)) ||
-      (st = pb_grow((void **)&w.pb_ctb, cp.ctb, (nct + 1) * 4, tcap * 4)) ||
-      (st = pb_grow((void **)&w.pb_PQh, cp.PQh, (nht + 1) * 16, tcap * 16)) ||
-      (st = pb_grow((void **)&w.pb_PQc, cp.PQc, (nct + 1) * 16, tcap * 16)) ||
-      (st = pb_grow((void **)&w.pb_cnt, cp.cnt, (table + 1) * 4)) ||
-      (st = pb_grow((void **)&w.pb_ngrp, cp.ngrp, ((size_t)NB + 1) * 4, (bcap + 1) * 4)) ||
-      (st = pb_grow((void **)&w.pb_nit, cp.nit, ((size_t)NS + 1) * 4 + 8)) ||
-      (st = pb_grow((void **)&w.pb_tmp, cp.tmp,
-                    std::max(scan_tmp_bytes(table + 1), scan_tmp_bytes((uint64_t)bcap + 1)))))
-    return st;
+  const PbLimits lim = pb_limits(w, capture);
+  k_pb_dims1<<<1, 1, 0, s>>>(w.pb_hoff, w.pb_coff, w.snap_off, V, NS, BT, lim, dims);
+  // launch sizes: the layout's own (read back) or, captured, the reserved ones
+  uint64_t nht, nct, NB, table;
+  if (capture) {
+    nht = cp.htb / 4 - 1;
+    nct = cp.ctb / 4 - 1;
+    NB = cp.ngrp / 4 - 1;
+    table = cp.cnt / 4 - 1;
+  } else {
+    GV_HIP(hipMemcpyAsync(w.h_pb_dims, dims, sizeof(PbDims), hipMemcpyDeviceToHost, s));
+    GV_HIP(hipStreamSynchronize(s));
+    const PbDims &h = *w.h_pb_dims;
+    nht = h.nht;
+    nct = h.nct;
+    NB = h.NB;
+    table = h.table;
+    if (table >= (1ull << 29)) return GV_E_UNSUPPORTED;
+    // the streams are sized for the snapshot buffer's capacity, so a caller
+    // that presized it (the request loop) does not reallocate here either
+    const uint64_t E = w.snap_E;
+    const uint64_t ecap = std::max<uint64_t>(E, w.snap_min_cap) + V;
+    const uint64_t tcap = cdiv(ecap, PF_T) + 2;
+    const uint64_t bcap = cdiv(tcap, PB_MIN_BIN_TILES) + 1;
+    const uint64_t hend = h.hend, cend = h.cend;
+    if ((st = pb_grow(w, (void **)&w.pb_hot, cp.hot, hend * 2, tcap * PF_T * 2)) ||
+        (st = pb_grow(w, (void **)&w.pb_cold, cp.cold, cend * 2, tcap * PF_T * 2)) ||
+        (st = pb_grow(w, (void **)&w.pb_csrc, cp.csrc, cend * 4, tcap * PF_T * 4)) ||
+        (st = pb_grow(w, (void **)&w.pb_htb, cp.htb, (nht + 1) * 4, tcap * 4)) ||
+        (st = pb_grow(w, (void **)&w.pb_ctb, cp.ctb, (nct + 1) * 4, tcap * 4)) ||
+        (st = pb_grow(w, (void **)&w.pb_PQh, cp.PQh, (nht + 1) * 16, tcap * 16)) ||
+        (st = pb_grow(w, (void **)&w.pb_PQc, cp.PQc, (nct + 1) * 16, tcap * 16)) ||
+        (st = pb_grow(w, (void **)&w.pb_cnt, cp.cnt, (table + 1) * 4)) ||
+        (st = pb_grow(w, (void **)&w.pb_ngrp, cp.ngrp, ((size_t)NB + 1) * 4, (bcap + 1) * 4)) ||
+        (st = pb_grow(w, (void **)&w.pb_nit, cp.nit, ((size_t)NS + 1) * 4 + 8)) ||
+        (st = pb_grow(w, (void **)&w.pb_tmp, cp.tmp,
+                      std::max(scan_tmp_bytes(table + 1), scan_tmp_bytes((uint64_t)bcap + 1)))))
+      return st;
+  }
   // both streams
   k_pb_split<<<grid_for((uint64_t)ni_cap * WAVE, 256, 2048), 256, 0, s>>>(
       w.snap_istart, w.snap_ivert, w.snap_ipos, V, w.rl_newid, w.snap_off, w.snap_cols,
-      w.pb_hitem, w.pb_hoff, w.pb_coff, w.pb_hot, w.pb_csrc);
+      w.pb_hitem, w.pb_hoff, w.pb_coff, w.pb_hot, w.pb_csrc, dims);
   k_pb_pads<<<grid_for(V), 256, 0, s>>>(w.snap_istart, w.pb_hitem, w.rl_order, w.snap_off, V,
-                                        w.pb_hoff, w.pb_coff, w.pb_hot, w.pb_csrc);
-  k_pb_tail<<<4, 256, 0, s>>>(w.pb_hoff, w.pb_coff, V, hend, cend, w.pb_hot, w.pb_csrc);
-  if (nht) k_pb_tbase<<<grid_for(nht), 256, 0, s>>>(w.pb_hoff, V, (uint32_t)nht, w.pb_htb);
-  if (nct) k_pb_tbase<<<grid_for(nct), 256, 0, s>>>(w.pb_coff, V, (uint32_t)nct, w.pb_ctb);
+                                        w.pb_hoff, w.pb_coff, w.pb_hot, w.pb_csrc, dims);
+  k_pb_tail<<<4, 256, 0, s>>>(w.pb_hoff, w.pb_coff, V, dims, w.pb_hot, w.pb_csrc);
+  if (nht) k_pb_tbase<<<grid_for(nht), 256, 0, s>>>(w.pb_hoff, V, dims, 0u, w.pb_htb);
+  if (nct) k_pb_tbase<<<grid_for(nct), 256, 0, s>>>(w.pb_coff, V, dims, 1u, w.pb_ctb);
   k_pb_longflag<<<grid_for(V), 256, 0, s>>>(w.pb_hoff, w.pb_coff, V, w.pb_lidxr);
   scan_u32(w.pb_lidxr, w.pb_lidxr, V, nullptr, w.snap_tmp, s);
   k_pb_longfill<<<grid_for(V), 256, 0, s>>>(w.pb_lidxr, V, w.pb_long);
   // cold bins: padded (source block, bin) counts, expansion positions, image
   // group offsets
   if (NS) {
-    k_pb_bincnt<<<NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, cend, NS, NB, bin, w.pb_cnt,
-                                                w.pb_ngrp);
+    k_pb_bincnt<<<(unsigned)NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, dims, NS, bin, w.pb_cnt,
+                                                          w.pb_ngrp);
   } else {
     fill_u32(w.pb_ngrp, 0, NB, s);
   }
-  scan_u32(w.pb_cnt, w.pb_cnt, table, nullptr, w.pb_tmp, s);
-  scan_u32(w.pb_ngrp, w.pb_ngrp, NB, nullptr, w.pb_tmp, s);
-  uint32_t n_long = 0, ptot[2] = {0, 0};
-  GV_HIP(hipMemcpyAsync(&n_long, w.pb_lidxr + V, 4, hipMemcpyDeviceToHost, s));
-  GV_HIP(hipMemcpyAsync(&ptot[0], w.pb_cnt + table, 4, hipMemcpyDeviceToHost, s));
-  GV_HIP(hipMemcpyAsync(&ptot[1], w.pb_ngrp + NB, 4, hipMemcpyDeviceToHost, s));
-  GV_HIP(hipStreamSynchronize(s));
-  const uint64_t P1 = ptot[0], G = ptot[1];
-  const uint64_t pcap = std::max<uint64_t>(P1, ecap + ecap / 8) + 8;
-  if ((st = pb_grow((void **)&w.pb_lidx, cp.lidx, (P1 + 8) * 2, pcap * 2)) ||
-      (st = pb_grow((void **)&w.pb_vals, cp.vals, (P1 + 8) * 4, pcap * 4)) ||
-      (st = pb_grow((void **)&w.pb_gaddr, cp.gaddr, (G + 8) * 4, pcap)))
-    return st;
+  scan_u32(w.pb_cnt, w.pb_cnt, table, capture ? &dims->table : nullptr, w.pb_tmp, s);
+  scan_u32(w.pb_ngrp, w.pb_ngrp, NB, capture ? &dims->NB : nullptr, w.pb_tmp, s);
+  k_pb_dims2<<<1, 1, 0, s>>>(w.pb_lidxr, w.pb_cnt, w.pb_ngrp, V, NS, lim, dims);
+  uint64_t pcap = 0;
+  if (!capture) {
+    GV_HIP(hipMemcpyAsync(w.h_pb_dims, dims, sizeof(PbDims), hipMemcpyDeviceToHost, s));
+    GV_HIP(hipStreamSynchronize(s));
+    const uint64_t P1 = w.h_pb_dims->P1, G = w.h_pb_dims->G;
+    const uint64_t ecap = std::max<uint64_t>(w.snap_E, w.snap_min_cap) + V;
+    pcap = std::max<uint64_t>(P1, ecap + ecap / 8) + 8;
+    if ((st = pb_grow(w, (void **)&w.pb_lidx, cp.lidx, (P1 + 8) * 2, pcap * 2)) ||
+        (st = pb_grow(w, (void **)&w.pb_vals, cp.vals, (P1 + 8) * 4, pcap * 4)) ||
+        (st = pb_grow(w, (void **)&w.pb_gaddr, cp.gaddr, (G + 8) * 4, pcap)))
+      return st;
+  }
   if (NS) {
-    k_pb_scatter<<<NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, cend, NS, NB, bin, w.pb_cnt,
-                                                 w.pb_ngrp, w.pb_gaddr, w.pb_lidx, w.pb_cold);
-    k_pb_itemcnt<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, NB, w.pb_nit);
+    k_pb_scatter<<<(unsigned)NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, dims, NS, bin, w.pb_cnt,
+                                                           w.pb_ngrp, w.pb_gaddr, w.pb_lidx,
+                                                           w.pb_cold);
+    k_pb_itemcnt<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, dims, w.pb_nit);
     scan_u32(w.pb_nit, w.pb_nit, NS, nullptr, w.pb_tmp, s);
-    const uint64_t icap = cdiv(std::max<uint64_t>(pcap, 1), PB_CH) + NS + 1;
-    if ((st = pb_grow((void **)&w.pb_items, cp.items, icap * 16))) return st;
-    k_pb_itemfill<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, NB, w.pb_nit, w.pb_items);
+    if (!capture) {
+      const uint64_t icap = cdiv(std::max<uint64_t>(pcap, 1), PB_CH) + NS + 1;
+      if ((st = pb_grow(w, (void **)&w.pb_items, cp.items, icap * 16))) return st;
+    }
+    k_pb_itemfill<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, dims, w.pb_nit, w.pb_items);
   } else {
     // every cold entry is padding: the stream's entries all take index PB_PAD
-    k_pb_scatter<<<NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, cend, 0, NB, bin, w.pb_cnt,
-                                                 w.pb_ngrp, w.pb_gaddr, w.pb_lidx, w.pb_cold);
+    k_pb_scatter<<<(unsigned)NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, dims, 0, bin, w.pb_cnt,
+                                                           w.pb_ngrp, w.pb_gaddr, w.pb_lidx,
+                                                           w.pb_cold);
     fill_u32(w.pb_nit, 0, 1, s);
   }
   GV_HIP(hipGetLastError());
   w.pb_NS = NS;
-  w.pb_NB = NB;
+  w.pb_NB = (uint32_t)NB;
   w.pb_bin_tiles = BT;
   w.pb_nht = (uint32_t)nht;
   w.pb_nct = (uint32_t)nct;
-  w.pb_n_long = n_long;
+  w.pb_n_long = capture ? 0u : w.h_pb_dims->n_long;
   w.pb_prepV = V;
-  w.pb_E = E;
+  w.pb_E = capture ? 0 : w.snap_E;
   w.pb_ready = 1;
   return GV_OK;
 }
 
+gv_status pagerank_pb_reserve(const Store &in, const uint32_t *deg_out, PrWork &w,
+                              uint64_t max_edges, hipStream_t s) {
+  const uint32_t V = in.V;
+  if (!V) return GV_E_UNSUPPORTED;
+  const uint32_t NS = V > PB_H ? (uint32_t)cdiv(V - PB_H, PB_SB) : 0u;
+  const uint32_t BT = pb_bin_tiles(NS);
+  if (NS > PB_MAX_NS || max_edges + 2ull * V + 2ull * BT * PF_T >= 0x7FFFFFFFull) {
+    set_error("gv_pagerank_reserve: the default layout's 32-bit positions do not cover this size");
+    return GV_E_UNSUPPORTED;
+  }
+  w.pb_res_V = 0;
+  if (w.snap_min_cap < max_edges) w.snap_min_cap = max_edges;
+  // a plain preparation of the current graph sizes every buffer with the
+  // headroom of snap_min_cap
+  gv_status st = pagerank_pb_prepare(in, deg_out, w, s, false);
+  if (st) return st;
+  if (w.snap_cap < w.snap_min_cap + PF_T) {
+    ++w.gen;
+    hipFree(w.snap_cols);
+    w.snap_cols = nullptr;
+    w.snap_cap = 0;
+    const uint64_t cap = w.snap_min_cap + w.snap_min_cap / 8 + PF_T;
+    GV_HIP(hipMalloc(&w.snap_cols, cap * sizeof(uint32_t)));
+    w.snap_cap = cap;
+  }
+  const uint64_t ecap = std::max<uint64_t>(w.snap_E, w.snap_min_cap) + V;
+  const uint64_t tcap = cdiv(ecap, PF_T) + 2;
+  const uint64_t bcap = cdiv(tcap, PB_MIN_BIN_TILES) + 1;
+  const uint64_t table = (uint64_t)NS * bcap;
+  if (table >= (1ull << 29)) {
+    set_error("gv_pagerank_reserve: too many (source block, bin) pairs for the default layout");
+    return GV_E_UNSUPPORTED;
+  }
+  PbCaps &cp = w.pb_caps;
+  if ((st = pb_grow(w, (void **)&w.pb_cnt, cp.cnt, (table + 1) * 4)) ||
+      (st = pb_grow(w, (void **)&w.pb_tmp, cp.tmp,
+                    std::max(scan_tmp_bytes(table + 1), scan_tmp_bytes(bcap + 1)))))
+    return st;
+  GV_HIP(hipStreamSynchronize(s));
+  w.pb_res_V = V;
+  w.pb_res_E = max_edges;
+  return GV_OK;
+}
+
+gv_status pagerank_pb_last(PrWork &w, int *iters, double *residual, hipStream_t s) {
+  if (!w.pb_dims || !w.ctl) {
+    set_error("no PageRank call has run on this handle");
+    return GV_E_INVALID;
+  }
+  GV_HIP(hipMemcpyAsync(w.h_pb_dims, w.pb_dims, sizeof(PbDims), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+  GV_HIP(hipStreamSynchronize(s));
+  if (w.h_pb_dims->fail) {
+    set_error("the last PageRank call's snapshot exceeded the buffers reserved for it "
+              "(gv_pagerank_reserve); nothing was computed");
+    return GV_E_INVALID;
+  }
+  if (iters) *iters = (int)w.h_ctl->iters;
+  if (residual) *residual = w.h_ctl->residual;
+  return GV_OK;
+}
+
 gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *iters,
-                              double *residual, hipStream_t s, double *iter_ms) {
+                              double *residual, hipStream_t s, double *iter_ms, bool capture) {
   if (iters) *iters = 0;
   if (residual) *residual = 0.0;
   if (!w.pb_ready || o.max_iter <= 0) {
     set_error("pagerank: no prepared snapshot, or max_iter <= 0");
+    return GV_E_INVALID;
+  }
+  if (capture && (iters || residual || iter_ms || o.yield)) {
+    set_error("pagerank capture: no statistics or yield hook inside a capture");
     return GV_E_INVALID;
   }
   const uint32_t V = w.pb_prepV;
@@ -1086,14 +1308,12 @@ gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *i
   a.longrows = w.pb_long;
   a.partials = w.pb_partials;
   a.ctl = w.ctl;
+  a.dims = w.pb_dims;
   a.V = V;
   a.nh = V < PB_H ? V : PB_H;
-  a.n_htiles = w.pb_nht;
-  a.n_ctiles = w.pb_nct;
-  a.n_hot_units = (uint32_t)cdiv(w.pb_nht, PB_HOT_UNIT);
-  a.n_bins = w.pb_NB;
+  // tile, bin and long-row counts are read from a.dims by the kernels
+  a.n_htiles = a.n_ctiles = a.n_hot_units = a.n_bins = a.n_long = 0;
   a.bin_tiles = w.pb_bin_tiles;
-  a.n_long = w.pb_n_long;
   a.compensated = o.compensated ? 1u : 0u;
   a.max_iter = (uint32_t)o.max_iter;
   a.alpha = o.alpha;
@@ -1107,6 +1327,27 @@ gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *i
   a.x = w.x[0];
   a.cb = w.contrib[0];
   if (w.pb_NS) k_pb_expand<<<egrid - 1, PB_BUILD_THREADS, 0, s>>>(a, 0u);
+  auto launch_iter = [&](int t) {
+    a.x = w.x[t & 1];
+    a.cb = w.contrib[t & 1];
+    a.xn = w.x[(t + 1) & 1];
+    a.cn = w.contrib[(t + 1) & 1];
+    if (a.compensated) k_pb_tiles<true><<<grid, PB_THREADS, 0, s>>>(a);
+    else k_pb_tiles<false><<<grid, PB_THREADS, 0, s>>>(a);
+    k_pb_rows<<<PB_PARTS, PB_ROW_THREADS, 0, s>>>(a);
+    // the residual, and the expansion of the contributions just written
+    PbArgs ae = a;
+    ae.cb = a.cn;
+    k_pb_expand<<<w.pb_NS ? egrid : 1, PB_BUILD_THREADS, 0, s>>>(ae, 1u);
+  };
+  if (capture) {
+    // every launch is recorded; those after convergence exit at once
+    for (int t = 0; t < o.max_iter; ++t) launch_iter(t);
+    k_pb_unpermute<<<grid_for(V), 256, 0, s>>>(w.x[0], w.x[1], w.ctl, w.pb_dims, w.rl_newid, V,
+                                               out_dev);
+    GV_HIP(hipGetLastError());
+    return GV_OK;
+  }
   if (iter_ms) {
     while (w.ev.size() < (size_t)o.max_iter + 1) {
       hipEvent_t e;
@@ -1129,17 +1370,7 @@ gv_status pagerank_pb_iterate(const PrOpts &o, PrWork &w, float *out_dev, int *i
   while (launched < o.max_iter) {
     const int t = launched;
     if (o.yield) o.yield(o.yield_arg);
-    a.x = w.x[t & 1];
-    a.cb = w.contrib[t & 1];
-    a.xn = w.x[(t + 1) & 1];
-    a.cn = w.contrib[(t + 1) & 1];
-    if (a.compensated) k_pb_tiles<true><<<grid, PB_THREADS, 0, s>>>(a);
-    else k_pb_tiles<false><<<grid, PB_THREADS, 0, s>>>(a);
-    k_pb_rows<<<PB_PARTS, PB_ROW_THREADS, 0, s>>>(a);
-    // the residual, and the expansion of the contributions just written
-    PbArgs ae = a;
-    ae.cb = a.cn;
-    k_pb_expand<<<w.pb_NS ? egrid : 1, PB_BUILD_THREADS, 0, s>>>(ae, 1u);
+    launch_iter(t);
     if (iter_ms) GV_HIP(hipEventRecord(w.ev[t + 1], s));
     ++launched;
     if (launched % group == 0 && launched < o.max_iter) {

@@ -185,6 +185,18 @@ typedef struct gv_pagerank_opts {
 gv_status gv_pagerank_opts_default(gv_pagerank_opts *opts);
 gv_status gv_pagerank_run(gv_graph *g, const gv_pagerank_opts *opts, float *out, int mem,
                           gv_pagerank_stats *stats);
+/* Sizes every buffer of the default PageRank layout for this graph's vertex
+ * count and up to max_edges live (in-)edges, so that gv_pagerank_run on
+ * such a graph allocates nothing and needs no host round trip inside a
+ * capture (below).  Needs pr_layout 5; GV_E_UNSUPPORTED when the layout's
+ * 32-bit positions do not cover max_edges.  No reference counterpart (for
+ * captured update -> compact -> PageRank pipelines). */
+gv_status gv_pagerank_reserve(gv_graph *g, uint64_t max_edges);
+/* Iterations and final residual of the last default-layout PageRank call on
+ * the handle, captured calls included (it waits for the handle's stream);
+ * GV_E_INVALID when that call's snapshot exceeded the reservation (nothing
+ * was computed then).  edges: live in-adjacency size now. */
+gv_status gv_pagerank_last(gv_graph *g, gv_pagerank_stats *stats);
 /* Personalized PageRank for up to 64 seeds in one FP32 column block: column
  * c starts at e_{seeds[c]} and teleports there, x_c' = (1-alpha) e_{s_c} +
  * alpha * sum_{u in N_in(v)} x_c[u]/|N(u)| (dangling mass dropped, as in
@@ -265,24 +277,32 @@ gv_status gv_read_matrix_market(const char *path, uint32_t *num_vertices, uint64
 gv_status gv_get_config(gv_graph *g, gv_config *cfg);
 gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg);
 
-/* Captured pipelines.  Records update and compaction calls issued on the
- * handle's stream into one HIP graph that gv_capture_launch replays with a
- * single launch.  No reference counterpart (the reference runs each stage
- * synchronously, graphvine.cu:222-464); for callers that replay a fixed
- * update -> compact sequence over device buffers they refill between
+/* Captured pipelines.  Records update, compaction and PageRank calls issued
+ * on the handle's stream into one HIP graph that gv_capture_launch replays
+ * with a single launch.  No reference counterpart (the reference runs each
+ * stage synchronously, graphvine.cu:222-464, with a host read per PageRank
+ * iteration, pagerank.cu:46); for callers that replay a fixed update ->
+ * compact -> PageRank sequence over device buffers they refill between
  * launches.
  *   - gv_reserve sizes the staging buffers for groups of up to max_pairs
  *     (src,dst) pairs and max_batches batches: a call inside a capture never
  *     allocates device memory and fails if the reservation is too small;
+ *   - gv_pagerank_reserve sizes the PageRank buffers the same way: a
+ *     captured gv_pagerank_run builds its snapshot and layout on the device
+ *     and records max_iter iterations (those after convergence exit at
+ *     once); a replay whose graph outgrew the reservation computes nothing
+ *     and gv_pagerank_last then fails;
  *   - between gv_capture_begin and gv_capture_end only gv_batch_insert,
  *     gv_batch_delete and gv_batch_apply with GV_MEM_DEVICE inputs and NULL
- *     stats, and gv_compact with a NULL count, are allowed; every other call
- *     on the handle fails and a failed call makes gv_capture_end fail;
+ *     stats, gv_compact with a NULL count, and gv_pagerank_run with device
+ *     vectors and NULL stats are allowed; every other call on the handle
+ *     fails and a failed call makes gv_capture_end fail;
  *   - a replay reads the device inputs at the addresses given at capture time
  *     (their contents may change between launches) and the batch offsets and
  *     kinds given at capture time;
  *   - a later reallocation of a buffer the capture uses (a group larger than
- *     reserved) invalidates it: gv_capture_launch then returns GV_E_INVALID;
+ *     reserved, a plain PageRank call on a graph larger than reserved)
+ *     invalidates it: gv_capture_launch then returns GV_E_INVALID;
  *   - gv_capture_launch with stats waits for the replay; the counters in
  *     stats describe its last update call, live_edges the whole graph.
  * A capture outliving its graph can only be destroyed. */
