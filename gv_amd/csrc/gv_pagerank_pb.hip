@@ -25,8 +25,11 @@
 // contributions are staged in LDS once per workgroup and iteration, as in
 // the flat stream.  Both streams store 2-byte entries: a 15-bit index and
 // the row-start flag.  Every vertex has a row in both streams (a padding
-// entry, index PB_PAD, whose LDS slot holds 0, stands in for an empty one),
-// so a stream's row number is the vertex label.
+// entry, index PB_PAD in the hot stream and PB_CPAD in the cold one, whose
+// LDS slot holds 0, stands in for an empty one), so a stream's row number is
+// the vertex label.  A workgroup's cold bins go through two LDS halves: the
+// next bin's image is loaded into registers while the current bin's tiles
+// are summed from one half, then written into the other.
 //
 // Per iteration: k_pb_tiles (hot tiles on some workgroups, cold bins on the
 // rest) -> k_pb_rows (joins the hot and cold row sums, the update, residual
@@ -37,11 +40,18 @@
 
 namespace gv {
 
-// LDS image of a workgroup: the hot contributions, or one cold bin's values;
-// its last slot, PB_PAD, holds 0 and is the index of every padding entry.
-// With the work-unit word it fills exactly 80 KiB: two workgroups per CU.
-constexpr uint32_t PB_IMG = 20479;            // floats
-constexpr uint32_t PB_PAD = PB_IMG - 1;
+// LDS of a tile workgroup, 80 KiB (two workgroups per CU): PB_LDS words
+// holding either the hot contributions (labels [0, PB_H), slot PB_PAD = 0,
+// the index of the hot stream's padding entries) or two cold bin images of
+// up to PB_CPAD values at PB_HALF * h (slot PB_HALF * h + PB_CPAD = 0, the
+// cold stream's padding index); the last three words deal the work units.
+constexpr uint32_t PB_LDS = 20480;            // words
+constexpr uint32_t PB_PAD = 20476;
+constexpr uint32_t PB_HALF = 10240;           // stride of the two cold halves
+constexpr uint32_t PB_CPAD = 10236;           // zero slot of a cold half (relative)
+constexpr uint32_t PB_W_UNIT = 20477;         // hot unit being worked on
+constexpr uint32_t PB_W_NEXT = 20478;         // [2]: the cold unit after the current one
+static_assert(PB_HALF + PB_CPAD == PB_PAD && PB_W_NEXT + 2 == PB_LDS, "LDS word map");
 constexpr uint32_t PB_F16 = 0x8000u;          // row-start flag of a stream entry
 constexpr uint32_t PB_IDX = 0x7FFFu;          // index field of a stream entry
 // hot source labels: [0, PB_H); a multiple of 4, so the expansion windows
@@ -50,22 +60,26 @@ constexpr uint32_t PB_H = PB_PAD & ~3u;
 constexpr uint32_t PB_SB = 32768;             // cold sources per expansion block (128 KiB)
 // A bin's image holds its cold values in source-block order, each block's
 // segment padded to whole groups of 4 (16 B): at most bin_tiles * PF_T +
-// 3 * NS slots, which must stay below PB_PAD.  bin_tiles is chosen per graph
-// as the largest multiple of the workgroup's wavefront count that fits: more
-// tiles per bin make longer segments and more work per image load.
+// 3 * NS slots, which must stay below PB_CPAD.  A bin is one tile per
+// wavefront of the workgroup (8 tiles), so that while its tiles are summed
+// the next bin's image loads are in flight; that bounds the cold source
+// blocks to PB_MAX_NS (V up to 22.3M; beyond, the flat layout is used).
 constexpr uint32_t PB_MIN_BIN_TILES = 8;
-constexpr uint32_t PB_MAX_NS = 4095;
-static_assert(PB_MIN_BIN_TILES * PF_T + 3 * PB_MAX_NS < PB_PAD, "bin image overflows");
+constexpr uint32_t PB_MAX_NS = 681;
+static_assert(PB_MIN_BIN_TILES * PF_T + 3 * PB_MAX_NS < PB_CPAD, "bin image overflows");
 __host__ __device__ inline uint32_t pb_bin_tiles(uint32_t NS) {
-  const uint32_t room = (PB_PAD - 1 - 3 * NS) / PF_T;
-  return room / PB_MIN_BIN_TILES * PB_MIN_BIN_TILES;
+  (void)NS;
+  return PB_MIN_BIN_TILES;
 }
 // The image is filled in PB_ROUNDS rounds of PB_PER_ROUND 16-byte loads per
 // thread, each round's loads issued before its LDS writes (two rounds keep
 // the loaded values within the register budget).
 constexpr uint32_t PB_PER_ROUND = 5;
 constexpr uint32_t PB_ROUNDS = 2;
-static_assert(PB_ROUNDS * PB_PER_ROUND * 512 >= PB_PAD / 4, "the rounds cover the image");
+static_assert(PB_ROUNDS * PB_PER_ROUND * 512 >= PB_PAD / 4, "the rounds cover the hot image");
+// A cold bin image in PB_CROUNDS groups of 16 bytes per thread.
+constexpr uint32_t PB_CROUNDS = 5;
+static_assert(PB_CROUNDS * 512 * 4 >= PB_CPAD, "the rounds cover a cold image");
 constexpr uint32_t PB_HOT_UNIT = 32;          // hot tiles per work unit (4 per wavefront)
 constexpr uint32_t PB_CH = 1u << 15;          // entries per expansion work item (multiple of 4)
 constexpr int PB_THREADS = 512;               // tile workgroups: 8 wavefronts
@@ -334,7 +348,7 @@ __global__ __launch_bounds__(PB_BUILD_THREADS) void k_pb_scatter(
     const uint32_t raw = csrc[q];
     const uint32_t f = raw & PB_CSRC_FLAG ? PB_F16 : 0u;
     const uint32_t src = raw & ~PB_CSRC_FLAG;
-    uint32_t idx = PB_PAD;
+    uint32_t idx = PB_CPAD;
     if (src != PB_CSRC_PAD) {
       const uint32_t k = (src - PB_H) / PB_SB;
       const uint32_t pos = atomicAdd(&ctr[k], 1u);
@@ -592,15 +606,51 @@ __device__ __forceinline__ void pb_fill_round(float4 *img4, const float4 *src,
   if (R + 1 < PB_ROUNDS || g4 < PB_PAD / 4) img4[g4] = v4;
 }
 
+// A cold bin's group map entries for this thread: group k * PB_THREADS +
+// threadIdx.x of the bin's ng groups in ga[k] (0 past the end).
+__device__ __forceinline__ void pb_cold_map(uint32_t (&ga)[PB_CROUNDS], const uint32_t *gmap,
+                                            uint32_t ng) {
+#pragma unroll
+  for (uint32_t k = 0; k < PB_CROUNDS; ++k) {
+    const uint32_t g = k * PB_THREADS + threadIdx.x;
+    ga[k] = g < ng ? gmap[g] : 0u;
+  }
+}
+
+// The bin's image groups named by ga into registers (past ng: not loaded).
+__device__ __forceinline__ void pb_cold_load(float4 (&r)[PB_CROUNDS], const float4 *vals,
+                                             const uint32_t (&ga)[PB_CROUNDS], uint32_t ng) {
+#pragma unroll
+  for (uint32_t k = 0; k < PB_CROUNDS; ++k) {
+    const uint32_t g = k * PB_THREADS + threadIdx.x;
+    r[k] = g < ng ? vals[ga[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// ... and from registers into one LDS half (never its zero slot: an image
+// holds at most PB_CPAD values).
+__device__ __forceinline__ void pb_cold_store(float4 *half4, const float4 (&r)[PB_CROUNDS],
+                                              uint32_t ng) {
+#pragma unroll
+  for (uint32_t k = 0; k < PB_CROUNDS; ++k) {
+    const uint32_t g = k * PB_THREADS + threadIdx.x;
+    if (g < ng) half4[g] = r[k];
+  }
+}
+
 // Work units from one counter per iteration: first the hot units (PB_HOT_UNIT
 // tiles of the hot stream, staged hot contributions), then the cold bins (a
-// bin's values loaded into the image, then its bin_tiles tiles, dealt to the
-// wavefronts in turn).  A workgroup stages the hot contributions once, at its first
-// hot unit; once it has taken a bin it never takes a hot unit again.
+// bin's values loaded into an LDS half, then its tiles, one per wavefront).
+// A workgroup stages the hot contributions once, at its first hot unit; once
+// it has taken a bin it never takes a hot unit again.  Bins go through the
+// two halves in turn: the image of the next bin the workgroup was dealt is
+// loaded into registers while the current bin's tiles are summed, then
+// written into the other half; one barrier per bin orders both.
 template <bool COMP>
 __global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
-  __shared__ float s_img[PB_IMG];
-  __shared__ uint32_t s_unit;
+  static_assert(PB_THREADS / WAVE == PB_MIN_BIN_TILES, "one cold tile per wavefront and bin");
+  __shared__ float s_lds[PB_LDS];
+  uint32_t *s_words = reinterpret_cast<uint32_t *>(s_lds);
   if (a.ctl->done || !pb_sizes(a)) return;
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x / WAVE;
@@ -611,85 +661,120 @@ __global__ __launch_bounds__(PB_THREADS, 4) void k_pb_tiles(PbArgs a) {
   // counter's round trip overlaps the work
   uint32_t next = 0;
   if (threadIdx.x == 0) next = atomicAdd(cursor, 1u);
+  uint32_t u;
   for (;;) {
     if (threadIdx.x == 0) {
-      s_unit = next;
+      s_words[PB_W_UNIT] = next;
       next = atomicAdd(cursor, 1u);
     }
     __syncthreads();
-    const uint32_t u = s_unit;
+    u = s_words[PB_W_UNIT];
     __syncthreads();
-    if (u >= n_units) break;
-    if (u < a.n_hot_units) {
-      if (!staged) {
-        // every load of the stage is issued before the first LDS write; the
-        // vector part covers whole groups of 4 labels below nh (groups past
-        // them reload the last one, writing image slots no hot entry names)
-        const uint32_t nh4 = a.nh / 4;
-        if (nh4) {
-          const float4 *c4 = reinterpret_cast<const float4 *>(a.cb);
-          float4 *img4 = reinterpret_cast<float4 *>(s_img);
-          pb_fill_round<0>(img4, c4, nullptr, nh4);
-          pb_fill_round<1>(img4, c4, nullptr, nh4);
-        }
-        // the last partial group of hot labels, then the zero slot; the
-        // vector rounds above rewrite that group's slots with the reloaded
-        // last whole group, so its labels are written after a barrier
-        if (a.nh & 3u) __syncthreads();
-        for (uint32_t i = nh4 * 4 + threadIdx.x; i < a.nh; i += PB_THREADS) s_img[i] = a.cb[i];
-        if (threadIdx.x == 0) s_img[PB_PAD] = 0.f;
-        __syncthreads();
-        staged = true;
+    if (u >= n_units) return;
+    if (u >= a.n_hot_units) break;
+    if (!staged) {
+      // every load of the stage is issued before the first LDS write; the
+      // vector part covers whole groups of 4 labels below nh (groups past
+      // them reload the last one, writing image slots no hot entry names)
+      const uint32_t nh4 = a.nh / 4;
+      if (nh4) {
+        const float4 *c4 = reinterpret_cast<const float4 *>(a.cb);
+        float4 *img4 = reinterpret_cast<float4 *>(s_lds);
+        pb_fill_round<0>(img4, c4, nullptr, nh4);
+        pb_fill_round<1>(img4, c4, nullptr, nh4);
       }
-      // the wavefront's tiles of the unit, the next one's entries loaded
-      // before the current one is summed
-      constexpr uint32_t STEP = PB_THREADS / WAVE;
-      const uint32_t t0 = u * PB_HOT_UNIT + wv;
-      uint4 cur[2], nxt[2];
-      pb_load(a.hot, t0 < a.n_htiles ? t0 : 0u, lane, cur);
-      int32_t tb_cur = a.htbase[t0 < a.n_htiles ? t0 : 0u];
-      for (uint32_t t = t0; t < (u + 1) * PB_HOT_UNIT && t < a.n_htiles; t += STEP) {
-        const uint32_t tn = t + STEP < (u + 1) * PB_HOT_UNIT && t + STEP < a.n_htiles ? t + STEP : t;
-        pb_load(a.hot, tn, lane, nxt);
-        const int32_t tb_nxt = a.htbase[tn];
-        pb_tile_sums<COMP>(a.Sh, a.Ph, a.Qh, t, tb_cur, cur, s_img, lane, a.V);
-        cur[0] = nxt[0];
-        cur[1] = nxt[1];
-        tb_cur = tb_nxt;
-      }
-      continue;
+      // the last partial group of hot labels, then the zero slot; the
+      // vector rounds above rewrite that group's slots with the reloaded
+      // last whole group, so its labels are written after a barrier
+      if (a.nh & 3u) __syncthreads();
+      for (uint32_t i = nh4 * 4 + threadIdx.x; i < a.nh; i += PB_THREADS) s_lds[i] = a.cb[i];
+      if (threadIdx.x == 0) s_lds[PB_PAD] = 0.f;
+      __syncthreads();
+      staged = true;
     }
-    // cold bin: its image, group by group (16 B each), then its tiles
-    const uint32_t b = u - a.n_hot_units;
-    const uint32_t t0 = b * a.bin_tiles + wv;
-    const uint32_t t1 = (b + 1) * a.bin_tiles < a.n_ctiles ? (b + 1) * a.bin_tiles : a.n_ctiles;
-    uint4 ent[2];
-    pb_load(a.cold, t0 < t1 ? t0 : 0u, lane, ent);
-    int32_t tb = a.ctbase[t0 < t1 ? t0 : 0u];
-    const uint32_t g0 = a.gbase[b], ng = a.gbase[b + 1] - g0;
-    const float4 *v4 = reinterpret_cast<const float4 *>(a.vals);
-    float4 *img4 = reinterpret_cast<float4 *>(s_img);
-    // all image loads are issued before any image write; groups past the
-    // bin's end reload its last group and write image slots past its image
-    // (never named by an entry, and below the zero slot)
-    if (ng) {
-      pb_fill_round<0>(img4, v4, a.gaddr + g0, ng);
-      pb_fill_round<1>(img4, v4, a.gaddr + g0, ng);
-    }
-    if (threadIdx.x == 0) s_img[PB_PAD] = 0.f;
-    staged = false;
-    __syncthreads();
+    // the wavefront's tiles of the unit, the next one's entries loaded
+    // before the current one is summed
     constexpr uint32_t STEP = PB_THREADS / WAVE;
-    for (uint32_t t = t0; t < t1; t += STEP) {
-      uint4 nxt[2];
-      const uint32_t tn = t + STEP < t1 ? t + STEP : t;
-      pb_load(a.cold, tn, lane, nxt);
-      const int32_t tb_nxt = a.ctbase[tn];
-      pb_tile_sums<COMP>(a.Sc, a.Pc, a.Qc, t, tb, ent, s_img, lane, a.V);
-      ent[0] = nxt[0];
-      ent[1] = nxt[1];
-      tb = tb_nxt;
+    const uint32_t t0 = u * PB_HOT_UNIT + wv;
+    uint4 cur[2], nxt[2];
+    pb_load(a.hot, t0 < a.n_htiles ? t0 : 0u, lane, cur);
+    int32_t tb_cur = a.htbase[t0 < a.n_htiles ? t0 : 0u];
+    for (uint32_t t = t0; t < (u + 1) * PB_HOT_UNIT && t < a.n_htiles; t += STEP) {
+      const uint32_t tn = t + STEP < (u + 1) * PB_HOT_UNIT && t + STEP < a.n_htiles ? t + STEP : t;
+      pb_load(a.hot, tn, lane, nxt);
+      const int32_t tb_nxt = a.htbase[tn];
+      pb_tile_sums<COMP>(a.Sh, a.Ph, a.Qh, t, tb_cur, cur, s_lds, lane, a.V);
+      cur[0] = nxt[0];
+      cur[1] = nxt[1];
+      tb_cur = tb_nxt;
     }
+  }
+  // cold bins, from u on (every unit after a cold one is cold).  Three bins
+  // are in flight: u's tiles are summed from LDS half h while the image of
+  // the next one, n, is loaded into registers (through its group map, read
+  // one bin earlier) and the group map of the one after, m, is loaded.
+  // Thread 0 deals n and m ahead through two LDS words used in turn.
+  const float4 *v4 = reinterpret_cast<const float4 *>(a.vals);
+  float4 *img4 = reinterpret_cast<float4 *>(s_lds);
+  uint32_t *slot[2] = {&s_words[PB_W_NEXT], &s_words[PB_W_UNIT]};
+  if (threadIdx.x == 0) {
+    *slot[1] = next;  // n
+    next = atomicAdd(cursor, 1u);
+  }
+  {
+    // the first image straight into half 0, and the zero slots of both
+    // halves (the hot image held values there)
+    const uint32_t b = u - a.n_hot_units;
+    const uint32_t g0 = a.gbase[b], ng = a.gbase[b + 1] - g0;
+    uint32_t ga[PB_CROUNDS];
+    float4 r[PB_CROUNDS];
+    pb_cold_map(ga, a.gaddr + g0, ng);
+    pb_cold_load(r, v4, ga, ng);
+    pb_cold_store(img4, r, ng);
+    if (threadIdx.x == 0) {
+      s_lds[PB_CPAD] = 0.f;
+      s_lds[PB_HALF + PB_CPAD] = 0.f;
+    }
+  }
+  __syncthreads();
+  uint32_t n = *slot[1], ng_n = 0;
+  uint32_t ga_n[PB_CROUNDS];
+  if (n < n_units) {
+    const uint32_t g0 = a.gbase[n - a.n_hot_units];
+    ng_n = a.gbase[n - a.n_hot_units + 1] - g0;
+    pb_cold_map(ga_n, a.gaddr + g0, ng_n);
+  }
+  for (uint32_t i = 0, h = 0;; ++i, h ^= 1u) {
+    // this wavefront's tile of bin u, loaded first
+    const uint32_t t = (u - a.n_hot_units) * PB_MIN_BIN_TILES + wv;
+    const bool has_t = t < a.n_ctiles;
+    uint4 ent[2];
+    pb_load(a.cold, has_t ? t : 0u, lane, ent);
+    const int32_t tb = a.ctbase[has_t ? t : 0u];
+    if (threadIdx.x == 0) {
+      *slot[i & 1u] = next;  // m
+      next = atomicAdd(cursor, 1u);
+    }
+    // u's image is complete in half h, and the bin before it is no longer
+    // read from half h ^ 1
+    __syncthreads();
+    const uint32_t m = *slot[i & 1u];
+    float4 r[PB_CROUNDS];
+    if (n < n_units) pb_cold_load(r, v4, ga_n, ng_n);
+    uint32_t ga_m[PB_CROUNDS], ng_m = 0;
+    if (m < n_units) {
+      const uint32_t g0 = a.gbase[m - a.n_hot_units];
+      ng_m = a.gbase[m - a.n_hot_units + 1] - g0;
+      pb_cold_map(ga_m, a.gaddr + g0, ng_m);
+    }
+    if (has_t) pb_tile_sums<COMP>(a.Sc, a.Pc, a.Qc, t, tb, ent, s_lds + h * PB_HALF, lane, a.V);
+    if (n >= n_units) break;
+    pb_cold_store(img4 + (h ^ 1u) * (PB_HALF / 4), r, ng_n);
+    u = n;
+    n = m;
+    ng_n = ng_m;
+#pragma unroll
+    for (uint32_t k = 0; k < PB_CROUNDS; ++k) ga_n[k] = ga_m[k];
   }
 }
 
@@ -1185,7 +1270,7 @@ gv_status pagerank_pb_prepare(const Store &in, const uint32_t *deg_out, PrWork &
     }
     k_pb_itemfill<<<grid_for(NS), 256, 0, s>>>(w.pb_cnt, NS, dims, w.pb_nit, w.pb_items);
   } else {
-    // every cold entry is padding: the stream's entries all take index PB_PAD
+    // every cold entry is padding: the stream's entries all take index PB_CPAD
     k_pb_scatter<<<(unsigned)NB, PB_BUILD_THREADS, 0, s>>>(w.pb_csrc, dims, 0, bin, w.pb_cnt,
                                                            w.pb_ngrp, w.pb_gaddr, w.pb_lidx,
                                                            w.pb_cold);
