@@ -349,17 +349,20 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
       if (newid) dst = row0 + (dst - rs);
     }
     if (hi - lo <= SNAP_LANE) {
+      // rounds of 16 slots, their loads and then their 16 new labels issued
+      // together
       uint32_t hot = 0;
-      for (uint32_t p = lo; p < hi; p += 4) {
-        uint32_t x[4];
+#pragma unroll 1
+      for (uint32_t p = lo; p < hi; p += 16) {
+        uint32_t x[16];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k)
+        for (uint32_t k = 0; k < 16; ++k)
           x[k] = p + k < hi ? st.slots[slot_index(st.arena, tbl, p + k, st.log2B, st.B)] : TOMB;
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k)
+        for (uint32_t k = 0; k < 16; ++k)
           if (newid && x[k] != TOMB) x[k] = newid[x[k]];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
+        for (uint32_t k = 0; k < 16; ++k) {
           if (x[k] == TOMB) continue;
           snap_put(cols, flat, dst, row0, x[k]);
           ++dst;

@@ -125,7 +125,7 @@ __global__ void k_pb_rowlen(const uint32_t *istart, const uint32_t *hscan, const
 // source list, at the positions its row and the hot counts before it give.
 // Items are dealt to lanes (wave w of W, step r, lane l takes item
 // (r * 64 + l) * W + w); an item of at most PB_SPLIT_LANE entries is split by
-// its lane alone, four entries per load round, the longer ones by the whole
+// its lane alone, 16 entries per load round, the longer ones by the whole
 // wavefront, one after another.
 constexpr uint32_t PB_SPLIT_LANE = 64;
 
@@ -159,12 +159,13 @@ __global__ __launch_bounds__(256) void k_pb_split(const uint32_t *istart, const 
       cb = c0 + (uint32_t)inrow - hin;
     }
     if (len <= PB_SPLIT_LANE) {
-      for (uint32_t p = 0; p < len; p += 4) {
-        uint32_t e[4];
+#pragma unroll 1
+      for (uint32_t p = 0; p < len; p += 16) {
+        uint32_t e[16];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) e[k] = p + k < len ? cols[base + p + k] : 0u;
+        for (uint32_t k = 0; k < 16; ++k) e[k] = p + k < len ? cols[base + p + k] : 0u;
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
+        for (uint32_t k = 0; k < 16; ++k) {
           if (p + k >= len) break;
           if (e[k] < PB_H) {
             hot[pb_phys(hb)] = (uint16_t)(e[k] | (hb == h0 ? PB_F16 : 0u));

@@ -593,10 +593,18 @@ __device__ __forceinline__ void probe_hit(const Store &st, const ProbeArgs &a, u
 
 constexpr int PROBE_ROUNDS = PROBE_CHUNK / WAVE;
 // A scan item of at most PROBE_LANE_SLOTS slots whose source has at most
-// PROBE_SMALL_KEYS batch keys is probed by one lane on its own, 16 slots per
-// round (a multiple of 16, at most PROBE_CHUNK).
+// LaneProbe::keys batch keys is probed by one lane on its own, 16 slots per
+// round (a multiple of 16, at most PROBE_CHUNK): 4 keys in a plain batch,
+// 8 in a merged group, whose touched sources carry several batches' keys
+// (about 5 at 10 R-MAT scale-22 batches, 10 at 20).
 constexpr uint32_t PROBE_LANE_SLOTS = 128;
-constexpr uint32_t PROBE_SMALL_KEYS = 4;  // key indices packed in two bits
+constexpr uint32_t PROBE_SMALL_KEYS = 4;
+template <int MODE>
+struct LaneProbe {
+  static constexpr uint32_t slots = PROBE_LANE_SLOTS;
+  static constexpr uint32_t keys = MODE == 2 ? 8 : PROBE_SMALL_KEYS;
+  static constexpr uint32_t kbits = MODE == 2 ? 3 : 2;  // bits of a packed key index
+};
 // A wavefront-wide item whose source has at most this many batch keys
 // compares each slot with every key.
 constexpr uint32_t PROBE_FEW_KEYS = 8;
@@ -788,7 +796,7 @@ __device__ __forceinline__ void probe_scan(const Store &st, const ProbeArgs &a, 
 // PROBE_CHUNK slots over the rest) are dealt to lanes: wave w of
 // W, step r, lane l takes item (r * 64 + l) * W + w, so the items of one hub
 // spread over many wavefronts.  A small item (<= PROBE_LANE_SLOTS slots,
-// <= PROBE_SMALL_KEYS batch keys -- most touched vertices of an R-MAT batch)
+// <= LaneProbe::keys batch keys -- most touched vertices of an R-MAT batch)
 // is probed by its lane alone: 64 of them share one chain of dependent loads
 // instead of paying it one after another.  The other items of the step are
 // then probed by the whole wavefront, one at a time, their metadata broadcast
@@ -830,13 +838,13 @@ __global__ __launch_bounds__(256, 8) void k_probe(Store st, ProbeArgs a, uint32_
         hi = lo + PROBE_CHUNK < uo ? lo + PROBE_CHUNK : uo;
       }
     }
-    const bool small =
-        valid && !fitem && hi - lo <= PROBE_LANE_SLOTS && ke - ks <= PROBE_SMALL_KEYS;
+    using LP = LaneProbe<MODE>;
+    const bool small = valid && !fitem && hi - lo <= LP::slots && ke - ks <= LP::keys;
     if (small) {
       // the keys' destinations, TOMB past the segment (never equal to a live slot)
-      uint32_t kv[PROBE_SMALL_KEYS];
+      uint32_t kv[LP::keys];
 #pragma unroll
-      for (uint32_t j = 0; j < PROBE_SMALL_KEYS; ++j)
+      for (uint32_t j = 0; j < LP::keys; ++j)
         kv[j] = ks + j < ke ? (uint32_t)(a.keys[ks + j] & mask) : TOMB;
       uint32_t cnt = 0;
       const uint32_t n = hi - lo;
@@ -860,23 +868,25 @@ __global__ __launch_bounds__(256, 8) void k_probe(Store st, ProbeArgs a, uint32_
           x[4 * q + 2] = v.z;
           x[4 * q + 3] = v.w;
         }
-        // matching slots (hm) and, two bits each, their first matching key
+        // matching slots (hm) and, LP::kbits bits each, their first matching key
         // (the lower bound, as on the wavefront path); hits are acted on after
-        uint32_t hm = 0, hp = 0;
+        uint32_t hm = 0;
+        uint64_t hp = 0;
 #pragma unroll
         for (uint32_t j = 0; j < 16; ++j) {
-          uint32_t p = PROBE_SMALL_KEYS;
+          uint32_t p = LP::keys;
 #pragma unroll
-          for (uint32_t i = PROBE_SMALL_KEYS; i-- > 0;)
+          for (uint32_t i = LP::keys; i-- > 0;)
             if (kv[i] == x[j]) p = i;
-          const bool hit = j < nr && x[j] != TOMB && p < PROBE_SMALL_KEYS;
+          const bool hit = j < nr && x[j] != TOMB && p < LP::keys;
           hm |= hit ? 1u << j : 0u;
-          hp |= hit ? p << (2 * j) : 0u;
+          hp |= hit ? (uint64_t)p << (LP::kbits * j) : 0ull;
         }
         while (hm) {
           const uint32_t j = (uint32_t)__ffs(hm) - 1;
           hm &= hm - 1;
-          probe_hit<MODE>(st, a, rb + j, ks + ((hp >> (2 * j)) & 3u), cnt);
+          probe_hit<MODE>(st, a, rb + j,
+                          ks + (uint32_t)((hp >> (LP::kbits * j)) & ((1u << LP::kbits) - 1)), cnt);
         }
       }
       if (MODE != PROBE_INSERT && cnt) atomicAdd(&a.seg_del[sidx], cnt);
@@ -1007,9 +1017,14 @@ __global__ void k_commit(Ctl *a, uint64_t Aa, Ctl *b, uint64_t Ab, const uint32_
   }
 }
 
+// seg_tbl / seg_rb (written here, read by k_ins_write): per segment with
+// survivors, its source's block table after growth and its survivors' rank
+// base; they reuse the segment arrays of the delete counts and item counts,
+// which nothing reads after the probe.
 struct TblArgs {
   const uint32_t *seg_src, *seg_used, *seg_start, *rank;
   const uint64_t *plan;
+  uint32_t *seg_tbl, *seg_rb;
 };
 
 __global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
@@ -1020,7 +1035,8 @@ __global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
        sidx += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t u = a.seg_src[sidx];
     uint32_t uo = a.seg_used[sidx];
-    uint32_t surv = a.rank[a.seg_start[sidx + 1]] - a.rank[a.seg_start[sidx]];
+    const uint32_t rb = a.rank[a.seg_start[sidx]];
+    uint32_t surv = a.rank[a.seg_start[sidx + 1]] - rb;
     if (!surv) continue;
     uint32_t nbo = (uo + st.B - 1) >> st.log2B;
     uint32_t nbn = (uo + surv + st.B - 1) >> st.log2B;
@@ -1040,14 +1056,20 @@ __global__ void k_ins_tables(Store st, TblArgs a, Ctl *ctl) {
     for (uint32_t j = nbo; j < nbn; ++j) st.arena[tbl + j] = st.free_ids[bb + (j - nbo)];
     st.used[u] = uo + surv;
     st.deg[u] += surv;
+    a.seg_tbl[sidx] = tbl;
+    a.seg_rb[sidx] = rb;
   }
 }
 
 struct WriteArgs {
   const uint64_t *keys;
-  const uint32_t *flag, *segof, *rank, *seg_start, *seg_src, *seg_used;
+  const uint32_t *flag, *segof, *rank, *seg_used, *seg_tbl, *seg_rb;
 };
 
+// One thread per staged key (the grid covers them): three dependent load
+// levels -- the key's own words, its segment's words, the block table --
+// then the slot.  The survivors of a source that was empty form its sorted
+// prefix (k_ins_tables set it), so only they write fences.
 __global__ void k_ins_write(Store st, WriteArgs a, uint32_t bV, Ctl *ctl) {
   if (ctl->abort) return;
   const uint32_t nv = ctl->n_valid;
@@ -1055,13 +1077,14 @@ __global__ void k_ins_write(Store st, WriteArgs a, uint32_t bV, Ctl *ctl) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (!a.flag[i]) continue;
-    uint32_t sidx = a.segof[i];
-    uint32_t pos = a.seg_used[sidx] + (a.rank[i] - a.rank[a.seg_start[sidx]]);
-    uint32_t u = a.seg_src[sidx];
-    uint32_t idx = slot_index(st.arena, st.tbl_off[u], pos, st.log2B, st.B);
-    const uint32_t v = (uint32_t)(a.keys[i] & mask);
+    const uint32_t sidx = a.segof[i], r = a.rank[i];
+    const uint64_t k = a.keys[i];
+    const uint32_t uo = a.seg_used[sidx];
+    const uint32_t pos = uo + (r - a.seg_rb[sidx]);
+    const uint32_t idx = slot_index(st.arena, a.seg_tbl[sidx], pos, st.log2B, st.B);
+    const uint32_t v = (uint32_t)(k & mask);
     st.slots[idx] = v;
-    if ((pos & (FENCE_SLOTS - 1)) == 0 && pos < st.sorted[u]) st.fence[idx / FENCE_SLOTS] = v;
+    if ((pos & (FENCE_SLOTS - 1)) == 0 && uo == 0) st.fence[idx / FENCE_SLOTS] = v;
   }
 }
 
@@ -1095,10 +1118,10 @@ void commit_pair(Store &a, Scratch &sa, Store *b, Scratch *sb, hipStream_t s) {
 void insert_write(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
   if (!m) return;
   const uint32_t bV = id_bits(st.V);
-  TblArgs ta{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.plan};
+  TblArgs ta{sc.seg_src, sc.seg_used, sc.seg_start, sc.rank, sc.plan, sc.seg_blk, sc.seg_cnt};
   k_ins_tables<<<grid_for(m), 256, 0, s>>>(st, ta, st.ctl);
-  WriteArgs wa{keys, sc.flag, sc.segof, sc.rank, sc.seg_start, sc.seg_src, sc.seg_used};
-  k_ins_write<<<grid_for(m), 256, 0, s>>>(st, wa, bV, st.ctl);
+  WriteArgs wa{keys, sc.flag, sc.segof, sc.rank, sc.seg_used, sc.seg_blk, sc.seg_cnt};
+  k_ins_write<<<grid_for(m, 256, 1u << 20), 256, 0, s>>>(st, wa, bV, st.ctl);
 }
 
 // ---------------------------------------------------------------- delete --
