@@ -1130,18 +1130,31 @@ __global__ void k_del_finish(Store st, const uint32_t *seg_src, const uint32_t *
   if (ctl->abort) return;
   const uint32_t T = ctl->n_segs;
   unsigned long long mine = 0;
-  for (uint64_t sidx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; sidx < T;
-       sidx += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t d = seg_del[sidx];
-    if (d) st.deg[seg_src[sidx]] -= d;
-    mine += d;
+  // four segments per thread and step, their loads issued together
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t s0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s0 < T; s0 += 4 * stride) {
+    uint32_t d[4], u[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t sidx = s0 + j * stride;
+      d[j] = sidx < T ? seg_del[sidx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) u[j] = d[j] ? seg_src[s0 + j * stride] : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (d[j]) st.deg[u[j]] -= d[j];
+      mine += d[j];
+    }
   }
-  // one atomic pair per block (the grid is capped at 2048 blocks)
+  // one atomic pair per block (the grid is capped at 256 blocks: the pairs
+  // all land on the same two words)
   unsigned long long tot = block_sum<unsigned long long, 256>(mine);
   if (threadIdx.x == 0 && tot) {
     atomicAdd(&ctl->applied, tot);
     atomicAdd(&ctl->live_edges, 0ull - tot);
   }
+}
 }
 
 void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {
@@ -1150,7 +1163,7 @@ void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipS
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, nullptr,
                sc.seg_blk, nullptr};
   k_probe<PROBE_DELETE><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
-  k_del_finish<<<grid_for(m, 256, 2048), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+  k_del_finish<<<grid_for(m, 256, 256), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
 }
 
 // Merged group: one probe pass tombstones keys whose last op is a delete and
@@ -1162,7 +1175,7 @@ void mixed_plan(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStr
   ProbeArgs pa{keys, sc.item_seg, sc.item_start, sc.seg_src, sc.seg_used, sc.seg_start, sc.flag,
                sc.seg_blk, sc.op};
   k_probe<PROBE_MIXED><<<probe_grid(m), 256, 0, s>>>(st, pa, bV, st.ctl);
-  k_del_finish<<<grid_for(m, 256, 2048), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
+  k_del_finish<<<grid_for(m, 256, 256), 256, 0, s>>>(st, sc.seg_src, sc.seg_blk, st.ctl);
   insert_alloc(st, sc, m, s);
 }
 
