@@ -37,7 +37,7 @@ GV_OP_DELETE = 1
 
 EXPORTED = [
     "gv_config_default", "gv_create", "gv_destroy", "gv_bulk_build", "gv_batch_insert",
-    "gv_batch_delete", "gv_batch_apply", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
+    "gv_batch_delete", "gv_batch_apply", "gv_batch_apply_device", "gv_compact", "gv_num_vertices", "gv_num_edges", "gv_degrees",
     "gv_pool_stats", "gv_export_csr", "gv_neighbors", "gv_neighbors_noalloc", "gv_pagerank",
     "gv_pagerank_ex",
     "gv_pagerank_opts_default", "gv_pagerank_run", "gv_pagerank_personalized",
@@ -130,6 +130,7 @@ def lib():
         "gv_compact": [P, ctypes.c_float, ctypes.POINTER(u64)],
         "gv_batch_apply": [P, P, P, P, P, u32, i32, ctypes.c_float,
                            ctypes.POINTER(GvBatchStats)],
+        "gv_batch_apply_device": [P, P, P, P, P, u32, u64, ctypes.c_float],
         "gv_num_vertices": [P, ctypes.POINTER(u32)],
         "gv_num_edges": [P, ctypes.POINTER(u64)],
         "gv_degrees": [P, P, i32],
@@ -217,6 +218,27 @@ def _ptr_u32(x):
     return a.ctypes.data_as(ctypes.c_void_p), a, GV_MEM_HOST
 
 
+def _dev_len(x):
+    return int(x.n) if hasattr(x, "ptr") and hasattr(x, "nbytes") else int(x.numel())
+
+
+def _dev_ptr(x, dtypes, name):
+    """(pointer, length) of a DeviceArray or contiguous device tensor whose
+    element type is one of dtypes (numpy types)."""
+    if hasattr(x, "ptr") and hasattr(x, "nbytes") and not isinstance(x, np.ndarray):
+        if np.dtype(x.dtype) not in [np.dtype(t) for t in dtypes]:
+            raise TypeError(f"{name}: device array of {dtypes} expected")
+        return x.ptr, int(x.n)
+    if _is_device(x):
+        import torch
+        tmap = {np.dtype(np.uint64): torch.uint64, np.dtype(np.int64): torch.int64,
+                np.dtype(np.int32): torch.int32, np.dtype(np.uint32): torch.uint32}
+        if x.dtype not in [tmap[np.dtype(t)] for t in dtypes] or not x.is_contiguous():
+            raise TypeError(f"{name}: contiguous device tensor of {dtypes} expected")
+        return ctypes.c_void_p(x.data_ptr()), int(x.numel())
+    raise TypeError(f"{name}: device array expected")
+
+
 class GraphVine:
     """Device-resident dynamic graph (reference: class GraphVine)."""
 
@@ -295,6 +317,28 @@ class GraphVine:
                                     kin.ctypes.data_as(ctypes.c_void_p), nb, ms,
                                     float(compact_fraction), ctypes.byref(st) if stats else None))
         return st.as_dict() if stats else None
+
+    def applyBatchesDevice(self, src, dst, offsets, kinds, compact_fraction=-1.0):
+        """Batched form over a device-resident queue (no host reads, capturable):
+        src/dst are uint32 device arrays of the queue's capacity (max_pairs),
+        offsets a uint64 device array of max_batches+1 entries and kinds an
+        int32 device array of max_batches entries (gv_amd.hip.DeviceArray or
+        contiguous device tensors), all read when the call runs on the
+        handle's stream.  Needs reserve(max_pairs, max_batches) first; errors
+        found on the device surface at the next call that reads the status."""
+        ps, ks, ms = _ptr_u32(src)
+        pd, kd, md = _ptr_u32(dst)
+        if ms != GV_MEM_DEVICE or md != GV_MEM_DEVICE:
+            raise ValueError("src and dst must be device arrays")
+        po, no = _dev_ptr(offsets, (np.uint64, np.int64), "offsets")
+        pk, nk = _dev_ptr(kinds, (np.int32,), "kinds")
+        if no != nk + 1:
+            raise ValueError("offsets must have max_batches+1 entries")
+        n = _dev_len(src)
+        if _dev_len(dst) != n:
+            raise ValueError("src and dst lengths differ")
+        _check(lib().gv_batch_apply_device(self.handle, ps, pd, po, pk, nk, n,
+                                           float(compact_fraction)))
 
     # reference: GraphVine::compaction (graphvine.cu:453-464)
     def compaction(self, tombstone_fraction=0.0):

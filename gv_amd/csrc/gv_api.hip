@@ -145,6 +145,11 @@ static gv_status take_abort(gv_graph *g) {
     set_error("edge-block pool exhausted; batch not applied");
     return GV_E_POOL;
   }
+  if (ab == ABORT_GROUP) {
+    set_error("group offsets or kinds invalid (offsets must start at 0, not decrease and end "
+              "within max_pairs; kinds GV_OP_INSERT or GV_OP_DELETE); group not applied");
+    return GV_E_INVALID;
+  }
   if (ab == ABORT_SORT) {
     set_error("radix sort of the staged keys failed (a tile's look-back gave up); nothing applied");
     return GV_E_HIP;
@@ -283,6 +288,10 @@ static gv_status apply_batch(gv_graph *g, const uint32_t *src, const uint32_t *d
   return take_abort(g);
 }
 
+static void group_pipeline(gv_graph *g, const uint32_t *ds, const uint32_t *dd, uint64_t n,
+                           const uint64_t *boff, const uint32_t *kinds, uint32_t nb);
+static void group_compact(gv_graph *g, uint64_t n, float compact_fraction);
+
 // A group of batches in one pipeline (see gv_batch_apply in gv.h).
 static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *dst,
                              const uint64_t *offsets, const int *kinds, uint32_t nb, int mem,
@@ -360,22 +369,44 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
     for (uint32_t b = 0; b < nb; ++b) hk[b] = kinds[b] == GV_OP_DELETE ? OP_DELETE : OP_INSERT;
     GV_HIP(hipMemcpyAsync(g->grp_off, hb, ob, hipMemcpyHostToDevice, s));
     GV_HIP(hipMemcpyAsync(g->grp_kind, hk, kb, hipMemcpyHostToDevice, s));
-    uint64_t m[2] = {0, 0};
-    uint64_t *keys[2] = {nullptr, nullptr};
-    for (int k = 0; k < nst; ++k) {
-      const int mirror = g->directed ? 0 : 1;
-      keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, g->grp_off, g->grp_kind, nb, mirror,
-                            k == 1 ? 1 : 0, m[k], s);
-    }
-    if (nst > 1) abort_join(g->st[0], g->st[1], g->sc[0], g->sc[1], s);
-    for (int k = 0; k < nst; ++k) {
-      segment_items(g->st[k], g->sc[k], m[k], s);
-      mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
-    }
-    commit_pair(g->st[0], g->sc[0], nst > 1 ? &g->st[1] : nullptr, nst > 1 ? &g->sc[1] : nullptr,
-                s);
-    for (int k = 0; k < nst; ++k) insert_write(g->st[k], g->sc[k], keys[k], m[k], s);
+    group_pipeline(g, ds, dd, n, g->grp_off, g->grp_kind, nb);
   }
+  group_compact(g, n, compact_fraction);
+  GV_HIP(hipGetLastError());
+  if (!stats) return GV_OK;
+  rc = read_ctl(g);
+  if (rc) return rc;
+  fill_stats(g, n, stats);
+  return take_abort(g);
+}
+
+// The group pipeline over device pairs: staging (entries past boff[nb]
+// dropped), one sort, selection and segmentation, the merged probe, the
+// joint pool commit and the writes, on the handle's stream.
+static void group_pipeline(gv_graph *g, const uint32_t *ds, const uint32_t *dd, uint64_t n,
+                           const uint64_t *boff, const uint32_t *kinds, uint32_t nb) {
+  hipStream_t s = g->stream;
+  const int nst = g->has_in ? 2 : 1;
+  uint64_t m[2] = {0, 0};
+  uint64_t *keys[2] = {nullptr, nullptr};
+  for (int k = 0; k < nst; ++k) {
+    const int mirror = g->directed ? 0 : 1;
+    keys[k] = stage_group(g->st[k], g->sc[k], ds, dd, n, boff, kinds, nb, mirror,
+                          k == 1 ? 1 : 0, m[k], s);
+  }
+  if (nst > 1) abort_join(g->st[0], g->st[1], g->sc[0], g->sc[1], s);
+  for (int k = 0; k < nst; ++k) {
+    segment_items(g->st[k], g->sc[k], m[k], s);
+    mixed_plan(g->st[k], g->sc[k], keys[k], m[k], s);
+  }
+  commit_pair(g->st[0], g->sc[0], nst > 1 ? &g->st[1] : nullptr, nst > 1 ? &g->sc[1] : nullptr,
+              s);
+  for (int k = 0; k < nst; ++k) insert_write(g->st[k], g->sc[k], keys[k], m[k], s);
+}
+
+// Compaction after a group of n pairs (see gv_batch_apply).
+static void group_compact(gv_graph *g, uint64_t n, float compact_fraction) {
+  hipStream_t s = g->stream;
   if (compact_fraction >= 0.f) {
     // with updates: only the vertices this group touched can have gained
     // tombstones, so only they are candidates; an empty group compacts all
@@ -388,12 +419,45 @@ static gv_status apply_group(gv_graph *g, const uint32_t *src, const uint32_t *d
       }
     }
   }
-  GV_HIP(hipGetLastError());
-  if (!stats) return GV_OK;
-  rc = read_ctl(g);
+}
+
+// A group whose pairs, batch offsets and kinds are all in device memory:
+// up to max_batches batches over up to max_pairs pairs, read when the call
+// runs on the stream (a producer fills them between launches of a capture).
+static gv_status apply_group_device(gv_graph *g, const uint32_t *src, const uint32_t *dst,
+                                    const uint64_t *offsets, const int *kinds,
+                                    uint32_t max_batches, uint64_t max_pairs,
+                                    float compact_fraction) {
+  gv_status rc = check_graph(g, true);
   if (rc) return rc;
-  fill_stats(g, n, stats);
-  return take_abort(g);
+  if (max_batches == 0 || max_pairs == 0) return GV_OK;
+  if (!src || !dst || !offsets || !kinds) {
+    set_error("null device pointer");
+    return GV_E_INVALID;
+  }
+  if (2 * id_bits(g->V) + id_bits(max_batches) > 64) {
+    set_error("too many batches in one group for this vertex count");
+    return GV_E_INVALID;
+  }
+  if (max_pairs > 0x7FFFFFFFull) {
+    set_error("group larger than 2^31 pairs; split it");
+    return GV_E_INVALID;
+  }
+  // the staging must already hold the capacity (no allocation here: the
+  // call may be captured)
+  const int nst = g->has_in ? 2 : 1;
+  for (int k = 0; k < nst; ++k) {
+    if (!scratch_fits(g->sc[k], g->st[k], g->directed ? max_pairs : 2 * max_pairs)) {
+      set_error("gv_batch_apply_device: call gv_reserve(g, max_pairs, max_batches) first");
+      return GV_E_INVALID;
+    }
+  }
+  if (g->capturing) g->cap_pairs += max_pairs;
+  group_pipeline(g, src, dst, max_pairs, reinterpret_cast<const uint64_t *>(offsets),
+                 reinterpret_cast<const uint32_t *>(kinds), max_batches);
+  group_compact(g, max_pairs, compact_fraction);
+  GV_HIP(hipGetLastError());
+  return GV_OK;
 }
 
 namespace gv {
@@ -433,6 +497,13 @@ gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
                          float compact_fraction, gv_batch_stats *stats) {
   return note_capture(
       g, apply_group(g, src, dst, offsets, kinds, n_batches, mem, compact_fraction, stats));
+}
+
+gv_status gv_batch_apply_device(gv_graph *g, const uint32_t *src, const uint32_t *dst,
+                                const uint64_t *offsets, const int *kinds, uint32_t max_batches,
+                                uint64_t max_pairs, float compact_fraction) {
+  return note_capture(g, apply_group_device(g, src, dst, offsets, kinds, max_batches, max_pairs,
+                                            compact_fraction));
 }
 
 gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed,

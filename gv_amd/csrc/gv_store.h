@@ -27,6 +27,7 @@ constexpr uint32_t OP_DELETE = 1;
 // Abort codes kept in Ctl::abort (magnitudes of the GV_E_* values).
 constexpr uint32_t ABORT_RANGE = 2;
 constexpr uint32_t ABORT_POOL = 4;
+constexpr uint32_t ABORT_GROUP = 6;  // group offsets or kinds invalid (device-filled group)
 
 gv_status store_alloc(Store &st, uint32_t V, uint32_t B, uint64_t P, uint64_t A);
 void store_release(Store &st);

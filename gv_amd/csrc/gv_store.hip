@@ -217,9 +217,9 @@ uint64_t *stage(Store &st, Scratch &sc, const uint32_t *src, const uint32_t *dst
 // Like k_stage, also starts the operation and clears the sort's workspace.
 constexpr uint32_t STAGE_SEQ_LDS = 2048;
 __global__ void k_stage_seq(const uint32_t *src, const uint32_t *dst, uint64_t n,
-                            const uint64_t *boff, uint32_t nb, uint32_t V, uint32_t bV,
-                            uint32_t sb, int mirror, int swap, uint64_t *keys, Ctl *ctl,
-                            uint32_t *clr, uint64_t nclr) {
+                            const uint64_t *boff, const uint32_t *kinds, uint32_t nb, uint32_t V,
+                            uint32_t bV, uint32_t sb, int mirror, int swap, uint64_t *keys,
+                            Ctl *ctl, uint32_t *clr, uint64_t nclr) {
   // the batch offsets in LDS when they fit (the search below runs per key)
   __shared__ uint64_t s_off[STAGE_SEQ_LDS];
   const bool in_lds = nb < STAGE_SEQ_LDS;
@@ -231,8 +231,32 @@ __global__ void k_stage_seq(const uint32_t *src, const uint32_t *dst, uint64_t n
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nclr;
        i += (uint64_t)gridDim.x * blockDim.x)
     clr[i] = 0;
+  // offsets start at 0, never decrease and end within the n entries; kinds
+  // are OP_INSERT or OP_DELETE (a device-filled group can break either: the
+  // group is then not applied)
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= nb;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t o = boff[k];
+    const bool bad = (k == 0 ? o != 0 : boff[k - 1] > o) || (k == nb && o > n) ||
+                     (k < nb && kinds[k] > OP_DELETE);
+    // ABORT_GROUP is the largest code: it replaces a range error raised by
+    // entries that bad offsets put inside the group
+    if (bad) atomicMax(&ctl->abort, ABORT_GROUP);
+  }
+  // entries past the last batch (a device-filled group shorter than its
+  // capacity n) are dropped like self-loops
+  const uint64_t n_used = in_lds ? s_off[nb] : boff[nb];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i >= n_used) {
+      if (mirror) {
+        keys[2 * i] = KEY_SENT;
+        keys[2 * i + 1] = KEY_SENT;
+      } else {
+        keys[i] = KEY_SENT;
+      }
+      continue;
+    }
     uint32_t lo = 0, hi = nb;  // largest b with boff[b] <= i
     if (in_lds) {
       while (hi - lo > 1) {
@@ -415,7 +439,7 @@ __global__ __launch_bounds__(256) void k_gs_write(const uint64_t *keys, uint64_t
       const uint32_t ph = (uint32_t)tb + (ex & 0xFFFFu) + mask_lt_count(mh);  // heads before
       const uint64_t kk = keys[i];
       const uint64_t e = kk >> sb;
-      const uint32_t o = kinds[kk & smask];
+      const uint32_t o = kinds[kk & smask] == OP_INSERT ? OP_INSERT : OP_DELETE;
       out[pk] = e;
       op[pk] = o;
       flag[pk] = o == OP_INSERT ? 1u : 0u;
@@ -447,8 +471,8 @@ uint64_t *stage_group(Store &st, Scratch &sc, const uint32_t *src, const uint32_
   }
   const uint32_t bV = id_bits(st.V), sb = id_bits(nb);
   const SortClear c = sort_clear_span(sc.tmp, m, sb, 2 * bV + sb);
-  k_stage_seq<<<grid_for(n), 256, 0, s>>>(src, dst, n, boff, nb, st.V, bV, sb, mirror, swap,
-                                          sc.k0, st.ctl, c.w, c.words);
+  k_stage_seq<<<grid_for(n), 256, 0, s>>>(src, dst, n, boff, kinds, nb, st.V, bV, sb, mirror,
+                                          swap, sc.k0, st.ctl, c.w, c.words);
   // the keys are staged in batch order, so a stable sort on the (u, v) bits
   // alone leaves equal edges in batch order: the same array as a sort on the
   // whole key, with one pass fewer when the tag crosses a digit boundary
@@ -1154,7 +1178,6 @@ __global__ void k_del_finish(Store st, const uint32_t *seg_src, const uint32_t *
     atomicAdd(&ctl->applied, tot);
     atomicAdd(&ctl->live_edges, 0ull - tot);
   }
-}
 }
 
 void delete_apply(Store &st, Scratch &sc, const uint64_t *keys, uint64_t m, hipStream_t s) {

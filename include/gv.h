@@ -118,6 +118,25 @@ gv_status gv_batch_delete(gv_graph *g, const uint32_t *src, const uint32_t *dst,
 gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
                          const uint64_t *offsets, const int *kinds, uint32_t n_batches, int mem,
                          float compact_fraction, gv_batch_stats *stats);
+/* The batched form over a device-resident queue: src, dst (max_pairs
+ * entries), offsets (max_batches+1) and kinds (max_batches) are all device
+ * pointers, read when the call runs on the handle's stream, so a producer can
+ * refill them between launches of a capture that recorded this call.  The
+ * queue holds offsets[max_batches] <= max_pairs pairs: unused trailing
+ * batches are empty (equal offsets) and entries past offsets[max_batches] are
+ * ignored.  Requires gv_reserve(g, max_pairs, max_batches) first (the call
+ * never allocates) and reads nothing back: offsets that do not start at 0, that
+ * decrease or that end past max_pairs, kinds other than GV_OP_INSERT or
+ * GV_OP_DELETE and out-of-range vertex IDs leave the graph unchanged and set
+ * the handle's sticky error, which skips later updates until gv_sync (or
+ * gv_capture_launch with stats) reports it (GV_E_INVALID / GV_E_RANGE) and
+ * clears it.  No reference
+ * counterpart: for callers that drain a device-side update queue without a
+ * host round trip; the reference takes one host-built CSR batch per call
+ * (graphvine.cu:222-451). */
+gv_status gv_batch_apply_device(gv_graph *g, const uint32_t *src, const uint32_t *dst,
+                                const uint64_t *offsets, const int *kinds, uint32_t max_batches,
+                                uint64_t max_pairs, float compact_fraction);
 /* GraphVine::compaction: pack every vertex whose tombstones exceed
  * tombstone_fraction of its slots (0 = any tombstone, < 0 = the configured
  * default) and return emptied blocks to the pool. */
@@ -294,12 +313,13 @@ gv_status gv_set_tuning(gv_graph *g, const gv_config *cfg);
  *     and gv_pagerank_last then fails;
  *   - between gv_capture_begin and gv_capture_end only gv_batch_insert,
  *     gv_batch_delete and gv_batch_apply with GV_MEM_DEVICE inputs and NULL
- *     stats, gv_compact with a NULL count, and gv_pagerank_run with device
- *     vectors and NULL stats are allowed; every other call on the handle
+ *     stats, gv_batch_apply_device, gv_compact with a NULL count, and
+ *     gv_pagerank_run with device vectors and NULL stats are allowed; every other call on the handle
  *     fails and a failed call makes gv_capture_end fail;
  *   - a replay reads the device inputs at the addresses given at capture time
  *     (their contents may change between launches) and the batch offsets and
- *     kinds given at capture time;
+ *     kinds given at capture time (gv_batch_apply_device reads those from
+ *     device memory too, so the batch count and sizes may change as well);
  *   - a later reallocation of a buffer the capture uses (a group larger than
  *     reserved, a plain PageRank call on a graph larger than reserved)
  *     invalidates it: gv_capture_launch then returns GV_E_INVALID;

@@ -111,6 +111,144 @@ def test_captured_group_replays_with_refilled_inputs(directed):
     g.close()
 
 
+QCAP, QB = 6000, 8
+REDEL = 2  # a delete batch drawn from the previous batch's pairs (test-only kind)
+
+
+def fill_queue(o, V, seed, plan):
+    """Host images of a device queue of capacity (QCAP pairs, QB batches)
+    holding the batches of plan [(size, kind)], applied to the oracle in
+    order.  Entries past the last batch hold out-of-range IDs (they must be
+    ignored) and the unused trailing batches are empty."""
+    srcs, dsts, kinds = [], [], []
+    prev = (np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    for b, (n, k) in enumerate(plan):
+        if k == G.GV_OP_INSERT:
+            s, d = O.gen_uniform(O.stream_key(seed, 30_000 + b), V, n)
+            o.insert(s, d)
+        elif k == G.GV_OP_DELETE:
+            off, cols = o.export_csr()
+            s, d = O.gen_sample(O.stream_key(seed, 40_000 + b), V, off, cols, n)
+            o.delete(s, d)
+        else:
+            s, d = prev[0][:n].copy(), prev[1][:n].copy()
+            o.delete(s, d)
+            k = G.GV_OP_DELETE
+        srcs.append(s)
+        dsts.append(d)
+        kinds.append(k)
+        prev = (s, d)
+    used = sum(len(x) for x in srcs)
+    assert used <= QCAP and len(plan) <= QB
+    hs = np.full(QCAP, 0xFFFFFFFF, np.uint32)
+    hd = np.full(QCAP, 0xFFFFFFFF, np.uint32)
+    if used:
+        hs[:used] = np.concatenate(srcs)
+        hd[:used] = np.concatenate(dsts)
+    off = np.full(QB + 1, used, np.uint64)
+    off[: len(plan) + 1] = np.concatenate([[0], np.cumsum([n for n, _ in plan])])
+    kin = np.full(QB, G.GV_OP_INSERT, np.int32)
+    kin[: len(plan)] = kinds
+    return hs, hd, off, kin
+
+
+I_, D_ = G.GV_OP_INSERT, G.GV_OP_DELETE
+QUEUE_PLANS = [
+    [(1500, I_), (800, D_), (0, I_), (1200, I_), (600, REDEL), (900, D_), (400, I_), (500, D_)],
+    [(2000, I_), (700, REDEL), (300, D_)],
+    [(400, D_)],
+    [],
+    [(3000, I_), (3000, REDEL)],
+]
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_device_queue_group_capture_replays_uneven_batches(directed):
+    V = 1 << 12
+    s, d = O.gen_rmat(O.stream_key(36, O.STREAM_RMAT), 12, 16 * V)
+    g = make(V, directed)
+    o = O.OracleGraph(V, directed)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    qs, qd = DeviceArray(QCAP, np.uint32), DeviceArray(QCAP, np.uint32)
+    qo, qk = DeviceArray(QB + 1, np.uint64), DeviceArray(QB, np.int32)
+    g.reserve(QCAP, QB)
+    # an eager call first
+    hs, hd, off, kin = fill_queue(o, V, 800, QUEUE_PLANS[0])
+    for arr, h in ((qs, hs), (qd, hd), (qo, off), (qk, kin)):
+        arr.upload(h)
+    g.applyBatchesDevice(qs, qd, qo, qk, compact_fraction=0.25)
+    g.sync()
+    assert_same(g, o, "eager device queue")
+    with g.capture() as cap:
+        g.applyBatchesDevice(qs, qd, qo, qk, compact_fraction=0.25)
+    c = cap.result
+    for rep, plan in enumerate(QUEUE_PLANS):
+        hs, hd, off, kin = fill_queue(o, V, 810 + rep, plan)
+        g.sync()
+        for arr, h in ((qs, hs), (qd, hd), (qo, off), (qk, kin)):
+            arr.upload(h)
+        if rep == 1:
+            st = c.launch(stats=True)
+            assert st["live_edges"] == o.num_edges()
+        else:
+            c.launch()
+        g.sync()
+        assert_same(g, o, f"replay {rep} ({len(plan)} batches)")
+        es, ed = O.gen_uniform(O.stream_key(820 + rep, 1), V, 500)
+        g.batchInsert(es, ed)
+        o.insert(es, ed)
+        assert_same(g, o, f"eager after replay {rep}")
+    c.close()
+    g.close()
+
+
+def test_device_queue_group_errors_leave_graph_unchanged():
+    V = 1 << 11
+    s, d = O.gen_rmat(O.stream_key(37, O.STREAM_RMAT), 11, 16 * V)
+    g = make(V, False)
+    o = O.OracleGraph(V, False)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    qs, qd = DeviceArray(QCAP, np.uint32), DeviceArray(QCAP, np.uint32)
+    qo, qk = DeviceArray(QB + 1, np.uint64), DeviceArray(QB, np.int32)
+    g.reserve(QCAP, QB)
+    hs, _, _, _ = fill_queue(O.OracleGraph(V, False), V, 830, [(1000, I_)])
+    hd = hs[::-1].copy()
+    hd[:1000] = (hs[:1000] + 1) % V
+    good_off = np.full(QB + 1, 1000, np.uint64)
+    good_off[0] = 0
+    good_kin = np.zeros(QB, np.int32)
+    cases = [
+        ("decreasing offsets", hs, hd, np.array([0, 600, 300] + [1000] * (QB - 2), np.uint64),
+         good_kin, G.GV_E_INVALID),
+        ("first offset not 0", hs, hd, np.array([5] + [1000] * QB, np.uint64), good_kin,
+         G.GV_E_INVALID),
+        ("past capacity", hs, hd, np.array([0] + [QCAP + 1] * QB, np.uint64), good_kin,
+         G.GV_E_INVALID),
+        ("bad kind", hs, hd, good_off, np.array([7] + [0] * (QB - 1), np.int32),
+         G.GV_E_INVALID),
+        ("vertex out of range", np.where(np.arange(QCAP) == 10, V + 3, hs).astype(np.uint32),
+         hd, good_off, good_kin, G.GV_E_RANGE),
+    ]
+    for name, cs, cd, co, ck, code in cases:
+        for arr, h in ((qs, cs), (qd, cd), (qo, co), (qk, ck)):
+            arr.upload(h)
+        g.applyBatchesDevice(qs, qd, qo, qk, compact_fraction=0.25)
+        with pytest.raises(G.GraphVineError) as ei:
+            g.sync()
+        assert ei.value.status == code, name
+        assert_same(g, o, name)
+    # the handle applies a valid queue afterwards
+    for arr, h in ((qs, hs), (qd, hd), (qo, good_off), (qk, good_kin)):
+        arr.upload(h)
+    g.applyBatchesDevice(qs, qd, qo, qk)
+    g.sync()
+    o.insert(hs[:1000], hd[:1000])
+    assert_same(g, o, "valid queue after errors")
+    g.close()
+
+
 def test_captured_plain_sequence_with_stats():
     V = 1 << 11
     s, d = O.gen_rmat(O.stream_key(32, O.STREAM_RMAT), 11, 16 * V)
