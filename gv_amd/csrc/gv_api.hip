@@ -554,7 +554,9 @@ gv_status gv_create(const gv_config *cfg_in, uint32_t num_vertices, int directed
     P = (uint64_t)((double)fr * frac / (double)bytes_per_block) / nst;
   }
   P = std::max<uint64_t>(P, 64);
-  P = std::min<uint64_t>(P, 0x3FFFFFFFull);
+  // slot indices are 32-bit (block ID * B + offset) and 0xFFFFFFFF marks
+  // "no slot", so every slot index must stay below it
+  P = std::min<uint64_t>(P, (1ull << 32) / cfg.block_slots - 1);
   uint64_t A = std::min<uint64_t>(4 * P + (uint64_t)num_vertices + 1024, 0xFFFFFFF0ull);
   gv_status rc = GV_OK;
   do {

@@ -21,6 +21,7 @@ def hip():
         L.hipMalloc.argtypes = [ctypes.POINTER(P), ctypes.c_size_t]
         L.hipFree.argtypes = [P]
         L.hipMemcpy.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int]
+        L.hipMemcpyAsync.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int, P]
         L.hipMemset.argtypes = [P, ctypes.c_int, ctypes.c_size_t]
         L.hipDeviceSynchronize.argtypes = []
         L.hipEventCreate.argtypes = [ctypes.POINTER(P)]
@@ -82,6 +83,16 @@ class DeviceArray:
         if len(a):
             _ok(hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p),
                                 len(a) * self.dtype.itemsize, H2D), "H2D")
+
+    def copy_from_async(self, src, stream):
+        """Enqueue a device-to-device copy of src (a DeviceArray or view of
+        the same dtype, at most len(self) entries) into the first entries on
+        the given stream."""
+        if src.dtype != self.dtype or src.n > self.n:
+            raise ValueError("source must have this dtype and fit")
+        if src.n:
+            _ok(hip().hipMemcpyAsync(self.ptr, src.ptr, src.nbytes, D2D,
+                                     ctypes.c_void_p(stream)), "D2D async")
 
     def numpy(self, n=None):
         n = self.n if n is None else int(n)

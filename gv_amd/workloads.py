@@ -54,8 +54,12 @@ def run_chunglu(args):
                 (25% tombstones) after every delete batch;
       batched:  one gv_batch_apply over all batches + one compaction;
       captured: the batched pipeline recorded once (gv_capture_*) and
-                replayed by one graph launch.
-    The three final graphs must be identical.  Then 64 personalized PageRank
+                replayed by one graph launch;
+      queue:    gv_batch_apply_device recorded once over a device queue and
+                replayed over four successive fills of that queue with
+                uneven batch counts (the fills are device-to-device copies
+                on the graph's stream, timed with the replays).
+    The four final graphs must be identical.  Then 64 personalized PageRank
     columns and BC from 256 sources (32 per pass) run on the result."""
     from . import (GV_MEM_DEVICE, GV_OP_INSERT, GraphVine, _check, default_config,
                    export_csr_device, gen_chung_lu, lib, stream_key)
@@ -150,8 +154,53 @@ def run_chunglu(args):
     forms["captured"] = {"ms": ms, "edge_updates_per_s": total / (ms / 1e3)}
     o3, c3 = h.export_csr()
     eq_captured = bool(np.array_equal(o3, ref_off) and np.array_equal(c3, ref_cols))
-    del o3, c3, ref_off, ref_cols
+    del o3, c3
     c.close()
+    h.close()
+    # device queue: fills of uneven batch counts, one capture for all
+    cuts = sorted({0, nb} | {max(1, min(nb - 1, int(round(nb * f)))) for f in (0.15, 0.45, 0.7)})
+    fills = list(zip(cuts[:-1], cuts[1:]))
+    qb = max(b1 - b0 for b0, b1 in fills)
+    qp = max(int(offs[b1] - offs[b0]) for b0, b1 in fills)
+    fill_dev = []
+    for b0, b1 in fills:
+        fo = np.full(qb + 1, offs[b1] - offs[b0], np.uint64)
+        fo[: b1 - b0 + 1] = offs[b0:b1 + 1] - offs[b0]
+        fk = np.full(qb, GV_OP_INSERT, np.int32)
+        fk[: b1 - b0] = kinds[b0:b1]
+        fill_dev.append((cs.view(int(offs[b0]), int(offs[b1] - offs[b0])),
+                         cd.view(int(offs[b0]), int(offs[b1] - offs[b0])),
+                         DeviceArray.from_numpy(fo), DeviceArray.from_numpy(fk)))
+    qs, qd = DeviceArray(qp, np.uint32), DeviceArray(qp, np.uint32)
+    qo, qk = DeviceArray(qb + 1, np.uint64), DeviceArray(qb, np.int32)
+    h = fresh()
+    h.reserve(qp, qb)
+    with h.capture() as cap:
+        h.applyBatchesDevice(qs, qd, qo, qk, compact_fraction=0.25)
+    c = cap.result
+    stream = h.stream()
+
+    def drain():
+        for vs, vd, fo, fk in fill_dev:
+            qs.copy_from_async(vs, stream)
+            qd.copy_from_async(vd, stream)
+            qo.copy_from_async(fo, stream)
+            qk.copy_from_async(fk, stream)
+            c.launch()
+    _, ms = _timed(h, drain)
+    h.sync()
+    forms["queue"] = {"ms": ms, "edge_updates_per_s": total / (ms / 1e3),
+                      "fills": [b1 - b0 for b0, b1 in fills], "queue_batches": qb,
+                      "queue_pairs": qp}
+    o4, c4 = h.export_csr()
+    eq_queue = bool(np.array_equal(o4, ref_off) and np.array_equal(c4, ref_cols))
+    del o4, c4, ref_off, ref_cols
+    c.close()
+    for a in (qs, qd, qo, qk):
+        a.free()
+    for _, _, fo, fk in fill_dev:
+        fo.free()
+        fk.free()
 
     # analytics on the updated graph
     rng = np.random.default_rng(seed + 100)
@@ -203,7 +252,8 @@ def run_chunglu(args):
         "speedup_batched_vs_plain": forms["batched"]["edge_updates_per_s"]
         / forms["plain"]["edge_updates_per_s"],
         "speedup_captured_vs_plain": cap_rate / forms["plain"]["edge_updates_per_s"],
-        "final_graphs_equal": {"batched": eq_batched, "captured": eq_captured},
+        "final_graphs_equal": {"batched": eq_batched, "captured": eq_captured,
+                               "queue": eq_queue},
         "personalized_pagerank_64": {"ms": ppr_ms, "iters": int(it.value),
                                      "max_residual": float(res.value),
                                      "gteps_columns": 64.0 * E * it.value / (ppr_ms / 1e3) / 1e9},
@@ -262,6 +312,12 @@ def run_serve(args):
     from .hip import DeviceArray
 
     scale = int(getattr(args, "serve_scale", 26))
+    if scale > 24:
+        # A run of this workload at scale 26 made a GPU memory fault whose
+        # cause is not found yet; until it is, only scales whose runs pass
+        # (24, as in the default bench's serving leg) are accepted.
+        raise RuntimeError(f"--workload serve at scale {scale} is disabled (an unresolved GPU "
+                           "memory fault at scale 26); use --serve-scale 24 or less")
     seconds = float(getattr(args, "serve_seconds", 10.0))
     seed = 5
     V = 1 << scale

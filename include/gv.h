@@ -52,7 +52,8 @@ typedef struct gv_graph gv_graph;
 /* Store configuration.  gv_config_default() fills every field. */
 typedef struct gv_config {
   uint32_t block_slots;       /* slots per edge block: 32, 64 or 128 (4 B each) */
-  uint64_t pool_blocks;       /* edge blocks in the pool; 0 = sized from free HBM */
+  uint64_t pool_blocks;       /* edge blocks in the pool; 0 = sized from free HBM; at most
+                                 2^32 / block_slots - 1 per store (32-bit slot indices) */
   double pool_fraction;       /* share of free HBM used when pool_blocks == 0 */
   float compact_threshold;    /* default tombstone fraction for gv_compact(g, -1) */
   uint32_t pr_thread_max;     /* degree bins of the PageRank row form (layout 4): a row with
