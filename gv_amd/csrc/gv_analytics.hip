@@ -349,18 +349,31 @@ __global__ __launch_bounds__(256) void k_snap_write(Store st, const uint32_t *is
       if (newid) dst = row0 + (dst - rs);
     }
     if (hi - lo <= SNAP_LANE) {
-      // rounds of 16 slots, their loads and then their 16 new labels issued
+      // rounds of 16 slots: lo is a multiple of SNAP_CH, so a round lies in
+      // one block -- one block-table load as a round enters a new block,
+      // four 16-byte slot loads, then the round's 16 new labels, all issued
       // together
-      uint32_t hot = 0;
+      uint32_t hot = 0, blk = 0;
 #pragma unroll 1
       for (uint32_t p = lo; p < hi; p += 16) {
+        if (p == lo || (p & (st.B - 1)) == 0) blk = st.arena[tbl + (p >> st.log2B)];
+        const uint4 *p4 =
+            reinterpret_cast<const uint4 *>(st.slots + (blk * st.B + (p & (st.B - 1))));
+        const uint32_t nr = hi - p;
         uint32_t x[16];
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k)
-          x[k] = p + k < hi ? st.slots[slot_index(st.arena, tbl, p + k, st.log2B, st.B)] : TOMB;
+        for (uint32_t q = 0; q < 4; ++q) {
+          const uint4 v = 4 * q < nr ? p4[q] : make_uint4(TOMB, TOMB, TOMB, TOMB);
+          x[4 * q] = v.x;
+          x[4 * q + 1] = v.y;
+          x[4 * q + 2] = v.z;
+          x[4 * q + 3] = v.w;
+        }
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k)
+        for (uint32_t k = 0; k < 16; ++k) {
+          if (k >= nr) x[k] = TOMB;
           if (newid && x[k] != TOMB) x[k] = newid[x[k]];
+        }
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) {
           if (x[k] == TOMB) continue;
