@@ -324,8 +324,9 @@ class GraphVine:
         offsets a uint64 device array of max_batches+1 entries and kinds an
         int32 device array of max_batches entries (gv_amd.hip.DeviceArray or
         contiguous device tensors), all read when the call runs on the
-        handle's stream.  Needs reserve(max_pairs, max_batches) first; errors
-        found on the device surface at the next call that reads the status."""
+        handle's stream.  Inside a capture it needs reserve(max_pairs,
+        max_batches) first; errors found on the device surface at the next
+        call that reads the status (sync())."""
         ps, ks, ms = _ptr_u32(src)
         pd, kd, md = _ptr_u32(dst)
         if ms != GV_MEM_DEVICE or md != GV_MEM_DEVICE:

@@ -443,14 +443,12 @@ static gv_status apply_group_device(gv_graph *g, const uint32_t *src, const uint
     set_error("group larger than 2^31 pairs; split it");
     return GV_E_INVALID;
   }
-  // the staging must already hold the capacity (no allocation here: the
-  // call may be captured)
+  // staging for the whole capacity (inside a capture it must already be
+  // reserved: reserve_keys then refuses instead of allocating)
   const int nst = g->has_in ? 2 : 1;
   for (int k = 0; k < nst; ++k) {
-    if (!scratch_fits(g->sc[k], g->st[k], g->directed ? max_pairs : 2 * max_pairs)) {
-      set_error("gv_batch_apply_device: call gv_reserve(g, max_pairs, max_batches) first");
-      return GV_E_INVALID;
-    }
+    rc = reserve_keys(g, k, g->directed ? max_pairs : 2 * max_pairs);
+    if (rc) return rc;
   }
   if (g->capturing) g->cap_pairs += max_pairs;
   group_pipeline(g, src, dst, max_pairs, reinterpret_cast<const uint64_t *>(offsets),

@@ -125,8 +125,9 @@ gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
  * refill them between launches of a capture that recorded this call.  The
  * queue holds offsets[max_batches] <= max_pairs pairs: unused trailing
  * batches are empty (equal offsets) and entries past offsets[max_batches] are
- * ignored.  Requires gv_reserve(g, max_pairs, max_batches) first (the call
- * never allocates) and reads nothing back: offsets that do not start at 0, that
+ * ignored.  Inside a capture it needs gv_reserve(g, max_pairs, max_batches)
+ * first (it never allocates there; an eager call grows the staging like
+ * gv_batch_apply), and it reads nothing back: offsets that do not start at 0, that
  * decrease or that end past max_pairs, kinds other than GV_OP_INSERT or
  * GV_OP_DELETE and out-of-range vertex IDs leave the graph unchanged and set
  * the handle's sticky error, which skips later updates until gv_sync (or
