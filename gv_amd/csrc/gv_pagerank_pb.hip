@@ -80,7 +80,7 @@ static_assert(PB_ROUNDS * PB_PER_ROUND * 512 >= PB_PAD / 4, "the rounds cover th
 // A cold bin image in PB_CROUNDS groups of 16 bytes per thread.
 constexpr uint32_t PB_CROUNDS = 5;
 static_assert(PB_CROUNDS * 512 * 4 >= PB_CPAD, "the rounds cover a cold image");
-constexpr uint32_t PB_HOT_UNIT = 32;          // hot tiles per work unit (4 per wavefront)
+constexpr uint32_t PB_HOT_UNIT = 64;          // hot tiles per work unit (8 per wavefront)
 constexpr uint32_t PB_CH = 1u << 15;          // entries per expansion work item (multiple of 4)
 constexpr int PB_THREADS = 512;               // tile workgroups: 8 wavefronts
 constexpr int PB_BUILD_THREADS = 1024;
