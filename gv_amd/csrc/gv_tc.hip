@@ -31,6 +31,9 @@ constexpr uint32_t TC_CH = 1024;    // store slots per build item
 constexpr uint32_t TC_TAB = 1024;
 constexpr uint32_t TC_PART = TC_TAB / 2;
 constexpr uint32_t TC_EMPTY = 0xFFFFFFFFu;
+// wedges per lane and step of the count, their loads issued together
+// (R-MAT 22: 8 is 1% faster than 4; 2 was 2% slower than 4)
+constexpr uint32_t TC_INFLIGHT = 8;
 
 __device__ __forceinline__ bool tc_higher(uint32_t dw, uint32_t w, uint32_t du, uint32_t u) {
   return dw > du || (dw == du && w > u);
@@ -269,14 +272,20 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
         uint32_t h = tc_slot(x, shift);
         while (atomicCAS(&tab[h], TC_EMPTY, x) != TC_EMPTY) h = (h + 1) & hmask;
       }
-      // four wedges per lane and step (f, f + 64, ...), so each lane has
-      // four independent loads in flight before it probes (two: 2% slower)
+      // TC_INFLIGHT wedges per lane and step (f, f + 64, ...), so each lane
+      // has that many independent loads in flight before it probes
       uint32_t k = 0;  // this lane's current block entry (advances monotonically)
-      for (uint32_t fb = 0; fb < nwed; fb += 4 * WAVE) {
-        uint32_t wv[4] = {TC_EMPTY, TC_EMPTY, TC_EMPTY, TC_EMPTY}, kv[4] = {0, 0, 0, 0};
-        uint64_t pv_pos[4] = {0, 0, 0, 0};
+      for (uint32_t fb = 0; fb < nwed; fb += TC_INFLIGHT * WAVE) {
+        uint32_t wv[TC_INFLIGHT], kv[TC_INFLIGHT];
+        uint64_t pv_pos[TC_INFLIGHT];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < (int)TC_INFLIGHT; ++r) {
+          wv[r] = TC_EMPTY;
+          kv[r] = 0;
+          pv_pos[r] = 0;
+        }
+#pragma unroll
+        for (int r = 0; r < (int)TC_INFLIGHT; ++r) {
           const uint32_t f = fb + r * WAVE + lane;
           if (f < nwed) {
             while (s_pre[wid][k] <= f) ++k;
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(256) void k_tc_count(const uint64_t *off, const uin
           }
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < (int)TC_INFLIGHT; ++r) {
           if (wv[r] == TC_EMPTY) continue;
           uint32_t h = tc_slot(wv[r], shift);
           bool found = false;
