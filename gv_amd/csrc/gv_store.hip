@@ -300,6 +300,9 @@ __global__ void k_stage_seq(const uint32_t *src, const uint32_t *dst, uint64_t n
 // so a key's neighbours in index order are its neighbour lanes.
 constexpr int GS_ROWS = 16;
 constexpr uint64_t GS_TILE = 256 * GS_ROWS;
+// k_gs_write scans the (row, wave) cells of a tile with one wavefront, a cell
+// per lane
+static_assert(GS_ROWS * 4 == WAVE, "one cell per lane");
 
 // A tile's keys are loaded before any flag is evaluated; the neighbour keys
 // that the shuffles of a row cannot supply (lane 63: key i + 1, lane 0: key
