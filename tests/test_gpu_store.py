@@ -352,3 +352,25 @@ def test_graph_from_matrix_market_file(tmp_path):
     o.bulk_build(s, d)
     assert_same(g, o)
     g.close()
+
+
+def test_pool_capped_to_32bit_slot_indices():
+    """A pool asked for more blocks than 32-bit slot indices address is capped
+    at 2^32 / B - 1 blocks; blocks are taken from the top of the free stack,
+    so the first inserts land on the highest slot indices, which must still
+    read back exactly."""
+    V = 1024
+    cfg = G.default_config(block_slots=128)
+    cfg.pool_blocks = 1 << 30
+    g = G.GraphVine(V, directed=False, config=cfg)
+    free, total = g.pool_stats()
+    assert total == (1 << 32) // 128 - 1 and free == total
+    o = O.OracleGraph(V, False)
+    s, d = O.gen_uniform(O.stream_key(41, 1), V, 20000)
+    g.batchInsert(s, d)
+    o.insert(s, d)
+    g.batchDelete(s[:5000], d[:5000])
+    o.delete(s[:5000], d[:5000])
+    g.compaction(0.0)
+    assert_same(g, o, "top of the slot index range")
+    g.close()
