@@ -359,3 +359,40 @@ def test_set_tuning_keeps_pagerank():
     with pytest.raises(ValueError):
         g.set_tuning(block_slots=32)
     g.close()
+
+
+def test_device_queue_with_more_batches_than_staged_offsets():
+    """A queue of 3000 batch slots (its offsets searched in global memory, not
+    staged) with a few non-empty batches among empty ones."""
+    V = 1 << 11
+    s, d = O.gen_rmat(O.stream_key(38, O.STREAM_RMAT), 11, 16 * V)
+    g = make(V, False)
+    o = O.OracleGraph(V, False)
+    g.bulkBuild(s, d)
+    o.bulk_build(s, d)
+    nb, cap = 3000, 8000
+    sizes = np.zeros(nb, np.int64)
+    kinds = np.full(nb, G.GV_OP_INSERT, np.int32)
+    hs = np.full(cap, 0xFFFFFFFF, np.uint32)
+    hd = np.full(cap, 0xFFFFFFFF, np.uint32)
+    at = 0
+    for j, b in enumerate((3, 700, 701, 1999, 2047, 2048, 2500, 2999)):
+        n = 500 + 100 * j
+        if j % 3 == 2:
+            off, cols = o.export_csr()
+            bs, bd = O.gen_sample(O.stream_key(38, 100 + j), V, off, cols, n)
+            o.delete(bs, bd)
+            kinds[b] = G.GV_OP_DELETE
+        else:
+            bs, bd = O.gen_uniform(O.stream_key(38, 200 + j), V, n)
+            o.insert(bs, bd)
+        sizes[b] = n
+        hs[at:at + n], hd[at:at + n] = bs, bd
+        at += n
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    qs, qd = DeviceArray.from_numpy(hs), DeviceArray.from_numpy(hd)
+    qo, qk = DeviceArray.from_numpy(offsets), DeviceArray.from_numpy(kinds)
+    g.applyBatchesDevice(qs, qd, qo, qk, compact_fraction=0.25)
+    g.sync()
+    assert_same(g, o, "3000-slot queue")
+    g.close()
