@@ -125,17 +125,16 @@ gv_status gv_batch_apply(gv_graph *g, const uint32_t *src, const uint32_t *dst,
  * refill them between launches of a capture that recorded this call.  The
  * queue holds offsets[max_batches] <= max_pairs pairs: unused trailing
  * batches are empty (equal offsets) and entries past offsets[max_batches] are
- * ignored.  Inside a capture it needs gv_reserve(g, max_pairs, max_batches)
- * first (it never allocates there; an eager call grows the staging like
- * gv_batch_apply), and it reads nothing back: offsets that do not start at 0, that
- * decrease or that end past max_pairs, kinds other than GV_OP_INSERT or
- * GV_OP_DELETE and out-of-range vertex IDs leave the graph unchanged and set
- * the handle's sticky error, which skips later updates until gv_sync (or
+ * ignored.  Inside a capture the call needs gv_reserve(g, max_pairs,
+ * max_batches) first and never allocates; an eager call grows the staging
+ * like gv_batch_apply.  Nothing is read back: offsets that do not start at
+ * 0, that decrease or that end past max_pairs, kinds other than GV_OP_INSERT
+ * or GV_OP_DELETE and out-of-range vertex IDs leave the graph unchanged and
+ * set the handle's sticky error, which skips later updates until gv_sync (or
  * gv_capture_launch with stats) reports it (GV_E_INVALID / GV_E_RANGE) and
- * clears it.  No reference
- * counterpart: for callers that drain a device-side update queue without a
- * host round trip; the reference takes one host-built CSR batch per call
- * (graphvine.cu:222-451). */
+ * clears it.  No reference counterpart: for callers that drain a device-side
+ * update queue without a host round trip; the reference takes one host-built
+ * CSR batch per call (graphvine.cu:222-451). */
 gv_status gv_batch_apply_device(gv_graph *g, const uint32_t *src, const uint32_t *dst,
                                 const uint64_t *offsets, const int *kinds, uint32_t max_batches,
                                 uint64_t max_pairs, float compact_fraction);
